@@ -1,0 +1,331 @@
+"""Recorded BLContext command streams ("scenes") and the synthetic workloads.
+
+A scene is a canvas description plus an ordered list of BLContext calls
+(set_comp_op, set_fill_style, fill_path, ...) serialized to a little-endian
+byte string.  The same bytes replay through the MI355X backend
+(`blend2d_amd.replay`), and the test suite replays them through the CPU
+oracle and the reference library for parity checks.
+
+The generators reproduce the benchmark workloads: seeded random polygons,
+cubic-Bezier paths with gradients, rotated rectangles/ellipses with patterns
+and glyph-like quadratic contours.
+"""
+from __future__ import annotations
+
+import math
+import struct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MAGIC = 0x43533242
+VERSION = 1
+
+FMT_PRGB32, FMT_XRGB32, FMT_A8 = 1, 2, 3
+
+OP_SET_COMP_OP = 1
+OP_SET_FILL_RULE = 2
+OP_SET_TRANSFORM = 3
+OP_SET_GLOBAL_ALPHA = 4
+OP_SET_STYLE_SOLID = 5
+OP_SET_STYLE_GRADIENT = 6
+OP_SET_STYLE_PATTERN = 7
+OP_SET_PATTERN_QUALITY = 8
+OP_FILL_PATH = 9
+OP_FILL_POLYGON = 10
+OP_FILL_RECT_D = 11
+OP_FILL_RECT_I = 12
+OP_FILL_ALL = 13
+
+# BLCompOp values (core/context.h).
+COMP_OP_SRC_OVER, COMP_OP_SRC_COPY = 0, 1
+COMP_OP_PLUS, COMP_OP_MULTIPLY, COMP_OP_SCREEN = 12, 15, 16
+
+# BLPathCmd values (core/path.h).
+CMD_MOVE, CMD_ON, CMD_QUAD, CMD_CUBIC, CMD_CLOSE = 0, 1, 2, 4, 5
+
+GRADIENT_LINEAR, GRADIENT_RADIAL, GRADIENT_CONIC = 0, 1, 2
+EXTEND_PAD, EXTEND_REPEAT, EXTEND_REFLECT = 0, 1, 2
+
+
+def _pad8(b: bytes) -> bytes:
+    return b + b"\0" * ((-len(b)) % 8)
+
+
+@dataclass
+class Scene:
+    """An in-memory BLContext command stream."""
+
+    width: int
+    height: int
+    format: int = FMT_PRGB32
+    clear_value: int = 0
+    images: list = field(default_factory=list)
+    cmds: list = field(default_factory=list)
+    fill_count: int = 0
+
+    # -- images ----------------------------------------------------------
+    def add_image(self, pixels: np.ndarray, fmt: int = FMT_PRGB32) -> int:
+        pixels = np.ascontiguousarray(pixels)
+        if fmt == FMT_A8:
+            assert pixels.dtype == np.uint8 and pixels.ndim == 2
+        else:
+            assert pixels.dtype == np.uint32 and pixels.ndim == 2
+        self.images.append((fmt, pixels))
+        return len(self.images) - 1
+
+    # -- state -----------------------------------------------------------
+    def _cmd(self, op: int, payload: bytes = b"") -> None:
+        self.cmds.append((op, _pad8(payload)))
+
+    def set_comp_op(self, op: int) -> None:
+        self._cmd(OP_SET_COMP_OP, struct.pack("<II", op, 0))
+
+    def set_fill_rule(self, rule: int) -> None:
+        self._cmd(OP_SET_FILL_RULE, struct.pack("<II", rule, 0))
+
+    def set_transform(self, m) -> None:
+        self._cmd(OP_SET_TRANSFORM, struct.pack("<6d", *m))
+
+    def set_global_alpha(self, a: float) -> None:
+        self._cmd(OP_SET_GLOBAL_ALPHA, struct.pack("<d", a))
+
+    def set_pattern_quality(self, q: int) -> None:
+        self._cmd(OP_SET_PATTERN_QUALITY, struct.pack("<II", q, 0))
+
+    def set_fill_style_rgba32(self, rgba32: int) -> None:
+        self._cmd(OP_SET_STYLE_SOLID, struct.pack("<II", rgba32 & 0xFFFFFFFF, 0))
+
+    def set_fill_style_gradient(self, gtype: int, values, stops, extend: int = EXTEND_PAD,
+                                transform=None) -> None:
+        vals = list(values) + [0.0] * (6 - len(values))
+        tr = list(transform) if transform is not None else [1.0, 0.0, 0.0, 1.0, 0.0, 0.0]
+        payload = struct.pack("<IIII6d6d", gtype, extend, len(stops), 1 if transform is not None else 0, *vals, *tr)
+        for off, rgba in stops:
+            payload += struct.pack("<dII", off, rgba & 0xFFFFFFFF, 0)
+        self._cmd(OP_SET_STYLE_GRADIENT, payload)
+
+    def set_fill_style_pattern(self, image_index: int, extend: int = EXTEND_REPEAT, transform=None) -> None:
+        tr = list(transform) if transform is not None else [1.0, 0.0, 0.0, 1.0, 0.0, 0.0]
+        self._cmd(OP_SET_STYLE_PATTERN, struct.pack("<IIII6d", image_index, extend,
+                                                     1 if transform is not None else 0, 0, *tr))
+
+    # -- fills -----------------------------------------------------------
+    def fill_path(self, cmds, vtx) -> None:
+        cmds = np.asarray(cmds, dtype=np.uint8)
+        vtx = np.asarray(vtx, dtype=np.float64).reshape(-1, 2)
+        assert len(cmds) == len(vtx)
+        self._cmd(OP_FILL_PATH, struct.pack("<II", len(cmds), 0) + _pad8(cmds.tobytes()) + vtx.tobytes())
+        self.fill_count += 1
+
+    def fill_polygon(self, vtx) -> None:
+        vtx = np.asarray(vtx, dtype=np.float64).reshape(-1, 2)
+        self._cmd(OP_FILL_POLYGON, struct.pack("<II", len(vtx), 0) + vtx.tobytes())
+        self.fill_count += 1
+
+    def fill_rect_d(self, x, y, w, h) -> None:
+        self._cmd(OP_FILL_RECT_D, struct.pack("<4d", x, y, w, h))
+        self.fill_count += 1
+
+    def fill_rect_i(self, x, y, w, h) -> None:
+        self._cmd(OP_FILL_RECT_I, struct.pack("<4i", x, y, w, h))
+        self.fill_count += 1
+
+    def fill_all(self) -> None:
+        self._cmd(OP_FILL_ALL)
+        self.fill_count += 1
+
+    # -- serialization ---------------------------------------------------
+    def to_bytes(self) -> bytes:
+        out = [struct.pack("<8I", MAGIC, VERSION, self.width, self.height, self.format,
+                           len(self.images), len(self.cmds), self.clear_value & 0xFFFFFFFF)]
+        for fmt, px in self.images:
+            h, w = px.shape
+            bpp = 1 if fmt == FMT_A8 else 4
+            out.append(struct.pack("<4I", w, h, fmt, w * bpp))
+            out.append(_pad8(px.tobytes()))
+        for op, payload in self.cmds:
+            out.append(struct.pack("<II", op, len(payload)))
+            out.append(payload)
+        return b"".join(out)
+
+    def pixel_bytes(self) -> int:
+        return self.width * self.height * (1 if self.format == FMT_A8 else 4)
+
+
+# ---------------------------------------------------------------------------
+# Geometry helpers
+# ---------------------------------------------------------------------------
+
+def ellipse_path(cx: float, cy: float, rx: float, ry: float):
+    """Ellipse as four cubic arcs, the shape BLPath::add_ellipse produces.
+
+    Uses the standard circle-to-cubic constant k = 0.5522847498307936.
+    """
+    k = 0.55228474983079339840
+    kx, ky = rx * k, ry * k
+    cmds = [CMD_MOVE] + [CMD_CUBIC, CMD_CUBIC, CMD_ON] * 4 + [CMD_CLOSE]
+    v = [(cx + rx, cy),
+         (cx + rx, cy + ky), (cx + kx, cy + ry), (cx, cy + ry),
+         (cx - kx, cy + ry), (cx - rx, cy + ky), (cx - rx, cy),
+         (cx - rx, cy - ky), (cx - kx, cy - ry), (cx, cy - ry),
+         (cx + kx, cy - ry), (cx + rx, cy - ky), (cx + rx, cy),
+         (cx + rx, cy)]
+    return cmds, v
+
+
+def rotation_about(angle: float, cx: float, cy: float):
+    """Matrix rotating by `angle` around (cx, cy): [m00 m01 m10 m11 m20 m21]."""
+    c, s = math.cos(angle), math.sin(angle)
+    return [c, s, -s, c, cx - c * cx + s * cy, cy - s * cx - c * cy]
+
+
+def premultiplied_noise(rng: np.random.Generator, w: int, h: int) -> np.ndarray:
+    """Seeded PRGB32 noise image (channels <= alpha)."""
+    a = rng.integers(0, 256, size=(h, w), dtype=np.uint32)
+    r = (rng.integers(0, 256, size=(h, w), dtype=np.uint32) * a + 127) // 255
+    g = (rng.integers(0, 256, size=(h, w), dtype=np.uint32) * a + 127) // 255
+    b = (rng.integers(0, 256, size=(h, w), dtype=np.uint32) * a + 127) // 255
+    return ((a << 24) | (r << 16) | (g << 8) | b).astype(np.uint32)
+
+
+def _rand_rgba(rng) -> int:
+    return int(rng.integers(0, 1 << 32, dtype=np.uint64))
+
+
+def _rand_stops(rng, n_min=2, n_max=8):
+    n = int(rng.integers(n_min, n_max + 1))
+    offs = np.sort(rng.random(n))
+    return [(float(o), _rand_rgba(rng)) for o in offs]
+
+
+# ---------------------------------------------------------------------------
+# Workloads
+# ---------------------------------------------------------------------------
+
+def polygons_scene(seed: int = 1, width: int = 512, height: int = 512, count: int = 1000) -> Scene:
+    """512x512 PRGB32 cleared to opaque white, `count` random simple polygons
+    (3-12 vertices uniform in the canvas), non-zero rule, random straight-alpha
+    solid colours, SRC_OVER, submission order."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(width, height, FMT_PRGB32, 0xFFFFFFFF)
+    sc.set_fill_rule(0)
+    for _ in range(count):
+        n = int(rng.integers(3, 13))
+        pts = rng.random((n, 2)) * np.array([width, height])
+        sc.set_fill_style_rgba32(_rand_rgba(rng))
+        sc.fill_polygon(pts)
+    return sc
+
+
+def cubic_gradient_scene(seed: int = 1, width: int = 1920, height: int = 1080, count: int = 10000) -> Scene:
+    """1920x1080 PRGB32, `count` closed cubic-Bezier paths (4-16 segments, control
+    points uniform in a random 16-600 px box), even-odd, 50% linear / 50% radial
+    gradients with 2-8 stops, PAD extend, SRC_OVER."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(width, height, FMT_PRGB32, 0xFF000000)
+    sc.set_fill_rule(1)
+    for i in range(count):
+        side = float(rng.uniform(16, 600))
+        bx = float(rng.uniform(-side * 0.25, width - side * 0.75))
+        by = float(rng.uniform(-side * 0.25, height - side * 0.75))
+        k = int(rng.integers(4, 17))
+        pts = rng.random((3 * k, 2)) * side + np.array([bx, by])
+        cmds = [CMD_MOVE] + [CMD_CUBIC, CMD_CUBIC, CMD_ON] * k + [CMD_CLOSE]
+        vtx = np.vstack([pts[-1:], pts, pts[-1:]])
+        stops = _rand_stops(rng)
+        if i % 2 == 0:
+            p0 = rng.random(2) * side + np.array([bx, by])
+            p1 = rng.random(2) * side + np.array([bx, by])
+            sc.set_fill_style_gradient(GRADIENT_LINEAR, [p0[0], p0[1], p1[0], p1[1]], stops, EXTEND_PAD)
+        else:
+            c = rng.random(2) * side + np.array([bx, by])
+            fpt = c + (rng.random(2) - 0.5) * side * 0.2
+            r = float(rng.uniform(8, 300))
+            sc.set_fill_style_gradient(GRADIENT_RADIAL, [c[0], c[1], fpt[0], fpt[1], r, 0.0], stops, EXTEND_PAD)
+        sc.fill_path(cmds, vtx)
+    return sc
+
+
+def pattern_shapes_scene(seed: int = 1, width: int = 3840, height: int = 2160, count: int = 2000,
+                         comp_ops=(COMP_OP_SRC_OVER, COMP_OP_MULTIPLY, COMP_OP_PLUS, COMP_OP_SCREEN),
+                         pattern_size: int = 512) -> Scene:
+    """3840x2160 PRGB32, `count` rotated rectangles and ellipses filled with an
+    affine pattern of a seeded 512x512 PRGB32 noise image (rotation 0-2pi, scale
+    log-uniform 0.5-2), bilinear, REPEAT, comp ops drawn from `comp_ops`."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(width, height, FMT_PRGB32, 0xFF808080)
+    img = sc.add_image(premultiplied_noise(rng, pattern_size, pattern_size))
+    sc.set_pattern_quality(1)
+    for i in range(count):
+        w = float(rng.uniform(16, 600))
+        h = float(rng.uniform(16, 600))
+        cx = float(rng.uniform(0, width))
+        cy = float(rng.uniform(0, height))
+        ang = float(rng.uniform(0, 2 * math.pi))
+        pang = float(rng.uniform(0, 2 * math.pi))
+        psc = float(2.0 ** rng.uniform(-1, 1))
+        op = int(comp_ops[int(rng.integers(0, len(comp_ops)))])
+        c, s = math.cos(pang) * psc, math.sin(pang) * psc
+        sc.set_comp_op(op)
+        if i % 2 == 0:
+            sc.set_transform(rotation_about(ang, cx, cy))
+            sc.set_fill_style_pattern(img, EXTEND_REPEAT, [c, s, -s, c, cx, cy])
+            sc.fill_rect_d(cx - w / 2, cy - h / 2, w, h)
+        else:
+            sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+            sc.set_fill_style_pattern(img, EXTEND_REPEAT, [c, s, -s, c, cx, cy])
+            cm, v = ellipse_path(cx, cy, w / 2, h / 2)
+            sc.fill_path(cm, v)
+    sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+    return sc
+
+
+def glyph_path(rng, size_w: float, size_h: float):
+    """Closed quadratic contour with 20-80 segments around a wobbly ring."""
+    n = int(rng.integers(20, 81))
+    cx, cy = size_w * 0.5, size_h * 0.5
+    ang = np.sort(rng.random(n)) * 2 * math.pi
+    rad = 0.25 + 0.25 * rng.random(n)
+    pts = np.stack([cx + np.cos(ang) * rad * size_w, cy + np.sin(ang) * rad * size_h], 1)
+    ctl_r = 0.2 + 0.3 * rng.random(n)
+    mid = ang + math.pi / n
+    ctl = np.stack([cx + np.cos(mid) * ctl_r * size_w, cy + np.sin(mid) * ctl_r * size_h], 1)
+    cmds = [CMD_MOVE]
+    vtx = [pts[0]]
+    for k in range(n):
+        cmds += [CMD_QUAD, CMD_ON]
+        vtx += [ctl[k], pts[(k + 1) % n]]
+    cmds.append(CMD_CLOSE)
+    vtx.append(pts[0])
+    return cmds, np.array(vtx)
+
+
+def glyph_canvases(seed: int = 1, count: int = 4096, max_paths: int = 200):
+    """`count` independent canvases, sides log-uniform 32-256 px, half A8 and
+    half PRGB32, each with 1-`max_paths` glyph-like quadratic contours,
+    non-zero, solid and linear-gradient fills, SRC_OVER."""
+    rng = np.random.default_rng(seed)
+    scenes = []
+    for i in range(count):
+        w = int(round(2.0 ** rng.uniform(5, 8)))
+        h = int(round(2.0 ** rng.uniform(5, 8)))
+        fmt = FMT_A8 if i % 2 == 0 else FMT_PRGB32
+        sc = Scene(w, h, fmt, 0)
+        sc.set_fill_rule(0)
+        for _ in range(int(rng.integers(1, max_paths + 1))):
+            gw = float(rng.uniform(8, w))
+            gh = float(rng.uniform(8, h))
+            ox = float(rng.uniform(-gw * 0.2, w - gw * 0.8))
+            oy = float(rng.uniform(-gh * 0.2, h - gh * 0.8))
+            cm, v = glyph_path(rng, gw, gh)
+            v = v + np.array([ox, oy])
+            if rng.random() < 0.5:
+                sc.set_fill_style_rgba32(_rand_rgba(rng))
+            else:
+                p0 = rng.random(2) * np.array([gw, gh]) + np.array([ox, oy])
+                p1 = rng.random(2) * np.array([gw, gh]) + np.array([ox, oy])
+                sc.set_fill_style_gradient(GRADIENT_LINEAR, [p0[0], p0[1], p1[0], p1[1]], _rand_stops(rng, 2, 4))
+            sc.fill_path(cm, v)
+        scenes.append(sc)
+    return scenes

@@ -1,0 +1,277 @@
+/* CPU oracle -- fetchers and composition operators.
+ *
+ * TEST INFRASTRUCTURE (see orc_core.h).  Restates
+ *   pipeline/reference/fetchgeneric_p.h  (solid, linear/radial/conic gradient,
+ *                                         affine NN/bilinear pattern)
+ *   pipeline/reference/compopgeneric_p.h (SRC_COPY, SRC_OVER, PLUS)
+ *   pipeline/reference/pixelgeneric_p.h  (div255, addus8, PixelIO)
+ *   pipeline/jit/compoppart.cpp:4150-4176, 4291-4460, 4592-4628
+ *                                        (PLUS, MULTIPLY, SCREEN; the non-JIT
+ *                                          reference has no MULTIPLY/SCREEN)
+ * The reference fetchers step their state per pixel and per row; every one of
+ * them is a function of the pixel position except the conic gradient, whose
+ * per-row origin accumulates in double precision from the row the fill call
+ * started at (fetchgeneric_p.h:1172-1173, 1227-1230).  That accumulation is
+ * kept here.
+ */
+#include <math.h>
+#include <string.h>
+
+#include "orc_core.h"
+
+/* pixelgeneric_p.h:386-389 (x <= 65025 + 128) */
+uint32_t orc_div255(uint32_t x) {
+  uint32_t u = x + 0x80u;
+  return ((u + ((u >> 8) & 0xFFu)) >> 8) & 0xFFu;
+}
+
+static inline uint32_t ch(uint32_t p, int i) { return (p >> (i * 8)) & 0xFFu; }
+
+/* x86 cvttss2si / cvtss2si behaviour: out-of-range and NaN give INT32_MIN. */
+static inline int32_t trunc_f32_to_i32(float v) {
+  if (!(v >= -2147483648.0f && v < 2147483648.0f)) return INT32_MIN;
+  return (int32_t)v;
+}
+static inline int32_t rint_f32_to_i32(float v) {
+  if (!(v >= -2147483648.0f && v < 2147483648.0f)) return INT32_MIN;
+  return (int32_t)rintf(v);
+}
+
+/* ---- source texel fetch (PixelIO, pixelgeneric_p.h:591-676) ---- */
+static inline uint32_t tex_fetch(const OrcFetch* f, uint32_t fmt, uint32_t x, uint32_t y) {
+  const uint8_t* row = f->img + (intptr_t)y * f->img_stride;
+  if (fmt == ORC_FMT_A8) return (uint32_t)row[x] * 0x01010101u;
+  uint32_t v;
+  memcpy(&v, row + (size_t)x * 4u, 4);
+  if (fmt == ORC_FMT_XRGB32) v |= 0xFF000000u;
+  return v;
+}
+
+/* FetchPatternAffineCtx::normalize_px_py (fetchgeneric_p.h:373-384) */
+static inline uint64_t pa_norm(int32_t tw, int32_t rx, int32_t ox, uint64_t v) {
+  uint32_t x = (uint32_t)((int32_t)(v >> 32) % tw);
+  if ((int32_t)x < 0) x += (uint32_t)rx;
+  if ((int32_t)x > ox) x -= (uint32_t)rx;
+  return ((uint64_t)x << 32) | (v & 0xFFFFFFFFu);
+}
+
+static inline int32_t pa_index(int32_t v, int32_t mn, int32_t mx, int32_t cor) {
+  if (v < mn) v = mn;
+  if (v > mx) v = cor;
+  return v ^ (v >> 31);
+}
+
+static uint32_t fetch_px(const OrcPipe* p, uint32_t x, uint32_t y) {
+  const OrcFetch* f = p->fetch;
+  switch (f->type) {
+    case ORC_FETCH_SOLID:
+      return f->solid_prgb32;
+
+    case ORC_FETCH_LINEAR: {
+      uint64_t pt = f->lin_pt0 + (uint64_t)y * f->lin_dy + (uint64_t)x * f->lin_dt;
+      uint32_t idx = (uint32_t)(pt >> 32);
+      if (f->extend == 0) {
+        int32_t i = (int32_t)idx;
+        if (i < 0) i = 0;
+        if (i > (int32_t)f->maxi) i = (int32_t)f->maxi;
+        idx = (uint32_t)i;
+      }
+      else {
+        uint32_t a = idx & f->maxi, b = a ^ f->rori;
+        idx = b < a ? b : a;
+      }
+      return f->lut[idx];
+    }
+
+    case ORC_FETCH_RADIAL: {
+      double yd = (double)(int32_t)y;
+      double ptx = f->rad_yx * yd + f->rad_tx;
+      double pty = f->rad_yy * yd + f->rad_ty;
+      double b = yd * f->rad_by + f->rad_b0;
+      double sq_dist = ptx * ptx + pty * pty;
+      float fb = (float)(b * f->rad_inv2a);
+      float fd = (float)((f->rad_amul4 * (sq_dist - f->rad_sq_fr) + b * b) * f->rad_sq_inv2a);
+      float fdd = (float)((yd * f->rad_ddy + f->rad_dd0) * f->rad_sq_inv2a);
+      float fx = (float)(int32_t)x;
+      float sq_x = fx * fx;
+      float a = sqrtf(fabsf(sq_x * f->rad_ddd + (fx * fdd + fd)));
+      float v = (fx * f->rad_bd + fb) + a;
+      uint32_t idx = (uint32_t)trunc_f32_to_i32(v);
+      if (f->extend == 0) {
+        int32_t i = (int32_t)idx;
+        if (i < 0) i = 0;
+        if (i > (int32_t)f->maxi) i = (int32_t)f->maxi;
+        idx = (uint32_t)i;
+      }
+      else {
+        uint32_t aa = idx & f->maxi, bb = aa ^ f->rori;
+        idx = bb < aa ? bb : aa;
+      }
+      return f->lut[idx];
+    }
+
+    case ORC_FETCH_CONIC: {
+      float tx = (float)p->con_tp_x;
+      float ay = (float)p->con_tp_y;
+      float by = signbit(ay) ? f->con_n1 : 0.0f;
+      ay = fabsf(ay);
+      float fx = (float)(int32_t)x;
+      float xv = fx * f->con_xx + tx;
+      float ax = fabsf(xv);
+      float xy_min = ay < ax ? ay : ax;
+      float xy_max = ax < ay ? ay : ax;
+      float s = (ax == xy_min) ? f->con_n4 : 0.0f;
+      float pp = xy_min / xy_max;
+      float p_sq = pp * pp;
+      float v = p_sq * f->con_q[3] + f->con_q[2];
+      v = v * p_sq + f->con_q[1];
+      v = v * p_sq + f->con_q[0];
+      v = fabsf(v * pp - s);
+      v = fabsf(v - (signbit(xv) ? f->con_n2 : 0.0f));
+      v = fabsf(v - by) + f->con_offset;
+      int32_t i = rint_f32_to_i32(v);
+      if (i > f->con_maxi) i = f->con_maxi;
+      return f->lut[(uint32_t)i & (uint32_t)f->con_rori];
+    }
+
+    case ORC_FETCH_PATTERN_AFFINE_NN:
+    case ORC_FETCH_PATTERN_AFFINE_BI: {
+      uint64_t tx = f->pa_tx + f->pa_yx * (uint64_t)y;
+      uint64_t ty = f->pa_ty + f->pa_yy * (uint64_t)y;
+      uint64_t px = pa_norm(f->pa_tw, f->pa_rx, f->pa_ox, tx + f->pa_xx * (uint64_t)x);
+      uint64_t py = pa_norm(f->pa_th, f->pa_ry, f->pa_oy, ty + f->pa_xy * (uint64_t)x);
+      int32_t ix = (int32_t)(px >> 32), iy = (int32_t)(py >> 32);
+      uint32_t fmt = p->src_fmt;
+      if (f->type == ORC_FETCH_PATTERN_AFFINE_NN) {
+        int32_t xi = pa_index(ix, f->pa_minx, f->pa_maxx, f->pa_corx);
+        int32_t yi = pa_index(iy, f->pa_miny, f->pa_maxy, f->pa_cory);
+        return tex_fetch(f, fmt, (uint32_t)xi, (uint32_t)yi);
+      }
+      int32_t x0 = pa_index(ix, f->pa_minx, f->pa_maxx, f->pa_corx);
+      int32_t y0 = pa_index(iy, f->pa_miny, f->pa_maxy, f->pa_cory);
+      int32_t x1 = pa_index(ix + 1, f->pa_minx, f->pa_maxx, f->pa_corx);
+      int32_t y1 = pa_index(iy + 1, f->pa_miny, f->pa_maxy, f->pa_cory);
+      uint32_t wx = (uint32_t)(px & 0xFFFFFFFFu) >> 24;
+      uint32_t wy = (uint32_t)(py & 0xFFFFFFFFu) >> 24;
+      uint32_t ixw = 256 - wx, iyw = 256 - wy;
+      uint32_t t00 = tex_fetch(f, fmt, (uint32_t)x0, (uint32_t)y0);
+      uint32_t t01 = tex_fetch(f, fmt, (uint32_t)x0, (uint32_t)y1);
+      uint32_t t10 = tex_fetch(f, fmt, (uint32_t)x1, (uint32_t)y0);
+      uint32_t t11 = tex_fetch(f, fmt, (uint32_t)x1, (uint32_t)y1);
+      uint32_t out = 0;
+      for (int c = 0; c < 4; c++) {
+        uint32_t p0 = ((ch(t00, c) * iyw + ch(t01, c) * wy) >> 8) & 0xFFu;
+        uint32_t p1 = ((ch(t10, c) * iyw + ch(t11, c) * wy) >> 8) & 0xFFu;
+        p0 *= ixw;
+        p1 *= wx;
+        out |= (((p0 + p1) >> 8) & 0xFFu) << (c * 8);
+      }
+      return out;
+    }
+
+    default:
+      return 0;
+  }
+}
+
+void orc_pipe_begin(OrcPipe* p, uint32_t y0) {
+  const OrcFetch* f = p->fetch;
+  p->y_start = y0;
+  p->con_row = (int32_t)y0;
+  if (f->type == ORC_FETCH_CONIC) {
+    p->con_tp_x = f->con_tx + f->con_yx * (double)(int32_t)y0;
+    p->con_tp_y = f->con_ty + f->con_yy * (double)(int32_t)y0;
+  }
+}
+
+void orc_pipe_row(OrcPipe* p, uint32_t y) {
+  const OrcFetch* f = p->fetch;
+  if (f->type == ORC_FETCH_CONIC) {
+    while (p->con_row < (int32_t)y) {
+      p->con_tp_x += f->con_yx;
+      p->con_tp_y += f->con_yy;
+      p->con_row++;
+    }
+  }
+}
+
+/* ---- composition (per channel on 8-bit values) ---- */
+
+static inline uint32_t op_channel(uint32_t op, uint32_t d, uint32_t s, uint32_t da, uint32_t sa, uint32_t m) {
+  (void)da; (void)sa;
+  switch (op) {
+    case ORC_OP_SRC_COPY:
+      return orc_div255(d * (m ^ 0xFFu) + s * m);
+    case ORC_OP_PLUS: {
+      uint32_t v = d + orc_div255(s * m);
+      return v > 255u ? 255u : v;
+    }
+    default:
+      return d;
+  }
+}
+
+void orc_pipe_composite(OrcPipe* p, OrcCanvas* c, uint32_t x, uint32_t y, uint32_t m) {
+  uint8_t* row = c->pixels + (intptr_t)y * c->stride;
+  uint32_t s = fetch_px(p, x, y);
+  uint32_t op = p->op;
+
+  if (c->format == ORC_FMT_A8) {
+    uint32_t d = row[x];
+    uint32_t sa = s >> 24;
+    uint32_t r;
+    switch (op) {
+      case ORC_OP_SRC_COPY: r = orc_div255(d * (m ^ 0xFFu) + sa * m); break;
+      case ORC_OP_SRC_OVER: {
+        uint32_t s1 = orc_div255(sa * m);
+        r = (s1 + orc_div255(d * (s1 ^ 0xFFu))) & 0xFFu;
+        break;
+      }
+      case ORC_OP_PLUS: { uint32_t v = d + orc_div255(sa * m); r = v > 255u ? 255u : v; break; }
+      default: r = d; break;
+    }
+    row[x] = (uint8_t)r;
+    return;
+  }
+
+  uint32_t d;
+  memcpy(&d, row + (size_t)x * 4u, 4);
+  uint32_t r = 0;
+  switch (op) {
+    case ORC_OP_SRC_COPY:
+    case ORC_OP_PLUS:
+      for (int i = 0; i < 4; i++) r |= op_channel(op, ch(d, i), ch(s, i), 0, 0, m) << (i * 8);
+      break;
+    case ORC_OP_SRC_OVER: {
+      uint32_t s1 = 0;
+      for (int i = 0; i < 4; i++) s1 |= orc_div255(ch(s, i) * m) << (i * 8);
+      uint32_t ia = (s1 >> 24) ^ 0xFFu, t = 0;
+      for (int i = 0; i < 4; i++) t |= orc_div255(ch(d, i) * ia) << (i * 8);
+      r = s1 + t;   /* packed 32-bit add, as P32_8888::operator+ */
+      break;
+    }
+    case ORC_OP_MULTIPLY: {
+      /* Dca' = Dca.(Sca.m + 1 - Sa.m) + Sca.m.(1 - Da), one div255 per channel */
+      uint32_t s1[4];
+      for (int i = 0; i < 4; i++) s1[i] = orc_div255(ch(s, i) * m);
+      uint32_t da = ch(d, 3);
+      for (int i = 0; i < 4; i++) {
+        uint32_t v = ch(d, i) * ((s1[i] + (s1[3] ^ 0xFFu)) & 0xFFFFu) + s1[i] * (da ^ 0xFFu);
+        r |= orc_div255(v & 0xFFFFu) << (i * 8);
+      }
+      break;
+    }
+    case ORC_OP_SCREEN: {
+      /* Dca' = Sca.m + Dca.(1 - Sca.m) */
+      for (int i = 0; i < 4; i++) {
+        uint32_t s1 = orc_div255(ch(s, i) * m);
+        r |= ((orc_div255(ch(d, i) * (s1 ^ 0xFFu)) + s1) & 0xFFu) << (i * 8);
+      }
+      break;
+    }
+    default:
+      r = d;
+      break;
+  }
+  memcpy(row + (size_t)x * 4u, &r, 4);
+}

@@ -1,0 +1,464 @@
+/* CPU oracle -- scene replay (the BLContext state machine of the sync path).
+ *
+ * TEST INFRASTRUCTURE (see orc_core.h).  Restates the frontend decisions of
+ * raster/rastercontext.cpp for the fill path:
+ *   resolve_*_style_op / formatFromRgba32     (:90-93, :1231-1342)
+ *   init_non_solid_fetch_data                  (:363-494)
+ *   fill_unclipped_path / polygon / box / rect (:2787-2917)
+ *   set_global_alpha                           (:2000-2029)
+ *   defaults after attach                      (:4383-4411)
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "orc.h"
+#include "orc_core.h"
+#include "scene_format.h"
+
+#define ERR_INVALID_VALUE 0x10001u
+#define ERR_NOT_IMPLEMENTED 0x10007u
+
+typedef struct { const uint8_t* p; const uint8_t* end; } Rd;
+
+static int rd(Rd* r, void* out, size_t n) {
+  if ((size_t)(r->end - r->p) < n) return 0;
+  memcpy(out, r->p, n);
+  r->p += n;
+  return 1;
+}
+static size_t pad8(size_t n) { return (n + 7u) & ~(size_t)7u; }
+
+typedef struct {
+  uint32_t type;      /* 0 solid, 1 gradient, 2 pattern */
+  uint32_t rgba32;
+  SceneGradient g;
+  OrcStop stops[64];
+  size_t n_stops;
+  ScenePattern pat;
+  OrcMat xform;       /* user transform captured by set_fill_style (rastercontext.cpp:653-695) */
+  uint32_t quality;   /* pattern quality hint captured by set_fill_style */
+} Style;
+
+typedef struct {
+  OrcCanvas canvas;
+  uint32_t comp_op, fill_rule, pattern_quality;
+  OrcMat user;
+  double global_alpha;
+  Style style;
+  SceneImage img_info[16];
+  const uint8_t* img_data[16];
+  uint32_t n_images;
+  OrcEdges edges;
+  uint32_t first_error;
+} Ctx;
+
+static void note(Ctx* c, uint32_t err) { if (err && !c->first_error) c->first_error = err; }
+
+static uint32_t alpha_i(const Ctx* c) {
+  return (uint32_t)orc_round_to_int(c->global_alpha * 255.0 * 1.0);
+}
+
+/* core/gradient.cpp:751-800 (add_stop keeps stops sorted; a third stop at an
+ * existing offset replaces the second one). */
+static void add_stop(Style* s, double offset, uint32_t rgba32) {
+  uint64_t v = 0;
+  for (int i = 0; i < 4; i++) {
+    uint64_t c = (rgba32 >> (i * 8)) & 0xFFu;
+    v |= (c * 0x101u) << (i * 16);
+  }
+  size_t n = s->n_stops, i = 0;
+  if (n && offset >= s->stops[0].offset) {
+    /* last index with offset <= new offset */
+    size_t k = 0;
+    while (k + 1 < n && s->stops[k + 1].offset <= offset) k++;
+    i = k;
+    if (i > 0 && s->stops[i - 1].offset == offset) {
+      s->stops[i].offset = offset;
+      s->stops[i].rgba64 = v;
+      return;
+    }
+    i++;
+  }
+  if (n >= 64) return;
+  memmove(&s->stops[i + 1], &s->stops[i], (n - i) * sizeof(OrcStop));
+  s->stops[i].offset = offset;
+  s->stops[i].rgba64 = v;
+  s->n_stops = n + 1;
+}
+
+/* Resolves style + comp op into a pipeline (fetch + op).  Returns 0 when the
+ * call renders, 1 when it is a NOP, or an error code. */
+static uint32_t resolve(Ctx* c, OrcFetch* f, uint32_t* lut_buf, OrcPipe* pipe) {
+  memset(f, 0, sizeof(*f));
+  memset(pipe, 0, sizeof(*pipe));
+  uint32_t src_fmt = ORC_FMT_NONE;
+  int solid = 0;
+  uint32_t lut_size = 0;
+
+  if (alpha_i(c) == 0) return 1;
+
+  Style* st = &c->style;
+  OrcMat ft = st->xform;  /* final_transform (user x identity meta) when the style was set */
+  if (st->type == 0) {
+    uint32_t v = st->rgba32;
+    src_fmt = v == 0 ? ORC_FMT_ZERO32 : (v >= 0xFF000000u ? ORC_FMT_FRGB32 : ORC_FMT_PRGB32);
+    f->type = ORC_FETCH_SOLID;
+    f->solid_prgb32 = orc_premultiply(v);
+  }
+  else if (st->type == 1) {
+    if (st->n_stops == 0) return 1;
+    lut_size = orc_gradient_info(st->stops, st->n_stops, &solid, &src_fmt);
+    if (solid) {
+      uint64_t last = st->stops[st->n_stops - 1].rgba64;
+      uint32_t c32 = (uint32_t)(((last >> 56) & 0xFFu) << 24 | ((last >> 40) & 0xFFu) << 16 | ((last >> 24) & 0xFFu) << 8 | ((last >> 8) & 0xFFu));
+      f->type = ORC_FETCH_SOLID;
+      f->solid_prgb32 = orc_premultiply(c32);
+    }
+  }
+  else {
+    if (st->pat.image_index >= c->n_images) return ERR_INVALID_VALUE;
+    src_fmt = c->img_info[st->pat.image_index].format;
+  }
+
+  uint32_t pd, ps, sid;
+  uint32_t op = orc_simplify(c->comp_op, c->canvas.format, src_fmt, &pd, &ps, &sid);
+  if (op == ORC_OP_NOP) return 1;
+  if (!(op == ORC_OP_SRC_COPY || op == ORC_OP_SRC_OVER || op == ORC_OP_PLUS ||
+        ((op == ORC_OP_MULTIPLY || op == ORC_OP_SCREEN) && pd == ORC_FMT_PRGB32)))
+    return ERR_NOT_IMPLEMENTED;
+  if (ps == ORC_FMT_A8 && pd != ORC_FMT_A8 && op != ORC_OP_SRC_COPY && op != ORC_OP_SRC_OVER && op != ORC_OP_PLUS)
+    return ERR_NOT_IMPLEMENTED;
+
+  pipe->op = op;
+  pipe->dst_fmt = pd;
+  pipe->src_fmt = ps;
+  pipe->fetch = f;
+
+  if (sid != ORC_SOLID_NONE) {
+    f->type = ORC_FETCH_SOLID;
+    f->solid_prgb32 = sid == ORC_SOLID_TRANSPARENT ? 0u : sid == ORC_SOLID_OPAQUE_BLACK ? 0xFF000000u : 0xFFFFFFFFu;
+    return 0;
+  }
+
+  if (st->type == 1 && !solid) {
+    OrcMat t = ft;
+    if (st->g.has_transform) {
+      OrcMat gm = { st->g.transform[0], st->g.transform[1], st->g.transform[2], st->g.transform[3], st->g.transform[4], st->g.transform[5] };
+      if (orc_mat_type(&gm) != ORC_TT_IDENTITY) orc_mat_multiply(&t, &gm, &ft);
+    }
+    orc_interpolate_lut(lut_buf, lut_size, st->stops, st->n_stops);
+    f->lut = lut_buf;
+    f->lut_size = lut_size;
+    int r;
+    if (st->g.type == 0) r = orc_init_linear(f, st->g.values, st->g.extend, &t);
+    else if (st->g.type == 1) r = orc_init_radial(f, st->g.values, st->g.extend, &t);
+    else r = orc_init_conic(f, st->g.values, &t);
+    if (r != 0) return 1;
+  }
+  else if (st->type == 2) {
+    OrcMat t = ft;
+    if (st->pat.has_transform) {
+      OrcMat pm = { st->pat.transform[0], st->pat.transform[1], st->pat.transform[2], st->pat.transform[3], st->pat.transform[4], st->pat.transform[5] };
+      if (orc_mat_type(&pm) != ORC_TT_IDENTITY) orc_mat_multiply(&t, &pm, &ft);
+    }
+    const SceneImage* ii = &c->img_info[st->pat.image_index];
+    f->img = c->img_data[st->pat.image_index];
+    f->img_stride = (intptr_t)ii->stride;
+    f->img_w = ii->width;
+    f->img_h = ii->height;
+    f->img_format = ii->format;
+    int r = orc_init_pattern(f, st->pat.extend, st->quality, &t);
+    if (r < 0) return 1;
+    if (r > 0) return ERR_NOT_IMPLEMENTED;
+  }
+  return 0;
+}
+
+static void fixed_transform(const Ctx* c, OrcMat* m, int* tt) {
+  *m = c->user;
+  m->m00 *= 256.0; m->m01 *= 256.0; m->m10 *= 256.0; m->m11 *= 256.0; m->m20 *= 256.0; m->m21 *= 256.0;
+  int t = orc_mat_type(&c->user);
+  *tt = t < ORC_TT_SCALE ? ORC_TT_SCALE : t;
+}
+
+static void clip_fixed(const Ctx* c, OrcBox* b) {
+  b->x0 = 0.0; b->y0 = 0.0;
+  b->x1 = (double)c->canvas.w * 256.0;
+  b->y1 = (double)c->canvas.h * 256.0;
+}
+
+static const double kToleranceSq = (0.2 * 256.0) * (0.2 * 256.0);
+
+static void fill_edges(Ctx* c, uint32_t fill_rule) {
+  OrcFetch f;
+  OrcPipe p;
+  static uint32_t lut[2048];
+  uint32_t r = resolve(c, &f, lut, &p);
+  if (r > 1) { note(c, r); return; }
+  if (r == 1) return;
+  orc_fill_analytic(&c->canvas, &c->edges, &p, alpha_i(c), fill_rule);
+}
+
+static void fill_path(Ctx* c, const uint8_t* cmds, const OrcPt* vtx, size_t n) {
+  OrcMat m; int tt; OrcBox clip;
+  fixed_transform(c, &m, &tt);
+  clip_fixed(c, &clip);
+  orc_edges_clear(&c->edges);
+  orc_edges_add_path(&c->edges, cmds, vtx, n, &m, tt, &clip, kToleranceSq);
+  fill_edges(c, c->fill_rule);
+}
+
+static void fill_poly(Ctx* c, const OrcPt* vtx, size_t n, uint32_t rule) {
+  if (n < 3) return;
+  OrcMat m; int tt; OrcBox clip;
+  fixed_transform(c, &m, &tt);
+  clip_fixed(c, &clip);
+  orc_edges_clear(&c->edges);
+  orc_edges_add_poly(&c->edges, vtx, n, &m, tt, &clip, kToleranceSq);
+  fill_edges(c, rule);
+}
+
+static void fill_box_i(Ctx* c, int x0, int y0, int x1, int y1) {
+  OrcFetch f;
+  OrcPipe p;
+  static uint32_t lut[2048];
+  uint32_t r = resolve(c, &f, lut, &p);
+  if (r > 1) { note(c, r); return; }
+  if (r == 1) return;
+  orc_fill_box_a(&c->canvas, &p, alpha_i(c), x0, y0, x1, y1);
+}
+
+/* fill_unclipped_box_d (rastercontext.cpp:2871-2891) */
+static void fill_box_d(Ctx* c, double bx0, double by0, double bx1, double by1) {
+  OrcMat m; int tt; OrcBox clip;
+  fixed_transform(c, &m, &tt);
+  clip_fixed(c, &clip);
+  if (tt <= ORC_TT_SWAP) {
+    double x0 = bx0 * m.m00 + by0 * m.m10 + m.m20;
+    double y0 = bx0 * m.m01 + by0 * m.m11 + m.m21;
+    double x1 = bx1 * m.m00 + by1 * m.m10 + m.m20;
+    double y1 = bx1 * m.m01 + by1 * m.m11 + m.m21;
+    OrcBox b = { orc_min(x0, x1), orc_min(y0, y1), orc_max(x0, x1), orc_max(y0, y1) };
+    OrcBox d = { orc_max(b.x0, clip.x0), orc_max(b.y0, clip.y0), orc_min(b.x1, clip.x1), orc_min(b.y1, clip.y1) };
+    if (!((d.x0 < d.x1) & (d.y0 < d.y1))) return;
+    int ix0 = orc_trunc(d.x0), iy0 = orc_trunc(d.y0), ix1 = orc_trunc(d.x1), iy1 = orc_trunc(d.y1);
+    if (ix0 >= ix1 || iy0 >= iy1) return;
+
+    OrcFetch f;
+    OrcPipe p;
+    static uint32_t lut[2048];
+    uint32_t r = resolve(c, &f, lut, &p);
+    if (r > 1) { note(c, r); return; }
+    if (r == 1) return;
+    if (((ix0 | iy0 | ix1 | iy1) & 0xFF) == 0)
+      orc_fill_box_a(&c->canvas, &p, alpha_i(c), ix0 >> 8, iy0 >> 8, ix1 >> 8, iy1 >> 8);
+    else
+      orc_fill_box_u(&c->canvas, &p, alpha_i(c), ix0, iy0, ix1, iy1);
+  }
+  else {
+    OrcPt poly[4] = { { bx0, by0 }, { bx1, by0 }, { bx1, by1 }, { bx0, by1 } };
+    fill_poly(c, poly, 4, 1 /* EVEN_ODD */);
+  }
+}
+
+/* fillUnclippedRectI (rastercontext.cpp:2898-2917) */
+static void fill_rect_i(Ctx* c, int x, int y, int w, int h) {
+  int tt = orc_mat_type(&c->user);
+  int integral = 0, tx = 0, ty = 0;
+  if (tt <= ORC_TT_TRANSLATE) {
+    double fx = c->user.m20 * 256.0, fy = c->user.m21 * 256.0;
+    int64_t tx64 = orc_floor_to_int64(fx), ty64 = orc_floor_to_int64(fy);
+    if (((tx64 | ty64) & 0xFF) == 0) { integral = 1; tx = (int)(tx64 >> 8); ty = (int)(ty64 >> 8); }
+  }
+  if (!integral) {
+    if (w <= 0 || h <= 0) return;
+    fill_box_d(c, (double)x, (double)y, (double)x + (double)w, (double)y + (double)h);
+    return;
+  }
+  int64_t x0 = (int64_t)x + tx, y0 = (int64_t)y + ty;
+  int64_t x1 = (int64_t)w + x0, y1 = (int64_t)h + y0;
+  if (x0 < 0) x0 = 0;
+  if (y0 < 0) y0 = 0;
+  if (x1 > (int64_t)c->canvas.w) x1 = c->canvas.w;
+  if (y1 > (int64_t)c->canvas.h) y1 = c->canvas.h;
+  if (x0 >= x1 || y0 >= y1) return;
+  fill_box_i(c, (int)x0, (int)y0, (int)x1, (int)y1);
+}
+
+uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_size) {
+  Rd r = { scene, scene + size };
+  SceneHeader hdr;
+  if (!rd(&r, &hdr, sizeof(hdr)) || hdr.magic != SCENE_MAGIC || hdr.version != SCENE_VERSION) return 0x7FFFFFFFu;
+  uint32_t bpp = hdr.format == SCENE_FMT_A8 ? 1u : 4u;
+  if (out_size < (size_t)hdr.width * hdr.height * bpp) return 0x7FFFFFFFu;
+  if (hdr.n_images > 16) return 0x7FFFFFFFu;
+
+  Ctx c;
+  memset(&c, 0, sizeof(c));
+  c.canvas.pixels = out;
+  c.canvas.stride = (intptr_t)hdr.width * bpp;
+  c.canvas.w = hdr.width;
+  c.canvas.h = hdr.height;
+  c.canvas.format = hdr.format;
+  c.comp_op = ORC_OP_SRC_OVER;
+  c.fill_rule = 0;
+  c.pattern_quality = 1;
+  c.user.m00 = 1.0; c.user.m11 = 1.0;
+  c.global_alpha = 1.0;
+  c.style.type = 0;
+  c.style.rgba32 = 0xFF000000u;
+  orc_edges_init(&c.edges);
+
+  for (uint32_t y = 0; y < hdr.height; y++) {
+    uint8_t* row = out + (size_t)y * c.canvas.stride;
+    if (bpp == 1) memset(row, (int)(hdr.clear_value & 0xFFu), hdr.width);
+    else for (uint32_t x = 0; x < hdr.width; x++) memcpy(row + x * 4u, &hdr.clear_value, 4);
+  }
+
+  for (uint32_t i = 0; i < hdr.n_images; i++) {
+    if (!rd(&r, &c.img_info[i], sizeof(SceneImage))) return 0x7FFFFFFFu;
+    c.img_data[i] = r.p;
+    r.p += pad8((size_t)c.img_info[i].height * c.img_info[i].stride);
+  }
+  c.n_images = hdr.n_images;
+
+  for (uint32_t i = 0; i < hdr.n_cmds; i++) {
+    SceneCmd cmd;
+    if (!rd(&r, &cmd, sizeof(cmd)) || (size_t)(r.end - r.p) < cmd.size) { orc_edges_free(&c.edges); return 0x7FFFFFFFu; }
+    const uint8_t* pl = r.p;
+    r.p += cmd.size;
+    switch (cmd.op) {
+      case SCENE_OP_SET_COMP_OP: memcpy(&c.comp_op, pl, 4); break;
+      case SCENE_OP_SET_FILL_RULE: memcpy(&c.fill_rule, pl, 4); break;
+      case SCENE_OP_SET_PATTERN_QUALITY: memcpy(&c.pattern_quality, pl, 4); break;
+      case SCENE_OP_SET_TRANSFORM: memcpy(&c.user, pl, 48); break;
+      case SCENE_OP_SET_GLOBAL_ALPHA: {
+        double a;
+        memcpy(&a, pl, 8);
+        if (isnan(a)) { note(&c, ERR_INVALID_VALUE); break; }
+        c.global_alpha = orc_clamp(a, 0.0, 1.0);
+        break;
+      }
+      case SCENE_OP_SET_STYLE_SOLID:
+        c.style.type = 0;
+        memcpy(&c.style.rgba32, pl, 4);
+        break;
+      case SCENE_OP_SET_STYLE_GRADIENT: {
+        c.style.type = 1;
+        c.style.xform = c.user;
+        memcpy(&c.style.g, pl, sizeof(SceneGradient));
+        c.style.n_stops = 0;
+        for (uint32_t k = 0; k < c.style.g.n_stops; k++) {
+          SceneStop s;
+          memcpy(&s, pl + sizeof(SceneGradient) + k * sizeof(SceneStop), sizeof(s));
+          if (!(s.offset >= 0.0 && s.offset <= 1.0)) continue;
+          add_stop(&c.style, s.offset, s.rgba32);
+        }
+        break;
+      }
+      case SCENE_OP_SET_STYLE_PATTERN:
+        c.style.type = 2;
+        c.style.xform = c.user;
+        c.style.quality = c.pattern_quality;
+        memcpy(&c.style.pat, pl, sizeof(ScenePattern));
+        break;
+      case SCENE_OP_FILL_PATH: {
+        uint32_t n;
+        memcpy(&n, pl, 4);
+        const uint8_t* cmds = pl + 8;
+        OrcPt* vtx = (OrcPt*)malloc((size_t)n * sizeof(OrcPt) + 1);
+        memcpy(vtx, cmds + pad8(n), (size_t)n * sizeof(OrcPt));
+        if (n) fill_path(&c, cmds, vtx, n);
+        free(vtx);
+        break;
+      }
+      case SCENE_OP_FILL_POLYGON: {
+        uint32_t n;
+        memcpy(&n, pl, 4);
+        OrcPt* vtx = (OrcPt*)malloc((size_t)n * sizeof(OrcPt) + 1);
+        memcpy(vtx, pl + 8, (size_t)n * sizeof(OrcPt));
+        fill_poly(&c, vtx, n, c.fill_rule);
+        free(vtx);
+        break;
+      }
+      case SCENE_OP_FILL_RECT_D: {
+        double d[4];
+        memcpy(d, pl, 32);
+        fill_box_d(&c, d[0], d[1], d[0] + d[2], d[1] + d[3]);
+        break;
+      }
+      case SCENE_OP_FILL_RECT_I: {
+        int32_t v[4];
+        memcpy(v, pl, 16);
+        fill_rect_i(&c, v[0], v[1], v[2], v[3]);
+        break;
+      }
+      case SCENE_OP_FILL_ALL:
+        fill_box_i(&c, 0, 0, (int)c.canvas.w, (int)c.canvas.h);
+        break;
+      default:
+        break;
+    }
+  }
+  orc_edges_free(&c.edges);
+  return c.first_error;
+}
+
+/* ---- test hooks ---- */
+
+size_t orc_path_segments(const uint8_t* cmds, const double* vtx, size_t n, const double* m6,
+                         uint32_t width, uint32_t height, int32_t* out, size_t cap) {
+  OrcEdges e;
+  orc_edges_init(&e);
+  OrcMat user = { m6[0], m6[1], m6[2], m6[3], m6[4], m6[5] };
+  OrcMat m = user;
+  m.m00 *= 256.0; m.m01 *= 256.0; m.m10 *= 256.0; m.m11 *= 256.0; m.m20 *= 256.0; m.m21 *= 256.0;
+  int tt = orc_mat_type(&user);
+  if (tt < ORC_TT_SCALE) tt = ORC_TT_SCALE;
+  OrcBox clip = { 0.0, 0.0, (double)width * 256.0, (double)height * 256.0 };
+  orc_edges_add_path(&e, cmds, (const OrcPt*)vtx, n, &m, tt, &clip, kToleranceSq);
+  size_t cnt = e.n;
+  for (size_t i = 0; i < cnt && i < cap; i++) {
+    out[i * 5 + 0] = e.segs[i].x0; out[i * 5 + 1] = e.segs[i].y0;
+    out[i * 5 + 2] = e.segs[i].x1; out[i * 5 + 3] = e.segs[i].y1;
+    out[i * 5 + 4] = (int32_t)e.segs[i].sign;
+  }
+  orc_edges_free(&e);
+  return cnt;
+}
+
+void orc_gradient_lut(const double* offsets, const uint32_t* rgba32, size_t n, uint32_t* out, uint32_t* lut_size_out) {
+  Style s;
+  memset(&s, 0, sizeof(s));
+  for (size_t i = 0; i < n; i++) add_stop(&s, offsets[i], rgba32[i]);
+  int solid;
+  uint32_t fmt;
+  uint32_t lut_size = orc_gradient_info(s.stops, s.n_stops, &solid, &fmt);
+  *lut_size_out = lut_size;
+  if (lut_size) orc_interpolate_lut(out, lut_size, s.stops, s.n_stops);
+}
+
+uint32_t orc_composite_pixel(uint32_t op, uint32_t dst_format, uint32_t d, uint32_t s, uint32_t m) {
+  OrcFetch f;
+  memset(&f, 0, sizeof(f));
+  f.type = ORC_FETCH_SOLID;
+  f.solid_prgb32 = s;
+  OrcPipe p;
+  memset(&p, 0, sizeof(p));
+  p.fetch = &f;
+  p.op = op;
+  uint8_t px[4];
+  memcpy(px, &d, 4);
+  OrcCanvas c = { px, 4, 1, 1, dst_format };
+  orc_pipe_composite(&p, &c, 0, 0, m);
+  uint32_t r = 0;
+  memcpy(&r, px, dst_format == ORC_FMT_A8 ? 1 : 4);
+  return r;
+}
+
+uint32_t orc_simplify_op(uint32_t op, uint32_t dst_fmt, uint32_t src_fmt) {
+  uint32_t d, s, id;
+  uint32_t r = orc_simplify(op, dst_fmt, src_fmt, &d, &s, &id);
+  return (r & 0xFFu) | (d << 8) | (s << 16) | (id << 24);
+}

@@ -1,0 +1,221 @@
+// Scene driver for the reference Blend2D library built by oracle/ref.mk.
+//
+// Test infrastructure only: it replays a scene (oracle/scene_format.h) through
+// the reference's public C++ API (BLContext, BLPath, BLGradient, BLPattern)
+// so the tests can compare the CPU oracle and the MI355X backend against the
+// real reference, and bench.py can time the reference CPU pipeline.  It is
+// never linked into the product library.
+
+#include <blend2d/blend2d.h>
+
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "scene_format.h"
+
+namespace {
+
+struct Reader {
+  const uint8_t* p;
+  const uint8_t* end;
+  template<typename T> bool get(T& out) {
+    if (size_t(end - p) < sizeof(T)) return false;
+    std::memcpy(&out, p, sizeof(T));
+    p += sizeof(T);
+    return true;
+  }
+};
+
+static inline size_t pad8(size_t n) { return (n + 7u) & ~size_t(7); }
+
+static BLMatrix2D matrix_from(const double m[6]) {
+  return BLMatrix2D(m[0], m[1], m[2], m[3], m[4], m[5]);
+}
+
+// A decoded call; objects are built before the timed region starts.
+struct Call {
+  uint32_t op = 0;
+  uint32_t u0 = 0;
+  double d[6] = {};
+  BLPath path;
+  std::vector<BLPoint> poly;
+  BLGradient gradient;
+  BLPattern pattern;
+};
+
+}  // namespace
+
+extern "C" {
+
+// Renders `scene` and writes the canvas rows (width * bytes-per-pixel each,
+// no padding) to `out`.  Returns the first non-zero BLResult reported by any
+// call (rendering continues after an error, as a caller ignoring errors
+// would), or 0x7FFFFFFF when the scene cannot be parsed.  `elapsed_ms`
+// receives the wall time from BLContext::begin() to BLContext::end().
+uint32_t ref_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_size,
+                    uint32_t thread_count, double* elapsed_ms) {
+  Reader r{scene, scene + size};
+  SceneHeader hdr;
+  if (!r.get(hdr) || hdr.magic != SCENE_MAGIC || hdr.version != SCENE_VERSION)
+    return 0x7FFFFFFFu;
+
+  const uint32_t bpp = hdr.format == SCENE_FMT_A8 ? 1u : 4u;
+  if (out_size < size_t(hdr.width) * hdr.height * bpp)
+    return 0x7FFFFFFFu;
+
+  std::vector<BLImage> images;
+  for (uint32_t i = 0; i < hdr.n_images; i++) {
+    SceneImage si;
+    if (!r.get(si)) return 0x7FFFFFFFu;
+    BLImage img(int(si.width), int(si.height), BLFormat(si.format));
+    BLImageData d;
+    img.make_mutable(&d);
+    uint32_t ibpp = si.format == SCENE_FMT_A8 ? 1u : 4u;
+    for (uint32_t y = 0; y < si.height; y++)
+      std::memcpy(static_cast<uint8_t*>(d.pixel_data) + intptr_t(y) * d.stride, r.p + size_t(y) * si.stride, size_t(si.width) * ibpp);
+    r.p += pad8(size_t(si.height) * si.stride);
+    images.push_back(img);
+  }
+
+  std::vector<Call> calls(hdr.n_cmds);
+  for (uint32_t i = 0; i < hdr.n_cmds; i++) {
+    SceneCmd c;
+    if (!r.get(c) || size_t(r.end - r.p) < c.size) return 0x7FFFFFFFu;
+    const uint8_t* payload = r.p;
+    r.p += c.size;
+    Call& call = calls[i];
+    call.op = c.op;
+
+    switch (c.op) {
+      case SCENE_OP_SET_COMP_OP:
+      case SCENE_OP_SET_FILL_RULE:
+      case SCENE_OP_SET_STYLE_SOLID:
+      case SCENE_OP_SET_PATTERN_QUALITY:
+        std::memcpy(&call.u0, payload, 4);
+        break;
+      case SCENE_OP_SET_TRANSFORM:
+        std::memcpy(call.d, payload, 48);
+        break;
+      case SCENE_OP_SET_GLOBAL_ALPHA:
+        std::memcpy(call.d, payload, 8);
+        break;
+      case SCENE_OP_FILL_RECT_D:
+        std::memcpy(call.d, payload, 32);
+        break;
+      case SCENE_OP_FILL_RECT_I: {
+        int32_t v[4];
+        std::memcpy(v, payload, 16);
+        for (int k = 0; k < 4; k++) call.d[k] = double(v[k]);
+        break;
+      }
+      case SCENE_OP_FILL_PATH: {
+        uint32_t n;
+        std::memcpy(&n, payload, 4);
+        const uint8_t* cmds = payload + 8;
+        const uint8_t* vtx = cmds + pad8(n);
+        uint8_t* cmd_out;
+        BLPoint* vtx_out;
+        call.path.modify_op(BL_MODIFY_OP_ASSIGN_FIT, n, &cmd_out, &vtx_out);
+        std::memcpy(cmd_out, cmds, n);
+        std::memcpy(vtx_out, vtx, size_t(n) * sizeof(BLPoint));
+        break;
+      }
+      case SCENE_OP_FILL_POLYGON: {
+        uint32_t n;
+        std::memcpy(&n, payload, 4);
+        call.poly.resize(n);
+        std::memcpy(call.poly.data(), payload + 8, size_t(n) * sizeof(BLPoint));
+        break;
+      }
+      case SCENE_OP_SET_STYLE_GRADIENT: {
+        SceneGradient g;
+        std::memcpy(&g, payload, sizeof(g));
+        call.gradient = (
+          g.type == 0 ? BLGradient(BLLinearGradientValues(g.values[0], g.values[1], g.values[2], g.values[3])) :
+          g.type == 1 ? BLGradient(BLRadialGradientValues(g.values[0], g.values[1], g.values[2], g.values[3], g.values[4], g.values[5])) :
+                        BLGradient(BLConicGradientValues(g.values[0], g.values[1], g.values[2], g.values[3])));
+        call.gradient.set_extend_mode(BLExtendMode(g.extend));
+        const uint8_t* sp = payload + sizeof(g);
+        for (uint32_t k = 0; k < g.n_stops; k++) {
+          SceneStop s;
+          std::memcpy(&s, sp + k * sizeof(SceneStop), sizeof(s));
+          call.gradient.add_stop(s.offset, BLRgba32(s.rgba32));
+        }
+        if (g.has_transform)
+          call.gradient.set_transform(matrix_from(g.transform));
+        break;
+      }
+      case SCENE_OP_SET_STYLE_PATTERN: {
+        ScenePattern p;
+        std::memcpy(&p, payload, sizeof(p));
+        if (p.image_index >= images.size()) return 0x7FFFFFFFu;
+        call.pattern = BLPattern(images[p.image_index], BLExtendMode(p.extend));
+        if (p.has_transform)
+          call.pattern.set_transform(matrix_from(p.transform));
+        break;
+      }
+      default:
+        break;
+    }
+  }
+
+  BLImage canvas(int(hdr.width), int(hdr.height), BLFormat(hdr.format));
+  {
+    BLImageData d;
+    canvas.make_mutable(&d);
+    for (uint32_t y = 0; y < hdr.height; y++) {
+      uint8_t* row = static_cast<uint8_t*>(d.pixel_data) + intptr_t(y) * d.stride;
+      if (bpp == 1) {
+        std::memset(row, int(hdr.clear_value & 0xFFu), hdr.width);
+      }
+      else {
+        uint32_t* row32 = reinterpret_cast<uint32_t*>(row);
+        for (uint32_t x = 0; x < hdr.width; x++) row32[x] = hdr.clear_value;
+      }
+    }
+  }
+
+  BLContextCreateInfo ci {};
+  ci.thread_count = thread_count;
+  ci.flags = BL_CONTEXT_CREATE_FLAG_DISABLE_JIT;
+
+  uint32_t first_error = 0;
+  auto note = [&](BLResult res) { if (res != BL_SUCCESS && first_error == 0) first_error = res; };
+
+  auto t0 = std::chrono::steady_clock::now();
+  BLContext ctx;
+  note(ctx.begin(canvas, ci));
+
+  for (Call& call : calls) {
+    switch (call.op) {
+      case SCENE_OP_SET_COMP_OP: note(ctx.set_comp_op(BLCompOp(call.u0))); break;
+      case SCENE_OP_SET_FILL_RULE: note(ctx.set_fill_rule(BLFillRule(call.u0))); break;
+      case SCENE_OP_SET_TRANSFORM: note(ctx.set_transform(matrix_from(call.d))); break;
+      case SCENE_OP_SET_GLOBAL_ALPHA: note(ctx.set_global_alpha(call.d[0])); break;
+      case SCENE_OP_SET_STYLE_SOLID: note(ctx.set_fill_style(BLRgba32(call.u0))); break;
+      case SCENE_OP_SET_STYLE_GRADIENT: note(ctx.set_fill_style(call.gradient)); break;
+      case SCENE_OP_SET_STYLE_PATTERN: note(ctx.set_fill_style(call.pattern)); break;
+      case SCENE_OP_SET_PATTERN_QUALITY: note(ctx.set_hint(BL_CONTEXT_HINT_PATTERN_QUALITY, call.u0)); break;
+      case SCENE_OP_FILL_PATH: note(ctx.fill_path(call.path)); break;
+      case SCENE_OP_FILL_POLYGON: note(ctx.fill_polygon(call.poly.data(), call.poly.size())); break;
+      case SCENE_OP_FILL_RECT_D: note(ctx.fill_rect(BLRect(call.d[0], call.d[1], call.d[2], call.d[3]))); break;
+      case SCENE_OP_FILL_RECT_I: note(ctx.fill_rect(BLRectI(int(call.d[0]), int(call.d[1]), int(call.d[2]), int(call.d[3])))); break;
+      case SCENE_OP_FILL_ALL: note(ctx.fill_all()); break;
+      default: break;
+    }
+  }
+  note(ctx.end());
+  auto t1 = std::chrono::steady_clock::now();
+  if (elapsed_ms)
+    *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+
+  BLImageData d;
+  canvas.get_data(&d);
+  for (uint32_t y = 0; y < hdr.height; y++)
+    std::memcpy(out + size_t(y) * hdr.width * bpp, static_cast<const uint8_t*>(d.pixel_data) + intptr_t(y) * d.stride, size_t(hdr.width) * bpp);
+  return first_error;
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+/* Binary scene format shared by the reference driver (ref_render.cpp), the CPU
+ * oracle (orc_*.c) and the Python scene writer (blend2d_amd/scene.py).
+ *
+ * A scene is a canvas description followed by a list of BLContext calls, so
+ * the same byte string can be replayed through the reference library, through
+ * the oracle and through the MI355X backend.  All fields are little endian.
+ *
+ *   SceneHeader
+ *   n_images x { SceneImage, pixel bytes (h * stride, padded to 8) }
+ *   n_cmds   x { SceneCmd, payload (size bytes, multiple of 8) }
+ *
+ * This header is test infrastructure; the product library never includes it.
+ */
+#ifndef B2AMD_SCENE_FORMAT_H
+#define B2AMD_SCENE_FORMAT_H
+
+#include <stdint.h>
+
+#define SCENE_MAGIC 0x43533242u /* "B2SC" */
+#define SCENE_VERSION 1u
+
+/* Pixel formats, values equal to BLFormat (core/format.h). */
+enum { SCENE_FMT_NONE = 0, SCENE_FMT_PRGB32 = 1, SCENE_FMT_XRGB32 = 2, SCENE_FMT_A8 = 3 };
+
+typedef struct SceneHeader {
+  uint32_t magic;
+  uint32_t version;
+  uint32_t width;
+  uint32_t height;
+  uint32_t format;
+  uint32_t n_images;
+  uint32_t n_cmds;
+  uint32_t clear_value;   /* raw pixel value the canvas starts with (A8 uses the low byte) */
+} SceneHeader;
+
+typedef struct SceneImage {
+  uint32_t width;
+  uint32_t height;
+  uint32_t format;
+  uint32_t stride;        /* bytes per row, the pixel rows follow */
+} SceneImage;
+
+typedef struct SceneCmd {
+  uint32_t op;
+  uint32_t size;          /* payload size in bytes */
+} SceneCmd;
+
+enum SceneOp {
+  SCENE_OP_SET_COMP_OP        = 1,  /* u32 comp_op, u32 pad */
+  SCENE_OP_SET_FILL_RULE      = 2,  /* u32 fill_rule, u32 pad */
+  SCENE_OP_SET_TRANSFORM      = 3,  /* f64[6] user transform (m00 m01 m10 m11 m20 m21) */
+  SCENE_OP_SET_GLOBAL_ALPHA   = 4,  /* f64 alpha */
+  SCENE_OP_SET_STYLE_SOLID    = 5,  /* u32 rgba32 (straight alpha, 0xAARRGGBB), u32 pad */
+  SCENE_OP_SET_STYLE_GRADIENT = 6,  /* SceneGradient + n_stops x SceneStop */
+  SCENE_OP_SET_STYLE_PATTERN  = 7,  /* ScenePattern */
+  SCENE_OP_SET_PATTERN_QUALITY = 8, /* u32 quality (0 nearest, 1 bilinear), u32 pad */
+  SCENE_OP_FILL_PATH          = 9,  /* u32 n, u32 pad, u8 cmds[n] padded to 8, f64 vtx[2n] */
+  SCENE_OP_FILL_POLYGON       = 10, /* u32 n, u32 pad, f64 vtx[2n] */
+  SCENE_OP_FILL_RECT_D        = 11, /* f64 x, y, w, h */
+  SCENE_OP_FILL_RECT_I        = 12, /* i32 x, y, w, h */
+  SCENE_OP_FILL_ALL           = 13  /* no payload */
+};
+
+typedef struct SceneGradient {
+  uint32_t type;          /* 0 linear, 1 radial, 2 conic (BLGradientType) */
+  uint32_t extend;        /* 0 pad, 1 repeat, 2 reflect (BLExtendMode) */
+  uint32_t n_stops;
+  uint32_t has_transform;
+  double values[6];       /* linear: x0 y0 x1 y1; radial: x0 y0 x1 y1 r0 r1; conic: x0 y0 angle repeat */
+  double transform[6];
+} SceneGradient;
+
+typedef struct SceneStop {
+  double offset;
+  uint32_t rgba32;        /* straight alpha 0xAARRGGBB */
+  uint32_t pad;
+} SceneStop;
+
+typedef struct ScenePattern {
+  uint32_t image_index;
+  uint32_t extend;        /* BLExtendMode, simple modes 0..2 */
+  uint32_t has_transform;
+  uint32_t pad;
+  double transform[6];
+} ScenePattern;
+
+#endif /* B2AMD_SCENE_FORMAT_H */
