@@ -1,0 +1,351 @@
+"""blend2d_amd -- MI355X backend for Blend2D's fill path.
+
+Python mirror of the reference's BLContext / BLImage / BLPath / BLGradient /
+BLPattern fill API (core/context.h, core/path.h, core/gradient.h,
+core/pattern.h), calling the HIP implementation through the C ABI in
+include/blend2d_amd.h (libblend2d_amd.so, built by blend2d_amd.build).
+There is no CPU fallback: without the built library, importing the GPU entry
+points raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import struct
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libblend2d_amd.so")
+
+FORMAT_PRGB32, FORMAT_XRGB32, FORMAT_A8 = 1, 2, 3
+COMP_OP_SRC_OVER, COMP_OP_SRC_COPY, COMP_OP_PLUS, COMP_OP_MULTIPLY, COMP_OP_SCREEN = 0, 1, 12, 15, 16
+FILL_RULE_NON_ZERO, FILL_RULE_EVEN_ODD = 0, 1
+EXTEND_PAD, EXTEND_REPEAT, EXTEND_REFLECT = 0, 1, 2
+GRADIENT_LINEAR, GRADIENT_RADIAL, GRADIENT_CONIC = 0, 1, 2
+PATTERN_QUALITY_NEAREST, PATTERN_QUALITY_BILINEAR = 0, 1
+FLUSH_SYNC = 0x80000000
+
+CMD_MOVE, CMD_ON, CMD_QUAD, CMD_CUBIC, CMD_CLOSE = 0, 1, 2, 4, 5
+
+SYMBOLS = [
+    "bl_amd_version", "bl_amd_device_count",
+    "bl_amd_image_create", "bl_amd_image_destroy", "bl_amd_image_upload", "bl_amd_image_download",
+    "bl_amd_image_fill_raw", "bl_amd_image_get_data",
+    "bl_amd_context_create", "bl_amd_context_destroy", "bl_amd_context_begin", "bl_amd_context_end",
+    "bl_amd_context_flush", "bl_amd_context_set_comp_op", "bl_amd_context_set_fill_rule",
+    "bl_amd_context_set_global_alpha", "bl_amd_context_set_transform", "bl_amd_context_set_pattern_quality",
+    "bl_amd_context_set_fill_style_rgba32", "bl_amd_context_set_fill_style_gradient",
+    "bl_amd_context_set_fill_style_pattern", "bl_amd_context_fill_path_d", "bl_amd_context_fill_polygon_d",
+    "bl_amd_context_fill_rect_d", "bl_amd_context_fill_rect_i", "bl_amd_context_fill_all",
+    "bl_amd_context_last_stats",
+    "bl_amd_batch_create", "bl_amd_batch_destroy", "bl_amd_batch_add", "bl_amd_batch_clear", "bl_amd_batch_flush",
+]
+
+
+class BLError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what} failed with BLResult 0x{code:X}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads libblend2d_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing; build it with `python -m blend2d_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    P, D, U32, SZ = ctypes.c_void_p, ctypes.c_double, ctypes.c_uint32, ctypes.c_size_t
+    I, IP = ctypes.c_int, ctypes.c_ssize_t
+    sig = {
+        "bl_amd_version": ([], U32),
+        "bl_amd_device_count": ([], I),
+        "bl_amd_image_create": ([ctypes.POINTER(P), I, I, U32], U32),
+        "bl_amd_image_destroy": ([P], U32),
+        "bl_amd_image_upload": ([P, P, IP], U32),
+        "bl_amd_image_download": ([P, P, IP], U32),
+        "bl_amd_image_fill_raw": ([P, U32], U32),
+        "bl_amd_image_get_data": ([P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(U32), ctypes.POINTER(P), ctypes.POINTER(IP)], U32),
+        "bl_amd_context_create": ([ctypes.POINTER(P)], U32),
+        "bl_amd_context_destroy": ([P], U32),
+        "bl_amd_context_begin": ([P, P], U32),
+        "bl_amd_context_end": ([P], U32),
+        "bl_amd_context_flush": ([P, U32], U32),
+        "bl_amd_context_set_comp_op": ([P, U32], U32),
+        "bl_amd_context_set_fill_rule": ([P, U32], U32),
+        "bl_amd_context_set_global_alpha": ([P, D], U32),
+        "bl_amd_context_set_transform": ([P, P], U32),
+        "bl_amd_context_set_pattern_quality": ([P, U32], U32),
+        "bl_amd_context_set_fill_style_rgba32": ([P, U32], U32),
+        "bl_amd_context_set_fill_style_gradient": ([P, U32, P, U32, P, P, SZ, P], U32),
+        "bl_amd_context_set_fill_style_pattern": ([P, P, U32, P], U32),
+        "bl_amd_context_fill_path_d": ([P, P, P, P, SZ], U32),
+        "bl_amd_context_fill_polygon_d": ([P, P, SZ], U32),
+        "bl_amd_context_fill_rect_d": ([P, D, D, D, D], U32),
+        "bl_amd_context_fill_rect_i": ([P, I, I, I, I], U32),
+        "bl_amd_context_fill_all": ([P], U32),
+        "bl_amd_context_last_stats": ([P, P, ctypes.POINTER(D)], U32),
+        "bl_amd_batch_create": ([ctypes.POINTER(P)], U32),
+        "bl_amd_batch_destroy": ([P], U32),
+        "bl_amd_batch_add": ([P, P], U32),
+        "bl_amd_batch_clear": ([P], U32),
+        "bl_amd_batch_flush": ([P], U32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _check(code: int, what: str) -> None:
+    if code:
+        raise BLError(code, what)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _mat(m):
+    return np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(6))
+
+
+class BLImage:
+    """Device-resident image (BLImage)."""
+
+    def __init__(self, w: int, h: int, fmt: int = FORMAT_PRGB32):
+        self._h = ctypes.c_void_p()
+        _check(lib().bl_amd_image_create(ctypes.byref(self._h), w, h, fmt), "bl_amd_image_create")
+        self.width, self.height, self.format = w, h, fmt
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def bpp(self) -> int:
+        return 1 if self.format == FORMAT_A8 else 4
+
+    def upload(self, pixels: np.ndarray) -> None:
+        pixels = np.ascontiguousarray(pixels)
+        _check(lib().bl_amd_image_upload(self._h, _dptr(pixels), self.width * self.bpp), "bl_amd_image_upload")
+
+    def download(self) -> np.ndarray:
+        if self.format == FORMAT_A8:
+            out = np.empty((self.height, self.width), dtype=np.uint8)
+        else:
+            out = np.empty((self.height, self.width), dtype=np.uint32)
+        _check(lib().bl_amd_image_download(self._h, _dptr(out), self.width * self.bpp), "bl_amd_image_download")
+        return out
+
+    def fill_raw(self, value: int) -> None:
+        _check(lib().bl_amd_image_fill_raw(self._h, value & 0xFFFFFFFF), "bl_amd_image_fill_raw")
+
+    def close(self) -> None:
+        if self._h:
+            lib().bl_amd_image_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BLPath:
+    """Command/vertex arrays of a path (BLPath, core/path.h)."""
+
+    def __init__(self):
+        self.cmds: list[int] = []
+        self.vtx: list[tuple[float, float]] = []
+
+    def move_to(self, x, y):
+        self.cmds.append(CMD_MOVE); self.vtx.append((x, y)); return self
+
+    def line_to(self, x, y):
+        self.cmds.append(CMD_ON); self.vtx.append((x, y)); return self
+
+    def quad_to(self, x1, y1, x2, y2):
+        self.cmds += [CMD_QUAD, CMD_ON]; self.vtx += [(x1, y1), (x2, y2)]; return self
+
+    def cubic_to(self, x1, y1, x2, y2, x3, y3):
+        self.cmds += [CMD_CUBIC, CMD_CUBIC, CMD_ON]; self.vtx += [(x1, y1), (x2, y2), (x3, y3)]; return self
+
+    def close(self):
+        self.cmds.append(CMD_CLOSE); self.vtx.append((float("nan"), float("nan"))); return self
+
+    def arrays(self):
+        return np.asarray(self.cmds, dtype=np.uint8), np.asarray(self.vtx, dtype=np.float64).reshape(-1, 2)
+
+
+class BLGradient:
+    """Gradient style (BLGradient): type, values, extend mode, stops, transform."""
+
+    def __init__(self, gtype: int, values, extend: int = EXTEND_PAD, stops=(), transform=None):
+        self.type = gtype
+        self.values = list(values) + [0.0] * (6 - len(values))
+        self.extend = extend
+        self.stops = list(stops)
+        self.transform = transform
+
+    def add_stop(self, offset: float, rgba32: int):
+        self.stops.append((offset, rgba32))
+        return self
+
+
+class BLPattern:
+    """Pattern style (BLPattern): image, extend mode, transform."""
+
+    def __init__(self, image: BLImage, extend: int = EXTEND_REPEAT, transform=None):
+        self.image = image
+        self.extend = extend
+        self.transform = transform
+
+
+class BLContext:
+    """Rendering context (BLContext) whose fills run on the MI355X."""
+
+    def __init__(self, image: BLImage | None = None):
+        self._h = ctypes.c_void_p()
+        _check(lib().bl_amd_context_create(ctypes.byref(self._h)), "bl_amd_context_create")
+        self.target = None
+        self._keep = []
+        if image is not None:
+            self.begin(image)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def begin(self, image: BLImage):
+        _check(lib().bl_amd_context_begin(self._h, image.handle), "bl_amd_context_begin")
+        self.target = image
+
+    def end(self):
+        r = lib().bl_amd_context_end(self._h)
+        self.target = None
+        self._keep.clear()
+        _check(r, "bl_amd_context_end")
+
+    def flush(self, flags: int = FLUSH_SYNC):
+        r = lib().bl_amd_context_flush(self._h, flags)
+        self._keep.clear()
+        _check(r, "bl_amd_context_flush")
+
+    def set_comp_op(self, op: int):
+        _check(lib().bl_amd_context_set_comp_op(self._h, op), "set_comp_op")
+
+    def set_fill_rule(self, rule: int):
+        _check(lib().bl_amd_context_set_fill_rule(self._h, rule), "set_fill_rule")
+
+    def set_global_alpha(self, a: float):
+        _check(lib().bl_amd_context_set_global_alpha(self._h, a), "set_global_alpha")
+
+    def set_transform(self, m):
+        m = _mat(m)
+        _check(lib().bl_amd_context_set_transform(self._h, _dptr(m)), "set_transform")
+
+    def set_pattern_quality(self, q: int):
+        _check(lib().bl_amd_context_set_pattern_quality(self._h, q), "set_pattern_quality")
+
+    def set_fill_style(self, style):
+        L = lib()
+        if isinstance(style, int):
+            return _check(L.bl_amd_context_set_fill_style_rgba32(self._h, style & 0xFFFFFFFF), "set_fill_style")
+        if isinstance(style, BLGradient):
+            vals = np.ascontiguousarray(style.values, dtype=np.float64)
+            offs = np.ascontiguousarray([s[0] for s in style.stops], dtype=np.float64)
+            cols = np.ascontiguousarray([s[1] & 0xFFFFFFFF for s in style.stops], dtype=np.uint32)
+            tr = _mat(style.transform) if style.transform is not None else None
+            return _check(L.bl_amd_context_set_fill_style_gradient(
+                self._h, style.type, _dptr(vals), style.extend, _dptr(offs), _dptr(cols), len(style.stops),
+                _dptr(tr) if tr is not None else None), "set_fill_style")
+        if isinstance(style, BLPattern):
+            tr = _mat(style.transform) if style.transform is not None else None
+            self._keep.append(style.image)
+            return _check(L.bl_amd_context_set_fill_style_pattern(
+                self._h, style.image.handle, style.extend, _dptr(tr) if tr is not None else None), "set_fill_style")
+        raise TypeError("unsupported style")
+
+    def fill_path(self, path, origin=None):
+        if isinstance(path, BLPath):
+            cmds, vtx = path.arrays()
+        else:
+            cmds, vtx = path
+            cmds = np.ascontiguousarray(cmds, dtype=np.uint8)
+            vtx = np.ascontiguousarray(vtx, dtype=np.float64).reshape(-1, 2)
+        org = np.ascontiguousarray(origin, dtype=np.float64) if origin is not None else None
+        _check(lib().bl_amd_context_fill_path_d(self._h, _dptr(org) if org is not None else None,
+                                                _dptr(cmds), _dptr(vtx), len(cmds)), "fill_path")
+
+    def fill_polygon(self, vtx):
+        vtx = np.ascontiguousarray(vtx, dtype=np.float64).reshape(-1, 2)
+        _check(lib().bl_amd_context_fill_polygon_d(self._h, _dptr(vtx), len(vtx)), "fill_polygon")
+
+    def fill_rect(self, x, y, w, h):
+        if all(isinstance(v, (int, np.integer)) for v in (x, y, w, h)):
+            _check(lib().bl_amd_context_fill_rect_i(self._h, int(x), int(y), int(w), int(h)), "fill_rect")
+        else:
+            _check(lib().bl_amd_context_fill_rect_d(self._h, float(x), float(y), float(w), float(h)), "fill_rect")
+
+    def fill_all(self):
+        _check(lib().bl_amd_context_fill_all(self._h), "fill_all")
+
+    def last_stats(self):
+        out = (ctypes.c_uint64 * 5)()
+        ms = ctypes.c_double()
+        _check(lib().bl_amd_context_last_stats(self._h, out, ctypes.byref(ms)), "last_stats")
+        keys = ("commands", "source_segments", "edge_segments", "binned_segments", "tiles")
+        d = dict(zip(keys, list(out)))
+        d["render_ms"] = ms.value
+        return d
+
+    def close(self):
+        if self._h:
+            lib().bl_amd_context_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BLBatch:
+    """Renders the pending calls of many contexts with one pass per kernel stage."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        _check(lib().bl_amd_batch_create(ctypes.byref(self._h)), "bl_amd_batch_create")
+
+    def add(self, ctx: BLContext):
+        _check(lib().bl_amd_batch_add(self._h, ctx.handle), "bl_amd_batch_add")
+
+    def clear(self):
+        _check(lib().bl_amd_batch_clear(self._h), "bl_amd_batch_clear")
+
+    def flush(self):
+        _check(lib().bl_amd_batch_flush(self._h), "bl_amd_batch_flush")
+
+    def close(self):
+        if self._h:
+            lib().bl_amd_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+from .replay import replay_into, render_scene, render_scenes_batched  # noqa: E402,F401

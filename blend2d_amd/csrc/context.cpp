@@ -1,0 +1,927 @@
+// Host frontend of the MI355X fill path: the BLContext state machine
+// (raster/rastercontext.cpp), command recording and the flush that runs the
+// three kernel stages.  Exported through the C ABI in include/blend2d_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "../../include/blend2d_amd.h"
+#include "device_types.h"
+#include "kernels.h"
+#include "style.h"
+
+using namespace b2amd;
+
+namespace {
+
+constexpr uint32_t kErrOOM = BL_AMD_ERROR_OUT_OF_MEMORY;
+constexpr uint32_t kErrInvalid = BL_AMD_ERROR_INVALID_VALUE;
+constexpr uint32_t kErrState = BL_AMD_ERROR_INVALID_STATE;
+constexpr uint32_t kErrNotImpl = BL_AMD_ERROR_NOT_IMPLEMENTED;
+constexpr uint32_t kErrDevice = BL_AMD_ERROR_DEVICE;
+
+#define HIP_OK(expr) do { if ((expr) != hipSuccess) return kErrDevice; } while (0)
+
+// Flattening tolerance: approximation_options().flatten_tolerance (0.2,
+// core/path_p.h:89-98) in 24.8 units, squared (rastercontext.cpp:208-211).
+const double kToleranceSq = (0.2 * 256.0) * (0.2 * 256.0);
+
+// Band height the reference picks for a canvas width with thread_count == 0
+// (rastercontext.cpp:4154-4193); the conic fetcher's row origin depends on it.
+uint32_t ref_band_height(uint32_t width) {
+  uint32_t bh = 64;
+  size_t pixels = (size_t)width * bh;
+  do {
+    if (pixels * 4 <= 1024 * 256) break;
+    bh >>= 1;
+    pixels >>= 1;
+  } while (bh > 8);
+  return bh;
+}
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  ~DevBuf() { if (ptr) (void)hipFree(ptr); }
+  bool reserve(size_t bytes) {
+    if (bytes <= cap) return true;
+    if (ptr) { (void)hipFree(ptr); ptr = nullptr; cap = 0; }
+    size_t n = std::max<size_t>(bytes + bytes / 4, 4096);
+    if (hipMalloc(&ptr, n) != hipSuccess) { ptr = nullptr; return false; }
+    cap = n;
+    return true;
+  }
+  template<typename T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Host-recorded commands of one flush, possibly merged from many contexts.
+struct FrameHost {
+  std::vector<DevCanvas> canvases;
+  std::vector<DevCmd> cmds;
+  std::vector<CmdXform> xforms;
+  std::vector<SrcSeg> src;
+  std::vector<double> vtx;
+  std::vector<DevStyle> styles;
+  std::vector<uint32_t> luts;
+  std::vector<DevImage> images;
+  void clear() {
+    canvases.clear(); cmds.clear(); xforms.clear(); src.clear(); vtx.clear();
+    styles.clear(); luts.clear(); images.clear();
+  }
+};
+
+struct Stats {
+  uint64_t cmds = 0, src = 0, segs = 0, binned = 0, tiles = 0;
+  double render_ms = 0.0;
+};
+
+// Device-side state of the pipeline: input copies and scratch.
+struct Runner {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf in;
+  DevBuf src_count, src_off, bounds, segs, seg_cmd, mat_cnt, mat_off, mat_pres, list_off, list, binned, cursor, scan_tmp, readback;
+  uint32_t* host_readback = nullptr;
+
+  bool init() {
+    if (stream) return true;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return false;
+    if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess) return false;
+    if (hipHostMalloc((void**)&host_readback, 64, hipHostMallocDefault) != hipSuccess) return false;
+    return true;
+  }
+  ~Runner() {
+    if (host_readback) (void)hipHostFree(host_readback);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  uint32_t read_u32(const uint32_t* dptr) {
+    if (hipMemcpyAsync(host_readback, dptr, 4, hipMemcpyDeviceToHost, stream) != hipSuccess) return 0xFFFFFFFFu;
+    if (hipStreamSynchronize(stream) != hipSuccess) return 0xFFFFFFFFu;
+    return host_readback[0];
+  }
+
+  uint32_t run(FrameHost& fh, Stats& stats);
+};
+
+template<typename T>
+static size_t append_bytes(std::vector<uint8_t>& blob, const std::vector<T>& v) {
+  size_t off = (blob.size() + 255) & ~(size_t)255;
+  blob.resize(off + v.size() * sizeof(T));
+  if (!v.empty()) memcpy(blob.data() + off, v.data(), v.size() * sizeof(T));
+  return off;
+}
+
+uint32_t Runner::run(FrameHost& fh, Stats& stats) {
+  stats = Stats{};
+  if (fh.cmds.empty()) return BL_AMD_SUCCESS;
+  if (!init()) return kErrDevice;
+
+  // Canvas layout: tiles, bands, (band, command) matrix.
+  uint64_t n_mat = 0;
+  uint32_t n_tiles = 0, band_base = 0;
+  for (DevCanvas& c : fh.canvases) {
+    c.tiles_x = (c.w + kTileW - 1) / kTileW;
+    c.bands = (c.h + kTileH - 1) / kTileH;
+    c.ref_band_h = ref_band_height(c.w);
+    c.band_base = band_base;
+    c.tile_base = n_tiles;
+    c.mat_base = n_mat;
+    band_base += c.bands;
+    n_tiles += c.tiles_x * c.bands;
+    n_mat += (uint64_t)c.bands * c.n_cmds;
+  }
+  if (n_mat >= 0xFFFFFFF0ull) return kErrOOM;
+
+  // Inputs: one packed upload.
+  std::vector<uint8_t> blob;
+  size_t o_canvas = append_bytes(blob, fh.canvases);
+  size_t o_cmds = append_bytes(blob, fh.cmds);
+  size_t o_xf = append_bytes(blob, fh.xforms);
+  size_t o_src = append_bytes(blob, fh.src);
+  size_t o_vtx = append_bytes(blob, fh.vtx);
+  size_t o_styles = append_bytes(blob, fh.styles);
+  size_t o_luts = append_bytes(blob, fh.luts);
+  size_t o_images = append_bytes(blob, fh.images);
+  if (!in.reserve(blob.size() + 256)) return kErrOOM;
+  HIP_OK(hipMemcpyAsync(in.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, stream));
+
+  const uint8_t* base = in.as<uint8_t>();
+  FrameDev f;
+  f.canvases = (const DevCanvas*)(base + o_canvas); f.n_canvases = (uint32_t)fh.canvases.size();
+  f.cmds = (const DevCmd*)(base + o_cmds); f.n_cmds = (uint32_t)fh.cmds.size();
+  f.xforms = (const CmdXform*)(base + o_xf);
+  f.src = (const SrcSeg*)(base + o_src); f.n_src = (uint32_t)fh.src.size();
+  f.vtx = (const double*)(base + o_vtx);
+  f.styles = (const DevStyle*)(base + o_styles);
+  f.luts = (const uint32_t*)(base + o_luts);
+  f.images = (const DevImage*)(base + o_images);
+  f.n_tiles = n_tiles;
+  f.n_mat = n_mat;
+  f.tol_sq = kToleranceSq;
+
+  size_t n_src = fh.src.size();
+  size_t scan_words = scan_tmp_words(std::max<size_t>(n_src, (size_t)n_mat) + 1);
+  if (!src_count.reserve((n_src + 1) * 4) || !src_off.reserve((n_src + 1) * 4) ||
+      !bounds.reserve(fh.cmds.size() * sizeof(CmdBounds)) ||
+      !mat_cnt.reserve((n_mat + 1) * 4) || !mat_off.reserve((n_mat + 1) * 4) ||
+      !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
+      !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4))
+    return kErrOOM;
+
+  FrameScratch s;
+  memset(&s, 0, sizeof(s));
+  s.src_count = src_count.as<uint32_t>();
+  s.src_off = src_off.as<uint32_t>();
+  s.bounds = bounds.as<CmdBounds>();
+  s.mat_cnt = mat_cnt.as<uint32_t>();
+  s.mat_off = mat_off.as<uint32_t>();
+  s.mat_pres = mat_pres.as<uint32_t>();
+  s.list_off = list_off.as<uint32_t>();
+  s.scan_tmp = scan_tmp.as<uint32_t>();
+
+  // Stage 1: edges (count, scan, emit).
+  launch_init_bounds(s.bounds, f.n_cmds, stream);
+  launch_edges(f, s, true, stream);
+  scan_exclusive(s.src_count, s.src_off, n_src, s.scan_tmp, stream);
+  uint32_t n_segs = read_u32(s.src_off + n_src);
+  if (n_segs == 0xFFFFFFFFu) return kErrDevice;
+  if (!segs.reserve((size_t)std::max<uint32_t>(n_segs, 1) * sizeof(DevSeg)) || !seg_cmd.reserve((size_t)std::max<uint32_t>(n_segs, 1) * 4))
+    return kErrOOM;
+  s.segs = segs.as<DevSeg>();
+  s.seg_cmd = seg_cmd.as<uint32_t>();
+  launch_edges(f, s, false, stream);
+
+  // Stage 2: binning.
+  launch_fill_u32(s.mat_cnt, 0u, n_mat + 1, stream);
+  launch_bin_count(f, s, n_segs, stream);
+  launch_presence(f, s, stream);
+  scan_exclusive(s.mat_cnt, s.mat_off, n_mat, s.scan_tmp, stream);
+  scan_exclusive(s.mat_pres, s.list_off, n_mat, s.scan_tmp, stream);
+  HIP_OK(hipMemcpyAsync(host_readback, s.mat_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync(host_readback + 1, s.list_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipStreamSynchronize(stream));
+  uint32_t n_binned = host_readback[0], n_list = host_readback[1];
+  if (!binned.reserve((size_t)std::max<uint32_t>(n_binned, 1) * sizeof(DevSeg)) || !list.reserve((size_t)std::max<uint32_t>(n_list, 1) * 4))
+    return kErrOOM;
+  s.binned = binned.as<DevSeg>();
+  s.list = list.as<uint32_t>();
+  HIP_OK(hipMemcpyAsync(cursor.ptr, s.mat_off, (n_mat + 1) * 4, hipMemcpyDeviceToDevice, stream));
+  launch_bin_scatter(f, s, n_segs, cursor.as<uint32_t>(), stream);
+  launch_list_build(f, s, stream);
+
+  // Stage 3: render.
+  HIP_OK(hipEventRecord(ev0, stream));
+  launch_render(f, s, stream);
+  HIP_OK(hipEventRecord(ev1, stream));
+  HIP_OK(hipStreamSynchronize(stream));
+  if (hipGetLastError() != hipSuccess) return kErrDevice;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ev0, ev1);
+
+  stats.cmds = fh.cmds.size();
+  stats.src = n_src;
+  stats.segs = n_segs;
+  stats.binned = n_binned;
+  stats.tiles = n_tiles;
+  stats.render_ms = ms;
+  return BL_AMD_SUCCESS;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Public objects
+// ---------------------------------------------------------------------------
+
+struct BLAmdImage {
+  void* dptr = nullptr;
+  int64_t stride = 0;
+  int w = 0, h = 0;
+  uint32_t format = 0;
+  ~BLAmdImage() { if (dptr) (void)hipFree(dptr); }
+};
+
+namespace {
+
+enum StyleType { kStyleSolid = 0, kStyleGradient = 1, kStylePattern = 2 };
+
+// Fill style captured at set time with the transform in effect then
+// (rastercontext.cpp:653-695 initializes the fetch data in set_style).
+struct StyleState {
+  uint32_t type = kStyleSolid;
+  uint32_t fmt = kFxFRGB32;     // FormatExt seen by comp-op simplification
+  uint32_t status = 0;          // 0 renders, 1 NOP, otherwise an error code
+  DevStyle dev{};
+  std::vector<uint32_t> lut;
+  const BLAmdImage* image = nullptr;
+  uint64_t version = 0;
+};
+
+}  // namespace
+
+struct BLAmdContext {
+  BLAmdImage* target = nullptr;
+  uint32_t comp_op = kCompSrcOver;
+  uint32_t fill_rule = 0;
+  uint32_t pattern_quality = 1;
+  Mat user{1, 0, 0, 1, 0, 0};
+  double global_alpha = 1.0;
+  StyleState style;
+  uint64_t style_counter = 0;
+  // style version -> index in frame.styles (single cached entry)
+  uint64_t frame_style_version = ~0ull;
+  uint32_t frame_style_index = 0;
+
+  FrameHost frame;          // commands since the last flush (one canvas)
+  std::vector<const BLAmdImage*> frame_images;
+  Runner runner;
+  Stats stats;
+
+  void reset_state() {
+    comp_op = kCompSrcOver;
+    fill_rule = 0;
+    pattern_quality = 1;
+    user = Mat{1, 0, 0, 1, 0, 0};
+    global_alpha = 1.0;
+    style = StyleState{};
+    style.dev.kind = kFetchSolid;
+    style.dev.solid = premultiply(0xFF000000u);
+    style.version = ++style_counter;
+    frame_style_version = ~0ull;
+  }
+};
+
+struct BLAmdBatch {
+  std::vector<BLAmdContext*> contexts;
+  Runner runner;
+  Stats stats;
+};
+
+namespace {
+
+uint32_t alpha_i(const BLAmdContext* c) { return (uint32_t)round_to_int(c->global_alpha * 255.0 * 1.0); }
+
+uint32_t image_index(BLAmdContext* c, const BLAmdImage* img) {
+  for (size_t i = 0; i < c->frame_images.size(); i++) if (c->frame_images[i] == img) return (uint32_t)i;
+  c->frame_images.push_back(img);
+  DevImage d{};
+  d.ptr = (uint64_t)(uintptr_t)img->dptr;
+  d.stride = img->stride;
+  d.w = (uint32_t)img->w;
+  d.h = (uint32_t)img->h;
+  d.format = img->format;
+  c->frame.images.push_back(d);
+  return (uint32_t)(c->frame_images.size() - 1);
+}
+
+// Resolves the current style and comp op into a command template
+// (rastercontext.cpp:1231-1342 + compopsimplifyimpl_p.h).  Returns 0 when the
+// call renders, 1 for a NOP, otherwise an error.
+uint32_t resolve(BLAmdContext* c, DevCmd& cmd) {
+  memset(&cmd, 0, sizeof(cmd));
+  if (alpha_i(c) == 0) return 1;
+  StyleState& st = c->style;
+  if (st.status) return st.status;
+
+  const uint32_t dst_fmt = c->target->format;
+  Simplified s = simplify(c->comp_op, dst_fmt, st.fmt);
+  if (s.op == kCompNop) return 1;
+  uint32_t op;
+  switch (s.op) {
+    case kCompSrcCopy: op = kOpSrcCopy; break;
+    case kCompSrcOver: op = kOpSrcOver; break;
+    case kCompPlus: op = kOpPlus; break;
+    case kCompMultiply: op = kOpMultiply; break;
+    case kCompScreen: op = kOpScreen; break;
+    default: return kErrNotImpl;
+  }
+  if ((op == kOpMultiply || op == kOpScreen) && s.dst != kFxPRGB32) return kErrNotImpl;
+  if (s.src == kFxA8 && s.dst != kFxA8 && op != kOpSrcCopy && op != kOpSrcOver && op != kOpPlus) return kErrNotImpl;
+
+  cmd.op = op;
+  cmd.alpha = alpha_i(c);
+  cmd.src_fmt = s.src;
+  cmd.canvas = 0;
+
+  if (s.solid != kSolidNone) {
+    DevStyle d{};
+    d.kind = kFetchSolid;
+    d.solid = s.solid == kSolidTransparent ? 0u : s.solid == kSolidOpaqueBlack ? 0xFF000000u : 0xFFFFFFFFu;
+    cmd.style = (uint32_t)c->frame.styles.size();
+    c->frame.styles.push_back(d);
+    return 0;
+  }
+
+  if (c->frame_style_version != st.version) {
+    DevStyle d = st.dev;
+    if (st.type == kStyleGradient && d.kind != kFetchSolid) {
+      d.lut = (uint32_t)c->frame.luts.size();
+      c->frame.luts.insert(c->frame.luts.end(), st.lut.begin(), st.lut.end());
+    }
+    if (st.type == kStylePattern) d.img = image_index(c, st.image);
+    c->frame_style_index = (uint32_t)c->frame.styles.size();
+    c->frame.styles.push_back(d);
+    c->frame_style_version = st.version;
+  }
+  cmd.style = c->frame_style_index;
+  return 0;
+}
+
+CmdXform fixed_xform(const BLAmdContext* c) {
+  CmdXform x;
+  x.m00 = c->user.m00 * 256.0; x.m01 = c->user.m01 * 256.0;
+  x.m10 = c->user.m10 * 256.0; x.m11 = c->user.m11 * 256.0;
+  x.m20 = c->user.m20 * 256.0; x.m21 = c->user.m21 * 256.0;
+  x.scale_only = transform_type(c->user) <= kTTScale ? 1u : 0u;
+  x.pad = 0;
+  return x;
+}
+
+uint32_t push_cmd(BLAmdContext* c, DevCmd& cmd, const CmdXform* xf) {
+  c->frame.cmds.push_back(cmd);
+  if (xf) c->frame.xforms.push_back(*xf);
+  else c->frame.xforms.push_back(CmdXform{});
+  return (uint32_t)(c->frame.cmds.size() - 1);
+}
+
+enum { kCmdMove = 0, kCmdOn = 1, kCmdQuad = 2, kCmdCubic = 4 };
+
+// Splits a path into source segments the way EdgeSourcePath iterates it
+// (edgebuilder_p.h:160-285, 1032-1070): figures start at move-to, quads take
+// two vertices, cubics three, anything else closes the figure.
+void record_path_segments(BLAmdContext* c, uint32_t cmd_i, const uint8_t* cmds, const double* vtx, size_t n) {
+  const uint32_t vbase = (uint32_t)(c->frame.vtx.size() / 2);
+  c->frame.vtx.insert(c->frame.vtx.end(), vtx, vtx + 2 * n);
+  size_t i = 0;
+  for (;;) {
+    while (i < n && cmds[i] != kCmdMove) i++;
+    if (i >= n) break;
+    uint32_t start = vbase + (uint32_t)i, prev = start;
+    i++;
+    for (;;) {
+      if (i < n && cmds[i] == kCmdOn) {
+        c->frame.src.push_back(SrcSeg{cmd_i, kSrcLine, prev, vbase + (uint32_t)i});
+        prev = vbase + (uint32_t)i;
+        i++;
+      }
+      else if (i + 2 <= n && cmds[i] == kCmdQuad) {
+        c->frame.src.push_back(SrcSeg{cmd_i, kSrcQuad, prev, vbase + (uint32_t)i});
+        prev = vbase + (uint32_t)i + 1;
+        i += 2;
+      }
+      else if (i + 3 <= n && cmds[i] == kCmdCubic) {
+        c->frame.src.push_back(SrcSeg{cmd_i, kSrcCubic, prev, vbase + (uint32_t)i});
+        prev = vbase + (uint32_t)i + 2;
+        i += 3;
+      }
+      else {
+        c->frame.src.push_back(SrcSeg{cmd_i, kSrcLine, prev, start});
+        break;
+      }
+    }
+  }
+}
+
+void record_poly_segments(BLAmdContext* c, uint32_t cmd_i, const double* vtx, size_t n) {
+  const uint32_t vbase = (uint32_t)(c->frame.vtx.size() / 2);
+  c->frame.vtx.insert(c->frame.vtx.end(), vtx, vtx + 2 * n);
+  for (size_t i = 1; i < n; i++) c->frame.src.push_back(SrcSeg{cmd_i, kSrcLine, vbase + (uint32_t)i - 1, vbase + (uint32_t)i});
+  c->frame.src.push_back(SrcSeg{cmd_i, kSrcLine, vbase + (uint32_t)n - 1, vbase});
+}
+
+uint32_t fill_poly_rule(BLAmdContext* c, const double* vtx, size_t n, uint32_t rule) {
+  if (n < 3) return BL_AMD_SUCCESS;
+  DevCmd cmd;
+  uint32_t r = resolve(c, cmd);
+  if (r == 1) return BL_AMD_SUCCESS;
+  if (r) return r;
+  cmd.kind = kCmdAnalytic;
+  cmd.rule_mask = rule == 0 ? 0xFFFFFFFFu : 0x1FFu;
+  CmdXform xf = fixed_xform(c);
+  uint32_t ci = push_cmd(c, cmd, &xf);
+  record_poly_segments(c, ci, vtx, n);
+  return BL_AMD_SUCCESS;
+}
+
+// FillBoxA (fillgeneric_p.h:22-65)
+uint32_t fill_box_a(BLAmdContext* c, int x0, int y0, int x1, int y1) {
+  DevCmd cmd;
+  uint32_t r = resolve(c, cmd);
+  if (r == 1) return BL_AMD_SUCCESS;
+  if (r) return r;
+  cmd.kind = kCmdBoxA;
+  cmd.box[0] = x0; cmd.box[1] = y0; cmd.box[2] = x1; cmd.box[3] = y1;
+  cmd.fetch_row0 = (uint32_t)y0;
+  push_cmd(c, cmd, nullptr);
+  return BL_AMD_SUCCESS;
+}
+
+// FillData::init_box_u_8bpc_24x8 (pipedefs_p.h:644-815) as per-pixel masks.
+uint32_t fill_box_u(BLAmdContext* c, int x0, int y0, int x1, int y1) {
+  DevCmd cmd;
+  uint32_t r = resolve(c, cmd);
+  if (r == 1) return BL_AMD_SUCCESS;
+  if (r) return r;
+  const uint32_t alpha = cmd.alpha;
+  uint32_t ax0 = (uint32_t)x0 >> 8, ay0 = (uint32_t)y0 >> 8;
+  uint32_t ax1 = (uint32_t)(x1 + 0xFF) >> 8, ay1 = (uint32_t)(y1 + 0xFF) >> 8;
+  uint32_t fx0 = (uint32_t)x0 & 0xFFu, fy0 = (uint32_t)y0 & 0xFFu;
+  uint32_t fx1 = ((uint32_t)(x1 - 1) & 0xFFu) + 1u, fy1 = ((uint32_t)(y1 - 1) & 0xFFu) + 1u;
+  uint32_t w = ax1 - ax0, h = ay1 - ay0;
+  fy0 = (h == 1 ? fy1 : 256u) - fy0;
+  uint32_t fy0_a = fy0 * alpha, fy1_a = fy1 * alpha;
+  uint32_t m[3][3];
+  if (w == 1) {
+    uint32_t fx = fx1 - fx0;
+    m[0][0] = m[0][1] = m[0][2] = (fx * fy0_a) >> 16;
+    m[1][0] = m[1][1] = m[1][2] = (fx * alpha) >> 8;
+    m[2][0] = m[2][1] = m[2][2] = (fx * fy1_a) >> 16;
+  }
+  else {
+    uint32_t lx = 256 - fx0;
+    m[0][0] = (lx * fy0_a) >> 16; m[0][1] = fy0_a >> 8; m[0][2] = (fx1 * fy0_a) >> 16;
+    m[1][0] = (lx * alpha) >> 8;  m[1][1] = alpha;      m[1][2] = (fx1 * alpha) >> 8;
+    m[2][0] = (lx * fy1_a) >> 16; m[2][1] = fy1_a >> 8; m[2][2] = (fx1 * fy1_a) >> 16;
+  }
+  uint32_t by0 = ay0, by1 = ay0 + h;
+  uint32_t top_inner = w == 1 ? m[0][0] : m[0][1];
+  uint32_t mid_inner = w == 1 ? m[1][0] : m[1][1];
+  uint32_t bot_inner = w == 1 ? m[2][0] : m[2][1];
+  if (h == 1) {
+    if (top_inner == 0) return BL_AMD_SUCCESS;
+  }
+  else {
+    by0 += (uint32_t)(top_inner == 0);
+    by1 -= (uint32_t)(bot_inner == 0);
+    if (!(by0 < by1)) return BL_AMD_SUCCESS;
+    if (w == 1 && mid_inner == 0) return BL_AMD_SUCCESS;
+  }
+  cmd.kind = kCmdBoxU;
+  cmd.box[0] = (int32_t)ax0; cmd.box[1] = (int32_t)by0; cmd.box[2] = (int32_t)ax1; cmd.box[3] = (int32_t)by1;
+  cmd.box_row_first = (int32_t)ay0;
+  cmd.box_row_last = (int32_t)(ay0 + h - 1);
+  for (int k = 0; k < 3; k++) cmd.box_mask[k] = m[k][0] | (m[k][1] << 8) | (m[k][2] << 16);
+  cmd.fetch_row0 = by0;
+  push_cmd(c, cmd, nullptr);
+  return BL_AMD_SUCCESS;
+}
+
+// fill_unclipped_box_d (rastercontext.cpp:2871-2891)
+uint32_t fill_box_d(BLAmdContext* c, double bx0, double by0, double bx1, double by1) {
+  CmdXform m = fixed_xform(c);
+  int tt = std::max(transform_type(c->user), (int)kTTScale);
+  double cx1 = (double)c->target->w * 256.0, cy1 = (double)c->target->h * 256.0;
+  if (tt <= kTTSwap) {
+    double x0 = bx0 * m.m00 + by0 * m.m10 + m.m20;
+    double y0 = bx0 * m.m01 + by0 * m.m11 + m.m21;
+    double x1 = bx1 * m.m00 + by1 * m.m10 + m.m20;
+    double y1 = bx1 * m.m01 + by1 * m.m11 + m.m21;
+    double bxa = x1 < x0 ? x1 : x0, bya = y1 < y0 ? y1 : y0;
+    double bxb = x0 < x1 ? x1 : x0, byb = y0 < y1 ? y1 : y0;
+    double dx0 = 0.0 < bxa ? bxa : 0.0, dy0 = 0.0 < bya ? bya : 0.0;
+    double dx1 = cx1 < bxb ? cx1 : bxb, dy1 = cy1 < byb ? cy1 : byb;
+    if (!((dx0 < dx1) & (dy0 < dy1))) return BL_AMD_SUCCESS;
+    int ix0 = (int)dx0, iy0 = (int)dy0, ix1 = (int)dx1, iy1 = (int)dy1;
+    if (ix0 >= ix1 || iy0 >= iy1) return BL_AMD_SUCCESS;
+    if (((ix0 | iy0 | ix1 | iy1) & 0xFF) == 0) return fill_box_a(c, ix0 >> 8, iy0 >> 8, ix1 >> 8, iy1 >> 8);
+    return fill_box_u(c, ix0, iy0, ix1, iy1);
+  }
+  double poly[8] = { bx0, by0, bx1, by0, bx1, by1, bx0, by1 };
+  return fill_poly_rule(c, poly, 4, 1);
+}
+
+uint32_t check_target(const BLAmdContext* c) { return c && c->target ? BL_AMD_SUCCESS : kErrState; }
+
+// Prepares a context's frame for merging: canvas record.
+void finalize_frame(BLAmdContext* c) {
+  c->frame.canvases.clear();
+  DevCanvas cv{};
+  cv.ptr = (uint64_t)(uintptr_t)c->target->dptr;
+  cv.stride = c->target->stride;
+  cv.w = (uint32_t)c->target->w;
+  cv.h = (uint32_t)c->target->h;
+  cv.format = c->target->format;
+  cv.cmd_base = 0;
+  cv.n_cmds = (uint32_t)c->frame.cmds.size();
+  c->frame.canvases.push_back(cv);
+}
+
+void reset_frame(BLAmdContext* c) {
+  c->frame.clear();
+  c->frame_images.clear();
+  c->frame_style_version = ~0ull;
+}
+
+uint32_t flush_one(BLAmdContext* c) {
+  finalize_frame(c);
+  uint32_t r = c->runner.run(c->frame, c->stats);
+  reset_frame(c);
+  return r;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+uint32_t bl_amd_version(void) { return (0u << 16) | (1u << 8) | 0u; }
+
+int bl_amd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+BLAmdResult bl_amd_image_create(BLAmdImage** out, int w, int h, uint32_t format) {
+  if (!out) return kErrInvalid;
+  *out = nullptr;
+  if (w <= 0 || h <= 0 || w > 65535 || h > 65535) return kErrInvalid;
+  if (format != BL_AMD_FORMAT_PRGB32 && format != BL_AMD_FORMAT_XRGB32 && format != BL_AMD_FORMAT_A8) return kErrInvalid;
+  std::unique_ptr<BLAmdImage> img(new (std::nothrow) BLAmdImage());
+  if (!img) return kErrOOM;
+  const int bpp = format == BL_AMD_FORMAT_A8 ? 1 : 4;
+  img->stride = ((int64_t)w * bpp + 15) & ~(int64_t)15;
+  img->w = w; img->h = h; img->format = format;
+  if (hipMalloc(&img->dptr, (size_t)img->stride * (size_t)h) != hipSuccess) { img->dptr = nullptr; return kErrOOM; }
+  if (hipMemset(img->dptr, 0, (size_t)img->stride * (size_t)h) != hipSuccess) return kErrDevice;
+  *out = img.release();
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_image_destroy(BLAmdImage* img) {
+  delete img;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_image_upload(BLAmdImage* img, const void* pixels, intptr_t stride) {
+  if (!img || !pixels) return kErrInvalid;
+  const size_t row = (size_t)img->w * (img->format == BL_AMD_FORMAT_A8 ? 1 : 4);
+  if (hipMemcpy2D(img->dptr, (size_t)img->stride, pixels, (size_t)stride, row, (size_t)img->h, hipMemcpyHostToDevice) != hipSuccess)
+    return kErrDevice;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_image_download(const BLAmdImage* img, void* pixels, intptr_t stride) {
+  if (!img || !pixels) return kErrInvalid;
+  const size_t row = (size_t)img->w * (img->format == BL_AMD_FORMAT_A8 ? 1 : 4);
+  if (hipMemcpy2D(pixels, (size_t)stride, img->dptr, (size_t)img->stride, row, (size_t)img->h, hipMemcpyDeviceToHost) != hipSuccess)
+    return kErrDevice;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_image_fill_raw(BLAmdImage* img, uint32_t value) {
+  if (!img) return kErrInvalid;
+  if (img->format == BL_AMD_FORMAT_A8) {
+    if (hipMemset(img->dptr, (int)(value & 0xFFu), (size_t)img->stride * img->h) != hipSuccess) return kErrDevice;
+    return BL_AMD_SUCCESS;
+  }
+  std::vector<uint32_t> row((size_t)img->stride / 4, value);
+  for (int y = 0; y < img->h; y++)
+    if (hipMemcpy((uint8_t*)img->dptr + (size_t)y * img->stride, row.data(), (size_t)img->stride, hipMemcpyHostToDevice) != hipSuccess)
+      return kErrDevice;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_image_get_data(const BLAmdImage* img, int* w, int* h, uint32_t* format, void** device_pixels, intptr_t* stride) {
+  if (!img) return kErrInvalid;
+  if (w) *w = img->w;
+  if (h) *h = img->h;
+  if (format) *format = img->format;
+  if (device_pixels) *device_pixels = img->dptr;
+  if (stride) *stride = (intptr_t)img->stride;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_create(BLAmdContext** out) {
+  if (!out) return kErrInvalid;
+  *out = new (std::nothrow) BLAmdContext();
+  if (!*out) return kErrOOM;
+  (*out)->reset_state();
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_destroy(BLAmdContext* ctx) {
+  delete ctx;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_begin(BLAmdContext* ctx, BLAmdImage* target) {
+  if (!ctx || !target) return kErrInvalid;
+  if (ctx->target) {
+    uint32_t r = bl_amd_context_end(ctx);
+    if (r) return r;
+  }
+  ctx->target = target;
+  ctx->reset_state();
+  reset_frame(ctx);
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_flush(BLAmdContext* ctx, uint32_t flags) {
+  (void)flags;
+  if (uint32_t r = check_target(ctx)) return r;
+  return flush_one(ctx);
+}
+
+BLAmdResult bl_amd_context_end(BLAmdContext* ctx) {
+  if (uint32_t r = check_target(ctx)) return r;
+  uint32_t r = flush_one(ctx);
+  ctx->target = nullptr;
+  return r;
+}
+
+BLAmdResult bl_amd_context_set_comp_op(BLAmdContext* ctx, uint32_t comp_op) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (comp_op > 28) return kErrInvalid;
+  ctx->comp_op = comp_op;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_set_fill_rule(BLAmdContext* ctx, uint32_t fill_rule) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (fill_rule > 1) return kErrInvalid;
+  ctx->fill_rule = fill_rule;
+  return BL_AMD_SUCCESS;
+}
+
+// set_global_alpha_impl (rastercontext.cpp:2000-2029)
+BLAmdResult bl_amd_context_set_global_alpha(BLAmdContext* ctx, double alpha) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (isnan(alpha)) return kErrInvalid;
+  ctx->global_alpha = alpha < 0.0 ? 0.0 : (alpha > 1.0 ? 1.0 : alpha);
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_set_transform(BLAmdContext* ctx, const double m[6]) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (!m) return kErrInvalid;
+  ctx->user = Mat{m[0], m[1], m[2], m[3], m[4], m[5]};
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_set_pattern_quality(BLAmdContext* ctx, uint32_t quality) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (quality > 1) return kErrInvalid;
+  ctx->pattern_quality = quality;
+  return BL_AMD_SUCCESS;
+}
+
+// resolve_explicit_solid_op (rastercontext.cpp:1259-1275, :90-93)
+BLAmdResult bl_amd_context_set_fill_style_rgba32(BLAmdContext* ctx, uint32_t rgba32) {
+  if (uint32_t r = check_target(ctx)) return r;
+  StyleState& st = ctx->style;
+  st = StyleState{};
+  st.type = kStyleSolid;
+  st.fmt = rgba32 == 0 ? kFxZERO32 : (rgba32 >= 0xFF000000u ? kFxFRGB32 : kFxPRGB32);
+  st.dev.kind = kFetchSolid;
+  st.dev.solid = premultiply(rgba32);
+  st.version = ++ctx->style_counter;
+  return BL_AMD_SUCCESS;
+}
+
+// init_non_solid_fetch_data, gradient branch (rastercontext.cpp:416-481)
+BLAmdResult bl_amd_context_set_fill_style_gradient(BLAmdContext* ctx, uint32_t type, const double* values, uint32_t extend,
+                                                   const double* stop_offsets, const uint32_t* stop_rgba32, size_t n_stops,
+                                                   const double* transform) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (type > 2 || extend > 2 || !values || (n_stops && (!stop_offsets || !stop_rgba32))) return kErrInvalid;
+  StyleState& st = ctx->style;
+  st = StyleState{};
+  st.type = kStyleGradient;
+  st.version = ++ctx->style_counter;
+
+  std::vector<Stop> stops;
+  for (size_t i = 0; i < n_stops; i++) add_stop(stops, stop_offsets[i], stop_rgba32[i]);
+  if (stops.empty()) { st.status = 1; return BL_AMD_SUCCESS; }
+
+  bool solid;
+  uint32_t fmt;
+  uint32_t lut_size = gradient_info(stops, solid, fmt);
+  st.fmt = fmt;
+  if (solid) {
+    uint64_t last = stops.back().rgba64;
+    uint32_t c32 = (uint32_t)(((last >> 56) & 0xFFu) << 24 | ((last >> 40) & 0xFFu) << 16 | ((last >> 24) & 0xFFu) << 8 | ((last >> 8) & 0xFFu));
+    st.dev.kind = kFetchSolid;
+    st.dev.solid = premultiply(c32);
+    return BL_AMD_SUCCESS;
+  }
+
+  Mat t = ctx->user;
+  if (transform) {
+    Mat gm{transform[0], transform[1], transform[2], transform[3], transform[4], transform[5]};
+    if (transform_type(gm) != kTTIdentity) t = multiply(gm, ctx->user);
+  }
+  st.lut.resize(lut_size);
+  interpolate_lut(st.lut.data(), lut_size, stops);
+  bool ok = type == 0 ? init_linear(st.dev, lut_size, values, extend, t)
+          : type == 1 ? init_radial(st.dev, lut_size, values, extend, t)
+                      : init_conic(st.dev, lut_size, values, t);
+  if (!ok) st.status = 1;
+  return BL_AMD_SUCCESS;
+}
+
+// init_non_solid_fetch_data, pattern branch (rastercontext.cpp:377-414)
+BLAmdResult bl_amd_context_set_fill_style_pattern(BLAmdContext* ctx, const BLAmdImage* image, uint32_t extend, const double* transform) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (!image || extend > 2) return kErrInvalid;
+  StyleState& st = ctx->style;
+  st = StyleState{};
+  st.type = kStylePattern;
+  st.version = ++ctx->style_counter;
+  st.fmt = image->format;
+  st.image = image;
+  Mat t = ctx->user;
+  if (transform) {
+    Mat pm{transform[0], transform[1], transform[2], transform[3], transform[4], transform[5]};
+    if (transform_type(pm) != kTTIdentity) t = multiply(pm, ctx->user);
+  }
+  int r = init_pattern(st.dev, (uint32_t)image->w, (uint32_t)image->h, extend, ctx->pattern_quality, t);
+  if (r == 0) st.status = 1;
+  else if (r < 0) st.status = kErrNotImpl;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_fill_path_d(BLAmdContext* ctx, const double* origin, const uint8_t* cmds, const double* vertices, size_t n) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (n == 0) return BL_AMD_SUCCESS;
+  if (!cmds || !vertices) return kErrInvalid;
+  DevCmd cmd;
+  uint32_t r = resolve(ctx, cmd);
+  if (r == 1) return BL_AMD_SUCCESS;
+  if (r) return r;
+  cmd.kind = kCmdAnalytic;
+  cmd.rule_mask = ctx->fill_rule == 0 ? 0xFFFFFFFFu : 0x1FFu;
+  CmdXform xf = fixed_xform(ctx);
+  if (origin) {
+    // fill_unclipped_path_with_origin (rastercontext.cpp:2812-2821)
+    double ox = origin[0] * xf.m00 + origin[1] * xf.m10 + xf.m20;
+    double oy = origin[0] * xf.m01 + origin[1] * xf.m11 + xf.m21;
+    xf.m20 = ox;
+    xf.m21 = oy;
+  }
+  uint32_t ci = push_cmd(ctx, cmd, &xf);
+  record_path_segments(ctx, ci, cmds, vertices, n);
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_fill_polygon_d(BLAmdContext* ctx, const double* vertices, size_t n) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (n && !vertices) return kErrInvalid;
+  return fill_poly_rule(ctx, vertices, n, ctx->fill_rule);
+}
+
+BLAmdResult bl_amd_context_fill_rect_d(BLAmdContext* ctx, double x, double y, double w, double h) {
+  if (uint32_t r = check_target(ctx)) return r;
+  return fill_box_d(ctx, x, y, x + w, y + h);
+}
+
+// fillUnclippedRectI (rastercontext.cpp:2898-2917)
+BLAmdResult bl_amd_context_fill_rect_i(BLAmdContext* ctx, int x, int y, int w, int h) {
+  if (uint32_t r = check_target(ctx)) return r;
+  int tt = transform_type(ctx->user);
+  bool integral = false;
+  int tx = 0, ty = 0;
+  if (tt <= kTTTranslate) {
+    int64_t tx64 = floor_to_int64(ctx->user.m20 * 256.0), ty64 = floor_to_int64(ctx->user.m21 * 256.0);
+    if (((tx64 | ty64) & 0xFF) == 0) { integral = true; tx = (int)(tx64 >> 8); ty = (int)(ty64 >> 8); }
+  }
+  if (!integral) {
+    if (w <= 0 || h <= 0) return BL_AMD_SUCCESS;
+    return fill_box_d(ctx, (double)x, (double)y, (double)x + (double)w, (double)y + (double)h);
+  }
+  int64_t x0 = (int64_t)x + tx, y0 = (int64_t)y + ty;
+  int64_t x1 = (int64_t)w + x0, y1 = (int64_t)h + y0;
+  x0 = std::max<int64_t>(x0, 0); y0 = std::max<int64_t>(y0, 0);
+  x1 = std::min<int64_t>(x1, ctx->target->w); y1 = std::min<int64_t>(y1, ctx->target->h);
+  if (x0 >= x1 || y0 >= y1) return BL_AMD_SUCCESS;
+  return fill_box_a(ctx, (int)x0, (int)y0, (int)x1, (int)y1);
+}
+
+BLAmdResult bl_amd_context_fill_all(BLAmdContext* ctx) {
+  if (uint32_t r = check_target(ctx)) return r;
+  return fill_box_a(ctx, 0, 0, ctx->target->w, ctx->target->h);
+}
+
+BLAmdResult bl_amd_context_last_stats(const BLAmdContext* ctx, uint64_t out[5], double* render_ms) {
+  if (!ctx || !out) return kErrInvalid;
+  out[0] = ctx->stats.cmds; out[1] = ctx->stats.src; out[2] = ctx->stats.segs;
+  out[3] = ctx->stats.binned; out[4] = ctx->stats.tiles;
+  if (render_ms) *render_ms = ctx->stats.render_ms;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_batch_create(BLAmdBatch** out) {
+  if (!out) return kErrInvalid;
+  *out = new (std::nothrow) BLAmdBatch();
+  return *out ? BL_AMD_SUCCESS : kErrOOM;
+}
+
+BLAmdResult bl_amd_batch_destroy(BLAmdBatch* b) {
+  delete b;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_batch_add(BLAmdBatch* b, BLAmdContext* ctx) {
+  if (!b || !ctx || !ctx->target) return kErrInvalid;
+  b->contexts.push_back(ctx);
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_batch_clear(BLAmdBatch* b) {
+  if (!b) return kErrInvalid;
+  b->contexts.clear();
+  return BL_AMD_SUCCESS;
+}
+
+// Merges the pending frames of every context into one frame (indices
+// rebased) and renders them with one pass of each kernel stage.
+BLAmdResult bl_amd_batch_flush(BLAmdBatch* b) {
+  if (!b) return kErrInvalid;
+  FrameHost all;
+  for (BLAmdContext* c : b->contexts) {
+    if (!c->target) return kErrState;
+    finalize_frame(c);
+    const uint32_t cmd_base = (uint32_t)all.cmds.size();
+    const uint32_t vtx_base = (uint32_t)(all.vtx.size() / 2);
+    const uint32_t style_base = (uint32_t)all.styles.size();
+    const uint32_t lut_base = (uint32_t)all.luts.size();
+    const uint32_t img_base = (uint32_t)all.images.size();
+    const uint32_t canvas_idx = (uint32_t)all.canvases.size();
+    DevCanvas cv = c->frame.canvases[0];
+    cv.cmd_base = cmd_base;
+    all.canvases.push_back(cv);
+    for (DevCmd cmd : c->frame.cmds) {
+      cmd.style += style_base;
+      cmd.canvas = canvas_idx;
+      all.cmds.push_back(cmd);
+    }
+    all.xforms.insert(all.xforms.end(), c->frame.xforms.begin(), c->frame.xforms.end());
+    for (SrcSeg s : c->frame.src) { s.cmd += cmd_base; s.i0 += vtx_base; s.i1 += vtx_base; all.src.push_back(s); }
+    all.vtx.insert(all.vtx.end(), c->frame.vtx.begin(), c->frame.vtx.end());
+    for (DevStyle st : c->frame.styles) {
+      if (st.kind != kFetchSolid) st.lut += lut_base;
+      st.img += img_base;
+      all.styles.push_back(st);
+    }
+    all.luts.insert(all.luts.end(), c->frame.luts.begin(), c->frame.luts.end());
+    all.images.insert(all.images.end(), c->frame.images.begin(), c->frame.images.end());
+  }
+  uint32_t r = b->runner.run(all, b->stats);
+  for (BLAmdContext* c : b->contexts) reset_frame(c);
+  return r;
+}
+
+}  // extern "C"

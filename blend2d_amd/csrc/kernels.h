@@ -1,0 +1,61 @@
+// Launch interface of the fill-path kernels (kernels.hip), used by the host
+// frontend (context.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "device_types.h"
+
+namespace b2amd {
+
+// Device views of one flush ("frame") of recorded commands.
+struct FrameDev {
+  const DevCanvas* canvases; uint32_t n_canvases;
+  const DevCmd* cmds; uint32_t n_cmds;
+  const CmdXform* xforms;
+  const SrcSeg* src; uint32_t n_src;
+  const double* vtx;
+  const DevStyle* styles;
+  const uint32_t* luts;
+  const DevImage* images;
+  uint32_t n_tiles;
+  uint64_t n_mat;          // (band, cmd) matrix entries over all canvases
+  double tol_sq;           // squared flattening tolerance in 24.8 units
+};
+
+// Scratch buffers the pipeline needs; sized by the caller.
+struct FrameScratch {
+  uint32_t* src_count;     // n_src + 1
+  uint32_t* src_off;       // n_src + 1
+  CmdBounds* bounds;       // n_cmds
+  DevSeg* segs;            // n_segs
+  uint32_t* seg_cmd;       // n_segs
+  uint32_t* mat_cnt;       // n_mat + 1
+  uint32_t* mat_off;       // n_mat + 1
+  uint32_t* mat_pres;      // n_mat + 1
+  uint32_t* list_off;      // n_mat + 1
+  uint32_t* list;          // n_list
+  DevSeg* binned;          // n_binned
+  uint32_t* scan_tmp;      // scan_tmp_words(max n)
+};
+
+size_t scan_tmp_words(size_t n);
+void scan_exclusive(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp, hipStream_t s);
+
+// Stage 1: edge building.  count == true writes per-source segment counts and
+// command bounds; count == false writes the segments at the scanned offsets.
+void launch_edges(const FrameDev& f, const FrameScratch& s, bool count, hipStream_t st);
+// Stage 2: binning of segments into (band, command) cells.
+void launch_bin_count(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, hipStream_t st);
+void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st);
+void launch_bin_scatter(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, uint32_t* cursor, hipStream_t st);
+void launch_list_build(const FrameDev& f, const FrameScratch& s, hipStream_t st);
+// Stage 3: tile rasterization + fetch + composite.
+void launch_render(const FrameDev& f, const FrameScratch& s, hipStream_t st);
+
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+void launch_init_bounds(CmdBounds* b, uint32_t n, hipStream_t st);
+
+}  // namespace b2amd

@@ -1,0 +1,534 @@
+// Fill-path kernels for gfx950.
+//
+//   edges    one thread per source segment (line / quad / cubic): transform,
+//            clip, split, flatten -> 24.8 edge segments (count pass, then an
+//            emit pass at prefix-scanned offsets).
+//   bin      per segment: count, scatter into (band, command) lists; per band
+//            a compacted, submission-ordered list of the commands touching it.
+//   render   one 256-thread workgroup per 64x16 tile: the destination tile is
+//            loaded once into registers (128-bit per lane), every command of
+//            the band is rasterized into LDS cells with LDS atomics, turned
+//            into coverage by a per-scanline DPP scan, fetched, composited
+//            in submission order, and the tile is stored once.
+#include <hip/hip_runtime.h>
+
+#include "device_types.h"
+#include "edge_device.h"
+#include "kernels.h"
+#include "pipe_device.h"
+#include "raster_device.h"
+
+namespace b2amd {
+
+// ---------------------------------------------------------------------------
+// Scan (exclusive, uint32) -- three-level: block sums, scan of sums, apply.
+// ---------------------------------------------------------------------------
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanBlock = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total in `total`.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total) {
+  __shared__ uint32_t wsum[kScanThreads / 64];
+  uint32_t incl = wave_incl_scan(v);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t wprefix = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / 64; w++) {
+    uint32_t s = wsum[w];
+    if (w < wave) wprefix += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return wprefix + incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(const uint32_t* __restrict__ in, size_t n, uint32_t* __restrict__ sums) {
+  size_t base = (size_t)blockIdx.x * kScanBlock + (size_t)threadIdx.x * kScanItems;
+  uint32_t v = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) if (base + k < n) v += in[base + k];
+  uint32_t total;
+  block_excl_scan(v, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, size_t n,
+                                                          const uint32_t* __restrict__ prefix) {
+  size_t base = (size_t)blockIdx.x * kScanBlock + (size_t)threadIdx.x * kScanItems;
+  uint32_t x[kScanItems];
+  uint32_t v = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) { x[k] = (base + k < n) ? in[base + k] : 0u; v += x[k]; }
+  uint32_t total;
+  uint32_t excl = block_excl_scan(v, total);
+  uint32_t p = (prefix ? prefix[blockIdx.x] : 0u) + excl;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    if (base + k < n) out[base + k] = p;
+    p += x[k];
+  }
+  // The element one past the end receives the grand total.
+  if (base <= n && n < base + kScanItems) {
+    uint32_t tail = (prefix ? prefix[blockIdx.x] : 0u) + excl;
+    for (int k = 0; k < kScanItems && base + k < n; k++) tail += x[k];
+    out[n] = tail;
+  }
+}
+
+size_t scan_tmp_words(size_t n) {
+  size_t words = 0;
+  while (n > (size_t)kScanBlock) {
+    size_t nb = (n + kScanBlock - 1) / kScanBlock;
+    words += 2 * (nb + 1);
+    n = nb;
+  }
+  return words + 64;
+}
+
+void scan_exclusive(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp, hipStream_t s) {
+  if (n == 0) {
+    hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(kScanThreads), 0, s, in, out, (size_t)0, (const uint32_t*)nullptr);
+    return;
+  }
+  size_t nb = (n + kScanBlock - 1) / kScanBlock;
+  if (nb == 1) {
+    hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(kScanThreads), 0, s, in, out, n, (const uint32_t*)nullptr);
+    return;
+  }
+  uint32_t* sums = tmp;
+  uint32_t* sums_scan = tmp + (nb + 1);
+  hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, sums);
+  scan_exclusive(sums, sums_scan, nb, tmp + 2 * (nb + 1), s);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, out, n, (const uint32_t*)sums_scan);
+}
+
+// ---------------------------------------------------------------------------
+// Utilities
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+}
+
+__global__ void k_init_bounds(CmdBounds* b, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) { b[i].min_x = INT32_MAX; b[i].max_x = INT32_MIN; b[i].min_y = INT32_MAX; b[i].max_y = INT32_MIN; }
+}
+
+void launch_init_bounds(CmdBounds* b, uint32_t n, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_init_bounds, dim3((n + 255) / 256), dim3(256), 0, st, b, n);
+}
+
+// ---------------------------------------------------------------------------
+// Stage 1: edges
+// ---------------------------------------------------------------------------
+
+struct CountEmit {
+  uint32_t n = 0;
+  int32_t min_x = INT32_MAX, max_x = INT32_MIN, min_y = INT32_MAX, max_y = INT32_MIN;
+  __device__ __forceinline__ void push(int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint32_t) {
+    n++;
+    int32_t lo = (x0 < x1 ? x0 : x1) >> 8, hi = (x0 < x1 ? x1 : x0) >> 8;
+    min_x = lo < min_x ? lo : min_x;
+    max_x = hi > max_x ? hi : max_x;
+    min_y = y0 < min_y ? y0 : min_y;
+    max_y = y1 > max_y ? y1 : max_y;
+  }
+};
+
+struct WriteEmit {
+  DevSeg* segs;
+  uint32_t* seg_cmd;
+  uint32_t pos;
+  uint32_t cmd;
+  __device__ __forceinline__ void push(int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint32_t sign) {
+    segs[pos] = DevSeg{x0, y0, (int32_t)((uint32_t)x1 | (sign << 31)), y1};
+    seg_cmd[pos] = cmd;
+    pos++;
+  }
+};
+
+__device__ __forceinline__ edge::P xform_pt(const CmdXform& m, const double* vtx, uint32_t i) {
+  double x = vtx[2 * i], y = vtx[2 * i + 1];
+  if (m.scale_only) return edge::P{x * m.m00 + m.m20, y * m.m11 + m.m21};
+  return edge::P{x * m.m00 + y * m.m10 + m.m20, x * m.m01 + y * m.m11 + m.m21};
+}
+
+template<typename Emit>
+__device__ __forceinline__ void build_one(const FrameDev& f, const SrcSeg& sg, Emit& e) {
+  const DevCmd& cmd = f.cmds[sg.cmd];
+  const DevCanvas& cv = f.canvases[cmd.canvas];
+  const CmdXform& m = f.xforms[sg.cmd];
+  edge::Clip clip;
+  clip.x0 = 0.0; clip.y0 = 0.0;
+  clip.x1 = (double)cv.w * 256.0;
+  clip.y1 = (double)cv.h * 256.0;
+  clip.ix0 = (int32_t)clip.x0;
+  clip.ix1 = (int32_t)clip.x1;
+  edge::Builder<Emit> b(e, clip, f.tol_sq);
+  edge::P stack[edge::Builder<Emit>::kLimit * 4];
+  if (sg.kind == kSrcLine) {
+    b.line(xform_pt(m, f.vtx, sg.i0), xform_pt(m, f.vtx, sg.i1));
+  }
+  else if (sg.kind == kSrcQuad) {
+    edge::P c[3] = { xform_pt(m, f.vtx, sg.i0), xform_pt(m, f.vtx, sg.i1), xform_pt(m, f.vtx, sg.i1 + 1) };
+    b.curve(c, 3, stack);
+  }
+  else {
+    edge::P c[4] = { xform_pt(m, f.vtx, sg.i0), xform_pt(m, f.vtx, sg.i1), xform_pt(m, f.vtx, sg.i1 + 1), xform_pt(m, f.vtx, sg.i1 + 2) };
+    b.curve(c, 4, stack);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_edges_count(FrameDev f, FrameScratch s) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n_src) return;
+  SrcSeg sg = f.src[i];
+  CountEmit e;
+  build_one(f, sg, e);
+  s.src_count[i] = e.n;
+  if (e.n) {
+    CmdBounds* bd = &s.bounds[sg.cmd];
+    atomicMin(&bd->min_x, e.min_x);
+    atomicMax(&bd->max_x, e.max_x);
+    atomicMin(&bd->min_y, e.min_y);
+    atomicMax(&bd->max_y, e.max_y);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_edges_emit(FrameDev f, FrameScratch s) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n_src) return;
+  SrcSeg sg = f.src[i];
+  WriteEmit e{s.segs, s.seg_cmd, s.src_off[i], sg.cmd};
+  build_one(f, sg, e);
+}
+
+void launch_edges(const FrameDev& f, const FrameScratch& s, bool count, hipStream_t st) {
+  if (!f.n_src) return;
+  dim3 grid((f.n_src + 63) / 64), block(64);
+  if (count) hipLaunchKernelGGL(k_edges_count, grid, block, 0, st, f, s);
+  else hipLaunchKernelGGL(k_edges_emit, grid, block, 0, st, f, s);
+}
+
+// ---------------------------------------------------------------------------
+// Stage 2: binning
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void seg_bands(const DevSeg& g, int bands, int& b0, int& b1) {
+  int ey0 = g.y0 >> 8;
+  int ey1 = (g.y1 - 1) >> 8;
+  b0 = ey0 / kTileH;
+  b1 = ey1 / kTileH;
+  b0 = b0 < 0 ? 0 : b0;
+  b1 = b1 > bands - 1 ? bands - 1 : b1;
+}
+
+__global__ void k_bin_count(FrameDev f, FrameScratch s, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DevSeg g = s.segs[i];
+  uint32_t cmd = s.seg_cmd[i];
+  const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
+  int b0, b1;
+  seg_bands(g, (int)cv.bands, b0, b1);
+  uint64_t row = cv.mat_base + (uint64_t)(cmd - cv.cmd_base);
+  for (int b = b0; b <= b1; b++) atomicAdd(&s.mat_cnt[row + (uint64_t)b * cv.n_cmds], 1u);
+}
+
+void launch_bin_count(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, hipStream_t st) {
+  if (!n_segs) return;
+  hipLaunchKernelGGL(k_bin_count, dim3((n_segs + 255) / 256), dim3(256), 0, st, f, s, n_segs);
+}
+
+__global__ void k_presence(FrameScratch s, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) s.mat_pres[i] = s.mat_cnt[i] ? 1u : 0u;
+}
+
+__global__ void k_box_presence(FrameDev f, FrameScratch s) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= f.n_cmds) return;
+  const DevCmd& cmd = f.cmds[c];
+  if (cmd.kind == kCmdAnalytic) return;
+  const DevCanvas& cv = f.canvases[cmd.canvas];
+  int b0 = cmd.box[1] / kTileH, b1 = (cmd.box[3] - 1) / kTileH;
+  uint64_t row = cv.mat_base + (uint64_t)(c - cv.cmd_base);
+  for (int b = b0; b <= b1; b++) s.mat_pres[row + (uint64_t)b * cv.n_cmds] = 1u;
+}
+
+void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
+  if (f.n_mat) hipLaunchKernelGGL(k_presence, dim3((unsigned)((f.n_mat + 255) / 256)), dim3(256), 0, st, s, f.n_mat);
+  if (f.n_cmds) hipLaunchKernelGGL(k_box_presence, dim3((f.n_cmds + 255) / 256), dim3(256), 0, st, f, s);
+}
+
+__global__ void k_bin_scatter(FrameDev f, FrameScratch s, uint32_t n, uint32_t* cursor) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DevSeg g = s.segs[i];
+  uint32_t cmd = s.seg_cmd[i];
+  const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
+  int b0, b1;
+  seg_bands(g, (int)cv.bands, b0, b1);
+  uint64_t row = cv.mat_base + (uint64_t)(cmd - cv.cmd_base);
+  for (int b = b0; b <= b1; b++) {
+    uint32_t pos = atomicAdd(&cursor[row + (uint64_t)b * cv.n_cmds], 1u);
+    s.binned[pos] = g;
+  }
+}
+
+void launch_bin_scatter(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, uint32_t* cursor, hipStream_t st) {
+  if (!n_segs) return;
+  hipLaunchKernelGGL(k_bin_scatter, dim3((n_segs + 255) / 256), dim3(256), 0, st, f, s, n_segs, cursor);
+}
+
+__global__ void k_list_build(FrameScratch s, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && s.mat_pres[i]) s.list[s.list_off[i]] = (uint32_t)i;
+}
+
+void launch_list_build(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
+  if (f.n_mat) hipLaunchKernelGGL(k_list_build, dim3((unsigned)((f.n_mat + 255) / 256)), dim3(256), 0, st, s, f.n_mat);
+}
+
+// ---------------------------------------------------------------------------
+// Stage 3: render
+// ---------------------------------------------------------------------------
+
+struct TileSink {
+  uint32_t* cells;
+  uint32_t* backdrop;
+  int x0, y0;
+  __device__ __forceinline__ void add(int row, int x, uint32_t v) {
+    int r = row - y0;
+    if ((unsigned)r >= (unsigned)kTileH) return;
+    int lx = x - x0;
+    if (lx < 0) atomicAdd(&backdrop[r], v);
+    else if (lx < kTileW) atomicAdd(&cells[r * kTileW + lx], v);
+  }
+};
+
+__device__ __forceinline__ uint32_t calc_mask(uint32_t cov, uint32_t rule_mask, uint32_t alpha) {
+  uint32_t m = ((uint32_t)((int32_t)cov >> 9) & rule_mask) - 256u;
+  int32_t mi = (int32_t)m;
+  uint32_t a = (uint32_t)(mi < 0 ? -mi : mi);
+  a = a > 256u ? 256u : a;
+  return (a * alpha) >> 8;
+}
+
+__device__ __forceinline__ uint32_t row_excl_scan16(uint32_t v) {
+  const int q = threadIdx.x & 15;
+  uint32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < 16; d <<= 1) {
+    uint32_t t = __shfl_up(incl, d, 16);
+    if (q >= d) incl += t;
+  }
+  return incl - v;
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratch s) {
+  __shared__ uint32_t cells[kTileH * kTileW];
+  __shared__ uint32_t backdrop[kTileH];
+
+  const uint32_t tile = blockIdx.x;
+  // canvas lookup (uniform)
+  uint32_t lo = 0, hi = f.n_canvases;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (f.canvases[mid].tile_base <= tile) lo = mid; else hi = mid;
+  }
+  const DevCanvas cv = f.canvases[lo];
+  const uint32_t local = tile - cv.tile_base;
+  const int band = (int)(local / cv.tiles_x);
+  const int X0 = (int)(local % cv.tiles_x) * kTileW;
+  const int Y0 = band * kTileH;
+
+  const int t = threadIdx.x;
+  const int row = t >> 4;
+  const int q = t & 15;
+  const int y = Y0 + row;
+  const int px = X0 + q * 4;
+  const bool row_ok = y < (int)cv.h;
+  const bool a8 = cv.format == kFmtA8;
+
+  // zero LDS
+  *(uint4*)&cells[row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
+  if (t < kTileH) backdrop[t] = 0;
+
+  // load the destination tile
+  uint32_t d[4] = {0, 0, 0, 0};
+  uint8_t* rowp = (uint8_t*)cv.ptr + (int64_t)y * cv.stride;
+  const bool full4 = row_ok && px + 3 < (int)cv.w;
+  if (row_ok) {
+    if (a8) {
+      if (full4 && (((uintptr_t)(rowp + px)) & 3) == 0) {
+        uint32_t v = *(const uint32_t*)(rowp + px);
+#pragma unroll
+        for (int k = 0; k < 4; k++) d[k] = (v >> (8 * k)) & 0xFFu;
+      }
+      else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) if (px + k < (int)cv.w) d[k] = rowp[px + k];
+      }
+    }
+    else {
+      uint32_t* r32 = (uint32_t*)rowp;
+      if (full4 && (((uintptr_t)(r32 + px)) & 15) == 0) {
+        uint4 v = *(const uint4*)(r32 + px);
+        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+      }
+      else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) if (px + k < (int)cv.w) d[k] = r32[px + k];
+      }
+    }
+  }
+
+  const uint64_t row_base = cv.mat_base + (uint64_t)band * cv.n_cmds;
+  const uint32_t lb = s.list_off[row_base];
+  const uint32_t le = s.list_off[row_base + cv.n_cmds];
+  bool touched = false;
+
+  for (uint32_t li = lb; li < le; li++) {
+    const uint32_t e = s.list[li];
+    const uint32_t cmd_i = cv.cmd_base + (uint32_t)(e - row_base);
+    const DevCmd c = f.cmds[cmd_i];
+    uint32_t m[4] = {0, 0, 0, 0};
+
+    if (c.kind == kCmdAnalytic) {
+      const CmdBounds bd = s.bounds[cmd_i];
+      if (bd.max_x + 1 < X0 || bd.min_x >= X0 + kTileW) continue;
+      const uint32_t sb = s.mat_off[e];
+      const uint32_t se = sb + s.mat_cnt[e];
+      __syncthreads();   // previous command's sweep finished clearing cells
+      TileSink sink{cells, backdrop, X0, Y0};
+      for (uint32_t si = sb + t; si < se; si += kTileThreads) {
+        DevSeg g = s.binned[si];
+        uint32_t sign = (uint32_t)g.x1 >> 31;
+        int32_t x1 = g.x1 & 0x7FFFFFFF;
+        int32_t cmin = (g.x0 < x1 ? g.x0 : x1) >> 8;
+        int32_t cmax = (g.x0 < x1 ? x1 : g.x0) >> 8;
+        if (cmin >= X0 + kTileW) continue;
+        if (cmax + 1 < X0) {
+          // All cells of this segment are left of the tile: each covered row
+          // receives its signed vertical extent as a backdrop.
+          int r0 = g.y0 >> 8, r1 = (g.y1 - 1) >> 8;
+          if (r0 < Y0) r0 = Y0;
+          if (r1 > Y0 + kTileH - 1) r1 = Y0 + kTileH - 1;
+          uint32_t smask = 0u - sign;
+          for (int r = r0; r <= r1; r++) {
+            int lo_y = g.y0 > r * 256 ? g.y0 : r * 256;
+            int hi_y = g.y1 < (r + 1) * 256 ? g.y1 : (r + 1) * 256;
+            uint32_t cover = ras::apply_sign((uint32_t)(hi_y - lo_y), smask);
+            atomicAdd(&backdrop[r - Y0], cover << 9);
+          }
+          continue;
+        }
+        ras::Rasterizer rz;
+        rz.sign_mask = 0u - sign;
+        if (!rz.prepare(g.x0, g.y0, x1, g.y1)) continue;
+        if (rz.st.ey0 < Y0) rz.advance_to_y(Y0);
+        rz.band_end = Y0 + kTileH - 1;
+        rz.rasterize(sink);
+      }
+      __syncthreads();
+
+      uint4 cv4 = *(uint4*)&cells[row * kTileW + q * 4];
+      uint32_t bdv = backdrop[row];
+      *(uint4*)&cells[row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
+      if (q == 0) backdrop[row] = 0;
+      uint32_t sum = cv4.x + cv4.y + cv4.z + cv4.w;
+      uint32_t base = (256u << 9) + bdv + row_excl_scan16(sum);
+      uint32_t c0 = base + cv4.x, c1 = c0 + cv4.y, c2 = c1 + cv4.z, c3 = c2 + cv4.w;
+      m[0] = calc_mask(c0, c.rule_mask, c.alpha);
+      m[1] = calc_mask(c1, c.rule_mask, c.alpha);
+      m[2] = calc_mask(c2, c.rule_mask, c.alpha);
+      m[3] = calc_mask(c3, c.rule_mask, c.alpha);
+    }
+    else {
+      if (c.box[2] <= X0 || c.box[0] >= X0 + kTileW) continue;
+      if (y >= c.box[1] && y < c.box[3]) {
+        if (c.kind == kCmdBoxA) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) m[k] = (px + k >= c.box[0] && px + k < c.box[2]) ? c.alpha : 0u;
+        }
+        else {
+          int kind = (y == c.box_row_first) ? 0 : (y == c.box_row_last) ? 2 : 1;
+          uint32_t rm = c.box_mask[kind];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            int x = px + k;
+            if (x >= c.box[0] && x < c.box[2]) {
+              int col = (x == c.box[0]) ? 0 : (x == c.box[2] - 1) ? 2 : 1;
+              m[k] = (rm >> (8 * col)) & 0xFFu;
+            }
+          }
+        }
+      }
+    }
+
+    if (!row_ok || (m[0] | m[1] | m[2] | m[3]) == 0) continue;
+    touched = true;
+
+    const DevStyle& st = f.styles[c.style];
+    const int fill_row0 = (c.kind == kCmdAnalytic) ? (s.bounds[cmd_i].min_y >> 8) : (int)c.fetch_row0;
+    const pipe::Row rs = pipe::row_setup(st, y, fill_row0, c.kind == kCmdAnalytic ? cv.ref_band_h : 0u);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (!m[k] || px + k >= (int)cv.w) continue;
+      uint32_t src = pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
+      d[k] = a8 ? pipe::composite8(c.op, d[k], src >> 24, m[k]) : pipe::composite32(c.op, d[k], src, m[k]);
+    }
+  }
+
+  if (!touched || !row_ok) return;
+  if (a8) {
+    if (full4 && (((uintptr_t)(rowp + px)) & 3) == 0) {
+      *(uint32_t*)(rowp + px) = d[0] | (d[1] << 8) | (d[2] << 16) | (d[3] << 24);
+    }
+    else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) if (px + k < (int)cv.w) rowp[px + k] = (uint8_t)d[k];
+    }
+  }
+  else {
+    uint32_t* r32 = (uint32_t*)rowp;
+    if (full4 && (((uintptr_t)(r32 + px)) & 15) == 0) {
+      *(uint4*)(r32 + px) = make_uint4(d[0], d[1], d[2], d[3]);
+    }
+    else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) if (px + k < (int)cv.w) r32[px + k] = d[k];
+    }
+  }
+}
+
+void launch_render(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
+  if (!f.n_tiles) return;
+  hipLaunchKernelGGL(k_render, dim3(f.n_tiles), dim3(kTileThreads), 0, st, f, s);
+}
+
+}  // namespace b2amd

@@ -1,0 +1,261 @@
+// Device fetchers and composition operators.
+//
+// Fetch: solid, linear/radial/conic gradient LUT lookup, affine pattern with
+// nearest or bilinear filtering (pipeline/reference/fetchgeneric_p.h).  The
+// reference steps each fetcher per pixel and per row; all of them are pure
+// functions of the pixel position except the conic gradient, whose per-row
+// origin accumulates in double precision from the first row of the fill call
+// (fetchgeneric_p.h:1172-1173, 1227-1230).  That accumulation is reproduced.
+//
+// Composite: SRC_COPY / SRC_OVER / PLUS (pipeline/reference/compopgeneric_p.h)
+// and MULTIPLY / SCREEN (pipeline/jit/compoppart.cpp:4291-4460, 4592-4628)
+// with Blend2D's 8-bit div255 rounding (pixelgeneric_p.h:386-389).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_types.h"
+
+namespace b2amd {
+namespace pipe {
+
+__device__ __forceinline__ uint32_t div255(uint32_t x) {
+  uint32_t u = x + 0x80u;
+  return ((u + ((u >> 8) & 0xFFu)) >> 8) & 0xFFu;
+}
+
+__device__ __forceinline__ uint32_t ch(uint32_t p, int i) { return (p >> (i * 8)) & 0xFFu; }
+
+// x86 cvttss2si / cvtss2si: out-of-range and NaN produce INT32_MIN.
+__device__ __forceinline__ int32_t trunc_f32_to_i32(float v) {
+  if (!(v >= -2147483648.0f && v < 2147483648.0f)) return INT32_MIN;
+  return (int32_t)v;
+}
+__device__ __forceinline__ int32_t rint_f32_to_i32(float v) {
+  if (!(v >= -2147483648.0f && v < 2147483648.0f)) return INT32_MIN;
+  return (int32_t)__builtin_rintf(v);
+}
+
+// Per (command, row) fetch state.
+struct Row {
+  uint64_t a, b;       // linear: pt at x = 0; pattern: tx, ty of the row
+  float f0, f1, f2;    // radial: b, d, dd;  conic: tx, ay, by
+};
+
+__device__ __forceinline__ Row row_setup(const DevStyle& s, int y, int fill_row0, uint32_t ref_band_h) {
+  Row r;
+  r.a = 0; r.b = 0; r.f0 = 0.f; r.f1 = 0.f; r.f2 = 0.f;
+  switch (s.kind) {
+    case kFetchLinear:
+      r.a = s.lin_pt0 + (uint64_t)(uint32_t)y * s.lin_dy;
+      break;
+    case kFetchRadial: {
+      double yd = (double)y;
+      double ptx = s.rad_yx * yd + s.rad_tx;
+      double pty = s.rad_yy * yd + s.rad_ty;
+      double b = yd * s.rad_by + s.rad_b0;
+      double sq_dist = ptx * ptx + pty * pty;
+      r.f0 = (float)(b * s.rad_inv2a);
+      r.f1 = (float)((s.rad_amul4 * (sq_dist - s.rad_sq_fr) + b * b) * s.rad_sq_inv2a);
+      r.f2 = (float)((yd * s.rad_ddy + s.rad_dd0) * s.rad_sq_inv2a);
+      break;
+    }
+    case kFetchConic: {
+      // First row of the reference fill call that covers `y`: analytic fills
+      // run per band (first band starts at the path's first row), box fills
+      // start at the box's first row.
+      int ys = fill_row0;
+      if (ref_band_h) {
+        int band_start = (y / (int)ref_band_h) * (int)ref_band_h;
+        if (band_start > ys) ys = band_start;
+      }
+      double tpx = s.con_tx + s.con_yx * (double)ys;
+      double tpy = s.con_ty + s.con_yy * (double)ys;
+      for (int k = ys; k < y; k++) { tpx += s.con_yx; tpy += s.con_yy; }
+      float ay = (float)tpy;
+      r.f0 = (float)tpx;
+      r.f2 = __builtin_signbit(ay) ? s.con_n1 : 0.0f;
+      r.f1 = __builtin_fabsf(ay);
+      break;
+    }
+    case kFetchPatternNN:
+    case kFetchPatternBI:
+      r.a = s.pa_tx + s.pa_yx * (uint64_t)(uint32_t)y;
+      r.b = s.pa_ty + s.pa_yy * (uint64_t)(uint32_t)y;
+      break;
+    default:
+      break;
+  }
+  return r;
+}
+
+__device__ __forceinline__ uint32_t grad_index(uint32_t idx, const DevStyle& s) {
+  if (s.extend == 0) {
+    int32_t i = (int32_t)idx;
+    if (i < 0) i = 0;
+    if (i > (int32_t)s.maxi) i = (int32_t)s.maxi;
+    return (uint32_t)i;
+  }
+  uint32_t a = idx & s.maxi, b = a ^ s.rori;
+  return b < a ? b : a;
+}
+
+__device__ __forceinline__ uint64_t pa_norm(int32_t tw, int32_t rx, int32_t ox, uint64_t v) {
+  uint32_t x = (uint32_t)((int32_t)(v >> 32) % tw);
+  if ((int32_t)x < 0) x += (uint32_t)rx;
+  if ((int32_t)x > ox) x -= (uint32_t)rx;
+  return ((uint64_t)x << 32) | (v & 0xFFFFFFFFull);
+}
+
+__device__ __forceinline__ int32_t pa_index(int32_t v, int32_t mn, int32_t mx, int32_t cor) {
+  if (v < mn) v = mn;
+  if (v > mx) v = cor;
+  return v ^ (v >> 31);
+}
+
+__device__ __forceinline__ uint32_t texel(const DevImage& im, uint32_t fmt, int32_t x, int32_t y) {
+  const uint8_t* row = (const uint8_t*)im.ptr + (int64_t)y * im.stride;
+  if (fmt == kFmtA8) return (uint32_t)row[x] * 0x01010101u;
+  uint32_t v = ((const uint32_t*)row)[x];
+  if (fmt == kFmtXRGB32) v |= 0xFF000000u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t fetch(const DevStyle& s, const Row& r, int x, const uint32_t* __restrict__ luts,
+                                          const DevImage* __restrict__ images, uint32_t src_fmt) {
+  switch (s.kind) {
+    case kFetchSolid:
+      return s.solid;
+    case kFetchLinear: {
+      uint64_t pt = r.a + (uint64_t)(uint32_t)x * s.lin_dt;
+      return luts[s.lut + grad_index((uint32_t)(pt >> 32), s)];
+    }
+    case kFetchRadial: {
+      float fx = (float)x;
+      float sq_x = fx * fx;
+      float a = __builtin_sqrtf(__builtin_fabsf(sq_x * s.rad_ddd + (fx * r.f2 + r.f1)));
+      float v = (fx * s.rad_bd + r.f0) + a;
+      return luts[s.lut + grad_index((uint32_t)trunc_f32_to_i32(v), s)];
+    }
+    case kFetchConic: {
+      float fx = (float)x;
+      float xv = fx * s.con_xx + r.f0;
+      float ax = __builtin_fabsf(xv);
+      float ay = r.f1;
+      float xy_min = ay < ax ? ay : ax;
+      float xy_max = ax < ay ? ay : ax;
+      float sv = (ax == xy_min) ? s.con_n4 : 0.0f;
+      float p = xy_min / xy_max;
+      float p_sq = p * p;
+      float v = p_sq * s.con_q[3] + s.con_q[2];
+      v = v * p_sq + s.con_q[1];
+      v = v * p_sq + s.con_q[0];
+      v = __builtin_fabsf(v * p - sv);
+      v = __builtin_fabsf(v - (__builtin_signbit(xv) ? s.con_n2 : 0.0f));
+      v = __builtin_fabsf(v - r.f2) + s.con_offset;
+      int32_t i = rint_f32_to_i32(v);
+      if (i > s.con_maxi) i = s.con_maxi;
+      return luts[s.lut + ((uint32_t)i & (uint32_t)s.con_rori)];
+    }
+    case kFetchPatternNN:
+    case kFetchPatternBI: {
+      const DevImage& im = images[s.img];
+      uint64_t px = pa_norm(s.pa_tw, s.pa_rx, s.pa_ox, r.a + s.pa_xx * (uint64_t)(uint32_t)x);
+      uint64_t py = pa_norm(s.pa_th, s.pa_ry, s.pa_oy, r.b + s.pa_xy * (uint64_t)(uint32_t)x);
+      int32_t ix = (int32_t)(px >> 32), iy = (int32_t)(py >> 32);
+      if (s.kind == kFetchPatternNN)
+        return texel(im, src_fmt, pa_index(ix, s.pa_minx, s.pa_maxx, s.pa_corx), pa_index(iy, s.pa_miny, s.pa_maxy, s.pa_cory));
+      int32_t x0 = pa_index(ix, s.pa_minx, s.pa_maxx, s.pa_corx);
+      int32_t y0 = pa_index(iy, s.pa_miny, s.pa_maxy, s.pa_cory);
+      int32_t x1 = pa_index(ix + 1, s.pa_minx, s.pa_maxx, s.pa_corx);
+      int32_t y1 = pa_index(iy + 1, s.pa_miny, s.pa_maxy, s.pa_cory);
+      uint32_t wx = (uint32_t)(px & 0xFFFFFFFFull) >> 24;
+      uint32_t wy = (uint32_t)(py & 0xFFFFFFFFull) >> 24;
+      uint32_t iw = 256 - wx, ih = 256 - wy;
+      uint32_t t00 = texel(im, src_fmt, x0, y0), t01 = texel(im, src_fmt, x0, y1);
+      uint32_t t10 = texel(im, src_fmt, x1, y0), t11 = texel(im, src_fmt, x1, y1);
+      uint32_t out = 0;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        uint32_t p0 = ((ch(t00, c) * ih + ch(t01, c) * wy) >> 8) & 0xFFu;
+        uint32_t p1 = ((ch(t10, c) * ih + ch(t11, c) * wy) >> 8) & 0xFFu;
+        out |= (((p0 * iw + p1 * wx) >> 8) & 0xFFu) << (c * 8);
+      }
+      return out;
+    }
+    default:
+      return 0;
+  }
+}
+
+// One PRGB32 destination pixel.
+__device__ __forceinline__ uint32_t composite32(uint32_t op, uint32_t d, uint32_t s, uint32_t m) {
+  uint32_t r = 0;
+  switch (op) {
+    case kOpSrcCopy: {
+#pragma unroll
+      for (int i = 0; i < 4; i++) r |= div255(ch(d, i) * (m ^ 0xFFu) + ch(s, i) * m) << (i * 8);
+      return r;
+    }
+    case kOpSrcOver: {
+      uint32_t s1 = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) s1 |= div255(ch(s, i) * m) << (i * 8);
+      uint32_t ia = (s1 >> 24) ^ 0xFFu, t = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) t |= div255(ch(d, i) * ia) << (i * 8);
+      return s1 + t;
+    }
+    case kOpPlus: {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        uint32_t v = ch(d, i) + div255(ch(s, i) * m);
+        r |= (v > 255u ? 255u : v) << (i * 8);
+      }
+      return r;
+    }
+    case kOpMultiply: {
+      uint32_t s1[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) s1[i] = div255(ch(s, i) * m);
+      uint32_t da = ch(d, 3);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        uint32_t v = ch(d, i) * ((s1[i] + (s1[3] ^ 0xFFu)) & 0xFFFFu) + s1[i] * (da ^ 0xFFu);
+        r |= div255(v & 0xFFFFu) << (i * 8);
+      }
+      return r;
+    }
+    case kOpScreen: {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        uint32_t s1 = div255(ch(s, i) * m);
+        r |= ((div255(ch(d, i) * (s1 ^ 0xFFu)) + s1) & 0xFFu) << (i * 8);
+      }
+      return r;
+    }
+    default:
+      return d;
+  }
+}
+
+// One A8 destination pixel; `sa` is the source alpha.
+__device__ __forceinline__ uint32_t composite8(uint32_t op, uint32_t d, uint32_t sa, uint32_t m) {
+  switch (op) {
+    case kOpSrcCopy: return div255(d * (m ^ 0xFFu) + sa * m);
+    case kOpSrcOver: {
+      uint32_t s1 = div255(sa * m);
+      return (s1 + div255(d * (s1 ^ 0xFFu))) & 0xFFu;
+    }
+    case kOpPlus: {
+      uint32_t v = d + div255(sa * m);
+      return v > 255u ? 255u : v;
+    }
+    default:
+      return d;
+  }
+}
+
+}  // namespace pipe
+}  // namespace b2amd

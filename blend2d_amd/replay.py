@@ -1,0 +1,110 @@
+"""Replays a recorded Scene (blend2d_amd.scene) through the MI355X backend."""
+from __future__ import annotations
+
+import struct
+
+import numpy as np
+
+from . import scene as S
+
+
+def _images(ctx_cls, sc):
+    from . import BLImage
+    out = []
+    for fmt, px in sc.images:
+        h, w = px.shape
+        img = BLImage(w, h, fmt)
+        img.upload(px)
+        out.append(img)
+    return out
+
+
+def replay_into(ctx, sc: "S.Scene", images) -> list:
+    """Issues the scene's calls on `ctx`; returns the BLResult errors seen
+    (calls keep going after an error, as the reference driver does)."""
+    from . import BLError, BLGradient, BLPattern
+    errors = []
+
+    def call(fn, *args):
+        try:
+            fn(*args)
+        except BLError as e:
+            errors.append(e.code)
+
+    for op, pl in sc.cmds:
+        if op == S.OP_SET_COMP_OP:
+            call(ctx.set_comp_op, struct.unpack_from("<I", pl)[0])
+        elif op == S.OP_SET_FILL_RULE:
+            call(ctx.set_fill_rule, struct.unpack_from("<I", pl)[0])
+        elif op == S.OP_SET_TRANSFORM:
+            call(ctx.set_transform, struct.unpack_from("<6d", pl))
+        elif op == S.OP_SET_GLOBAL_ALPHA:
+            call(ctx.set_global_alpha, struct.unpack_from("<d", pl)[0])
+        elif op == S.OP_SET_PATTERN_QUALITY:
+            call(ctx.set_pattern_quality, struct.unpack_from("<I", pl)[0])
+        elif op == S.OP_SET_STYLE_SOLID:
+            call(ctx.set_fill_style, struct.unpack_from("<I", pl)[0])
+        elif op == S.OP_SET_STYLE_GRADIENT:
+            h = struct.unpack_from("<IIII6d6d", pl)
+            gtype, extend, n, has_tr = h[:4]
+            values, tr = list(h[4:10]), list(h[10:16])
+            stops = [struct.unpack_from("<dI", pl, 112 + 16 * k) for k in range(n)]
+            call(ctx.set_fill_style, BLGradient(gtype, values, extend, stops, tr if has_tr else None))
+        elif op == S.OP_SET_STYLE_PATTERN:
+            h = struct.unpack_from("<IIII6d", pl)
+            idx, extend, has_tr = h[0], h[1], h[2]
+            call(ctx.set_fill_style, BLPattern(images[idx], extend, list(h[4:10]) if has_tr else None))
+        elif op == S.OP_FILL_PATH:
+            n = struct.unpack_from("<I", pl)[0]
+            cmds = np.frombuffer(pl, dtype=np.uint8, count=n, offset=8)
+            vtx = np.frombuffer(pl, dtype=np.float64, count=2 * n, offset=8 + ((n + 7) & ~7)).reshape(-1, 2)
+            call(ctx.fill_path, (cmds, vtx))
+        elif op == S.OP_FILL_POLYGON:
+            n = struct.unpack_from("<I", pl)[0]
+            call(ctx.fill_polygon, np.frombuffer(pl, dtype=np.float64, count=2 * n, offset=8).reshape(-1, 2))
+        elif op == S.OP_FILL_RECT_D:
+            call(ctx.fill_rect, *[float(v) for v in struct.unpack_from("<4d", pl)])
+        elif op == S.OP_FILL_RECT_I:
+            call(ctx.fill_rect, *[int(v) for v in struct.unpack_from("<4i", pl)])
+        elif op == S.OP_FILL_ALL:
+            call(ctx.fill_all)
+    return errors
+
+
+def render_scene(sc: "S.Scene", return_context: bool = False):
+    """Renders one scene on the GPU; returns (pixels, first_error[, ctx])."""
+    from . import BLContext, BLImage
+    images = _images(BLContext, sc)
+    img = BLImage(sc.width, sc.height, sc.format)
+    img.fill_raw(sc.clear_value)
+    ctx = BLContext(img)
+    errs = replay_into(ctx, sc, images)
+    ctx.flush()
+    stats = ctx.last_stats()
+    ctx.end()
+    out = img.download()
+    if return_context:
+        return out, (errs[0] if errs else 0), stats
+    return out, (errs[0] if errs else 0)
+
+
+def render_scenes_batched(scenes):
+    """Renders many scenes (canvases) with one batched flush."""
+    from . import BLBatch, BLContext, BLImage
+    batch = BLBatch()
+    keep = []
+    errors = []
+    for sc in scenes:
+        images = _images(BLContext, sc)
+        img = BLImage(sc.width, sc.height, sc.format)
+        img.fill_raw(sc.clear_value)
+        ctx = BLContext(img)
+        errs = replay_into(ctx, sc, images)
+        errors.append(errs[0] if errs else 0)
+        batch.add(ctx)
+        keep.append((img, ctx, images))
+    batch.flush()
+    outs = [img.download() for img, _, _ in keep]
+    for img, ctx, _ in keep:
+        ctx.end()
+    return outs, errors

@@ -1,0 +1,135 @@
+/* blend2d_amd -- MI355X (gfx950) backend for Blend2D's fill path.
+ *
+ * C ABI of libblend2d_amd.so.  Every entry point returns a BLResult-compatible
+ * code (0 = success, errors from 0x10000, core/api.h:1143-1152 of the
+ * reference) and never throws.  Types are plain: geometry arrives as the
+ * arrays a BLPath stores (command bytes + double vertex pairs,
+ * core/path.h:22-39, 411), colours as 0xAARRGGBB straight-alpha rgba32,
+ * matrices as six doubles [m00 m01 m10 m11 m20 m21] (core/matrix.h).
+ *
+ * Each entry cites the reference C API it replaces.  The context records
+ * calls and renders them on the GPU at flush/end, the asynchronous model of
+ * the reference (BLContextCreateInfo::thread_count > 0, context.h:103-110),
+ * so pixels become visible after bl_amd_context_flush(SYNC) or end().
+ */
+#ifndef BLEND2D_AMD_H
+#define BLEND2D_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef uint32_t BLAmdResult;
+typedef struct BLAmdImage BLAmdImage;
+typedef struct BLAmdContext BLAmdContext;
+typedef struct BLAmdBatch BLAmdBatch;
+
+enum {
+  BL_AMD_SUCCESS = 0,
+  BL_AMD_ERROR_OUT_OF_MEMORY = 0x10000u,
+  BL_AMD_ERROR_INVALID_VALUE = 0x10001u,
+  BL_AMD_ERROR_INVALID_STATE = 0x10002u,
+  BL_AMD_ERROR_NOT_IMPLEMENTED = 0x10007u,
+  BL_AMD_ERROR_DEVICE = 0x1F000u
+};
+
+/* Pixel formats: BLFormat (core/format.h). */
+enum { BL_AMD_FORMAT_PRGB32 = 1, BL_AMD_FORMAT_XRGB32 = 2, BL_AMD_FORMAT_A8 = 3 };
+
+/* ---- runtime --------------------------------------------------------- */
+
+/* Library version: major << 16 | minor << 8 | patch. */
+uint32_t bl_amd_version(void);
+/* Returns the number of HIP devices (0 when none is usable). */
+int bl_amd_device_count(void);
+
+/* ---- images (BLImage, core/image.h) ---------------------------------- */
+
+/* bl_image_create (core/image.h:111): device-resident image. */
+BLAmdResult bl_amd_image_create(BLAmdImage** out, int w, int h, uint32_t format);
+/* bl_image_destroy. */
+BLAmdResult bl_amd_image_destroy(BLAmdImage* img);
+/* Copies host pixels into the image (bl_image_make_mutable + write). */
+BLAmdResult bl_amd_image_upload(BLAmdImage* img, const void* pixels, intptr_t stride);
+/* Copies the image to host memory (bl_image_get_data + read). */
+BLAmdResult bl_amd_image_download(const BLAmdImage* img, void* pixels, intptr_t stride);
+/* Fills every pixel with a raw value (A8 uses the low byte). */
+BLAmdResult bl_amd_image_fill_raw(BLAmdImage* img, uint32_t value);
+/* BLImageData: size, format, device address of row 0, stride in bytes. */
+BLAmdResult bl_amd_image_get_data(const BLAmdImage* img, int* w, int* h, uint32_t* format,
+                                  void** device_pixels, intptr_t* stride);
+
+/* ---- rendering context (BLContext, core/context.h:631-760) ------------- */
+
+BLAmdResult bl_amd_context_create(BLAmdContext** out);
+BLAmdResult bl_amd_context_destroy(BLAmdContext* ctx);
+/* bl_context_begin (context.h:634): attach a target, reset the state. */
+BLAmdResult bl_amd_context_begin(BLAmdContext* ctx, BLAmdImage* target);
+/* bl_context_end (context.h:636): flush synchronously and detach. */
+BLAmdResult bl_amd_context_end(BLAmdContext* ctx);
+/* bl_context_flush (context.h:637); flags: BL_CONTEXT_FLUSH_SYNC = 0x80000000. */
+BLAmdResult bl_amd_context_flush(BLAmdContext* ctx, uint32_t flags);
+
+/* bl_context_set_comp_op (context.h:697): BLCompOp values; SRC_OVER, SRC_COPY,
+ * PLUS, MULTIPLY, SCREEN render, others report NOT_IMPLEMENTED at fill time. */
+BLAmdResult bl_amd_context_set_comp_op(BLAmdContext* ctx, uint32_t comp_op);
+/* bl_context_set_fill_rule (context.h:700): 0 non-zero, 1 even-odd. */
+BLAmdResult bl_amd_context_set_fill_rule(BLAmdContext* ctx, uint32_t fill_rule);
+/* bl_context_set_global_alpha (context.h:694). */
+BLAmdResult bl_amd_context_set_global_alpha(BLAmdContext* ctx, double alpha);
+/* bl_context_apply_transform_op(BL_TRANSFORM_OP_ASSIGN) (context.h:468 virt). */
+BLAmdResult bl_amd_context_set_transform(BLAmdContext* ctx, const double m[6]);
+/* bl_context_set_hint(BL_CONTEXT_HINT_PATTERN_QUALITY) (context.h:408-422):
+ * 0 nearest, 1 bilinear (default). */
+BLAmdResult bl_amd_context_set_pattern_quality(BLAmdContext* ctx, uint32_t quality);
+
+/* bl_context_set_fill_style_rgba32 (context.h:673). */
+BLAmdResult bl_amd_context_set_fill_style_rgba32(BLAmdContext* ctx, uint32_t rgba32);
+/* bl_context_set_fill_style(BLGradient) (context.h:671) with the gradient
+ * passed by value: type 0 linear [x0 y0 x1 y1], 1 radial [x0 y0 x1 y1 r0 r1],
+ * 2 conic [x0 y0 angle repeat]; extend 0 pad, 1 repeat, 2 reflect; stops as
+ * BLGradient::add_stop would receive them; transform may be NULL. */
+BLAmdResult bl_amd_context_set_fill_style_gradient(BLAmdContext* ctx, uint32_t type, const double* values,
+                                                   uint32_t extend, const double* stop_offsets,
+                                                   const uint32_t* stop_rgba32, size_t n_stops,
+                                                   const double* transform);
+/* bl_context_set_fill_style(BLPattern) (context.h:671): image + extend mode +
+ * optional pattern transform. */
+BLAmdResult bl_amd_context_set_fill_style_pattern(BLAmdContext* ctx, const BLAmdImage* image, uint32_t extend,
+                                                  const double* transform);
+
+/* bl_context_fill_path_d (context.h:750): BLPath command bytes and vertices;
+ * origin may be NULL (same as fill_geometry(BL_GEOMETRY_TYPE_PATH)). */
+BLAmdResult bl_amd_context_fill_path_d(BLAmdContext* ctx, const double* origin, const uint8_t* cmds,
+                                       const double* vertices, size_t n);
+/* bl_context_fill_geometry(BL_GEOMETRY_TYPE_POLYGOND) (context.h:755). */
+BLAmdResult bl_amd_context_fill_polygon_d(BLAmdContext* ctx, const double* vertices, size_t n);
+/* bl_context_fill_rect_d (context.h:745). */
+BLAmdResult bl_amd_context_fill_rect_d(BLAmdContext* ctx, double x, double y, double w, double h);
+/* bl_context_fill_rect_i (context.h:740). */
+BLAmdResult bl_amd_context_fill_rect_i(BLAmdContext* ctx, int x, int y, int w, int h);
+/* bl_context_fill_all (context.h:735). */
+BLAmdResult bl_amd_context_fill_all(BLAmdContext* ctx);
+
+/* Statistics of the last flush: commands, source segments, edge segments,
+ * binned segments, tiles; device time of the render kernel in ms. */
+BLAmdResult bl_amd_context_last_stats(const BLAmdContext* ctx, uint64_t out[5], double* render_ms);
+
+/* ---- batching (no counterpart in the reference) ------------------------
+ * Renders the pending calls of many contexts (canvases of any size and
+ * format) with one set of kernel launches: one edge pass, one binning pass
+ * and one render grid covering every tile of every canvas. */
+BLAmdResult bl_amd_batch_create(BLAmdBatch** out);
+BLAmdResult bl_amd_batch_destroy(BLAmdBatch* batch);
+BLAmdResult bl_amd_batch_add(BLAmdBatch* batch, BLAmdContext* ctx);
+BLAmdResult bl_amd_batch_clear(BLAmdBatch* batch);
+BLAmdResult bl_amd_batch_flush(BLAmdBatch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLEND2D_AMD_H */

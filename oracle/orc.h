@@ -33,6 +33,7 @@ uint32_t orc_composite_pixel(uint32_t op, uint32_t dst_format, uint32_t d, uint3
 uint32_t orc_simplify_op(uint32_t op, uint32_t dst_fmt, uint32_t src_fmt);
 
 uint32_t orc_div255(uint32_t x);
+void orc_segments_mask(const int32_t* segs, size_t n, uint32_t w, uint32_t h, uint32_t fill_rule, uint8_t* out);
 uint32_t orc_premultiply(uint32_t argb32);
 
 #ifdef __cplusplus

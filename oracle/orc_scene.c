@@ -462,3 +462,32 @@ uint32_t orc_simplify_op(uint32_t op, uint32_t dst_fmt, uint32_t src_fmt) {
   uint32_t r = orc_simplify(op, dst_fmt, src_fmt, &d, &s, &id);
   return (r & 0xFFu) | (d << 8) | (s << 16) | (id << 24);
 }
+
+/* Coverage masks of a segment list: the segments rasterized as one fill
+ * (SRC_COPY, opaque white, alpha 255) onto an A8 canvas cleared to zero, so
+ * each pixel holds its mask.  `segs` holds (x0, y0, x1, y1, sign) records. */
+void orc_segments_mask(const int32_t* segs, size_t n, uint32_t w, uint32_t h, uint32_t fill_rule, uint8_t* out) {
+  OrcEdges e;
+  orc_edges_init(&e);
+  e.segs = (OrcSeg*)malloc((n ? n : 1) * sizeof(OrcSeg));
+  e.cap = n;
+  for (size_t i = 0; i < n; i++) {
+    OrcSeg* s = &e.segs[e.n++];
+    s->x0 = segs[i * 5 + 0]; s->y0 = segs[i * 5 + 1]; s->x1 = segs[i * 5 + 2]; s->y1 = segs[i * 5 + 3];
+    s->sign = (uint32_t)segs[i * 5 + 4];
+    if (s->y0 < e.bbox_y0) e.bbox_y0 = s->y0;
+    if (s->y1 > e.bbox_y1) e.bbox_y1 = s->y1;
+  }
+  memset(out, 0, (size_t)w * h);
+  OrcCanvas c = { out, (intptr_t)w, w, h, ORC_FMT_A8 };
+  OrcFetch f;
+  memset(&f, 0, sizeof(f));
+  f.type = ORC_FETCH_SOLID;
+  f.solid_prgb32 = 0xFFFFFFFFu;
+  OrcPipe p;
+  memset(&p, 0, sizeof(p));
+  p.fetch = &f;
+  p.op = ORC_OP_SRC_COPY;
+  orc_fill_analytic(&c, &e, &p, 255, fill_rule);
+  orc_edges_free(&e);
+}
