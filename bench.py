@@ -1,0 +1,266 @@
+"""Fill-path throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4|srcover4k]
+
+A step renders one whole synthetic frame through the device pipeline (edge
+building, binning, tile rasterization + fetch + composite) from a frame
+already resident in HBM (bl_amd_context_replay / bl_amd_batch_replay): the
+commands, geometry, styles and LUTs were recorded and uploaded once before
+the timed region.
+
+value      Mpix/s = pixels composited with a non-zero mask per frame x K /
+           wall time of the K steps (host clock, device synchronized on both
+           sides).  paths/s and canvas Mpix/s are reported beside it.
+roofline   the render kernel (k_render), timed with HIP events on the stream
+           it runs on, against the HBM roofline: algorithmic bytes per launch
+           = 8 B per composited PRGB32 pixel (2 B for A8: destination read +
+           write) + gradient LUT bytes + pattern image bytes + 16 B per binned
+           edge segment it reads.
+cpu_baseline  the reference library (oracle/_ref, compiled from the reference
+           sources without its JIT) rendering the same frame with worker
+           threads on this host's cores; its first frame is also the
+           bit-exactness check of the GPU output.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpix/s + paths/s at bit-exact output, batched+learned vs plain, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+HOST_CORES_MAX = 16          # CPU share of a one-GPU box
+
+
+def _sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def _workload(name: str, seed: int):
+    from blend2d_amd import scene as S
+    if name == "c1":
+        return [S.polygons_scene(seed, 512, 512, 1000)], {
+            "workload": "512x512 PRGB32, 1000 random polygons (3-12 vertices), non-zero, solid, SRC_OVER",
+            "canvases": 1, "paths": 1000, "seed": seed}
+    if name == "c2":
+        return [S.cubic_gradient_scene(seed, 1920, 1080, 10000)], {
+            "workload": "1920x1080 PRGB32, 10000 closed cubic paths (4-16 segments), even-odd, "
+                        "50% linear / 50% radial gradients (2-8 stops, PAD), SRC_OVER",
+            "canvases": 1, "paths": 10000, "seed": seed}
+    if name == "c3":
+        return [S.pattern_shapes_scene(seed, 3840, 2160, 2000)], {
+            "workload": "3840x2160 PRGB32, 2000 rotated rects / ellipses, 512x512 bilinear REPEAT pattern, "
+                        "SRC_OVER/MULTIPLY/PLUS/SCREEN",
+            "canvases": 1, "paths": 2000, "seed": seed}
+    if name == "c4":
+        scenes = S.glyph_canvases(seed, 4096, 200)
+        return scenes, {
+            "workload": "4096 canvases 32-256 px (half A8, half PRGB32), 1-200 quadratic glyph contours each, "
+                        "non-zero, solid / linear gradient, SRC_OVER, one batched frame",
+            "canvases": len(scenes), "paths": int(sum(sc.fill_count for sc in scenes)), "seed": seed}
+    if name == "srcover4k":
+        scenes = [S.srcover_layers_scene(seed + i, 3840, 2160, 1) for i in range(10)]
+        return scenes, {
+            "workload": "10 distinct 3840x2160 PRGB32 canvases, one full-canvas semi-transparent SRC_OVER "
+                        "layer each, one batched frame",
+            "canvases": 10, "paths": 10, "seed": seed}
+    raise SystemExit(f"unknown workload {name}")
+
+
+class _Frame:
+    """One recorded frame (single context, or a batch of contexts) on the GPU."""
+
+    def __init__(self, scenes):
+        import blend2d_amd as B
+        self.B = B
+        self.scenes = scenes
+        self.keep = []
+        self.targets = []
+        self.batch = B.BLBatch() if len(scenes) > 1 else None
+        self.errors = []
+        for sc in scenes:
+            images = []
+            for fmt, px in sc.images:
+                h, w = px.shape
+                im = B.BLImage(w, h, fmt)
+                im.upload(px)
+                images.append(im)
+            img = B.BLImage(sc.width, sc.height, sc.format)
+            img.fill_raw(sc.clear_value)
+            ctx = B.BLContext(img)
+            errs = B.replay_into(ctx, sc, images)
+            self.errors.append(errs[0] if errs else 0)
+            if self.batch is not None:
+                self.batch.add(ctx)
+            self.keep.append((ctx, images))
+            self.targets.append(img)
+        # first execution: upload + render onto the cleared canvases
+        if self.batch is not None:
+            self.batch.flush()
+        else:
+            self.keep[0][0].flush()
+
+    def replay(self):
+        if self.batch is not None:
+            self.batch.replay()
+        else:
+            self.keep[0][0].replay()
+
+    def stats(self):
+        return self.batch.last_stats() if self.batch is not None else self.keep[0][0].last_stats()
+
+    def download(self):
+        return [img.download() for img in self.targets]
+
+
+def _render_bytes(st, scenes):
+    a8 = all(sc.format == 3 for sc in scenes)
+    per_px = 2 if a8 else 8
+    if not a8 and any(sc.format == 3 for sc in scenes):
+        # mixed batch: weight by canvas area
+        area32 = sum(sc.width * sc.height for sc in scenes if sc.format != 3)
+        area8 = sum(sc.width * sc.height for sc in scenes if sc.format == 3)
+        per_px = (8 * area32 + 2 * area8) / float(area32 + area8)
+    return per_px * st["pixels"] + st["lut_bytes"] + st["image_bytes"] + 16 * st["binned"]
+
+
+def _cpu_baseline(scenes, first_gpu, pixels_per_frame, budget_s: float):
+    """Reference library on this host's cores (bounded sample), plus the
+    bit-exactness check of the GPU's first frame against its output."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import checkers
+    threads = max(1, min(HOST_CORES_MAX, os.cpu_count() or 1))
+    if checkers.have_ref():
+        kind = "reference"
+
+        def render(sc):
+            out, err, _ = checkers.render_reference(sc, threads=threads)
+            return out, err
+    else:
+        kind, threads = "port", 1
+
+        def render(sc):
+            return checkers.render_oracle(sc)
+
+    parity = {"against": kind, "max_diff": 0, "mean_diff": 0.0, "channels_differing": 0, "errors_match": True}
+    t0 = time.perf_counter()
+    frames = 0
+    done_check = False
+    while True:
+        for i, sc in enumerate(scenes):
+            out, err = render(sc)
+            if not done_check:
+                mx, mean, n = checkers.diff_info(first_gpu[i], out)
+                parity["max_diff"] = max(parity["max_diff"], int(mx))
+                parity["channels_differing"] += int(n)
+                parity["mean_diff"] = max(parity["mean_diff"], float(mean))
+        done_check = True
+        frames += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or frames >= 50:
+            break
+    mpix = pixels_per_frame * frames / el / 1e6
+    sample = (f"{frames} whole frame(s) of the same workload, {el:.1f} s, "
+              f"{'reference library (no JIT) with ' + str(threads) + ' worker thread(s)' if kind == 'reference' else 'scalar oracle port'}")
+    return {"value": round(mpix, 3), "unit": "Mpix/s", "cores": threads, "kind": kind, "sample": sample}, parity
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "srcover4k"])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a frame's tiles are rendered by one device)", file=sys.stderr)
+        sys.exit(2)
+
+    import blend2d_amd as B
+    if B.lib().bl_amd_device_count() < 1:
+        raise SystemExit("no HIP device visible")
+
+    scenes, config = _workload(args.workload, args.seed)
+    frame = _Frame(scenes)
+    first = frame.download()
+    st0 = frame.stats()
+    print(f"# workload {args.workload}: {st0}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        frame.replay()
+    _sync()
+    render_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        frame.replay()
+        render_ms.append(frame.stats()["render_ms"])
+    _sync()
+    el = time.perf_counter() - t0
+    st = frame.stats()
+
+    pixels = st["pixels"]
+    ms_per_step = el * 1e3 / args.steps
+    value = pixels * args.steps / el / 1e6
+    canvas_px = sum(sc.width * sc.height for sc in scenes)
+    avg_render_ms = float(np.mean(render_ms))
+    bytes_render = _render_bytes(st, scenes)
+    achieved = bytes_render / (avg_render_ms * 1e-3) / 1e9
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": config,
+        "paths_per_s": round(config["paths"] * args.steps / el, 1),
+        "canvas_mpix_per_s": round(canvas_px * args.steps / el / 1e6, 3),
+        "pixels_per_frame": int(pixels),
+        "render_ms": round(avg_render_ms, 4),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_render",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": int(bytes_render),
+        },
+        "cpu_baseline": None,
+    }
+    if any(frame.errors):
+        line["errors"] = frame.errors
+    if not args.no_cpu:
+        cpu, parity = _cpu_baseline(scenes, first, pixels, args.cpu_seconds)
+        line["cpu_baseline"] = cpu
+        line["parity"] = parity
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -38,9 +38,21 @@ SYMBOLS = [
     "bl_amd_context_set_fill_style_rgba32", "bl_amd_context_set_fill_style_gradient",
     "bl_amd_context_set_fill_style_pattern", "bl_amd_context_fill_path_d", "bl_amd_context_fill_polygon_d",
     "bl_amd_context_fill_rect_d", "bl_amd_context_fill_rect_i", "bl_amd_context_fill_all",
-    "bl_amd_context_last_stats",
+    "bl_amd_context_get_stats", "bl_amd_context_replay",
     "bl_amd_batch_create", "bl_amd_batch_destroy", "bl_amd_batch_add", "bl_amd_batch_clear", "bl_amd_batch_flush",
+    "bl_amd_batch_replay", "bl_amd_batch_get_stats",
 ]
+
+
+class BLAmdStats(ctypes.Structure):
+    """Mirror of BLAmdStats (include/blend2d_amd.h)."""
+    _fields_ = [("commands", ctypes.c_uint64), ("sources", ctypes.c_uint64), ("segments", ctypes.c_uint64),
+                ("binned", ctypes.c_uint64), ("tiles", ctypes.c_uint64), ("pixels", ctypes.c_uint64),
+                ("frame_bytes", ctypes.c_uint64), ("lut_bytes", ctypes.c_uint64), ("image_bytes", ctypes.c_uint64),
+                ("render_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class BLError(RuntimeError):
@@ -89,12 +101,15 @@ def lib():
         "bl_amd_context_fill_rect_d": ([P, D, D, D, D], U32),
         "bl_amd_context_fill_rect_i": ([P, I, I, I, I], U32),
         "bl_amd_context_fill_all": ([P], U32),
-        "bl_amd_context_last_stats": ([P, P, ctypes.POINTER(D)], U32),
+        "bl_amd_context_get_stats": ([P, ctypes.POINTER(BLAmdStats)], U32),
+        "bl_amd_context_replay": ([P], U32),
         "bl_amd_batch_create": ([ctypes.POINTER(P)], U32),
         "bl_amd_batch_destroy": ([P], U32),
         "bl_amd_batch_add": ([P, P], U32),
         "bl_amd_batch_clear": ([P], U32),
         "bl_amd_batch_flush": ([P], U32),
+        "bl_amd_batch_replay": ([P], U32),
+        "bl_amd_batch_get_stats": ([P, ctypes.POINTER(BLAmdStats)], U32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -299,14 +314,15 @@ class BLContext:
     def fill_all(self):
         _check(lib().bl_amd_context_fill_all(self._h), "fill_all")
 
-    def last_stats(self):
-        out = (ctypes.c_uint64 * 5)()
-        ms = ctypes.c_double()
-        _check(lib().bl_amd_context_last_stats(self._h, out, ctypes.byref(ms)), "last_stats")
-        keys = ("commands", "source_segments", "edge_segments", "binned_segments", "tiles")
-        d = dict(zip(keys, list(out)))
-        d["render_ms"] = ms.value
-        return d
+    def last_stats(self) -> dict:
+        """Counters of the last flush or replay (BLAmdStats)."""
+        st = BLAmdStats()
+        _check(lib().bl_amd_context_get_stats(self._h, ctypes.byref(st)), "get_stats")
+        return st.as_dict()
+
+    def replay(self):
+        """Executes the last flushed frame again from its device-resident copy."""
+        _check(lib().bl_amd_context_replay(self._h), "replay")
 
     def close(self):
         if self._h:
@@ -335,6 +351,14 @@ class BLBatch:
 
     def flush(self):
         _check(lib().bl_amd_batch_flush(self._h), "bl_amd_batch_flush")
+
+    def replay(self):
+        _check(lib().bl_amd_batch_replay(self._h), "bl_amd_batch_replay")
+
+    def last_stats(self) -> dict:
+        st = BLAmdStats()
+        _check(lib().bl_amd_batch_get_stats(self._h, ctypes.byref(st)), "bl_amd_batch_get_stats")
+        return st.as_dict()
 
     def close(self):
         if self._h:

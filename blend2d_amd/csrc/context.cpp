@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <memory>
 #include <new>
 #include <vector>
@@ -78,17 +79,26 @@ struct FrameHost {
 };
 
 struct Stats {
-  uint64_t cmds = 0, src = 0, segs = 0, binned = 0, tiles = 0;
-  double render_ms = 0.0;
+  uint64_t cmds = 0, src = 0, segs = 0, binned = 0, tiles = 0, pixels = 0;
+  uint64_t frame_bytes = 0, lut_bytes = 0, image_bytes = 0;
+  double render_ms = 0.0, total_ms = 0.0;
 };
 
-// Device-side state of the pipeline: input copies and scratch.
+// Device-side state of the pipeline: the resident copy of the last uploaded
+// frame and the scratch of its stages.  A frame stays resident until the next
+// upload, so it can be executed again (replayed) with no host-side work.
 struct Runner {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf in;
-  DevBuf src_count, src_off, bounds, segs, seg_cmd, mat_cnt, mat_off, mat_pres, list_off, list, binned, cursor, scan_tmp, readback;
+  DevBuf src_count, src_off, bounds, segs, seg_cmd, mat_cnt, mat_off, mat_pres, list_off, list, binned, cursor, scan_tmp, counters;
   uint32_t* host_readback = nullptr;
+
+  // Resident frame.
+  bool resident = false;
+  FrameDev f{};
+  uint64_t n_src = 0;
+  uint64_t frame_bytes = 0, lut_bytes = 0, image_bytes = 0;
 
   bool init() {
     if (stream) return true;
@@ -104,13 +114,13 @@ struct Runner {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  uint32_t read_u32(const uint32_t* dptr) {
-    if (hipMemcpyAsync(host_readback, dptr, 4, hipMemcpyDeviceToHost, stream) != hipSuccess) return 0xFFFFFFFFu;
-    if (hipStreamSynchronize(stream) != hipSuccess) return 0xFFFFFFFFu;
-    return host_readback[0];
+  uint32_t upload(FrameHost& fh);
+  uint32_t execute(Stats& stats);
+  uint32_t run(FrameHost& fh, Stats& stats) {
+    stats = Stats{};
+    uint32_t r = upload(fh);
+    return r ? r : execute(stats);
   }
-
-  uint32_t run(FrameHost& fh, Stats& stats);
 };
 
 template<typename T>
@@ -121,12 +131,13 @@ static size_t append_bytes(std::vector<uint8_t>& blob, const std::vector<T>& v) 
   return off;
 }
 
-uint32_t Runner::run(FrameHost& fh, Stats& stats) {
-  stats = Stats{};
+// Lays out tiles, bands and the (band, command) matrix of every canvas and
+// copies the frame's inputs to the device in one transfer.
+uint32_t Runner::upload(FrameHost& fh) {
+  resident = false;
   if (fh.cmds.empty()) return BL_AMD_SUCCESS;
   if (!init()) return kErrDevice;
 
-  // Canvas layout: tiles, bands, (band, command) matrix.
   uint64_t n_mat = 0;
   uint32_t n_tiles = 0, band_base = 0;
   for (DevCanvas& c : fh.canvases) {
@@ -140,9 +151,8 @@ uint32_t Runner::run(FrameHost& fh, Stats& stats) {
     n_tiles += c.tiles_x * c.bands;
     n_mat += (uint64_t)c.bands * c.n_cmds;
   }
-  if (n_mat >= 0xFFFFFFF0ull) return kErrOOM;
+  if (n_mat >= 0xFFFFFFF0ull || fh.src.size() >= 0xFFFFFFF0ull) return kErrOOM;
 
-  // Inputs: one packed upload.
   std::vector<uint8_t> blob;
   size_t o_canvas = append_bytes(blob, fh.canvases);
   size_t o_cmds = append_bytes(blob, fh.cmds);
@@ -154,9 +164,11 @@ uint32_t Runner::run(FrameHost& fh, Stats& stats) {
   size_t o_images = append_bytes(blob, fh.images);
   if (!in.reserve(blob.size() + 256)) return kErrOOM;
   HIP_OK(hipMemcpyAsync(in.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, stream));
+  // The blob is host pageable memory that goes out of scope here.
+  HIP_OK(hipStreamSynchronize(stream));
 
   const uint8_t* base = in.as<uint8_t>();
-  FrameDev f;
+  f = FrameDev{};
   f.canvases = (const DevCanvas*)(base + o_canvas); f.n_canvases = (uint32_t)fh.canvases.size();
   f.cmds = (const DevCmd*)(base + o_cmds); f.n_cmds = (uint32_t)fh.cmds.size();
   f.xforms = (const CmdXform*)(base + o_xf);
@@ -168,14 +180,29 @@ uint32_t Runner::run(FrameHost& fh, Stats& stats) {
   f.n_tiles = n_tiles;
   f.n_mat = n_mat;
   f.tol_sq = kToleranceSq;
+  n_src = fh.src.size();
+  frame_bytes = blob.size();
+  lut_bytes = fh.luts.size() * 4;
+  image_bytes = 0;
+  for (const DevImage& im : fh.images) image_bytes += (uint64_t)im.w * im.h * (im.format == kFmtA8 ? 1u : 4u);
+  resident = true;
+  return BL_AMD_SUCCESS;
+}
 
-  size_t n_src = fh.src.size();
+// Runs the three stages over the resident frame.  Two host round trips read
+// back the sizes the next stage allocates (segment count, binned count).
+uint32_t Runner::execute(Stats& stats) {
+  stats = Stats{};
+  if (!resident) return BL_AMD_SUCCESS;
+  auto t0 = std::chrono::steady_clock::now();
+  const uint64_t n_mat = f.n_mat;
+
   size_t scan_words = scan_tmp_words(std::max<size_t>(n_src, (size_t)n_mat) + 1);
   if (!src_count.reserve((n_src + 1) * 4) || !src_off.reserve((n_src + 1) * 4) ||
-      !bounds.reserve(fh.cmds.size() * sizeof(CmdBounds)) ||
+      !bounds.reserve((size_t)f.n_cmds * sizeof(CmdBounds)) ||
       !mat_cnt.reserve((n_mat + 1) * 4) || !mat_off.reserve((n_mat + 1) * 4) ||
       !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
-      !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4))
+      !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) || !counters.reserve(64))
     return kErrOOM;
 
   FrameScratch s;
@@ -188,13 +215,16 @@ uint32_t Runner::run(FrameHost& fh, Stats& stats) {
   s.mat_pres = mat_pres.as<uint32_t>();
   s.list_off = list_off.as<uint32_t>();
   s.scan_tmp = scan_tmp.as<uint32_t>();
+  s.pixels = counters.as<unsigned long long>();
 
   // Stage 1: edges (count, scan, emit).
+  launch_fill_u32(counters.as<uint32_t>(), 0u, 2, stream);
   launch_init_bounds(s.bounds, f.n_cmds, stream);
   launch_edges(f, s, true, stream);
   scan_exclusive(s.src_count, s.src_off, n_src, s.scan_tmp, stream);
-  uint32_t n_segs = read_u32(s.src_off + n_src);
-  if (n_segs == 0xFFFFFFFFu) return kErrDevice;
+  HIP_OK(hipMemcpyAsync(host_readback, s.src_off + n_src, 4, hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipStreamSynchronize(stream));
+  const uint32_t n_segs = host_readback[0];
   if (!segs.reserve((size_t)std::max<uint32_t>(n_segs, 1) * sizeof(DevSeg)) || !seg_cmd.reserve((size_t)std::max<uint32_t>(n_segs, 1) * 4))
     return kErrOOM;
   s.segs = segs.as<DevSeg>();
@@ -210,7 +240,7 @@ uint32_t Runner::run(FrameHost& fh, Stats& stats) {
   HIP_OK(hipMemcpyAsync(host_readback, s.mat_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipMemcpyAsync(host_readback + 1, s.list_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipStreamSynchronize(stream));
-  uint32_t n_binned = host_readback[0], n_list = host_readback[1];
+  const uint32_t n_binned = host_readback[0], n_list = host_readback[1];
   if (!binned.reserve((size_t)std::max<uint32_t>(n_binned, 1) * sizeof(DevSeg)) || !list.reserve((size_t)std::max<uint32_t>(n_list, 1) * 4))
     return kErrOOM;
   s.binned = binned.as<DevSeg>();
@@ -223,17 +253,26 @@ uint32_t Runner::run(FrameHost& fh, Stats& stats) {
   HIP_OK(hipEventRecord(ev0, stream));
   launch_render(f, s, stream);
   HIP_OK(hipEventRecord(ev1, stream));
+  HIP_OK(hipMemcpyAsync(host_readback + 2, s.pixels, 8, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipStreamSynchronize(stream));
   if (hipGetLastError() != hipSuccess) return kErrDevice;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, ev0, ev1);
+  auto t1 = std::chrono::steady_clock::now();
 
-  stats.cmds = fh.cmds.size();
+  uint64_t pixels;
+  memcpy(&pixels, host_readback + 2, 8);
+  stats.cmds = f.n_cmds;
   stats.src = n_src;
   stats.segs = n_segs;
   stats.binned = n_binned;
-  stats.tiles = n_tiles;
+  stats.tiles = f.n_tiles;
+  stats.pixels = pixels;
+  stats.frame_bytes = frame_bytes;
+  stats.lut_bytes = lut_bytes;
+  stats.image_bytes = image_bytes;
   stats.render_ms = ms;
+  stats.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   return BL_AMD_SUCCESS;
 }
 
@@ -855,12 +894,29 @@ BLAmdResult bl_amd_context_fill_all(BLAmdContext* ctx) {
   return fill_box_a(ctx, 0, 0, ctx->target->w, ctx->target->h);
 }
 
-BLAmdResult bl_amd_context_last_stats(const BLAmdContext* ctx, uint64_t out[5], double* render_ms) {
+static void copy_stats(const Stats& st, BLAmdStats* out) {
+  out->commands = st.cmds;
+  out->sources = st.src;
+  out->segments = st.segs;
+  out->binned = st.binned;
+  out->tiles = st.tiles;
+  out->pixels = st.pixels;
+  out->frame_bytes = st.frame_bytes;
+  out->lut_bytes = st.lut_bytes;
+  out->image_bytes = st.image_bytes;
+  out->render_ms = st.render_ms;
+  out->total_ms = st.total_ms;
+}
+
+BLAmdResult bl_amd_context_get_stats(const BLAmdContext* ctx, BLAmdStats* out) {
   if (!ctx || !out) return kErrInvalid;
-  out[0] = ctx->stats.cmds; out[1] = ctx->stats.src; out[2] = ctx->stats.segs;
-  out[3] = ctx->stats.binned; out[4] = ctx->stats.tiles;
-  if (render_ms) *render_ms = ctx->stats.render_ms;
+  copy_stats(ctx->stats, out);
   return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_replay(BLAmdContext* ctx) {
+  if (uint32_t r = check_target(ctx)) return r;
+  return ctx->runner.execute(ctx->stats);
 }
 
 BLAmdResult bl_amd_batch_create(BLAmdBatch** out) {
@@ -922,6 +978,17 @@ BLAmdResult bl_amd_batch_flush(BLAmdBatch* b) {
   uint32_t r = b->runner.run(all, b->stats);
   for (BLAmdContext* c : b->contexts) reset_frame(c);
   return r;
+}
+
+BLAmdResult bl_amd_batch_replay(BLAmdBatch* b) {
+  if (!b) return kErrInvalid;
+  return b->runner.execute(b->stats);
+}
+
+BLAmdResult bl_amd_batch_get_stats(const BLAmdBatch* b, BLAmdStats* out) {
+  if (!b || !out) return kErrInvalid;
+  copy_stats(b->stats, out);
+  return BL_AMD_SUCCESS;
 }
 
 }  // extern "C"

@@ -39,6 +39,7 @@ struct FrameScratch {
   uint32_t* list;          // n_list
   DevSeg* binned;          // n_binned
   uint32_t* scan_tmp;      // scan_tmp_words(max n)
+  unsigned long long* pixels;  // composited pixels (mask > 0), accumulated by render
 };
 
 size_t scan_tmp_words(size_t n);

@@ -411,6 +411,7 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
   const uint32_t lb = s.list_off[row_base];
   const uint32_t le = s.list_off[row_base + cv.n_cmds];
   bool touched = false;
+  uint32_t npix = 0;
 
   for (uint32_t li = lb; li < le; li++) {
     const uint32_t e = s.list[li];
@@ -501,8 +502,14 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
       if (!m[k] || px + k >= (int)cv.w) continue;
       uint32_t src = pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
       d[k] = a8 ? pipe::composite8(c.op, d[k], src >> 24, m[k]) : pipe::composite32(c.op, d[k], src, m[k]);
+      npix++;
     }
   }
+
+  // Composited-pixel statistics: one atomic per wave.
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) npix += __shfl_xor(npix, o, 64);
+  if ((t & 63) == 0 && npix) atomicAdd(s.pixels, (unsigned long long)npix);
 
   if (!touched || !row_ok) return;
   if (a8) {

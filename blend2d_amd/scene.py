@@ -158,10 +158,8 @@ class Scene:
 # ---------------------------------------------------------------------------
 
 def ellipse_path(cx: float, cy: float, rx: float, ry: float):
-    """Ellipse as four cubic arcs, the shape BLPath::add_ellipse produces.
-
-    Uses the standard circle-to-cubic constant k = 0.5522847498307936.
-    """
+    """Ellipse as a closed figure of four cubic arcs (one per quadrant, starting
+    at angle 0), with the circle-to-cubic constant k = 4/3 (sqrt(2) - 1)."""
     k = 0.55228474983079339840
     kx, ky = rx * k, ry * k
     cmds = [CMD_MOVE] + [CMD_CUBIC, CMD_CUBIC, CMD_ON] * 4 + [CMD_CLOSE]
@@ -329,3 +327,149 @@ def glyph_canvases(seed: int = 1, count: int = 4096, max_paths: int = 200):
             sc.fill_path(cm, v)
         scenes.append(sc)
     return scenes
+
+
+def srcover_layers_scene(seed: int = 1, width: int = 3840, height: int = 2160, layers: int = 1) -> Scene:
+    """A canvas cleared to seeded opaque colour with `layers` full-canvas
+    SRC_OVER fills of semi-transparent solid colours (fill_all): the pure
+    compositing load, 8 bytes of destination traffic per pixel and layer."""
+    rng = np.random.default_rng(seed)
+    clear = 0xFF000000 | int(rng.integers(0, 1 << 24))
+    sc = Scene(width, height, FMT_PRGB32, clear)
+    for _ in range(layers):
+        a = int(rng.integers(16, 240))
+        sc.set_fill_style_rgba32((a << 24) | int(rng.integers(0, 1 << 24)))
+        sc.fill_all()
+    return sc
+
+
+def mixed_scene(seed: int = 1, width: int = 513, height: int = 513, count: int = 120,
+                fmt: int = FMT_PRGB32, comp_ops=(COMP_OP_SRC_OVER, COMP_OP_SRC_COPY),
+                pattern_size: int = 64) -> Scene:
+    """Every call of the fill path in one seeded stream: integer and fractional
+    rects, polygons, cubic ellipses, quadratic contours and fill_all; solid,
+    linear / radial / conic gradients (PAD / REPEAT / REFLECT, optional
+    gradient transform) and nearest / bilinear affine patterns; both fill
+    rules, global alpha, scale / rotation / skew user transforms and the
+    given comp ops.  Pattern transforms always rotate, so the affine fetchers
+    are used."""
+    rng = np.random.default_rng(seed)
+    clear = 0xFF000000 | int(rng.integers(0, 1 << 24)) if fmt != FMT_A8 else int(rng.integers(0, 256))
+    sc = Scene(width, height, fmt, clear)
+    img = sc.add_image(premultiplied_noise(rng, pattern_size, pattern_size))
+    ident = [1.0, 0.0, 0.0, 1.0, 0.0, 0.0]
+    for i in range(count):
+        # state
+        sc.set_comp_op(int(comp_ops[int(rng.integers(0, len(comp_ops)))]))
+        sc.set_fill_rule(int(rng.integers(0, 2)))
+        sc.set_global_alpha(1.0 if rng.random() < 0.6 else float(rng.uniform(0.0, 1.0)))
+        t = rng.random()
+        cx, cy = float(rng.uniform(0, width)), float(rng.uniform(0, height))
+        if t < 0.45:
+            xf = ident
+        elif t < 0.6:
+            xf = [1.0, 0.0, 0.0, 1.0, float(rng.integers(-8, 9)), float(rng.integers(-8, 9))]
+        elif t < 0.75:
+            sx, sy = float(2.0 ** rng.uniform(-1, 1)), float(2.0 ** rng.uniform(-1, 1))
+            xf = [sx, 0.0, 0.0, sy, cx - sx * cx, cy - sy * cy]
+        elif t < 0.9:
+            xf = rotation_about(float(rng.uniform(0, 2 * math.pi)), cx, cy)
+        else:
+            k = float(rng.uniform(-0.5, 0.5))
+            xf = [1.0, 0.0, k, 1.0, -k * cy, 0.0]
+        sc.set_transform(xf)
+        # style
+        st = rng.random()
+        size = float(rng.uniform(8, max(width, height) * 0.6))
+        if st < 0.3:
+            sc.set_fill_style_rgba32(_rand_rgba(rng))
+        elif st < 0.75:
+            gtype = int(rng.integers(0, 3))
+            extend = int(rng.integers(0, 3))
+            stops = _rand_stops(rng, 1, 6)
+            gtr = None
+            if rng.random() < 0.3:
+                a = float(rng.uniform(0, 2 * math.pi))
+                gtr = rotation_about(a, cx, cy)
+            if gtype == GRADIENT_LINEAR:
+                p = rng.random(4) * size + np.array([cx, cy, cx, cy]) - size / 2
+                vals = [float(v) for v in p]
+            elif gtype == GRADIENT_RADIAL:
+                r = float(rng.uniform(4, size))
+                f = (rng.random(2) - 0.5) * r
+                vals = [cx, cy, cx + float(f[0]), cy + float(f[1]), r, 0.0]
+            else:
+                vals = [cx, cy, float(rng.uniform(0, 2 * math.pi)), float(rng.choice([1.0, 1.0, 2.0, 0.5]))]
+            sc.set_fill_style_gradient(gtype, vals, stops, extend, gtr)
+        else:
+            sc.set_pattern_quality(int(rng.integers(0, 2)))
+            a = float(rng.uniform(0.1, 2 * math.pi - 0.1))
+            s = float(2.0 ** rng.uniform(-1, 1))
+            c_, s_ = math.cos(a) * s, math.sin(a) * s
+            sc.set_fill_style_pattern(img, int(rng.integers(0, 3)), [c_, s_, -s_, c_, cx, cy])
+        # geometry
+        g = rng.random()
+        x0, y0 = float(rng.uniform(-0.2, 1.0) * width), float(rng.uniform(-0.2, 1.0) * height)
+        w, h = float(rng.uniform(1, size)), float(rng.uniform(1, size))
+        if g < 0.15:
+            sc.fill_rect_i(int(x0), int(y0), int(w), int(h))
+        elif g < 0.3:
+            q = 1.0 / 256.0
+            sc.fill_rect_d(round(x0 / q) * q if rng.random() < 0.5 else x0, y0, w, h)
+        elif g < 0.5:
+            n = int(rng.integers(3, 13))
+            sc.fill_polygon(rng.random((n, 2)) * np.array([w, h]) + np.array([x0, y0]))
+        elif g < 0.7:
+            cm, v = ellipse_path(x0 + w / 2, y0 + h / 2, w / 2, h / 2)
+            sc.fill_path(cm, v)
+        elif g < 0.98:
+            cm, v = glyph_path(rng, w, h)
+            sc.fill_path(cm, np.asarray(v) + np.array([x0, y0]))
+        else:
+            sc.set_global_alpha(float(rng.uniform(0.05, 0.3)))
+            sc.fill_all()
+    sc.set_transform(ident)
+    return sc
+
+
+def edge_case_scene(width: int = 257, height: int = 131, fmt: int = FMT_PRGB32) -> Scene:
+    """Degenerate and boundary inputs: empty and one-point paths, zero-area
+    and zero-size rects, geometry exactly on and far outside the canvas
+    edges, huge coordinates, horizontal-only polygons, self-intersecting
+    figures under both fill rules and unclosed figures."""
+    sc = Scene(width, height, fmt, 0xFF102030 if fmt != FMT_A8 else 0x10)
+    W, H = float(width), float(height)
+    sc.set_fill_style_rgba32(0xC0FF8040)
+    sc.fill_path([], np.zeros((0, 2)))
+    sc.fill_path([CMD_MOVE], [(5.0, 5.0)])
+    sc.fill_polygon([(1.0, 1.0), (9.0, 1.0)])
+    sc.fill_polygon([(0.0, 10.0), (W, 10.0), (W / 2, 10.0)])
+    sc.fill_rect_i(3, 3, 0, 10)
+    sc.fill_rect_i(-5, -5, 10, 10)
+    sc.fill_rect_i(width - 4, height - 4, 100, 100)
+    sc.fill_rect_d(10.25, 20.5, 0.0, 5.0)
+    sc.fill_rect_d(10.25, 20.5, 0.5, 0.25)
+    sc.fill_rect_d(-1e9, -1e9, 2e9, 12.75)
+    sc.fill_polygon([(0.0, 0.0), (W, 0.0), (W, 0.5), (0.0, 0.5)])
+    sc.fill_polygon([(W - 0.5, 0.0), (W, 0.0), (W, H), (W - 0.5, H)])
+    sc.fill_polygon([(-1e7, 30.0), (1e7, 40.0), (1e7, 60.0), (-1e7, 50.0)])
+    sc.fill_polygon([(W + 10, 10.0), (W + 50, 10.0), (W + 30, 90.0)])
+    sc.fill_polygon([(-50.0, 10.0), (-10.0, 10.0), (-30.0, 90.0)])
+    sc.fill_polygon([(20.0, H + 5), (60.0, H + 5), (40.0, H + 50)])
+    sc.set_fill_style_rgba32(0x8020C0F0)
+    for rule in (0, 1):
+        sc.set_fill_rule(rule)
+        ox = 60.0 + rule * 90.0
+        star = [(ox + 40 * math.cos(2 * math.pi * k * 2 / 5), 70 + 40 * math.sin(2 * math.pi * k * 2 / 5)) for k in range(5)]
+        sc.fill_polygon(star)
+        sc.fill_path([CMD_MOVE, CMD_ON, CMD_ON, CMD_MOVE, CMD_ON, CMD_ON],
+                     [(ox - 30, 100.0), (ox + 30, 100.0), (ox, 125.0), (ox - 20, 105.0), (ox + 20, 105.0), (ox, 120.0)])
+    sc.set_fill_style_rgba32(0xFFFFFFFF)
+    sc.fill_polygon([(0.0, 0.0), (1.0, 0.0), (1.0, 1.0)])
+    sc.fill_polygon([(W - 1.0, H - 1.0), (W, H - 1.0), (W, H)])
+    sc.set_fill_style_rgba32(0x30FFFFFF)
+    sc.fill_path([CMD_MOVE, CMD_CUBIC, CMD_CUBIC, CMD_ON],
+                 [(0.0, H), (W * 3, -H * 2), (-W * 2, -H * 2), (W, H)])
+    sc.fill_path([CMD_MOVE, CMD_QUAD, CMD_ON, CMD_QUAD, CMD_ON],
+                 [(W * 0.5, -H), (W * 4, H * 0.5), (W * 0.5, H * 2), (-W * 3, H * 0.5), (W * 0.5, -H)])
+    return sc

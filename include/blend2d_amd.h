@@ -114,9 +114,30 @@ BLAmdResult bl_amd_context_fill_rect_i(BLAmdContext* ctx, int x, int y, int w, i
 /* bl_context_fill_all (context.h:735). */
 BLAmdResult bl_amd_context_fill_all(BLAmdContext* ctx);
 
-/* Statistics of the last flush: commands, source segments, edge segments,
- * binned segments, tiles; device time of the render kernel in ms. */
-BLAmdResult bl_amd_context_last_stats(const BLAmdContext* ctx, uint64_t out[5], double* render_ms);
+/* ---- statistics and replay (no counterpart in the reference) ----------- */
+
+/* Counters of the last flush or replay. */
+typedef struct BLAmdStats {
+  uint64_t commands;   /* fill commands rendered */
+  uint64_t sources;    /* source segments (lines, quads, cubics) */
+  uint64_t segments;   /* 24.8 edge segments after clip + flatten */
+  uint64_t binned;     /* (segment, band) pairs */
+  uint64_t tiles;      /* 64x16 tiles in the render grid */
+  uint64_t pixels;     /* pixels composited with a non-zero mask */
+  uint64_t frame_bytes;  /* bytes of the device-resident frame (commands, geometry, styles, LUTs) */
+  uint64_t lut_bytes;    /* gradient LUT bytes of the frame */
+  uint64_t image_bytes;  /* pixel bytes of the pattern images the frame samples */
+  double render_ms;    /* device time of the render kernel */
+  double total_ms;     /* host wall time of the device stages */
+} BLAmdStats;
+
+BLAmdResult bl_amd_context_get_stats(const BLAmdContext* ctx, BLAmdStats* out);
+
+/* Executes the last flushed frame again from its device-resident copy
+ * (commands, geometry, styles), with no recording or upload: a retained
+ * display list.  The target and every pattern image the frame used must
+ * still be alive; the frame is replaced by the next flush that renders. */
+BLAmdResult bl_amd_context_replay(BLAmdContext* ctx);
 
 /* ---- batching (no counterpart in the reference) ------------------------
  * Renders the pending calls of many contexts (canvases of any size and
@@ -127,6 +148,9 @@ BLAmdResult bl_amd_batch_destroy(BLAmdBatch* batch);
 BLAmdResult bl_amd_batch_add(BLAmdBatch* batch, BLAmdContext* ctx);
 BLAmdResult bl_amd_batch_clear(BLAmdBatch* batch);
 BLAmdResult bl_amd_batch_flush(BLAmdBatch* batch);
+/* Same as bl_amd_context_replay for the batch's last merged frame. */
+BLAmdResult bl_amd_batch_replay(BLAmdBatch* batch);
+BLAmdResult bl_amd_batch_get_stats(const BLAmdBatch* batch, BLAmdStats* out);
 
 #ifdef __cplusplus
 }

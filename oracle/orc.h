@@ -36,6 +36,10 @@ uint32_t orc_div255(uint32_t x);
 void orc_segments_mask(const int32_t* segs, size_t n, uint32_t w, uint32_t h, uint32_t fill_rule, uint8_t* out);
 uint32_t orc_premultiply(uint32_t argb32);
 
+/* Rasterizer invariant of raster/analyticrasterizer_test.cpp:34-158: band-wise
+ * rasterize() and advance_to_y() reach the same state.  Returns mismatches. */
+size_t orc_raster_advance_check(const int32_t* edges, size_t n, uint32_t band_height, uint32_t w);
+
 #ifdef __cplusplus
 }
 #endif

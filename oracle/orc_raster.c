@@ -728,6 +728,54 @@ HorzLeftToRightInside:
   }
 }
 
+/* Test hook: the invariant of raster/analyticrasterizer_test.cpp:34-158.
+ * For each edge (x0, y0, x1, y1; 24.8, y0 < y1) one rasterizer is stepped
+ * band by band with rasterize() while a second one jumps to the same row
+ * with advance_to_y(); their states must agree after every band (yDlt is
+ * compared modulo 256 when dy >= dx).  Returns the number of mismatches. */
+static int ras_state_equal(const RasState* a, const RasState* b) {
+  int m = (a->dy >= a->dx) ? 255 : -1;
+  return a->dx == b->dx && a->dy == b->dy && a->ex0 == b->ex0 && a->ey0 == b->ey0 &&
+         a->ex1 == b->ex1 && a->ey1 == b->ey1 && a->fx0 == b->fx0 && a->fy0 == b->fy0 &&
+         a->fx1 == b->fx1 && a->fy1 == b->fy1 && a->xErr == b->xErr && a->yErr == b->yErr &&
+         a->xDlt == b->xDlt && (a->yDlt & m) == (b->yDlt & m) && a->xRem == b->xRem &&
+         a->yRem == b->yRem && a->xLift == b->xLift && a->yLift == b->yLift && a->savedFy1 == b->savedFy1;
+}
+
+size_t orc_raster_advance_check(const int32_t* edges, size_t n, uint32_t band_height, uint32_t w) {
+  const size_t stride = (size_t)w + 2;
+  uint32_t* cells = (uint32_t*)calloc(stride * band_height, sizeof(uint32_t));
+  size_t bad = 0;
+  if (!cells) return (size_t)-1;
+  for (size_t i = 0; i < n; i++) {
+    const int x0 = edges[4 * i + 0], y0 = edges[4 * i + 1], x1 = edges[4 * i + 2], y1 = edges[4 * i + 3];
+    Ras a, b;
+    memset(&a, 0, sizeof(a));
+    memset(&b, 0, sizeof(b));
+    a.cells = b.cells = cells;
+    a.cell_stride = b.cell_stride = stride;
+    a.band_height = b.band_height = band_height;
+    uint32_t band_y0 = (uint32_t)(y0 >> A8_SHIFT), band_y1 = band_y0 + band_height;
+    a.band_offset = b.band_offset = band_y0;
+    a.band_end = b.band_end = band_y0 + band_height - 1;
+    a.cell_min_x = b.cell_min_x = 0xFFFFFFFFu;
+    int pa = ras_prepare(&a, x0, y0, x1, y1), pb = ras_prepare(&b, x0, y0, x1, y1);
+    if (!pa || !pb || !ras_state_equal(&a.s, &b.s)) { bad++; continue; }
+    for (int guard = 0; guard < 1 << 20; guard++) {
+      if (ras_rasterize(&a, 1)) break;
+      ras_advance_to_y(&b, (int)band_y1);
+      if (!ras_state_equal(&a.s, &b.s)) { bad++; break; }
+      band_y0 = band_y1;
+      band_y1 += band_height;
+      uint32_t last = (uint32_t)((y1 - 1) >> A8_SHIFT);
+      a.band_offset = b.band_offset = band_y0;
+      a.band_end = b.band_end = (band_y1 - 1 < last) ? band_y1 - 1 : last;
+    }
+  }
+  free(cells);
+  return bad;
+}
+
 /* rastercontext.cpp:4154-4193 with thread_count == 0 */
 uint32_t orc_band_height(uint32_t width) {
   uint32_t bh = 64;

@@ -1,0 +1,182 @@
+"""Pins the CPU oracle's building blocks on known answers and on what the
+mathematics fixes, independently of any renderer:
+
+* udiv255 and premultiply: the reference's own KATs
+  (pixelops/scalar_test.cpp:19-31 and :97-117) plus the closed form
+  round(x / 255);
+* the analytic rasterizer: the reference's advance invariant
+  (raster/analyticrasterizer_test.cpp:34-158) for band heights 1..32;
+* comp ops: every executed op against its real-valued Porter-Duff /
+  separable-blend formula (error bound) and exact identities;
+* gradient LUT: end points, monotonicity and the linear ramp.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import checkers
+
+ORC = checkers.oracle()
+ORC.orc_raster_advance_check.restype = ctypes.c_size_t
+ORC.orc_raster_advance_check.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32]
+
+SRC_OVER, SRC_COPY, PLUS, MULTIPLY, SCREEN = 0, 1, 12, 15, 16
+PRGB32, XRGB32, A8 = 1, 2, 3
+
+
+def test_udiv255_matches_reference_identity_and_rounding():
+    for i in range(255 * 255):
+        j = i + 128
+        want = (j + (j >> 8)) >> 8
+        got = ORC.orc_div255(i)
+        assert got == want, i
+        # x / 255 is never exactly halfway between integers, so this is round-to-nearest
+        assert got == (2 * i + 255) // 510, i
+
+
+def test_premultiply_kat():
+    # the reference walks c += 7919 over 10M colours; the first 200k here
+    c = 0
+    for _ in range(200000):
+        a, r, g, b = c >> 24 & 255, c >> 16 & 255, c >> 8 & 255, c & 255
+
+        def d(x):
+            x += 128
+            return (x + (x >> 8)) >> 8
+        want = (a << 24) | (d(r * a) << 16) | (d(g * a) << 8) | d(b * a)
+        assert ORC.orc_premultiply(c) == want, hex(c)
+        c = (c + 7919) & 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("band_height", [1, 2, 4, 8, 16, 32])
+def test_rasterizer_advance_invariant(band_height):
+    w = h = 1000
+    rng = np.random.default_rng(0x1234 + band_height)
+    n = 5000
+    e = (rng.random((n, 4)) * np.array([w, h, w, h]) * 256).astype(np.int32)
+    e[0, 2] = e[0, 0]                       # one vertical line
+    swap = e[:, 1] > e[:, 3]
+    e[swap] = e[swap][:, [2, 3, 0, 1]]
+    e[:, 3] = np.minimum(e[:, 3] + (band_height + 1) * 256, h * 256)
+    e = np.ascontiguousarray(e)
+    bad = ORC.orc_raster_advance_check(e.ctypes.data, n, band_height, w)
+    assert bad == 0
+
+
+def _unpack(p):
+    return np.array([(p >> 24) & 255, (p >> 16) & 255, (p >> 8) & 255, p & 255], dtype=np.float64)
+
+
+def _rand_prgb(rng):
+    a = int(rng.integers(0, 256))
+    c = [int(rng.integers(0, a + 1)) for _ in range(3)]
+    return (a << 24) | (c[0] << 16) | (c[1] << 8) | c[2]
+
+
+def _real(op, d, s, m):
+    """Real-valued result of `op` on normalized premultiplied colours
+    (channel order A, R, G, B); mask m scales the source (Porter-Duff with
+    coverage) except SRC_COPY, which interpolates."""
+    d = d / 255.0
+    s = s / 255.0
+    k = m / 255.0
+    if op == SRC_COPY:
+        r = d + (s - d) * k
+    else:
+        s = s * k
+        sa, da = s[0], d[0]
+        if op == SRC_OVER:
+            r = s + d * (1 - sa)
+        elif op == PLUS:
+            r = np.minimum(1.0, s + d)
+        elif op == MULTIPLY:
+            r = s * d + s * (1 - da) + d * (1 - sa)
+        elif op == SCREEN:
+            r = s + d - s * d
+        else:
+            raise ValueError(op)
+    return r * 255.0
+
+
+@pytest.mark.parametrize("op", [SRC_OVER, SRC_COPY, PLUS, MULTIPLY, SCREEN])
+def test_composite_prgb32_within_one_lsb_of_real_formula(op):
+    rng = np.random.default_rng(op)
+    worst = 0.0
+    for _ in range(20000):
+        d, s = _rand_prgb(rng), _rand_prgb(rng)
+        m = int(rng.integers(1, 256))
+        got = _unpack(ORC.orc_composite_pixel(op, PRGB32, d, s, m))
+        want = _real(op, _unpack(d), _unpack(s), m)
+        worst = max(worst, float(np.max(np.abs(got - want))))
+    # integer div255 rounding at each of up to two products: < 1.5 LSB
+    assert worst < 1.5, worst
+
+
+@pytest.mark.parametrize("op", [SRC_OVER, PLUS, MULTIPLY, SCREEN])
+def test_composite_transparent_source_is_identity(op):
+    rng = np.random.default_rng(100 + op)
+    for _ in range(2000):
+        d = _rand_prgb(rng)
+        m = int(rng.integers(1, 256))
+        assert ORC.orc_composite_pixel(op, PRGB32, d, 0, m) == d
+
+
+def test_composite_opaque_identities():
+    rng = np.random.default_rng(7)
+    for _ in range(2000):
+        d = _rand_prgb(rng)
+        s = 0xFF000000 | int(rng.integers(0, 1 << 24))
+        assert ORC.orc_composite_pixel(SRC_OVER, PRGB32, d, s, 255) == s
+        assert ORC.orc_composite_pixel(SRC_COPY, PRGB32, d, s, 255) == s
+        d1 = d | 0xFF000000
+        # white is the unit of MULTIPLY over an opaque destination
+        assert ORC.orc_composite_pixel(MULTIPLY, PRGB32, d1, 0xFFFFFFFF, 255) == d1
+        assert ORC.orc_composite_pixel(SCREEN, PRGB32, d, 0xFFFFFFFF, 255) == 0xFFFFFFFF
+        assert ORC.orc_composite_pixel(PLUS, PRGB32, 0xFFFFFFFF, d, 255) == 0xFFFFFFFF
+        # opaque operands: MULTIPLY and SCREEN are symmetric
+        assert ORC.orc_composite_pixel(MULTIPLY, PRGB32, d1, s, 255) == ORC.orc_composite_pixel(MULTIPLY, PRGB32, s, d1, 255)
+        assert ORC.orc_composite_pixel(SCREEN, PRGB32, d1, s, 255) == ORC.orc_composite_pixel(SCREEN, PRGB32, s, d1, 255)
+
+
+def test_composite_a8_matches_alpha_channel_of_prgb32():
+    rng = np.random.default_rng(11)
+    for op in (SRC_OVER, SRC_COPY, PLUS):
+        for _ in range(3000):
+            da, sa = int(rng.integers(0, 256)), int(rng.integers(0, 256))
+            m = int(rng.integers(1, 256))
+            got = ORC.orc_composite_pixel(op, A8, da, sa << 24, m)
+            full = ORC.orc_composite_pixel(op, PRGB32, da << 24, sa << 24, m)
+            assert got == full >> 24, (op, da, sa, m)
+
+
+def _lut(offsets, colors):
+    off = np.ascontiguousarray(offsets, dtype=np.float64)
+    col = np.ascontiguousarray(colors, dtype=np.uint32)
+    out = np.zeros(1024, dtype=np.uint32)
+    size = ctypes.c_uint32()
+    ORC.orc_gradient_lut(off.ctypes.data, col.ctypes.data, len(off), out.ctypes.data, ctypes.byref(size))
+    return out[:size.value]
+
+
+def test_gradient_lut_black_to_white_ramp():
+    lut = _lut([0.0, 1.0], [0xFF000000, 0xFFFFFFFF])
+    n = len(lut)
+    assert n in (256, 512, 1024)
+    assert lut[0] == 0xFF000000 and lut[-1] == 0xFFFFFFFF
+    ch = (lut >> 8) & 255
+    assert np.all(np.diff(ch.astype(np.int64)) >= 0)
+    ideal = np.arange(n) * 255.0 / (n - 1)
+    assert np.max(np.abs(ch - ideal)) <= 1.0
+    assert np.all((lut >> 24) == 255)
+
+
+def test_gradient_lut_is_premultiplied_and_hits_stop_colours():
+    lut = _lut([0.0, 0.5, 1.0], [0x80FF0000, 0xFF00FF00, 0x000000FF])
+    a = (lut >> 24) & 255
+    for sh in (16, 8, 0):
+        assert np.all(((lut >> sh) & 255) <= a)
+    assert lut[0] == 0x80800000            # 0x80FF0000 premultiplied
+    assert lut[-1] == 0
+    mid = len(lut) // 2
+    assert any(v == 0xFF00FF00 for v in lut[mid - 2: mid + 2])

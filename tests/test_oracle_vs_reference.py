@@ -1,0 +1,44 @@
+"""The CPU oracle against the reference library itself (compiled from the
+reference sources into oracle/_ref by oracle/ref.mk) on seeds beyond the
+golden fixtures.  Skipped where that build is absent."""
+import pytest
+
+import checkers
+from blend2d_amd import scene as S
+
+pytestmark = [pytest.mark.ref,
+              pytest.mark.skipif(not checkers.have_ref(), reason="reference library not built (oracle/_ref)")]
+
+
+def _same(sc, threads=0):
+    want, rerr, _ = checkers.render_reference(sc, threads=threads)
+    got, oerr = checkers.render_oracle(sc)
+    assert oerr == rerr
+    mx, mean, n = checkers.diff_info(got, want)
+    assert mx == 0, f"max {mx} mean {mean} channels {n}"
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13])
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_XRGB32, S.FMT_A8])
+def test_mixed(seed, fmt):
+    _same(S.mixed_scene(seed, 301, 257, 200, fmt))
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_cubic_gradients(seed):
+    _same(S.cubic_gradient_scene(seed, 800, 450, 400))
+
+
+def test_patterns_src_over_copy():
+    _same(S.pattern_shapes_scene(31, 640, 480, 60, comp_ops=(S.COMP_OP_SRC_OVER, S.COMP_OP_SRC_COPY), pattern_size=100))
+
+
+def test_wide_canvas_band_height():
+    # 4096 px wide: the reference band height drops to 16 rows, which moves
+    # the conic fetcher's row origin
+    _same(S.mixed_scene(41, 4100, 96, 80, S.FMT_PRGB32))
+
+
+def test_worker_threads_do_not_change_pixels():
+    # reference with worker threads (its own mt test: tolerance 0) vs oracle
+    _same(S.mixed_scene(51, 513, 513, 150, S.FMT_PRGB32), threads=4)
