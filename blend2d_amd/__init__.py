@@ -39,6 +39,10 @@ SYMBOLS = [
     "bl_amd_context_set_fill_style_pattern", "bl_amd_context_fill_path_d", "bl_amd_context_fill_polygon_d",
     "bl_amd_context_fill_rect_d", "bl_amd_context_fill_rect_i", "bl_amd_context_fill_all",
     "bl_amd_context_get_stats", "bl_amd_context_replay",
+    "bl_amd_context_fill_all_rgba32", "bl_amd_context_fill_rect_i_rgba32", "bl_amd_context_fill_rect_d_rgba32",
+    "bl_amd_context_fill_path_d_rgba32", "bl_amd_context_fill_geometry", "bl_amd_context_fill_geometry_rgba32",
+    "bl_amd_context_clear_all", "bl_amd_context_clear_rect_i", "bl_amd_context_clear_rect_d",
+    "bl_amd_context_set_fill_style_rgba64", "bl_amd_context_set_fill_style_rgba", "bl_amd_context_set_fill_alpha",
     "bl_amd_batch_create", "bl_amd_batch_destroy", "bl_amd_batch_add", "bl_amd_batch_clear", "bl_amd_batch_flush",
     "bl_amd_batch_replay", "bl_amd_batch_get_stats",
 ]
@@ -53,6 +57,45 @@ class BLAmdStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BLRgba64(int):
+    """A 64-bit colour (BLRgba64, core/rgba.h): 16 bits per channel, ARGB order."""
+
+
+class BLRgba(tuple):
+    """A float colour (BLRgba, core/rgba.h): (r, g, b, a) in [0, 1]."""
+
+    def __new__(cls, r, g, b, a=1.0):
+        return super().__new__(cls, (float(r), float(g), float(b), float(a)))
+
+
+# BLGeometryType values (core/geometry.h) and the struct / element layouts
+# fill_geometry packs (same memory layout as the reference's structs).
+from .scene import (GEOMETRY_ARRAY as _GEOMETRY_ARRAY, GEOMETRY_STRUCT as _GEOMETRY_STRUCT,  # noqa: E402
+                    GEOMETRY_NONE, GEOMETRY_BOXI, GEOMETRY_BOXD, GEOMETRY_RECTI, GEOMETRY_RECTD,
+                    GEOMETRY_CIRCLE, GEOMETRY_ELLIPSE, GEOMETRY_ROUND_RECT, GEOMETRY_ARC, GEOMETRY_CHORD,
+                    GEOMETRY_PIE, GEOMETRY_LINE, GEOMETRY_TRIANGLE, GEOMETRY_POLYLINEI, GEOMETRY_POLYLINED,
+                    GEOMETRY_POLYGONI, GEOMETRY_POLYGOND, GEOMETRY_ARRAY_VIEW_BOXI, GEOMETRY_ARRAY_VIEW_BOXD,
+                    GEOMETRY_ARRAY_VIEW_RECTI, GEOMETRY_ARRAY_VIEW_RECTD)
+
+
+class _ArrayView(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("size", ctypes.c_size_t)]
+
+
+def _geometry_arg(gtype: int, data):
+    """ctypes buffer (kept alive by the caller) holding the geometry struct."""
+    if gtype in _GEOMETRY_ARRAY:
+        items = list(data)
+        body = b"".join(struct.pack(_GEOMETRY_ARRAY[gtype], *it) for it in items)
+        buf = ctypes.create_string_buffer(body, max(len(body), 1))
+        view = _ArrayView(ctypes.cast(buf, ctypes.c_void_p), len(items))
+        return view, (buf, view)
+    if gtype in _GEOMETRY_STRUCT:
+        buf = ctypes.create_string_buffer(struct.pack(_GEOMETRY_STRUCT[gtype], *data))
+        return buf, (buf,)
+    return None, ()
 
 
 class BLError(RuntimeError):
@@ -103,6 +146,18 @@ def lib():
         "bl_amd_context_fill_all": ([P], U32),
         "bl_amd_context_get_stats": ([P, ctypes.POINTER(BLAmdStats)], U32),
         "bl_amd_context_replay": ([P], U32),
+        "bl_amd_context_fill_all_rgba32": ([P, U32], U32),
+        "bl_amd_context_fill_rect_i_rgba32": ([P, I, I, I, I, U32], U32),
+        "bl_amd_context_fill_rect_d_rgba32": ([P, D, D, D, D, U32], U32),
+        "bl_amd_context_fill_path_d_rgba32": ([P, P, P, P, SZ, U32], U32),
+        "bl_amd_context_fill_geometry": ([P, U32, P], U32),
+        "bl_amd_context_fill_geometry_rgba32": ([P, U32, P, U32], U32),
+        "bl_amd_context_clear_all": ([P], U32),
+        "bl_amd_context_clear_rect_i": ([P, I, I, I, I], U32),
+        "bl_amd_context_clear_rect_d": ([P, D, D, D, D], U32),
+        "bl_amd_context_set_fill_style_rgba64": ([P, ctypes.c_uint64], U32),
+        "bl_amd_context_set_fill_style_rgba": ([P, P], U32),
+        "bl_amd_context_set_fill_alpha": ([P, D], U32),
         "bl_amd_batch_create": ([ctypes.POINTER(P)], U32),
         "bl_amd_batch_destroy": ([P], U32),
         "bl_amd_batch_add": ([P, P], U32),
@@ -273,6 +328,11 @@ class BLContext:
 
     def set_fill_style(self, style):
         L = lib()
+        if isinstance(style, BLRgba64):
+            return _check(L.bl_amd_context_set_fill_style_rgba64(self._h, int(style) & 0xFFFFFFFFFFFFFFFF), "set_fill_style")
+        if isinstance(style, BLRgba):
+            arr = (ctypes.c_float * 4)(*style)
+            return _check(L.bl_amd_context_set_fill_style_rgba(self._h, arr), "set_fill_style")
         if isinstance(style, int):
             return _check(L.bl_amd_context_set_fill_style_rgba32(self._h, style & 0xFFFFFFFF), "set_fill_style")
         if isinstance(style, BLGradient):
@@ -290,7 +350,29 @@ class BLContext:
                 self._h, style.image.handle, style.extend, _dptr(tr) if tr is not None else None), "set_fill_style")
         raise TypeError("unsupported style")
 
-    def fill_path(self, path, origin=None):
+    def set_fill_alpha(self, a: float):
+        _check(lib().bl_amd_context_set_fill_alpha(self._h, a), "set_fill_alpha")
+
+    def clear_all(self):
+        _check(lib().bl_amd_context_clear_all(self._h), "clear_all")
+
+    def clear_rect(self, x, y, w, h):
+        if all(isinstance(v, (int, np.integer)) for v in (x, y, w, h)):
+            _check(lib().bl_amd_context_clear_rect_i(self._h, int(x), int(y), int(w), int(h)), "clear_rect")
+        else:
+            _check(lib().bl_amd_context_clear_rect_d(self._h, float(x), float(y), float(w), float(h)), "clear_rect")
+
+    def fill_geometry(self, gtype: int, data=(), rgba32: int | None = None):
+        arg, keep = _geometry_arg(gtype, data)
+        ptr = ctypes.byref(arg) if arg is not None else None
+        if rgba32 is None:
+            r = lib().bl_amd_context_fill_geometry(self._h, gtype, ptr)
+        else:
+            r = lib().bl_amd_context_fill_geometry_rgba32(self._h, gtype, ptr, rgba32 & 0xFFFFFFFF)
+        del keep
+        _check(r, "fill_geometry")
+
+    def fill_path(self, path, origin=None, rgba32: int | None = None):
         if isinstance(path, BLPath):
             cmds, vtx = path.arrays()
         else:
@@ -298,21 +380,33 @@ class BLContext:
             cmds = np.ascontiguousarray(cmds, dtype=np.uint8)
             vtx = np.ascontiguousarray(vtx, dtype=np.float64).reshape(-1, 2)
         org = np.ascontiguousarray(origin, dtype=np.float64) if origin is not None else None
-        _check(lib().bl_amd_context_fill_path_d(self._h, _dptr(org) if org is not None else None,
-                                                _dptr(cmds), _dptr(vtx), len(cmds)), "fill_path")
+        if rgba32 is None:
+            r = lib().bl_amd_context_fill_path_d(self._h, _dptr(org) if org is not None else None,
+                                                 _dptr(cmds), _dptr(vtx), len(cmds))
+        else:
+            r = lib().bl_amd_context_fill_path_d_rgba32(self._h, _dptr(org) if org is not None else None,
+                                                        _dptr(cmds), _dptr(vtx), len(cmds), rgba32 & 0xFFFFFFFF)
+        _check(r, "fill_path")
 
     def fill_polygon(self, vtx):
         vtx = np.ascontiguousarray(vtx, dtype=np.float64).reshape(-1, 2)
         _check(lib().bl_amd_context_fill_polygon_d(self._h, _dptr(vtx), len(vtx)), "fill_polygon")
 
-    def fill_rect(self, x, y, w, h):
+    def fill_rect(self, x, y, w, h, rgba32: int | None = None):
+        L = lib()
         if all(isinstance(v, (int, np.integer)) for v in (x, y, w, h)):
-            _check(lib().bl_amd_context_fill_rect_i(self._h, int(x), int(y), int(w), int(h)), "fill_rect")
+            a = (self._h, int(x), int(y), int(w), int(h))
+            r = L.bl_amd_context_fill_rect_i(*a) if rgba32 is None else L.bl_amd_context_fill_rect_i_rgba32(*a, rgba32 & 0xFFFFFFFF)
         else:
-            _check(lib().bl_amd_context_fill_rect_d(self._h, float(x), float(y), float(w), float(h)), "fill_rect")
+            a = (self._h, float(x), float(y), float(w), float(h))
+            r = L.bl_amd_context_fill_rect_d(*a) if rgba32 is None else L.bl_amd_context_fill_rect_d_rgba32(*a, rgba32 & 0xFFFFFFFF)
+        _check(r, "fill_rect")
 
-    def fill_all(self):
-        _check(lib().bl_amd_context_fill_all(self._h), "fill_all")
+    def fill_all(self, rgba32: int | None = None):
+        if rgba32 is None:
+            _check(lib().bl_amd_context_fill_all(self._h), "fill_all")
+        else:
+            _check(lib().bl_amd_context_fill_all_rgba32(self._h, rgba32 & 0xFFFFFFFF), "fill_all")
 
     def last_stats(self) -> dict:
         """Counters of the last flush or replay (BLAmdStats)."""

@@ -315,6 +315,7 @@ struct BLAmdContext {
   uint32_t pattern_quality = 1;
   Mat user{1, 0, 0, 1, 0, 0};
   double global_alpha = 1.0;
+  double fill_alpha = 1.0;
   StyleState style;
   uint64_t style_counter = 0;
   // style version -> index in frame.styles (single cached entry)
@@ -332,6 +333,7 @@ struct BLAmdContext {
     pattern_quality = 1;
     user = Mat{1, 0, 0, 1, 0, 0};
     global_alpha = 1.0;
+    fill_alpha = 1.0;
     style = StyleState{};
     style.dev.kind = kFetchSolid;
     style.dev.solid = premultiply(0xFF000000u);
@@ -348,7 +350,22 @@ struct BLAmdBatch {
 
 namespace {
 
-uint32_t alpha_i(const BLAmdContext* c) { return (uint32_t)round_to_int(c->global_alpha * 255.0 * 1.0); }
+// styleAlphaI of the fill slot: round(global_alpha * 255 * fill_alpha)
+// (rastercontext.cpp:1920-1939, :2000-2029).
+uint32_t alpha_i(const BLAmdContext* c) { return (uint32_t)round_to_int(c->global_alpha * 255.0 * c->fill_alpha); }
+
+// FormatExt of a solid colour (formatFromRgba32, rastercontext.cpp:90-93).
+uint32_t solid_format(uint32_t rgba32) { return rgba32 == 0 ? kFxZERO32 : (rgba32 >= 0xFF000000u ? kFxFRGB32 : kFxPRGB32); }
+
+StyleState make_solid_style(BLAmdContext* c, uint32_t rgba32) {
+  StyleState st;
+  st.type = kStyleSolid;
+  st.fmt = solid_format(rgba32);
+  st.dev.kind = kFetchSolid;
+  st.dev.solid = premultiply(rgba32);
+  st.version = ++c->style_counter;
+  return st;
+}
 
 uint32_t image_index(BLAmdContext* c, const BLAmdImage* img) {
   for (size_t i = 0; i < c->frame_images.size(); i++) if (c->frame_images[i] == img) return (uint32_t)i;
@@ -363,32 +380,47 @@ uint32_t image_index(BLAmdContext* c, const BLAmdImage* img) {
   return (uint32_t)(c->frame_images.size() - 1);
 }
 
-// Resolves the current style and comp op into a command template
-// (rastercontext.cpp:1231-1342 + compopsimplifyimpl_p.h).  Returns 0 when the
-// call renders, 1 for a NOP, otherwise an error.
-uint32_t resolve(BLAmdContext* c, DevCmd& cmd) {
-  memset(&cmd, 0, sizeof(cmd));
-  if (alpha_i(c) == 0) return 1;
-  StyleState& st = c->style;
-  if (st.status) return st.status;
+// A resolved render call: the command template and the error the call
+// reports if its geometry turns out to be non-empty (the reference looks the
+// pipeline up only then, rastercontext.cpp:2734-2749).
+struct Resolved {
+  DevCmd cmd;
+  uint32_t nop = 0;       // 1: the call renders nothing
+  uint32_t status = 0;    // deferred error
+};
 
+uint32_t exec_op(uint32_t comp) {
+  switch (comp) {
+    case kCompSrcCopy: return kOpSrcCopy;
+    case kCompSrcOver: return kOpSrcOver;
+    case kCompPlus: return kOpPlus;
+    case kCompMultiply: return kOpMultiply;
+    case kCompScreen: return kOpScreen;
+    default: return 0xFFFFFFFFu;
+  }
+}
+
+// Comp op + style -> command (rastercontext.cpp:1231-1342 and
+// compopsimplifyimpl_p.h).  `nop_transform` marks calls that a non-invertible
+// user transform turns into NOPs (geometry fills, not fill_all).
+Resolved resolve_with(BLAmdContext* c, StyleState& st, uint32_t alpha, bool nop_transform) {
+  Resolved r;
+  memset(&r.cmd, 0, sizeof(r.cmd));
+  if (alpha == 0 || st.status == 1 || (nop_transform && transform_type(c->user) == kTTInvalid)) { r.nop = 1; return r; }
   const uint32_t dst_fmt = c->target->format;
   Simplified s = simplify(c->comp_op, dst_fmt, st.fmt);
-  if (s.op == kCompNop) return 1;
-  uint32_t op;
-  switch (s.op) {
-    case kCompSrcCopy: op = kOpSrcCopy; break;
-    case kCompSrcOver: op = kOpSrcOver; break;
-    case kCompPlus: op = kOpPlus; break;
-    case kCompMultiply: op = kOpMultiply; break;
-    case kCompScreen: op = kOpScreen; break;
-    default: return kErrNotImpl;
+  if (s.op == kCompNop) { r.nop = 1; return r; }
+  if (st.status) { r.status = st.status; return r; }
+  uint32_t op = exec_op(s.op);
+  if (op == 0xFFFFFFFFu ||
+      ((op == kOpMultiply || op == kOpScreen) && s.dst != kFxPRGB32) ||
+      (s.src == kFxA8 && s.dst != kFxA8 && op != kOpSrcCopy && op != kOpSrcOver && op != kOpPlus)) {
+    r.status = kErrNotImpl;
+    return r;
   }
-  if ((op == kOpMultiply || op == kOpScreen) && s.dst != kFxPRGB32) return kErrNotImpl;
-  if (s.src == kFxA8 && s.dst != kFxA8 && op != kOpSrcCopy && op != kOpSrcOver && op != kOpPlus) return kErrNotImpl;
-
+  DevCmd& cmd = r.cmd;
   cmd.op = op;
-  cmd.alpha = alpha_i(c);
+  cmd.alpha = alpha;
   cmd.src_fmt = s.src;
   cmd.canvas = 0;
 
@@ -398,9 +430,8 @@ uint32_t resolve(BLAmdContext* c, DevCmd& cmd) {
     d.solid = s.solid == kSolidTransparent ? 0u : s.solid == kSolidOpaqueBlack ? 0xFF000000u : 0xFFFFFFFFu;
     cmd.style = (uint32_t)c->frame.styles.size();
     c->frame.styles.push_back(d);
-    return 0;
+    return r;
   }
-
   if (c->frame_style_version != st.version) {
     DevStyle d = st.dev;
     if (st.type == kStyleGradient && d.kind != kFetchSolid) {
@@ -413,7 +444,38 @@ uint32_t resolve(BLAmdContext* c, DevCmd& cmd) {
     c->frame_style_version = st.version;
   }
   cmd.style = c->frame_style_index;
-  return 0;
+  return r;
+}
+
+Resolved resolve_fill(BLAmdContext* c, bool nop_transform = true) { return resolve_with(c, c->style, alpha_i(c), nop_transform); }
+
+// Explicit solid colour (BL_CONTEXT_RESOLVE_EXPLICIT_SOLID_OP,
+// rastercontext.cpp:1258-1275): the context's style is not touched.
+Resolved resolve_solid(BLAmdContext* c, uint32_t rgba32, bool nop_transform = true) {
+  StyleState st = make_solid_style(c, rgba32);
+  return resolve_with(c, st, alpha_i(c), nop_transform);
+}
+
+// clear_* (resolve_clear_op, rastercontext.cpp:1232-1242): SRC_COPY of the
+// solid-override colour (transparent unless the simplification picks one)
+// at full alpha; global alpha, fill alpha and style do not apply.
+Resolved resolve_clear(BLAmdContext* c, bool nop_transform) {
+  Resolved r;
+  memset(&r.cmd, 0, sizeof(r.cmd));
+  if (nop_transform && transform_type(c->user) == kTTInvalid) { r.nop = 1; return r; }
+  Simplified s = simplify(kCompSrcCopy, c->target->format, kFxPRGB32);
+  if (s.op == kCompNop) { r.nop = 1; return r; }
+  uint32_t op = exec_op(s.op);
+  if (op == 0xFFFFFFFFu) { r.status = kErrNotImpl; return r; }
+  DevStyle d{};
+  d.kind = kFetchSolid;
+  d.solid = s.solid == kSolidOpaqueBlack ? 0xFF000000u : s.solid == kSolidOpaqueWhite ? 0xFFFFFFFFu : 0u;
+  r.cmd.op = op;
+  r.cmd.alpha = 255;
+  r.cmd.src_fmt = s.src;
+  r.cmd.style = (uint32_t)c->frame.styles.size();
+  c->frame.styles.push_back(d);
+  return r;
 }
 
 CmdXform fixed_xform(const BLAmdContext* c) {
@@ -426,7 +488,7 @@ CmdXform fixed_xform(const BLAmdContext* c) {
   return x;
 }
 
-uint32_t push_cmd(BLAmdContext* c, DevCmd& cmd, const CmdXform* xf) {
+uint32_t push_cmd(BLAmdContext* c, const DevCmd& cmd, const CmdXform* xf) {
   c->frame.cmds.push_back(cmd);
   if (xf) c->frame.xforms.push_back(*xf);
   else c->frame.xforms.push_back(CmdXform{});
@@ -478,26 +540,57 @@ void record_poly_segments(BLAmdContext* c, uint32_t cmd_i, const double* vtx, si
   c->frame.src.push_back(SrcSeg{cmd_i, kSrcLine, vbase + (uint32_t)n - 1, vbase});
 }
 
-uint32_t fill_poly_rule(BLAmdContext* c, const double* vtx, size_t n, uint32_t rule) {
-  if (n < 3) return BL_AMD_SUCCESS;
-  DevCmd cmd;
-  uint32_t r = resolve(c, cmd);
-  if (r == 1) return BL_AMD_SUCCESS;
-  if (r) return r;
+// Whether the vertices' hull reaches the canvas rows: the reference builds
+// edges first and reports a pipeline error only for non-empty edges; the
+// hull is the host-side stand-in for that test.
+bool hull_reaches_rows(const BLAmdContext* c, const CmdXform& xf, const double* vtx, size_t n) {
+  double lo = INFINITY, hi = -INFINITY;
+  for (size_t i = 0; i < n; i++) {
+    double y = vtx[2 * i] * xf.m01 + vtx[2 * i + 1] * xf.m11 + xf.m21;
+    if (y != y) continue;
+    lo = y < lo ? y : lo;
+    hi = y > hi ? y : hi;
+  }
+  return lo < hi && hi > 0.0 && lo < (double)c->target->h * 256.0;
+}
+
+uint32_t fill_poly_rule(BLAmdContext* c, const Resolved& r, const double* vtx, size_t n, uint32_t rule) {
+  if (n < 3 || r.nop) return BL_AMD_SUCCESS;
+  CmdXform xf = fixed_xform(c);
+  if (r.status) return hull_reaches_rows(c, xf, vtx, n) ? r.status : BL_AMD_SUCCESS;
+  DevCmd cmd = r.cmd;
   cmd.kind = kCmdAnalytic;
   cmd.rule_mask = rule == 0 ? 0xFFFFFFFFu : 0x1FFu;
-  CmdXform xf = fixed_xform(c);
   uint32_t ci = push_cmd(c, cmd, &xf);
   record_poly_segments(c, ci, vtx, n);
   return BL_AMD_SUCCESS;
 }
 
+uint32_t fill_path_rule(BLAmdContext* c, const Resolved& r, const double* origin, const uint8_t* cmds, const double* vtx, size_t n,
+                        uint32_t rule) {
+  if (n == 0 || r.nop) return BL_AMD_SUCCESS;
+  CmdXform xf = fixed_xform(c);
+  if (origin) {
+    // fill_unclipped_path_with_origin (rastercontext.cpp:2812-2821)
+    double ox = origin[0] * xf.m00 + origin[1] * xf.m10 + xf.m20;
+    double oy = origin[0] * xf.m01 + origin[1] * xf.m11 + xf.m21;
+    xf.m20 = ox;
+    xf.m21 = oy;
+  }
+  if (r.status) return hull_reaches_rows(c, xf, vtx, n) ? r.status : BL_AMD_SUCCESS;
+  DevCmd cmd = r.cmd;
+  cmd.kind = kCmdAnalytic;
+  cmd.rule_mask = rule == 0 ? 0xFFFFFFFFu : 0x1FFu;
+  uint32_t ci = push_cmd(c, cmd, &xf);
+  record_path_segments(c, ci, cmds, vtx, n);
+  return BL_AMD_SUCCESS;
+}
+
 // FillBoxA (fillgeneric_p.h:22-65)
-uint32_t fill_box_a(BLAmdContext* c, int x0, int y0, int x1, int y1) {
-  DevCmd cmd;
-  uint32_t r = resolve(c, cmd);
-  if (r == 1) return BL_AMD_SUCCESS;
-  if (r) return r;
+uint32_t fill_box_a(BLAmdContext* c, const Resolved& r, int x0, int y0, int x1, int y1) {
+  if (r.nop) return BL_AMD_SUCCESS;
+  if (r.status) return r.status;
+  DevCmd cmd = r.cmd;
   cmd.kind = kCmdBoxA;
   cmd.box[0] = x0; cmd.box[1] = y0; cmd.box[2] = x1; cmd.box[3] = y1;
   cmd.fetch_row0 = (uint32_t)y0;
@@ -506,11 +599,10 @@ uint32_t fill_box_a(BLAmdContext* c, int x0, int y0, int x1, int y1) {
 }
 
 // FillData::init_box_u_8bpc_24x8 (pipedefs_p.h:644-815) as per-pixel masks.
-uint32_t fill_box_u(BLAmdContext* c, int x0, int y0, int x1, int y1) {
-  DevCmd cmd;
-  uint32_t r = resolve(c, cmd);
-  if (r == 1) return BL_AMD_SUCCESS;
-  if (r) return r;
+uint32_t fill_box_u(BLAmdContext* c, const Resolved& r, int x0, int y0, int x1, int y1) {
+  if (r.nop) return BL_AMD_SUCCESS;
+  if (r.status) return r.status;
+  DevCmd cmd = r.cmd;
   const uint32_t alpha = cmd.alpha;
   uint32_t ax0 = (uint32_t)x0 >> 8, ay0 = (uint32_t)y0 >> 8;
   uint32_t ax1 = (uint32_t)(x1 + 0xFF) >> 8, ay1 = (uint32_t)(y1 + 0xFF) >> 8;
@@ -556,7 +648,8 @@ uint32_t fill_box_u(BLAmdContext* c, int x0, int y0, int x1, int y1) {
 }
 
 // fill_unclipped_box_d (rastercontext.cpp:2871-2891)
-uint32_t fill_box_d(BLAmdContext* c, double bx0, double by0, double bx1, double by1) {
+uint32_t fill_box_d(BLAmdContext* c, const Resolved& r, double bx0, double by0, double bx1, double by1) {
+  if (r.nop) return BL_AMD_SUCCESS;
   CmdXform m = fixed_xform(c);
   int tt = std::max(transform_type(c->user), (int)kTTScale);
   double cx1 = (double)c->target->w * 256.0, cy1 = (double)c->target->h * 256.0;
@@ -572,12 +665,66 @@ uint32_t fill_box_d(BLAmdContext* c, double bx0, double by0, double bx1, double 
     if (!((dx0 < dx1) & (dy0 < dy1))) return BL_AMD_SUCCESS;
     int ix0 = (int)dx0, iy0 = (int)dy0, ix1 = (int)dx1, iy1 = (int)dy1;
     if (ix0 >= ix1 || iy0 >= iy1) return BL_AMD_SUCCESS;
-    if (((ix0 | iy0 | ix1 | iy1) & 0xFF) == 0) return fill_box_a(c, ix0 >> 8, iy0 >> 8, ix1 >> 8, iy1 >> 8);
-    return fill_box_u(c, ix0, iy0, ix1, iy1);
+    if (((ix0 | iy0 | ix1 | iy1) & 0xFF) == 0) return fill_box_a(c, r, ix0 >> 8, iy0 >> 8, ix1 >> 8, iy1 >> 8);
+    return fill_box_u(c, r, ix0, iy0, ix1, iy1);
   }
   double poly[8] = { bx0, by0, bx1, by0, bx1, by1, bx0, by1 };
-  return fill_poly_rule(c, poly, 4, 1);
+  return fill_poly_rule(c, r, poly, 4, 1);
 }
+
+// fillUnclippedRectI (rastercontext.cpp:2898-2917)
+uint32_t fill_rect_i(BLAmdContext* c, const Resolved& r, int x, int y, int w, int h) {
+  if (r.nop) return BL_AMD_SUCCESS;
+  int tt = transform_type(c->user);
+  bool integral = false;
+  int tx = 0, ty = 0;
+  if (tt <= kTTTranslate) {
+    double fx = c->user.m20 * 256.0, fy = c->user.m21 * 256.0;
+    // rastercontext.cpp:773-789: only translations within the safe fixed-point
+    // range (fp_min/max_safe_coord_d, rastercontext.cpp:4357-4358)
+    const double lo = floor((double)(INT32_MIN + 1) * 256.0);
+    const double hi = floor((double)(INT32_MAX - 1 - std::max(c->target->w, c->target->h)) * 256.0);
+    if (fx >= lo && fx <= hi && fy >= lo && fy <= hi) {
+      int64_t tx64 = floor_to_int64(fx), ty64 = floor_to_int64(fy);
+      if (((tx64 | ty64) & 0xFF) == 0) { integral = true; tx = (int)(tx64 >> 8); ty = (int)(ty64 >> 8); }
+    }
+  }
+  if (!integral) {
+    if (w <= 0 || h <= 0) return BL_AMD_SUCCESS;
+    return fill_box_d(c, r, (double)x, (double)y, (double)x + (double)w, (double)y + (double)h);
+  }
+  int64_t x0 = (int64_t)x + tx, y0 = (int64_t)y + ty;
+  int64_t x1 = (int64_t)w + x0, y1 = (int64_t)h + y0;
+  x0 = std::max<int64_t>(x0, 0); y0 = std::max<int64_t>(y0, 0);
+  x1 = std::min<int64_t>(x1, c->target->w); y1 = std::min<int64_t>(y1, c->target->h);
+  if (x0 >= x1 || y0 >= y1) return BL_AMD_SUCCESS;
+  return fill_box_a(c, r, (int)x0, (int)y0, (int)x1, (int)y1);
+}
+
+// fill_all (rastercontext.cpp:2720-2725) with the canvas as the clip box.
+uint32_t fill_all_box(BLAmdContext* c, const Resolved& r) {
+  return fill_box_a(c, r, 0, 0, c->target->w, c->target->h);
+}
+
+// The path BLPath::add_geometry builds for a geometry (core/path.cpp:1396-1640),
+// clockwise direction.
+const double kKappa = 0.55228474983079339840;   // 4/3 (sqrt(2) - 1)
+
+struct TmpPath {
+  std::vector<uint8_t> cmd;
+  std::vector<double> vtx;
+  void move_to(double x, double y) { cmd.push_back(kCmdMove); vtx.push_back(x); vtx.push_back(y); }
+  void line_to(double x, double y) { cmd.push_back(kCmdOn); vtx.push_back(x); vtx.push_back(y); }
+  void cubic_to(double x1, double y1, double x2, double y2, double x3, double y3) {
+    cmd.push_back(kCmdCubic); cmd.push_back(kCmdCubic); cmd.push_back(kCmdOn);
+    vtx.insert(vtx.end(), {x1, y1, x2, y2, x3, y3});
+  }
+  void close() { cmd.push_back(5); vtx.push_back(NAN); vtx.push_back(NAN); }
+  void box(double x0, double y0, double x1, double y1) {
+    // append_box_internal, clockwise
+    move_to(x0, y0); line_to(x1, y0); line_to(x1, y1); line_to(x0, y1); close();
+  }
+};
 
 uint32_t check_target(const BLAmdContext* c) { return c && c->target ? BL_AMD_SUCCESS : kErrState; }
 
@@ -837,61 +984,254 @@ BLAmdResult bl_amd_context_fill_path_d(BLAmdContext* ctx, const double* origin, 
   if (uint32_t r = check_target(ctx)) return r;
   if (n == 0) return BL_AMD_SUCCESS;
   if (!cmds || !vertices) return kErrInvalid;
-  DevCmd cmd;
-  uint32_t r = resolve(ctx, cmd);
-  if (r == 1) return BL_AMD_SUCCESS;
-  if (r) return r;
-  cmd.kind = kCmdAnalytic;
-  cmd.rule_mask = ctx->fill_rule == 0 ? 0xFFFFFFFFu : 0x1FFu;
-  CmdXform xf = fixed_xform(ctx);
-  if (origin) {
-    // fill_unclipped_path_with_origin (rastercontext.cpp:2812-2821)
-    double ox = origin[0] * xf.m00 + origin[1] * xf.m10 + xf.m20;
-    double oy = origin[0] * xf.m01 + origin[1] * xf.m11 + xf.m21;
-    xf.m20 = ox;
-    xf.m21 = oy;
-  }
-  uint32_t ci = push_cmd(ctx, cmd, &xf);
-  record_path_segments(ctx, ci, cmds, vertices, n);
-  return BL_AMD_SUCCESS;
+  return fill_path_rule(ctx, resolve_fill(ctx), origin, cmds, vertices, n, ctx->fill_rule);
+}
+
+BLAmdResult bl_amd_context_fill_path_d_rgba32(BLAmdContext* ctx, const double* origin, const uint8_t* cmds, const double* vertices,
+                                              size_t n, uint32_t rgba32) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (n == 0) return BL_AMD_SUCCESS;
+  if (!cmds || !vertices) return kErrInvalid;
+  return fill_path_rule(ctx, resolve_solid(ctx, rgba32), origin, cmds, vertices, n, ctx->fill_rule);
 }
 
 BLAmdResult bl_amd_context_fill_polygon_d(BLAmdContext* ctx, const double* vertices, size_t n) {
   if (uint32_t r = check_target(ctx)) return r;
   if (n && !vertices) return kErrInvalid;
-  return fill_poly_rule(ctx, vertices, n, ctx->fill_rule);
+  if (n < 3) return BL_AMD_SUCCESS;
+  return fill_poly_rule(ctx, resolve_fill(ctx), vertices, n, ctx->fill_rule);
 }
 
 BLAmdResult bl_amd_context_fill_rect_d(BLAmdContext* ctx, double x, double y, double w, double h) {
   if (uint32_t r = check_target(ctx)) return r;
-  return fill_box_d(ctx, x, y, x + w, y + h);
+  return fill_box_d(ctx, resolve_fill(ctx), x, y, x + w, y + h);
 }
 
-// fillUnclippedRectI (rastercontext.cpp:2898-2917)
+BLAmdResult bl_amd_context_fill_rect_d_rgba32(BLAmdContext* ctx, double x, double y, double w, double h, uint32_t rgba32) {
+  if (uint32_t r = check_target(ctx)) return r;
+  return fill_box_d(ctx, resolve_solid(ctx, rgba32), x, y, x + w, y + h);
+}
+
 BLAmdResult bl_amd_context_fill_rect_i(BLAmdContext* ctx, int x, int y, int w, int h) {
   if (uint32_t r = check_target(ctx)) return r;
-  int tt = transform_type(ctx->user);
-  bool integral = false;
-  int tx = 0, ty = 0;
-  if (tt <= kTTTranslate) {
-    int64_t tx64 = floor_to_int64(ctx->user.m20 * 256.0), ty64 = floor_to_int64(ctx->user.m21 * 256.0);
-    if (((tx64 | ty64) & 0xFF) == 0) { integral = true; tx = (int)(tx64 >> 8); ty = (int)(ty64 >> 8); }
-  }
-  if (!integral) {
-    if (w <= 0 || h <= 0) return BL_AMD_SUCCESS;
-    return fill_box_d(ctx, (double)x, (double)y, (double)x + (double)w, (double)y + (double)h);
-  }
-  int64_t x0 = (int64_t)x + tx, y0 = (int64_t)y + ty;
-  int64_t x1 = (int64_t)w + x0, y1 = (int64_t)h + y0;
-  x0 = std::max<int64_t>(x0, 0); y0 = std::max<int64_t>(y0, 0);
-  x1 = std::min<int64_t>(x1, ctx->target->w); y1 = std::min<int64_t>(y1, ctx->target->h);
-  if (x0 >= x1 || y0 >= y1) return BL_AMD_SUCCESS;
-  return fill_box_a(ctx, (int)x0, (int)y0, (int)x1, (int)y1);
+  return fill_rect_i(ctx, resolve_fill(ctx), x, y, w, h);
+}
+
+BLAmdResult bl_amd_context_fill_rect_i_rgba32(BLAmdContext* ctx, int x, int y, int w, int h, uint32_t rgba32) {
+  if (uint32_t r = check_target(ctx)) return r;
+  return fill_rect_i(ctx, resolve_solid(ctx, rgba32), x, y, w, h);
 }
 
 BLAmdResult bl_amd_context_fill_all(BLAmdContext* ctx) {
   if (uint32_t r = check_target(ctx)) return r;
-  return fill_box_a(ctx, 0, 0, ctx->target->w, ctx->target->h);
+  return fill_all_box(ctx, resolve_fill(ctx, false));
+}
+
+BLAmdResult bl_amd_context_fill_all_rgba32(BLAmdContext* ctx, uint32_t rgba32) {
+  if (uint32_t r = check_target(ctx)) return r;
+  return fill_all_box(ctx, resolve_solid(ctx, rgba32, false));
+}
+
+BLAmdResult bl_amd_context_clear_all(BLAmdContext* ctx) {
+  if (uint32_t r = check_target(ctx)) return r;
+  return fill_all_box(ctx, resolve_clear(ctx, false));
+}
+
+BLAmdResult bl_amd_context_clear_rect_i(BLAmdContext* ctx, int x, int y, int w, int h) {
+  if (uint32_t r = check_target(ctx)) return r;
+  return fill_rect_i(ctx, resolve_clear(ctx, true), x, y, w, h);
+}
+
+BLAmdResult bl_amd_context_clear_rect_d(BLAmdContext* ctx, double x, double y, double w, double h) {
+  if (uint32_t r = check_target(ctx)) return r;
+  return fill_box_d(ctx, resolve_clear(ctx, true), x, y, x + w, y + h);
+}
+
+// set_style_alpha_impl (rastercontext.cpp:1920-1939)
+BLAmdResult bl_amd_context_set_fill_alpha(BLAmdContext* ctx, double alpha) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (isnan(alpha)) return kErrInvalid;
+  ctx->fill_alpha = alpha < 0.0 ? 0.0 : (alpha > 1.0 ? 1.0 : alpha);
+  return BL_AMD_SUCCESS;
+}
+
+// set_style_rgba64_impl (rastercontext.cpp:588-614): the high byte of each
+// 16-bit channel (RgbaInternal::rgba32FromRgba64, core/rgba_p.h:67-74).
+BLAmdResult bl_amd_context_set_fill_style_rgba64(BLAmdContext* ctx, uint64_t rgba64) {
+  uint32_t rgba32 = (uint32_t)(((rgba64 >> 56) & 0xFFu) << 24 | ((rgba64 >> 40) & 0xFFu) << 16 |
+                               ((rgba64 >> 24) & 0xFFu) << 8 | ((rgba64 >> 8) & 0xFFu));
+  return bl_amd_context_set_fill_style_rgba32(ctx, rgba32);
+}
+
+// set_style_rgba_impl (rastercontext.cpp:616-655): NaN disables the style;
+// channels are clamped, premultiplied in float and rounded half-to-even.
+BLAmdResult bl_amd_context_set_fill_style_rgba(BLAmdContext* ctx, const float rgba[4]) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (!rgba) return kErrInvalid;
+  StyleState& st = ctx->style;
+  st = StyleState{};
+  st.version = ++ctx->style_counter;
+  st.type = kStyleSolid;
+  st.dev.kind = kFetchSolid;
+  if (isnan(rgba[0]) || isnan(rgba[1]) || isnan(rgba[2]) || isnan(rgba[3])) { st.status = 1; return BL_AMD_SUCCESS; }
+  float n[4];
+  for (int i = 0; i < 4; i++) n[i] = rgba[i] < 0.0f ? 0.0f : (rgba[i] > 1.0f ? 1.0f : rgba[i]);
+  float a_scale = n[3] * 255.0f;
+  uint32_t r = (uint32_t)nearbyintf(n[0] * a_scale);
+  uint32_t g = (uint32_t)nearbyintf(n[1] * a_scale);
+  uint32_t b = (uint32_t)nearbyintf(n[2] * a_scale);
+  uint32_t a = (uint32_t)nearbyintf(a_scale);
+  uint32_t p = (a << 24) | (r << 16) | (g << 8) | b;
+  st.fmt = solid_format(p);
+  st.dev.solid = p;
+  return BL_AMD_SUCCESS;
+}
+
+// fill_unclipped_geometry (rastercontext.cpp:2925-2980) + BLPath::add_geometry
+// (core/path.cpp:1396-1640) for the geometry types that do not need the
+// arc approximation.
+static uint32_t fill_geometry_resolved(BLAmdContext* ctx, const Resolved& r, uint32_t type, const void* data) {
+  if (type != BL_AMD_GEOMETRY_NONE && !data) return kErrInvalid;
+  switch (type) {
+    case BL_AMD_GEOMETRY_NONE:
+      return BL_AMD_SUCCESS;
+    case BL_AMD_GEOMETRY_BOXI: {
+      const int32_t* b = (const int32_t*)data;
+      return fill_box_d(ctx, r, (double)b[0], (double)b[1], (double)b[2], (double)b[3]);
+    }
+    case BL_AMD_GEOMETRY_BOXD: {
+      const double* b = (const double*)data;
+      return fill_box_d(ctx, r, b[0], b[1], b[2], b[3]);
+    }
+    case BL_AMD_GEOMETRY_RECTI: {
+      const int32_t* b = (const int32_t*)data;
+      return fill_rect_i(ctx, r, b[0], b[1], b[2], b[3]);
+    }
+    case BL_AMD_GEOMETRY_RECTD: {
+      const double* b = (const double*)data;
+      return fill_box_d(ctx, r, b[0], b[1], b[0] + b[2], b[1] + b[3]);
+    }
+    case BL_AMD_GEOMETRY_POLYGONI:
+    case BL_AMD_GEOMETRY_POLYLINEI:
+    case BL_AMD_GEOMETRY_POLYGOND:
+    case BL_AMD_GEOMETRY_POLYLINED: {
+      const BLAmdArrayView* v = (const BLAmdArrayView*)data;
+      if (v->size < 3) return BL_AMD_SUCCESS;
+      if (!v->data) return kErrInvalid;
+      if (type == BL_AMD_GEOMETRY_POLYGOND || type == BL_AMD_GEOMETRY_POLYLINED)
+        return fill_poly_rule(ctx, r, (const double*)v->data, v->size, ctx->fill_rule);
+      std::vector<double> pts(2 * v->size);
+      const int32_t* src = (const int32_t*)v->data;
+      for (size_t i = 0; i < 2 * v->size; i++) pts[i] = (double)src[i];
+      return fill_poly_rule(ctx, r, pts.data(), v->size, ctx->fill_rule);
+    }
+    default:
+      break;
+  }
+
+  TmpPath p;
+  switch (type) {
+    case BL_AMD_GEOMETRY_CIRCLE:
+    case BL_AMD_GEOMETRY_ELLIPSE: {
+      const double* e = (const double*)data;
+      double x0 = e[0], y0 = e[1], rx, ry;
+      if (type == BL_AMD_GEOMETRY_CIRCLE) { rx = e[2]; ry = fabs(rx); }
+      else { rx = e[2]; ry = e[3]; }
+      double kx = rx * kKappa, ky = ry * kKappa;
+      p.move_to(x0 + rx, y0);
+      p.cubic_to(x0 + rx, y0 + ky, x0 + kx, y0 + ry, x0, y0 + ry);
+      p.cubic_to(x0 - kx, y0 + ry, x0 - rx, y0 + ky, x0 - rx, y0);
+      p.cubic_to(x0 - rx, y0 - ky, x0 - kx, y0 - ry, x0, y0 - ry);
+      p.cubic_to(x0 + kx, y0 - ry, x0 + rx, y0 - ky, x0 + rx, y0);
+      p.close();
+      break;
+    }
+    case BL_AMD_GEOMETRY_ROUND_RECT: {
+      const double* q = (const double*)data;
+      double x0 = q[0], y0 = q[1], x1 = q[0] + q[2], y1 = q[1] + q[3];
+      double rx = std::min(fabs(q[4]), q[2] * 0.5);
+      double ry = std::min(fabs(q[5]), q[3] * 0.5);
+      if (!(rx > 1e-14 && ry > 1e-14)) { p.box(x0, y0, x1, y1); break; }
+      double kx = rx * (1.0 - kKappa), ky = ry * (1.0 - kKappa);
+      p.move_to(x0 + rx, y0);
+      p.line_to(x1 - rx, y0);
+      p.cubic_to(x1 - kx, y0, x1, y0 + ky, x1, y0 + ry);
+      p.line_to(x1, y1 - ry);
+      p.cubic_to(x1, y1 - ky, x1 - kx, y1, x1 - rx, y1);
+      p.line_to(x0 + rx, y1);
+      p.cubic_to(x0 + kx, y1, x0, y1 - ky, x0, y1 - ry);
+      p.line_to(x0, y0 + ry);
+      p.cubic_to(x0, y0 + ky, x0 + kx, y0, x0 + rx, y0);
+      p.close();
+      break;
+    }
+    case BL_AMD_GEOMETRY_LINE: {
+      const double* l = (const double*)data;
+      p.move_to(l[0], l[1]);
+      p.line_to(l[2], l[3]);
+      break;
+    }
+    case BL_AMD_GEOMETRY_TRIANGLE: {
+      const double* t = (const double*)data;
+      p.move_to(t[0], t[1]);
+      p.line_to(t[2], t[3]);
+      p.line_to(t[4], t[5]);
+      p.close();
+      break;
+    }
+    case BL_AMD_GEOMETRY_ARRAY_VIEW_BOXI:
+    case BL_AMD_GEOMETRY_ARRAY_VIEW_BOXD:
+    case BL_AMD_GEOMETRY_ARRAY_VIEW_RECTI:
+    case BL_AMD_GEOMETRY_ARRAY_VIEW_RECTD: {
+      const BLAmdArrayView* v = (const BLAmdArrayView*)data;
+      if (!v->size) return BL_AMD_SUCCESS;
+      if (!v->data) return kErrInvalid;
+      // items that are not valid boxes / rects are skipped (geometry/commons_p.h:42-56)
+      for (size_t i = 0; i < v->size; i++) {
+        if (type == BL_AMD_GEOMETRY_ARRAY_VIEW_BOXI) {
+          const int32_t* b = (const int32_t*)v->data + 4 * i;
+          if (b[0] < b[2] && b[1] < b[3]) p.box((double)b[0], (double)b[1], (double)b[2], (double)b[3]);
+        }
+        else if (type == BL_AMD_GEOMETRY_ARRAY_VIEW_BOXD) {
+          const double* b = (const double*)v->data + 4 * i;
+          if (b[0] < b[2] && b[1] < b[3]) p.box(b[0], b[1], b[2], b[3]);
+        }
+        else if (type == BL_AMD_GEOMETRY_ARRAY_VIEW_RECTI) {
+          const int32_t* b = (const int32_t*)v->data + 4 * i;
+          int64_t x1 = (int64_t)b[0] + b[2], y1 = (int64_t)b[1] + b[3];
+          bool overflow = x1 != (int32_t)x1 || y1 != (int32_t)y1;
+          if (overflow || !(b[0] < x1 && b[1] < y1)) continue;
+          double x0 = (double)b[0], y0 = (double)b[1];
+          p.box(x0, y0, (double)b[2] + x0, (double)b[3] + y0);
+        }
+        else {
+          const double* b = (const double*)v->data + 4 * i;
+          double x1 = b[0] + b[2], y1 = b[1] + b[3];
+          if (b[0] < x1 && b[1] < y1) p.box(b[0], b[1], b[2] + b[0], b[3] + b[1]);
+        }
+      }
+      break;
+    }
+    case BL_AMD_GEOMETRY_ARC:
+    case BL_AMD_GEOMETRY_CHORD:
+    case BL_AMD_GEOMETRY_PIE:
+      return r.nop ? BL_AMD_SUCCESS : kErrNotImpl;
+    default:
+      return kErrInvalid;
+  }
+  return fill_path_rule(ctx, r, nullptr, p.cmd.data(), p.vtx.data(), p.cmd.size(), ctx->fill_rule);
+}
+
+BLAmdResult bl_amd_context_fill_geometry(BLAmdContext* ctx, uint32_t type, const void* data) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (type > BL_AMD_GEOMETRY_MAX_VALUE) return kErrInvalid;
+  return fill_geometry_resolved(ctx, resolve_fill(ctx), type, data);
+}
+
+BLAmdResult bl_amd_context_fill_geometry_rgba32(BLAmdContext* ctx, uint32_t type, const void* data, uint32_t rgba32) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (type > BL_AMD_GEOMETRY_MAX_VALUE) return kErrInvalid;
+  return fill_geometry_resolved(ctx, resolve_solid(ctx, rgba32), type, data);
 }
 
 static void copy_stats(const Stats& st, BLAmdStats* out) {

@@ -22,7 +22,7 @@ def _images(ctx_cls, sc):
 def replay_into(ctx, sc: "S.Scene", images) -> list:
     """Issues the scene's calls on `ctx`; returns the BLResult errors seen
     (calls keep going after an error, as the reference driver does)."""
-    from . import BLError, BLGradient, BLPattern
+    from . import BLError, BLGradient, BLPattern, BLRgba, BLRgba64
     errors = []
 
     def call(fn, *args):
@@ -68,6 +68,42 @@ def replay_into(ctx, sc: "S.Scene", images) -> list:
             call(ctx.fill_rect, *[int(v) for v in struct.unpack_from("<4i", pl)])
         elif op == S.OP_FILL_ALL:
             call(ctx.fill_all)
+        elif op == S.OP_SET_FILL_ALPHA:
+            call(ctx.set_fill_alpha, struct.unpack_from("<d", pl)[0])
+        elif op == S.OP_SET_STYLE_RGBA64:
+            call(ctx.set_fill_style, BLRgba64(struct.unpack_from("<Q", pl)[0]))
+        elif op == S.OP_SET_STYLE_RGBA:
+            call(ctx.set_fill_style, BLRgba(*struct.unpack_from("<4f", pl)))
+        elif op == S.OP_CLEAR_ALL:
+            call(ctx.clear_all)
+        elif op == S.OP_CLEAR_RECT_I:
+            call(ctx.clear_rect, *[int(v) for v in struct.unpack_from("<4i", pl)])
+        elif op == S.OP_CLEAR_RECT_D:
+            call(ctx.clear_rect, *[float(v) for v in struct.unpack_from("<4d", pl)])
+        elif op == S.OP_FILL_ALL_RGBA32:
+            call(ctx.fill_all, struct.unpack_from("<I", pl)[0])
+        elif op == S.OP_FILL_RECT_I_RGBA32:
+            v = struct.unpack_from("<4iI", pl)
+            call(ctx.fill_rect, *[int(x) for x in v[:4]], v[4])
+        elif op == S.OP_FILL_RECT_D_RGBA32:
+            v = struct.unpack_from("<4dI", pl)
+            call(ctx.fill_rect, *[float(x) for x in v[:4]], v[4])
+        elif op == S.OP_FILL_GEOMETRY:
+            gtype, has_col, col, count = struct.unpack_from("<4I", pl)
+            if gtype in S.GEOMETRY_ARRAY:
+                fmt = S.GEOMETRY_ARRAY[gtype]
+                size = struct.calcsize(fmt)
+                data = [struct.unpack_from(fmt, pl, 16 + k * size) for k in range(count)]
+            elif gtype in S.GEOMETRY_STRUCT:
+                data = struct.unpack_from(S.GEOMETRY_STRUCT[gtype], pl, 16)
+            else:
+                data = ()
+            call(ctx.fill_geometry, gtype, data, col if has_col else None)
+        elif op == S.OP_FILL_PATH_EX:
+            n, flags, col, _, ox, oy = struct.unpack_from("<4I2d", pl)
+            cmds = np.frombuffer(pl, dtype=np.uint8, count=n, offset=32)
+            vtx = np.frombuffer(pl, dtype=np.float64, count=2 * n, offset=32 + ((n + 7) & ~7)).reshape(-1, 2)
+            call(ctx.fill_path, (cmds, vtx), (ox, oy) if flags & 1 else None, col if flags & 2 else None)
     return errors
 
 

@@ -36,6 +36,36 @@ OP_FILL_POLYGON = 10
 OP_FILL_RECT_D = 11
 OP_FILL_RECT_I = 12
 OP_FILL_ALL = 13
+OP_SET_FILL_ALPHA = 14
+OP_SET_STYLE_RGBA64 = 15
+OP_SET_STYLE_RGBA = 16
+OP_CLEAR_ALL = 17
+OP_CLEAR_RECT_I = 18
+OP_CLEAR_RECT_D = 19
+OP_FILL_GEOMETRY = 20
+OP_FILL_ALL_RGBA32 = 21
+OP_FILL_RECT_I_RGBA32 = 22
+OP_FILL_RECT_D_RGBA32 = 23
+OP_FILL_PATH_EX = 24
+
+# BLGeometryType values (core/geometry.h).
+(GEOMETRY_NONE, GEOMETRY_BOXI, GEOMETRY_BOXD, GEOMETRY_RECTI, GEOMETRY_RECTD, GEOMETRY_CIRCLE, GEOMETRY_ELLIPSE,
+ GEOMETRY_ROUND_RECT, GEOMETRY_ARC, GEOMETRY_CHORD, GEOMETRY_PIE, GEOMETRY_LINE, GEOMETRY_TRIANGLE,
+ GEOMETRY_POLYLINEI, GEOMETRY_POLYLINED, GEOMETRY_POLYGONI, GEOMETRY_POLYGOND, GEOMETRY_ARRAY_VIEW_BOXI,
+ GEOMETRY_ARRAY_VIEW_BOXD, GEOMETRY_ARRAY_VIEW_RECTI, GEOMETRY_ARRAY_VIEW_RECTD) = range(21)
+
+# struct layout of each fixed-size geometry: (struct format, number of fields)
+GEOMETRY_STRUCT = {
+    GEOMETRY_BOXI: "<4i", GEOMETRY_BOXD: "<4d", GEOMETRY_RECTI: "<4i", GEOMETRY_RECTD: "<4d",
+    GEOMETRY_CIRCLE: "<3d", GEOMETRY_ELLIPSE: "<4d", GEOMETRY_ROUND_RECT: "<6d", GEOMETRY_ARC: "<6d",
+    GEOMETRY_CHORD: "<6d", GEOMETRY_PIE: "<6d", GEOMETRY_LINE: "<4d", GEOMETRY_TRIANGLE: "<6d",
+}
+# element layout of array geometries
+GEOMETRY_ARRAY = {
+    GEOMETRY_POLYLINEI: "<2i", GEOMETRY_POLYLINED: "<2d", GEOMETRY_POLYGONI: "<2i", GEOMETRY_POLYGOND: "<2d",
+    GEOMETRY_ARRAY_VIEW_BOXI: "<4i", GEOMETRY_ARRAY_VIEW_BOXD: "<4d",
+    GEOMETRY_ARRAY_VIEW_RECTI: "<4i", GEOMETRY_ARRAY_VIEW_RECTD: "<4d",
+}
 
 # BLCompOp values (core/context.h).
 COMP_OP_SRC_OVER, COMP_OP_SRC_COPY = 0, 1
@@ -133,6 +163,66 @@ class Scene:
 
     def fill_all(self) -> None:
         self._cmd(OP_FILL_ALL)
+        self.fill_count += 1
+
+    def set_fill_alpha(self, a: float) -> None:
+        self._cmd(OP_SET_FILL_ALPHA, struct.pack("<d", a))
+
+    def set_fill_style_rgba64(self, rgba64: int) -> None:
+        self._cmd(OP_SET_STYLE_RGBA64, struct.pack("<Q", rgba64 & 0xFFFFFFFFFFFFFFFF))
+
+    def set_fill_style_rgba(self, r: float, g: float, b: float, a: float) -> None:
+        self._cmd(OP_SET_STYLE_RGBA, struct.pack("<4f", r, g, b, a))
+
+    def clear_all(self) -> None:
+        self._cmd(OP_CLEAR_ALL)
+        self.fill_count += 1
+
+    def clear_rect_i(self, x, y, w, h) -> None:
+        self._cmd(OP_CLEAR_RECT_I, struct.pack("<4i", x, y, w, h))
+        self.fill_count += 1
+
+    def clear_rect_d(self, x, y, w, h) -> None:
+        self._cmd(OP_CLEAR_RECT_D, struct.pack("<4d", x, y, w, h))
+        self.fill_count += 1
+
+    def fill_all_rgba32(self, rgba32: int) -> None:
+        self._cmd(OP_FILL_ALL_RGBA32, struct.pack("<II", rgba32 & 0xFFFFFFFF, 0))
+        self.fill_count += 1
+
+    def fill_rect_i_rgba32(self, x, y, w, h, rgba32: int) -> None:
+        self._cmd(OP_FILL_RECT_I_RGBA32, struct.pack("<4iII", x, y, w, h, rgba32 & 0xFFFFFFFF, 0))
+        self.fill_count += 1
+
+    def fill_rect_d_rgba32(self, x, y, w, h, rgba32: int) -> None:
+        self._cmd(OP_FILL_RECT_D_RGBA32, struct.pack("<4dII", x, y, w, h, rgba32 & 0xFFFFFFFF, 0))
+        self.fill_count += 1
+
+    def fill_geometry(self, gtype: int, data, rgba32: int | None = None) -> None:
+        """`data`: the geometry's fields (fixed-size types) or a sequence of
+        elements, each a tuple of fields (array types)."""
+        if gtype in GEOMETRY_ARRAY:
+            fmt = GEOMETRY_ARRAY[gtype]
+            items = list(data)
+            body = b"".join(struct.pack(fmt, *it) for it in items)
+            count = len(items)
+        elif gtype in GEOMETRY_STRUCT:
+            body = struct.pack(GEOMETRY_STRUCT[gtype], *data)
+            count = 0
+        else:
+            body, count = b"", 0
+        head = struct.pack("<4I", gtype, 0 if rgba32 is None else 1, 0 if rgba32 is None else rgba32 & 0xFFFFFFFF, count)
+        self._cmd(OP_FILL_GEOMETRY, head + body)
+        self.fill_count += 1
+
+    def fill_path_ex(self, cmds, vtx, origin=None, rgba32: int | None = None) -> None:
+        cmds = np.asarray(cmds, dtype=np.uint8)
+        vtx = np.asarray(vtx, dtype=np.float64).reshape(-1, 2)
+        assert len(cmds) == len(vtx)
+        flags = (1 if origin is not None else 0) | (2 if rgba32 is not None else 0)
+        ox, oy = origin if origin is not None else (0.0, 0.0)
+        head = struct.pack("<4I2d", len(cmds), flags, 0 if rgba32 is None else rgba32 & 0xFFFFFFFF, 0, ox, oy)
+        self._cmd(OP_FILL_PATH_EX, head + _pad8(cmds.tobytes()) + vtx.tobytes())
         self.fill_count += 1
 
     # -- serialization ---------------------------------------------------
@@ -289,14 +379,17 @@ def glyph_path(rng, size_w: float, size_h: float):
     ctl_r = 0.2 + 0.3 * rng.random(n)
     mid = ang + math.pi / n
     ctl = np.stack([cx + np.cos(mid) * ctl_r * size_w, cy + np.sin(mid) * ctl_r * size_h], 1)
-    cmds = [CMD_MOVE]
-    vtx = [pts[0]]
-    for k in range(n):
-        cmds += [CMD_QUAD, CMD_ON]
-        vtx += [ctl[k], pts[(k + 1) % n]]
-    cmds.append(CMD_CLOSE)
-    vtx.append(pts[0])
-    return cmds, np.array(vtx)
+    cmds = np.empty(2 * n + 2, dtype=np.uint8)
+    cmds[0] = CMD_MOVE
+    cmds[1:-1:2] = CMD_QUAD
+    cmds[2:-1:2] = CMD_ON
+    cmds[-1] = CMD_CLOSE
+    vtx = np.empty((2 * n + 2, 2))
+    vtx[0] = pts[0]
+    vtx[1:-1:2] = ctl
+    vtx[2:-1:2] = np.roll(pts, -1, axis=0)
+    vtx[-1] = pts[0]
+    return cmds, vtx
 
 
 def glyph_canvases(seed: int = 1, count: int = 4096, max_paths: int = 200):
@@ -472,4 +565,107 @@ def edge_case_scene(width: int = 257, height: int = 131, fmt: int = FMT_PRGB32) 
                  [(0.0, H), (W * 3, -H * 2), (-W * 2, -H * 2), (W, H)])
     sc.fill_path([CMD_MOVE, CMD_QUAD, CMD_ON, CMD_QUAD, CMD_ON],
                  [(W * 0.5, -H), (W * 4, H * 0.5), (W * 0.5, H * 2), (-W * 3, H * 0.5), (W * 0.5, -H)])
+    return sc
+
+
+def frontend_scene(seed: int = 1, width: int = 300, height: int = 220, fmt: int = FMT_PRGB32, count: int = 120) -> Scene:
+    """The rest of the fill frontend in one seeded stream: fill_geometry of
+    every non-arc geometry type, explicit-colour fills, clear_all /
+    clear_rect, fill alpha, 64-bit and float colours (NaN disables the
+    style), fill_path with an origin, and non-invertible user transforms
+    (geometry fills become no-ops, fill_all still renders)."""
+    rng = np.random.default_rng(seed)
+    clear = 0xFF000000 | int(rng.integers(0, 1 << 24)) if fmt != FMT_A8 else int(rng.integers(0, 256))
+    sc = Scene(width, height, fmt, clear)
+    W, H = float(width), float(height)
+
+    def pt():
+        return float(rng.uniform(-0.1, 1.1) * W), float(rng.uniform(-0.1, 1.1) * H)
+
+    for i in range(count):
+        r = rng.random()
+        if r < 0.08:
+            sc.set_fill_alpha(float(rng.uniform(0, 1)) if rng.random() < 0.8 else 1.0)
+        elif r < 0.14:
+            sc.set_fill_style_rgba64(int(rng.integers(0, 1 << 63)) * 2 + int(rng.integers(0, 2)))
+        elif r < 0.2:
+            v = [float(x) for x in rng.uniform(-0.2, 1.2, 4)]
+            if rng.random() < 0.1:
+                v[int(rng.integers(0, 4))] = float("nan")
+            sc.set_fill_style_rgba(*v)
+        elif r < 0.26:
+            sc.set_fill_style_rgba32(_rand_rgba(rng))
+        elif r < 0.3:
+            t = rng.random()
+            if t < 0.25:
+                sc.set_transform([0.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+            elif t < 0.5:
+                sc.set_transform(rotation_about(float(rng.uniform(0, 6.3)), W / 2, H / 2))
+            else:
+                sc.set_transform([1.0, 0.0, 0.0, 1.0, float(rng.integers(-5, 6)), float(rng.integers(-5, 6))])
+        elif r < 0.33:
+            sc.set_comp_op(int(rng.choice([COMP_OP_SRC_OVER, COMP_OP_SRC_COPY])))
+        elif r < 0.35:
+            if rng.random() < 0.2:
+                sc.clear_all()
+            else:
+                sc.fill_all_rgba32((int(rng.integers(0, 0x80)) << 24) | int(rng.integers(0, 1 << 24)))
+        elif r < 0.4:
+            x, y = pt()
+            w, h = float(rng.uniform(-5, W / 2)), float(rng.uniform(-5, H / 2))
+            k = int(rng.integers(0, 4))
+            if k == 0:
+                sc.clear_rect_i(int(x), int(y), int(w), int(h))
+            elif k == 1:
+                sc.clear_rect_d(x, y, w, h)
+            elif k == 2:
+                sc.fill_rect_i_rgba32(int(x), int(y), int(w), int(h), _rand_rgba(rng))
+            else:
+                sc.fill_rect_d_rgba32(x, y, w, h, _rand_rgba(rng))
+        elif r < 0.46:
+            cm, v = glyph_path(rng, float(rng.uniform(10, W / 2)), float(rng.uniform(10, H / 2)))
+            org = pt() if rng.random() < 0.7 else None
+            col = _rand_rgba(rng) if rng.random() < 0.5 else None
+            sc.fill_path_ex(cm, v, org, col)
+        else:
+            g = int(rng.choice([GEOMETRY_BOXI, GEOMETRY_BOXD, GEOMETRY_RECTI, GEOMETRY_RECTD, GEOMETRY_CIRCLE,
+                                GEOMETRY_ELLIPSE, GEOMETRY_ROUND_RECT, GEOMETRY_LINE, GEOMETRY_TRIANGLE,
+                                GEOMETRY_POLYLINEI, GEOMETRY_POLYLINED, GEOMETRY_POLYGONI, GEOMETRY_POLYGOND,
+                                GEOMETRY_ARRAY_VIEW_BOXI, GEOMETRY_ARRAY_VIEW_BOXD, GEOMETRY_ARRAY_VIEW_RECTI,
+                                GEOMETRY_ARRAY_VIEW_RECTD, GEOMETRY_NONE]))
+            x, y = pt()
+            w, h = float(rng.uniform(-4, W / 2)), float(rng.uniform(-4, H / 2))
+            if g in (GEOMETRY_BOXI, GEOMETRY_RECTI):
+                data = (int(x), int(y), int(x + w) if g == GEOMETRY_BOXI else int(w), int(y + h) if g == GEOMETRY_BOXI else int(h))
+            elif g == GEOMETRY_BOXD:
+                data = (x, y, x + w, y + h)
+            elif g == GEOMETRY_RECTD:
+                data = (x, y, w, h)
+            elif g == GEOMETRY_CIRCLE:
+                data = (x, y, float(rng.uniform(-40, 80)))
+            elif g == GEOMETRY_ELLIPSE:
+                data = (x, y, float(rng.uniform(-40, 80)), float(rng.uniform(-40, 80)))
+            elif g == GEOMETRY_ROUND_RECT:
+                data = (x, y, abs(w), abs(h), float(rng.uniform(-10, 40)), float(rng.uniform(0, 40)) if rng.random() < 0.9 else 0.0)
+            elif g == GEOMETRY_LINE:
+                data = (*pt(), *pt())
+            elif g == GEOMETRY_TRIANGLE:
+                data = (*pt(), *pt(), *pt())
+            elif g in (GEOMETRY_POLYLINEI, GEOMETRY_POLYGONI):
+                data = [(int(a), int(b)) for a, b in (pt() for _ in range(int(rng.integers(0, 9))))]
+            elif g in (GEOMETRY_POLYLINED, GEOMETRY_POLYGOND):
+                data = [pt() for _ in range(int(rng.integers(0, 9)))]
+            elif g in (GEOMETRY_ARRAY_VIEW_BOXI, GEOMETRY_ARRAY_VIEW_RECTI):
+                data = [tuple(int(v) for v in (*pt(), float(rng.uniform(-20, 60)), float(rng.uniform(-20, 60))))
+                        for _ in range(int(rng.integers(0, 5)))]
+                if g == GEOMETRY_ARRAY_VIEW_BOXI:
+                    data = [(a, b, a + c, b + d) for a, b, c, d in data]
+            elif g in (GEOMETRY_ARRAY_VIEW_BOXD, GEOMETRY_ARRAY_VIEW_RECTD):
+                data = [(*pt(), float(rng.uniform(-20, 60)), float(rng.uniform(-20, 60))) for _ in range(int(rng.integers(0, 5)))]
+                if g == GEOMETRY_ARRAY_VIEW_BOXD:
+                    data = [(a, b, a + c, b + d) for a, b, c, d in data]
+            else:
+                data = ()
+            sc.fill_geometry(g, data, _rand_rgba(rng) if rng.random() < 0.3 else None)
+    sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
     return sc

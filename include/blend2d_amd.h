@@ -39,6 +39,20 @@ enum {
 /* Pixel formats: BLFormat (core/format.h). */
 enum { BL_AMD_FORMAT_PRGB32 = 1, BL_AMD_FORMAT_XRGB32 = 2, BL_AMD_FORMAT_A8 = 3 };
 
+/* BLGeometryType values (core/geometry.h:40-86). */
+enum {
+  BL_AMD_GEOMETRY_NONE = 0, BL_AMD_GEOMETRY_BOXI = 1, BL_AMD_GEOMETRY_BOXD = 2, BL_AMD_GEOMETRY_RECTI = 3,
+  BL_AMD_GEOMETRY_RECTD = 4, BL_AMD_GEOMETRY_CIRCLE = 5, BL_AMD_GEOMETRY_ELLIPSE = 6, BL_AMD_GEOMETRY_ROUND_RECT = 7,
+  BL_AMD_GEOMETRY_ARC = 8, BL_AMD_GEOMETRY_CHORD = 9, BL_AMD_GEOMETRY_PIE = 10, BL_AMD_GEOMETRY_LINE = 11,
+  BL_AMD_GEOMETRY_TRIANGLE = 12, BL_AMD_GEOMETRY_POLYLINEI = 13, BL_AMD_GEOMETRY_POLYLINED = 14,
+  BL_AMD_GEOMETRY_POLYGONI = 15, BL_AMD_GEOMETRY_POLYGOND = 16, BL_AMD_GEOMETRY_ARRAY_VIEW_BOXI = 17,
+  BL_AMD_GEOMETRY_ARRAY_VIEW_BOXD = 18, BL_AMD_GEOMETRY_ARRAY_VIEW_RECTI = 19, BL_AMD_GEOMETRY_ARRAY_VIEW_RECTD = 20,
+  BL_AMD_GEOMETRY_PATH = 21, BL_AMD_GEOMETRY_MAX_VALUE = 21
+};
+
+/* BLArrayView<T> (core/api.h:1850-1853): pointer + element count. */
+typedef struct BLAmdArrayView { const void* data; size_t size; } BLAmdArrayView;
+
 /* ---- runtime --------------------------------------------------------- */
 
 /* Library version: major << 16 | minor << 8 | patch. */
@@ -48,13 +62,13 @@ int bl_amd_device_count(void);
 
 /* ---- images (BLImage, core/image.h) ---------------------------------- */
 
-/* bl_image_create (core/image.h:111): device-resident image. */
+/* bl_image_create (core/image.h:157): device-resident image. */
 BLAmdResult bl_amd_image_create(BLAmdImage** out, int w, int h, uint32_t format);
-/* bl_image_destroy. */
+/* bl_image_destroy (core/image.h:152). */
 BLAmdResult bl_amd_image_destroy(BLAmdImage* img);
-/* Copies host pixels into the image (bl_image_make_mutable + write). */
+/* Copies host pixels into the image (bl_image_make_mutable, core/image.h:160, + write). */
 BLAmdResult bl_amd_image_upload(BLAmdImage* img, const void* pixels, intptr_t stride);
-/* Copies the image to host memory (bl_image_get_data + read). */
+/* Copies the image to host memory (bl_image_get_data, core/image.h:159, + read). */
 BLAmdResult bl_amd_image_download(const BLAmdImage* img, void* pixels, intptr_t stride);
 /* Fills every pixel with a raw value (A8 uses the low byte). */
 BLAmdResult bl_amd_image_fill_raw(BLAmdImage* img, uint32_t value);
@@ -62,15 +76,16 @@ BLAmdResult bl_amd_image_fill_raw(BLAmdImage* img, uint32_t value);
 BLAmdResult bl_amd_image_get_data(const BLAmdImage* img, int* w, int* h, uint32_t* format,
                                   void** device_pixels, intptr_t* stride);
 
-/* ---- rendering context (BLContext, core/context.h:631-760) ------------- */
+/* ---- rendering context (BLContext, core/context.h:631-767) ------------- */
 
+/* bl_context_init (context.h:631) / bl_context_destroy (context.h:635). */
 BLAmdResult bl_amd_context_create(BLAmdContext** out);
 BLAmdResult bl_amd_context_destroy(BLAmdContext* ctx);
-/* bl_context_begin (context.h:634): attach a target, reset the state. */
+/* bl_context_begin (context.h:646): attach a target, reset the state. */
 BLAmdResult bl_amd_context_begin(BLAmdContext* ctx, BLAmdImage* target);
-/* bl_context_end (context.h:636): flush synchronously and detach. */
+/* bl_context_end (context.h:647): flush synchronously and detach. */
 BLAmdResult bl_amd_context_end(BLAmdContext* ctx);
-/* bl_context_flush (context.h:637); flags: BL_CONTEXT_FLUSH_SYNC = 0x80000000. */
+/* bl_context_flush (context.h:649); flags: BL_CONTEXT_FLUSH_SYNC = 0x80000000 (context.h:107). */
 BLAmdResult bl_amd_context_flush(BLAmdContext* ctx, uint32_t flags);
 
 /* bl_context_set_comp_op (context.h:697): BLCompOp values; SRC_OVER, SRC_COPY,
@@ -80,14 +95,20 @@ BLAmdResult bl_amd_context_set_comp_op(BLAmdContext* ctx, uint32_t comp_op);
 BLAmdResult bl_amd_context_set_fill_rule(BLAmdContext* ctx, uint32_t fill_rule);
 /* bl_context_set_global_alpha (context.h:694). */
 BLAmdResult bl_amd_context_set_global_alpha(BLAmdContext* ctx, double alpha);
-/* bl_context_apply_transform_op(BL_TRANSFORM_OP_ASSIGN) (context.h:468 virt). */
+/* bl_context_apply_transform_op(BL_TRANSFORM_OP_ASSIGN) (context.h:658). */
 BLAmdResult bl_amd_context_set_transform(BLAmdContext* ctx, const double m[6]);
 /* bl_context_set_hint(BL_CONTEXT_HINT_PATTERN_QUALITY) (context.h:408-422):
  * 0 nearest, 1 bilinear (default). */
 BLAmdResult bl_amd_context_set_pattern_quality(BLAmdContext* ctx, uint32_t quality);
 
-/* bl_context_set_fill_style_rgba32 (context.h:673). */
+/* bl_context_set_fill_style_rgba32 (context.h:674). */
 BLAmdResult bl_amd_context_set_fill_style_rgba32(BLAmdContext* ctx, uint32_t rgba32);
+/* bl_context_set_fill_style_rgba64 (context.h:675). */
+BLAmdResult bl_amd_context_set_fill_style_rgba64(BLAmdContext* ctx, uint64_t rgba64);
+/* bl_context_set_fill_style_rgba (context.h:673): BLRgba {r, g, b, a} floats. */
+BLAmdResult bl_amd_context_set_fill_style_rgba(BLAmdContext* ctx, const float rgba[4]);
+/* bl_context_set_fill_alpha (context.h:678). */
+BLAmdResult bl_amd_context_set_fill_alpha(BLAmdContext* ctx, double alpha);
 /* bl_context_set_fill_style(BLGradient) (context.h:671) with the gradient
  * passed by value: type 0 linear [x0 y0 x1 y1], 1 radial [x0 y0 x1 y1 r0 r1],
  * 2 conic [x0 y0 angle repeat]; extend 0 pad, 1 repeat, 2 reflect; stops as
@@ -113,6 +134,35 @@ BLAmdResult bl_amd_context_fill_rect_d(BLAmdContext* ctx, double x, double y, do
 BLAmdResult bl_amd_context_fill_rect_i(BLAmdContext* ctx, int x, int y, int w, int h);
 /* bl_context_fill_all (context.h:735). */
 BLAmdResult bl_amd_context_fill_all(BLAmdContext* ctx);
+
+/* Explicit-colour fills: the colour replaces the fill style for this call
+ * only (BL_CONTEXT_RESOLVE_EXPLICIT_SOLID_OP, rastercontext.cpp:1258-1275). */
+/* bl_context_fill_all_rgba32 (context.h:736). */
+BLAmdResult bl_amd_context_fill_all_rgba32(BLAmdContext* ctx, uint32_t rgba32);
+/* bl_context_fill_rect_i_rgba32 (context.h:741). */
+BLAmdResult bl_amd_context_fill_rect_i_rgba32(BLAmdContext* ctx, int x, int y, int w, int h, uint32_t rgba32);
+/* bl_context_fill_rect_d_rgba32 (context.h:746). */
+BLAmdResult bl_amd_context_fill_rect_d_rgba32(BLAmdContext* ctx, double x, double y, double w, double h, uint32_t rgba32);
+/* bl_context_fill_path_d_rgba32 (context.h:751). */
+BLAmdResult bl_amd_context_fill_path_d_rgba32(BLAmdContext* ctx, const double* origin, const uint8_t* cmds,
+                                              const double* vertices, size_t n, uint32_t rgba32);
+
+/* bl_context_fill_geometry (context.h:755) / _rgba32 (context.h:756).
+ * `data` points to the reference's geometry struct for `type` (same memory
+ * layout: BLBoxI {int x0,y0,x1,y1}, BLBox {double ...}, BLRectI {x,y,w,h},
+ * BLRect, BLCircle {cx,cy,r}, BLEllipse {cx,cy,rx,ry}, BLRoundRect
+ * {x,y,w,h,rx,ry}, BLLine {x0,y0,x1,y1}, BLTriangle {x0,y0,x1,y1,x2,y2},
+ * BLArrayView {data, size} of BLPointI / BLPoint / boxes / rects).  ARC,
+ * CHORD and PIE report BL_ERROR_NOT_IMPLEMENTED; paths go through
+ * bl_amd_context_fill_path_d. */
+BLAmdResult bl_amd_context_fill_geometry(BLAmdContext* ctx, uint32_t type, const void* data);
+BLAmdResult bl_amd_context_fill_geometry_rgba32(BLAmdContext* ctx, uint32_t type, const void* data, uint32_t rgba32);
+
+/* bl_context_clear_all (context.h:731), clear_rect_i (:732), clear_rect_d
+ * (:733): SRC_COPY of transparent (opaque black on XRGB32) at full alpha. */
+BLAmdResult bl_amd_context_clear_all(BLAmdContext* ctx);
+BLAmdResult bl_amd_context_clear_rect_i(BLAmdContext* ctx, int x, int y, int w, int h);
+BLAmdResult bl_amd_context_clear_rect_d(BLAmdContext* ctx, double x, double y, double w, double h);
 
 /* ---- statistics and replay (no counterpart in the reference) ----------- */
 

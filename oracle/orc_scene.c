@@ -30,7 +30,7 @@ static int rd(Rd* r, void* out, size_t n) {
 static size_t pad8(size_t n) { return (n + 7u) & ~(size_t)7u; }
 
 typedef struct {
-  uint32_t type;      /* 0 solid, 1 gradient, 2 pattern */
+  uint32_t type;      /* 0 solid, 1 gradient, 2 pattern, 3 disabled (NaN colour), 4 premultiplied solid */
   uint32_t rgba32;
   SceneGradient g;
   OrcStop stops[64];
@@ -45,7 +45,11 @@ typedef struct {
   uint32_t comp_op, fill_rule, pattern_quality;
   OrcMat user;
   double global_alpha;
+  double fill_alpha;
   Style style;
+  uint32_t mode;          /* 0 fill with the style, 1 explicit colour, 2 clear */
+  uint32_t explicit_rgba32;
+  uint32_t call_all;      /* fill_all / clear_all: an invalid transform is no NOP */
   SceneImage img_info[16];
   const uint8_t* img_data[16];
   uint32_t n_images;
@@ -55,8 +59,11 @@ typedef struct {
 
 static void note(Ctx* c, uint32_t err) { if (err && !c->first_error) c->first_error = err; }
 
+/* styleAlphaI of the fill slot (rastercontext.cpp:1920-1939, :2000-2029);
+ * clear calls use the full alpha (rastercontext.cpp:1385-1391). */
 static uint32_t alpha_i(const Ctx* c) {
-  return (uint32_t)orc_round_to_int(c->global_alpha * 255.0 * 1.0);
+  if (c->mode == 2) return 255u;
+  return (uint32_t)orc_round_to_int(c->global_alpha * 255.0 * c->fill_alpha);
 }
 
 /* core/gradient.cpp:751-800 (add_stop keeps stops sorted; a third stop at an
@@ -96,15 +103,43 @@ static uint32_t resolve(Ctx* c, OrcFetch* f, uint32_t* lut_buf, OrcPipe* pipe) {
   int solid = 0;
   uint32_t lut_size = 0;
 
+  /* kNoUserTransform is part of every geometry call's NOP flags, not of
+   * fill_all / clear_all (raster/rasterdefs_p.h:114-131) */
+  if (!c->call_all && orc_mat_type(&c->user) == ORC_TT_INVALID) return 1;
+
+  if (c->mode == 2) {
+    /* resolve_clear_op (rastercontext.cpp:1232-1242) */
+    uint32_t pd, ps, sid;
+    uint32_t op = orc_simplify(ORC_OP_SRC_COPY, c->canvas.format, ORC_FMT_PRGB32, &pd, &ps, &sid);
+    if (op == ORC_OP_NOP) return 1;
+    if (op != ORC_OP_SRC_COPY) return ERR_NOT_IMPLEMENTED;
+    pipe->op = op;
+    pipe->dst_fmt = pd;
+    pipe->src_fmt = ps;
+    pipe->fetch = f;
+    f->type = ORC_FETCH_SOLID;
+    f->solid_prgb32 = sid == ORC_SOLID_OPAQUE_BLACK ? 0xFF000000u : sid == ORC_SOLID_OPAQUE_WHITE ? 0xFFFFFFFFu : 0u;
+    return 0;
+  }
+
   if (alpha_i(c) == 0) return 1;
 
+  Style tmp;
   Style* st = &c->style;
+  if (c->mode == 1) {
+    memset(&tmp, 0, sizeof(tmp));
+    tmp.type = 0;
+    tmp.rgba32 = c->explicit_rgba32;
+    st = &tmp;
+  }
   OrcMat ft = st->xform;  /* final_transform (user x identity meta) when the style was set */
-  if (st->type == 0) {
-    uint32_t v = st->rgba32;
-    src_fmt = v == 0 ? ORC_FMT_ZERO32 : (v >= 0xFF000000u ? ORC_FMT_FRGB32 : ORC_FMT_PRGB32);
+  if (st->type == 3) return 1;
+  if (st->type == 0 || st->type == 4) {
+    uint32_t v = st->type == 0 ? orc_premultiply(st->rgba32) : st->rgba32;
+    uint32_t fv = st->type == 0 ? st->rgba32 : v;
+    src_fmt = fv == 0 ? ORC_FMT_ZERO32 : (fv >= 0xFF000000u ? ORC_FMT_FRGB32 : ORC_FMT_PRGB32);
     f->type = ORC_FETCH_SOLID;
-    f->solid_prgb32 = orc_premultiply(v);
+    f->solid_prgb32 = v;
   }
   else if (st->type == 1) {
     if (st->n_stops == 0) return 1;
@@ -194,20 +229,32 @@ static void fill_edges(Ctx* c, uint32_t fill_rule) {
   OrcFetch f;
   OrcPipe p;
   static uint32_t lut[2048];
+  /* the pipeline (and its error) is looked up only for non-empty edges
+   * (rastercontext.cpp:2734-2749) */
+  if (c->edges.n == 0 || c->edges.bbox_y0 >= c->edges.bbox_y1) return;
   uint32_t r = resolve(c, &f, lut, &p);
   if (r > 1) { note(c, r); return; }
   if (r == 1) return;
   orc_fill_analytic(&c->canvas, &c->edges, &p, alpha_i(c), fill_rule);
 }
 
-static void fill_path(Ctx* c, const uint8_t* cmds, const OrcPt* vtx, size_t n) {
+static void fill_path_origin(Ctx* c, const uint8_t* cmds, const OrcPt* vtx, size_t n, const double* origin) {
   OrcMat m; int tt; OrcBox clip;
   fixed_transform(c, &m, &tt);
+  if (origin) {
+    /* fill_unclipped_path_with_origin (rastercontext.cpp:2812-2821) */
+    double ox = origin[0] * m.m00 + origin[1] * m.m10 + m.m20;
+    double oy = origin[0] * m.m01 + origin[1] * m.m11 + m.m21;
+    m.m20 = ox;
+    m.m21 = oy;
+  }
   clip_fixed(c, &clip);
   orc_edges_clear(&c->edges);
   orc_edges_add_path(&c->edges, cmds, vtx, n, &m, tt, &clip, kToleranceSq);
   fill_edges(c, c->fill_rule);
 }
+
+static void fill_path(Ctx* c, const uint8_t* cmds, const OrcPt* vtx, size_t n) { fill_path_origin(c, cmds, vtx, n, NULL); }
 
 static void fill_poly(Ctx* c, const OrcPt* vtx, size_t n, uint32_t rule) {
   if (n < 3) return;
@@ -286,6 +333,141 @@ static void fill_rect_i(Ctx* c, int x, int y, int w, int h) {
   fill_box_i(c, (int)x0, (int)y0, (int)x1, (int)y1);
 }
 
+/* Path builder for BLPath::add_geometry (core/path.cpp:1396-1640),
+ * clockwise direction. */
+typedef struct { uint8_t* cmd; OrcPt* vtx; size_t n, cap; } TmpPath;
+
+static void tp_push(TmpPath* p, uint8_t cmd, double x, double y) {
+  if (p->n == p->cap) {
+    p->cap = p->cap ? p->cap * 2 : 64;
+    p->cmd = (uint8_t*)realloc(p->cmd, p->cap);
+    p->vtx = (OrcPt*)realloc(p->vtx, p->cap * sizeof(OrcPt));
+  }
+  p->cmd[p->n] = cmd;
+  p->vtx[p->n].x = x;
+  p->vtx[p->n].y = y;
+  p->n++;
+}
+static void tp_cubic(TmpPath* p, double x1, double y1, double x2, double y2, double x3, double y3) {
+  tp_push(p, 4, x1, y1); tp_push(p, 4, x2, y2); tp_push(p, 1, x3, y3);
+}
+static void tp_close(TmpPath* p) { tp_push(p, 5, NAN, NAN); }
+static void tp_box(TmpPath* p, double x0, double y0, double x1, double y1) {
+  tp_push(p, 0, x0, y0); tp_push(p, 1, x1, y0); tp_push(p, 1, x1, y1); tp_push(p, 1, x0, y1); tp_close(p);
+}
+
+static const double kKappa = 0.55228474983079339840;
+
+/* fill_unclipped_geometry (rastercontext.cpp:2925-2980) */
+static void fill_geometry(Ctx* c, uint32_t type, const uint8_t* data, uint32_t count) {
+  switch (type) {
+    case 0: return;
+    case 1: { int32_t b[4]; memcpy(b, data, 16); fill_box_d(c, b[0], b[1], b[2], b[3]); return; }
+    case 2: { double b[4]; memcpy(b, data, 32); fill_box_d(c, b[0], b[1], b[2], b[3]); return; }
+    case 3: { int32_t b[4]; memcpy(b, data, 16); fill_rect_i(c, b[0], b[1], b[2], b[3]); return; }
+    case 4: { double b[4]; memcpy(b, data, 32); fill_box_d(c, b[0], b[1], b[0] + b[2], b[1] + b[3]); return; }
+    case 13: case 14: case 15: case 16: {
+      if (count < 3) return;
+      OrcPt* v = (OrcPt*)malloc(count * sizeof(OrcPt));
+      for (uint32_t i = 0; i < count; i++) {
+        if (type == 13 || type == 15) { int32_t q[2]; memcpy(q, data + 8 * i, 8); v[i].x = q[0]; v[i].y = q[1]; }
+        else memcpy(&v[i], data + 16 * i, 16);
+      }
+      fill_poly(c, v, count, c->fill_rule);
+      free(v);
+      return;
+    }
+    default: break;
+  }
+  TmpPath p;
+  memset(&p, 0, sizeof(p));
+  switch (type) {
+    case 5: case 6: {
+      double e[4];
+      memcpy(e, data, type == 5 ? 24 : 32);
+      double x0 = e[0], y0 = e[1], rx = e[2], ry = type == 5 ? fabs(e[2]) : e[3];
+      double kx = rx * kKappa, ky = ry * kKappa;
+      tp_push(&p, 0, x0 + rx, y0);
+      tp_cubic(&p, x0 + rx, y0 + ky, x0 + kx, y0 + ry, x0, y0 + ry);
+      tp_cubic(&p, x0 - kx, y0 + ry, x0 - rx, y0 + ky, x0 - rx, y0);
+      tp_cubic(&p, x0 - rx, y0 - ky, x0 - kx, y0 - ry, x0, y0 - ry);
+      tp_cubic(&p, x0 + kx, y0 - ry, x0 + rx, y0 - ky, x0 + rx, y0);
+      tp_close(&p);
+      break;
+    }
+    case 7: {
+      double q[6];
+      memcpy(q, data, 48);
+      double x0 = q[0], y0 = q[1], x1 = q[0] + q[2], y1 = q[1] + q[3];
+      double wh = q[2] * 0.5, hh = q[3] * 0.5;
+      double rx = fabs(q[4]) < wh ? fabs(q[4]) : wh;
+      double ry = fabs(q[5]) < hh ? fabs(q[5]) : hh;
+      if (!(rx > 1e-14 && ry > 1e-14)) { tp_box(&p, x0, y0, x1, y1); break; }
+      double kx = rx * (1.0 - kKappa), ky = ry * (1.0 - kKappa);
+      tp_push(&p, 0, x0 + rx, y0);
+      tp_push(&p, 1, x1 - rx, y0);
+      tp_cubic(&p, x1 - kx, y0, x1, y0 + ky, x1, y0 + ry);
+      tp_push(&p, 1, x1, y1 - ry);
+      tp_cubic(&p, x1, y1 - ky, x1 - kx, y1, x1 - rx, y1);
+      tp_push(&p, 1, x0 + rx, y1);
+      tp_cubic(&p, x0 + kx, y1, x0, y1 - ky, x0, y1 - ry);
+      tp_push(&p, 1, x0, y0 + ry);
+      tp_cubic(&p, x0, y0 + ky, x0 + kx, y0, x0 + rx, y0);
+      tp_close(&p);
+      break;
+    }
+    case 11: {
+      double l[4]; memcpy(l, data, 32);
+      tp_push(&p, 0, l[0], l[1]); tp_push(&p, 1, l[2], l[3]);
+      break;
+    }
+    case 12: {
+      double t[6]; memcpy(t, data, 48);
+      tp_push(&p, 0, t[0], t[1]); tp_push(&p, 1, t[2], t[3]); tp_push(&p, 1, t[4], t[5]); tp_close(&p);
+      break;
+    }
+    case 17: case 18: case 19: case 20:
+      /* invalid items are skipped (geometry/commons_p.h:42-56) */
+      for (uint32_t i = 0; i < count; i++) {
+        if (type == 17) {
+          int32_t b[4]; memcpy(b, data + 16 * i, 16);
+          if (b[0] < b[2] && b[1] < b[3]) tp_box(&p, b[0], b[1], b[2], b[3]);
+        }
+        else if (type == 18) {
+          double b[4]; memcpy(b, data + 32 * i, 32);
+          if (b[0] < b[2] && b[1] < b[3]) tp_box(&p, b[0], b[1], b[2], b[3]);
+        }
+        else if (type == 19) {
+          int32_t b[4]; memcpy(b, data + 16 * i, 16);
+          int64_t x1 = (int64_t)b[0] + b[2], y1 = (int64_t)b[1] + b[3];
+          if (x1 != (int32_t)x1 || y1 != (int32_t)y1 || !(b[0] < x1 && b[1] < y1)) continue;
+          double x0 = b[0], y0 = b[1];
+          tp_box(&p, x0, y0, (double)b[2] + x0, (double)b[3] + y0);
+        }
+        else {
+          double b[4]; memcpy(b, data + 32 * i, 32);
+          double x1 = b[0] + b[2], y1 = b[1] + b[3];
+          if (b[0] < x1 && b[1] < y1) tp_box(&p, b[0], b[1], b[2] + b[0], b[3] + b[1]);
+        }
+      }
+      break;
+    case 8: case 9: case 10: {
+      /* arcs need the reference's arc approximation, which is not restated */
+      OrcFetch f; OrcPipe pp; static uint32_t lut[2048];
+      uint32_t r = resolve(c, &f, lut, &pp);
+      if (r > 1) note(c, r);
+      else if (r == 0) note(c, ERR_NOT_IMPLEMENTED);
+      return;
+    }
+    default:
+      note(c, ERR_INVALID_VALUE);
+      return;
+  }
+  if (p.n) fill_path(c, p.cmd, p.vtx, p.n);
+  free(p.cmd);
+  free(p.vtx);
+}
+
 uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_size) {
   Rd r = { scene, scene + size };
   SceneHeader hdr;
@@ -306,6 +488,7 @@ uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
   c.pattern_quality = 1;
   c.user.m00 = 1.0; c.user.m11 = 1.0;
   c.global_alpha = 1.0;
+  c.fill_alpha = 1.0;
   c.style.type = 0;
   c.style.rgba32 = 0xFF000000u;
   orc_edges_init(&c.edges);
@@ -395,8 +578,103 @@ uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
         break;
       }
       case SCENE_OP_FILL_ALL:
+        c.call_all = 1;
         fill_box_i(&c, 0, 0, (int)c.canvas.w, (int)c.canvas.h);
+        c.call_all = 0;
         break;
+      case SCENE_OP_SET_FILL_ALPHA: {
+        double a;
+        memcpy(&a, pl, 8);
+        if (isnan(a)) { note(&c, ERR_INVALID_VALUE); break; }
+        c.fill_alpha = orc_clamp(a, 0.0, 1.0);
+        break;
+      }
+      case SCENE_OP_SET_STYLE_RGBA64: {
+        /* rgba32FromRgba64 (core/rgba_p.h:67-74): the high byte of each channel */
+        uint64_t v;
+        memcpy(&v, pl, 8);
+        c.style.type = 0;
+        c.style.rgba32 = (uint32_t)(((v >> 56) & 0xFFu) << 24 | ((v >> 40) & 0xFFu) << 16 | ((v >> 24) & 0xFFu) << 8 | ((v >> 8) & 0xFFu));
+        break;
+      }
+      case SCENE_OP_SET_STYLE_RGBA: {
+        /* set_style_rgba_impl (rastercontext.cpp:616-655) */
+        float f[4];
+        memcpy(f, pl, 16);
+        if (isnan(f[0]) || isnan(f[1]) || isnan(f[2]) || isnan(f[3])) { c.style.type = 3; break; }
+        for (int k = 0; k < 4; k++) f[k] = f[k] < 0.0f ? 0.0f : (f[k] > 1.0f ? 1.0f : f[k]);
+        float as = f[3] * 255.0f;
+        uint32_t rr = (uint32_t)lrintf(f[0] * as), gg = (uint32_t)lrintf(f[1] * as), bb = (uint32_t)lrintf(f[2] * as);
+        uint32_t aa = (uint32_t)lrintf(as);
+        c.style.type = 4;
+        c.style.rgba32 = (aa << 24) | (rr << 16) | (gg << 8) | bb;
+        break;
+      }
+      case SCENE_OP_CLEAR_ALL:
+        c.mode = 2; c.call_all = 1;
+        fill_box_i(&c, 0, 0, (int)c.canvas.w, (int)c.canvas.h);
+        c.mode = 0; c.call_all = 0;
+        break;
+      case SCENE_OP_CLEAR_RECT_I: {
+        int32_t v[4];
+        memcpy(v, pl, 16);
+        c.mode = 2;
+        fill_rect_i(&c, v[0], v[1], v[2], v[3]);
+        c.mode = 0;
+        break;
+      }
+      case SCENE_OP_CLEAR_RECT_D: {
+        double d[4];
+        memcpy(d, pl, 32);
+        c.mode = 2;
+        fill_box_d(&c, d[0], d[1], d[0] + d[2], d[1] + d[3]);
+        c.mode = 0;
+        break;
+      }
+      case SCENE_OP_FILL_ALL_RGBA32:
+        memcpy(&c.explicit_rgba32, pl, 4);
+        c.mode = 1; c.call_all = 1;
+        fill_box_i(&c, 0, 0, (int)c.canvas.w, (int)c.canvas.h);
+        c.mode = 0; c.call_all = 0;
+        break;
+      case SCENE_OP_FILL_RECT_I_RGBA32: {
+        int32_t v[4];
+        memcpy(v, pl, 16);
+        memcpy(&c.explicit_rgba32, pl + 16, 4);
+        c.mode = 1;
+        fill_rect_i(&c, v[0], v[1], v[2], v[3]);
+        c.mode = 0;
+        break;
+      }
+      case SCENE_OP_FILL_RECT_D_RGBA32: {
+        double d[4];
+        memcpy(d, pl, 32);
+        memcpy(&c.explicit_rgba32, pl + 32, 4);
+        c.mode = 1;
+        fill_box_d(&c, d[0], d[1], d[0] + d[2], d[1] + d[3]);
+        c.mode = 0;
+        break;
+      }
+      case SCENE_OP_FILL_GEOMETRY: {
+        SceneGeometry g;
+        memcpy(&g, pl, sizeof(g));
+        if (g.has_rgba32) { c.mode = 1; c.explicit_rgba32 = g.rgba32; }
+        fill_geometry(&c, g.type, pl + sizeof(g), g.count);
+        c.mode = 0;
+        break;
+      }
+      case SCENE_OP_FILL_PATH_EX: {
+        ScenePathEx h;
+        memcpy(&h, pl, sizeof(h));
+        const uint8_t* cmds = pl + sizeof(h);
+        OrcPt* vtx = (OrcPt*)malloc((size_t)h.n * sizeof(OrcPt) + 1);
+        memcpy(vtx, cmds + pad8(h.n), (size_t)h.n * sizeof(OrcPt));
+        if (h.flags & 2u) { c.mode = 1; c.explicit_rgba32 = h.rgba32; }
+        if (h.n) fill_path_origin(&c, cmds, vtx, h.n, (h.flags & 1u) ? h.origin : NULL);
+        c.mode = 0;
+        free(vtx);
+        break;
+      }
       default:
         break;
     }

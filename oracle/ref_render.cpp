@@ -43,6 +43,13 @@ struct Call {
   std::vector<BLPoint> poly;
   BLGradient gradient;
   BLPattern pattern;
+  uint64_t u64 = 0;
+  float f[4] = {};
+  int32_t i[4] = {};
+  uint32_t rgba32 = 0;
+  uint32_t flags = 0;
+  uint32_t gtype = 0;
+  std::vector<uint8_t> blob;   // geometry data
 };
 
 }  // namespace
@@ -99,8 +106,58 @@ uint32_t ref_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
         std::memcpy(call.d, payload, 48);
         break;
       case SCENE_OP_SET_GLOBAL_ALPHA:
+      case SCENE_OP_SET_FILL_ALPHA:
         std::memcpy(call.d, payload, 8);
         break;
+      case SCENE_OP_SET_STYLE_RGBA64:
+        std::memcpy(&call.u64, payload, 8);
+        break;
+      case SCENE_OP_SET_STYLE_RGBA:
+        std::memcpy(call.f, payload, 16);
+        break;
+      case SCENE_OP_CLEAR_RECT_I:
+        std::memcpy(call.i, payload, 16);
+        break;
+      case SCENE_OP_CLEAR_RECT_D:
+        std::memcpy(call.d, payload, 32);
+        break;
+      case SCENE_OP_FILL_ALL_RGBA32:
+        std::memcpy(&call.rgba32, payload, 4);
+        break;
+      case SCENE_OP_FILL_RECT_I_RGBA32:
+        std::memcpy(call.i, payload, 16);
+        std::memcpy(&call.rgba32, payload + 16, 4);
+        break;
+      case SCENE_OP_FILL_RECT_D_RGBA32:
+        std::memcpy(call.d, payload, 32);
+        std::memcpy(&call.rgba32, payload + 32, 4);
+        break;
+      case SCENE_OP_FILL_GEOMETRY: {
+        SceneGeometry g;
+        std::memcpy(&g, payload, sizeof(g));
+        call.gtype = g.type;
+        call.flags = g.has_rgba32;
+        call.rgba32 = g.rgba32;
+        call.u0 = g.count;
+        call.blob.assign(payload + sizeof(g), payload + c.size);
+        break;
+      }
+      case SCENE_OP_FILL_PATH_EX: {
+        ScenePathEx h;
+        std::memcpy(&h, payload, sizeof(h));
+        call.flags = h.flags;
+        call.rgba32 = h.rgba32;
+        call.d[0] = h.origin[0];
+        call.d[1] = h.origin[1];
+        const uint8_t* cmds = payload + sizeof(h);
+        const uint8_t* vtx = cmds + pad8(h.n);
+        uint8_t* cmd_out;
+        BLPoint* vtx_out;
+        call.path.modify_op(BL_MODIFY_OP_ASSIGN_FIT, h.n, &cmd_out, &vtx_out);
+        std::memcpy(cmd_out, cmds, h.n);
+        std::memcpy(vtx_out, vtx, size_t(h.n) * sizeof(BLPoint));
+        break;
+      }
       case SCENE_OP_FILL_RECT_D:
         std::memcpy(call.d, payload, 32);
         break;
@@ -203,6 +260,38 @@ uint32_t ref_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
       case SCENE_OP_FILL_RECT_D: note(ctx.fill_rect(BLRect(call.d[0], call.d[1], call.d[2], call.d[3]))); break;
       case SCENE_OP_FILL_RECT_I: note(ctx.fill_rect(BLRectI(int(call.d[0]), int(call.d[1]), int(call.d[2]), int(call.d[3])))); break;
       case SCENE_OP_FILL_ALL: note(ctx.fill_all()); break;
+      case SCENE_OP_SET_FILL_ALPHA: note(ctx.set_fill_alpha(call.d[0])); break;
+      case SCENE_OP_SET_STYLE_RGBA64: note(ctx.set_fill_style(BLRgba64(call.u64))); break;
+      case SCENE_OP_SET_STYLE_RGBA: note(ctx.set_fill_style(BLRgba(call.f[0], call.f[1], call.f[2], call.f[3]))); break;
+      case SCENE_OP_CLEAR_ALL: note(ctx.clear_all()); break;
+      case SCENE_OP_CLEAR_RECT_I: note(ctx.clear_rect(BLRectI(call.i[0], call.i[1], call.i[2], call.i[3]))); break;
+      case SCENE_OP_CLEAR_RECT_D: note(ctx.clear_rect(BLRect(call.d[0], call.d[1], call.d[2], call.d[3]))); break;
+      case SCENE_OP_FILL_ALL_RGBA32: note(ctx.fill_all(BLRgba32(call.rgba32))); break;
+      case SCENE_OP_FILL_RECT_I_RGBA32: note(ctx.fill_rect(BLRectI(call.i[0], call.i[1], call.i[2], call.i[3]), BLRgba32(call.rgba32))); break;
+      case SCENE_OP_FILL_RECT_D_RGBA32: note(ctx.fill_rect(BLRect(call.d[0], call.d[1], call.d[2], call.d[3]), BLRgba32(call.rgba32))); break;
+      case SCENE_OP_FILL_PATH_EX: {
+        BLPoint origin(call.d[0], call.d[1]);
+        if (!(call.flags & 1u)) origin.reset(0.0, 0.0);
+        if (call.flags & 2u) note(ctx.fill_path(origin, call.path, BLRgba32(call.rgba32)));
+        else note(ctx.fill_path(origin, call.path));
+        break;
+      }
+      case SCENE_OP_FILL_GEOMETRY: {
+        const void* data = call.blob.data();
+        BLArrayView<uint8_t> view;
+        bool is_array = (call.gtype >= BL_GEOMETRY_TYPE_POLYLINEI && call.gtype <= BL_GEOMETRY_TYPE_ARRAY_VIEW_RECTD);
+        if (is_array) {
+          view.data = call.blob.data();
+          view.size = call.u0;
+          data = &view;
+        }
+        BLContextCore* core = &ctx;
+        if (call.flags)
+          note(bl_context_fill_geometry_rgba32(core, BLGeometryType(call.gtype), data, call.rgba32));
+        else
+          note(bl_context_fill_geometry(core, BLGeometryType(call.gtype), data));
+        break;
+      }
       default: break;
     }
   }

@@ -58,8 +58,37 @@ enum SceneOp {
   SCENE_OP_FILL_POLYGON       = 10, /* u32 n, u32 pad, f64 vtx[2n] */
   SCENE_OP_FILL_RECT_D        = 11, /* f64 x, y, w, h */
   SCENE_OP_FILL_RECT_I        = 12, /* i32 x, y, w, h */
-  SCENE_OP_FILL_ALL           = 13  /* no payload */
+  SCENE_OP_FILL_ALL           = 13, /* no payload */
+  SCENE_OP_SET_FILL_ALPHA     = 14, /* f64 alpha */
+  SCENE_OP_SET_STYLE_RGBA64   = 15, /* u64 rgba64 (16 bits per channel, straight alpha) */
+  SCENE_OP_SET_STYLE_RGBA     = 16, /* f32 r, g, b, a */
+  SCENE_OP_CLEAR_ALL          = 17, /* no payload */
+  SCENE_OP_CLEAR_RECT_I       = 18, /* i32 x, y, w, h */
+  SCENE_OP_CLEAR_RECT_D       = 19, /* f64 x, y, w, h */
+  SCENE_OP_FILL_GEOMETRY      = 20, /* SceneGeometry + geometry data (see below) */
+  SCENE_OP_FILL_ALL_RGBA32    = 21, /* u32 rgba32, u32 pad */
+  SCENE_OP_FILL_RECT_I_RGBA32 = 22, /* i32 x, y, w, h, u32 rgba32, u32 pad */
+  SCENE_OP_FILL_RECT_D_RGBA32 = 23, /* f64 x, y, w, h, u32 rgba32, u32 pad */
+  SCENE_OP_FILL_PATH_EX       = 24  /* ScenePathEx, u8 cmds[n] padded to 8, f64 vtx[2n] */
 };
+
+/* FILL_GEOMETRY: BLGeometryType `type`; for array types (POLYGON*, POLYLINE*,
+ * ARRAY_VIEW_*) `count` elements follow (BLPointI / BLPoint / BLBoxI / BLBox /
+ * BLRectI / BLRect), otherwise the geometry struct itself. */
+typedef struct SceneGeometry {
+  uint32_t type;
+  uint32_t has_rgba32;    /* 1: explicit colour (fill_geometry_rgba32) */
+  uint32_t rgba32;
+  uint32_t count;
+} SceneGeometry;
+
+typedef struct ScenePathEx {
+  uint32_t n;
+  uint32_t flags;         /* bit 0: origin present, bit 1: explicit colour */
+  uint32_t rgba32;
+  uint32_t pad;
+  double origin[2];
+} ScenePathEx;
 
 typedef struct SceneGradient {
   uint32_t type;          /* 0 linear, 1 radial, 2 conic (BLGradientType) */

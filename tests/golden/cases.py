@@ -17,6 +17,8 @@ def cases():
             out[f"mixed_{fname}_s{seed}"] = lambda seed=seed, fmt=fmt: [S.mixed_scene(seed, 513, 513, 150, fmt)]
     for fmt, fname in ((S.FMT_PRGB32, "prgb32"), (S.FMT_XRGB32, "xrgb32"), (S.FMT_A8, "a8")):
         out[f"edge_cases_{fname}"] = lambda fmt=fmt: [S.edge_case_scene(fmt=fmt)]
+        for seed in (1, 2, 3):
+            out[f"frontend_{fname}_s{seed}"] = lambda seed=seed, fmt=fmt: [S.frontend_scene(seed, fmt=fmt)]
     return out
 
 

@@ -42,3 +42,11 @@ def test_wide_canvas_band_height():
 def test_worker_threads_do_not_change_pixels():
     # reference with worker threads (its own mt test: tolerance 0) vs oracle
     _same(S.mixed_scene(51, 513, 513, 150, S.FMT_PRGB32), threads=4)
+
+
+@pytest.mark.parametrize("seed", [61, 62, 63, 64])
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_XRGB32, S.FMT_A8])
+def test_frontend_calls(seed, fmt):
+    # fill_geometry, explicit colours, clears, fill alpha, 64-bit / float
+    # colours, origins, non-invertible transforms
+    _same(S.frontend_scene(seed, 300, 220, fmt, 150))
