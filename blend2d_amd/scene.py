@@ -394,30 +394,63 @@ def glyph_path(rng, size_w: float, size_h: float):
 
 def glyph_canvases(seed: int = 1, count: int = 4096, max_paths: int = 200):
     """`count` independent canvases, sides log-uniform 32-256 px, half A8 and
-    half PRGB32, each with 1-`max_paths` glyph-like quadratic contours,
-    non-zero, solid and linear-gradient fills, SRC_OVER."""
+    half PRGB32, each with 1-`max_paths` glyph-like quadratic contours
+    (20-80 segments around a wobbly ring, see glyph_path), non-zero, solid
+    and linear-gradient fills, SRC_OVER.  Generated per canvas with array
+    operations."""
     rng = np.random.default_rng(seed)
     scenes = []
+    two_pi = 2 * math.pi
     for i in range(count):
         w = int(round(2.0 ** rng.uniform(5, 8)))
         h = int(round(2.0 ** rng.uniform(5, 8)))
         fmt = FMT_A8 if i % 2 == 0 else FMT_PRGB32
         sc = Scene(w, h, fmt, 0)
         sc.set_fill_rule(0)
-        for _ in range(int(rng.integers(1, max_paths + 1))):
-            gw = float(rng.uniform(8, w))
-            gh = float(rng.uniform(8, h))
-            ox = float(rng.uniform(-gw * 0.2, w - gw * 0.8))
-            oy = float(rng.uniform(-gh * 0.2, h - gh * 0.8))
-            cm, v = glyph_path(rng, gw, gh)
-            v = v + np.array([ox, oy])
-            if rng.random() < 0.5:
-                sc.set_fill_style_rgba32(_rand_rgba(rng))
+        k = int(rng.integers(1, max_paths + 1))
+        gw = rng.uniform(8, w, k)
+        gh = rng.uniform(8, h, k)
+        ox = rng.uniform(-gw * 0.2, w - gw * 0.8)
+        oy = rng.uniform(-gh * 0.2, h - gh * 0.8)
+        ns = rng.integers(20, 81, k)
+        total = int(ns.sum())
+        pid = np.repeat(np.arange(k), ns)
+        ang = np.sort(rng.random(total) + pid) - pid          # sorted within each path
+        ang *= two_pi
+        rad = 0.25 + 0.25 * rng.random(total)
+        ctl_r = 0.2 + 0.3 * rng.random(total)
+        gwp, ghp = gw[pid], gh[pid]
+        cx, cy = gwp * 0.5 + ox[pid], ghp * 0.5 + oy[pid]
+        px = cx + np.cos(ang) * rad * gwp
+        py = cy + np.sin(ang) * rad * ghp
+        mid = ang + math.pi / ns[pid]
+        qx = cx + np.cos(mid) * ctl_r * gwp
+        qy = cy + np.sin(mid) * ctl_r * ghp
+        solid = rng.random(k) < 0.5
+        colors = rng.integers(0, 1 << 32, k, dtype=np.uint64)
+        gp = rng.random((k, 4))
+        starts = np.concatenate([[0], np.cumsum(ns)])
+        for j in range(k):
+            s0, s1 = starts[j], starts[j + 1]
+            n = s1 - s0
+            if solid[j]:
+                sc.set_fill_style_rgba32(int(colors[j]))
             else:
-                p0 = rng.random(2) * np.array([gw, gh]) + np.array([ox, oy])
-                p1 = rng.random(2) * np.array([gw, gh]) + np.array([ox, oy])
-                sc.set_fill_style_gradient(GRADIENT_LINEAR, [p0[0], p0[1], p1[0], p1[1]], _rand_stops(rng, 2, 4))
-            sc.fill_path(cm, v)
+                p0x, p0y = gp[j, 0] * gw[j] + ox[j], gp[j, 1] * gh[j] + oy[j]
+                p1x, p1y = gp[j, 2] * gw[j] + ox[j], gp[j, 3] * gh[j] + oy[j]
+                sc.set_fill_style_gradient(GRADIENT_LINEAR, [p0x, p0y, p1x, p1y], _rand_stops(rng, 2, 4))
+            cmds = np.empty(2 * n + 2, dtype=np.uint8)
+            cmds[0] = CMD_MOVE
+            cmds[1:-1:2] = CMD_QUAD
+            cmds[2:-1:2] = CMD_ON
+            cmds[-1] = CMD_CLOSE
+            vtx = np.empty((2 * n + 2, 2))
+            vtx[0, 0], vtx[0, 1] = px[s0], py[s0]
+            vtx[1:-1:2, 0], vtx[1:-1:2, 1] = qx[s0:s1], qy[s0:s1]
+            vtx[2:-2:2, 0], vtx[2:-2:2, 1] = px[s0 + 1:s1], py[s0 + 1:s1]
+            vtx[-2] = vtx[0]
+            vtx[-1] = vtx[0]
+            sc.fill_path(cmds, vtx)
         scenes.append(sc)
     return scenes
 
