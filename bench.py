@@ -80,15 +80,18 @@ def _workload(name: str, seed: int):
 
 
 class _Frame:
-    """One recorded frame (single context, or a batch of contexts) on the GPU."""
+    """One recorded frame on the GPU: a single context, a batch of contexts
+    (one set of kernel launches for all canvases), or -- `plain` -- every
+    canvas flushed and replayed on its own (one set of launches per canvas)."""
 
-    def __init__(self, scenes):
+    def __init__(self, scenes, plain: bool = False):
         import blend2d_amd as B
         self.B = B
         self.scenes = scenes
         self.keep = []
         self.targets = []
-        self.batch = B.BLBatch() if len(scenes) > 1 else None
+        self.plain = plain
+        self.batch = B.BLBatch() if len(scenes) > 1 and not plain else None
         self.errors = []
         for sc in scenes:
             images = []
@@ -110,19 +113,60 @@ class _Frame:
         if self.batch is not None:
             self.batch.flush()
         else:
-            self.keep[0][0].flush()
+            for ctx, _ in self.keep:
+                ctx.flush()
 
     def replay(self):
         if self.batch is not None:
             self.batch.replay()
         else:
-            self.keep[0][0].replay()
+            for ctx, _ in self.keep:
+                ctx.replay()
 
     def stats(self):
-        return self.batch.last_stats() if self.batch is not None else self.keep[0][0].last_stats()
+        if self.batch is not None:
+            return self.batch.last_stats()
+        if len(self.keep) == 1:
+            return self.keep[0][0].last_stats()
+        out = {}
+        for ctx, _ in self.keep:
+            for k, v in ctx.last_stats().items():
+                out[k] = out.get(k, 0) + v
+        return out
 
     def download(self):
         return [img.download() for img in self.targets]
+
+
+def _time_frame(frame, steps, warmup):
+    for _ in range(warmup):
+        frame.replay()
+    _sync()
+    render_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        frame.replay()
+        render_ms.append(frame.stats()["render_ms"])
+    _sync()
+    return time.perf_counter() - t0, render_ms
+
+
+def _batched_vs_plain(seed: int, steps: int):
+    """The 4096-canvas glyph workload rendered plain (one flush per canvas)
+    and batched (one flush for all canvases)."""
+    from blend2d_amd import scene as S
+    scenes = S.glyph_canvases(seed, 4096, 200)
+    out = {"workload": "4096 canvases 32-256 px (half A8, half PRGB32), 1-200 glyph contours each",
+           "paths": int(sum(sc.fill_count for sc in scenes))}
+    for mode in ("plain", "batched"):
+        frame = _Frame(scenes, plain=(mode == "plain"))
+        el, _ = _time_frame(frame, steps, 1)
+        px = frame.stats()["pixels"]
+        out[mode + "_mpix_per_s"] = round(px * steps / el / 1e6, 3)
+        out[mode + "_ms_per_frame"] = round(el * 1e3 / steps, 3)
+        del frame
+    out["batched_over_plain"] = round(out["batched_mpix_per_s"] / out["plain_mpix_per_s"], 3)
+    return out
 
 
 def _render_bytes(st, scenes):
@@ -186,6 +230,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-batching", action="store_true", help="skip the 4096-canvas batched-vs-plain measurement")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -203,16 +248,7 @@ def main():
     st0 = frame.stats()
     print(f"# workload {args.workload}: {st0}", file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
-        frame.replay()
-    _sync()
-    render_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        frame.replay()
-        render_ms.append(frame.stats()["render_ms"])
-    _sync()
-    el = time.perf_counter() - t0
+    el, render_ms = _time_frame(frame, args.steps, args.warmup)
     st = frame.stats()
 
     pixels = st["pixels"]
@@ -255,6 +291,9 @@ def main():
     }
     if any(frame.errors):
         line["errors"] = frame.errors
+    if not args.no_batching:
+        print("# batched vs plain (4096 canvases)", file=sys.stderr, flush=True)
+        line["batched_vs_plain"] = _batched_vs_plain(args.seed, 3)
     if not args.no_cpu:
         cpu, parity = _cpu_baseline(scenes, first, pixels, args.cpu_seconds)
         line["cpu_baseline"] = cpu

@@ -45,7 +45,12 @@ SYMBOLS = [
     "bl_amd_context_set_fill_style_rgba64", "bl_amd_context_set_fill_style_rgba", "bl_amd_context_set_fill_alpha",
     "bl_amd_batch_create", "bl_amd_batch_destroy", "bl_amd_batch_add", "bl_amd_batch_clear", "bl_amd_batch_flush",
     "bl_amd_batch_replay", "bl_amd_batch_get_stats",
+    "bl_amd_set_sizing", "bl_amd_cost_model_set", "bl_amd_context_collect_costs",
+    "bl_amd_cost_model_forward", "bl_amd_cost_model_grad", "bl_amd_cost_model_train",
 ]
+
+SIZING_EXACT, SIZING_LEARNED, SIZING_HISTORY = 0, 1, 2
+COST_FEATURES, COST_OUTPUTS, COST_PARAMS = 16, 2, 5378
 
 
 class BLAmdStats(ctypes.Structure):
@@ -53,6 +58,9 @@ class BLAmdStats(ctypes.Structure):
     _fields_ = [("commands", ctypes.c_uint64), ("sources", ctypes.c_uint64), ("segments", ctypes.c_uint64),
                 ("binned", ctypes.c_uint64), ("tiles", ctypes.c_uint64), ("pixels", ctypes.c_uint64),
                 ("frame_bytes", ctypes.c_uint64), ("lut_bytes", ctypes.c_uint64), ("image_bytes", ctypes.c_uint64),
+                ("pred_segs", ctypes.c_uint64), ("pred_bins", ctypes.c_uint64),
+                ("sizing", ctypes.c_uint32), ("fallback", ctypes.c_uint32), ("syncs", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32),
                 ("render_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
 
     def as_dict(self):
@@ -165,6 +173,12 @@ def lib():
         "bl_amd_batch_flush": ([P], U32),
         "bl_amd_batch_replay": ([P], U32),
         "bl_amd_batch_get_stats": ([P, ctypes.POINTER(BLAmdStats)], U32),
+        "bl_amd_set_sizing": ([U32], U32),
+        "bl_amd_cost_model_set": ([P], U32),
+        "bl_amd_context_collect_costs": ([P, P, P, SZ, ctypes.POINTER(SZ)], U32),
+        "bl_amd_cost_model_forward": ([P, SZ, P, P], U32),
+        "bl_amd_cost_model_grad": ([P, P, SZ, P, P, ctypes.POINTER(ctypes.c_float)], U32),
+        "bl_amd_cost_model_train": ([P, P, SZ, P, U32, U32, ctypes.c_float, P], U32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -418,6 +432,17 @@ class BLContext:
         """Executes the last flushed frame again from its device-resident copy."""
         _check(lib().bl_amd_context_replay(self._h), "replay")
 
+    def collect_costs(self):
+        """Features and exact-count targets of the resident frame's fill
+        commands (re-executes the frame with exact counting)."""
+        n = ctypes.c_size_t()
+        _check(lib().bl_amd_context_collect_costs(self._h, None, None, 0, ctypes.byref(n)), "collect_costs")
+        X = np.zeros((n.value, COST_FEATURES), dtype=np.float32)
+        T = np.zeros((n.value, COST_OUTPUTS), dtype=np.float32)
+        if n.value:
+            _check(lib().bl_amd_context_collect_costs(self._h, _dptr(X), _dptr(T), n.value, ctypes.byref(n)), "collect_costs")
+        return X, T
+
     def close(self):
         if self._h:
             lib().bl_amd_context_destroy(self._h)
@@ -467,3 +492,51 @@ class BLBatch:
 
 
 from .replay import replay_into, render_scene, render_scenes_batched  # noqa: E402,F401
+
+
+# ---------------------------------------------------------------------------
+# Learned sizing (cost model)
+# ---------------------------------------------------------------------------
+
+def set_sizing(mode: int) -> None:
+    """Process-wide sizing mode: SIZING_EXACT, SIZING_LEARNED or SIZING_HISTORY."""
+    _check(lib().bl_amd_set_sizing(mode), "set_sizing")
+
+
+def cost_model_set(params) -> None:
+    """Installs cost-model parameters (COST_PARAMS float32) or removes them (None)."""
+    if params is None:
+        _check(lib().bl_amd_cost_model_set(None), "cost_model_set")
+        return
+    p = np.ascontiguousarray(params, dtype=np.float32).reshape(COST_PARAMS)
+    _check(lib().bl_amd_cost_model_set(_dptr(p)), "cost_model_set")
+
+
+def cost_model_forward(X, params) -> np.ndarray:
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, COST_FEATURES)
+    p = np.ascontiguousarray(params, dtype=np.float32).reshape(COST_PARAMS)
+    Y = np.zeros((len(X), COST_OUTPUTS), dtype=np.float32)
+    _check(lib().bl_amd_cost_model_forward(_dptr(X), len(X), _dptr(p), _dptr(Y)), "cost_model_forward")
+    return Y
+
+
+def cost_model_grad(X, T, params):
+    """(gradient of the mean squared error, loss) computed on the GPU."""
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, COST_FEATURES)
+    T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1, COST_OUTPUTS)
+    p = np.ascontiguousarray(params, dtype=np.float32).reshape(COST_PARAMS)
+    g = np.zeros(COST_PARAMS, dtype=np.float32)
+    loss = ctypes.c_float()
+    _check(lib().bl_amd_cost_model_grad(_dptr(X), _dptr(T), len(X), _dptr(p), _dptr(g), ctypes.byref(loss)), "cost_model_grad")
+    return g, loss.value
+
+
+def cost_model_train(X, T, params, epochs: int = 10, batch: int = 4096, lr: float = 1e-3):
+    """Adam training on the GPU; returns (trained params, loss per epoch)."""
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, COST_FEATURES)
+    T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1, COST_OUTPUTS)
+    p = np.array(params, dtype=np.float32).reshape(COST_PARAMS)
+    losses = np.zeros(epochs, dtype=np.float32)
+    _check(lib().bl_amd_cost_model_train(_dptr(X), _dptr(T), len(X), _dptr(p), epochs, batch, lr, _dptr(losses)),
+           "cost_model_train")
+    return p, losses

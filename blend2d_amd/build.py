@@ -51,7 +51,7 @@ def build(verbose: bool = False) -> str:
     hipcc = _hipcc()
     os.makedirs(OBJ_DIR, exist_ok=True)
     objs = []
-    for src, device in (("kernels.hip", True), ("context.cpp", False), ("style.cpp", False)):
+    for src, device in (("kernels.hip", True), ("cost_model.hip", True), ("context.cpp", False), ("style.cpp", False)):
         obj = os.path.join(OBJ_DIR, src + ".o")
         flags = COMMON + (DEVICE if device else [])
         _run([hipcc, *flags, "-c", os.path.join(CSRC, src), "-o", obj], verbose)

@@ -14,6 +14,7 @@
 
 #include "../../include/blend2d_amd.h"
 #include "device_types.h"
+#include "cost_model.h"
 #include "kernels.h"
 #include "style.h"
 
@@ -81,8 +82,22 @@ struct FrameHost {
 struct Stats {
   uint64_t cmds = 0, src = 0, segs = 0, binned = 0, tiles = 0, pixels = 0;
   uint64_t frame_bytes = 0, lut_bytes = 0, image_bytes = 0;
+  uint64_t pred_segs = 0, pred_bins = 0;
+  uint32_t sizing = 0, fallback = 0, syncs = 0;
   double render_ms = 0.0, total_ms = 0.0;
 };
+
+// How a frame sizes its segment and bin buffers.
+enum Sizing : uint32_t {
+  kSizeExact = 0,     // counting pass of the edge builder + host read-back
+  kSizeLearned = 1,   // the cost model's predictions (pool mode)
+  kSizeHistory = 2,   // the previous frame's exact totals (pool mode)
+};
+
+// Process-wide sizing mode and cost-model parameters.
+uint32_t g_sizing = kSizeExact;
+std::vector<float> g_model;
+uint64_t g_model_version = 0;
 
 // Device-side state of the pipeline: the resident copy of the last uploaded
 // frame and the scratch of its stages.  A frame stays resident until the next
@@ -91,14 +106,22 @@ struct Runner {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf in;
-  DevBuf src_count, src_off, bounds, segs, seg_cmd, mat_cnt, mat_off, mat_pres, list_off, list, binned, cursor, scan_tmp, counters;
+  DevBuf src_count, src_off, bounds, segs, seg_cmd, mat_cnt, mat_off, mat_pres, list_off, list, binned, cursor, scan_tmp, counters, cmd_tiles;
+  DevBuf cm_params, cm_acc, cm_feat, cm_pred, cm_act;
+  uint64_t cm_version = 0;
   uint32_t* host_readback = nullptr;
 
   // Resident frame.
   bool resident = false;
   FrameDev f{};
-  uint64_t n_src = 0;
+  uint64_t n_src = 0, n_src_lines = 0;
   uint64_t frame_bytes = 0, lut_bytes = 0, image_bytes = 0;
+
+  // History sizing: exact totals of the last exactly counted frame.
+  uint64_t hist_segs = 0, hist_bins = 0;
+  // Cost-model logging: per analytic command, features and log2(1 + exact count).
+  bool collect = false;
+  std::vector<float> collected_x, collected_t;
 
   bool init() {
     if (stream) return true;
@@ -116,6 +139,10 @@ struct Runner {
 
   uint32_t upload(FrameHost& fh);
   uint32_t execute(Stats& stats);
+  uint32_t execute_exact(Stats& stats);
+  // 0 done, 1 pool overflow (redo exactly), otherwise an error
+  uint32_t execute_pool(Stats& stats, uint32_t mode);
+  bool model_views(cm::FrameModel& m);
   uint32_t run(FrameHost& fh, Stats& stats) {
     stats = Stats{};
     uint32_t r = upload(fh);
@@ -181,6 +208,8 @@ uint32_t Runner::upload(FrameHost& fh) {
   f.n_mat = n_mat;
   f.tol_sq = kToleranceSq;
   n_src = fh.src.size();
+  n_src_lines = 0;
+  for (const SrcSeg& sg : fh.src) n_src_lines += sg.kind == kSrcLine ? 1u : 0u;
   frame_bytes = blob.size();
   lut_bytes = fh.luts.size() * 4;
   image_bytes = 0;
@@ -189,11 +218,160 @@ uint32_t Runner::upload(FrameHost& fh) {
   return BL_AMD_SUCCESS;
 }
 
-// Runs the three stages over the resident frame.  Two host round trips read
-// back the sizes the next stage allocates (segment count, binned count).
+// Device buffers of the cost model for the resident frame.
+bool Runner::model_views(cm::FrameModel& m) {
+  const size_t nc = std::max<uint32_t>(f.n_cmds, 1);
+  if (!cm_acc.reserve(nc * sizeof(cm::CmdAcc)) || !cm_feat.reserve(nc * cm::kIn * 4) || !cm_pred.reserve(nc * cm::kOut * 4) ||
+      !cm_act.reserve(nc * 8 + 64) || !cm_params.reserve(cm::kParams * 4))
+    return false;
+  if (!g_model.empty() && cm_version != g_model_version) {
+    if (hipMemcpyAsync(cm_params.ptr, g_model.data(), cm::kParams * 4, hipMemcpyHostToDevice, stream) != hipSuccess) return false;
+    if (hipStreamSynchronize(stream) != hipSuccess) return false;
+    cm_version = g_model_version;
+  }
+  m.params = cm_params.as<float>();
+  m.acc = cm_acc.as<cm::CmdAcc>();
+  m.features = cm_feat.as<float>();
+  m.pred = cm_pred.as<float>();
+  m.actual_segs = cm_act.as<uint32_t>();
+  m.actual_bins = cm_act.as<uint32_t>() + nc;
+  m.totals = (unsigned long long*)(cm_act.as<uint8_t>() + nc * 8);
+  return true;
+}
+
 uint32_t Runner::execute(Stats& stats) {
   stats = Stats{};
   if (!resident) return BL_AMD_SUCCESS;
+  uint32_t mode = g_sizing;
+  if (collect || (mode == kSizeLearned && g_model.empty())) mode = kSizeExact;
+  if (mode != kSizeExact) {
+    uint32_t r = execute_pool(stats, mode);
+    if (r != 1) return r;
+    // pool overflow: the frame was not composited; redo it exactly
+    Stats exact;
+    r = execute_exact(exact);
+    exact.sizing = mode;
+    exact.fallback = 1;
+    exact.syncs += stats.syncs;
+    exact.pred_segs = stats.pred_segs;
+    exact.pred_bins = stats.pred_bins;
+    exact.total_ms += stats.total_ms;
+    stats = exact;
+    return r;
+  }
+  return execute_exact(stats);
+}
+
+// Pool mode: segment and bin buffers are sized ahead of time (from the cost
+// model's predictions or from the last exact frame) and the edge builder
+// runs once, without the counting pass and without host round trips in
+// between; any overflow is detected at the end of the frame.
+uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
+  auto t0 = std::chrono::steady_clock::now();
+  const uint64_t n_mat = f.n_mat;
+  size_t scan_words = scan_tmp_words((size_t)n_mat + 1);
+  if (!bounds.reserve((size_t)f.n_cmds * sizeof(CmdBounds)) || !mat_cnt.reserve((n_mat + 1) * 4) ||
+      !mat_off.reserve((n_mat + 1) * 4) || !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
+      !list.reserve((n_mat + 1) * 4) || !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) ||
+      !counters.reserve(64) || !cmd_tiles.reserve((size_t)f.n_cmds * 4))
+    return kErrOOM;
+
+  FrameScratch s;
+  memset(&s, 0, sizeof(s));
+  s.bounds = bounds.as<CmdBounds>();
+  s.mat_cnt = mat_cnt.as<uint32_t>();
+  s.mat_off = mat_off.as<uint32_t>();
+  s.mat_pres = mat_pres.as<uint32_t>();
+  s.list_off = list_off.as<uint32_t>();
+  s.list = list.as<uint32_t>();
+  s.scan_tmp = scan_tmp.as<uint32_t>();
+  s.pixels = counters.as<unsigned long long>();
+  s.seg_cursor = counters.as<uint32_t>() + 2;
+  s.flags = counters.as<uint32_t>() + 3;
+  s.cmd_tiles = cmd_tiles.as<uint32_t>();
+
+  launch_fill_u32(counters.as<uint32_t>(), 0u, 4, stream);
+  launch_init_bounds(s.bounds, f.n_cmds, stream);
+
+  // Pool sizes.  Chunked emission leaves at most one partly used chunk per
+  // source (4 slots for lines, 16 for curves).
+  const uint64_t waste = n_src_lines * 4 + (n_src - n_src_lines) * 16;
+  uint64_t want_segs, want_bins;
+  if (mode == kSizeLearned) {
+    cm::FrameModel m;
+    if (!model_views(m)) return kErrOOM;
+    cm::launch_features(f, m, stream);
+    cm::launch_infer_frame(f, m, stream);
+    HIP_OK(hipMemcpyAsync(host_readback + 4, m.totals, 16, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    stats.syncs++;
+    uint64_t tot[2];
+    memcpy(tot, host_readback + 4, 16);
+    stats.pred_segs = tot[0];
+    stats.pred_bins = tot[1];
+    want_segs = tot[0] + tot[0] / 4 + waste + 1024;
+    want_bins = tot[1] + tot[1] / 4 + 1024;
+  }
+  else {
+    want_segs = hist_segs + hist_segs / 4 + waste + 1024;
+    want_bins = hist_bins + hist_bins / 4 + 1024;
+  }
+  if (want_segs > 0xFFFFF000ull || want_bins > 0xFFFFF000ull) return 1;
+  if (!segs.reserve(want_segs * sizeof(DevSeg)) || !seg_cmd.reserve(want_segs * 4) || !binned.reserve(want_bins * sizeof(DevSeg)))
+    return kErrOOM;
+  const uint32_t cap_segs = (uint32_t)std::min<size_t>(std::min(segs.cap / sizeof(DevSeg), seg_cmd.cap / 4), 0xFFFFF000ull);
+  const uint32_t cap_bins = (uint32_t)std::min<size_t>(binned.cap / sizeof(DevSeg), 0xFFFFF000ull);
+  s.segs = segs.as<DevSeg>();
+  s.seg_cmd = seg_cmd.as<uint32_t>();
+  s.binned = binned.as<DevSeg>();
+  s.binned_cap = cap_bins;
+
+  launch_edges_pool(f, s, cap_segs, stream);
+  launch_cmd_tiles(f, s, stream);
+  launch_fill_u32(s.mat_cnt, 0u, n_mat + 1, stream);
+  launch_bin_count(f, s, cap_segs, stream);
+  launch_presence(f, s, stream);
+  scan_exclusive(s.mat_cnt, s.mat_off, n_mat, s.scan_tmp, stream);
+  scan_exclusive(s.mat_pres, s.list_off, n_mat, s.scan_tmp, stream);
+  HIP_OK(hipMemcpyAsync(cursor.ptr, s.mat_off, (n_mat + 1) * 4, hipMemcpyDeviceToDevice, stream));
+  launch_bin_scatter(f, s, cap_segs, cursor.as<uint32_t>(), stream);
+  launch_list_build(f, s, stream);
+
+  HIP_OK(hipEventRecord(ev0, stream));
+  launch_render(f, s, stream);
+  HIP_OK(hipEventRecord(ev1, stream));
+  HIP_OK(hipMemcpyAsync(host_readback, counters.ptr, 16, hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync(host_readback + 8, s.mat_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipStreamSynchronize(stream));
+  stats.syncs++;
+  if (hipGetLastError() != hipSuccess) return kErrDevice;
+  auto t1 = std::chrono::steady_clock::now();
+  stats.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  stats.sizing = mode;
+  if (host_readback[3]) return 1;
+
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ev0, ev1);
+  uint64_t pixels;
+  memcpy(&pixels, host_readback, 8);
+  stats.cmds = f.n_cmds;
+  stats.src = n_src;
+  stats.segs = std::min(host_readback[2], cap_segs);   // slots taken, unused chunk tails included
+  stats.binned = host_readback[8];
+  stats.tiles = f.n_tiles;
+  stats.pixels = pixels;
+  stats.frame_bytes = frame_bytes;
+  stats.lut_bytes = lut_bytes;
+  stats.image_bytes = image_bytes;
+  stats.render_ms = ms;
+  return BL_AMD_SUCCESS;
+}
+
+// Exact sizing: counting pass of the edge builder, scan, read-back, emit;
+// bin count, scans, read-back, scatter.  Two host round trips read back the
+// sizes the next stage allocates (segment count, binned count).
+uint32_t Runner::execute_exact(Stats& stats) {
+  stats = Stats{};
   auto t0 = std::chrono::steady_clock::now();
   const uint64_t n_mat = f.n_mat;
 
@@ -202,7 +380,8 @@ uint32_t Runner::execute(Stats& stats) {
       !bounds.reserve((size_t)f.n_cmds * sizeof(CmdBounds)) ||
       !mat_cnt.reserve((n_mat + 1) * 4) || !mat_off.reserve((n_mat + 1) * 4) ||
       !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
-      !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) || !counters.reserve(64))
+      !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) || !counters.reserve(64) ||
+      !cmd_tiles.reserve((size_t)f.n_cmds * 4))
     return kErrOOM;
 
   FrameScratch s;
@@ -216,14 +395,23 @@ uint32_t Runner::execute(Stats& stats) {
   s.list_off = list_off.as<uint32_t>();
   s.scan_tmp = scan_tmp.as<uint32_t>();
   s.pixels = counters.as<unsigned long long>();
+  s.cmd_tiles = cmd_tiles.as<uint32_t>();
+
+  cm::FrameModel model{};
+  if (collect) {
+    if (!model_views(model)) return kErrOOM;
+    cm::launch_features(f, model, stream);
+  }
 
   // Stage 1: edges (count, scan, emit).
-  launch_fill_u32(counters.as<uint32_t>(), 0u, 2, stream);
+  launch_fill_u32(counters.as<uint32_t>(), 0u, 4, stream);
   launch_init_bounds(s.bounds, f.n_cmds, stream);
   launch_edges(f, s, true, stream);
+  launch_cmd_tiles(f, s, stream);
   scan_exclusive(s.src_count, s.src_off, n_src, s.scan_tmp, stream);
   HIP_OK(hipMemcpyAsync(host_readback, s.src_off + n_src, 4, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipStreamSynchronize(stream));
+  stats.syncs++;
   const uint32_t n_segs = host_readback[0];
   if (!segs.reserve((size_t)std::max<uint32_t>(n_segs, 1) * sizeof(DevSeg)) || !seg_cmd.reserve((size_t)std::max<uint32_t>(n_segs, 1) * 4))
     return kErrOOM;
@@ -240,6 +428,7 @@ uint32_t Runner::execute(Stats& stats) {
   HIP_OK(hipMemcpyAsync(host_readback, s.mat_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipMemcpyAsync(host_readback + 1, s.list_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipStreamSynchronize(stream));
+  stats.syncs++;
   const uint32_t n_binned = host_readback[0], n_list = host_readback[1];
   if (!binned.reserve((size_t)std::max<uint32_t>(n_binned, 1) * sizeof(DevSeg)) || !list.reserve((size_t)std::max<uint32_t>(n_list, 1) * 4))
     return kErrOOM;
@@ -254,8 +443,29 @@ uint32_t Runner::execute(Stats& stats) {
   launch_render(f, s, stream);
   HIP_OK(hipEventRecord(ev1, stream));
   HIP_OK(hipMemcpyAsync(host_readback + 2, s.pixels, 8, hipMemcpyDeviceToHost, stream));
+  if (collect) {
+    cm::launch_actuals(f, s, model, n_segs, stream);
+    const size_t nc = f.n_cmds;
+    std::vector<float> feat(nc * cm::kIn);
+    std::vector<uint32_t> act(2 * std::max<size_t>(nc, 1));
+    HIP_OK(hipMemcpyAsync(feat.data(), model.features, feat.size() * 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(act.data(), model.actual_segs, act.size() * 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    collected_x.clear();
+    collected_t.clear();
+    for (size_t c = 0; c < nc; c++) {
+      const float* x = &feat[c * cm::kIn];
+      if (x[0] + x[1] + x[2] <= 0.0f) continue;   // no source segments (or not an analytic fill)
+      collected_x.insert(collected_x.end(), x, x + cm::kIn);
+      collected_t.push_back((float)log2(1.0 + act[c]));
+      collected_t.push_back((float)log2(1.0 + act[std::max<size_t>(nc, 1) + c]));
+    }
+  }
   HIP_OK(hipStreamSynchronize(stream));
+  stats.syncs++;
   if (hipGetLastError() != hipSuccess) return kErrDevice;
+  hist_segs = n_segs;
+  hist_bins = n_binned;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, ev0, ev1);
   auto t1 = std::chrono::steady_clock::now();
@@ -1244,6 +1454,11 @@ static void copy_stats(const Stats& st, BLAmdStats* out) {
   out->frame_bytes = st.frame_bytes;
   out->lut_bytes = st.lut_bytes;
   out->image_bytes = st.image_bytes;
+  out->pred_segs = st.pred_segs;
+  out->pred_bins = st.pred_bins;
+  out->sizing = st.sizing;
+  out->fallback = st.fallback;
+  out->syncs = st.syncs;
   out->render_ms = st.render_ms;
   out->total_ms = st.total_ms;
 }
@@ -1323,6 +1538,158 @@ BLAmdResult bl_amd_batch_flush(BLAmdBatch* b) {
 BLAmdResult bl_amd_batch_replay(BLAmdBatch* b) {
   if (!b) return kErrInvalid;
   return b->runner.execute(b->stats);
+}
+
+// ---------------------------------------------------------------------------
+// Learned sizing: cost model
+// ---------------------------------------------------------------------------
+
+BLAmdResult bl_amd_set_sizing(uint32_t mode) {
+  if (mode > kSizeHistory) return kErrInvalid;
+  g_sizing = mode;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_cost_model_set(const float* params) {
+  if (params) g_model.assign(params, params + cm::kParams);
+  else g_model.clear();
+  g_model_version++;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_collect_costs(BLAmdContext* ctx, float* features, float* targets, size_t cap, size_t* n_out) {
+  if (!ctx || !n_out) return kErrInvalid;
+  if (uint32_t r = check_target(ctx)) return r;
+  ctx->runner.collect = true;
+  uint32_t r = ctx->runner.execute(ctx->stats);
+  ctx->runner.collect = false;
+  if (r) return r;
+  const size_t rows = ctx->runner.collected_t.size() / cm::kOut;
+  const size_t k = std::min(rows, cap);
+  if (k && (!features || !targets)) return kErrInvalid;
+  if (k) {
+    memcpy(features, ctx->runner.collected_x.data(), k * cm::kIn * sizeof(float));
+    memcpy(targets, ctx->runner.collected_t.data(), k * cm::kOut * sizeof(float));
+  }
+  *n_out = rows;
+  return BL_AMD_SUCCESS;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Device state for the MLP entry points (training and tests).
+struct Trainer {
+  hipStream_t stream = nullptr;
+  DevBuf x, t, y, params, partial, grad, m1, m2;
+  float* host = nullptr;
+  size_t host_cap = 0;
+  bool init() {
+    if (stream) return true;
+    return hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess;
+  }
+  ~Trainer() {
+    if (host) (void)hipHostFree(host);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  bool host_reserve(size_t n) {
+    if (n <= host_cap) return true;
+    if (host) (void)hipHostFree(host);
+    host = nullptr;
+    host_cap = 0;
+    if (hipHostMalloc((void**)&host, n * sizeof(float), hipHostMallocDefault) != hipSuccess) return false;
+    host_cap = n;
+    return true;
+  }
+};
+
+Trainer& trainer() {
+  static Trainer t;
+  return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+BLAmdResult bl_amd_cost_model_forward(const float* X, size_t n, const float* params, float* Y) {
+  if (!params || (n && (!X || !Y)) || n > 0x7FFFFFFFu) return kErrInvalid;
+  if (!n) return BL_AMD_SUCCESS;
+  Trainer& tr = trainer();
+  if (!tr.init()) return kErrDevice;
+  if (!tr.x.reserve(n * cm::kIn * 4) || !tr.y.reserve(n * cm::kOut * 4) || !tr.params.reserve(cm::kParams * 4)) return kErrOOM;
+  HIP_OK(hipMemcpyAsync(tr.x.ptr, X, n * cm::kIn * 4, hipMemcpyHostToDevice, tr.stream));
+  HIP_OK(hipMemcpyAsync(tr.params.ptr, params, cm::kParams * 4, hipMemcpyHostToDevice, tr.stream));
+  cm::launch_mlp_forward(tr.x.as<float>(), (uint32_t)n, tr.params.as<float>(), tr.y.as<float>(), tr.stream);
+  HIP_OK(hipMemcpyAsync(Y, tr.y.ptr, n * cm::kOut * 4, hipMemcpyDeviceToHost, tr.stream));
+  HIP_OK(hipStreamSynchronize(tr.stream));
+  return hipGetLastError() == hipSuccess ? BL_AMD_SUCCESS : kErrDevice;
+}
+
+BLAmdResult bl_amd_cost_model_grad(const float* X, const float* T, size_t n, const float* params, float* grad, float* loss) {
+  if (!params || !grad || !n || !X || !T || n > 0x7FFFFFFFu) return kErrInvalid;
+  Trainer& tr = trainer();
+  if (!tr.init()) return kErrDevice;
+  const uint32_t groups = cm::mlp_groups((uint32_t)n);
+  if (!tr.x.reserve(n * cm::kIn * 4) || !tr.t.reserve(n * cm::kOut * 4) || !tr.params.reserve(cm::kParams * 4) ||
+      !tr.partial.reserve((size_t)groups * (cm::kParams + 1) * 4) || !tr.grad.reserve((cm::kParams + 1) * 4))
+    return kErrOOM;
+  HIP_OK(hipMemcpyAsync(tr.x.ptr, X, n * cm::kIn * 4, hipMemcpyHostToDevice, tr.stream));
+  HIP_OK(hipMemcpyAsync(tr.t.ptr, T, n * cm::kOut * 4, hipMemcpyHostToDevice, tr.stream));
+  HIP_OK(hipMemcpyAsync(tr.params.ptr, params, cm::kParams * 4, hipMemcpyHostToDevice, tr.stream));
+  cm::launch_mlp_grad(tr.x.as<float>(), tr.t.as<float>(), (uint32_t)n, tr.params.as<float>(), tr.partial.as<float>(), tr.stream);
+  cm::launch_mlp_reduce(tr.partial.as<float>(), groups, (uint32_t)n, tr.grad.as<float>(), tr.stream);
+  std::vector<float> g(cm::kParams + 1);
+  HIP_OK(hipMemcpyAsync(g.data(), tr.grad.ptr, g.size() * 4, hipMemcpyDeviceToHost, tr.stream));
+  HIP_OK(hipStreamSynchronize(tr.stream));
+  if (hipGetLastError() != hipSuccess) return kErrDevice;
+  memcpy(grad, g.data(), cm::kParams * 4);
+  if (loss) *loss = g[cm::kParams];
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_cost_model_train(const float* X, const float* T, size_t n, float* params, uint32_t epochs, uint32_t batch,
+                                    float lr, float* epoch_loss) {
+  if (!X || !T || !params || !n || !batch || n > 0x7FFFFFFFu || batch > (1u << 20)) return kErrInvalid;
+  Trainer& tr = trainer();
+  if (!tr.init()) return kErrDevice;
+  const uint32_t groups = cm::mlp_groups(batch);
+  const size_t n_batches = (n + batch - 1) / batch;
+  if (!tr.x.reserve(n * cm::kIn * 4) || !tr.t.reserve(n * cm::kOut * 4) || !tr.params.reserve(cm::kParams * 4) ||
+      !tr.partial.reserve((size_t)groups * (cm::kParams + 1) * 4) || !tr.grad.reserve((cm::kParams + 1) * 4) ||
+      !tr.m1.reserve(cm::kParams * 4) || !tr.m2.reserve(cm::kParams * 4) || !tr.host_reserve(n_batches))
+    return kErrOOM;
+  HIP_OK(hipMemcpyAsync(tr.x.ptr, X, n * cm::kIn * 4, hipMemcpyHostToDevice, tr.stream));
+  HIP_OK(hipMemcpyAsync(tr.t.ptr, T, n * cm::kOut * 4, hipMemcpyHostToDevice, tr.stream));
+  HIP_OK(hipMemcpyAsync(tr.params.ptr, params, cm::kParams * 4, hipMemcpyHostToDevice, tr.stream));
+  launch_fill_u32(tr.m1.as<uint32_t>(), 0u, cm::kParams, tr.stream);   // 0.0f
+  launch_fill_u32(tr.m2.as<uint32_t>(), 0u, cm::kParams, tr.stream);
+  uint32_t step = 0;
+  for (uint32_t e = 0; e < epochs; e++) {
+    for (size_t b = 0; b < n_batches; b++) {
+      const size_t start = b * batch;
+      const uint32_t rows = (uint32_t)std::min<size_t>(batch, n - start);
+      const float* xb = tr.x.as<float>() + start * cm::kIn;
+      const float* tb = tr.t.as<float>() + start * cm::kOut;
+      cm::launch_mlp_grad(xb, tb, rows, tr.params.as<float>(), tr.partial.as<float>(), tr.stream);
+      cm::launch_mlp_reduce(tr.partial.as<float>(), cm::mlp_groups(rows), rows, tr.grad.as<float>(), tr.stream);
+      cm::launch_adam(tr.params.as<float>(), tr.grad.as<float>(), tr.m1.as<float>(), tr.m2.as<float>(), ++step, lr, tr.stream);
+      HIP_OK(hipMemcpyAsync(tr.host + b, tr.grad.as<float>() + cm::kParams, 4, hipMemcpyDeviceToHost, tr.stream));
+    }
+    HIP_OK(hipStreamSynchronize(tr.stream));
+    if (epoch_loss) {
+      double acc = 0.0;
+      for (size_t b = 0; b < n_batches; b++) {
+        const size_t rows = std::min<size_t>(batch, n - b * batch);
+        acc += (double)tr.host[b] * (double)rows;
+      }
+      epoch_loss[e] = (float)(acc / (double)n);
+    }
+  }
+  HIP_OK(hipMemcpyAsync(params, tr.params.ptr, cm::kParams * 4, hipMemcpyDeviceToHost, tr.stream));
+  HIP_OK(hipStreamSynchronize(tr.stream));
+  return hipGetLastError() == hipSuccess ? BL_AMD_SUCCESS : kErrDevice;
 }
 
 BLAmdResult bl_amd_batch_get_stats(const BLAmdBatch* b, BLAmdStats* out) {

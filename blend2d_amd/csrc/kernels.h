@@ -40,6 +40,11 @@ struct FrameScratch {
   DevSeg* binned;          // n_binned
   uint32_t* scan_tmp;      // scan_tmp_words(max n)
   unsigned long long* pixels;  // composited pixels (mask > 0), accumulated by render
+  uint32_t* cmd_tiles;     // n_cmds: tile-column range per command
+  // Pool mode (sizes predicted ahead of time; null / 0 otherwise):
+  uint32_t* seg_cursor;    // segments taken from the pool
+  uint32_t* flags;         // bit 0: segment pool overflow, bit 1: bin pool overflow
+  uint32_t binned_cap;     // capacity of `binned`
 };
 
 size_t scan_tmp_words(size_t n);
@@ -48,11 +53,14 @@ void scan_exclusive(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp, 
 // Stage 1: edge building.  count == true writes per-source segment counts and
 // command bounds; count == false writes the segments at the scanned offsets.
 void launch_edges(const FrameDev& f, const FrameScratch& s, bool count, hipStream_t st);
+// Single pass into a pool of `cap` segments (pool mode).
+void launch_edges_pool(const FrameDev& f, const FrameScratch& s, uint32_t cap, hipStream_t st);
 // Stage 2: binning of segments into (band, command) cells.
 void launch_bin_count(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, hipStream_t st);
 void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_bin_scatter(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, uint32_t* cursor, hipStream_t st);
 void launch_list_build(const FrameDev& f, const FrameScratch& s, hipStream_t st);
+void launch_cmd_tiles(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 // Stage 3: tile rasterization + fetch + composite.
 void launch_render(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 

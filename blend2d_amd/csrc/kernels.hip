@@ -226,6 +226,70 @@ __global__ __launch_bounds__(64) void k_edges_emit(FrameDev f, FrameScratch s) {
   build_one(f, sg, e);
 }
 
+// Single-pass emission into a pool sized ahead of time: each thread takes
+// chunks of slots from a global cursor, so the number of atomics is about
+// segments / chunk.  Unused slots of a chunk are marked with command
+// index 0xFFFFFFFF, which binning skips.  Reaching the pool capacity raises the
+// overflow flag; the frame is then redone with exact counting.
+struct ChunkEmit {
+  DevSeg* segs;
+  uint32_t* seg_cmd;
+  uint32_t* cursor;
+  uint32_t* flags;
+  uint32_t cap, chunk, cmd;
+  uint32_t pos = 0, end = 0;
+  bool dead = false;
+  int32_t min_x = INT32_MAX, max_x = INT32_MIN, min_y = INT32_MAX, max_y = INT32_MIN;
+  __device__ __forceinline__ void push(int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint32_t sign) {
+    int32_t lo = (x0 < x1 ? x0 : x1) >> 8, hi = (x0 < x1 ? x1 : x0) >> 8;
+    min_x = lo < min_x ? lo : min_x;
+    max_x = hi > max_x ? hi : max_x;
+    min_y = y0 < min_y ? y0 : min_y;
+    max_y = y1 > max_y ? y1 : max_y;
+    if (dead) return;
+    if (pos == end) {
+      uint32_t base = atomicAdd(cursor, chunk);
+      if (base > cap || cap - base < chunk) {
+        atomicOr(flags, 1u);
+        dead = true;
+        return;
+      }
+      pos = base;
+      end = base + chunk;
+    }
+    segs[pos] = DevSeg{x0, y0, (int32_t)((uint32_t)x1 | (sign << 31)), y1};
+    seg_cmd[pos] = cmd;
+    pos++;
+  }
+  __device__ __forceinline__ void finish() {
+    for (; pos < end; pos++) {
+      segs[pos] = DevSeg{0, 0, 0, 0};
+      seg_cmd[pos] = 0xFFFFFFFFu;   // unused slot
+    }
+  }
+};
+
+__global__ __launch_bounds__(64) void k_edges_emit_pool(FrameDev f, FrameScratch s, uint32_t cap) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n_src) return;
+  SrcSeg sg = f.src[i];
+  ChunkEmit e{s.segs, s.seg_cmd, s.seg_cursor, s.flags, cap, sg.kind == kSrcLine ? 4u : 16u, sg.cmd};
+  build_one(f, sg, e);
+  e.finish();
+  if (e.min_x <= e.max_x) {
+    CmdBounds* bd = &s.bounds[sg.cmd];
+    atomicMin(&bd->min_x, e.min_x);
+    atomicMax(&bd->max_x, e.max_x);
+    atomicMin(&bd->min_y, e.min_y);
+    atomicMax(&bd->max_y, e.max_y);
+  }
+}
+
+void launch_edges_pool(const FrameDev& f, const FrameScratch& s, uint32_t cap, hipStream_t st) {
+  if (!f.n_src) return;
+  hipLaunchKernelGGL(k_edges_emit_pool, dim3((f.n_src + 63) / 64), dim3(64), 0, st, f, s, cap);
+}
+
 void launch_edges(const FrameDev& f, const FrameScratch& s, bool count, hipStream_t st) {
   if (!f.n_src) return;
   dim3 grid((f.n_src + 63) / 64), block(64);
@@ -238,6 +302,7 @@ void launch_edges(const FrameDev& f, const FrameScratch& s, bool count, hipStrea
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void seg_bands(const DevSeg& g, int bands, int& b0, int& b1) {
+  if (g.y1 <= g.y0) { b0 = 1; b1 = 0; return; }
   int ey0 = g.y0 >> 8;
   int ey1 = (g.y1 - 1) >> 8;
   b0 = ey0 / kTileH;
@@ -246,11 +311,19 @@ __device__ __forceinline__ void seg_bands(const DevSeg& g, int bands, int& b0, i
   b1 = b1 > bands - 1 ? bands - 1 : b1;
 }
 
+// Number of valid segments: `n`, or in pool mode min(*seg_cursor, n).
+__device__ __forceinline__ uint32_t seg_limit(const FrameScratch& s, uint32_t n) {
+  if (!s.seg_cursor) return n;
+  uint32_t c = *s.seg_cursor;
+  return c < n ? c : n;
+}
+
 __global__ void k_bin_count(FrameDev f, FrameScratch s, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= seg_limit(s, n)) return;
   DevSeg g = s.segs[i];
   uint32_t cmd = s.seg_cmd[i];
+  if (cmd == 0xFFFFFFFFu) return;
   const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
   int b0, b1;
   seg_bands(g, (int)cv.bands, b0, b1);
@@ -286,9 +359,16 @@ void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
 
 __global__ void k_bin_scatter(FrameDev f, FrameScratch s, uint32_t n, uint32_t* cursor) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= seg_limit(s, n)) return;
+  // pool mode: the binned pool is sized ahead of time; a larger total is an
+  // overflow (the frame falls back to exact counting)
+  if (s.binned_cap && s.mat_off[f.n_mat] > s.binned_cap) {
+    if (i == 0) atomicOr(s.flags, 2u);
+    return;
+  }
   DevSeg g = s.segs[i];
   uint32_t cmd = s.seg_cmd[i];
+  if (cmd == 0xFFFFFFFFu) return;
   const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
   int b0, b1;
   seg_bands(g, (int)cv.bands, b0, b1);
@@ -311,6 +391,35 @@ __global__ void k_list_build(FrameScratch s, uint64_t n) {
 
 void launch_list_build(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
   if (f.n_mat) hipLaunchKernelGGL(k_list_build, dim3((unsigned)((f.n_mat + 255) / 256)), dim3(256), 0, st, s, f.n_mat);
+}
+
+// Per command, the range of tile columns it can touch, packed as
+// first | last << 16 (first > last when it touches none).  Analytic fills
+// touch cells min_x .. max_x + 1 of their edges; boxes their pixel box.
+__global__ void k_cmd_tiles(FrameDev f, FrameScratch s) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= f.n_cmds) return;
+  const DevCmd& cmd = f.cmds[c];
+  const DevCanvas& cv = f.canvases[cmd.canvas];
+  int x0, x1;
+  if (cmd.kind == kCmdAnalytic) {
+    const CmdBounds bd = s.bounds[c];
+    x0 = bd.min_x;
+    x1 = bd.max_x + 1;
+  }
+  else {
+    x0 = cmd.box[0];
+    x1 = cmd.box[2] - 1;
+  }
+  const int last = (int)cv.tiles_x - 1;
+  int t0 = x0 < 0 ? 0 : x0 / kTileW;
+  int t1 = x1 < 0 ? -1 : x1 / kTileW;
+  t1 = t1 > last ? last : t1;
+  s.cmd_tiles[c] = (x0 > x1 || t0 > t1) ? 0xFFFFu : ((uint32_t)t0 | ((uint32_t)t1 << 16));
+}
+
+void launch_cmd_tiles(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
+  if (f.n_cmds) hipLaunchKernelGGL(k_cmd_tiles, dim3((f.n_cmds + 255) / 256), dim3(256), 0, st, f, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -352,6 +461,9 @@ __device__ __forceinline__ uint32_t row_excl_scan16(uint32_t v) {
 __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratch s) {
   __shared__ uint32_t cells[kTileH * kTileW];
   __shared__ uint32_t backdrop[kTileH];
+  // A pool overflow leaves the frame incomplete: nothing is composited and
+  // the host redoes the frame with exact counting.
+  if (s.flags && *s.flags) return;
 
   const uint32_t tile = blockIdx.x;
   // canvas lookup (uniform)
@@ -410,11 +522,40 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
   const uint64_t row_base = cv.mat_base + (uint64_t)band * cv.n_cmds;
   const uint32_t lb = s.list_off[row_base];
   const uint32_t le = s.list_off[row_base + cv.n_cmds];
+  const uint32_t tcol = local % cv.tiles_x;
   bool touched = false;
   uint32_t npix = 0;
 
-  for (uint32_t li = lb; li < le; li++) {
-    const uint32_t e = s.list[li];
+  // The band's command list is filtered 256 entries at a time down to the
+  // commands whose tile-column range holds this tile (order preserved by a
+  // ballot compaction), then those are applied one by one.
+  __shared__ uint32_t sel[kTileThreads];
+  __shared__ uint32_t sel_wave[kTileThreads / 64];
+  const int lane = t & 63, wave = t >> 6;
+  for (uint32_t chunk = lb; chunk < le; chunk += kTileThreads) {
+    const uint32_t li = chunk + (uint32_t)t;
+    uint32_t e = 0;
+    bool keep = false;
+    if (li < le) {
+      e = s.list[li];
+      const uint32_t tr = s.cmd_tiles[cv.cmd_base + (uint32_t)(e - row_base)];
+      keep = tcol >= (tr & 0xFFFFu) && tcol <= (tr >> 16);
+    }
+    const uint64_t bal = __ballot(keep);
+    if (lane == 0) sel_wave[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t off = 0, n_sel = 0;
+#pragma unroll
+    for (int w = 0; w < kTileThreads / 64; w++) {
+      const uint32_t cw = sel_wave[w];
+      off += w < wave ? cw : 0u;
+      n_sel += cw;
+    }
+    if (keep) sel[off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = e;
+    __syncthreads();
+
+  for (uint32_t ks = 0; ks < n_sel; ks++) {
+    const uint32_t e = sel[ks];
     const uint32_t cmd_i = cv.cmd_base + (uint32_t)(e - row_base);
     const DevCmd c = f.cmds[cmd_i];
     uint32_t m[4] = {0, 0, 0, 0};
@@ -504,6 +645,8 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
       d[k] = a8 ? pipe::composite8(c.op, d[k], src >> 24, m[k]) : pipe::composite32(c.op, d[k], src, m[k]);
       npix++;
     }
+  }
+    __syncthreads();   // sel / sel_wave are rewritten by the next chunk
   }
 
   // Composited-pixel statistics: one atomic per wave.

@@ -702,3 +702,62 @@ def frontend_scene(seed: int = 1, width: int = 300, height: int = 220, fmt: int 
             sc.fill_geometry(g, data, _rand_rgba(rng) if rng.random() < 0.3 else None)
     sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
     return sc
+
+
+def cost_canvases(seed: int = 1, count: int = 256, side_lo: int = 1024, side_hi: int = 4096, paths: int = 64,
+                  max_segments: int = 20000):
+    """Cost-model workload: `count` PRGB32 canvases, sides log-uniform in
+    [side_lo, side_hi]; per path a segment count clip(Pareto(1.5) * 8, 8,
+    max_segments) of lines, quads, cubics or a mix, control points uniform in
+    a box of random size; solid / linear / radial / pattern fills and comp
+    ops SRC_OVER, MULTIPLY, PLUS, SCREEN; both fill rules."""
+    rng = np.random.default_rng(seed)
+    ops = (COMP_OP_SRC_OVER, COMP_OP_MULTIPLY, COMP_OP_PLUS, COMP_OP_SCREEN)
+    scenes = []
+    for _ in range(count):
+        w = int(round(2.0 ** rng.uniform(math.log2(side_lo), math.log2(side_hi))))
+        h = int(round(2.0 ** rng.uniform(math.log2(side_lo), math.log2(side_hi))))
+        sc = Scene(w, h, FMT_PRGB32, 0xFF000000 | int(rng.integers(0, 1 << 24)))
+        img = sc.add_image(premultiplied_noise(rng, 64, 64))
+        k = int(rng.integers(1, paths + 1))
+        segs = np.clip(((rng.pareto(1.5, k) + 1.0) * 8.0).astype(np.int64), 8, max_segments)
+        for j in range(k):
+            n = int(segs[j])
+            side = float(2.0 ** rng.uniform(4, math.log2(max(w, h))))
+            bx = float(rng.uniform(-side * 0.3, w - side * 0.7))
+            by = float(rng.uniform(-side * 0.3, h - side * 0.7))
+            mix = int(rng.integers(0, 4))          # 0 lines, 1 quads, 2 cubics, 3 mixed
+            kinds = np.full(n, mix) if mix < 3 else rng.integers(0, 3, n)
+            per = np.array([1, 2, 3])[kinds]
+            nv = int(per.sum())
+            pts = rng.random((nv, 2)) * side + np.array([bx, by])
+            cmds = np.empty(nv + 2, dtype=np.uint8)
+            cmds[0] = CMD_MOVE
+            # quads: QUAD ON, cubics: CUBIC CUBIC ON, lines: ON
+            pos = 1 + np.concatenate([[0], np.cumsum(per)[:-1]])
+            body = np.full(nv, CMD_ON, dtype=np.uint8)
+            q = pos[kinds == 1] - 1
+            body[q] = CMD_QUAD
+            c = pos[kinds == 2] - 1
+            body[c] = CMD_CUBIC
+            body[c + 1] = CMD_CUBIC
+            cmds[1:-1] = body
+            cmds[-1] = CMD_CLOSE
+            vtx = np.vstack([pts[-1:], pts, pts[-1:]])
+            sc.set_comp_op(int(ops[int(rng.integers(0, 4))]))
+            sc.set_fill_rule(int(rng.integers(0, 2)))
+            st = rng.random()
+            if st < 0.3:
+                sc.set_fill_style_rgba32(_rand_rgba(rng))
+            elif st < 0.55:
+                p = rng.random(4) * side + np.array([bx, by, bx, by])
+                sc.set_fill_style_gradient(GRADIENT_LINEAR, [float(v) for v in p], _rand_stops(rng, 2, 4))
+            elif st < 0.8:
+                cx, cy = float(bx + side / 2), float(by + side / 2)
+                sc.set_fill_style_gradient(GRADIENT_RADIAL, [cx, cy, cx, cy, float(side / 2), 0.0], _rand_stops(rng, 2, 4))
+            else:
+                a = float(rng.uniform(0.1, 6.1))
+                sc.set_fill_style_pattern(img, EXTEND_REPEAT, [math.cos(a), math.sin(a), -math.sin(a), math.cos(a), bx, by])
+            sc.fill_path(cmds, vtx)
+        scenes.append(sc)
+    return scenes

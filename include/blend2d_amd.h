@@ -177,6 +177,12 @@ typedef struct BLAmdStats {
   uint64_t frame_bytes;  /* bytes of the device-resident frame (commands, geometry, styles, LUTs) */
   uint64_t lut_bytes;    /* gradient LUT bytes of the frame */
   uint64_t image_bytes;  /* pixel bytes of the pattern images the frame samples */
+  uint64_t pred_segs;    /* learned sizing: predicted segments (0 otherwise) */
+  uint64_t pred_bins;    /* learned sizing: predicted (segment, band) copies */
+  uint32_t sizing;       /* BL_AMD_SIZING_* the frame used */
+  uint32_t fallback;     /* 1: a pool overflowed and the frame was redone with exact counting */
+  uint32_t syncs;        /* host round trips of the frame */
+  uint32_t reserved;
   double render_ms;    /* device time of the render kernel */
   double total_ms;     /* host wall time of the device stages */
 } BLAmdStats;
@@ -201,6 +207,41 @@ BLAmdResult bl_amd_batch_flush(BLAmdBatch* batch);
 /* Same as bl_amd_context_replay for the batch's last merged frame. */
 BLAmdResult bl_amd_batch_replay(BLAmdBatch* batch);
 BLAmdResult bl_amd_batch_get_stats(const BLAmdBatch* batch, BLAmdStats* out);
+
+/* ---- learned sizing (no counterpart in the reference) --------------------
+ * A frame must size its edge-segment and bin buffers before the kernels that
+ * fill them run.  BL_AMD_SIZING_EXACT runs a counting pass of the edge
+ * builder and reads the sizes back; BL_AMD_SIZING_LEARNED predicts them with
+ * a small MLP (16 per-command features -> 64 -> 64 -> 2, fp32, trained on
+ * the GPU by the kernels below) and runs the edge builder once;
+ * BL_AMD_SIZING_HISTORY reuses the totals of the last exact frame.  In both
+ * predicted modes an overflow is detected on the device, nothing is
+ * composited, and the frame is redone exactly: pixels never depend on the
+ * model. */
+enum { BL_AMD_SIZING_EXACT = 0, BL_AMD_SIZING_LEARNED = 1, BL_AMD_SIZING_HISTORY = 2 };
+enum { BL_AMD_COST_FEATURES = 16, BL_AMD_COST_OUTPUTS = 2, BL_AMD_COST_PARAMS = 5378 };
+
+/* Process-wide sizing mode (default EXACT). */
+BLAmdResult bl_amd_set_sizing(uint32_t mode);
+/* Installs the model parameters (BL_AMD_COST_PARAMS floats: W1[16][64] b1[64]
+ * W2[64][64] b2[64] W3[64][2] b3[2]); NULL removes it (LEARNED then counts
+ * exactly). */
+BLAmdResult bl_amd_cost_model_set(const float* params);
+/* Re-executes the context's resident frame with exact counting and returns,
+ * per fill command with source geometry, its 16 features and the targets
+ * log2(1 + segments), log2(1 + (segment, band) copies).  `n_out` receives
+ * the number of rows (rows beyond `cap` are not written). */
+BLAmdResult bl_amd_context_collect_costs(BLAmdContext* ctx, float* features, float* targets, size_t cap, size_t* n_out);
+/* MLP forward pass over n rows on the GPU. */
+BLAmdResult bl_amd_cost_model_forward(const float* features, size_t n, const float* params, float* outputs);
+/* Gradient of the mean squared error over n rows (and the loss) on the GPU. */
+BLAmdResult bl_amd_cost_model_grad(const float* features, const float* targets, size_t n, const float* params,
+                                   float* grad, float* loss);
+/* Adam training on the GPU, mini-batches of `batch` rows in the given row
+ * order; `params` holds the initial parameters and receives the trained
+ * ones; `epoch_loss` (optional, `epochs` floats) the mean loss per epoch. */
+BLAmdResult bl_amd_cost_model_train(const float* features, const float* targets, size_t n, float* params,
+                                    uint32_t epochs, uint32_t batch, float lr, float* epoch_loss);
 
 #ifdef __cplusplus
 }
