@@ -80,18 +80,16 @@ def _workload(name: str, seed: int):
 
 
 class _Frame:
-    """One recorded frame on the GPU: a single context, a batch of contexts
-    (one set of kernel launches for all canvases), or -- `plain` -- every
-    canvas flushed and replayed on its own (one set of launches per canvas)."""
+    """One recorded frame on the GPU: a single context, or a batch of
+    contexts (one set of kernel launches for all canvases)."""
 
-    def __init__(self, scenes, plain: bool = False):
+    def __init__(self, scenes):
         import blend2d_amd as B
         self.B = B
         self.scenes = scenes
         self.keep = []
         self.targets = []
-        self.plain = plain
-        self.batch = B.BLBatch() if len(scenes) > 1 and not plain else None
+        self.batch = B.BLBatch() if len(scenes) > 1 else None
         self.errors = []
         for sc in scenes:
             images = []
@@ -113,26 +111,16 @@ class _Frame:
         if self.batch is not None:
             self.batch.flush()
         else:
-            for ctx, _ in self.keep:
-                ctx.flush()
+            self.keep[0][0].flush()
 
     def replay(self):
         if self.batch is not None:
             self.batch.replay()
         else:
-            for ctx, _ in self.keep:
-                ctx.replay()
+            self.keep[0][0].replay()
 
     def stats(self):
-        if self.batch is not None:
-            return self.batch.last_stats()
-        if len(self.keep) == 1:
-            return self.keep[0][0].last_stats()
-        out = {}
-        for ctx, _ in self.keep:
-            for k, v in ctx.last_stats().items():
-                out[k] = out.get(k, 0) + v
-        return out
+        return self.batch.last_stats() if self.batch is not None else self.keep[0][0].last_stats()
 
     def download(self):
         return [img.download() for img in self.targets]
@@ -151,22 +139,80 @@ def _time_frame(frame, steps, warmup):
     return time.perf_counter() - t0, render_ms
 
 
-def _batched_vs_plain(seed: int, steps: int):
-    """The 4096-canvas glyph workload rendered plain (one flush per canvas)
-    and batched (one flush for all canvases)."""
+def _batched_vs_plain(seed: int, repeats: int, learned: bool):
+    """The 4096-canvas glyph workload, device pipeline time only (recording
+    of the calls excluded): plain = one flush per canvas through one
+    context (exact counting); batched = the same calls of all canvases in
+    one batch flush (learned sizing when a trained model is installed), and
+    its replay from device-resident inputs."""
+    import blend2d_amd as B
     from blend2d_amd import scene as S
     scenes = S.glyph_canvases(seed, 4096, 200)
     out = {"workload": "4096 canvases 32-256 px (half A8, half PRGB32), 1-200 glyph contours each",
-           "paths": int(sum(sc.fill_count for sc in scenes))}
-    for mode in ("plain", "batched"):
-        frame = _Frame(scenes, plain=(mode == "plain"))
-        el, _ = _time_frame(frame, steps, 1)
-        px = frame.stats()["pixels"]
-        out[mode + "_mpix_per_s"] = round(px * steps / el / 1e6, 3)
-        out[mode + "_ms_per_frame"] = round(el * 1e3 / steps, 3)
-        del frame
+           "paths": int(sum(sc.fill_count for sc in scenes)),
+           "batched_sizing": "learned" if learned else "exact"}
+    targets = [B.BLImage(sc.width, sc.height, sc.format) for sc in scenes]
+
+    B.set_sizing(B.SIZING_EXACT)
+    best = None
+    for _ in range(repeats):
+        ctx = B.BLContext()
+        el = 0.0
+        px = 0
+        for sc, img in zip(scenes, targets):
+            img.fill_raw(sc.clear_value)
+            ctx.begin(img)
+            B.replay_into(ctx, sc, [])
+            _sync()
+            t0 = time.perf_counter()
+            ctx.flush()
+            el += time.perf_counter() - t0
+            px += ctx.last_stats()["pixels"]
+            ctx.end()
+        best = el if best is None else min(best, el)
+    out["plain_ms_per_frame"] = round(best * 1e3, 3)
+    out["plain_mpix_per_s"] = round(px / best / 1e6, 3)
+    out["plain_paths_per_s"] = round(out["paths"] / best, 1)
+
+    B.set_sizing(B.SIZING_LEARNED if learned else B.SIZING_EXACT)
+    best = None
+    for _ in range(repeats):
+        batch = B.BLBatch()
+        ctxs = []
+        for sc, img in zip(scenes, targets):
+            img.fill_raw(sc.clear_value)
+            ctx = B.BLContext(img)
+            B.replay_into(ctx, sc, [])
+            batch.add(ctx)
+            ctxs.append(ctx)
+        _sync()
+        t0 = time.perf_counter()
+        batch.flush()
+        el = time.perf_counter() - t0
+        st = batch.last_stats()
+        best = el if best is None else min(best, el)
+    out["batched_ms_per_frame"] = round(best * 1e3, 3)
+    out["batched_mpix_per_s"] = round(st["pixels"] / best / 1e6, 3)
+    out["batched_paths_per_s"] = round(out["paths"] / best, 1)
+    out["batched_fallback"] = st.get("fallback")
+    el, _ = _time_frame(_BatchHandle(batch), 3, 1)
+    out["batched_replay_ms_per_frame"] = round(el * 1e3 / 3, 3)
+    for c in ctxs:
+        c.end()
+    B.set_sizing(B.SIZING_EXACT)
     out["batched_over_plain"] = round(out["batched_mpix_per_s"] / out["plain_mpix_per_s"], 3)
     return out
+
+
+class _BatchHandle:
+    def __init__(self, batch):
+        self.batch = batch
+
+    def replay(self):
+        self.batch.replay()
+
+    def stats(self):
+        return self.batch.last_stats()
 
 
 def _render_bytes(st, scenes):
@@ -239,6 +285,7 @@ def main():
         sys.exit(2)
 
     import blend2d_amd as B
+    from blend2d_amd import train_cost_model
     if B.lib().bl_amd_device_count() < 1:
         raise SystemExit("no HIP device visible")
 
@@ -248,8 +295,25 @@ def main():
     st0 = frame.stats()
     print(f"# workload {args.workload}: {st0}", file=sys.stderr, flush=True)
 
-    el, render_ms = _time_frame(frame, args.steps, args.warmup)
-    st = frame.stats()
+    # Sizing: exact counting, and learned sizing when trained parameters are
+    # present; the faster one is the headline (pixels are identical).
+    sizing = {}
+    params = train_cost_model.load()
+    modes = [("exact", B.SIZING_EXACT)] + ([("learned", B.SIZING_LEARNED)] if params is not None else [])
+    if params is not None:
+        B.cost_model_set(params)
+    best = None
+    for name, mode in modes:
+        B.set_sizing(mode)
+        el_m, render_m = _time_frame(frame, args.steps, args.warmup)
+        st_m = frame.stats()
+        sizing[name] = {"ms_per_step": round(el_m * 1e3 / args.steps, 4), "syncs": st_m.get("syncs"),
+                        "fallback": st_m.get("fallback"), "pred_segs": st_m.get("pred_segs")}
+        if best is None or el_m < best[1]:
+            best = (name, el_m, render_m, st_m)
+    B.set_sizing(B.SIZING_EXACT)
+    config["sizing"] = best[0]
+    _, el, render_ms, st = best
 
     pixels = st["pixels"]
     ms_per_step = el * 1e3 / args.steps
@@ -277,6 +341,7 @@ def main():
         "canvas_mpix_per_s": round(canvas_px * args.steps / el / 1e6, 3),
         "pixels_per_frame": int(pixels),
         "render_ms": round(avg_render_ms, 4),
+        "sizing": sizing,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_render",
@@ -293,7 +358,7 @@ def main():
         line["errors"] = frame.errors
     if not args.no_batching:
         print("# batched vs plain (4096 canvases)", file=sys.stderr, flush=True)
-        line["batched_vs_plain"] = _batched_vs_plain(args.seed, 3)
+        line["batched_vs_plain"] = _batched_vs_plain(args.seed, 2, params is not None)
     if not args.no_cpu:
         cpu, parity = _cpu_baseline(scenes, first, pixels, args.cpu_seconds)
         line["cpu_baseline"] = cpu

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -99,6 +100,33 @@ uint32_t g_sizing = kSizeExact;
 std::vector<float> g_model;
 uint64_t g_model_version = 0;
 
+// The device's work queue, shared by every context of the process: one HIP
+// stream, the timing events and a pinned read-back area.  Frames of
+// different contexts are executed one after another (the mutex also makes
+// flushes from different threads safe).  Created on first use and kept
+// until the process ends.
+struct DeviceQueue {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint32_t* host_readback = nullptr;
+  bool ready = false;
+  bool init() {
+    if (ready) return true;
+    if (!stream && hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) { stream = nullptr; return false; }
+    if (!ev0 && hipEventCreate(&ev0) != hipSuccess) { ev0 = nullptr; return false; }
+    if (!ev1 && hipEventCreate(&ev1) != hipSuccess) { ev1 = nullptr; return false; }
+    if (!host_readback && hipHostMalloc((void**)&host_readback, 256, hipHostMallocDefault) != hipSuccess) { host_readback = nullptr; return false; }
+    ready = true;
+    return true;
+  }
+};
+
+DeviceQueue& device_queue() {
+  static DeviceQueue* q = new DeviceQueue();
+  return *q;
+}
+
 // Device-side state of the pipeline: the resident copy of the last uploaded
 // frame and the scratch of its stages.  A frame stays resident until the next
 // upload, so it can be executed again (replayed) with no host-side work.
@@ -123,18 +151,15 @@ struct Runner {
   bool collect = false;
   std::vector<float> collected_x, collected_t;
 
+  // Binds the process-wide queue (caller holds its mutex).
   bool init() {
-    if (stream) return true;
-    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return false;
-    if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess) return false;
-    if (hipHostMalloc((void**)&host_readback, 64, hipHostMallocDefault) != hipSuccess) return false;
+    DeviceQueue& q = device_queue();
+    if (!q.init()) return false;
+    stream = q.stream;
+    ev0 = q.ev0;
+    ev1 = q.ev1;
+    host_readback = q.host_readback;
     return true;
-  }
-  ~Runner() {
-    if (host_readback) (void)hipHostFree(host_readback);
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
   }
 
   uint32_t upload(FrameHost& fh);
@@ -144,9 +169,14 @@ struct Runner {
   uint32_t execute_pool(Stats& stats, uint32_t mode);
   bool model_views(cm::FrameModel& m);
   uint32_t run(FrameHost& fh, Stats& stats) {
+    std::lock_guard<std::mutex> lock(device_queue().mu);
     stats = Stats{};
     uint32_t r = upload(fh);
     return r ? r : execute(stats);
+  }
+  uint32_t replay(Stats& stats) {
+    std::lock_guard<std::mutex> lock(device_queue().mu);
+    return execute(stats);
   }
 };
 
@@ -1471,7 +1501,7 @@ BLAmdResult bl_amd_context_get_stats(const BLAmdContext* ctx, BLAmdStats* out) {
 
 BLAmdResult bl_amd_context_replay(BLAmdContext* ctx) {
   if (uint32_t r = check_target(ctx)) return r;
-  return ctx->runner.execute(ctx->stats);
+  return ctx->runner.replay(ctx->stats);
 }
 
 BLAmdResult bl_amd_batch_create(BLAmdBatch** out) {
@@ -1537,7 +1567,7 @@ BLAmdResult bl_amd_batch_flush(BLAmdBatch* b) {
 
 BLAmdResult bl_amd_batch_replay(BLAmdBatch* b) {
   if (!b) return kErrInvalid;
-  return b->runner.execute(b->stats);
+  return b->runner.replay(b->stats);
 }
 
 // ---------------------------------------------------------------------------
@@ -1561,7 +1591,7 @@ BLAmdResult bl_amd_context_collect_costs(BLAmdContext* ctx, float* features, flo
   if (!ctx || !n_out) return kErrInvalid;
   if (uint32_t r = check_target(ctx)) return r;
   ctx->runner.collect = true;
-  uint32_t r = ctx->runner.execute(ctx->stats);
+  uint32_t r = ctx->runner.replay(ctx->stats);
   ctx->runner.collect = false;
   if (r) return r;
   const size_t rows = ctx->runner.collected_t.size() / cm::kOut;

@@ -458,6 +458,61 @@ __device__ __forceinline__ uint32_t row_excl_scan16(uint32_t v) {
   return incl - v;
 }
 
+// Per-command record staged in LDS for the render loop.
+struct RenderRecData {
+  DevCmd cmd;
+  DevStyle style;
+  int32_t min_y;       // fill-row origin of analytic fills (24.8)
+  int32_t pad;
+};
+union RenderRec {
+  RenderRecData r;
+  uint32_t w[sizeof(RenderRecData) / 4];
+};
+constexpr int kRecWords = (int)(sizeof(RenderRecData) / 4);
+static_assert(kRecWords <= kTileThreads, "one record word per thread");
+static_assert(sizeof(DevCmd) % 8 == 0, "DevStyle must stay 8-byte aligned in the record");
+
+// Word `w` of the record of command `cmd_i` (style index `style_i`).
+__device__ __forceinline__ uint32_t rec_word(const FrameDev& f, const FrameScratch& s, uint32_t cmd_i, uint32_t style_i, int w) {
+  constexpr int kCmdW = (int)(sizeof(DevCmd) / 4), kStyleW = (int)(sizeof(DevStyle) / 4);
+  if (w < kCmdW) return ((const uint32_t*)(f.cmds + cmd_i))[w];
+  if (w < kCmdW + kStyleW) return ((const uint32_t*)(f.styles + style_i))[w - kCmdW];
+  if (w == kCmdW + kStyleW) return (uint32_t)s.bounds[cmd_i].min_y;
+  return 0u;
+}
+
+// One binned segment of an analytic fill into the tile's LDS cells.
+__device__ __forceinline__ void raster_seg(const DevSeg& g, uint32_t* cells, uint32_t* backdrop, int X0, int Y0) {
+  uint32_t sign = (uint32_t)g.x1 >> 31;
+  int32_t x1 = g.x1 & 0x7FFFFFFF;
+  int32_t cmin = (g.x0 < x1 ? g.x0 : x1) >> 8;
+  int32_t cmax = (g.x0 < x1 ? x1 : g.x0) >> 8;
+  if (cmin >= X0 + kTileW) return;
+  if (cmax + 1 < X0) {
+    // All cells of this segment are left of the tile: each covered row
+    // receives its signed vertical extent as a backdrop.
+    int r0 = g.y0 >> 8, r1 = (g.y1 - 1) >> 8;
+    if (r0 < Y0) r0 = Y0;
+    if (r1 > Y0 + kTileH - 1) r1 = Y0 + kTileH - 1;
+    uint32_t smask = 0u - sign;
+    for (int r = r0; r <= r1; r++) {
+      int lo_y = g.y0 > r * 256 ? g.y0 : r * 256;
+      int hi_y = g.y1 < (r + 1) * 256 ? g.y1 : (r + 1) * 256;
+      uint32_t cover = ras::apply_sign((uint32_t)(hi_y - lo_y), smask);
+      atomicAdd(&backdrop[r - Y0], cover << 9);
+    }
+    return;
+  }
+  TileSink sink{cells, backdrop, X0, Y0};
+  ras::Rasterizer rz;
+  rz.sign_mask = 0u - sign;
+  if (!rz.prepare(g.x0, g.y0, x1, g.y1)) return;
+  if (rz.st.ey0 < Y0) rz.advance_to_y(Y0);
+  rz.band_end = Y0 + kTileH - 1;
+  rz.rasterize(sink);
+}
+
 __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratch s) {
   __shared__ uint32_t cells[kTileH * kTileW];
   __shared__ uint32_t backdrop[kTileH];
@@ -523,22 +578,31 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
   const uint32_t lb = s.list_off[row_base];
   const uint32_t le = s.list_off[row_base + cv.n_cmds];
   const uint32_t tcol = local % cv.tiles_x;
+  const int lane = t & 63, wave = t >> 6;
   bool touched = false;
   uint32_t npix = 0;
 
   // The band's command list is filtered 256 entries at a time down to the
-  // commands whose tile-column range holds this tile (order preserved by a
-  // ballot compaction), then those are applied one by one.
-  __shared__ uint32_t sel[kTileThreads];
+  // commands whose tile-column range holds this tile (ballot compaction,
+  // submission order kept); the filter also gathers, in parallel, each
+  // selected command's (band, command) cell extent and style index.
+  __shared__ uint32_t sel_cmd[kTileThreads], sel_off[kTileThreads], sel_cnt[kTileThreads], sel_style[kTileThreads];
   __shared__ uint32_t sel_wave[kTileThreads / 64];
-  const int lane = t & 63, wave = t >> 6;
+  // Command record, style and fill-row origin of the command being applied
+  // and of the next one (double buffer): the next command's record and its
+  // first 256 segments are loaded while the current one is rasterized and
+  // composited, so the per-command chain of dependent global loads is off
+  // the critical path.
+  __shared__ RenderRec rec[2];
+
   for (uint32_t chunk = lb; chunk < le; chunk += kTileThreads) {
     const uint32_t li = chunk + (uint32_t)t;
-    uint32_t e = 0;
+    uint32_t e = 0, cmd_i = 0;
     bool keep = false;
     if (li < le) {
       e = s.list[li];
-      const uint32_t tr = s.cmd_tiles[cv.cmd_base + (uint32_t)(e - row_base)];
+      cmd_i = cv.cmd_base + (uint32_t)(e - row_base);
+      const uint32_t tr = s.cmd_tiles[cmd_i];
       keep = tcol >= (tr & 0xFFFFu) && tcol <= (tr >> 16);
     }
     const uint64_t bal = __ballot(keep);
@@ -551,68 +615,53 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
       off += w < wave ? cw : 0u;
       n_sel += cw;
     }
-    if (keep) sel[off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = e;
+    if (keep) {
+      const uint32_t slot = off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+      sel_cmd[slot] = cmd_i;
+      sel_off[slot] = s.mat_off[e];
+      sel_cnt[slot] = s.mat_cnt[e];
+      sel_style[slot] = f.cmds[cmd_i].style;
+    }
+    __syncthreads();
+    if (n_sel == 0) continue;   // uniform; sel_wave is rewritten only after the next chunk's ballot
+
+    // first command of the chunk
+    uint32_t pre_w = t < kRecWords ? rec_word(f, s, sel_cmd[0], sel_style[0], t) : 0u;
+    DevSeg pre_seg = (uint32_t)t < sel_cnt[0] ? s.binned[sel_off[0] + t] : DevSeg{0, 0, 0, 0};
+    if (t < kRecWords) rec[0].w[t] = pre_w;
     __syncthreads();
 
-  for (uint32_t ks = 0; ks < n_sel; ks++) {
-    const uint32_t e = sel[ks];
-    const uint32_t cmd_i = cv.cmd_base + (uint32_t)(e - row_base);
-    const DevCmd c = f.cmds[cmd_i];
-    uint32_t m[4] = {0, 0, 0, 0};
-
-    if (c.kind == kCmdAnalytic) {
-      const CmdBounds bd = s.bounds[cmd_i];
-      if (bd.max_x + 1 < X0 || bd.min_x >= X0 + kTileW) continue;
-      const uint32_t sb = s.mat_off[e];
-      const uint32_t se = sb + s.mat_cnt[e];
-      __syncthreads();   // previous command's sweep finished clearing cells
-      TileSink sink{cells, backdrop, X0, Y0};
-      for (uint32_t si = sb + t; si < se; si += kTileThreads) {
-        DevSeg g = s.binned[si];
-        uint32_t sign = (uint32_t)g.x1 >> 31;
-        int32_t x1 = g.x1 & 0x7FFFFFFF;
-        int32_t cmin = (g.x0 < x1 ? g.x0 : x1) >> 8;
-        int32_t cmax = (g.x0 < x1 ? x1 : g.x0) >> 8;
-        if (cmin >= X0 + kTileW) continue;
-        if (cmax + 1 < X0) {
-          // All cells of this segment are left of the tile: each covered row
-          // receives its signed vertical extent as a backdrop.
-          int r0 = g.y0 >> 8, r1 = (g.y1 - 1) >> 8;
-          if (r0 < Y0) r0 = Y0;
-          if (r1 > Y0 + kTileH - 1) r1 = Y0 + kTileH - 1;
-          uint32_t smask = 0u - sign;
-          for (int r = r0; r <= r1; r++) {
-            int lo_y = g.y0 > r * 256 ? g.y0 : r * 256;
-            int hi_y = g.y1 < (r + 1) * 256 ? g.y1 : (r + 1) * 256;
-            uint32_t cover = ras::apply_sign((uint32_t)(hi_y - lo_y), smask);
-            atomicAdd(&backdrop[r - Y0], cover << 9);
-          }
-          continue;
-        }
-        ras::Rasterizer rz;
-        rz.sign_mask = 0u - sign;
-        if (!rz.prepare(g.x0, g.y0, x1, g.y1)) continue;
-        if (rz.st.ey0 < Y0) rz.advance_to_y(Y0);
-        rz.band_end = Y0 + kTileH - 1;
-        rz.rasterize(sink);
+    for (uint32_t ks = 0; ks < n_sel; ks++) {
+      const int cur = (int)(ks & 1u);
+      const DevCmd& c = rec[cur].r.cmd;
+      const DevSeg seg = pre_seg;
+      const uint32_t cnt = sel_cnt[ks], soff = sel_off[ks];
+      const bool has_next = ks + 1 < n_sel;
+      if (has_next) {
+        if (t < kRecWords) pre_w = rec_word(f, s, sel_cmd[ks + 1], sel_style[ks + 1], t);
+        pre_seg = (uint32_t)t < sel_cnt[ks + 1] ? s.binned[sel_off[ks + 1] + t] : DevSeg{0, 0, 0, 0};
       }
-      __syncthreads();
+      uint32_t m[4] = {0, 0, 0, 0};
 
-      uint4 cv4 = *(uint4*)&cells[row * kTileW + q * 4];
-      uint32_t bdv = backdrop[row];
-      *(uint4*)&cells[row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
-      if (q == 0) backdrop[row] = 0;
-      uint32_t sum = cv4.x + cv4.y + cv4.z + cv4.w;
-      uint32_t base = (256u << 9) + bdv + row_excl_scan16(sum);
-      uint32_t c0 = base + cv4.x, c1 = c0 + cv4.y, c2 = c1 + cv4.z, c3 = c2 + cv4.w;
-      m[0] = calc_mask(c0, c.rule_mask, c.alpha);
-      m[1] = calc_mask(c1, c.rule_mask, c.alpha);
-      m[2] = calc_mask(c2, c.rule_mask, c.alpha);
-      m[3] = calc_mask(c3, c.rule_mask, c.alpha);
-    }
-    else {
-      if (c.box[2] <= X0 || c.box[0] >= X0 + kTileW) continue;
-      if (y >= c.box[1] && y < c.box[3]) {
+      if (c.kind == kCmdAnalytic) {
+        if ((uint32_t)t < cnt) raster_seg(seg, cells, backdrop, X0, Y0);
+        for (uint32_t si = soff + kTileThreads + t; si < soff + cnt; si += kTileThreads) raster_seg(s.binned[si], cells, backdrop, X0, Y0);
+        __syncthreads();
+
+        uint4 cv4 = *(uint4*)&cells[row * kTileW + q * 4];
+        uint32_t bdv = backdrop[row];
+        *(uint4*)&cells[row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
+        uint32_t sum = cv4.x + cv4.y + cv4.z + cv4.w;
+        uint32_t base = (256u << 9) + bdv + row_excl_scan16(sum);
+        // every lane of the row has read the backdrop before the scan
+        if (q == 0) backdrop[row] = 0;
+        uint32_t c0 = base + cv4.x, c1 = c0 + cv4.y, c2 = c1 + cv4.z, c3 = c2 + cv4.w;
+        m[0] = calc_mask(c0, c.rule_mask, c.alpha);
+        m[1] = calc_mask(c1, c.rule_mask, c.alpha);
+        m[2] = calc_mask(c2, c.rule_mask, c.alpha);
+        m[3] = calc_mask(c3, c.rule_mask, c.alpha);
+      }
+      else if (y >= c.box[1] && y < c.box[3]) {
         if (c.kind == kCmdBoxA) {
 #pragma unroll
           for (int k = 0; k < 4; k++) m[k] = (px + k >= c.box[0] && px + k < c.box[2]) ? c.alpha : 0u;
@@ -630,23 +679,23 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
           }
         }
       }
-    }
 
-    if (!row_ok || (m[0] | m[1] | m[2] | m[3]) == 0) continue;
-    touched = true;
-
-    const DevStyle& st = f.styles[c.style];
-    const int fill_row0 = (c.kind == kCmdAnalytic) ? (s.bounds[cmd_i].min_y >> 8) : (int)c.fetch_row0;
-    const pipe::Row rs = pipe::row_setup(st, y, fill_row0, c.kind == kCmdAnalytic ? cv.ref_band_h : 0u);
+      if (row_ok && (m[0] | m[1] | m[2] | m[3]) != 0) {
+        touched = true;
+        const DevStyle& st = rec[cur].r.style;
+        const int fill_row0 = (c.kind == kCmdAnalytic) ? (rec[cur].r.min_y >> 8) : (int)c.fetch_row0;
+        const pipe::Row rs = pipe::row_setup(st, y, fill_row0, c.kind == kCmdAnalytic ? cv.ref_band_h : 0u);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      if (!m[k] || px + k >= (int)cv.w) continue;
-      uint32_t src = pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
-      d[k] = a8 ? pipe::composite8(c.op, d[k], src >> 24, m[k]) : pipe::composite32(c.op, d[k], src, m[k]);
-      npix++;
+        for (int k = 0; k < 4; k++) {
+          if (!m[k] || px + k >= (int)cv.w) continue;
+          uint32_t src = pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
+          d[k] = a8 ? pipe::composite8(c.op, d[k], src >> 24, m[k]) : pipe::composite32(c.op, d[k], src, m[k]);
+          npix++;
+        }
+      }
+      if (has_next && t < kRecWords) rec[cur ^ 1].w[t] = pre_w;
+      __syncthreads();   // next record visible; cells cleared; this record no longer read
     }
-  }
-    __syncthreads();   // sel / sel_wave are rewritten by the next chunk
   }
 
   // Composited-pixel statistics: one atomic per wave.
