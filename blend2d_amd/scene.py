@@ -724,8 +724,8 @@ def cost_canvases(seed: int = 1, count: int = 256, side_lo: int = 1024, side_hi:
         for j in range(k):
             n = int(segs[j])
             side = float(2.0 ** rng.uniform(4, math.log2(max(w, h))))
-            bx = float(rng.uniform(-side * 0.3, w - side * 0.7))
-            by = float(rng.uniform(-side * 0.3, h - side * 0.7))
+            bx = float(rng.uniform(-side * 0.3, max(w - side * 0.7, 1.0 - side * 0.3)))
+            by = float(rng.uniform(-side * 0.3, max(h - side * 0.7, 1.0 - side * 0.3)))
             mix = int(rng.integers(0, 4))          # 0 lines, 1 quads, 2 cubics, 3 mixed
             kinds = np.full(n, mix) if mix < 3 else rng.integers(0, 3, n)
             per = np.array([1, 2, 3])[kinds]
