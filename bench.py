@@ -299,6 +299,11 @@ def main():
     # present; the faster one is the headline (pixels are identical).
     sizing = {}
     params = train_cost_model.load()
+    try:
+        B.set_sizing(B.SIZING_LEARNED)
+        B.set_sizing(B.SIZING_EXACT)
+    except B.BLError:
+        params = None          # learned sizing not available in this build
     modes = [("exact", B.SIZING_EXACT)] + ([("learned", B.SIZING_LEARNED)] if params is not None else [])
     if params is not None:
         B.cost_model_set(params)

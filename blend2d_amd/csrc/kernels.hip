@@ -82,8 +82,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __r
     if (base + k < n) out[base + k] = p;
     p += x[k];
   }
-  // The element one past the end receives the grand total.
-  if (base <= n && n < base + kScanItems) {
+  // The element one past the end receives the grand total, written by the
+  // thread that owns the last element (thread 0 of block 0 when n == 0).
+  const bool owner = n ? (base <= n - 1 && n - 1 < base + kScanItems) : base == 0;
+  if (owner) {
     uint32_t tail = (prefix ? prefix[blockIdx.x] : 0u) + excl;
     for (int k = 0; k < kScanItems && base + k < n; k++) tail += x[k];
     out[n] = tail;

@@ -102,3 +102,35 @@ def test_replay_reproduces_the_frame(amd):
     assert st1["pixels"] > 0
     want, _ = checkers.render_oracle(sc)
     assert checkers.diff_info(first, want)[0] == 0
+
+
+def _star(n, cx, cy, r0, r1, phase=0.0):
+    a = np.arange(n) * (2 * np.pi / n) + phase
+    r = np.where(np.arange(n) % 2 == 0, r0, r1)
+    return np.stack([cx + np.cos(a) * r, cy + np.sin(a) * r], 1)
+
+
+def scan_boundary_scenes():
+    """Frames whose scanned arrays have exactly 1024 * k elements (the scan
+    block): (band, command) matrices of 1024 and 2048 cells and source
+    segment counts of 1024, 2048 and 4096 -- the sizes at which the grand
+    total of a scan is written by the last thread of the last block."""
+    out = []
+    rng = np.random.default_rng(77)
+    for h, n in ((128, 128), (256, 128)):              # 8 or 16 bands x 128 commands
+        sc = S.Scene(64, h, S.FMT_PRGB32, 0xFF000000)
+        for _ in range(n):
+            sc.set_fill_style_rgba32(int(rng.integers(0x40000000, 1 << 32)))
+            sc.fill_polygon(rng.random((5, 2)) * np.array([64, h]))
+        out.append(sc)
+    for n in (1024, 2048, 4096):                         # one polygon: n line sources
+        sc = S.Scene(300, 200, S.FMT_PRGB32, 0xFF102030)
+        sc.set_fill_style_rgba32(0xC08040FF)
+        sc.fill_polygon(_star(n, 150, 100, 90, 40))
+        out.append(sc)
+    return out
+
+
+def test_scan_block_boundaries(amd):
+    for sc in scan_boundary_scenes():
+        _cmp(amd, sc)
