@@ -277,12 +277,26 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batching", action="store_true", help="skip the 4096-canvas batched-vs-plain measurement")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (scale for FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(a frame's tiles are rendered by one device)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+        n = 64 * 1024 * 1024
+        x = torch.ones(n, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(f"copy probe: 3 launches, {4 * n} bytes read and {4 * n} bytes written each", flush=True)
+        return
 
     import blend2d_amd as B
     from blend2d_amd import train_cost_model

@@ -142,6 +142,7 @@ struct Runner {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf in;
   DevBuf src_count, src_off, bounds, segs, seg_cmd, mat_cnt, mat_off, mat_pres, list_off, list, binned, cursor, scan_tmp, counters, cmd_tiles;
+  DevBuf tile_canvas, tile_px;
   DevBuf cm_params, cm_acc, cm_feat, cm_pred, cm_act;
   uint64_t cm_version = 0;
   uint32_t* host_readback = nullptr;
@@ -314,11 +315,14 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   if (!bounds.reserve((size_t)f.n_cmds * sizeof(CmdBounds)) || !mat_cnt.reserve((n_mat + 1) * 4) ||
       !mat_off.reserve((n_mat + 1) * 4) || !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
       !list.reserve((n_mat + 1) * 4) || !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) ||
-      !counters.reserve(64) || !cmd_tiles.reserve((size_t)f.n_cmds * 4))
+      !counters.reserve(64) || !cmd_tiles.reserve((size_t)f.n_cmds * 4) ||
+      !tile_canvas.reserve((size_t)f.n_tiles * 4) || !tile_px.reserve((size_t)f.n_tiles * 8))
     return kErrOOM;
 
   FrameScratch s;
   memset(&s, 0, sizeof(s));
+  s.tile_canvas = tile_canvas.as<uint32_t>();
+  s.tile_px = tile_px.as<unsigned long long>();
   s.bounds = bounds.as<CmdBounds>();
   s.mat_cnt = mat_cnt.as<uint32_t>();
   s.mat_off = mat_off.as<uint32_t>();
@@ -381,9 +385,11 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   launch_bin_scatter(f, s, cap_segs, cursor.as<uint32_t>(), stream);
   launch_list_build(f, s, stream);
 
+  launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
   launch_render(f, s, stream);
   HIP_OK(hipEventRecord(ev1, stream));
+  launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback, counters.ptr, 16, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipMemcpyAsync(host_readback + 8, s.mat_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipStreamSynchronize(stream));
@@ -425,11 +431,14 @@ uint32_t Runner::execute_exact(Stats& stats) {
       !mat_cnt.reserve((n_mat + 1) * 4) || !mat_off.reserve((n_mat + 1) * 4) ||
       !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
       !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) || !counters.reserve(64) ||
-      !cmd_tiles.reserve((size_t)f.n_cmds * 4))
+      !cmd_tiles.reserve((size_t)f.n_cmds * 4) || !tile_canvas.reserve((size_t)f.n_tiles * 4) ||
+      !tile_px.reserve((size_t)f.n_tiles * 8))
     return kErrOOM;
 
   FrameScratch s;
   memset(&s, 0, sizeof(s));
+  s.tile_canvas = tile_canvas.as<uint32_t>();
+  s.tile_px = tile_px.as<unsigned long long>();
   s.src_count = src_count.as<uint32_t>();
   s.src_off = src_off.as<uint32_t>();
   s.bounds = bounds.as<CmdBounds>();
@@ -488,9 +497,11 @@ uint32_t Runner::execute_exact(Stats& stats) {
   launch_list_build(f, s, stream);
 
   // Stage 3: render.
+  launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
   launch_render(f, s, stream);
   HIP_OK(hipEventRecord(ev1, stream));
+  launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback + 2, s.pixels, 8, hipMemcpyDeviceToHost, stream));
   if (collect) {
     cm::launch_actuals(f, s, model, n_segs, stream);

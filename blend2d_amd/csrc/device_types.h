@@ -100,22 +100,30 @@ struct DevStyle {
   uint32_t maxi, rori;
   uint32_t img;          // image index
   uint32_t pad0;
-  // linear
-  uint64_t lin_pt0, lin_dt, lin_dy;
-  // radial
-  double rad_tx, rad_ty, rad_yx, rad_yy, rad_b0, rad_by, rad_dd0, rad_ddy;
-  double rad_inv2a, rad_amul4, rad_sq_inv2a, rad_sq_fr;
-  float rad_bd, rad_ddd;
-  // conic
-  double con_tx, con_ty, con_yx, con_yy;
-  float con_q[4];
-  float con_n1, con_n2, con_n4, con_offset, con_xx;
-  int32_t con_maxi, con_rori;
-  // affine pattern
-  uint64_t pa_xx, pa_xy, pa_yx, pa_yy, pa_tx, pa_ty;
-  int32_t pa_ox, pa_oy, pa_rx, pa_ry, pa_minx, pa_miny, pa_maxx, pa_maxy;
-  int32_t pa_corx, pa_cory, pa_tw, pa_th;
+  // Fetcher parameters; only those of `kind` are meaningful.
+  union {
+    struct {             // linear
+      uint64_t lin_pt0, lin_dt, lin_dy;
+    };
+    struct {             // radial
+      double rad_tx, rad_ty, rad_yx, rad_yy, rad_b0, rad_by, rad_dd0, rad_ddy;
+      double rad_inv2a, rad_amul4, rad_sq_inv2a, rad_sq_fr;
+      float rad_bd, rad_ddd;
+    };
+    struct {             // conic
+      double con_tx, con_ty, con_yx, con_yy;
+      float con_q[4];
+      float con_n1, con_n2, con_n4, con_offset, con_xx;
+      int32_t con_maxi, con_rori;
+    };
+    struct {             // affine pattern
+      uint64_t pa_xx, pa_xy, pa_yx, pa_yy, pa_tx, pa_ty;
+      int32_t pa_ox, pa_oy, pa_rx, pa_ry, pa_minx, pa_miny, pa_maxx, pa_maxy;
+      int32_t pa_corx, pa_cory, pa_tw, pa_th;
+    };
+  };
 };
+static_assert(sizeof(DevStyle) == 136, "DevStyle layout");
 
 struct DevImage {
   uint64_t ptr;          // device address of pixel row 0

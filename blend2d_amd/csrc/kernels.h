@@ -39,8 +39,10 @@ struct FrameScratch {
   uint32_t* list;          // n_list
   DevSeg* binned;          // n_binned
   uint32_t* scan_tmp;      // scan_tmp_words(max n)
-  unsigned long long* pixels;  // composited pixels (mask > 0), accumulated by render
+  unsigned long long* pixels;  // composited pixels (mask > 0), summed after render
   uint32_t* cmd_tiles;     // n_cmds: tile-column range per command
+  uint32_t* tile_canvas;   // n_tiles: canvas index of each tile
+  unsigned long long* tile_px; // n_tiles: composited pixels of each tile
   // Pool mode (sizes predicted ahead of time; null / 0 otherwise):
   uint32_t* seg_cursor;    // segments taken from the pool
   uint32_t* flags;         // bit 0: segment pool overflow, bit 1: bin pool overflow
@@ -61,8 +63,12 @@ void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_bin_scatter(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, uint32_t* cursor, hipStream_t st);
 void launch_list_build(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_cmd_tiles(const FrameDev& f, const FrameScratch& s, hipStream_t st);
-// Stage 3: tile rasterization + fetch + composite.
+// Stage 3: tile -> canvas table (setup), tile rasterization + fetch +
+// composite (render), composited pixels summed into *s.pixels, which must
+// start at zero (stats).
+void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_render(const FrameDev& f, const FrameScratch& s, hipStream_t st);
+void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 void launch_init_bounds(CmdBounds* b, uint32_t n, hipStream_t st);

@@ -472,8 +472,18 @@ union RenderRec {
   uint32_t w[sizeof(RenderRecData) / 4];
 };
 constexpr int kRecWords = (int)(sizeof(RenderRecData) / 4);
-static_assert(kRecWords <= kTileThreads, "one record word per thread");
 static_assert(sizeof(DevCmd) % 8 == 0, "DevStyle must stay 8-byte aligned in the record");
+
+// Commands rasterized together.  The segments of a batch's commands are
+// spread over all lanes of the workgroup, each command into its own LDS cell
+// buffer; after one barrier every thread composites its pixels through the
+// batch's commands in submission order.  Rasterization of different commands
+// is independent, so only the composite has to follow the command order.
+#ifndef B2AMD_RENDER_BATCH
+#define B2AMD_RENDER_BATCH 8
+#endif
+constexpr int kBatch = B2AMD_RENDER_BATCH;
+static_assert(kBatch >= 1 && kBatch * kRecWords <= 2 * kTileThreads, "two record words per thread");
 
 // Word `w` of the record of command `cmd_i` (style index `style_i`).
 __device__ __forceinline__ uint32_t rec_word(const FrameDev& f, const FrameScratch& s, uint32_t cmd_i, uint32_t style_i, int w) {
@@ -482,6 +492,34 @@ __device__ __forceinline__ uint32_t rec_word(const FrameDev& f, const FrameScrat
   if (w < kCmdW + kStyleW) return ((const uint32_t*)(f.styles + style_i))[w - kCmdW];
   if (w == kCmdW + kStyleW) return (uint32_t)s.bounds[cmd_i].min_y;
   return 0u;
+}
+
+// Word `gw` of the records of the batch that starts at selected command `kb`.
+__device__ __forceinline__ uint32_t batch_rec_word(const FrameDev& f, const FrameScratch& s, const uint32_t* sel_cmd,
+                                                   const uint32_t* sel_style, uint32_t kb, int gw) {
+  const int b = gw / kRecWords, w = gw - b * kRecWords;
+  return rec_word(f, s, sel_cmd[kb + b], sel_style[kb + b], w);
+}
+
+// Inclusive prefix of the segment counts of the batch's `nb` commands.
+__device__ __forceinline__ void batch_prefix(const uint32_t* sel_cnt, uint32_t kb, uint32_t nb, uint32_t end[kBatch]) {
+  uint32_t acc = 0;
+#pragma unroll
+  for (int b = 0; b < kBatch; b++) {
+    acc += (uint32_t)b < nb ? sel_cnt[kb + b] : 0u;
+    end[b] = acc;
+  }
+}
+
+// Batch slot and in-command index of the batch's segment `j` (j < end[kBatch - 1]).
+__device__ __forceinline__ uint32_t batch_find(const uint32_t end[kBatch], uint32_t j, uint32_t* local) {
+  uint32_t b = 0, base = 0;
+#pragma unroll
+  for (int i = 0; i < kBatch - 1; i++) {
+    if (j >= end[i]) { b = (uint32_t)i + 1u; base = end[i]; }
+  }
+  *local = j - base;
+  return b;
 }
 
 // One binned segment of an analytic fill into the tile's LDS cells.
@@ -515,27 +553,34 @@ __device__ __forceinline__ void raster_seg(const DevSeg& g, uint32_t* cells, uin
   rz.rasterize(sink);
 }
 
+// Workgroup -> tile.  Workgroups are dealt to the 8 XCDs round-robin; the
+// mapping gives each XCD a contiguous range of tiles, so the tiles of a band
+// (which read the same command lists, records and segments) share one L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t wg, uint32_t n_tiles) {
+  const uint32_t per = (n_tiles + kXcds - 1) / kXcds;
+  return (wg % kXcds) * per + wg / kXcds;
+}
+
 __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratch s) {
-  __shared__ uint32_t cells[kTileH * kTileW];
-  __shared__ uint32_t backdrop[kTileH];
+  __shared__ uint32_t cells[kBatch][kTileH * kTileW];
+  __shared__ uint32_t backdrop[kBatch][kTileH];
+  const uint32_t tile = xcd_tile(blockIdx.x, f.n_tiles);
+  if (tile >= f.n_tiles) return;   // uniform: padding workgroups of the XCD mapping
+  const int t = threadIdx.x;
   // A pool overflow leaves the frame incomplete: nothing is composited and
   // the host redoes the frame with exact counting.
-  if (s.flags && *s.flags) return;
-
-  const uint32_t tile = blockIdx.x;
-  // canvas lookup (uniform)
-  uint32_t lo = 0, hi = f.n_canvases;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (f.canvases[mid].tile_base <= tile) lo = mid; else hi = mid;
+  if (s.flags && *s.flags) {
+    if (t == 0) s.tile_px[tile] = 0;
+    return;
   }
-  const DevCanvas cv = f.canvases[lo];
+
+  const DevCanvas cv = f.canvases[s.tile_canvas[tile]];
   const uint32_t local = tile - cv.tile_base;
   const int band = (int)(local / cv.tiles_x);
   const int X0 = (int)(local % cv.tiles_x) * kTileW;
   const int Y0 = band * kTileH;
 
-  const int t = threadIdx.x;
   const int row = t >> 4;
   const int q = t & 15;
   const int y = Y0 + row;
@@ -544,8 +589,9 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
   const bool a8 = cv.format == kFmtA8;
 
   // zero LDS
-  *(uint4*)&cells[row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
-  if (t < kTileH) backdrop[t] = 0;
+#pragma unroll
+  for (int b = 0; b < kBatch; b++) *(uint4*)&cells[b][row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
+  for (int i = t; i < kBatch * kTileH; i += kTileThreads) backdrop[i / kTileH][i % kTileH] = 0;
 
   // load the destination tile
   uint32_t d[4] = {0, 0, 0, 0};
@@ -581,6 +627,10 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
   const uint32_t le = s.list_off[row_base + cv.n_cmds];
   const uint32_t tcol = local % cv.tiles_x;
   const int lane = t & 63, wave = t >> 6;
+  // Batch segment index of this thread in the first 256 of a batch: the
+  // batch's segments are dealt to the four waves in turn, so every SIMD of
+  // the CU takes part in rasterization.
+  const uint32_t jt = ((uint32_t)lane << 2) | (uint32_t)wave;
   bool touched = false;
   uint32_t npix = 0;
 
@@ -590,12 +640,12 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
   // selected command's (band, command) cell extent and style index.
   __shared__ uint32_t sel_cmd[kTileThreads], sel_off[kTileThreads], sel_cnt[kTileThreads], sel_style[kTileThreads];
   __shared__ uint32_t sel_wave[kTileThreads / 64];
-  // Command record, style and fill-row origin of the command being applied
-  // and of the next one (double buffer): the next command's record and its
-  // first 256 segments are loaded while the current one is rasterized and
-  // composited, so the per-command chain of dependent global loads is off
-  // the critical path.
-  __shared__ RenderRec rec[2];
+  // Records (command, style, fill-row origin) of the batch being applied
+  // and of the next one (double buffer): the next batch's records and first
+  // 256 segments are loaded while the current batch is rasterized and
+  // composited, so the chain of dependent global loads is off the critical
+  // path.
+  __shared__ RenderRec rec[2][kBatch];
 
   for (uint32_t chunk = lb; chunk < le; chunk += kTileThreads) {
     const uint32_t li = chunk + (uint32_t)t;
@@ -627,83 +677,141 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
     __syncthreads();
     if (n_sel == 0) continue;   // uniform; sel_wave is rewritten only after the next chunk's ballot
 
-    // first command of the chunk
-    uint32_t pre_w = t < kRecWords ? rec_word(f, s, sel_cmd[0], sel_style[0], t) : 0u;
-    DevSeg pre_seg = (uint32_t)t < sel_cnt[0] ? s.binned[sel_off[0] + t] : DevSeg{0, 0, 0, 0};
-    if (t < kRecWords) rec[0].w[t] = pre_w;
+    // first batch of the chunk: records and first segments
+    DevSeg pre_seg{0, 0, 0, 0};
+    uint32_t pre_b = 0;
+    {
+      const uint32_t nb = n_sel < (uint32_t)kBatch ? n_sel : (uint32_t)kBatch;
+      uint32_t end[kBatch];
+      batch_prefix(sel_cnt, 0, nb, end);
+      if (t < (int)nb * kRecWords) rec[0][t / kRecWords].w[t % kRecWords] = batch_rec_word(f, s, sel_cmd, sel_style, 0, t);
+      if (t + kTileThreads < (int)nb * kRecWords) {
+        const int gw = t + kTileThreads;
+        rec[0][gw / kRecWords].w[gw % kRecWords] = batch_rec_word(f, s, sel_cmd, sel_style, 0, gw);
+      }
+      if (jt < end[kBatch - 1]) {
+        uint32_t loc;
+        pre_b = batch_find(end, jt, &loc);
+        pre_seg = s.binned[sel_off[pre_b] + loc];
+      }
+    }
     __syncthreads();
 
-    for (uint32_t ks = 0; ks < n_sel; ks++) {
-      const int cur = (int)(ks & 1u);
-      const DevCmd& c = rec[cur].r.cmd;
+    for (uint32_t kb = 0, bi = 0; kb < n_sel; kb += kBatch, bi++) {
+      const int cur = (int)(bi & 1u);
+      const uint32_t nb = n_sel - kb < (uint32_t)kBatch ? n_sel - kb : (uint32_t)kBatch;
+      uint32_t end[kBatch];
+      batch_prefix(sel_cnt, kb, nb, end);
+      const uint32_t total = end[kBatch - 1];
       const DevSeg seg = pre_seg;
-      const uint32_t cnt = sel_cnt[ks], soff = sel_off[ks];
-      const bool has_next = ks + 1 < n_sel;
+      const uint32_t seg_b = pre_b;
+
+      // prefetch the next batch
+      const uint32_t kn = kb + kBatch;
+      const bool has_next = kn < n_sel;
+      uint32_t pw0 = 0, pw1 = 0;
+      int nwn = 0;
       if (has_next) {
-        if (t < kRecWords) pre_w = rec_word(f, s, sel_cmd[ks + 1], sel_style[ks + 1], t);
-        pre_seg = (uint32_t)t < sel_cnt[ks + 1] ? s.binned[sel_off[ks + 1] + t] : DevSeg{0, 0, 0, 0};
-      }
-      uint32_t m[4] = {0, 0, 0, 0};
-
-      if (c.kind == kCmdAnalytic) {
-        if ((uint32_t)t < cnt) raster_seg(seg, cells, backdrop, X0, Y0);
-        for (uint32_t si = soff + kTileThreads + t; si < soff + cnt; si += kTileThreads) raster_seg(s.binned[si], cells, backdrop, X0, Y0);
-        __syncthreads();
-
-        uint4 cv4 = *(uint4*)&cells[row * kTileW + q * 4];
-        uint32_t bdv = backdrop[row];
-        *(uint4*)&cells[row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
-        uint32_t sum = cv4.x + cv4.y + cv4.z + cv4.w;
-        uint32_t base = (256u << 9) + bdv + row_excl_scan16(sum);
-        // every lane of the row has read the backdrop before the scan
-        if (q == 0) backdrop[row] = 0;
-        uint32_t c0 = base + cv4.x, c1 = c0 + cv4.y, c2 = c1 + cv4.z, c3 = c2 + cv4.w;
-        m[0] = calc_mask(c0, c.rule_mask, c.alpha);
-        m[1] = calc_mask(c1, c.rule_mask, c.alpha);
-        m[2] = calc_mask(c2, c.rule_mask, c.alpha);
-        m[3] = calc_mask(c3, c.rule_mask, c.alpha);
-      }
-      else if (y >= c.box[1] && y < c.box[3]) {
-        if (c.kind == kCmdBoxA) {
-#pragma unroll
-          for (int k = 0; k < 4; k++) m[k] = (px + k >= c.box[0] && px + k < c.box[2]) ? c.alpha : 0u;
+        const uint32_t nbn = n_sel - kn < (uint32_t)kBatch ? n_sel - kn : (uint32_t)kBatch;
+        nwn = (int)nbn * kRecWords;
+        if (t < nwn) pw0 = batch_rec_word(f, s, sel_cmd, sel_style, kn, t);
+        if (t + kTileThreads < nwn) pw1 = batch_rec_word(f, s, sel_cmd, sel_style, kn, t + kTileThreads);
+        uint32_t endn[kBatch];
+        batch_prefix(sel_cnt, kn, nbn, endn);
+        pre_seg = DevSeg{0, 0, 0, 0};
+        pre_b = 0;
+        if (jt < endn[kBatch - 1]) {
+          uint32_t loc;
+          pre_b = batch_find(endn, jt, &loc);
+          pre_seg = s.binned[sel_off[kn + pre_b] + loc];
         }
-        else {
-          int kind = (y == c.box_row_first) ? 0 : (y == c.box_row_last) ? 2 : 1;
-          uint32_t rm = c.box_mask[kind];
+      }
+
+      // rasterize the batch
+      if (jt < total) raster_seg(seg, cells[seg_b], backdrop[seg_b], X0, Y0);
+      for (uint32_t j = jt + kTileThreads; j < total; j += kTileThreads) {
+        uint32_t loc;
+        const uint32_t b = batch_find(end, j, &loc);
+        raster_seg(s.binned[sel_off[kb + b] + loc], cells[b], backdrop[b], X0, Y0);
+      }
+      __syncthreads();
+
+      // composite the batch's commands in order
+      for (uint32_t b = 0; b < nb; b++) {
+        const DevCmd& c = rec[cur][b].r.cmd;
+        uint32_t m[4] = {0, 0, 0, 0};
+
+        if (c.kind == kCmdAnalytic) {
+          uint32_t* cb = cells[b];
+          uint4 cv4 = *(uint4*)&cb[row * kTileW + q * 4];
+          uint32_t bdv = backdrop[b][row];
+          *(uint4*)&cb[row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
+          uint32_t sum = cv4.x + cv4.y + cv4.z + cv4.w;
+          uint32_t base = (256u << 9) + bdv + row_excl_scan16(sum);
+          // every lane of the row has read the backdrop before the scan
+          if (q == 0) backdrop[b][row] = 0;
+          uint32_t c0 = base + cv4.x, c1 = c0 + cv4.y, c2 = c1 + cv4.z, c3 = c2 + cv4.w;
+          m[0] = calc_mask(c0, c.rule_mask, c.alpha);
+          m[1] = calc_mask(c1, c.rule_mask, c.alpha);
+          m[2] = calc_mask(c2, c.rule_mask, c.alpha);
+          m[3] = calc_mask(c3, c.rule_mask, c.alpha);
+        }
+        else if (y >= c.box[1] && y < c.box[3]) {
+          if (c.kind == kCmdBoxA) {
 #pragma unroll
-          for (int k = 0; k < 4; k++) {
-            int x = px + k;
-            if (x >= c.box[0] && x < c.box[2]) {
-              int col = (x == c.box[0]) ? 0 : (x == c.box[2] - 1) ? 2 : 1;
-              m[k] = (rm >> (8 * col)) & 0xFFu;
+            for (int k = 0; k < 4; k++) m[k] = (px + k >= c.box[0] && px + k < c.box[2]) ? c.alpha : 0u;
+          }
+          else {
+            int kind = (y == c.box_row_first) ? 0 : (y == c.box_row_last) ? 2 : 1;
+            uint32_t rm = c.box_mask[kind];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              int x = px + k;
+              if (x >= c.box[0] && x < c.box[2]) {
+                int col = (x == c.box[0]) ? 0 : (x == c.box[2] - 1) ? 2 : 1;
+                m[k] = (rm >> (8 * col)) & 0xFFu;
+              }
             }
           }
         }
-      }
 
-      if (row_ok && (m[0] | m[1] | m[2] | m[3]) != 0) {
-        touched = true;
-        const DevStyle& st = rec[cur].r.style;
-        const int fill_row0 = (c.kind == kCmdAnalytic) ? (rec[cur].r.min_y >> 8) : (int)c.fetch_row0;
-        const pipe::Row rs = pipe::row_setup(st, y, fill_row0, c.kind == kCmdAnalytic ? cv.ref_band_h : 0u);
+        if (row_ok && (m[0] | m[1] | m[2] | m[3]) != 0) {
+          touched = true;
+          const DevStyle& st = rec[cur][b].r.style;
+          const int fill_row0 = (c.kind == kCmdAnalytic) ? (rec[cur][b].r.min_y >> 8) : (int)c.fetch_row0;
+          const pipe::Row rs = pipe::row_setup(st, y, fill_row0, c.kind == kCmdAnalytic ? cv.ref_band_h : 0u);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          if (!m[k] || px + k >= (int)cv.w) continue;
-          uint32_t src = pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
-          d[k] = a8 ? pipe::composite8(c.op, d[k], src >> 24, m[k]) : pipe::composite32(c.op, d[k], src, m[k]);
-          npix++;
+          for (int k = 0; k < 4; k++) {
+            if (!m[k] || px + k >= (int)cv.w) continue;
+            uint32_t src = pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
+            d[k] = a8 ? pipe::composite8(c.op, d[k], src >> 24, m[k]) : pipe::composite32(c.op, d[k], src, m[k]);
+            npix++;
+          }
         }
       }
-      if (has_next && t < kRecWords) rec[cur ^ 1].w[t] = pre_w;
-      __syncthreads();   // next record visible; cells cleared; this record no longer read
+      if (has_next) {
+        if (t < nwn) rec[cur ^ 1][t / kRecWords].w[t % kRecWords] = pw0;
+        if (t + kTileThreads < nwn) {
+          const int gw = t + kTileThreads;
+          rec[cur ^ 1][gw / kRecWords].w[gw % kRecWords] = pw1;
+        }
+      }
+      __syncthreads();   // next records visible; cells cleared; these records no longer read
     }
   }
 
-  // Composited-pixel statistics: one atomic per wave.
+  // Composited-pixel statistics: one value per tile, summed by k_sum_u64.
+  __shared__ uint32_t wave_px[kTileThreads / 64];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) npix += __shfl_xor(npix, o, 64);
-  if ((t & 63) == 0 && npix) atomicAdd(s.pixels, (unsigned long long)npix);
+  if (lane == 0) wave_px[wave] = npix;
+  __syncthreads();
+  if (t == 0) {
+    unsigned long long tp = 0;
+#pragma unroll
+    for (int w = 0; w < kTileThreads / 64; w++) tp += wave_px[w];
+    s.tile_px[tile] = tp;
+  }
 
   if (!touched || !row_ok) return;
   if (a8) {
@@ -727,9 +835,49 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
   }
 }
 
+// Canvas index of every tile (binary search over the canvases' first tiles).
+__global__ void k_tile_canvas(FrameDev f, uint32_t* tile_canvas) {
+  const uint32_t tile = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tile >= f.n_tiles) return;
+  uint32_t lo = 0, hi = f.n_canvases;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (f.canvases[mid].tile_base <= tile) lo = mid; else hi = mid;
+  }
+  tile_canvas[tile] = lo;
+}
+
+// *out += sum of v[0, n): per-workgroup partial sums, one atomic each
+// (integer addition, so the result does not depend on the order).
+__global__ __launch_bounds__(256) void k_sum_u64(const unsigned long long* v, uint32_t n, unsigned long long* out) {
+  __shared__ unsigned long long part[256];
+  unsigned long long acc = 0;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) acc += v[i];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && part[0]) atomicAdd(out, part[0]);
+}
+
+void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
+  if (!f.n_tiles) return;
+  hipLaunchKernelGGL(k_tile_canvas, dim3((f.n_tiles + 255) / 256), dim3(256), 0, st, f, s.tile_canvas);
+}
+
 void launch_render(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
   if (!f.n_tiles) return;
-  hipLaunchKernelGGL(k_render, dim3(f.n_tiles), dim3(kTileThreads), 0, st, f, s);
+  const uint32_t grid = (f.n_tiles + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL(k_render, dim3(grid), dim3(kTileThreads), 0, st, f, s);
+}
+
+void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
+  if (!f.n_tiles) return;
+  uint32_t g = (f.n_tiles + 4095) / 4096;
+  if (g > 256) g = 256;
+  hipLaunchKernelGGL(k_sum_u64, dim3(g), dim3(256), 0, st, (const unsigned long long*)s.tile_px, f.n_tiles, s.pixels);
 }
 
 }  // namespace b2amd

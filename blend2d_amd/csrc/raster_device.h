@@ -27,6 +27,17 @@ struct RasState {
 
 __device__ __forceinline__ uint32_t apply_sign(uint32_t x, uint32_t mask) { return (x ^ mask) - mask; }
 
+// floor(n / d) for d > 0.  Below 2^53 the quotient comes from the correctly
+// rounded double division: with Q = n / d < 2^53 / d the rounding error is
+// at most 2^-53 * Q < 1 / d, while a non-integral Q lies at least 1 / d away
+// from every integer, so truncating the rounded quotient gives floor(Q).
+// The 24.8 coordinates the rasterizer sees keep every dividend far below
+// 2^53; the integer division stays as the exact path for anything larger.
+__device__ __forceinline__ uint64_t udiv_floor(uint64_t n, uint32_t d) {
+  if (n < (1ull << 53)) return (uint64_t)__builtin_trunc((double)n / (double)d);
+  return n / (uint64_t)d;
+}
+
 __device__ __forceinline__ void acc_err_step(int* acc, int* iter, int step, int correction) {
   *iter -= step;
   int mask = *iter >> 31;
@@ -38,7 +49,7 @@ __device__ __forceinline__ void acc_err_multi_step(int* acc, int* iter, int step
   int64_t i = (int64_t)(uint32_t)(*iter);
   i -= (int64_t)((uint64_t)(uint32_t)step * (uint32_t)count);
   if (i < 0) {
-    int n = (int)(((uint64_t)(-i) + (uint32_t)correction - 1u) / (uint64_t)(uint32_t)correction);
+    int n = (int)udiv_floor((uint64_t)(-i) + (uint32_t)correction - 1u, (uint32_t)correction);
     *acc += n;
     i += (int64_t)correction * n;
   }
@@ -87,10 +98,12 @@ struct Rasterizer {
     uint64_t x_base = (uint64_t)(uint32_t)s->dx * A8_SCALE;
     uint64_t y_base = (uint64_t)(uint32_t)s->dy * A8_SCALE;
 
-    s->xLift = (int)(x_base / (unsigned)s->dy);
-    s->xRem = (int)(x_base % (unsigned)s->dy);
-    s->yLift = (int)(y_base / (unsigned)s->dx);
-    s->yRem = (int)(y_base % (unsigned)s->dx);
+    const uint64_t xq = udiv_floor(x_base, (uint32_t)s->dy);
+    const uint64_t yq = udiv_floor(y_base, (uint32_t)s->dx);
+    s->xLift = (int)xq;
+    s->xRem = (int)(x_base - xq * (uint32_t)s->dy);
+    s->yLift = (int)yq;
+    s->yRem = (int)(y_base - yq * (uint32_t)s->dx);
 
     s->xDlt = s->dx;
     s->yDlt = s->dy;
@@ -99,15 +112,17 @@ struct Rasterizer {
 
     if (s->ey0 != s->ey1) {
       uint64_t p = (uint64_t)(A8_SCALE - (uint32_t)s->fy0) * (uint32_t)s->dx;
-      s->xDlt = (int)(p / (unsigned)s->dy);
-      s->xErr -= (int)(p % (unsigned)s->dy);
+      const uint64_t q = udiv_floor(p, (uint32_t)s->dy);
+      s->xDlt = (int)q;
+      s->xErr -= (int)(p - q * (uint32_t)s->dy);
       acc_err_step(&s->xDlt, &s->xErr, 0, s->dy);
     }
 
     if (s->ex0 != s->ex1) {
       uint64_t p = (uint64_t)((s->flags & F_RTL) ? (uint32_t)s->fx0 : A8_SCALE - (uint32_t)s->fx0) * (uint32_t)s->dy;
-      s->yDlt = (int)(p / (unsigned)s->dx);
-      s->yErr -= (int)(p % (unsigned)s->dx);
+      const uint64_t q = udiv_floor(p, (uint32_t)s->dx);
+      s->yDlt = (int)q;
+      s->yErr -= (int)(p - q * (uint32_t)s->dx);
       acc_err_step(&s->yDlt, &s->yErr, 0, s->dx);
     }
 
