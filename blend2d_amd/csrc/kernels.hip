@@ -428,16 +428,19 @@ void launch_cmd_tiles(const FrameDev& f, const FrameScratch& s, hipStream_t st) 
 // Stage 3: render
 // ---------------------------------------------------------------------------
 
+// LDS cell buffer of one command in a tile: 16 rows x 64 cells, then one
+// backdrop word per row (the summed contributions of cells left of the tile).
+constexpr int kCellWords = kTileH * kTileW + kTileH;
+constexpr int kBackdrop = kTileH * kTileW;
+
 struct TileSink {
   uint32_t* cells;
-  uint32_t* backdrop;
   int x0, y0;
   __device__ __forceinline__ void add(int row, int x, uint32_t v) {
-    int r = row - y0;
-    if ((unsigned)r >= (unsigned)kTileH) return;
-    int lx = x - x0;
-    if (lx < 0) atomicAdd(&backdrop[r], v);
-    else if (lx < kTileW) atomicAdd(&cells[r * kTileW + lx], v);
+    const int r = row - y0;
+    const int lx = x - x0;
+    const int idx = lx < 0 ? kBackdrop + r : r * kTileW + lx;
+    if ((unsigned)r < (unsigned)kTileH && lx < kTileW) atomicAdd(&cells[idx], v);
   }
 };
 
@@ -480,25 +483,25 @@ static_assert(sizeof(DevCmd) % 8 == 0, "DevStyle must stay 8-byte aligned in the
 // batch's commands in submission order.  Rasterization of different commands
 // is independent, so only the composite has to follow the command order.
 #ifndef B2AMD_RENDER_BATCH
-#define B2AMD_RENDER_BATCH 8
+#define B2AMD_RENDER_BATCH 7
 #endif
 constexpr int kBatch = B2AMD_RENDER_BATCH;
 static_assert(kBatch >= 1 && kBatch * kRecWords <= 2 * kTileThreads, "two record words per thread");
 
-// Word `w` of the record of command `cmd_i` (style index `style_i`).
-__device__ __forceinline__ uint32_t rec_word(const FrameDev& f, const FrameScratch& s, uint32_t cmd_i, uint32_t style_i, int w) {
+// Word `w` of the record of command `cmd_i`.
+__device__ __forceinline__ uint32_t rec_word(const FrameDev& f, const FrameScratch& s, uint32_t cmd_i, int w) {
   constexpr int kCmdW = (int)(sizeof(DevCmd) / 4), kStyleW = (int)(sizeof(DevStyle) / 4);
   if (w < kCmdW) return ((const uint32_t*)(f.cmds + cmd_i))[w];
-  if (w < kCmdW + kStyleW) return ((const uint32_t*)(f.styles + style_i))[w - kCmdW];
+  if (w < kCmdW + kStyleW) return ((const uint32_t*)(f.styles + f.cmds[cmd_i].style))[w - kCmdW];
   if (w == kCmdW + kStyleW) return (uint32_t)s.bounds[cmd_i].min_y;
   return 0u;
 }
 
 // Word `gw` of the records of the batch that starts at selected command `kb`.
 __device__ __forceinline__ uint32_t batch_rec_word(const FrameDev& f, const FrameScratch& s, const uint32_t* sel_cmd,
-                                                   const uint32_t* sel_style, uint32_t kb, int gw) {
+                                                   uint32_t kb, int gw) {
   const int b = gw / kRecWords, w = gw - b * kRecWords;
-  return rec_word(f, s, sel_cmd[kb + b], sel_style[kb + b], w);
+  return rec_word(f, s, sel_cmd[kb + b], w);
 }
 
 // Inclusive prefix of the segment counts of the batch's `nb` commands.
@@ -522,34 +525,42 @@ __device__ __forceinline__ uint32_t batch_find(const uint32_t end[kBatch], uint3
   return b;
 }
 
-// One binned segment of an analytic fill into the tile's LDS cells.
-__device__ __forceinline__ void raster_seg(const DevSeg& g, uint32_t* cells, uint32_t* backdrop, int X0, int Y0) {
-  uint32_t sign = (uint32_t)g.x1 >> 31;
-  int32_t x1 = g.x1 & 0x7FFFFFFF;
-  int32_t cmin = (g.x0 < x1 ? g.x0 : x1) >> 8;
-  int32_t cmax = (g.x0 < x1 ? x1 : g.x0) >> 8;
-  if (cmin >= X0 + kTileW) return;
+// First pass over one binned segment of an analytic fill: a segment right of
+// the tile is dropped; one left of it adds its signed vertical extent per
+// covered row to the backdrop; for one that reaches the tile the number of
+// tile rows it covers is returned (rasterized by raster_row, one row each).
+__device__ __forceinline__ uint32_t seg_tile_rows(const DevSeg& g, uint32_t* cells, int X0, int Y0) {
+  const uint32_t sign = (uint32_t)g.x1 >> 31;
+  const int32_t x1 = g.x1 & 0x7FFFFFFF;
+  const int32_t cmin = (g.x0 < x1 ? g.x0 : x1) >> 8;
+  const int32_t cmax = (g.x0 < x1 ? x1 : g.x0) >> 8;
+  int r0 = g.y0 >> 8, r1 = (g.y1 - 1) >> 8;
+  if (r0 < Y0) r0 = Y0;
+  if (r1 > Y0 + kTileH - 1) r1 = Y0 + kTileH - 1;
+  if (cmin >= X0 + kTileW || r1 < r0) return 0u;
   if (cmax + 1 < X0) {
-    // All cells of this segment are left of the tile: each covered row
-    // receives its signed vertical extent as a backdrop.
-    int r0 = g.y0 >> 8, r1 = (g.y1 - 1) >> 8;
-    if (r0 < Y0) r0 = Y0;
-    if (r1 > Y0 + kTileH - 1) r1 = Y0 + kTileH - 1;
-    uint32_t smask = 0u - sign;
+    const uint32_t smask = 0u - sign;
     for (int r = r0; r <= r1; r++) {
-      int lo_y = g.y0 > r * 256 ? g.y0 : r * 256;
-      int hi_y = g.y1 < (r + 1) * 256 ? g.y1 : (r + 1) * 256;
-      uint32_t cover = ras::apply_sign((uint32_t)(hi_y - lo_y), smask);
-      atomicAdd(&backdrop[r - Y0], cover << 9);
+      const int lo_y = g.y0 > r * 256 ? g.y0 : r * 256;
+      const int hi_y = g.y1 < (r + 1) * 256 ? g.y1 : (r + 1) * 256;
+      const uint32_t cover = ras::apply_sign((uint32_t)(hi_y - lo_y), smask);
+      atomicAdd(&cells[kBackdrop + r - Y0], cover << 9);
     }
-    return;
+    return 0u;
   }
-  TileSink sink{cells, backdrop, X0, Y0};
+  return (uint32_t)(r1 - r0 + 1);
+}
+
+// Row `row` of a segment that reaches the tile.  Advancing the rasterizer to
+// any row and rasterizing that row alone gives the cells the full walk gives
+// for it (the rasterizer's advance invariant, band height 1).
+__device__ __forceinline__ void raster_row(const DevSeg& g, uint32_t* cells, int X0, int Y0, int row) {
+  TileSink sink{cells, X0, Y0};
   ras::Rasterizer rz;
-  rz.sign_mask = 0u - sign;
-  if (!rz.prepare(g.x0, g.y0, x1, g.y1)) return;
-  if (rz.st.ey0 < Y0) rz.advance_to_y(Y0);
-  rz.band_end = Y0 + kTileH - 1;
+  rz.sign_mask = 0u - ((uint32_t)g.x1 >> 31);
+  if (!rz.prepare(g.x0, g.y0, g.x1 & 0x7FFFFFFF, g.y1)) return;
+  if (rz.st.ey0 < row) rz.advance_to_y(row);
+  rz.band_end = row;
   rz.rasterize(sink);
 }
 
@@ -562,9 +573,10 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t wg, uint32_t n_tiles) {
   return (wg % kXcds) * per + wg / kXcds;
 }
 
-__global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratch s) {
-  __shared__ uint32_t cells[kBatch][kTileH * kTileW];
-  __shared__ uint32_t backdrop[kBatch][kTileH];
+// Four workgroups per CU: LDS (7 cell buffers) and the register budget are
+// sized for it.
+__global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScratch s) {
+  __shared__ uint32_t cells[kBatch][kCellWords];
   const uint32_t tile = xcd_tile(blockIdx.x, f.n_tiles);
   if (tile >= f.n_tiles) return;   // uniform: padding workgroups of the XCD mapping
   const int t = threadIdx.x;
@@ -591,7 +603,7 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
   // zero LDS
 #pragma unroll
   for (int b = 0; b < kBatch; b++) *(uint4*)&cells[b][row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
-  for (int i = t; i < kBatch * kTileH; i += kTileThreads) backdrop[i / kTileH][i % kTileH] = 0;
+  for (int i = t; i < kBatch * kTileH; i += kTileThreads) cells[i / kTileH][kBackdrop + i % kTileH] = 0;
 
   // load the destination tile
   uint32_t d[4] = {0, 0, 0, 0};
@@ -638,8 +650,13 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
   // commands whose tile-column range holds this tile (ballot compaction,
   // submission order kept); the filter also gathers, in parallel, each
   // selected command's (band, command) cell extent and style index.
-  __shared__ uint32_t sel_cmd[kTileThreads], sel_off[kTileThreads], sel_cnt[kTileThreads], sel_style[kTileThreads];
+  __shared__ uint32_t sel_cmd[kTileThreads], sel_off[kTileThreads], sel_cnt[kTileThreads];
   __shared__ uint32_t sel_wave[kTileThreads / 64];
+  // Row tasks of the segments that reach the tile: the segment of each
+  // thread, and its first task index (low 12 bits) and batch slot (bits 12+).
+  __shared__ DevSeg task_seg[kTileThreads];
+  __shared__ uint16_t task_start[kTileThreads];
+  __shared__ uint32_t task_wave[kTileThreads / 64];
   // Records (command, style, fill-row origin) of the batch being applied
   // and of the next one (double buffer): the next batch's records and first
   // 256 segments are loaded while the current batch is rasterized and
@@ -672,7 +689,6 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
       sel_cmd[slot] = cmd_i;
       sel_off[slot] = s.mat_off[e];
       sel_cnt[slot] = s.mat_cnt[e];
-      sel_style[slot] = f.cmds[cmd_i].style;
     }
     __syncthreads();
     if (n_sel == 0) continue;   // uniform; sel_wave is rewritten only after the next chunk's ballot
@@ -684,10 +700,10 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
       const uint32_t nb = n_sel < (uint32_t)kBatch ? n_sel : (uint32_t)kBatch;
       uint32_t end[kBatch];
       batch_prefix(sel_cnt, 0, nb, end);
-      if (t < (int)nb * kRecWords) rec[0][t / kRecWords].w[t % kRecWords] = batch_rec_word(f, s, sel_cmd, sel_style, 0, t);
+      if (t < (int)nb * kRecWords) rec[0][t / kRecWords].w[t % kRecWords] = batch_rec_word(f, s, sel_cmd, 0, t);
       if (t + kTileThreads < (int)nb * kRecWords) {
         const int gw = t + kTileThreads;
-        rec[0][gw / kRecWords].w[gw % kRecWords] = batch_rec_word(f, s, sel_cmd, sel_style, 0, gw);
+        rec[0][gw / kRecWords].w[gw % kRecWords] = batch_rec_word(f, s, sel_cmd, 0, gw);
       }
       if (jt < end[kBatch - 1]) {
         uint32_t loc;
@@ -714,8 +730,8 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
       if (has_next) {
         const uint32_t nbn = n_sel - kn < (uint32_t)kBatch ? n_sel - kn : (uint32_t)kBatch;
         nwn = (int)nbn * kRecWords;
-        if (t < nwn) pw0 = batch_rec_word(f, s, sel_cmd, sel_style, kn, t);
-        if (t + kTileThreads < nwn) pw1 = batch_rec_word(f, s, sel_cmd, sel_style, kn, t + kTileThreads);
+        if (t < nwn) pw0 = batch_rec_word(f, s, sel_cmd, kn, t);
+        if (t + kTileThreads < nwn) pw1 = batch_rec_word(f, s, sel_cmd, kn, t + kTileThreads);
         uint32_t endn[kBatch];
         batch_prefix(sel_cnt, kn, nbn, endn);
         pre_seg = DevSeg{0, 0, 0, 0};
@@ -727,14 +743,55 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
         }
       }
 
-      // rasterize the batch
-      if (jt < total) raster_seg(seg, cells[seg_b], backdrop[seg_b], X0, Y0);
-      for (uint32_t j = jt + kTileThreads; j < total; j += kTileThreads) {
-        uint32_t loc;
-        const uint32_t b = batch_find(end, j, &loc);
-        raster_seg(s.binned[sel_off[kb + b] + loc], cells[b], backdrop[b], X0, Y0);
+      // Rasterize the batch.  Segments left of the tile add their row
+      // covers to the backdrop; every segment that reaches the tile becomes
+      // one task per tile row it covers, so a lane's serial cell walk is one
+      // row long and the rows are spread over all lanes.
+      for (uint32_t rb = 0; rb < total; rb += kTileThreads) {
+        const uint32_t j = rb + jt;
+        DevSeg g = seg;
+        uint32_t gb = seg_b;
+        if (rb) {
+          g = DevSeg{0, 0, 0, 0};
+          gb = 0;
+          if (j < total) {
+            uint32_t loc;
+            gb = batch_find(end, j, &loc);
+            g = s.binned[sel_off[kb + gb] + loc];
+          }
+        }
+        const uint32_t rows = j < total ? seg_tile_rows(g, cells[gb], X0, Y0) : 0u;
+        uint32_t incl = rows;
+#pragma unroll
+        for (int dd = 1; dd < 64; dd <<= 1) {
+          const uint32_t v = __shfl_up(incl, dd, 64);
+          if (lane >= dd) incl += v;
+        }
+        if (lane == 63) task_wave[wave] = incl;
+        if (rows) task_seg[t] = g;
+        __syncthreads();
+        uint32_t before = 0, n_tasks = 0;
+#pragma unroll
+        for (int w = 0; w < kTileThreads / 64; w++) {
+          const uint32_t cw = task_wave[w];
+          before += w < wave ? cw : 0u;
+          n_tasks += cw;
+        }
+        task_start[t] = (uint16_t)((before + incl - rows) | (gb << 12));
+        __syncthreads();
+        for (uint32_t k = (uint32_t)t; k < n_tasks; k += kTileThreads) {
+          // owner: the last thread whose first task index is <= k
+          int o = 0;
+#pragma unroll
+          for (int step = kTileThreads / 2; step > 0; step >>= 1)
+            if ((uint32_t)(task_start[o + step] & 0xFFFu) <= k) o += step;
+          const uint32_t ts = task_start[o];
+          const DevSeg og = task_seg[o];
+          const int r0 = (og.y0 >> 8) > Y0 ? (og.y0 >> 8) : Y0;
+          raster_row(og, cells[ts >> 12], X0, Y0, r0 + (int)(k - (ts & 0xFFFu)));
+        }
+        __syncthreads();   // cells complete (and task arrays free for the next round)
       }
-      __syncthreads();
 
       // composite the batch's commands in order
       for (uint32_t b = 0; b < nb; b++) {
@@ -744,12 +801,12 @@ __global__ __launch_bounds__(kTileThreads) void k_render(FrameDev f, FrameScratc
         if (c.kind == kCmdAnalytic) {
           uint32_t* cb = cells[b];
           uint4 cv4 = *(uint4*)&cb[row * kTileW + q * 4];
-          uint32_t bdv = backdrop[b][row];
+          uint32_t bdv = cb[kBackdrop + row];
           *(uint4*)&cb[row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
           uint32_t sum = cv4.x + cv4.y + cv4.z + cv4.w;
           uint32_t base = (256u << 9) + bdv + row_excl_scan16(sum);
           // every lane of the row has read the backdrop before the scan
-          if (q == 0) backdrop[b][row] = 0;
+          if (q == 0) cb[kBackdrop + row] = 0;
           uint32_t c0 = base + cv4.x, c1 = c0 + cv4.y, c2 = c1 + cv4.z, c3 = c2 + cv4.w;
           m[0] = calc_mask(c0, c.rule_mask, c.alpha);
           m[1] = calc_mask(c1, c.rule_mask, c.alpha);
