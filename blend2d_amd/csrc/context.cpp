@@ -142,7 +142,7 @@ struct Runner {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf in;
   DevBuf src_count, src_off, bounds, segs, seg_cmd, mat_cnt, mat_off, mat_pres, list_off, list, binned, cursor, scan_tmp, counters, cmd_tiles;
-  DevBuf tile_canvas, tile_px;
+  DevBuf tile_hdr, group_hdr, tile_px, list_style;
   DevBuf cm_params, cm_acc, cm_feat, cm_pred, cm_act;
   uint64_t cm_version = 0;
   uint32_t* host_readback = nullptr;
@@ -206,16 +206,19 @@ uint32_t Runner::upload(FrameHost& fh) {
   if (!init()) return kErrDevice;
 
   uint64_t n_mat = 0;
-  uint32_t n_tiles = 0, band_base = 0;
+  uint32_t n_tiles = 0, n_groups = 0, band_base = 0;
   for (DevCanvas& c : fh.canvases) {
     c.tiles_x = (c.w + kTileW - 1) / kTileW;
     c.bands = (c.h + kTileH - 1) / kTileH;
+    c.groups_x = (c.tiles_x + kGroupTiles - 1) / kGroupTiles;
     c.ref_band_h = ref_band_height(c.w);
     c.band_base = band_base;
     c.tile_base = n_tiles;
+    c.group_base = n_groups;
     c.mat_base = n_mat;
     band_base += c.bands;
     n_tiles += c.tiles_x * c.bands;
+    n_groups += c.groups_x * c.bands;
     n_mat += (uint64_t)c.bands * c.n_cmds;
   }
   if (n_mat >= 0xFFFFFFF0ull || fh.src.size() >= 0xFFFFFFF0ull) return kErrOOM;
@@ -245,6 +248,7 @@ uint32_t Runner::upload(FrameHost& fh) {
   f.luts = (const uint32_t*)(base + o_luts);
   f.images = (const DevImage*)(base + o_images);
   f.n_tiles = n_tiles;
+  f.n_groups = n_groups;
   f.n_mat = n_mat;
   f.tol_sq = kToleranceSq;
   n_src = fh.src.size();
@@ -314,21 +318,24 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   size_t scan_words = scan_tmp_words((size_t)n_mat + 1);
   if (!bounds.reserve((size_t)f.n_cmds * sizeof(CmdBounds)) || !mat_cnt.reserve((n_mat + 1) * 4) ||
       !mat_off.reserve((n_mat + 1) * 4) || !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
-      !list.reserve((n_mat + 1) * 4) || !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) ||
+      !list.reserve((n_mat + 1) * 16) || !list_style.reserve((n_mat + 1) * 4) || !cursor.reserve((n_mat + 1) * 4) ||
+      !scan_tmp.reserve(scan_words * 4) ||
       !counters.reserve(64) || !cmd_tiles.reserve((size_t)f.n_cmds * 4) ||
-      !tile_canvas.reserve((size_t)f.n_tiles * 4) || !tile_px.reserve((size_t)f.n_tiles * 8))
+      !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 16) || !tile_px.reserve((size_t)f.n_tiles * 8))
     return kErrOOM;
 
   FrameScratch s;
   memset(&s, 0, sizeof(s));
-  s.tile_canvas = tile_canvas.as<uint32_t>();
+  s.tile_hdr = tile_hdr.as<uint4>();
+  s.group_hdr = group_hdr.as<uint4>();
   s.tile_px = tile_px.as<unsigned long long>();
   s.bounds = bounds.as<CmdBounds>();
   s.mat_cnt = mat_cnt.as<uint32_t>();
   s.mat_off = mat_off.as<uint32_t>();
   s.mat_pres = mat_pres.as<uint32_t>();
   s.list_off = list_off.as<uint32_t>();
-  s.list = list.as<uint32_t>();
+  s.list_ent = list.as<uint4>();
+  s.list_style = list_style.as<uint32_t>();
   s.scan_tmp = scan_tmp.as<uint32_t>();
   s.pixels = counters.as<unsigned long long>();
   s.seg_cursor = counters.as<uint32_t>() + 2;
@@ -431,13 +438,14 @@ uint32_t Runner::execute_exact(Stats& stats) {
       !mat_cnt.reserve((n_mat + 1) * 4) || !mat_off.reserve((n_mat + 1) * 4) ||
       !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
       !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) || !counters.reserve(64) ||
-      !cmd_tiles.reserve((size_t)f.n_cmds * 4) || !tile_canvas.reserve((size_t)f.n_tiles * 4) ||
+      !cmd_tiles.reserve((size_t)f.n_cmds * 4) || !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 16) ||
       !tile_px.reserve((size_t)f.n_tiles * 8))
     return kErrOOM;
 
   FrameScratch s;
   memset(&s, 0, sizeof(s));
-  s.tile_canvas = tile_canvas.as<uint32_t>();
+  s.tile_hdr = tile_hdr.as<uint4>();
+  s.group_hdr = group_hdr.as<uint4>();
   s.tile_px = tile_px.as<unsigned long long>();
   s.src_count = src_count.as<uint32_t>();
   s.src_off = src_off.as<uint32_t>();
@@ -488,10 +496,12 @@ uint32_t Runner::execute_exact(Stats& stats) {
   // the command lists hold at most one entry per (band, command) cell
   if (n_binned < n_segs || (uint64_t)n_binned > (uint64_t)n_segs * std::max<uint32_t>(max_bands(), 1) || n_list > n_mat)
     return kErrDevice;
-  if (!binned.reserve((size_t)std::max<uint32_t>(n_binned, 1) * sizeof(DevSeg)) || !list.reserve((size_t)std::max<uint32_t>(n_list, 1) * 4))
+  if (!binned.reserve((size_t)std::max<uint32_t>(n_binned, 1) * sizeof(DevSeg)) || !list.reserve((size_t)std::max<uint32_t>(n_list, 1) * 16) ||
+      !list_style.reserve((size_t)std::max<uint32_t>(n_list, 1) * 4))
     return kErrOOM;
   s.binned = binned.as<DevSeg>();
-  s.list = list.as<uint32_t>();
+  s.list_ent = list.as<uint4>();
+  s.list_style = list_style.as<uint32_t>();
   HIP_OK(hipMemcpyAsync(cursor.ptr, s.mat_off, (n_mat + 1) * 4, hipMemcpyDeviceToDevice, stream));
   launch_bin_scatter(f, s, n_segs, cursor.as<uint32_t>(), stream);
   launch_list_build(f, s, stream);

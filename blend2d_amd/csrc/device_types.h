@@ -13,6 +13,9 @@ namespace b2amd {
 constexpr int kTileW = 64;
 constexpr int kTileH = 16;
 constexpr int kTileThreads = 256;
+// Bands with box fills only are composited by tile groups: up to 4
+// consecutive tiles of one band (256 x 16 pixels) per workgroup.
+constexpr int kGroupTiles = 4;
 
 // Executed comp ops (after the frontend's simplification).
 enum ExecOp : uint32_t {
@@ -142,6 +145,8 @@ struct DevCanvas {
   uint32_t band_base;             // first global band index
   uint32_t tile_base;             // first global tile index
   uint64_t mat_base;              // (band, cmd) matrix offset
+  uint32_t group_base;            // first global tile group (4 tiles of a band)
+  uint32_t groups_x;              // tile groups per band
 };
 
 // One emitted edge segment, 24.8 fixed point, y0 < y1.

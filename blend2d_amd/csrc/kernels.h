@@ -21,6 +21,7 @@ struct FrameDev {
   const uint32_t* luts;
   const DevImage* images;
   uint32_t n_tiles;
+  uint32_t n_groups;       // tile groups: up to 4 consecutive tiles of one band
   uint64_t n_mat;          // (band, cmd) matrix entries over all canvases
   double tol_sq;           // squared flattening tolerance in 24.8 units
 };
@@ -36,12 +37,14 @@ struct FrameScratch {
   uint32_t* mat_off;       // n_mat + 1
   uint32_t* mat_pres;      // n_mat + 1
   uint32_t* list_off;      // n_mat + 1
-  uint32_t* list;          // n_list
+  uint4* list_ent;         // n_list: per listed (band, command) cell {command, tile-column range, binned offset, binned count}
+  uint32_t* list_style;    // n_list: style index of the entry's command
   DevSeg* binned;          // n_binned
   uint32_t* scan_tmp;      // scan_tmp_words(max n)
   unsigned long long* pixels;  // composited pixels (mask > 0), summed after render
   uint32_t* cmd_tiles;     // n_cmds: tile-column range per command
-  uint32_t* tile_canvas;   // n_tiles: canvas index of each tile
+  uint4* tile_hdr;         // n_tiles: {canvas, band list begin, band list end, tile column | band << 16 | analytic << 31}
+  uint4* group_hdr;        // n_groups: the same per tile group (group column in place of the tile column)
   unsigned long long* tile_px; // n_tiles: composited pixels of each tile
   // Pool mode (sizes predicted ahead of time; null / 0 otherwise):
   uint32_t* seg_cursor;    // segments taken from the pool
@@ -63,7 +66,7 @@ void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_bin_scatter(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, uint32_t* cursor, hipStream_t st);
 void launch_list_build(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_cmd_tiles(const FrameDev& f, const FrameScratch& s, hipStream_t st);
-// Stage 3: tile -> canvas table (setup), tile rasterization + fetch +
+// Stage 3: per-tile headers (setup), tile rasterization + fetch +
 // composite (render), composited pixels summed into *s.pixels, which must
 // start at zero (stats).
 void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t st);

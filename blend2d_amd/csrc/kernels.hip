@@ -386,13 +386,27 @@ void launch_bin_scatter(const FrameDev& f, const FrameScratch& s, uint32_t n_seg
   hipLaunchKernelGGL(k_bin_scatter, dim3((n_segs + 255) / 256), dim3(256), 0, st, f, s, n_segs, cursor);
 }
 
-__global__ void k_list_build(FrameScratch s, uint64_t n) {
+// Command lists of the bands: every present (band, command) cell becomes one
+// entry, in command order within its band, carrying what the render filter
+// and the record loads need (so they are one dependent load away).
+__global__ void k_list_build(FrameDev f, FrameScratch s, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && s.mat_pres[i]) s.list[s.list_off[i]] = (uint32_t)i;
+  if (i >= n || !s.mat_pres[i]) return;
+  // canvas of matrix cell i: the last one whose matrix base is <= i
+  uint32_t lo = 0, hi = f.n_canvases;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (f.canvases[mid].mat_base <= i) lo = mid; else hi = mid;
+  }
+  const DevCanvas& cv = f.canvases[lo];
+  const uint32_t cmd = cv.cmd_base + (uint32_t)((i - cv.mat_base) % cv.n_cmds);
+  const uint32_t li = s.list_off[i];
+  s.list_ent[li] = make_uint4(cmd, s.cmd_tiles[cmd], s.mat_off[i], s.mat_cnt[i]);
+  s.list_style[li] = f.cmds[cmd].style;
 }
 
 void launch_list_build(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
-  if (f.n_mat) hipLaunchKernelGGL(k_list_build, dim3((unsigned)((f.n_mat + 255) / 256)), dim3(256), 0, st, s, f.n_mat);
+  if (f.n_mat) hipLaunchKernelGGL(k_list_build, dim3((unsigned)((f.n_mat + 255) / 256)), dim3(256), 0, st, f, s, f.n_mat);
 }
 
 // Per command, the range of tile columns it can touch, packed as
@@ -488,20 +502,20 @@ static_assert(sizeof(DevCmd) % 8 == 0, "DevStyle must stay 8-byte aligned in the
 constexpr int kBatch = B2AMD_RENDER_BATCH;
 static_assert(kBatch >= 1 && kBatch * kRecWords <= 2 * kTileThreads, "two record words per thread");
 
-// Word `w` of the record of command `cmd_i`.
-__device__ __forceinline__ uint32_t rec_word(const FrameDev& f, const FrameScratch& s, uint32_t cmd_i, int w) {
+// Word `w` of the record of command `cmd_i` (style index `style_i`).
+__device__ __forceinline__ uint32_t rec_word(const FrameDev& f, const FrameScratch& s, uint32_t cmd_i, uint32_t style_i, int w) {
   constexpr int kCmdW = (int)(sizeof(DevCmd) / 4), kStyleW = (int)(sizeof(DevStyle) / 4);
   if (w < kCmdW) return ((const uint32_t*)(f.cmds + cmd_i))[w];
-  if (w < kCmdW + kStyleW) return ((const uint32_t*)(f.styles + f.cmds[cmd_i].style))[w - kCmdW];
+  if (w < kCmdW + kStyleW) return ((const uint32_t*)(f.styles + style_i))[w - kCmdW];
   if (w == kCmdW + kStyleW) return (uint32_t)s.bounds[cmd_i].min_y;
   return 0u;
 }
 
 // Word `gw` of the records of the batch that starts at selected command `kb`.
 __device__ __forceinline__ uint32_t batch_rec_word(const FrameDev& f, const FrameScratch& s, const uint32_t* sel_cmd,
-                                                   uint32_t kb, int gw) {
+                                                   const uint32_t* sel_style, uint32_t kb, int gw) {
   const int b = gw / kRecWords, w = gw - b * kRecWords;
-  return rec_word(f, s, sel_cmd[kb + b], w);
+  return rec_word(f, s, sel_cmd[kb + b], sel_style[kb + b], w);
 }
 
 // Inclusive prefix of the segment counts of the batch's `nb` commands.
@@ -564,6 +578,94 @@ __device__ __forceinline__ void raster_row(const DevSeg& g, uint32_t* cells, int
   rz.rasterize(sink);
 }
 
+// Destination pixels px .. px + 3 of a row (zero where outside the canvas).
+__device__ __forceinline__ void load4(const uint8_t* rowp, int px, int w, bool a8, uint32_t d[4]) {
+  const bool full4 = px + 3 < w;
+  if (a8) {
+    if (full4 && (((uintptr_t)(rowp + px)) & 3) == 0) {
+      const uint32_t v = *(const uint32_t*)(rowp + px);
+#pragma unroll
+      for (int k = 0; k < 4; k++) d[k] = (v >> (8 * k)) & 0xFFu;
+    }
+    else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) d[k] = px + k < w ? rowp[px + k] : 0u;
+    }
+  }
+  else {
+    const uint32_t* r32 = (const uint32_t*)rowp;
+    if (full4 && (((uintptr_t)(r32 + px)) & 15) == 0) {
+      const uint4 v = *(const uint4*)(r32 + px);
+      d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    }
+    else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) d[k] = px + k < w ? r32[px + k] : 0u;
+    }
+  }
+}
+
+__device__ __forceinline__ void store4(uint8_t* rowp, int px, int w, bool a8, const uint32_t d[4]) {
+  const bool full4 = px + 3 < w;
+  if (a8) {
+    if (full4 && (((uintptr_t)(rowp + px)) & 3) == 0) {
+      *(uint32_t*)(rowp + px) = d[0] | (d[1] << 8) | (d[2] << 16) | (d[3] << 24);
+    }
+    else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) if (px + k < w) rowp[px + k] = (uint8_t)d[k];
+    }
+  }
+  else {
+    uint32_t* r32 = (uint32_t*)rowp;
+    if (full4 && (((uintptr_t)(r32 + px)) & 15) == 0) {
+      *(uint4*)(r32 + px) = make_uint4(d[0], d[1], d[2], d[3]);
+    }
+    else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) if (px + k < w) r32[px + k] = d[k];
+    }
+  }
+}
+
+// Masks of pixels px .. px + 3 of row y for a box fill.
+__device__ __forceinline__ void box_masks4(const DevCmd& c, int y, int px, uint32_t m[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) m[k] = 0u;
+  if (y < c.box[1] || y >= c.box[3]) return;
+  if (c.kind == kCmdBoxA) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) m[k] = (px + k >= c.box[0] && px + k < c.box[2]) ? c.alpha : 0u;
+  }
+  else {
+    const int kind = (y == c.box_row_first) ? 0 : (y == c.box_row_last) ? 2 : 1;
+    const uint32_t rm = c.box_mask[kind];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int x = px + k;
+      if (x >= c.box[0] && x < c.box[2]) {
+        const int col = (x == c.box[0]) ? 0 : (x == c.box[2] - 1) ? 2 : 1;
+        m[k] = (rm >> (8 * col)) & 0xFFu;
+      }
+    }
+  }
+}
+
+// Fetch and composite of pixels px .. px + 3 under masks m; returns how many
+// were composited.
+__device__ __forceinline__ uint32_t composite4(const FrameDev& f, const DevCmd& c, const DevStyle& st, const pipe::Row& rs,
+                                               int px, int w, bool a8, const uint32_t m[4], uint32_t d[4]) {
+  uint32_t n = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (!m[k] || px + k >= w) continue;
+    const uint32_t src = pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
+    d[k] = a8 ? pipe::composite8(c.op, d[k], src >> 24, m[k]) : pipe::composite32(c.op, d[k], src, m[k]);
+    n++;
+  }
+  return n;
+}
+
 // Workgroup -> tile.  Workgroups are dealt to the 8 XCDs round-robin; the
 // mapping gives each XCD a contiguous range of tiles, so the tiles of a band
 // (which read the same command lists, records and segments) share one L2.
@@ -580,18 +682,19 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
   const uint32_t tile = xcd_tile(blockIdx.x, f.n_tiles);
   if (tile >= f.n_tiles) return;   // uniform: padding workgroups of the XCD mapping
   const int t = threadIdx.x;
+  const uint4 th = s.tile_hdr[tile];
+  if (!(th.w >> 31)) return;   // box fills only in this band: k_render_box composites it
   // A pool overflow leaves the frame incomplete: nothing is composited and
   // the host redoes the frame with exact counting.
   if (s.flags && *s.flags) {
     if (t == 0) s.tile_px[tile] = 0;
     return;
   }
-
-  const DevCanvas cv = f.canvases[s.tile_canvas[tile]];
-  const uint32_t local = tile - cv.tile_base;
-  const int band = (int)(local / cv.tiles_x);
-  const int X0 = (int)(local % cv.tiles_x) * kTileW;
-  const int Y0 = band * kTileH;
+  const DevCanvas cv = f.canvases[th.x];
+  const uint32_t lb = th.y, le = th.z;
+  const uint32_t tcol = th.w & 0xFFFFu;
+  const int X0 = (int)tcol * kTileW;
+  const int Y0 = (int)((th.w >> 16) & 0x7FFFu) * kTileH;
 
   const int row = t >> 4;
   const int q = t & 15;
@@ -608,36 +711,8 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
   // load the destination tile
   uint32_t d[4] = {0, 0, 0, 0};
   uint8_t* rowp = (uint8_t*)cv.ptr + (int64_t)y * cv.stride;
-  const bool full4 = row_ok && px + 3 < (int)cv.w;
-  if (row_ok) {
-    if (a8) {
-      if (full4 && (((uintptr_t)(rowp + px)) & 3) == 0) {
-        uint32_t v = *(const uint32_t*)(rowp + px);
-#pragma unroll
-        for (int k = 0; k < 4; k++) d[k] = (v >> (8 * k)) & 0xFFu;
-      }
-      else {
-#pragma unroll
-        for (int k = 0; k < 4; k++) if (px + k < (int)cv.w) d[k] = rowp[px + k];
-      }
-    }
-    else {
-      uint32_t* r32 = (uint32_t*)rowp;
-      if (full4 && (((uintptr_t)(r32 + px)) & 15) == 0) {
-        uint4 v = *(const uint4*)(r32 + px);
-        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
-      }
-      else {
-#pragma unroll
-        for (int k = 0; k < 4; k++) if (px + k < (int)cv.w) d[k] = r32[px + k];
-      }
-    }
-  }
+  if (row_ok) load4(rowp, px, (int)cv.w, a8, d);
 
-  const uint64_t row_base = cv.mat_base + (uint64_t)band * cv.n_cmds;
-  const uint32_t lb = s.list_off[row_base];
-  const uint32_t le = s.list_off[row_base + cv.n_cmds];
-  const uint32_t tcol = local % cv.tiles_x;
   const int lane = t & 63, wave = t >> 6;
   // Batch segment index of this thread in the first 256 of a batch: the
   // batch's segments are dealt to the four waves in turn, so every SIMD of
@@ -650,7 +725,7 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
   // commands whose tile-column range holds this tile (ballot compaction,
   // submission order kept); the filter also gathers, in parallel, each
   // selected command's (band, command) cell extent and style index.
-  __shared__ uint32_t sel_cmd[kTileThreads], sel_off[kTileThreads], sel_cnt[kTileThreads];
+  __shared__ uint32_t sel_cmd[kTileThreads], sel_off[kTileThreads], sel_cnt[kTileThreads], sel_style[kTileThreads];
   __shared__ uint32_t sel_wave[kTileThreads / 64];
   // Row tasks of the segments that reach the tile: the segment of each
   // thread, and its first task index (low 12 bits) and batch slot (bits 12+).
@@ -666,13 +741,13 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
 
   for (uint32_t chunk = lb; chunk < le; chunk += kTileThreads) {
     const uint32_t li = chunk + (uint32_t)t;
-    uint32_t e = 0, cmd_i = 0;
+    uint4 ent = make_uint4(0, 0, 0, 0);
+    uint32_t style_i = 0;
     bool keep = false;
     if (li < le) {
-      e = s.list[li];
-      cmd_i = cv.cmd_base + (uint32_t)(e - row_base);
-      const uint32_t tr = s.cmd_tiles[cmd_i];
-      keep = tcol >= (tr & 0xFFFFu) && tcol <= (tr >> 16);
+      ent = s.list_ent[li];
+      style_i = s.list_style[li];
+      keep = tcol >= (ent.y & 0xFFFFu) && tcol <= (ent.y >> 16);
     }
     const uint64_t bal = __ballot(keep);
     if (lane == 0) sel_wave[wave] = (uint32_t)__popcll(bal);
@@ -686,9 +761,10 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
     }
     if (keep) {
       const uint32_t slot = off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-      sel_cmd[slot] = cmd_i;
-      sel_off[slot] = s.mat_off[e];
-      sel_cnt[slot] = s.mat_cnt[e];
+      sel_cmd[slot] = ent.x;
+      sel_off[slot] = ent.z;
+      sel_cnt[slot] = ent.w;
+      sel_style[slot] = style_i;
     }
     __syncthreads();
     if (n_sel == 0) continue;   // uniform; sel_wave is rewritten only after the next chunk's ballot
@@ -700,10 +776,10 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
       const uint32_t nb = n_sel < (uint32_t)kBatch ? n_sel : (uint32_t)kBatch;
       uint32_t end[kBatch];
       batch_prefix(sel_cnt, 0, nb, end);
-      if (t < (int)nb * kRecWords) rec[0][t / kRecWords].w[t % kRecWords] = batch_rec_word(f, s, sel_cmd, 0, t);
+      if (t < (int)nb * kRecWords) rec[0][t / kRecWords].w[t % kRecWords] = batch_rec_word(f, s, sel_cmd, sel_style, 0, t);
       if (t + kTileThreads < (int)nb * kRecWords) {
         const int gw = t + kTileThreads;
-        rec[0][gw / kRecWords].w[gw % kRecWords] = batch_rec_word(f, s, sel_cmd, 0, gw);
+        rec[0][gw / kRecWords].w[gw % kRecWords] = batch_rec_word(f, s, sel_cmd, sel_style, 0, gw);
       }
       if (jt < end[kBatch - 1]) {
         uint32_t loc;
@@ -730,8 +806,8 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
       if (has_next) {
         const uint32_t nbn = n_sel - kn < (uint32_t)kBatch ? n_sel - kn : (uint32_t)kBatch;
         nwn = (int)nbn * kRecWords;
-        if (t < nwn) pw0 = batch_rec_word(f, s, sel_cmd, kn, t);
-        if (t + kTileThreads < nwn) pw1 = batch_rec_word(f, s, sel_cmd, kn, t + kTileThreads);
+        if (t < nwn) pw0 = batch_rec_word(f, s, sel_cmd, sel_style, kn, t);
+        if (t + kTileThreads < nwn) pw1 = batch_rec_word(f, s, sel_cmd, sel_style, kn, t + kTileThreads);
         uint32_t endn[kBatch];
         batch_prefix(sel_cnt, kn, nbn, endn);
         pre_seg = DevSeg{0, 0, 0, 0};
@@ -813,23 +889,8 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
           m[2] = calc_mask(c2, c.rule_mask, c.alpha);
           m[3] = calc_mask(c3, c.rule_mask, c.alpha);
         }
-        else if (y >= c.box[1] && y < c.box[3]) {
-          if (c.kind == kCmdBoxA) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) m[k] = (px + k >= c.box[0] && px + k < c.box[2]) ? c.alpha : 0u;
-          }
-          else {
-            int kind = (y == c.box_row_first) ? 0 : (y == c.box_row_last) ? 2 : 1;
-            uint32_t rm = c.box_mask[kind];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-              int x = px + k;
-              if (x >= c.box[0] && x < c.box[2]) {
-                int col = (x == c.box[0]) ? 0 : (x == c.box[2] - 1) ? 2 : 1;
-                m[k] = (rm >> (8 * col)) & 0xFFu;
-              }
-            }
-          }
+        else {
+          box_masks4(c, y, px, m);
         }
 
         if (row_ok && (m[0] | m[1] | m[2] | m[3]) != 0) {
@@ -837,13 +898,7 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
           const DevStyle& st = rec[cur][b].r.style;
           const int fill_row0 = (c.kind == kCmdAnalytic) ? (rec[cur][b].r.min_y >> 8) : (int)c.fetch_row0;
           const pipe::Row rs = pipe::row_setup(st, y, fill_row0, c.kind == kCmdAnalytic ? cv.ref_band_h : 0u);
-#pragma unroll
-          for (int k = 0; k < 4; k++) {
-            if (!m[k] || px + k >= (int)cv.w) continue;
-            uint32_t src = pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
-            d[k] = a8 ? pipe::composite8(c.op, d[k], src >> 24, m[k]) : pipe::composite32(c.op, d[k], src, m[k]);
-            npix++;
-          }
+          npix += composite4(f, c, st, rs, px, (int)cv.w, a8, m, d);
         }
       }
       if (has_next) {
@@ -871,29 +926,13 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
   }
 
   if (!touched || !row_ok) return;
-  if (a8) {
-    if (full4 && (((uintptr_t)(rowp + px)) & 3) == 0) {
-      *(uint32_t*)(rowp + px) = d[0] | (d[1] << 8) | (d[2] << 16) | (d[3] << 24);
-    }
-    else {
-#pragma unroll
-      for (int k = 0; k < 4; k++) if (px + k < (int)cv.w) rowp[px + k] = (uint8_t)d[k];
-    }
-  }
-  else {
-    uint32_t* r32 = (uint32_t*)rowp;
-    if (full4 && (((uintptr_t)(r32 + px)) & 15) == 0) {
-      *(uint4*)(r32 + px) = make_uint4(d[0], d[1], d[2], d[3]);
-    }
-    else {
-#pragma unroll
-      for (int k = 0; k < 4; k++) if (px + k < (int)cv.w) r32[px + k] = d[k];
-    }
-  }
+  store4(rowp, px, (int)cv.w, a8, d);
 }
 
-// Canvas index of every tile (binary search over the canvases' first tiles).
-__global__ void k_tile_canvas(FrameDev f, uint32_t* tile_canvas) {
+// Per-tile header: canvas (binary search over the canvases' first tiles),
+// the band's command list range, tile column and band, and whether the band
+// holds analytic fills (a non-empty binned range).
+__global__ void k_tile_setup(FrameDev f, FrameScratch s) {
   const uint32_t tile = blockIdx.x * blockDim.x + threadIdx.x;
   if (tile >= f.n_tiles) return;
   uint32_t lo = 0, hi = f.n_canvases;
@@ -901,7 +940,177 @@ __global__ void k_tile_canvas(FrameDev f, uint32_t* tile_canvas) {
     const uint32_t mid = (lo + hi) >> 1;
     if (f.canvases[mid].tile_base <= tile) lo = mid; else hi = mid;
   }
-  tile_canvas[tile] = lo;
+  const DevCanvas& cv = f.canvases[lo];
+  const uint32_t local = tile - cv.tile_base;
+  const uint32_t band = local / cv.tiles_x, tcol = local % cv.tiles_x;
+  const uint64_t row_base = cv.mat_base + (uint64_t)band * cv.n_cmds;
+  const uint64_t row_end = row_base + cv.n_cmds;
+  const uint32_t analytic = s.mat_off[row_end] != s.mat_off[row_base] ? 1u : 0u;
+  s.tile_hdr[tile] = make_uint4(lo, s.list_off[row_base], s.list_off[row_end], tcol | (band << 16) | (analytic << 31));
+}
+
+// The same per tile group.
+__global__ void k_group_setup(FrameDev f, FrameScratch s) {
+  const uint32_t grp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (grp >= f.n_groups) return;
+  uint32_t lo = 0, hi = f.n_canvases;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (f.canvases[mid].group_base <= grp) lo = mid; else hi = mid;
+  }
+  const DevCanvas& cv = f.canvases[lo];
+  const uint32_t local = grp - cv.group_base;
+  const uint32_t band = local / cv.groups_x, gcol = local % cv.groups_x;
+  const uint64_t row_base = cv.mat_base + (uint64_t)band * cv.n_cmds;
+  const uint64_t row_end = row_base + cv.n_cmds;
+  const uint32_t analytic = s.mat_off[row_end] != s.mat_off[row_base] ? 1u : 0u;
+  s.group_hdr[grp] = make_uint4(lo, s.list_off[row_base], s.list_off[row_end], gcol | (band << 16) | (analytic << 31));
+}
+
+// Bands with box fills only: one workgroup per group of up to 4 tiles of a
+// band (256 x 16 pixels, 16 consecutive pixels per thread).  No coverage
+// cells are needed, so the workgroup is light, and one walk of the band's
+// command list and records serves four tiles.
+__global__ __launch_bounds__(kTileThreads) void k_render_box(FrameDev f, FrameScratch s) {
+  const uint32_t grp = xcd_tile(blockIdx.x, f.n_groups);
+  if (grp >= f.n_groups) return;   // uniform: padding workgroups of the XCD mapping
+  const uint4 gh = s.group_hdr[grp];
+  if (gh.w >> 31) return;          // the band has analytic fills: k_render takes its tiles
+  const int t = threadIdx.x;
+  const DevCanvas cv = f.canvases[gh.x];
+  const uint32_t lb = gh.y, le = gh.z;
+  const uint32_t band = (gh.w >> 16) & 0x7FFFu;
+  const uint32_t tc0 = (gh.w & 0xFFFFu) * kGroupTiles;
+  const uint32_t tc1 = tc0 + kGroupTiles - 1 < cv.tiles_x - 1 ? tc0 + kGroupTiles - 1 : cv.tiles_x - 1;
+  const uint32_t tile0 = cv.tile_base + band * cv.tiles_x + tc0;
+  if (s.flags && *s.flags) {
+    if ((uint32_t)t <= tc1 - tc0) s.tile_px[tile0 + t] = 0;
+    return;
+  }
+
+  const int row = t >> 4;
+  const int q = t & 15;
+  const int y = (int)band * kTileH + row;
+  const int px0 = (int)tc0 * kTileW + q * 16;
+  const int w = (int)cv.w;
+  const bool row_ok = y < (int)cv.h;
+  const bool a8 = cv.format == kFmtA8;
+
+  uint32_t d[4][4];
+  uint8_t* rowp = (uint8_t*)cv.ptr + (int64_t)y * cv.stride;
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) d[g][k] = 0u;
+    if (row_ok && px0 + 4 * g < w) load4(rowp, px0 + 4 * g, w, a8, d[g]);
+  }
+
+  const int lane = t & 63, wave = t >> 6;
+  bool touched = false;
+  uint32_t npix = 0;
+  __shared__ uint32_t sel_cmd[kTileThreads], sel_style[kTileThreads];
+  __shared__ uint32_t sel_wave[kTileThreads / 64];
+  __shared__ RenderRec rec[2][kBatch];
+
+  for (uint32_t chunk = lb; chunk < le; chunk += kTileThreads) {
+    const uint32_t li = chunk + (uint32_t)t;
+    uint4 ent = make_uint4(0, 0, 0, 0);
+    uint32_t style_i = 0;
+    bool keep = false;
+    if (li < le) {
+      ent = s.list_ent[li];
+      style_i = s.list_style[li];
+      keep = (ent.y & 0xFFFFu) <= tc1 && (ent.y >> 16) >= tc0;
+    }
+    const uint64_t bal = __ballot(keep);
+    if (lane == 0) sel_wave[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t off = 0, n_sel = 0;
+#pragma unroll
+    for (int wv = 0; wv < kTileThreads / 64; wv++) {
+      const uint32_t cw = sel_wave[wv];
+      off += wv < wave ? cw : 0u;
+      n_sel += cw;
+    }
+    if (keep) {
+      const uint32_t slot = off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+      sel_cmd[slot] = ent.x;
+      sel_style[slot] = style_i;
+    }
+    __syncthreads();
+    if (n_sel == 0) continue;
+
+    {
+      const uint32_t nb = n_sel < (uint32_t)kBatch ? n_sel : (uint32_t)kBatch;
+      if (t < (int)nb * kRecWords) rec[0][t / kRecWords].w[t % kRecWords] = batch_rec_word(f, s, sel_cmd, sel_style, 0, t);
+      if (t + kTileThreads < (int)nb * kRecWords) {
+        const int gw = t + kTileThreads;
+        rec[0][gw / kRecWords].w[gw % kRecWords] = batch_rec_word(f, s, sel_cmd, sel_style, 0, gw);
+      }
+    }
+    __syncthreads();
+
+    for (uint32_t kb = 0, bi = 0; kb < n_sel; kb += kBatch, bi++) {
+      const int cur = (int)(bi & 1u);
+      const uint32_t nb = n_sel - kb < (uint32_t)kBatch ? n_sel - kb : (uint32_t)kBatch;
+      const uint32_t kn = kb + kBatch;
+      const bool has_next = kn < n_sel;
+      uint32_t pw0 = 0, pw1 = 0;
+      int nwn = 0;
+      if (has_next) {
+        const uint32_t nbn = n_sel - kn < (uint32_t)kBatch ? n_sel - kn : (uint32_t)kBatch;
+        nwn = (int)nbn * kRecWords;
+        if (t < nwn) pw0 = batch_rec_word(f, s, sel_cmd, sel_style, kn, t);
+        if (t + kTileThreads < nwn) pw1 = batch_rec_word(f, s, sel_cmd, sel_style, kn, t + kTileThreads);
+      }
+      for (uint32_t b = 0; b < nb; b++) {
+        const DevCmd& c = rec[cur][b].r.cmd;
+        if (c.kind == kCmdAnalytic || !row_ok) continue;
+        if (y < c.box[1] || y >= c.box[3] || px0 + 16 <= c.box[0] || px0 >= c.box[2]) continue;
+        touched = true;
+        const DevStyle& st = rec[cur][b].r.style;
+        const pipe::Row rs = pipe::row_setup(st, y, (int)c.fetch_row0, 0u);
+        // One composite body for the four pixel quads: quad g is always in
+        // d[0], the quads rotate through it.
+#pragma unroll 1
+        for (int g = 0; g < 4; g++) {
+          uint32_t m[4];
+          box_masks4(c, y, px0 + 4 * g, m);
+          npix += composite4(f, c, st, rs, px0 + 4 * g, w, a8, m, d[0]);
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const uint32_t r0 = d[0][k];
+            d[0][k] = d[1][k];
+            d[1][k] = d[2][k];
+            d[2][k] = d[3][k];
+            d[3][k] = r0;
+          }
+        }
+      }
+      if (has_next) {
+        if (t < nwn) rec[cur ^ 1][t / kRecWords].w[t % kRecWords] = pw0;
+        if (t + kTileThreads < nwn) {
+          const int gw = t + kTileThreads;
+          rec[cur ^ 1][gw / kRecWords].w[gw % kRecWords] = pw1;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // Composited pixels per tile of the group (a thread's 16 pixels lie in
+  // tile q / 4 of the group).
+  __shared__ uint32_t tile_cnt[kGroupTiles];
+  if (t < kGroupTiles) tile_cnt[t] = 0;
+  __syncthreads();
+  if (npix) atomicAdd(&tile_cnt[q >> 2], npix);
+  __syncthreads();
+  if ((uint32_t)t <= tc1 - tc0) s.tile_px[tile0 + t] = tile_cnt[t];
+
+  if (!touched || !row_ok) return;
+#pragma unroll
+  for (int g = 0; g < 4; g++)
+    if (px0 + 4 * g < w) store4(rowp, px0 + 4 * g, w, a8, d[g]);
 }
 
 // *out += sum of v[0, n): per-workgroup partial sums, one atomic each
@@ -921,13 +1130,16 @@ __global__ __launch_bounds__(256) void k_sum_u64(const unsigned long long* v, ui
 
 void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
   if (!f.n_tiles) return;
-  hipLaunchKernelGGL(k_tile_canvas, dim3((f.n_tiles + 255) / 256), dim3(256), 0, st, f, s.tile_canvas);
+  hipLaunchKernelGGL(k_tile_setup, dim3((f.n_tiles + 255) / 256), dim3(256), 0, st, f, s);
+  hipLaunchKernelGGL(k_group_setup, dim3((f.n_groups + 255) / 256), dim3(256), 0, st, f, s);
 }
 
 void launch_render(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
   if (!f.n_tiles) return;
   const uint32_t grid = (f.n_tiles + kXcds - 1) / kXcds * kXcds;
   hipLaunchKernelGGL(k_render, dim3(grid), dim3(kTileThreads), 0, st, f, s);
+  const uint32_t ggrid = (f.n_groups + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL(k_render_box, dim3(ggrid), dim3(kTileThreads), 0, st, f, s);
 }
 
 void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st) {

@@ -558,6 +558,70 @@ def mixed_scene(seed: int = 1, width: int = 513, height: int = 513, count: int =
     return sc
 
 
+def box_scene(seed: int = 1, width: int = 333, height: int = 77, count: int = 60, fmt: int = FMT_PRGB32,
+              comp_ops=(COMP_OP_SRC_OVER, COMP_OP_SRC_COPY)) -> Scene:
+    """Box fills only (integer and fractional rects, fill_all, clears) under
+    identity, translate and scale transforms, so every band holds box fills
+    only: solid, linear / radial / conic gradients (PAD / REPEAT / REFLECT)
+    and nearest / bilinear affine patterns, global and fill alpha, the given
+    comp ops.  Canvas sizes need not be multiples of the tile size."""
+    rng = np.random.default_rng(seed)
+    clear = 0xFF000000 | int(rng.integers(0, 1 << 24)) if fmt != FMT_A8 else int(rng.integers(0, 256))
+    sc = Scene(width, height, fmt, clear)
+    img = sc.add_image(premultiplied_noise(rng, 32, 32))
+    for i in range(count):
+        sc.set_comp_op(int(comp_ops[int(rng.integers(0, len(comp_ops)))]))
+        sc.set_global_alpha(1.0 if rng.random() < 0.7 else float(rng.uniform(0.0, 1.0)))
+        sc.set_fill_alpha(1.0 if rng.random() < 0.8 else float(rng.uniform(0.0, 1.0)))
+        t = rng.random()
+        if t < 0.5:
+            sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+        elif t < 0.75:
+            sc.set_transform([1.0, 0.0, 0.0, 1.0, float(rng.uniform(-9, 9)), float(rng.uniform(-9, 9))])
+        else:
+            sc.set_transform([float(2.0 ** rng.uniform(-1, 1)), 0.0, 0.0, float(2.0 ** rng.uniform(-1, 1)),
+                              float(rng.uniform(-20, 20)), float(rng.uniform(-20, 20))])
+        cx, cy = float(rng.uniform(0, width)), float(rng.uniform(0, height))
+        size = float(rng.uniform(8, max(width, height, 9)))
+        st = rng.random()
+        if st < 0.3:
+            sc.set_fill_style_rgba32(_rand_rgba(rng))
+        elif st < 0.8:
+            gtype = int(rng.integers(0, 3))
+            extend = int(rng.integers(0, 3))
+            stops = _rand_stops(rng, 1, 6)
+            if gtype == GRADIENT_LINEAR:
+                vals = [float(v) for v in rng.random(4) * size + np.array([cx, cy, cx, cy]) - size / 2]
+            elif gtype == GRADIENT_RADIAL:
+                r = float(rng.uniform(4, size))
+                f = (rng.random(2) - 0.5) * r
+                vals = [cx, cy, cx + float(f[0]), cy + float(f[1]), r, 0.0]
+            else:
+                vals = [cx, cy, float(rng.uniform(0, 2 * math.pi)), float(rng.choice([1.0, 2.0, 0.5]))]
+            sc.set_fill_style_gradient(gtype, vals, stops, extend)
+        else:
+            sc.set_pattern_quality(int(rng.integers(0, 2)))
+            a = float(rng.uniform(0.1, 2 * math.pi - 0.1))
+            s = float(2.0 ** rng.uniform(-1, 1))
+            c_, s_ = math.cos(a) * s, math.sin(a) * s
+            sc.set_fill_style_pattern(img, int(rng.integers(0, 3)), [c_, s_, -s_, c_, cx, cy])
+        g = rng.random()
+        x0, y0 = float(rng.uniform(-0.2, 1.0) * width), float(rng.uniform(-0.2, 1.0) * height)
+        w, h = float(rng.uniform(1, size)), float(rng.uniform(1, size * 0.5))
+        if g < 0.4:
+            sc.fill_rect_i(int(x0), int(y0), int(w), int(h))
+        elif g < 0.85:
+            sc.fill_rect_d(x0, y0, w, h)
+        elif g < 0.9:
+            sc.fill_all()
+        elif g < 0.95:
+            sc.clear_rect_d(x0, y0, w, h)
+        else:
+            sc.fill_rect_d_rgba32(x0, y0, w, h, _rand_rgba(rng))
+    sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+    return sc
+
+
 def edge_case_scene(width: int = 257, height: int = 131, fmt: int = FMT_PRGB32) -> Scene:
     """Degenerate and boundary inputs: empty and one-point paths, zero-area
     and zero-size rects, geometry exactly on and far outside the canvas

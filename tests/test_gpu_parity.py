@@ -70,6 +70,27 @@ def test_mixed_all_comp_ops(amd, seed, fmt):
     _cmp(amd, S.mixed_scene(seed, 389, 301, 160, fmt, comp_ops=ops), tol=1)
 
 
+@pytest.mark.parametrize("seed", [1, 2])
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_XRGB32, S.FMT_A8])
+def test_box_only_bands(amd, seed, fmt):
+    # every band holds box fills only: the tile-group compositor path, with
+    # canvas widths that are not multiples of a tile group
+    ops = (S.COMP_OP_SRC_OVER, S.COMP_OP_SRC_COPY, S.COMP_OP_PLUS, S.COMP_OP_MULTIPLY, S.COMP_OP_SCREEN)
+    if fmt != S.FMT_PRGB32:
+        ops = ops[:3]
+    _cmp(amd, S.box_scene(seed, 333 + 257 * seed, 77, 80, fmt, comp_ops=ops), tol=1)
+
+
+def test_box_and_analytic_canvases_batched(amd):
+    scenes = [S.box_scene(3, 255, 33, 30), S.polygons_scene(9, 70, 40, 50), S.box_scene(4, 1, 1, 5, S.FMT_A8),
+              S.box_scene(5, 64, 16, 20, S.FMT_XRGB32)]
+    outs, errs = amd.render_scenes_batched(scenes)
+    for sc, out, err in zip(scenes, outs, errs):
+        want, oerr = checkers.render_oracle(sc)
+        assert err == oerr
+        assert checkers.diff_info(out, want)[0] <= 1
+
+
 def test_edge_cases_exact(amd):
     for fmt in (S.FMT_PRGB32, S.FMT_XRGB32, S.FMT_A8):
         _cmp(amd, S.edge_case_scene(fmt=fmt))

@@ -50,3 +50,10 @@ def test_frontend_calls(seed, fmt):
     # fill_geometry, explicit colours, clears, fill alpha, 64-bit / float
     # colours, origins, non-invertible transforms
     _same(S.frontend_scene(seed, 300, 220, fmt, 150))
+
+
+@pytest.mark.parametrize("seed", [71, 72])
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_XRGB32, S.FMT_A8])
+def test_box_fills(seed, fmt):
+    # rects, fill_all and clears only, under translate / scale transforms
+    _same(S.box_scene(seed, 333, 77, 80, fmt))
