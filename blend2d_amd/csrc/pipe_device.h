@@ -27,6 +27,19 @@ __device__ __forceinline__ uint32_t div255(uint32_t x) {
 
 __device__ __forceinline__ uint32_t ch(uint32_t p, int i) { return (p >> (i * 8)) & 0xFFu; }
 
+// div255 of two 16-bit lanes at once (bits 0-15 and 16-31, each <= 255 * 255):
+// every intermediate of a lane stays below 65536, so no carry crosses lanes.
+__device__ __forceinline__ uint32_t div255x2(uint32_t x) {
+  const uint32_t u = x + 0x00800080u;
+  return ((u + ((u >> 8) & 0x00FF00FFu)) >> 8) & 0x00FF00FFu;
+}
+
+// Per-lane min(v, 255) of two lanes holding values <= 510.
+__device__ __forceinline__ uint32_t sat255x2(uint32_t v) {
+  const uint32_t o = v & 0x01000100u;
+  return (v | (o - (o >> 8))) & 0x00FF00FFu;
+}
+
 // x86 cvttss2si / cvtss2si: out-of-range and NaN produce INT32_MIN.
 __device__ __forceinline__ int32_t trunc_f32_to_i32(float v) {
   if (!(v >= -2147483648.0f && v < 2147483648.0f)) return INT32_MIN;
@@ -193,27 +206,25 @@ __device__ __forceinline__ uint32_t fetch(const DevStyle& s, const Row& r, int x
 __device__ __forceinline__ uint32_t composite32(uint32_t op, uint32_t d, uint32_t s, uint32_t m) {
   uint32_t r = 0;
   switch (op) {
+    // SRC_COPY, SRC_OVER and PLUS work on the R/B and A/G channel pairs
+    // (two 16-bit lanes per register); per channel the arithmetic is the
+    // scalar div255 form.
     case kOpSrcCopy: {
-#pragma unroll
-      for (int i = 0; i < 4; i++) r |= div255(ch(d, i) * (m ^ 0xFFu) + ch(s, i) * m) << (i * 8);
-      return r;
+      const uint32_t im = m ^ 0xFFu;
+      const uint32_t rb = div255x2((d & 0x00FF00FFu) * im + (s & 0x00FF00FFu) * m);
+      const uint32_t ag = div255x2(((d >> 8) & 0x00FF00FFu) * im + ((s >> 8) & 0x00FF00FFu) * m);
+      return rb | (ag << 8);
     }
     case kOpSrcOver: {
-      uint32_t s1 = 0;
-#pragma unroll
-      for (int i = 0; i < 4; i++) s1 |= div255(ch(s, i) * m) << (i * 8);
-      uint32_t ia = (s1 >> 24) ^ 0xFFu, t = 0;
-#pragma unroll
-      for (int i = 0; i < 4; i++) t |= div255(ch(d, i) * ia) << (i * 8);
+      const uint32_t s1 = div255x2((s & 0x00FF00FFu) * m) | (div255x2(((s >> 8) & 0x00FF00FFu) * m) << 8);
+      const uint32_t ia = (s1 >> 24) ^ 0xFFu;
+      const uint32_t t = div255x2((d & 0x00FF00FFu) * ia) | (div255x2(((d >> 8) & 0x00FF00FFu) * ia) << 8);
       return s1 + t;
     }
     case kOpPlus: {
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        uint32_t v = ch(d, i) + div255(ch(s, i) * m);
-        r |= (v > 255u ? 255u : v) << (i * 8);
-      }
-      return r;
+      const uint32_t rb = sat255x2((d & 0x00FF00FFu) + div255x2((s & 0x00FF00FFu) * m));
+      const uint32_t ag = sat255x2(((d >> 8) & 0x00FF00FFu) + div255x2(((s >> 8) & 0x00FF00FFu) * m));
+      return rb | (ag << 8);
     }
     case kOpMultiply: {
       uint32_t s1[4];
