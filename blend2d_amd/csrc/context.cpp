@@ -142,7 +142,7 @@ struct Runner {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf in;
   DevBuf src_count, src_off, bounds, segs, seg_cmd, mat_cnt, mat_off, mat_pres, list_off, list, binned, cursor, scan_tmp, counters, cmd_tiles;
-  DevBuf tile_hdr, group_hdr, tile_px, list_style;
+  DevBuf tile_hdr, group_hdr, tile_px, list_style, band_buf;
   DevBuf cm_params, cm_acc, cm_feat, cm_pred, cm_act;
   uint64_t cm_version = 0;
   uint32_t* host_readback = nullptr;
@@ -249,6 +249,7 @@ uint32_t Runner::upload(FrameHost& fh) {
   f.images = (const DevImage*)(base + o_images);
   f.n_tiles = n_tiles;
   f.n_groups = n_groups;
+  f.n_bands = band_base;
   f.n_mat = n_mat;
   f.tol_sq = kToleranceSq;
   n_src = fh.src.size();
@@ -315,19 +316,24 @@ uint32_t Runner::execute(Stats& stats) {
 uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   auto t0 = std::chrono::steady_clock::now();
   const uint64_t n_mat = f.n_mat;
-  size_t scan_words = scan_tmp_words((size_t)n_mat + 1);
+  size_t scan_words = scan_tmp_words(std::max<size_t>((size_t)n_mat, f.n_bands) + 1);
   if (!bounds.reserve((size_t)f.n_cmds * sizeof(CmdBounds)) || !mat_cnt.reserve((n_mat + 1) * 4) ||
       !mat_off.reserve((n_mat + 1) * 4) || !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
       !list.reserve((n_mat + 1) * 16) || !list_style.reserve((n_mat + 1) * 4) || !cursor.reserve((n_mat + 1) * 4) ||
       !scan_tmp.reserve(scan_words * 4) ||
       !counters.reserve(64) || !cmd_tiles.reserve((size_t)f.n_cmds * 4) ||
-      !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 16) || !tile_px.reserve((size_t)f.n_tiles * 8))
+      !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 16) ||
+      !band_buf.reserve(((size_t)f.n_bands + 1) * 16) || !tile_px.reserve((size_t)f.n_tiles * 8))
     return kErrOOM;
 
   FrameScratch s;
   memset(&s, 0, sizeof(s));
   s.tile_hdr = tile_hdr.as<uint4>();
   s.group_hdr = group_hdr.as<uint4>();
+  s.band_tiles = band_buf.as<uint32_t>();
+  s.band_groups = s.band_tiles + (f.n_bands + 1);
+  s.tile_off = s.band_groups + (f.n_bands + 1);
+  s.group_off = s.tile_off + (f.n_bands + 1);
   s.tile_px = tile_px.as<unsigned long long>();
   s.bounds = bounds.as<CmdBounds>();
   s.mat_cnt = mat_cnt.as<uint32_t>();
@@ -388,13 +394,14 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   launch_presence(f, s, stream);
   scan_exclusive(s.mat_cnt, s.mat_off, n_mat, s.scan_tmp, stream);
   scan_exclusive(s.mat_pres, s.list_off, n_mat, s.scan_tmp, stream);
+  launch_band_class(f, s, stream);
   HIP_OK(hipMemcpyAsync(cursor.ptr, s.mat_off, (n_mat + 1) * 4, hipMemcpyDeviceToDevice, stream));
   launch_bin_scatter(f, s, cap_segs, cursor.as<uint32_t>(), stream);
   launch_list_build(f, s, stream);
 
   launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
-  launch_render(f, s, stream);
+  launch_render(f, s, f.n_tiles, f.n_groups, stream);
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback, counters.ptr, 16, hipMemcpyDeviceToHost, stream));
@@ -432,13 +439,14 @@ uint32_t Runner::execute_exact(Stats& stats) {
   auto t0 = std::chrono::steady_clock::now();
   const uint64_t n_mat = f.n_mat;
 
-  size_t scan_words = scan_tmp_words(std::max<size_t>(n_src, (size_t)n_mat) + 1);
+  size_t scan_words = scan_tmp_words(std::max<size_t>(std::max<size_t>(n_src, (size_t)n_mat), f.n_bands) + 1);
   if (!src_count.reserve((n_src + 1) * 4) || !src_off.reserve((n_src + 1) * 4) ||
       !bounds.reserve((size_t)f.n_cmds * sizeof(CmdBounds)) ||
       !mat_cnt.reserve((n_mat + 1) * 4) || !mat_off.reserve((n_mat + 1) * 4) ||
       !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
       !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) || !counters.reserve(64) ||
       !cmd_tiles.reserve((size_t)f.n_cmds * 4) || !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 16) ||
+      !band_buf.reserve(((size_t)f.n_bands + 1) * 16) ||
       !tile_px.reserve((size_t)f.n_tiles * 8))
     return kErrOOM;
 
@@ -446,6 +454,10 @@ uint32_t Runner::execute_exact(Stats& stats) {
   memset(&s, 0, sizeof(s));
   s.tile_hdr = tile_hdr.as<uint4>();
   s.group_hdr = group_hdr.as<uint4>();
+  s.band_tiles = band_buf.as<uint32_t>();
+  s.band_groups = s.band_tiles + (f.n_bands + 1);
+  s.tile_off = s.band_groups + (f.n_bands + 1);
+  s.group_off = s.tile_off + (f.n_bands + 1);
   s.tile_px = tile_px.as<unsigned long long>();
   s.src_count = src_count.as<uint32_t>();
   s.src_off = src_off.as<uint32_t>();
@@ -487,11 +499,18 @@ uint32_t Runner::execute_exact(Stats& stats) {
   launch_presence(f, s, stream);
   scan_exclusive(s.mat_cnt, s.mat_off, n_mat, s.scan_tmp, stream);
   scan_exclusive(s.mat_pres, s.list_off, n_mat, s.scan_tmp, stream);
+  launch_band_class(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback, s.mat_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipMemcpyAsync(host_readback + 1, s.list_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
+  if (f.n_tiles) {
+    HIP_OK(hipMemcpyAsync(host_readback + 12, s.tile_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(host_readback + 13, s.group_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+  }
   HIP_OK(hipStreamSynchronize(stream));
   stats.syncs++;
   const uint32_t n_binned = host_readback[0], n_list = host_readback[1];
+  const uint32_t n_atiles = f.n_tiles ? host_readback[12] : 0u, n_bgroups = f.n_tiles ? host_readback[13] : 0u;
+  if (n_atiles > f.n_tiles || n_bgroups > f.n_groups) return kErrDevice;
   // every edge segment crosses at least one band and at most all of them;
   // the command lists hold at most one entry per (band, command) cell
   if (n_binned < n_segs || (uint64_t)n_binned > (uint64_t)n_segs * std::max<uint32_t>(max_bands(), 1) || n_list > n_mat)
@@ -509,7 +528,7 @@ uint32_t Runner::execute_exact(Stats& stats) {
   // Stage 3: render.
   launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
-  launch_render(f, s, stream);
+  launch_render(f, s, n_atiles, n_bgroups, stream);
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback + 2, s.pixels, 8, hipMemcpyDeviceToHost, stream));

@@ -22,6 +22,7 @@ struct FrameDev {
   const DevImage* images;
   uint32_t n_tiles;
   uint32_t n_groups;       // tile groups: up to 4 consecutive tiles of one band
+  uint32_t n_bands;        // bands over all canvases
   uint64_t n_mat;          // (band, cmd) matrix entries over all canvases
   double tol_sq;           // squared flattening tolerance in 24.8 units
 };
@@ -43,8 +44,15 @@ struct FrameScratch {
   uint32_t* scan_tmp;      // scan_tmp_words(max n)
   unsigned long long* pixels;  // composited pixels (mask > 0), summed after render
   uint32_t* cmd_tiles;     // n_cmds: tile-column range per command
-  uint4* tile_hdr;         // n_tiles: {canvas, band list begin, band list end, tile column | band << 16 | analytic << 31}
-  uint4* group_hdr;        // n_groups: the same per tile group (group column in place of the tile column)
+  // Tiles of bands with analytic fills and tile groups of bands with box
+  // fills only, each compacted in band order: per band the tile (group)
+  // counts and their exclusive scans (totals at [n_bands]).
+  uint32_t* band_tiles;    // n_bands + 1
+  uint32_t* band_groups;   // n_bands + 1
+  uint32_t* tile_off;      // n_bands + 1
+  uint32_t* group_off;     // n_bands + 1
+  uint4* tile_hdr;         // <= n_tiles: {canvas, band list begin, band list end, tile column | band << 16}
+  uint4* group_hdr;        // <= n_groups: the same per tile group (group column in place of the tile column)
   unsigned long long* tile_px; // n_tiles: composited pixels of each tile
   // Pool mode (sizes predicted ahead of time; null / 0 otherwise):
   uint32_t* seg_cursor;    // segments taken from the pool
@@ -66,11 +74,14 @@ void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_bin_scatter(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, uint32_t* cursor, hipStream_t st);
 void launch_list_build(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_cmd_tiles(const FrameDev& f, const FrameScratch& s, hipStream_t st);
-// Stage 3: per-tile headers (setup), tile rasterization + fetch +
-// composite (render), composited pixels summed into *s.pixels, which must
+// Stage 3: band classes (after the binned scan), per-tile and per-group
+// headers (setup, after the list build), tile rasterization + fetch +
+// composite (render: n_a analytic tiles, n_g box tile groups, or upper
+// bounds of them), composited pixels summed into *s.pixels, which must
 // start at zero (stats).
+void launch_band_class(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t st);
-void launch_render(const FrameDev& f, const FrameScratch& s, hipStream_t st);
+void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint32_t n_g, hipStream_t st);
 void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);

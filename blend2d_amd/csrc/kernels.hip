@@ -677,24 +677,27 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t wg, uint32_t n_tiles) {
 
 // Four workgroups per CU: LDS (7 cell buffers) and the register budget are
 // sized for it.
-__global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScratch s) {
+__global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScratch s, uint32_t n) {
   __shared__ uint32_t cells[kBatch][kCellWords];
-  const uint32_t tile = xcd_tile(blockIdx.x, f.n_tiles);
-  if (tile >= f.n_tiles) return;   // uniform: padding workgroups of the XCD mapping
+  const uint32_t slot = xcd_tile(blockIdx.x, n);
+  // uniform: padding workgroups of the XCD mapping, and (when n is an upper
+  // bound) slots past the analytic tiles
+  if (slot >= n || slot >= s.tile_off[f.n_bands]) return;
   const int t = threadIdx.x;
-  const uint4 th = s.tile_hdr[tile];
-  if (!(th.w >> 31)) return;   // box fills only in this band: k_render_box composites it
+  const uint4 th = s.tile_hdr[slot];
+  const DevCanvas cv = f.canvases[th.x];
+  const uint32_t lb = th.y, le = th.z;
+  const uint32_t tcol = th.w & 0xFFFFu;
+  const uint32_t band = th.w >> 16;
+  const uint32_t tile = cv.tile_base + band * cv.tiles_x + tcol;
   // A pool overflow leaves the frame incomplete: nothing is composited and
   // the host redoes the frame with exact counting.
   if (s.flags && *s.flags) {
     if (t == 0) s.tile_px[tile] = 0;
     return;
   }
-  const DevCanvas cv = f.canvases[th.x];
-  const uint32_t lb = th.y, le = th.z;
-  const uint32_t tcol = th.w & 0xFFFFu;
   const int X0 = (int)tcol * kTileW;
-  const int Y0 = (int)((th.w >> 16) & 0x7FFFu) * kTileH;
+  const int Y0 = (int)band * kTileH;
 
   const int row = t >> 4;
   const int q = t & 15;
@@ -929,57 +932,76 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
   store4(rowp, px, (int)cv.w, a8, d);
 }
 
-// Per-tile header: canvas (binary search over the canvases' first tiles),
-// the band's command list range, tile column and band, and whether the band
-// holds analytic fills (a non-empty binned range).
-__global__ void k_tile_setup(FrameDev f, FrameScratch s) {
-  const uint32_t tile = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tile >= f.n_tiles) return;
+// Canvas of a global band: the last one whose first band is <= b.
+__device__ __forceinline__ uint32_t band_canvas(const FrameDev& f, uint32_t b) {
   uint32_t lo = 0, hi = f.n_canvases;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (f.canvases[mid].tile_base <= tile) lo = mid; else hi = mid;
+    if (f.canvases[mid].band_base <= b) lo = mid; else hi = mid;
   }
-  const DevCanvas& cv = f.canvases[lo];
-  const uint32_t local = tile - cv.tile_base;
-  const uint32_t band = local / cv.tiles_x, tcol = local % cv.tiles_x;
-  const uint64_t row_base = cv.mat_base + (uint64_t)band * cv.n_cmds;
-  const uint64_t row_end = row_base + cv.n_cmds;
-  const uint32_t analytic = s.mat_off[row_end] != s.mat_off[row_base] ? 1u : 0u;
-  s.tile_hdr[tile] = make_uint4(lo, s.list_off[row_base], s.list_off[row_end], tcol | (band << 16) | (analytic << 31));
+  return lo;
 }
 
-// The same per tile group.
-__global__ void k_group_setup(FrameDev f, FrameScratch s) {
-  const uint32_t grp = blockIdx.x * blockDim.x + threadIdx.x;
-  if (grp >= f.n_groups) return;
+// Per band: analytic fills present (a non-empty binned range) -> its tiles
+// go to k_render; otherwise its tile groups go to k_render_box.
+__global__ void k_band_class(FrameDev f, FrameScratch s) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= f.n_bands) return;
+  const DevCanvas& cv = f.canvases[band_canvas(f, b)];
+  const uint64_t row_base = cv.mat_base + (uint64_t)(b - cv.band_base) * cv.n_cmds;
+  const bool analytic = s.mat_off[row_base + cv.n_cmds] != s.mat_off[row_base];
+  s.band_tiles[b] = analytic ? cv.tiles_x : 0u;
+  s.band_groups[b] = analytic ? 0u : cv.groups_x;
+}
+
+// Compacted headers: {canvas, the band's command list range, column | band
+// << 16} for every tile of an analytic band and every group of a box band.
+__global__ void k_tile_setup(FrameDev f, FrameScratch s) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n_tiles) return;
   uint32_t lo = 0, hi = f.n_canvases;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (f.canvases[mid].group_base <= grp) lo = mid; else hi = mid;
+    if (f.canvases[mid].tile_base <= i) lo = mid; else hi = mid;
   }
   const DevCanvas& cv = f.canvases[lo];
-  const uint32_t local = grp - cv.group_base;
-  const uint32_t band = local / cv.groups_x, gcol = local % cv.groups_x;
+  const uint32_t local = i - cv.tile_base;
+  const uint32_t band = local / cv.tiles_x, col = local % cv.tiles_x;
+  const uint32_t gb = cv.band_base + band;
+  if (!s.band_tiles[gb]) return;
   const uint64_t row_base = cv.mat_base + (uint64_t)band * cv.n_cmds;
-  const uint64_t row_end = row_base + cv.n_cmds;
-  const uint32_t analytic = s.mat_off[row_end] != s.mat_off[row_base] ? 1u : 0u;
-  s.group_hdr[grp] = make_uint4(lo, s.list_off[row_base], s.list_off[row_end], gcol | (band << 16) | (analytic << 31));
+  s.tile_hdr[s.tile_off[gb] + col] = make_uint4(lo, s.list_off[row_base], s.list_off[row_base + cv.n_cmds], col | (band << 16));
+}
+
+__global__ void k_group_setup(FrameDev f, FrameScratch s) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n_groups) return;
+  uint32_t lo = 0, hi = f.n_canvases;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (f.canvases[mid].group_base <= i) lo = mid; else hi = mid;
+  }
+  const DevCanvas& cv = f.canvases[lo];
+  const uint32_t local = i - cv.group_base;
+  const uint32_t band = local / cv.groups_x, col = local % cv.groups_x;
+  const uint32_t gb = cv.band_base + band;
+  if (!s.band_groups[gb]) return;
+  const uint64_t row_base = cv.mat_base + (uint64_t)band * cv.n_cmds;
+  s.group_hdr[s.group_off[gb] + col] = make_uint4(lo, s.list_off[row_base], s.list_off[row_base + cv.n_cmds], col | (band << 16));
 }
 
 // Bands with box fills only: one workgroup per group of up to 4 tiles of a
 // band (256 x 16 pixels, 16 consecutive pixels per thread).  No coverage
 // cells are needed, so the workgroup is light, and one walk of the band's
 // command list and records serves four tiles.
-__global__ __launch_bounds__(kTileThreads) void k_render_box(FrameDev f, FrameScratch s) {
-  const uint32_t grp = xcd_tile(blockIdx.x, f.n_groups);
-  if (grp >= f.n_groups) return;   // uniform: padding workgroups of the XCD mapping
-  const uint4 gh = s.group_hdr[grp];
-  if (gh.w >> 31) return;          // the band has analytic fills: k_render takes its tiles
+__global__ __launch_bounds__(kTileThreads) void k_render_box(FrameDev f, FrameScratch s, uint32_t n) {
+  const uint32_t slot = xcd_tile(blockIdx.x, n);
+  if (slot >= n || slot >= s.group_off[f.n_bands]) return;   // uniform, as in k_render
+  const uint4 gh = s.group_hdr[slot];
   const int t = threadIdx.x;
   const DevCanvas cv = f.canvases[gh.x];
   const uint32_t lb = gh.y, le = gh.z;
-  const uint32_t band = (gh.w >> 16) & 0x7FFFu;
+  const uint32_t band = gh.w >> 16;
   const uint32_t tc0 = (gh.w & 0xFFFFu) * kGroupTiles;
   const uint32_t tc1 = tc0 + kGroupTiles - 1 < cv.tiles_x - 1 ? tc0 + kGroupTiles - 1 : cv.tiles_x - 1;
   const uint32_t tile0 = cv.tile_base + band * cv.tiles_x + tc0;
@@ -1128,18 +1150,23 @@ __global__ __launch_bounds__(256) void k_sum_u64(const unsigned long long* v, ui
   if (threadIdx.x == 0 && part[0]) atomicAdd(out, part[0]);
 }
 
+void launch_band_class(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
+  if (!f.n_tiles) return;
+  hipLaunchKernelGGL(k_band_class, dim3((f.n_bands + 255) / 256), dim3(256), 0, st, f, s);
+  scan_exclusive(s.band_tiles, s.tile_off, f.n_bands, s.scan_tmp, st);
+  scan_exclusive(s.band_groups, s.group_off, f.n_bands, s.scan_tmp, st);
+}
+
 void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
   if (!f.n_tiles) return;
   hipLaunchKernelGGL(k_tile_setup, dim3((f.n_tiles + 255) / 256), dim3(256), 0, st, f, s);
   hipLaunchKernelGGL(k_group_setup, dim3((f.n_groups + 255) / 256), dim3(256), 0, st, f, s);
 }
 
-void launch_render(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
+void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint32_t n_g, hipStream_t st) {
   if (!f.n_tiles) return;
-  const uint32_t grid = (f.n_tiles + kXcds - 1) / kXcds * kXcds;
-  hipLaunchKernelGGL(k_render, dim3(grid), dim3(kTileThreads), 0, st, f, s);
-  const uint32_t ggrid = (f.n_groups + kXcds - 1) / kXcds * kXcds;
-  hipLaunchKernelGGL(k_render_box, dim3(ggrid), dim3(kTileThreads), 0, st, f, s);
+  if (n_a) hipLaunchKernelGGL(k_render, dim3((n_a + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_a);
+  if (n_g) hipLaunchKernelGGL(k_render_box, dim3((n_g + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_g);
 }
 
 void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
