@@ -292,10 +292,16 @@ def main():
         x = torch.ones(n, device="cuda")
         y = torch.empty_like(x)
         torch.cuda.synchronize()
-        for _ in range(3):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        for i in range(3):
             torch.mul(x, 1.0, out=y)
+            ev[i + 1].record()
         torch.cuda.synchronize()
-        print(f"copy probe: 3 launches, {4 * n} bytes read and {4 * n} bytes written each", flush=True)
+        ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(3)]
+        print(f"copy probe: 3 launches, {4 * n} bytes read and {4 * n} bytes written each; "
+              f"{', '.join(f'{m:.3f}' for m in ms)} ms = {', '.join(f'{8 * n / m / 1e6:.0f}' for m in ms)} GB/s",
+              flush=True)
         return
 
     import blend2d_amd as B

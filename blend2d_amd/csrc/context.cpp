@@ -322,7 +322,7 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
       !list.reserve((n_mat + 1) * 16) || !list_style.reserve((n_mat + 1) * 4) || !cursor.reserve((n_mat + 1) * 4) ||
       !scan_tmp.reserve(scan_words * 4) ||
       !counters.reserve(64) || !cmd_tiles.reserve((size_t)f.n_cmds * 4) ||
-      !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 16) ||
+      !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 32) ||
       !band_buf.reserve(((size_t)f.n_bands + 1) * 16) || !tile_px.reserve((size_t)f.n_tiles * 8))
     return kErrOOM;
 
@@ -445,7 +445,7 @@ uint32_t Runner::execute_exact(Stats& stats) {
       !mat_cnt.reserve((n_mat + 1) * 4) || !mat_off.reserve((n_mat + 1) * 4) ||
       !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
       !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) || !counters.reserve(64) ||
-      !cmd_tiles.reserve((size_t)f.n_cmds * 4) || !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 16) ||
+      !cmd_tiles.reserve((size_t)f.n_cmds * 4) || !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 32) ||
       !band_buf.reserve(((size_t)f.n_bands + 1) * 16) ||
       !tile_px.reserve((size_t)f.n_tiles * 8))
     return kErrOOM;

@@ -52,7 +52,7 @@ struct FrameScratch {
   uint32_t* tile_off;      // n_bands + 1
   uint32_t* group_off;     // n_bands + 1
   uint4* tile_hdr;         // <= n_tiles: {canvas, band list begin, band list end, tile column | band << 16}
-  uint4* group_hdr;        // <= n_groups: the same per tile group (group column in place of the tile column)
+  uint4* group_hdr;        // <= 2 n_groups: two header words per box tile group (see k_group_setup)
   unsigned long long* tile_px; // n_tiles: composited pixels of each tile
   // Pool mode (sizes predicted ahead of time; null / 0 otherwise):
   uint32_t* seg_cursor;    // segments taken from the pool

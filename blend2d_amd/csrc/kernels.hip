@@ -973,6 +973,11 @@ __global__ void k_tile_setup(FrameDev f, FrameScratch s) {
   s.tile_hdr[s.tile_off[gb] + col] = make_uint4(lo, s.list_off[row_base], s.list_off[row_base + cv.n_cmds], col | (band << 16));
 }
 
+// Per box tile group two header words: {first tile index, the band's command
+// list range, group column | band << 16} and the destination address of the
+// group's first pixel {address low, address high, stride, width left of the
+// group's first pixel | valid rows << 16 | A8 << 21 | tiles << 24}, so the
+// compositor needs no canvas record.
 __global__ void k_group_setup(FrameDev f, FrameScratch s) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= f.n_groups) return;
@@ -987,39 +992,54 @@ __global__ void k_group_setup(FrameDev f, FrameScratch s) {
   const uint32_t gb = cv.band_base + band;
   if (!s.band_groups[gb]) return;
   const uint64_t row_base = cv.mat_base + (uint64_t)band * cv.n_cmds;
-  s.group_hdr[s.group_off[gb] + col] = make_uint4(lo, s.list_off[row_base], s.list_off[row_base + cv.n_cmds], col | (band << 16));
+  const uint32_t tc0 = col * kGroupTiles;
+  const uint32_t ntiles = cv.tiles_x - tc0 < (uint32_t)kGroupTiles ? cv.tiles_x - tc0 : (uint32_t)kGroupTiles;
+  const uint32_t x0 = tc0 * kTileW, y0 = band * kTileH;
+  const uint32_t rows = cv.h - y0 < (uint32_t)kTileH ? cv.h - y0 : (uint32_t)kTileH;
+  const uint32_t a8 = cv.format == kFmtA8 ? 1u : 0u;
+  const uint64_t ptr = cv.ptr + (uint64_t)((int64_t)y0 * cv.stride) + (uint64_t)x0 * (a8 ? 1u : 4u);
+  uint4* h = s.group_hdr + 2 * (size_t)(s.group_off[gb] + col);
+  h[0] = make_uint4(cv.tile_base + band * cv.tiles_x + tc0, s.list_off[row_base], s.list_off[row_base + cv.n_cmds],
+                    col | (band << 16));
+  h[1] = make_uint4((uint32_t)ptr, (uint32_t)(ptr >> 32), (uint32_t)cv.stride,
+                    (cv.w - x0) | (rows << 16) | (a8 << 21) | (ntiles << 24));
 }
 
 // Bands with box fills only: one workgroup per group of up to 4 tiles of a
-// band (256 x 16 pixels, 16 consecutive pixels per thread).  No coverage
+// band (256 x 16 pixels, 16 pixels per thread).  No coverage
 // cells are needed, so the workgroup is light, and one walk of the band's
 // command list and records serves four tiles.
 __global__ __launch_bounds__(kTileThreads) void k_render_box(FrameDev f, FrameScratch s, uint32_t n) {
   const uint32_t slot = xcd_tile(blockIdx.x, n);
   if (slot >= n || slot >= s.group_off[f.n_bands]) return;   // uniform, as in k_render
-  const uint4 gh = s.group_hdr[slot];
+  const uint4 gh = s.group_hdr[2 * (size_t)slot];
+  const uint4 gd = s.group_hdr[2 * (size_t)slot + 1];
   const int t = threadIdx.x;
-  const DevCanvas cv = f.canvases[gh.x];
   const uint32_t lb = gh.y, le = gh.z;
   const uint32_t band = gh.w >> 16;
   const uint32_t tc0 = (gh.w & 0xFFFFu) * kGroupTiles;
-  const uint32_t tc1 = tc0 + kGroupTiles - 1 < cv.tiles_x - 1 ? tc0 + kGroupTiles - 1 : cv.tiles_x - 1;
-  const uint32_t tile0 = cv.tile_base + band * cv.tiles_x + tc0;
+  const uint32_t ntiles = gd.w >> 24;
+  const uint32_t tc1 = tc0 + ntiles - 1;
+  const uint32_t tile0 = gh.x;
   if (s.flags && *s.flags) {
-    if ((uint32_t)t <= tc1 - tc0) s.tile_px[tile0 + t] = 0;
+    if ((uint32_t)t < ntiles) s.tile_px[tile0 + t] = 0;
     return;
   }
 
   const int row = t >> 4;
   const int q = t & 15;
   const int y = (int)band * kTileH + row;
-  const int px0 = (int)tc0 * kTileW + q * 16;
-  const int w = (int)cv.w;
-  const bool row_ok = y < (int)cv.h;
-  const bool a8 = cv.format == kFmtA8;
+  const int X0 = (int)tc0 * kTileW;
+  // The thread's pixels: 16 consecutive ones from px0, in tile q / 4 of the
+  // group (measured faster than interleaving the quads over the 4 tiles).
+  const int px0 = X0 + q * 16;
+  const int w = X0 + (int)(gd.w & 0xFFFFu);        // canvas width
+  const bool row_ok = row < (int)((gd.w >> 16) & 0x1Fu);
+  const bool a8 = (gd.w >> 21) & 1u;
 
+  // destination row pointer, offset so that pixel x is at rowp + x * bpp
   uint32_t d[4][4];
-  uint8_t* rowp = (uint8_t*)cv.ptr + (int64_t)y * cv.stride;
+  uint8_t* rowp = (uint8_t*)(((uint64_t)gd.y << 32) | gd.x) + (int64_t)row * (int32_t)gd.z - (int64_t)X0 * (a8 ? 1 : 4);
 #pragma unroll
   for (int g = 0; g < 4; g++) {
 #pragma unroll
@@ -1097,8 +1117,9 @@ __global__ __launch_bounds__(kTileThreads) void k_render_box(FrameDev f, FrameSc
 #pragma unroll 1
         for (int g = 0; g < 4; g++) {
           uint32_t m[4];
-          box_masks4(c, y, px0 + 4 * g, m);
-          npix += composite4(f, c, st, rs, px0 + 4 * g, w, a8, m, d[0]);
+          const int px = px0 + 4 * g;
+          box_masks4(c, y, px, m);
+          npix += composite4(f, c, st, rs, px, w, a8, m, d[0]);
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             const uint32_t r0 = d[0][k];
@@ -1120,14 +1141,14 @@ __global__ __launch_bounds__(kTileThreads) void k_render_box(FrameDev f, FrameSc
     }
   }
 
-  // Composited pixels per tile of the group (a thread's 16 pixels lie in
-  // tile q / 4 of the group).
+  // Composited pixels per tile of the group (a thread's pixels lie in tile
+  // q / 4).
   __shared__ uint32_t tile_cnt[kGroupTiles];
   if (t < kGroupTiles) tile_cnt[t] = 0;
   __syncthreads();
   if (npix) atomicAdd(&tile_cnt[q >> 2], npix);
   __syncthreads();
-  if ((uint32_t)t <= tc1 - tc0) s.tile_px[tile0 + t] = tile_cnt[t];
+  if ((uint32_t)t < ntiles) s.tile_px[tile0 + t] = tile_cnt[t];
 
   if (!touched || !row_ok) return;
 #pragma unroll
