@@ -323,17 +323,20 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
       !scan_tmp.reserve(scan_words * 4) ||
       !counters.reserve(64) || !cmd_tiles.reserve((size_t)f.n_cmds * 4) ||
       !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 32) ||
-      !band_buf.reserve(((size_t)f.n_bands + 1) * 16) || !tile_px.reserve((size_t)f.n_tiles * 8))
+      !band_buf.reserve(((size_t)f.n_bands + 1) * 28) || !tile_px.reserve((size_t)f.n_tiles * 8))
     return kErrOOM;
 
   FrameScratch s;
   memset(&s, 0, sizeof(s));
   s.tile_hdr = tile_hdr.as<uint4>();
   s.group_hdr = group_hdr.as<uint4>();
-  s.band_tiles = band_buf.as<uint32_t>();
-  s.band_groups = s.band_tiles + (f.n_bands + 1);
-  s.tile_off = s.band_groups + (f.n_bands + 1);
-  s.group_off = s.tile_off + (f.n_bands + 1);
+  s.band_nonsolid = band_buf.as<uint32_t>();
+  s.band_tiles = s.band_nonsolid + (f.n_bands + 1);
+  s.band_sgroups = s.band_tiles + (f.n_bands + 1);
+  s.band_ggroups = s.band_sgroups + (f.n_bands + 1);
+  s.tile_off = s.band_ggroups + (f.n_bands + 1);
+  s.sgroup_off = s.tile_off + (f.n_bands + 1);
+  s.ggroup_off = s.sgroup_off + (f.n_bands + 1);
   s.tile_px = tile_px.as<unsigned long long>();
   s.bounds = bounds.as<CmdBounds>();
   s.mat_cnt = mat_cnt.as<uint32_t>();
@@ -401,7 +404,7 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
 
   launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
-  launch_render(f, s, f.n_tiles, f.n_groups, stream);
+  launch_render(f, s, f.n_tiles, f.n_groups, f.n_groups, stream);
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback, counters.ptr, 16, hipMemcpyDeviceToHost, stream));
@@ -446,7 +449,7 @@ uint32_t Runner::execute_exact(Stats& stats) {
       !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
       !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) || !counters.reserve(64) ||
       !cmd_tiles.reserve((size_t)f.n_cmds * 4) || !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 32) ||
-      !band_buf.reserve(((size_t)f.n_bands + 1) * 16) ||
+      !band_buf.reserve(((size_t)f.n_bands + 1) * 28) ||
       !tile_px.reserve((size_t)f.n_tiles * 8))
     return kErrOOM;
 
@@ -454,10 +457,13 @@ uint32_t Runner::execute_exact(Stats& stats) {
   memset(&s, 0, sizeof(s));
   s.tile_hdr = tile_hdr.as<uint4>();
   s.group_hdr = group_hdr.as<uint4>();
-  s.band_tiles = band_buf.as<uint32_t>();
-  s.band_groups = s.band_tiles + (f.n_bands + 1);
-  s.tile_off = s.band_groups + (f.n_bands + 1);
-  s.group_off = s.tile_off + (f.n_bands + 1);
+  s.band_nonsolid = band_buf.as<uint32_t>();
+  s.band_tiles = s.band_nonsolid + (f.n_bands + 1);
+  s.band_sgroups = s.band_tiles + (f.n_bands + 1);
+  s.band_ggroups = s.band_sgroups + (f.n_bands + 1);
+  s.tile_off = s.band_ggroups + (f.n_bands + 1);
+  s.sgroup_off = s.tile_off + (f.n_bands + 1);
+  s.ggroup_off = s.sgroup_off + (f.n_bands + 1);
   s.tile_px = tile_px.as<unsigned long long>();
   s.src_count = src_count.as<uint32_t>();
   s.src_off = src_off.as<uint32_t>();
@@ -504,13 +510,15 @@ uint32_t Runner::execute_exact(Stats& stats) {
   HIP_OK(hipMemcpyAsync(host_readback + 1, s.list_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
   if (f.n_tiles) {
     HIP_OK(hipMemcpyAsync(host_readback + 12, s.tile_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
-    HIP_OK(hipMemcpyAsync(host_readback + 13, s.group_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(host_readback + 13, s.sgroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(host_readback + 14, s.ggroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
   }
   HIP_OK(hipStreamSynchronize(stream));
   stats.syncs++;
   const uint32_t n_binned = host_readback[0], n_list = host_readback[1];
-  const uint32_t n_atiles = f.n_tiles ? host_readback[12] : 0u, n_bgroups = f.n_tiles ? host_readback[13] : 0u;
-  if (n_atiles > f.n_tiles || n_bgroups > f.n_groups) return kErrDevice;
+  const uint32_t n_atiles = f.n_tiles ? host_readback[12] : 0u;
+  const uint32_t n_sgroups = f.n_tiles ? host_readback[13] : 0u, n_ggroups = f.n_tiles ? host_readback[14] : 0u;
+  if (n_atiles > f.n_tiles || (uint64_t)n_sgroups + n_ggroups > f.n_groups) return kErrDevice;
   // every edge segment crosses at least one band and at most all of them;
   // the command lists hold at most one entry per (band, command) cell
   if (n_binned < n_segs || (uint64_t)n_binned > (uint64_t)n_segs * std::max<uint32_t>(max_bands(), 1) || n_list > n_mat)
@@ -528,7 +536,7 @@ uint32_t Runner::execute_exact(Stats& stats) {
   // Stage 3: render.
   launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
-  launch_render(f, s, n_atiles, n_bgroups, stream);
+  launch_render(f, s, n_atiles, n_sgroups, n_ggroups, stream);
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback + 2, s.pixels, 8, hipMemcpyDeviceToHost, stream));

@@ -44,15 +44,20 @@ struct FrameScratch {
   uint32_t* scan_tmp;      // scan_tmp_words(max n)
   unsigned long long* pixels;  // composited pixels (mask > 0), summed after render
   uint32_t* cmd_tiles;     // n_cmds: tile-column range per command
-  // Tiles of bands with analytic fills and tile groups of bands with box
-  // fills only, each compacted in band order: per band the tile (group)
-  // counts and their exclusive scans (totals at [n_bands]).
+  // Tiles of bands with analytic fills, tile groups of bands with solid box
+  // fills only and tile groups of the other box-only bands, each compacted in
+  // band order: per band the tile (group) counts and their exclusive scans
+  // (totals at [n_bands]).
+  uint32_t* band_nonsolid; // n_bands + 1: a box fill with a non-solid style touches the band
   uint32_t* band_tiles;    // n_bands + 1
-  uint32_t* band_groups;   // n_bands + 1
+  uint32_t* band_sgroups;  // n_bands + 1
+  uint32_t* band_ggroups;  // n_bands + 1
   uint32_t* tile_off;      // n_bands + 1
-  uint32_t* group_off;     // n_bands + 1
+  uint32_t* sgroup_off;    // n_bands + 1
+  uint32_t* ggroup_off;    // n_bands + 1
   uint4* tile_hdr;         // <= n_tiles: {canvas, band list begin, band list end, tile column | band << 16}
-  uint4* group_hdr;        // <= 2 n_groups: two header words per box tile group (see k_group_setup)
+  uint4* group_hdr;        // <= 2 n_groups: two header words per box tile group (see k_group_setup),
+                           // the solid bands' groups first
   unsigned long long* tile_px; // n_tiles: composited pixels of each tile
   // Pool mode (sizes predicted ahead of time; null / 0 otherwise):
   uint32_t* seg_cursor;    // segments taken from the pool
@@ -76,12 +81,12 @@ void launch_list_build(const FrameDev& f, const FrameScratch& s, hipStream_t st)
 void launch_cmd_tiles(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 // Stage 3: band classes (after the binned scan), per-tile and per-group
 // headers (setup, after the list build), tile rasterization + fetch +
-// composite (render: n_a analytic tiles, n_g box tile groups, or upper
-// bounds of them), composited pixels summed into *s.pixels, which must
+// composite (render: n_a analytic tiles, n_s solid-box and n_g other box
+// tile groups, or upper bounds of them), composited pixels summed into *s.pixels, which must
 // start at zero (stats).
 void launch_band_class(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t st);
-void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint32_t n_g, hipStream_t st);
+void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint32_t n_s, uint32_t n_g, hipStream_t st);
 void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);

@@ -343,6 +343,8 @@ __global__ void k_presence(FrameScratch s, uint64_t n) {
   if (i < n) s.mat_pres[i] = s.mat_cnt[i] ? 1u : 0u;
 }
 
+// Box fills are present in every band their box touches; a box fill with a
+// non-solid style also marks its bands (they take the general compositor).
 __global__ void k_box_presence(FrameDev f, FrameScratch s) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= f.n_cmds) return;
@@ -351,11 +353,16 @@ __global__ void k_box_presence(FrameDev f, FrameScratch s) {
   const DevCanvas& cv = f.canvases[cmd.canvas];
   int b0 = cmd.box[1] / kTileH, b1 = (cmd.box[3] - 1) / kTileH;
   uint64_t row = cv.mat_base + (uint64_t)(c - cv.cmd_base);
-  for (int b = b0; b <= b1; b++) s.mat_pres[row + (uint64_t)b * cv.n_cmds] = 1u;
+  const bool nonsolid = f.styles[cmd.style].kind != kFetchSolid;
+  for (int b = b0; b <= b1; b++) {
+    s.mat_pres[row + (uint64_t)b * cv.n_cmds] = 1u;
+    if (nonsolid) s.band_nonsolid[cv.band_base + b] = 1u;
+  }
 }
 
 void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
   if (f.n_mat) hipLaunchKernelGGL(k_presence, dim3((unsigned)((f.n_mat + 255) / 256)), dim3(256), 0, st, s, f.n_mat);
+  if (f.n_bands) launch_fill_u32(s.band_nonsolid, 0u, f.n_bands, st);
   if (f.n_cmds) hipLaunchKernelGGL(k_box_presence, dim3((f.n_cmds + 255) / 256), dim3(256), 0, st, f, s);
 }
 
@@ -653,13 +660,14 @@ __device__ __forceinline__ void box_masks4(const DevCmd& c, int y, int px, uint3
 
 // Fetch and composite of pixels px .. px + 3 under masks m; returns how many
 // were composited.
+template<bool kSolid = false>
 __device__ __forceinline__ uint32_t composite4(const FrameDev& f, const DevCmd& c, const DevStyle& st, const pipe::Row& rs,
                                                int px, int w, bool a8, const uint32_t m[4], uint32_t d[4]) {
   uint32_t n = 0;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     if (!m[k] || px + k >= w) continue;
-    const uint32_t src = pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
+    const uint32_t src = kSolid ? st.solid : pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
     d[k] = a8 ? pipe::composite8(c.op, d[k], src >> 24, m[k]) : pipe::composite32(c.op, d[k], src, m[k]);
     n++;
   }
@@ -943,15 +951,18 @@ __device__ __forceinline__ uint32_t band_canvas(const FrameDev& f, uint32_t b) {
 }
 
 // Per band: analytic fills present (a non-empty binned range) -> its tiles
-// go to k_render; otherwise its tile groups go to k_render_box.
+// go to k_render; otherwise its tile groups go to k_render_box, the solid or
+// the general instance.
 __global__ void k_band_class(FrameDev f, FrameScratch s) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= f.n_bands) return;
   const DevCanvas& cv = f.canvases[band_canvas(f, b)];
   const uint64_t row_base = cv.mat_base + (uint64_t)(b - cv.band_base) * cv.n_cmds;
   const bool analytic = s.mat_off[row_base + cv.n_cmds] != s.mat_off[row_base];
+  const bool solid = !s.band_nonsolid[b];
   s.band_tiles[b] = analytic ? cv.tiles_x : 0u;
-  s.band_groups[b] = analytic ? 0u : cv.groups_x;
+  s.band_sgroups[b] = !analytic && solid ? cv.groups_x : 0u;
+  s.band_ggroups[b] = !analytic && !solid ? cv.groups_x : 0u;
 }
 
 // Compacted headers: {canvas, the band's command list range, column | band
@@ -990,7 +1001,10 @@ __global__ void k_group_setup(FrameDev f, FrameScratch s) {
   const uint32_t local = i - cv.group_base;
   const uint32_t band = local / cv.groups_x, col = local % cv.groups_x;
   const uint32_t gb = cv.band_base + band;
-  if (!s.band_groups[gb]) return;
+  uint32_t slot;
+  if (s.band_sgroups[gb]) slot = s.sgroup_off[gb] + col;
+  else if (s.band_ggroups[gb]) slot = s.sgroup_off[f.n_bands] + s.ggroup_off[gb] + col;
+  else return;
   const uint64_t row_base = cv.mat_base + (uint64_t)band * cv.n_cmds;
   const uint32_t tc0 = col * kGroupTiles;
   const uint32_t ntiles = cv.tiles_x - tc0 < (uint32_t)kGroupTiles ? cv.tiles_x - tc0 : (uint32_t)kGroupTiles;
@@ -998,7 +1012,7 @@ __global__ void k_group_setup(FrameDev f, FrameScratch s) {
   const uint32_t rows = cv.h - y0 < (uint32_t)kTileH ? cv.h - y0 : (uint32_t)kTileH;
   const uint32_t a8 = cv.format == kFmtA8 ? 1u : 0u;
   const uint64_t ptr = cv.ptr + (uint64_t)((int64_t)y0 * cv.stride) + (uint64_t)x0 * (a8 ? 1u : 4u);
-  uint4* h = s.group_hdr + 2 * (size_t)(s.group_off[gb] + col);
+  uint4* h = s.group_hdr + 2 * (size_t)slot;
   h[0] = make_uint4(cv.tile_base + band * cv.tiles_x + tc0, s.list_off[row_base], s.list_off[row_base + cv.n_cmds],
                     col | (band << 16));
   h[1] = make_uint4((uint32_t)ptr, (uint32_t)(ptr >> 32), (uint32_t)cv.stride,
@@ -1009,11 +1023,14 @@ __global__ void k_group_setup(FrameDev f, FrameScratch s) {
 // band (256 x 16 pixels, 16 pixels per thread).  No coverage
 // cells are needed, so the workgroup is light, and one walk of the band's
 // command list and records serves four tiles.
-__global__ __launch_bounds__(kTileThreads) void k_render_box(FrameDev f, FrameScratch s, uint32_t n) {
+template<bool kSolid>
+__device__ __forceinline__ void render_box(const FrameDev& f, const FrameScratch& s, uint32_t n) {
   const uint32_t slot = xcd_tile(blockIdx.x, n);
-  if (slot >= n || slot >= s.group_off[f.n_bands]) return;   // uniform, as in k_render
-  const uint4 gh = s.group_hdr[2 * (size_t)slot];
-  const uint4 gd = s.group_hdr[2 * (size_t)slot + 1];
+  // uniform, as in k_render; the solid bands' groups come first
+  if (slot >= n || slot >= (kSolid ? s.sgroup_off : s.ggroup_off)[f.n_bands]) return;
+  const size_t hi = 2 * ((size_t)slot + (kSolid ? 0u : s.sgroup_off[f.n_bands]));
+  const uint4 gh = s.group_hdr[hi];
+  const uint4 gd = s.group_hdr[hi + 1];
   const int t = threadIdx.x;
   const uint32_t lb = gh.y, le = gh.z;
   const uint32_t band = gh.w >> 16;
@@ -1111,7 +1128,7 @@ __global__ __launch_bounds__(kTileThreads) void k_render_box(FrameDev f, FrameSc
         if (y < c.box[1] || y >= c.box[3] || px0 + 16 <= c.box[0] || px0 >= c.box[2]) continue;
         touched = true;
         const DevStyle& st = rec[cur][b].r.style;
-        const pipe::Row rs = pipe::row_setup(st, y, (int)c.fetch_row0, 0u);
+        const pipe::Row rs = kSolid ? pipe::Row{} : pipe::row_setup(st, y, (int)c.fetch_row0, 0u);
         // One composite body for the four pixel quads: quad g is always in
         // d[0], the quads rotate through it.
 #pragma unroll 1
@@ -1119,7 +1136,7 @@ __global__ __launch_bounds__(kTileThreads) void k_render_box(FrameDev f, FrameSc
           uint32_t m[4];
           const int px = px0 + 4 * g;
           box_masks4(c, y, px, m);
-          npix += composite4(f, c, st, rs, px, w, a8, m, d[0]);
+          npix += composite4<kSolid>(f, c, st, rs, px, w, a8, m, d[0]);
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             const uint32_t r0 = d[0][k];
@@ -1171,11 +1188,22 @@ __global__ __launch_bounds__(256) void k_sum_u64(const unsigned long long* v, ui
   if (threadIdx.x == 0 && part[0]) atomicAdd(out, part[0]);
 }
 
+// Solid styles need few registers (about 70: seven workgroups per CU, where
+// the general instance fits four; capping it at 64 spills and runs slower).
+__global__ __launch_bounds__(kTileThreads) void k_render_box_solid(FrameDev f, FrameScratch s, uint32_t n) {
+  render_box<true>(f, s, n);
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_render_box(FrameDev f, FrameScratch s, uint32_t n) {
+  render_box<false>(f, s, n);
+}
+
 void launch_band_class(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
   if (!f.n_tiles) return;
   hipLaunchKernelGGL(k_band_class, dim3((f.n_bands + 255) / 256), dim3(256), 0, st, f, s);
   scan_exclusive(s.band_tiles, s.tile_off, f.n_bands, s.scan_tmp, st);
-  scan_exclusive(s.band_groups, s.group_off, f.n_bands, s.scan_tmp, st);
+  scan_exclusive(s.band_sgroups, s.sgroup_off, f.n_bands, s.scan_tmp, st);
+  scan_exclusive(s.band_ggroups, s.ggroup_off, f.n_bands, s.scan_tmp, st);
 }
 
 void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
@@ -1184,9 +1212,10 @@ void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t s
   hipLaunchKernelGGL(k_group_setup, dim3((f.n_groups + 255) / 256), dim3(256), 0, st, f, s);
 }
 
-void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint32_t n_g, hipStream_t st) {
+void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint32_t n_s, uint32_t n_g, hipStream_t st) {
   if (!f.n_tiles) return;
   if (n_a) hipLaunchKernelGGL(k_render, dim3((n_a + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_a);
+  if (n_s) hipLaunchKernelGGL(k_render_box_solid, dim3((n_s + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_s);
   if (n_g) hipLaunchKernelGGL(k_render_box, dim3((n_g + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_g);
 }
 
