@@ -11,8 +11,10 @@ the timed region.
 value      Mpix/s = pixels composited with a non-zero mask per frame x K /
            wall time of the K steps (host clock, device synchronized on both
            sides).  paths/s and canvas Mpix/s are reported beside it.
-roofline   the render kernel (k_render), timed with HIP events on the stream
-           it runs on, against the HBM roofline: algorithmic bytes per launch
+roofline   the render stage (k_render for bands with analytic fills,
+           k_render_box_solid / k_render_box for box-only bands), timed with HIP
+           events on the stream it runs on, against the HBM roofline: algorithmic
+           bytes per launch
            = 8 B per composited PRGB32 pixel (2 B for A8: destination read +
            write) + gradient LUT bytes + pattern image bytes + 16 B per binned
            edge segment it reads.
@@ -226,6 +228,24 @@ def _render_bytes(st, scenes):
     return per_px * st["pixels"] + st["lut_bytes"] + st["image_bytes"] + 16 * st["binned"]
 
 
+def _render_kernels(mask: int) -> str:
+    names = [n for bit, n in ((1, "k_render"), (2, "k_render_box_solid"), (4, "k_render_box")) if mask & bit]
+    return "+".join(names) if names else "none"
+
+
+def _traffic(workload: str):
+    """HBM bytes per render launch of this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic.json), each
+    counter scaled by what it reads for a plain copy of known size; None when
+    the workload was not profiled."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as fh:
+            rec = json.load(fh).get(workload)
+    except (OSError, ValueError):
+        return None
+    return int(rec["traffic_bytes"]) if rec else None
+
+
 def _cpu_baseline(scenes, first_gpu, pixels_per_frame, budget_s: float):
     """Reference library on this host's cores (bounded sample), plus the
     bit-exactness check of the GPU's first frame against its output."""
@@ -369,12 +389,12 @@ def main():
         "sizing": sizing,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_render",
+            "kernel": _render_kernels(st.get("render_kernels", 1)),
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": _traffic(args.workload),
             "bytes_per_launch": int(bytes_render),
         },
         "cpu_baseline": None,

@@ -60,7 +60,7 @@ class BLAmdStats(ctypes.Structure):
                 ("frame_bytes", ctypes.c_uint64), ("lut_bytes", ctypes.c_uint64), ("image_bytes", ctypes.c_uint64),
                 ("pred_segs", ctypes.c_uint64), ("pred_bins", ctypes.c_uint64),
                 ("sizing", ctypes.c_uint32), ("fallback", ctypes.c_uint32), ("syncs", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32),
+                ("render_kernels", ctypes.c_uint32),
                 ("render_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
 
     def as_dict(self):

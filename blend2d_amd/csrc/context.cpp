@@ -92,6 +92,7 @@ struct Stats {
   uint64_t frame_bytes = 0, lut_bytes = 0, image_bytes = 0;
   uint64_t pred_segs = 0, pred_bins = 0;
   uint32_t sizing = 0, fallback = 0, syncs = 0;
+  uint32_t render_kernels = 0;   // bit 0 k_render, 1 k_render_box_solid, 2 k_render_box
   double render_ms = 0.0, total_ms = 0.0;
 };
 
@@ -405,6 +406,7 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
   launch_render(f, s, f.n_tiles, f.n_groups, f.n_groups, stream);
+  stats.render_kernels = f.n_tiles ? 7u : 0u;
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback, counters.ptr, 16, hipMemcpyDeviceToHost, stream));
@@ -537,6 +539,7 @@ uint32_t Runner::execute_exact(Stats& stats) {
   launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
   launch_render(f, s, n_atiles, n_sgroups, n_ggroups, stream);
+  stats.render_kernels = (n_atiles ? 1u : 0u) | (n_sgroups ? 2u : 0u) | (n_ggroups ? 4u : 0u);
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback + 2, s.pixels, 8, hipMemcpyDeviceToHost, stream));
@@ -1556,6 +1559,7 @@ static void copy_stats(const Stats& st, BLAmdStats* out) {
   out->sizing = st.sizing;
   out->fallback = st.fallback;
   out->syncs = st.syncs;
+  out->render_kernels = st.render_kernels;
   out->render_ms = st.render_ms;
   out->total_ms = st.total_ms;
 }

@@ -182,8 +182,9 @@ typedef struct BLAmdStats {
   uint32_t sizing;       /* BL_AMD_SIZING_* the frame used */
   uint32_t fallback;     /* 1: a pool overflowed and the frame was redone with exact counting */
   uint32_t syncs;        /* host round trips of the frame */
-  uint32_t reserved;
-  double render_ms;    /* device time of the render kernel */
+  uint32_t render_kernels; /* render kernels the frame launched: 1 k_render (bands with analytic
+                              fills), 2 k_render_box_solid (solid box fills only), 4 k_render_box */
+  double render_ms;    /* device time of the render stage (k_render, k_render_box_solid, k_render_box) */
   double total_ms;     /* host wall time of the device stages */
 } BLAmdStats;
 
