@@ -161,6 +161,7 @@ def _batched_vs_plain(seed: int, repeats: int, learned: bool):
         ctx = B.BLContext()
         el = 0.0
         px = 0
+        up = dev = 0.0
         for sc, img in zip(scenes, targets):
             img.fill_raw(sc.clear_value)
             ctx.begin(img)
@@ -169,8 +170,13 @@ def _batched_vs_plain(seed: int, repeats: int, learned: bool):
             t0 = time.perf_counter()
             ctx.flush()
             el += time.perf_counter() - t0
-            px += ctx.last_stats()["pixels"]
+            st = ctx.last_stats()
+            px += st["pixels"]
+            up += st.get("upload_ms", 0.0)
+            dev += st.get("total_ms", 0.0)
             ctx.end()
+        if best is None or el < best:
+            out["plain_flush_breakdown_ms"] = {"upload": round(up, 3), "device_stages": round(dev, 3)}
         best = el if best is None else min(best, el)
     out["plain_ms_per_frame"] = round(best * 1e3, 3)
     out["plain_mpix_per_s"] = round(px / best / 1e6, 3)
@@ -197,6 +203,9 @@ def _batched_vs_plain(seed: int, repeats: int, learned: bool):
     out["batched_mpix_per_s"] = round(st["pixels"] / best / 1e6, 3)
     out["batched_paths_per_s"] = round(out["paths"] / best, 1)
     out["batched_fallback"] = st.get("fallback")
+    out["batched_flush_breakdown_ms"] = {"merge": round(st.get("merge_ms", 0.0), 3), "upload": round(st.get("upload_ms", 0.0), 3),
+                                         "device_stages": round(st.get("total_ms", 0.0), 3),
+                                         "render": round(st.get("render_ms", 0.0), 3)}
     el, _ = _time_frame(_BatchHandle(batch), 3, 1)
     out["batched_replay_ms_per_frame"] = round(el * 1e3 / 3, 3)
     for c in ctxs:

@@ -61,7 +61,8 @@ class BLAmdStats(ctypes.Structure):
                 ("pred_segs", ctypes.c_uint64), ("pred_bins", ctypes.c_uint64),
                 ("sizing", ctypes.c_uint32), ("fallback", ctypes.c_uint32), ("syncs", ctypes.c_uint32),
                 ("render_kernels", ctypes.c_uint32),
-                ("render_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+                ("render_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("merge_ms", ctypes.c_double), ("upload_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

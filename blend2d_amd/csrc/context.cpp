@@ -11,6 +11,7 @@
 #include <chrono>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <new>
 #include <vector>
 
@@ -94,6 +95,7 @@ struct Stats {
   uint32_t sizing = 0, fallback = 0, syncs = 0;
   uint32_t render_kernels = 0;   // bit 0 k_render, 1 k_render_box_solid, 2 k_render_box
   double render_ms = 0.0, total_ms = 0.0;
+  double merge_ms = 0.0, upload_ms = 0.0;
 };
 
 // How a frame sizes its segment and bin buffers.
@@ -118,7 +120,18 @@ struct DeviceQueue {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint32_t* host_readback = nullptr;
+  // Pinned staging memory of frame uploads (grown on demand, kept).
+  uint8_t* staging = nullptr;
+  size_t staging_cap = 0;
   bool ready = false;
+  bool staging_reserve(size_t bytes) {
+    if (bytes <= staging_cap) return true;
+    if (staging) { (void)hipHostFree(staging); staging = nullptr; staging_cap = 0; }
+    const size_t n = std::max<size_t>(bytes + bytes / 4, (size_t)1 << 20);
+    if (hipHostMalloc((void**)&staging, n, hipHostMallocDefault) != hipSuccess) { staging = nullptr; return false; }
+    staging_cap = n;
+    return true;
+  }
   bool init() {
     if (ready) return true;
     if (!stream && hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) { stream = nullptr; return false; }
@@ -173,17 +186,25 @@ struct Runner {
     return true;
   }
 
-  uint32_t upload(FrameHost& fh);
+  uint32_t upload(const std::vector<FrameHost*>& parts);
   uint32_t execute(Stats& stats);
   uint32_t execute_exact(Stats& stats);
   // 0 done, 1 pool overflow (redo exactly), otherwise an error
   uint32_t execute_pool(Stats& stats, uint32_t mode);
   bool model_views(cm::FrameModel& m);
-  uint32_t run(FrameHost& fh, Stats& stats) {
+  // `parts`: the recorded frames of one context, or of every context of a
+  // batch (one canvas each), packed into one device frame.
+  uint32_t run(const std::vector<FrameHost*>& parts, Stats& stats, double merge_ms = 0.0) {
     std::lock_guard<std::mutex> lock(device_queue().mu);
     stats = Stats{};
-    uint32_t r = upload(fh);
-    return r ? r : execute(stats);
+    const auto t0 = std::chrono::steady_clock::now();
+    uint32_t r = upload(parts);
+    const double up = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (r) return r;
+    r = execute(stats);
+    stats.merge_ms = merge_ms;
+    stats.upload_ms = up;
+    return r;
   }
   uint32_t replay(Stats& stats) {
     std::lock_guard<std::mutex> lock(device_queue().mu);
@@ -191,24 +212,82 @@ struct Runner {
   }
 };
 
-template<typename T>
-static size_t append_bytes(std::vector<uint8_t>& blob, const std::vector<T>& v) {
-  size_t off = (blob.size() + 255) & ~(size_t)255;
-  blob.resize(off + v.size() * sizeof(T));
-  if (!v.empty()) memcpy(blob.data() + off, v.data(), v.size() * sizeof(T));
-  return off;
+// Section sizes of a packed frame: per part (context) and in total.
+struct PartBase {
+  size_t cmds, xforms, src, vtx, styles, luts, images;
+};
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Copies part `k` of a frame into the packed staging layout, rebasing its
+// indices (canvas, command, vertex, style, LUT and image bases).
+static void pack_part(uint8_t* st, const FrameHost& fh, uint32_t k, const PartBase& b, const size_t off[8]) {
+  DevCmd* cmds = (DevCmd*)(st + off[1]) + b.cmds;
+  for (size_t i = 0; i < fh.cmds.size(); i++) {
+    DevCmd c = fh.cmds[i];
+    c.style += (uint32_t)b.styles;
+    c.canvas = k;
+    cmds[i] = c;
+  }
+  if (!fh.xforms.empty()) memcpy((CmdXform*)(st + off[2]) + b.xforms, fh.xforms.data(), fh.xforms.size() * sizeof(CmdXform));
+  SrcSeg* src = (SrcSeg*)(st + off[3]) + b.src;
+  const uint32_t vb = (uint32_t)(b.vtx / 2);
+  for (size_t i = 0; i < fh.src.size(); i++) {
+    SrcSeg g = fh.src[i];
+    g.cmd += (uint32_t)b.cmds;
+    g.i0 += vb;
+    g.i1 += vb;
+    src[i] = g;
+  }
+  if (!fh.vtx.empty()) memcpy((double*)(st + off[4]) + b.vtx, fh.vtx.data(), fh.vtx.size() * sizeof(double));
+  DevStyle* styles = (DevStyle*)(st + off[5]) + b.styles;
+  for (size_t i = 0; i < fh.styles.size(); i++) {
+    DevStyle d = fh.styles[i];
+    if (d.kind != kFetchSolid) d.lut += (uint32_t)b.luts;
+    d.img += (uint32_t)b.images;
+    styles[i] = d;
+  }
+  if (!fh.luts.empty()) memcpy((uint32_t*)(st + off[6]) + b.luts, fh.luts.data(), fh.luts.size() * 4);
+  if (!fh.images.empty()) memcpy((DevImage*)(st + off[7]) + b.images, fh.images.data(), fh.images.size() * sizeof(DevImage));
 }
 
-// Lays out tiles, bands and the (band, command) matrix of every canvas and
-// copies the frame's inputs to the device in one transfer.
-uint32_t Runner::upload(FrameHost& fh) {
+// Lays out tiles, bands and the (band, command) matrix of every canvas, packs
+// the parts into pinned staging memory (several host threads for a large
+// batch) and copies the frame to the device in one transfer.
+uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   resident = false;
-  if (fh.cmds.empty()) return BL_AMD_SUCCESS;
+  size_t n_cmds_total = 0;
+  for (const FrameHost* p : parts) n_cmds_total += p->cmds.size();
+  if (n_cmds_total == 0) return BL_AMD_SUCCESS;
   if (!init()) return kErrDevice;
+
+  // canvases (one per part) and the per-part section bases
+  std::vector<DevCanvas> canvases;
+  canvases.reserve(parts.size());
+  std::vector<PartBase> bases(parts.size());
+  PartBase tot{0, 0, 0, 0, 0, 0, 0};
+  uint64_t n_src_total = 0;
+  n_src_lines = 0;
+  for (size_t k = 0; k < parts.size(); k++) {
+    const FrameHost& fh = *parts[k];
+    bases[k] = tot;
+    DevCanvas cv = fh.canvases[0];
+    cv.cmd_base = (uint32_t)tot.cmds;
+    canvases.push_back(cv);
+    tot.cmds += fh.cmds.size();
+    tot.xforms += fh.xforms.size();
+    tot.src += fh.src.size();
+    tot.vtx += fh.vtx.size();
+    tot.styles += fh.styles.size();
+    tot.luts += fh.luts.size();
+    tot.images += fh.images.size();
+    n_src_total += fh.src.size();
+    for (const SrcSeg& sg : fh.src) n_src_lines += sg.kind == kSrcLine ? 1u : 0u;
+  }
 
   uint64_t n_mat = 0;
   uint32_t n_tiles = 0, n_groups = 0, band_base = 0;
-  for (DevCanvas& c : fh.canvases) {
+  for (DevCanvas& c : canvases) {
     c.tiles_x = (c.w + kTileW - 1) / kTileW;
     c.bands = (c.h + kTileH - 1) / kTileH;
     c.groups_x = (c.tiles_x + kGroupTiles - 1) / kGroupTiles;
@@ -222,46 +301,67 @@ uint32_t Runner::upload(FrameHost& fh) {
     n_groups += c.groups_x * c.bands;
     n_mat += (uint64_t)c.bands * c.n_cmds;
   }
-  if (n_mat >= 0xFFFFFFF0ull || fh.src.size() >= 0xFFFFFFF0ull) return kErrOOM;
+  if (n_mat >= 0xFFFFFFF0ull || n_src_total >= 0xFFFFFFF0ull || tot.vtx / 2 >= 0xFFFFFFF0ull ||
+      tot.styles >= 0xFFFFFFF0ull || tot.luts >= 0xFFFFFFF0ull)
+    return kErrOOM;
 
-  std::vector<uint8_t> blob;
-  size_t o_canvas = append_bytes(blob, fh.canvases);
-  size_t o_cmds = append_bytes(blob, fh.cmds);
-  size_t o_xf = append_bytes(blob, fh.xforms);
-  size_t o_src = append_bytes(blob, fh.src);
-  size_t o_vtx = append_bytes(blob, fh.vtx);
-  size_t o_styles = append_bytes(blob, fh.styles);
-  size_t o_luts = append_bytes(blob, fh.luts);
-  size_t o_images = append_bytes(blob, fh.images);
-  if (!in.reserve(blob.size() + 256)) return kErrOOM;
-  HIP_OK(hipMemcpyAsync(in.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, stream));
-  // The blob is host pageable memory that goes out of scope here.
+  // section offsets
+  size_t off[8];
+  size_t at = 0;
+  const size_t sizes[8] = {canvases.size() * sizeof(DevCanvas), tot.cmds * sizeof(DevCmd), tot.xforms * sizeof(CmdXform),
+                           tot.src * sizeof(SrcSeg), tot.vtx * sizeof(double), tot.styles * sizeof(DevStyle),
+                           tot.luts * 4, tot.images * sizeof(DevImage)};
+  for (int i = 0; i < 8; i++) { off[i] = at; at = align256(at + sizes[i]); }
+  const size_t blob_size = at;
+
+  DeviceQueue& q = device_queue();
+  if (!q.staging_reserve(blob_size)) return kErrOOM;
+  uint8_t* st = q.staging;
+  memcpy(st + off[0], canvases.data(), sizes[0]);
+  const unsigned hw = std::thread::hardware_concurrency();
+  const size_t n_threads = std::min<size_t>({parts.size(), (size_t)(hw ? hw : 1), (size_t)8,
+                                            blob_size / ((size_t)16 << 20) + 1});
+  if (n_threads <= 1) {
+    for (size_t k = 0; k < parts.size(); k++) pack_part(st, *parts[k], (uint32_t)k, bases[k], off);
+  }
+  else {
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < n_threads; t++) {
+      pool.emplace_back([&, t]() {
+        for (size_t k = t; k < parts.size(); k += n_threads) pack_part(st, *parts[k], (uint32_t)k, bases[k], off);
+      });
+    }
+    for (std::thread& th : pool) th.join();
+  }
+
+  if (!in.reserve(blob_size + 256)) return kErrOOM;
+  HIP_OK(hipMemcpyAsync(in.ptr, st, blob_size, hipMemcpyHostToDevice, stream));
+  // the staging memory is reused by the next upload
   HIP_OK(hipStreamSynchronize(stream));
 
   const uint8_t* base = in.as<uint8_t>();
   f = FrameDev{};
-  f.canvases = (const DevCanvas*)(base + o_canvas); f.n_canvases = (uint32_t)fh.canvases.size();
-  f.cmds = (const DevCmd*)(base + o_cmds); f.n_cmds = (uint32_t)fh.cmds.size();
-  f.xforms = (const CmdXform*)(base + o_xf);
-  f.src = (const SrcSeg*)(base + o_src); f.n_src = (uint32_t)fh.src.size();
-  f.vtx = (const double*)(base + o_vtx);
-  f.styles = (const DevStyle*)(base + o_styles);
-  f.luts = (const uint32_t*)(base + o_luts);
-  f.images = (const DevImage*)(base + o_images);
+  f.canvases = (const DevCanvas*)(base + off[0]); f.n_canvases = (uint32_t)canvases.size();
+  f.cmds = (const DevCmd*)(base + off[1]); f.n_cmds = (uint32_t)tot.cmds;
+  f.xforms = (const CmdXform*)(base + off[2]);
+  f.src = (const SrcSeg*)(base + off[3]); f.n_src = (uint32_t)tot.src;
+  f.vtx = (const double*)(base + off[4]);
+  f.styles = (const DevStyle*)(base + off[5]);
+  f.luts = (const uint32_t*)(base + off[6]);
+  f.images = (const DevImage*)(base + off[7]);
   f.n_tiles = n_tiles;
   f.n_groups = n_groups;
   f.n_bands = band_base;
   f.n_mat = n_mat;
   f.tol_sq = kToleranceSq;
-  n_src = fh.src.size();
+  n_src = n_src_total;
   bands_max = 0;
-  for (const DevCanvas& c : fh.canvases) bands_max = std::max(bands_max, c.bands);
-  n_src_lines = 0;
-  for (const SrcSeg& sg : fh.src) n_src_lines += sg.kind == kSrcLine ? 1u : 0u;
-  frame_bytes = blob.size();
-  lut_bytes = fh.luts.size() * 4;
+  for (const DevCanvas& c : canvases) bands_max = std::max(bands_max, c.bands);
+  frame_bytes = blob_size;
+  lut_bytes = tot.luts * 4;
   image_bytes = 0;
-  for (const DevImage& im : fh.images) image_bytes += (uint64_t)im.w * im.h * (im.format == kFmtA8 ? 1u : 4u);
+  for (const FrameHost* p : parts)
+    for (const DevImage& im : p->images) image_bytes += (uint64_t)im.w * im.h * (im.format == kFmtA8 ? 1u : 4u);
   resident = true;
   return BL_AMD_SUCCESS;
 }
@@ -489,11 +589,15 @@ uint32_t Runner::execute_exact(Stats& stats) {
   launch_init_bounds(s.bounds, f.n_cmds, stream);
   launch_edges(f, s, true, stream);
   launch_cmd_tiles(f, s, stream);
-  scan_exclusive(s.src_count, s.src_off, n_src, s.scan_tmp, stream);
-  HIP_OK(hipMemcpyAsync(host_readback, s.src_off + n_src, 4, hipMemcpyDeviceToHost, stream));
-  HIP_OK(hipStreamSynchronize(stream));
-  stats.syncs++;
-  const uint32_t n_segs = host_readback[0];
+  // A frame of box fills only has no edges: no counting round trip.
+  uint32_t n_segs = 0;
+  if (n_src) {
+    scan_exclusive(s.src_count, s.src_off, n_src, s.scan_tmp, stream);
+    HIP_OK(hipMemcpyAsync(host_readback, s.src_off + n_src, 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    stats.syncs++;
+    n_segs = host_readback[0];
+  }
   if (n_segs >= 0x80000000u) return kErrDevice;
   if (!segs.reserve((size_t)std::max<uint32_t>(n_segs, 1) * sizeof(DevSeg)) || !seg_cmd.reserve((size_t)std::max<uint32_t>(n_segs, 1) * 4))
     return kErrOOM;
@@ -1060,7 +1164,7 @@ void reset_frame(BLAmdContext* c) {
 
 uint32_t flush_one(BLAmdContext* c) {
   finalize_frame(c);
-  uint32_t r = c->runner.run(c->frame, c->stats);
+  uint32_t r = c->runner.run(std::vector<FrameHost*>{&c->frame}, c->stats);
   reset_frame(c);
   return r;
 }
@@ -1560,6 +1664,8 @@ static void copy_stats(const Stats& st, BLAmdStats* out) {
   out->fallback = st.fallback;
   out->syncs = st.syncs;
   out->render_kernels = st.render_kernels;
+  out->merge_ms = st.merge_ms;
+  out->upload_ms = st.upload_ms;
   out->render_ms = st.render_ms;
   out->total_ms = st.total_ms;
 }
@@ -1602,36 +1708,16 @@ BLAmdResult bl_amd_batch_clear(BLAmdBatch* b) {
 // rebased) and renders them with one pass of each kernel stage.
 BLAmdResult bl_amd_batch_flush(BLAmdBatch* b) {
   if (!b) return kErrInvalid;
-  FrameHost all;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<FrameHost*> parts;
+  parts.reserve(b->contexts.size());
   for (BLAmdContext* c : b->contexts) {
     if (!c->target) return kErrState;
     finalize_frame(c);
-    const uint32_t cmd_base = (uint32_t)all.cmds.size();
-    const uint32_t vtx_base = (uint32_t)(all.vtx.size() / 2);
-    const uint32_t style_base = (uint32_t)all.styles.size();
-    const uint32_t lut_base = (uint32_t)all.luts.size();
-    const uint32_t img_base = (uint32_t)all.images.size();
-    const uint32_t canvas_idx = (uint32_t)all.canvases.size();
-    DevCanvas cv = c->frame.canvases[0];
-    cv.cmd_base = cmd_base;
-    all.canvases.push_back(cv);
-    for (DevCmd cmd : c->frame.cmds) {
-      cmd.style += style_base;
-      cmd.canvas = canvas_idx;
-      all.cmds.push_back(cmd);
-    }
-    all.xforms.insert(all.xforms.end(), c->frame.xforms.begin(), c->frame.xforms.end());
-    for (SrcSeg s : c->frame.src) { s.cmd += cmd_base; s.i0 += vtx_base; s.i1 += vtx_base; all.src.push_back(s); }
-    all.vtx.insert(all.vtx.end(), c->frame.vtx.begin(), c->frame.vtx.end());
-    for (DevStyle st : c->frame.styles) {
-      if (st.kind != kFetchSolid) st.lut += lut_base;
-      st.img += img_base;
-      all.styles.push_back(st);
-    }
-    all.luts.insert(all.luts.end(), c->frame.luts.begin(), c->frame.luts.end());
-    all.images.insert(all.images.end(), c->frame.images.begin(), c->frame.images.end());
+    parts.push_back(&c->frame);
   }
-  uint32_t r = b->runner.run(all, b->stats);
+  const double merge_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  uint32_t r = b->runner.run(parts, b->stats, merge_ms);
   for (BLAmdContext* c : b->contexts) reset_frame(c);
   return r;
 }

@@ -345,16 +345,18 @@ __global__ void k_presence(FrameScratch s, uint64_t n) {
 
 // Box fills are present in every band their box touches; a box fill with a
 // non-solid style also marks its bands (they take the general compositor).
+// 16 threads per command, each taking every 16th band of the box.
 __global__ void k_box_presence(FrameDev f, FrameScratch s) {
-  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = i >> 4;
   if (c >= f.n_cmds) return;
   const DevCmd& cmd = f.cmds[c];
   if (cmd.kind == kCmdAnalytic) return;
   const DevCanvas& cv = f.canvases[cmd.canvas];
-  int b0 = cmd.box[1] / kTileH, b1 = (cmd.box[3] - 1) / kTileH;
-  uint64_t row = cv.mat_base + (uint64_t)(c - cv.cmd_base);
+  const int b0 = cmd.box[1] / kTileH, b1 = (cmd.box[3] - 1) / kTileH;
+  const uint64_t row = cv.mat_base + (uint64_t)(c - cv.cmd_base);
   const bool nonsolid = f.styles[cmd.style].kind != kFetchSolid;
-  for (int b = b0; b <= b1; b++) {
+  for (int b = b0 + (int)(i & 15u); b <= b1; b += 16) {
     s.mat_pres[row + (uint64_t)b * cv.n_cmds] = 1u;
     if (nonsolid) s.band_nonsolid[cv.band_base + b] = 1u;
   }
@@ -363,7 +365,7 @@ __global__ void k_box_presence(FrameDev f, FrameScratch s) {
 void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
   if (f.n_mat) hipLaunchKernelGGL(k_presence, dim3((unsigned)((f.n_mat + 255) / 256)), dim3(256), 0, st, s, f.n_mat);
   if (f.n_bands) launch_fill_u32(s.band_nonsolid, 0u, f.n_bands, st);
-  if (f.n_cmds) hipLaunchKernelGGL(k_box_presence, dim3((f.n_cmds + 255) / 256), dim3(256), 0, st, f, s);
+  if (f.n_cmds) hipLaunchKernelGGL(k_box_presence, dim3((unsigned)(((uint64_t)f.n_cmds * 16 + 255) / 256)), dim3(256), 0, st, f, s);
 }
 
 __global__ void k_bin_scatter(FrameDev f, FrameScratch s, uint32_t n, uint32_t* cursor) {
@@ -546,10 +548,17 @@ __device__ __forceinline__ uint32_t batch_find(const uint32_t end[kBatch], uint3
   return b;
 }
 
+// Tile rows one raster task covers (a task pays the rasterizer's set-up and
+// advance once for its rows).
+#ifndef B2AMD_ROWS_PER_TASK
+#define B2AMD_ROWS_PER_TASK 1
+#endif
+constexpr int kRowsPerTask = B2AMD_ROWS_PER_TASK;
+
 // First pass over one binned segment of an analytic fill: a segment right of
 // the tile is dropped; one left of it adds its signed vertical extent per
 // covered row to the backdrop; for one that reaches the tile the number of
-// tile rows it covers is returned (rasterized by raster_row, one row each).
+// raster tasks for the tile rows it covers is returned (raster_rows).
 __device__ __forceinline__ uint32_t seg_tile_rows(const DevSeg& g, uint32_t* cells, int X0, int Y0) {
   const uint32_t sign = (uint32_t)g.x1 >> 31;
   const int32_t x1 = g.x1 & 0x7FFFFFFF;
@@ -569,19 +578,21 @@ __device__ __forceinline__ uint32_t seg_tile_rows(const DevSeg& g, uint32_t* cel
     }
     return 0u;
   }
-  return (uint32_t)(r1 - r0 + 1);
+  return (uint32_t)((r1 - r0 + kRowsPerTask) / kRowsPerTask);
 }
 
-// Row `row` of a segment that reaches the tile.  Advancing the rasterizer to
-// any row and rasterizing that row alone gives the cells the full walk gives
-// for it (the rasterizer's advance invariant, band height 1).
-__device__ __forceinline__ void raster_row(const DevSeg& g, uint32_t* cells, int X0, int Y0, int row) {
+// Rows `row` .. `row` + kRowsPerTask - 1 (within the tile) of a segment that
+// reaches the tile.  Advancing the rasterizer to any row and rasterizing
+// from there gives the cells the full walk gives for those rows (the
+// rasterizer's advance invariant, for any band height).
+__device__ __forceinline__ void raster_rows(const DevSeg& g, uint32_t* cells, int X0, int Y0, int row) {
   TileSink sink{cells, X0, Y0};
   ras::Rasterizer rz;
   rz.sign_mask = 0u - ((uint32_t)g.x1 >> 31);
   if (!rz.prepare(g.x0, g.y0, g.x1 & 0x7FFFFFFF, g.y1)) return;
   if (rz.st.ey0 < row) rz.advance_to_y(row);
-  rz.band_end = row;
+  const int last = row + kRowsPerTask - 1;
+  rz.band_end = last < Y0 + kTileH - 1 ? last : Y0 + kTileH - 1;
   rz.rasterize(sink);
 }
 
@@ -875,7 +886,7 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
           const uint32_t ts = task_start[o];
           const DevSeg og = task_seg[o];
           const int r0 = (og.y0 >> 8) > Y0 ? (og.y0 >> 8) : Y0;
-          raster_row(og, cells[ts >> 12], X0, Y0, r0 + (int)(k - (ts & 0xFFFu)));
+          raster_rows(og, cells[ts >> 12], X0, Y0, r0 + kRowsPerTask * (int)(k - (ts & 0xFFFu)));
         }
         __syncthreads();   // cells complete (and task arrays free for the next round)
       }

@@ -186,6 +186,8 @@ typedef struct BLAmdStats {
                               fills), 2 k_render_box_solid (solid box fills only), 4 k_render_box */
   double render_ms;    /* device time of the render stage (k_render, k_render_box_solid, k_render_box) */
   double total_ms;     /* host wall time of the device stages */
+  double merge_ms;     /* flush: host time merging the batch's recorded frames (0 for one context) */
+  double upload_ms;    /* flush: host time packing and uploading the frame (0 for a replay) */
 } BLAmdStats;
 
 BLAmdResult bl_amd_context_get_stats(const BLAmdContext* ctx, BLAmdStats* out);
