@@ -685,6 +685,30 @@ __device__ __forceinline__ uint32_t composite4(const FrameDev& f, const DevCmd& 
   return n;
 }
 
+// The same with all four fetches issued before the first is used, so their
+// gradient-LUT / texel loads are in flight together (more registers).
+__device__ __forceinline__ uint32_t composite4_overlapped(const FrameDev& f, const DevCmd& c, const DevStyle& st,
+                                                          const pipe::Row& rs, int px, int w, bool a8,
+                                                          const uint32_t m[4], uint32_t d[4]) {
+  constexpr bool kSolid = false;
+  bool on[4];
+  uint32_t src[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    on[k] = m[k] && px + k < w;
+    src[k] = 0u;
+    if (on[k]) src[k] = kSolid ? st.solid : pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
+  }
+  uint32_t n = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (!on[k]) continue;
+    d[k] = a8 ? pipe::composite8(c.op, d[k], src[k] >> 24, m[k]) : pipe::composite32(c.op, d[k], src[k], m[k]);
+    n++;
+  }
+  return n;
+}
+
 // Workgroup -> tile.  Workgroups are dealt to the 8 XCDs round-robin; the
 // mapping gives each XCD a contiguous range of tiles, so the tiles of a band
 // (which read the same command lists, records and segments) share one L2.
@@ -920,7 +944,7 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
           const DevStyle& st = rec[cur][b].r.style;
           const int fill_row0 = (c.kind == kCmdAnalytic) ? (rec[cur][b].r.min_y >> 8) : (int)c.fetch_row0;
           const pipe::Row rs = pipe::row_setup(st, y, fill_row0, c.kind == kCmdAnalytic ? cv.ref_band_h : 0u);
-          npix += composite4(f, c, st, rs, px, (int)cv.w, a8, m, d);
+          npix += composite4_overlapped(f, c, st, rs, px, (int)cv.w, a8, m, d);
         }
       }
       if (has_next) {
@@ -1199,9 +1223,10 @@ __global__ __launch_bounds__(256) void k_sum_u64(const unsigned long long* v, ui
   if (threadIdx.x == 0 && part[0]) atomicAdd(out, part[0]);
 }
 
-// Solid styles need few registers (about 70: seven workgroups per CU, where
-// the general instance fits four; capping it at 64 spills and runs slower).
-__global__ __launch_bounds__(kTileThreads) void k_render_box_solid(FrameDev f, FrameScratch s, uint32_t n) {
+// Solid styles need few registers: seven workgroups per CU (72 VGPRs, no
+// spills), where the general instance fits four.  Capping it at 64 for eight
+// spills and runs slower.
+__global__ __launch_bounds__(kTileThreads, 7) void k_render_box_solid(FrameDev f, FrameScratch s, uint32_t n) {
   render_box<true>(f, s, n);
 }
 
