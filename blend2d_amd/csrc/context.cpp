@@ -156,7 +156,7 @@ struct Runner {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf in;
   DevBuf src_count, src_off, bounds, segs, seg_cmd, mat_cnt, mat_off, mat_pres, list_off, list, binned, cursor, scan_tmp, counters, cmd_tiles;
-  DevBuf tile_hdr, group_hdr, tile_px, list_style, band_buf;
+  DevBuf tile_hdr, group_hdr, tile_px, list_style, band_buf, item_cnt, item_off, items;
   DevBuf cm_params, cm_acc, cm_feat, cm_pred, cm_act;
   uint64_t cm_version = 0;
   uint32_t* host_readback = nullptr;
@@ -165,6 +165,7 @@ struct Runner {
   bool resident = false;
   FrameDev f{};
   uint64_t n_src = 0, n_src_lines = 0;
+  uint32_t item_cap = 0;   // upper bound of edge work items: lines + 3 per quad + 8 per cubic
   uint32_t bands_max = 0;
   uint32_t max_bands() const { return bands_max; }
   uint64_t frame_bytes = 0, lut_bytes = 0, image_bytes = 0;
@@ -268,6 +269,7 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   PartBase tot{0, 0, 0, 0, 0, 0, 0};
   uint64_t n_src_total = 0;
   n_src_lines = 0;
+  uint64_t n_items_max = 0;
   for (size_t k = 0; k < parts.size(); k++) {
     const FrameHost& fh = *parts[k];
     bases[k] = tot;
@@ -282,7 +284,10 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
     tot.luts += fh.luts.size();
     tot.images += fh.images.size();
     n_src_total += fh.src.size();
-    for (const SrcSeg& sg : fh.src) n_src_lines += sg.kind == kSrcLine ? 1u : 0u;
+    for (const SrcSeg& sg : fh.src) {
+      n_src_lines += sg.kind == kSrcLine ? 1u : 0u;
+      n_items_max += sg.kind == kSrcLine ? 1u : sg.kind == kSrcQuad ? 3u : 8u;
+    }
   }
 
   uint64_t n_mat = 0;
@@ -301,7 +306,7 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
     n_groups += c.groups_x * c.bands;
     n_mat += (uint64_t)c.bands * c.n_cmds;
   }
-  if (n_mat >= 0xFFFFFFF0ull || n_src_total >= 0xFFFFFFF0ull || tot.vtx / 2 >= 0xFFFFFFF0ull ||
+  if (n_mat >= 0xFFFFFFF0ull || n_src_total >= 0xFFFFFFF0ull || n_items_max >= 0xFFFFFFF0ull || tot.vtx / 2 >= 0xFFFFFFF0ull ||
       tot.styles >= 0xFFFFFFF0ull || tot.luts >= 0xFFFFFFF0ull)
     return kErrOOM;
 
@@ -355,6 +360,7 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   f.n_mat = n_mat;
   f.tol_sq = kToleranceSq;
   n_src = n_src_total;
+  item_cap = (uint32_t)n_items_max;
   bands_max = 0;
   for (const DevCanvas& c : canvases) bands_max = std::max(bands_max, c.bands);
   frame_bytes = blob_size;
@@ -544,8 +550,10 @@ uint32_t Runner::execute_exact(Stats& stats) {
   auto t0 = std::chrono::steady_clock::now();
   const uint64_t n_mat = f.n_mat;
 
-  size_t scan_words = scan_tmp_words(std::max<size_t>(std::max<size_t>(n_src, (size_t)n_mat), f.n_bands) + 1);
-  if (!src_count.reserve((n_src + 1) * 4) || !src_off.reserve((n_src + 1) * 4) ||
+  size_t scan_words = scan_tmp_words(std::max<size_t>({(size_t)n_src, (size_t)item_cap, (size_t)n_mat, (size_t)f.n_bands}) + 1);
+  if (!item_cnt.reserve((n_src + 1) * 4) || !item_off.reserve((n_src + 1) * 4) ||
+      !items.reserve(std::max<size_t>(item_cap, 1) * sizeof(EdgeItem)) ||
+      !src_count.reserve(((size_t)item_cap + 1) * 4) || !src_off.reserve(((size_t)item_cap + 1) * 4) ||
       !bounds.reserve((size_t)f.n_cmds * sizeof(CmdBounds)) ||
       !mat_cnt.reserve((n_mat + 1) * 4) || !mat_off.reserve((n_mat + 1) * 4) ||
       !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
@@ -567,6 +575,9 @@ uint32_t Runner::execute_exact(Stats& stats) {
   s.sgroup_off = s.tile_off + (f.n_bands + 1);
   s.ggroup_off = s.sgroup_off + (f.n_bands + 1);
   s.tile_px = tile_px.as<unsigned long long>();
+  s.item_cnt = item_cnt.as<uint32_t>();
+  s.item_off = item_off.as<uint32_t>();
+  s.items = items.as<EdgeItem>();
   s.src_count = src_count.as<uint32_t>();
   s.src_off = src_off.as<uint32_t>();
   s.bounds = bounds.as<CmdBounds>();
@@ -584,16 +595,17 @@ uint32_t Runner::execute_exact(Stats& stats) {
     cm::launch_features(f, model, stream);
   }
 
-  // Stage 1: edges (count, scan, emit).
+  // Stage 1: edges (work items; count, scan, emit).
   launch_fill_u32(counters.as<uint32_t>(), 0u, 4, stream);
   launch_init_bounds(s.bounds, f.n_cmds, stream);
-  launch_edges(f, s, true, stream);
+  launch_edge_items(f, s, stream);
+  launch_edges(f, s, true, item_cap, stream);
   launch_cmd_tiles(f, s, stream);
   // A frame of box fills only has no edges: no counting round trip.
   uint32_t n_segs = 0;
   if (n_src) {
-    scan_exclusive(s.src_count, s.src_off, n_src, s.scan_tmp, stream);
-    HIP_OK(hipMemcpyAsync(host_readback, s.src_off + n_src, 4, hipMemcpyDeviceToHost, stream));
+    scan_exclusive(s.src_count, s.src_off, item_cap, s.scan_tmp, stream);
+    HIP_OK(hipMemcpyAsync(host_readback, s.src_off + item_cap, 4, hipMemcpyDeviceToHost, stream));
     HIP_OK(hipStreamSynchronize(stream));
     stats.syncs++;
     n_segs = host_readback[0];
@@ -603,7 +615,7 @@ uint32_t Runner::execute_exact(Stats& stats) {
     return kErrOOM;
   s.segs = segs.as<DevSeg>();
   s.seg_cmd = seg_cmd.as<uint32_t>();
-  launch_edges(f, s, false, stream);
+  launch_edges(f, s, false, item_cap, stream);
 
   // Stage 2: binning.
   launch_fill_u32(s.mat_cnt, 0u, n_mat + 1, stream);

@@ -214,11 +214,10 @@ void launch_infer_frame(const FrameDev& f, const FrameModel& m, hipStream_t st) 
 
 // Exact per-command counts of a frame that ran the counting pass (the
 // training targets).
-__global__ void k_actual_segs(FrameDev f, FrameScratch s, FrameModel m) {
+__global__ void k_actual_segs(FrameDev f, FrameScratch s, FrameModel m, uint32_t n_segs) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.n_src) return;
-  uint32_t n = s.src_count[i];
-  if (n) atomicAdd(&m.actual_segs[f.src[i].cmd], n);
+  if (i >= n_segs) return;
+  atomicAdd(&m.actual_segs[s.seg_cmd[i]], 1u);
 }
 
 __global__ void k_actual_bins(FrameDev f, FrameScratch s, FrameModel m, uint32_t n_segs) {
@@ -234,7 +233,7 @@ __global__ void k_actual_bins(FrameDev f, FrameScratch s, FrameModel m, uint32_t
 }
 
 void launch_actuals(const FrameDev& f, const FrameScratch& s, const FrameModel& m, uint32_t n_segs, hipStream_t st) {
-  if (f.n_src) hipLaunchKernelGGL(k_actual_segs, dim3((f.n_src + 255) / 256), dim3(256), 0, st, f, s, m);
+  if (n_segs) hipLaunchKernelGGL(k_actual_segs, dim3((n_segs + 255) / 256), dim3(256), 0, st, f, s, m, n_segs);
   if (n_segs) hipLaunchKernelGGL(k_actual_bins, dim3((n_segs + 255) / 256), dim3(256), 0, st, f, s, m, n_segs);
 }
 

@@ -617,28 +617,41 @@ RTL_AddLine:
     return pieces;
   }
 
-  // quad_to / cubic_to for one source curve (edgebuilder_p.h:1618-1890).
-  __device__ void curve(const P* cv, int npts, P* stack) {
+  // quad_to / cubic_to for one source curve (edgebuilder_p.h:1618-1890),
+  // in two parts.  curve_split: a curve wholly on one side of the clip box
+  // emits its border line here and gives 0; otherwise the monotonic pieces
+  // are written to `spline` (npts - 1 points apart, end point shared) and
+  // their number returned, with `any` set when a control point lies outside
+  // the clip box.  curve_piece flattens one piece; pieces are independent.
+  __device__ int curve_split(const P* cv, int npts, P* spline, uint32_t* any_out) {
     uint32_t common = 0xFu, any = 0u;
     for (int i = 0; i < npts; i++) { uint32_t f = fxy(c, cv[i]); common &= f; any |= f; }
+    *any_out = any;
     if (common) {
-      if (common & CY0) return;
-      if (common & CY1) return;
+      if (common & CY0) return 0;
+      if (common & CY1) return 0;
       double y0 = dclamp(cv[0].y, c.y0, c.y1);
       double y1 = dclamp(cv[npts - 1].y, c.y0, c.y1);
       if (common & CX0) acc_left(y0, y1); else acc_right(y0, y1);
-      return;
+      return 0;
     }
-    P spline[8 * 3 + 1];
-    int pieces = npts == 3 ? split_quad(cv, spline) : split_cubic(cv, spline);
+    return npts == 3 ? split_quad(cv, spline) : split_cubic(cv, spline);
+  }
+
+  __device__ void curve_piece(const P* sp, int npts, uint32_t any, P* stack) {
     Mono m;
     m.n = npts;
+    const uint32_t sign = sp[0].y > sp[npts - 1].y;
+    if (any) flatten_unsafe(m, sp, sign, stack);
+    else flatten_safe(m, sp, sign, stack);
+  }
+
+  __device__ void curve(const P* cv, int npts, P* stack) {
+    P spline[8 * 3 + 1];
+    uint32_t any;
+    const int pieces = curve_split(cv, npts, spline, &any);
     const P* sp = spline;
-    for (int k = 0; k < pieces; k++, sp += npts - 1) {
-      uint32_t sign = sp[0].y > sp[npts - 1].y;
-      if (any) flatten_unsafe(m, sp, sign, stack);
-      else flatten_safe(m, sp, sign, stack);
-    }
+    for (int k = 0; k < pieces; k++, sp += npts - 1) curve_piece(sp, npts, any, stack);
   }
 };
 

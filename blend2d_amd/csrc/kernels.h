@@ -28,9 +28,18 @@ struct FrameDev {
 };
 
 // Scratch buffers the pipeline needs; sized by the caller.
+// One edge work item (see k_items_count): up to 4 points (x, y pairs).
+struct EdgeItem {
+  double p[8];
+  uint32_t cmd, kind, any, pad;
+};
+
 struct FrameScratch {
-  uint32_t* src_count;     // n_src + 1
-  uint32_t* src_off;       // n_src + 1
+  uint32_t* item_cnt;      // n_src + 1: edge work items per source segment
+  uint32_t* item_off;      // n_src + 1: their exclusive scan
+  EdgeItem* items;         // <= item cap: the work items
+  uint32_t* src_count;     // item cap + 1: edge segments per work item (pool mode: per source)
+  uint32_t* src_off;       // item cap + 1: their exclusive scan
   CmdBounds* bounds;       // n_cmds
   DevSeg* segs;            // n_segs
   uint32_t* seg_cmd;       // n_segs
@@ -68,9 +77,12 @@ struct FrameScratch {
 size_t scan_tmp_words(size_t n);
 void scan_exclusive(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp, hipStream_t s);
 
-// Stage 1: edge building.  count == true writes per-source segment counts and
-// command bounds; count == false writes the segments at the scanned offsets.
-void launch_edges(const FrameDev& f, const FrameScratch& s, bool count, hipStream_t st);
+// Stage 1: edge building.  Work items (lines, monotonic curve pieces) per
+// source segment, scanned and written; then count == true writes per-item
+// segment counts and command bounds, count == false writes the segments at
+// the scanned offsets.  `cap` bounds the number of items.
+void launch_edge_items(const FrameDev& f, const FrameScratch& s, hipStream_t st);
+void launch_edges(const FrameDev& f, const FrameScratch& s, bool count, uint32_t cap, hipStream_t st);
 // Single pass into a pool of `cap` segments (pool mode).
 void launch_edges_pool(const FrameDev& f, const FrameScratch& s, uint32_t cap, hipStream_t st);
 // Stage 2: binning of segments into (band, command) cells.

@@ -204,15 +204,114 @@ __device__ __forceinline__ void build_one(const FrameDev& f, const SrcSeg& sg, E
   }
 }
 
-__global__ __launch_bounds__(64) void k_edges_count(FrameDev f, FrameScratch s) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// Edge work items: a line, one monotonic piece of a curve, or the border line
+// of a curve lying wholly left or right of the clip box.
+// Splitting curves into their pieces up front gives the flattening kernels
+// several independent, shorter work items per curve.
+enum ItemKind : uint32_t { kItemLine = 0, kItemQuad = 1, kItemCubic = 2, kItemBorderLeft = 3, kItemBorderRight = 4 };
+
+__device__ __forceinline__ edge::Clip canvas_clip(const DevCanvas& cv) {
+  edge::Clip clip;
+  clip.x0 = 0.0; clip.y0 = 0.0;
+  clip.x1 = (double)cv.w * 256.0;
+  clip.y1 = (double)cv.h * 256.0;
+  clip.ix0 = (int32_t)clip.x0;
+  clip.ix1 = (int32_t)clip.x1;
+  return clip;
+}
+
+// Work items of source segment `sg`; written to `out` when it is not null.
+__device__ __forceinline__ uint32_t source_items(const FrameDev& f, const SrcSeg& sg, EdgeItem* out) {
+  const CmdXform& m = f.xforms[sg.cmd];
+  if (sg.kind == kSrcLine) {
+    if (out) {
+      edge::P* op = (edge::P*)out->p;
+      op[0] = xform_pt(m, f.vtx, sg.i0);
+      op[1] = xform_pt(m, f.vtx, sg.i1);
+      out->cmd = sg.cmd;
+      out->kind = kItemLine;
+      out->any = 0u;
+    }
+    return 1u;
+  }
+  const int npts = sg.kind == kSrcQuad ? 3 : 4;
+  edge::P cv[4];
+  cv[0] = xform_pt(m, f.vtx, sg.i0);
+  cv[1] = xform_pt(m, f.vtx, sg.i1);
+  cv[2] = xform_pt(m, f.vtx, sg.i1 + 1);
+  cv[3] = npts == 4 ? xform_pt(m, f.vtx, sg.i1 + 2) : cv[2];
+  const edge::Clip clip = canvas_clip(f.canvases[f.cmds[sg.cmd].canvas]);
+  uint32_t common = 0xFu, any = 0u;
+  for (int i = 0; i < npts; i++) { const uint32_t fl = edge::fxy(clip, cv[i]); common &= fl; any |= fl; }
+  if (common) {
+    if (common & (edge::CY0 | edge::CY1)) return 0u;   // above or below the canvas: nothing
+    // wholly left or right of the canvas: the border line over its y range
+    // (edgebuilder_p.h:1618-1890, Builder::curve_split)
+    if (out) {
+      edge::P* op = (edge::P*)out->p;
+      op[0] = edge::P{0.0, edge::dclamp(cv[0].y, clip.y0, clip.y1)};
+      op[1] = edge::P{0.0, edge::dclamp(cv[npts - 1].y, clip.y0, clip.y1)};
+      out->cmd = sg.cmd;
+      out->kind = (common & edge::CX0) ? kItemBorderLeft : kItemBorderRight;
+      out->any = any;
+    }
+    return 1u;
+  }
+  CountEmit dummy;
+  edge::Builder<CountEmit> b(dummy, clip, f.tol_sq);
+  edge::P spline[8 * 3 + 1];
+  const int pieces = npts == 3 ? b.split_quad(cv, spline) : b.split_cubic(cv, spline);
+  if (out) {
+    for (int k = 0; k < pieces; k++) {
+      const edge::P* sp = spline + k * (npts - 1);
+      edge::P* op = (edge::P*)out[k].p;
+      for (int i = 0; i < npts; i++) op[i] = sp[i];
+      out[k].cmd = sg.cmd;
+      out[k].kind = npts == 3 ? kItemQuad : kItemCubic;
+      out[k].any = any;
+    }
+  }
+  return (uint32_t)pieces;
+}
+
+__global__ void k_items_count(FrameDev f, FrameScratch s) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= f.n_src) return;
-  SrcSeg sg = f.src[i];
+  s.item_cnt[i] = source_items(f, f.src[i], nullptr);
+}
+
+__global__ void k_items_write(FrameDev f, FrameScratch s) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n_src) return;
+  (void)source_items(f, f.src[i], s.items + s.item_off[i]);
+}
+
+template<typename Emit>
+__device__ __forceinline__ void build_item(const FrameDev& f, const EdgeItem& it, Emit& e) {
+  edge::Builder<Emit> b(e, canvas_clip(f.canvases[f.cmds[it.cmd].canvas]), f.tol_sq);
+  edge::P stack[edge::Builder<Emit>::kLimit * 4];
+  const edge::P* ip = (const edge::P*)it.p;
+  switch (it.kind) {
+    case kItemLine: b.line(ip[0], ip[1]); break;
+    case kItemQuad: b.curve_piece(ip, 3, it.any, stack); break;
+    case kItemCubic: b.curve_piece(ip, 4, it.any, stack); break;
+    case kItemBorderLeft: b.acc_left(ip[0].y, ip[1].y); break;
+    default: b.acc_right(ip[0].y, ip[1].y); break;
+  }
+}
+
+// Counting pass over the work items (slots past the item total get 0, so
+// the scan may run over the host's upper bound `cap`).
+__global__ __launch_bounds__(64) void k_edges_count(FrameDev f, FrameScratch s, uint32_t cap) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cap) return;
+  if (i >= s.item_off[f.n_src]) { s.src_count[i] = 0u; return; }
+  const EdgeItem it = s.items[i];
   CountEmit e;
-  build_one(f, sg, e);
+  build_item(f, it, e);
   s.src_count[i] = e.n;
   if (e.n) {
-    CmdBounds* bd = &s.bounds[sg.cmd];
+    CmdBounds* bd = &s.bounds[it.cmd];
     atomicMin(&bd->min_x, e.min_x);
     atomicMax(&bd->max_x, e.max_x);
     atomicMin(&bd->min_y, e.min_y);
@@ -220,12 +319,12 @@ __global__ __launch_bounds__(64) void k_edges_count(FrameDev f, FrameScratch s) 
   }
 }
 
-__global__ __launch_bounds__(64) void k_edges_emit(FrameDev f, FrameScratch s) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.n_src) return;
-  SrcSeg sg = f.src[i];
-  WriteEmit e{s.segs, s.seg_cmd, s.src_off[i], sg.cmd};
-  build_one(f, sg, e);
+__global__ __launch_bounds__(64) void k_edges_emit(FrameDev f, FrameScratch s, uint32_t cap) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cap || i >= s.item_off[f.n_src]) return;
+  const EdgeItem it = s.items[i];
+  WriteEmit e{s.segs, s.seg_cmd, s.src_off[i], it.cmd};
+  build_item(f, it, e);
 }
 
 // Single-pass emission into a pool sized ahead of time: each thread takes
@@ -292,11 +391,18 @@ void launch_edges_pool(const FrameDev& f, const FrameScratch& s, uint32_t cap, h
   hipLaunchKernelGGL(k_edges_emit_pool, dim3((f.n_src + 63) / 64), dim3(64), 0, st, f, s, cap);
 }
 
-void launch_edges(const FrameDev& f, const FrameScratch& s, bool count, hipStream_t st) {
+void launch_edge_items(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
   if (!f.n_src) return;
-  dim3 grid((f.n_src + 63) / 64), block(64);
-  if (count) hipLaunchKernelGGL(k_edges_count, grid, block, 0, st, f, s);
-  else hipLaunchKernelGGL(k_edges_emit, grid, block, 0, st, f, s);
+  hipLaunchKernelGGL(k_items_count, dim3((f.n_src + 255) / 256), dim3(256), 0, st, f, s);
+  scan_exclusive(s.item_cnt, s.item_off, f.n_src, s.scan_tmp, st);
+  hipLaunchKernelGGL(k_items_write, dim3((f.n_src + 255) / 256), dim3(256), 0, st, f, s);
+}
+
+void launch_edges(const FrameDev& f, const FrameScratch& s, bool count, uint32_t cap, hipStream_t st) {
+  if (!f.n_src || !cap) return;
+  dim3 grid((cap + 63) / 64), block(64);
+  if (count) hipLaunchKernelGGL(k_edges_count, grid, block, 0, st, f, s, cap);
+  else hipLaunchKernelGGL(k_edges_emit, grid, block, 0, st, f, s, cap);
 }
 
 // ---------------------------------------------------------------------------
