@@ -1,6 +1,6 @@
 """Fill-path throughput on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4|srcover4k]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4|c5|srcover4k]
 
 A step renders one whole synthetic frame through the device pipeline (edge
 building, binning, tile rasterization + fetch + composite) from a frame
@@ -71,6 +71,13 @@ def _workload(name: str, seed: int):
         return scenes, {
             "workload": "4096 canvases 32-256 px (half A8, half PRGB32), 1-200 quadratic glyph contours each, "
                         "non-zero, solid / linear gradient, SRC_OVER, one batched frame",
+            "canvases": len(scenes), "paths": int(sum(sc.fill_count for sc in scenes)), "seed": seed}
+    if name == "c5":
+        scenes = S.cost_canvases(seed, 256, 1024, 4096, 64)
+        return scenes, {
+            "workload": "256 PRGB32 canvases 1024-4096 px, 1-64 paths each with clip(Pareto(1.5) * 8, 8, 20000) "
+                        "line / quad / cubic segments, solid / linear / radial / pattern, SRC_OVER / MULTIPLY / "
+                        "PLUS / SCREEN, both fill rules, one batched frame",
             "canvases": len(scenes), "paths": int(sum(sc.fill_count for sc in scenes)), "seed": seed}
     if name == "srcover4k":
         scenes = [S.srcover_layers_scene(seed + i, 3840, 2160, 1) for i in range(10)]
@@ -301,7 +308,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "srcover4k"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "srcover4k"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

@@ -91,6 +91,18 @@ def test_box_and_analytic_canvases_batched(amd):
         assert checkers.diff_info(out, want)[0] <= 1
 
 
+def test_pareto_mix_canvases_batched(amd):
+    # the many-canvas mixed workload at test size: Pareto-distributed
+    # segment counts of lines / quads / cubics, every fetcher and four comp
+    # ops, both fill rules, one batched frame
+    scenes = S.cost_canvases(7, 6, 96, 400, 16, 3000)
+    outs, errs = amd.render_scenes_batched(scenes)
+    for sc, out, err in zip(scenes, outs, errs):
+        want, oerr = checkers.render_oracle(sc)
+        assert err == oerr
+        assert checkers.diff_info(out, want)[0] <= 1
+
+
 def test_edge_cases_exact(amd):
     for fmt in (S.FMT_PRGB32, S.FMT_XRGB32, S.FMT_A8):
         _cmp(amd, S.edge_case_scene(fmt=fmt))
