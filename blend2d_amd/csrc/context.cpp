@@ -1402,7 +1402,6 @@ BLAmdResult bl_amd_context_set_fill_style_pattern(BLAmdContext* ctx, const BLAmd
   }
   int r = init_pattern(st.dev, (uint32_t)image->w, (uint32_t)image->h, extend, ctx->pattern_quality, t);
   if (r == 0) st.status = 1;
-  else if (r < 0) st.status = kErrNotImpl;
   return BL_AMD_SUCCESS;
 }
 

@@ -41,6 +41,8 @@ enum FetchKind : uint32_t {
   kFetchConic = 3,
   kFetchPatternNN = 4,
   kFetchPatternBI = 5,
+  kFetchPatternAligned = 6,   // pure translation, integer texel offset
+  kFetchPatternFxFy = 7,      // pure translation, fractional offset, bilinear
 };
 
 // Command kinds.
@@ -123,6 +125,12 @@ struct DevStyle {
       uint64_t pa_xx, pa_xy, pa_yx, pa_yy, pa_tx, pa_ty;
       int32_t pa_ox, pa_oy, pa_rx, pa_ry, pa_minx, pa_miny, pa_maxx, pa_maxy;
       int32_t pa_corx, pa_cory, pa_tw, pa_th;
+    };
+    struct {             // translated pattern (aligned / FxFy)
+      int32_t sp_tx, sp_ty;    // texel offset added to the pixel position
+      int32_t sp_rx, sp_ry;    // repeat (w) or reflect (2w) period, 0 = pad
+      int32_t sp_w, sp_h;      // image size
+      uint32_t sp_wa, sp_wb, sp_wc, sp_wd;   // FxFy weights, sum <= 256
     };
   };
 };

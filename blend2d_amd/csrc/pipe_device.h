@@ -1,7 +1,8 @@
 // Device fetchers and composition operators.
 //
 // Fetch: solid, linear/radial/conic gradient LUT lookup, affine pattern with
-// nearest or bilinear filtering (pipeline/reference/fetchgeneric_p.h).  The
+// nearest or bilinear filtering, translated pattern at an integer or
+// fractional offset (pipeline/reference/fetchgeneric_p.h).  The
 // reference steps each fetcher per pixel and per row; all of them are pure
 // functions of the pixel position except the conic gradient, whose per-row
 // origin accumulates in double precision from the first row of the fill call
@@ -50,6 +51,30 @@ __device__ __forceinline__ int32_t rint_f32_to_i32(float v) {
   return (int32_t)__builtin_rintf(v);
 }
 
+// Translated pattern, texel row of pattern row v (fetchgeneric_p.h:80-165):
+// pad clamps to [0, h-1]; repeat wraps by h; reflect wraps by 2h and mirrors
+// the second half.
+__device__ __forceinline__ uint32_t sp_row(const DevStyle& s, int64_t v) {
+  const int64_t h = (int64_t)s.sp_h;
+  if (s.sp_ry == 0) return (uint32_t)(v < 0 ? 0 : v > h - 1 ? h - 1 : v);
+  int64_t r = v % s.sp_ry;
+  if (r < 0) r += s.sp_ry;
+  if (r >= h) r = s.sp_ry - 1 - r;
+  return (uint32_t)r;
+}
+
+// Translated pattern, texel column of pattern column u
+// (fetchgeneric_p.h:187-352): pad clamps; a period rx (repeat: w, reflect:
+// 2w) wraps, and a wrapped position v >= w maps to rx - 1 - v.  For repeat
+// and reflect the offset is kept in [0, rx), so u >= 0 there.
+__device__ __forceinline__ uint32_t sp_col(const DevStyle& s, int64_t u) {
+  const int64_t w = (int64_t)s.sp_w;
+  if (s.sp_rx == 0) return (uint32_t)(u < 0 ? 0 : u > w - 1 ? w - 1 : u);
+  uint32_t v = (uint32_t)u % (uint32_t)s.sp_rx;
+  if (v >= (uint32_t)w) v = (uint32_t)s.sp_rx - 1u - v;
+  return v;
+}
+
 // Per (command, row) fetch state.
 struct Row {
   uint64_t a, b;       // linear: pt at x = 0; pattern: tx, ty of the row
@@ -96,6 +121,12 @@ __device__ __forceinline__ Row row_setup(const DevStyle& s, int y, int fill_row0
     case kFetchPatternBI:
       r.a = s.pa_tx + s.pa_yx * (uint64_t)(uint32_t)y;
       r.b = s.pa_ty + s.pa_yy * (uint64_t)(uint32_t)y;
+      break;
+    case kFetchPatternAligned:
+    case kFetchPatternFxFy:
+      // texel rows of this row and of the next one (FxFy)
+      r.a = sp_row(s, (int64_t)y + s.sp_ty);
+      r.b = sp_row(s, (int64_t)y + 1 + s.sp_ty);
       break;
     default:
       break;
@@ -196,6 +227,23 @@ __device__ __forceinline__ uint32_t fetch(const DevStyle& s, const Row& r, int x
         out |= (((p0 * iw + p1 * wx) >> 8) & 0xFFu) << (c * 8);
       }
       return out;
+    }
+    case kFetchPatternAligned:
+      return texel(images[s.img], src_fmt, (int32_t)sp_col(s, (int64_t)x + s.sp_tx), (int32_t)r.a);
+    case kFetchPatternFxFy: {
+      // fetchgeneric_p.h:601-687: columns x0 = x + tx and x1 = x0 + 1 of rows
+      // y0, y1, weighted wa (x0 y0), wb (x1 y0), wc (x0 y1), wd (x1 y1).  The
+      // weights sum to at most 256, so each 16-bit lane of the R/B and A/G
+      // pairs stays below 65536.
+      const DevImage& im = images[s.img];
+      const int32_t x0 = (int32_t)sp_col(s, (int64_t)x + s.sp_tx), x1 = (int32_t)sp_col(s, (int64_t)x + 1 + s.sp_tx);
+      const uint32_t t00 = texel(im, src_fmt, x0, (int32_t)r.a), t01 = texel(im, src_fmt, x0, (int32_t)r.b);
+      const uint32_t t10 = texel(im, src_fmt, x1, (int32_t)r.a), t11 = texel(im, src_fmt, x1, (int32_t)r.b);
+      const uint32_t m = 0x00FF00FFu;
+      const uint32_t rb = (t00 & m) * s.sp_wa + (t01 & m) * s.sp_wc + (t10 & m) * s.sp_wb + (t11 & m) * s.sp_wd;
+      const uint32_t ag = ((t00 >> 8) & m) * s.sp_wa + ((t01 >> 8) & m) * s.sp_wc +
+                          ((t10 >> 8) & m) * s.sp_wb + ((t11 >> 8) & m) * s.sp_wd;
+      return ((rb >> 8) & m) | (ag & ~m);
     }
     default:
       return 0;

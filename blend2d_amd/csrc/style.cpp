@@ -328,15 +328,64 @@ bool init_conic(DevStyle& f, uint32_t lut_size, const double v[4], const Mat& t)
   return true;
 }
 
-// pipeline/pipedefs.cpp:162-381 (affine fetchers)
+// pipeline/pipedefs.cpp:119-160 (init_pattern_fx_fy) and :52-111
+// (init_pattern_tx_ty): the 24.8 translation splits into a texel offset and,
+// for bilinear quality, four weights of the 2x2 neighbourhood; nearest
+// quality rounds the offset instead.  Repeat / reflect keep the offset in
+// [0, period); the fractional fetcher always uses the repeat-or-reflect
+// column mapping (reflect period 2w, repeat period w).
+static void init_pattern_translated(DevStyle& f, uint32_t img_w, uint32_t img_h, uint32_t extend, uint32_t quality,
+                                    int64_t tx64, int64_t ty64) {
+  const uint32_t wx = (uint32_t)(tx64 & 0xFF), wy = (uint32_t)(ty64 & 0xFF);
+  int tx = -(int)(tx64 >> 8), ty = -(int)(ty64 >> 8);
+  bool fractional = (wx | wy) != 0;
+  f.sp_wa = f.sp_wb = f.sp_wc = f.sp_wd = 0;
+  if (fractional) {
+    if (quality == 0) {
+      tx -= (wx >= 128);
+      ty -= (wy >= 128);
+      fractional = false;
+    }
+    else {
+      f.sp_wa = (wy * wx) >> 8;                        // texel (x0, y0)
+      f.sp_wb = (wy * (256 - wx) + 255) >> 8;          // texel (x1, y0)
+      f.sp_wc = ((256 - wy) * wx) >> 8;                // texel (x0, y1)
+      f.sp_wd = ((256 - wy) * (256 - wx) + 255) >> 8;  // texel (x1, y1)
+      tx--;
+      ty--;
+    }
+  }
+  f.kind = fractional ? kFetchPatternFxFy : kFetchPatternAligned;
+  const uint32_t ex = img_w <= 1 ? 0u : extend, ey = img_h <= 1 ? 0u : extend;
+  int rx = 0, ry = 0;
+  if (ex != 0) {
+    rx = (int)img_w << (ex == 2);
+    if ((unsigned)tx >= (unsigned)rx) tx %= rx;
+    if (tx < 0) tx += rx;
+  }
+  if (ey != 0) {
+    ry = (int)img_h << (ey == 2);
+    if ((unsigned)ty >= (unsigned)ry) ty %= ry;
+    if (ty < 0) ty += ry;
+  }
+  f.sp_tx = tx; f.sp_ty = ty;
+  f.sp_rx = rx; f.sp_ry = ry;
+  f.sp_w = (int32_t)img_w; f.sp_h = (int32_t)img_h;
+}
+
+// pipeline/pipedefs.cpp:162-381
 int init_pattern(DevStyle& f, uint32_t img_w, uint32_t img_h, uint32_t extend, uint32_t quality, const Mat& t) {
   Mat inv;
   if (!invert(inv, t)) return 0;
   int tw = (int)img_w, th = (int)img_h;
   if (tw == 0) return 0;
   double xx = inv.m00, xy = inv.m01, yx = inv.m10, yy = inv.m11;
-  const double eps = 1e-14;
-  if (fabs(xx - 1.0) <= eps && fabs(xy) <= eps && fabs(yx) <= eps && fabs(yy - 1.0) <= eps) return -1;
+  const double eps = 1e-14;   // Math::epsilon<double>, support/math_p.h:42
+  if (fabs(xx - 1.0) <= eps && fabs(xy) <= eps && fabs(yx) <= eps && fabs(yy - 1.0) <= eps) {
+    init_pattern_translated(f, img_w, img_h, extend, quality, floor_to_int64(-inv.m20 * 256.0),
+                            floor_to_int64(-inv.m21 * 256.0));
+    return 1;
+  }
 
   f.kind = quality == 0 ? kFetchPatternNN : kFetchPatternBI;
   uint32_t ex = extend, ey = extend;

@@ -50,8 +50,8 @@ void interpolate_lut(uint32_t* dst, uint32_t size, const std::vector<Stop>& stop
 bool init_linear(DevStyle& s, uint32_t lut_size, const double v[4], uint32_t extend, const Mat& t);
 bool init_radial(DevStyle& s, uint32_t lut_size, const double v[6], uint32_t extend, const Mat& t);
 bool init_conic(DevStyle& s, uint32_t lut_size, const double v[4], const Mat& t);
-// Returns 1 on success, 0 when the transform is not invertible, -1 when the
-// transform selects the aligned/fractional-translation fetchers.
+// Returns 1 on success, 0 when the transform is not invertible or the image
+// is empty (the fill is a no-op).
 int init_pattern(DevStyle& s, uint32_t img_w, uint32_t img_h, uint32_t extend, uint32_t quality, const Mat& t);
 
 Simplified simplify(uint32_t op, uint32_t dst_fmt, uint32_t src_fmt);

@@ -558,6 +558,63 @@ def mixed_scene(seed: int = 1, width: int = 513, height: int = 513, count: int =
     return sc
 
 
+def translated_patterns_scene(seed: int = 1, width: int = 301, height: int = 211, count: int = 120,
+                              fmt: int = FMT_PRGB32,
+                              comp_ops=(COMP_OP_SRC_OVER, COMP_OP_SRC_COPY, COMP_OP_PLUS, COMP_OP_MULTIPLY)) -> Scene:
+    """Patterns whose final transform is a pure translation, the case that
+    selects the aligned and fractional-translation fetchers: integer, 1/256
+    and arbitrary fractional offsets (positive, negative and far outside the
+    image), set on the pattern, on the user transform or split between them;
+    PAD / REPEAT / REFLECT; nearest and bilinear; PRGB32, XRGB32 and A8
+    images including 1-pixel-wide / -high and tiny (3x2) ones; aligned and
+    fractional rects, ellipses and quadratic contours."""
+    rng = np.random.default_rng(seed)
+    clear = 0xFF000000 | int(rng.integers(0, 1 << 24)) if fmt != FMT_A8 else int(rng.integers(0, 256))
+    sc = Scene(width, height, fmt, clear)
+    imgs = [sc.add_image(premultiplied_noise(rng, 61, 47)),
+            sc.add_image(premultiplied_noise(rng, 1, 37)),
+            sc.add_image(premultiplied_noise(rng, 29, 1)),
+            sc.add_image(premultiplied_noise(rng, 3, 2)),
+            sc.add_image(rng.integers(0, 1 << 32, size=(23, 19), dtype=np.uint32), FMT_XRGB32),
+            sc.add_image(rng.integers(0, 256, size=(17, 33), dtype=np.uint8), FMT_A8)]
+
+    def offset():
+        t = rng.random()
+        if t < 0.3:
+            return float(rng.integers(-400, 400))
+        if t < 0.6:
+            return float(rng.integers(-400 * 256, 400 * 256)) / 256.0
+        if t < 0.9:
+            return float(rng.uniform(-400, 400))
+        return float(rng.choice([-1e5, 1e5])) + float(rng.uniform(-1, 1))
+
+    for _ in range(count):
+        sc.set_comp_op(int(comp_ops[int(rng.integers(0, len(comp_ops)))]))
+        sc.set_global_alpha(1.0 if rng.random() < 0.7 else float(rng.uniform(0.0, 1.0)))
+        sc.set_pattern_quality(int(rng.integers(0, 2)))
+        u = rng.random()
+        ux, uy = (offset(), offset()) if u < 0.3 else (float(rng.integers(-20, 20)), 0.0) if u < 0.5 else (0.0, 0.0)
+        sc.set_transform([1.0, 0.0, 0.0, 1.0, ux, uy])
+        img = imgs[int(rng.integers(0, len(imgs)))]
+        ptr = [1.0, 0.0, 0.0, 1.0, offset(), offset()] if rng.random() < 0.8 else None
+        sc.set_fill_style_pattern(img, int(rng.integers(0, 3)), ptr)
+        x0, y0 = float(rng.uniform(-0.2, 1.0) * width) - ux, float(rng.uniform(-0.2, 1.0) * height) - uy
+        w, h = float(rng.uniform(1, width * 0.7)), float(rng.uniform(1, height * 0.7))
+        g = rng.random()
+        if g < 0.35:
+            sc.fill_rect_i(int(x0), int(y0), int(w), int(h))
+        elif g < 0.6:
+            sc.fill_rect_d(x0, y0, w, h)
+        elif g < 0.8:
+            cm, v = ellipse_path(x0 + w / 2, y0 + h / 2, w / 2, h / 2)
+            sc.fill_path(cm, v)
+        else:
+            cm, v = glyph_path(rng, w, h)
+            sc.fill_path(cm, np.asarray(v) + np.array([x0, y0]))
+    sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+    return sc
+
+
 def box_scene(seed: int = 1, width: int = 333, height: int = 77, count: int = 60, fmt: int = FMT_PRGB32,
               comp_ops=(COMP_OP_SRC_OVER, COMP_OP_SRC_COPY)) -> Scene:
     """Box fills only (integer and fractional rects, fill_all, clears) under

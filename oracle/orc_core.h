@@ -51,7 +51,7 @@ typedef struct {
 /* Resolved fetch description (pipeline/pipedefs_p.h FetchData). */
 enum { ORC_FETCH_SOLID = 0, ORC_FETCH_LINEAR = 1, ORC_FETCH_RADIAL = 2, ORC_FETCH_CONIC = 3,
        ORC_FETCH_PATTERN_AFFINE_NN = 4, ORC_FETCH_PATTERN_AFFINE_BI = 5,
-       ORC_FETCH_PATTERN_ALIGNED = 6, ORC_FETCH_PATTERN_FX = 7, ORC_FETCH_PATTERN_FY = 8, ORC_FETCH_PATTERN_FXFY = 9 };
+       ORC_FETCH_PATTERN_ALIGNED = 6, ORC_FETCH_PATTERN_FXFY = 7 };
 
 typedef struct {
   uint32_t type;
@@ -75,10 +75,10 @@ typedef struct {
   uint32_t img_w, img_h, img_format;
   uint64_t pa_xx, pa_xy, pa_yx, pa_yy, pa_tx, pa_ty;
   int32_t pa_ox, pa_oy, pa_rx, pa_ry, pa_minx, pa_miny, pa_maxx, pa_maxy, pa_corx, pa_cory, pa_tw, pa_th;
-  /* simple (aligned / fractional translation) pattern */
-  int32_t ps_tx, ps_ty, ps_rx, ps_ry;
-  uint32_t ps_extend_x, ps_extend_y;
-  uint32_t ps_wa, ps_wb, ps_wc, ps_wd;
+  /* aligned / fractional-translation pattern (pipedefs.cpp:60-160): texel
+   * offset, repeat-or-reflect periods (0 = pad) and bilinear weights */
+  int32_t sp_tx, sp_ty, sp_rx, sp_ry;
+  uint32_t sp_wa, sp_wb, sp_wc, sp_wd;
 } OrcFetch;
 
 /* ---- numeric helpers (support/math_p.h, core/api.h) ---- */

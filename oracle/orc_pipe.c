@@ -47,6 +47,29 @@ static inline uint32_t tex_fetch(const OrcFetch* f, uint32_t fmt, uint32_t x, ui
   return v;
 }
 
+/* Texel column of pattern position u: pad clamps; a repeat-or-reflect period
+ * rx wraps (fetchgeneric_p.h:264-352: positions past the width index from the
+ * far end, x ^ (x >> 31)). */
+static uint32_t sp_index_x(const OrcFetch* f, int64_t u) {
+  int64_t w = (int64_t)f->img_w;
+  if (f->sp_rx == 0) return (uint32_t)(u < 0 ? 0 : u > w - 1 ? w - 1 : u);
+  int64_t v = u % f->sp_rx;
+  if (v < 0) v += f->sp_rx;
+  if (v >= w) v -= f->sp_rx;
+  return (uint32_t)(v ^ (v >> 63));
+}
+
+/* Texel row (fetchgeneric_p.h:80-165): pad clamps; repeat wraps by h;
+ * reflect wraps by 2h and mirrors the second half. */
+static uint32_t sp_index_y(const OrcFetch* f, int64_t v) {
+  int64_t h = (int64_t)f->img_h;
+  if (f->sp_ry == 0) return (uint32_t)(v < 0 ? 0 : v > h - 1 ? h - 1 : v);
+  int64_t r = v % f->sp_ry;
+  if (r < 0) r += f->sp_ry;
+  if (r >= h) r = f->sp_ry - 1 - r;
+  return (uint32_t)r;
+}
+
 /* FetchPatternAffineCtx::normalize_px_py (fetchgeneric_p.h:373-384) */
 static inline uint64_t pa_norm(int32_t tw, int32_t rx, int32_t ox, uint64_t v) {
   uint32_t x = (uint32_t)((int32_t)(v >> 32) % tw);
@@ -165,6 +188,30 @@ static uint32_t fetch_px(const OrcPipe* p, uint32_t x, uint32_t y) {
         p0 *= ixw;
         p1 *= wx;
         out |= (((p0 + p1) >> 8) & 0xFFu) << (c * 8);
+      }
+      return out;
+    }
+
+    /* fetchgeneric_p.h:80-184 (vertical extend), :186-352 (horizontal
+     * extend: pad / repeat / repeat-or-reflect), :470-540 (aligned),
+     * :543-667 (FxFy); each context is stepped per pixel and row there, here
+     * its position is computed directly from the absolute x and y. */
+    case ORC_FETCH_PATTERN_ALIGNED:
+      return tex_fetch(f, p->src_fmt, sp_index_x(f, (int64_t)(int32_t)x + f->sp_tx),
+                       sp_index_y(f, (int64_t)(int32_t)y + f->sp_ty));
+
+    case ORC_FETCH_PATTERN_FXFY: {
+      uint32_t fmt = p->src_fmt;
+      uint32_t x0 = sp_index_x(f, (int64_t)(int32_t)x + f->sp_tx), x1 = sp_index_x(f, (int64_t)(int32_t)x + 1 + f->sp_tx);
+      uint32_t y0 = sp_index_y(f, (int64_t)(int32_t)y + f->sp_ty), y1 = sp_index_y(f, (int64_t)(int32_t)y + 1 + f->sp_ty);
+      uint32_t p00 = tex_fetch(f, fmt, x0, y0), p01 = tex_fetch(f, fmt, x0, y1);
+      uint32_t p10 = tex_fetch(f, fmt, x1, y0), p11 = tex_fetch(f, fmt, x1, y1);
+      uint32_t out = 0;
+      for (int c = 0; c < 4; c++) {
+        /* 16-bit lanes, as the unpacked pixel arithmetic */
+        uint32_t acc = ((ch(p00, c) * f->sp_wa) & 0xFFFFu) + ((ch(p01, c) * f->sp_wc) & 0xFFFFu);
+        acc = (((ch(p10, c) * f->sp_wb) & 0xFFFFu) + ((ch(p11, c) * f->sp_wd) & 0xFFFFu) + (acc & 0xFFFFu)) & 0xFFFFu;
+        out |= ((acc >> 8) & 0xFFu) << (c * 8);
       }
       return out;
     }

@@ -350,8 +350,52 @@ int orc_init_conic(OrcFetch* f, const double v[4], const OrcMat* t) {
   return 0;
 }
 
-/* pipedefs.cpp:162-381, affine fetchers only (a near-identity transform
- * selects the aligned/fractional fetchers; those return 1 = unsupported). */
+/* pipedefs.cpp:60-111 (init_pattern_tx_ty) and :119-160 (init_pattern_fx_fy):
+ * a pattern whose inverse transform is a pure translation samples texels at
+ * integer offsets, or (bilinear, fractional offset) blends 2x2 texels with
+ * fixed weights.  Simple extend modes apply to both axes. */
+static void init_pattern_fx_fy(OrcFetch* f, uint32_t extend, uint32_t quality, int64_t tx64, int64_t ty64) {
+  uint32_t wx = (uint32_t)(tx64 & 0xFF);
+  uint32_t wy = (uint32_t)(ty64 & 0xFF);
+  int tx = -(int)(tx64 >> 8);
+  int ty = -(int)(ty64 >> 8);
+  int fractional = (wx | wy) != 0;
+  f->sp_wa = f->sp_wb = f->sp_wc = f->sp_wd = 0;
+  if (fractional) {
+    if (quality == 0) {
+      tx -= (wx >= 128);
+      ty -= (wy >= 128);
+      fractional = 0;
+    }
+    else {
+      f->sp_wa = ((wy) * (wx)) >> 8;
+      f->sp_wb = ((wy) * (256 - wx) + 255) >> 8;
+      f->sp_wc = ((256 - wy) * (wx)) >> 8;
+      f->sp_wd = ((256 - wy) * (256 - wx) + 255) >> 8;
+      tx--;
+      ty--;
+    }
+  }
+  f->type = fractional ? ORC_FETCH_PATTERN_FXFY : ORC_FETCH_PATTERN_ALIGNED;
+  uint32_t ex = extend, ey = extend;
+  if (f->img_w <= 1) ex = 0;
+  if (f->img_h <= 1) ey = 0;
+  int rx = 0, ry = 0;
+  if (ex >= 1) {
+    rx = (int)f->img_w << (ex == 2);
+    if ((unsigned)tx >= (unsigned)rx) tx %= rx;
+    if (tx < 0) tx += rx;
+  }
+  if (ey >= 1) {
+    ry = (int)f->img_h << (ey == 2);
+    if ((unsigned)ty >= (unsigned)ry) ty %= ry;
+    if (ty < 0) ty += ry;
+  }
+  f->sp_tx = tx; f->sp_ty = ty;
+  f->sp_rx = rx; f->sp_ry = ry;
+}
+
+/* pipedefs.cpp:162-381 */
 int orc_init_pattern(OrcFetch* f, uint32_t extend, uint32_t quality, const OrcMat* t) {
   OrcMat inv;
   if (orc_mat_invert(&inv, t) != 0) return -1;
@@ -359,9 +403,11 @@ int orc_init_pattern(OrcFetch* f, uint32_t extend, uint32_t quality, const OrcMa
   if (tw == 0) return -1;
 
   double xx = inv.m00, xy = inv.m01, yx = inv.m10, yy = inv.m11;
-  const double eps = 1e-14;
-  if (fabs(xx - 1.0) <= eps && fabs(xy) <= eps && fabs(yx) <= eps && fabs(yy - 1.0) <= eps)
-    return 1;
+  const double eps = 1e-14;   /* Math::epsilon<double>, support/math_p.h:42 */
+  if (fabs(xx - 1.0) <= eps && fabs(xy) <= eps && fabs(yx) <= eps && fabs(yy - 1.0) <= eps) {
+    init_pattern_fx_fy(f, extend, quality, orc_floor_to_int64(-inv.m20 * 256.0), orc_floor_to_int64(-inv.m21 * 256.0));
+    return 0;
+  }
 
   f->type = quality == 0 ? ORC_FETCH_PATTERN_AFFINE_NN : ORC_FETCH_PATTERN_AFFINE_BI;
   uint32_t extend_x = extend, extend_y = extend;   /* simple extend modes only */

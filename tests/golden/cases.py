@@ -19,6 +19,8 @@ def cases():
         out[f"edge_cases_{fname}"] = lambda fmt=fmt: [S.edge_case_scene(fmt=fmt)]
         for seed in (1, 2, 3):
             out[f"frontend_{fname}_s{seed}"] = lambda seed=seed, fmt=fmt: [S.frontend_scene(seed, fmt=fmt)]
+        out[f"translated_patterns_{fname}_s1"] = lambda fmt=fmt: [S.translated_patterns_scene(
+            1, 301, 211, 120, fmt, comp_ops=(S.COMP_OP_SRC_OVER, S.COMP_OP_SRC_COPY))]
     return out
 
 

@@ -2,8 +2,8 @@
 library when it is built) on seeded scenes, through the C ABI.
 
 Tolerances (see the module docstrings of the kernels): coverage, solid fills,
-linear gradients, nearest and bilinear patterns and every integer comp op are
-compared bit-exact; radial and conic gradients are computed with the same
+linear gradients, nearest and bilinear patterns (affine and translated) and
+every integer comp op are compared bit-exact; radial and conic gradients are computed with the same
 float operation order and are also expected bit-exact, with at most 1 LSB
 allowed per channel."""
 import numpy as np
@@ -79,6 +79,17 @@ def test_box_only_bands(amd, seed, fmt):
     if fmt != S.FMT_PRGB32:
         ops = ops[:3]
     _cmp(amd, S.box_scene(seed, 333 + 257 * seed, 77, 80, fmt, comp_ops=ops), tol=1)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_XRGB32, S.FMT_A8])
+def test_translated_patterns(amd, seed, fmt):
+    # pure-translation pattern transforms: aligned and fractional (FxFy)
+    # fetchers, every extend mode, tiny and 1-pixel images, bit-exact
+    ops = (S.COMP_OP_SRC_OVER, S.COMP_OP_SRC_COPY, S.COMP_OP_PLUS, S.COMP_OP_MULTIPLY)
+    if fmt != S.FMT_PRGB32:
+        ops = ops[:3]
+    _cmp(amd, S.translated_patterns_scene(seed, 301 + 64 * seed, 211, 120, fmt, comp_ops=ops))
 
 
 def test_box_and_analytic_canvases_batched(amd):

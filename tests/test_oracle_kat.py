@@ -8,7 +8,9 @@ mathematics fixes, independently of any renderer:
   (raster/analyticrasterizer_test.cpp:34-158) for band heights 1..32;
 * comp ops: every executed op against its real-valued Porter-Duff /
   separable-blend formula (error bound) and exact identities;
-* gradient LUT: end points, monotonicity and the linear ramp.
+* gradient LUT: end points, monotonicity and the linear ramp;
+* translated patterns: constant images, integer shifts under each extend
+  mode and the half-pixel 2x2 mean.
 """
 import ctypes
 
@@ -180,3 +182,57 @@ def test_gradient_lut_is_premultiplied_and_hits_stop_colours():
     assert lut[-1] == 0
     mid = len(lut) // 2
     assert any(v == 0xFF00FF00 for v in lut[mid - 2: mid + 2])
+
+
+def _translated_fill(img, dx, dy, extend, quality, w=40, h=24):
+    from blend2d_amd import scene as S
+    sc = S.Scene(w, h, S.FMT_PRGB32, 0xFF000000)
+    i = sc.add_image(img)
+    sc.set_comp_op(S.COMP_OP_SRC_COPY)
+    sc.set_pattern_quality(quality)
+    sc.set_fill_style_pattern(i, extend, [1.0, 0.0, 0.0, 1.0, dx, dy])
+    sc.fill_all()
+    px, err = checkers.render_oracle(sc)
+    assert err == 0
+    return np.asarray(px).reshape(h, w)
+
+
+@pytest.mark.parametrize("extend", [0, 1, 2])
+def test_translated_pattern_constant_image_is_constant(extend):
+    # bilinear weights of any fractional offset sum to 256: a constant image
+    # renders unchanged
+    img = np.full((5, 7), 0xC0806040, dtype=np.uint32)
+    for dx, dy in ((0.5, 0.25), (1 / 256, -3.75), (-17.3, 2.9), (100.99, -0.01)):
+        assert (_translated_fill(img, dx, dy, extend, 1) == 0xC0806040).all(), (dx, dy)
+
+
+def test_translated_pattern_integer_offset_is_a_shift():
+    # integer offsets: repeat is a cyclic shift, reflect a mirrored tiling,
+    # pad clamps to the border texels; nearest and bilinear agree
+    rng = np.random.default_rng(3)
+    img = (rng.integers(0, 1 << 24, size=(5, 7), dtype=np.uint32) | 0xFF000000).astype(np.uint32)
+    ys, xs = np.mgrid[0:24, 0:40]
+    for dx, dy in ((3, 2), (-9, 11), (0, -4)):
+        u, v = xs - dx, ys - dy
+        rep = img[v % 5, u % 7]
+        ru, rv = u % 14, v % 10
+        ref = img[np.where(rv < 5, rv, 9 - rv), np.where(ru < 7, ru, 13 - ru)]
+        pad = img[np.clip(v, 0, 4), np.clip(u, 0, 6)]
+        for q in (0, 1):
+            assert (_translated_fill(img, dx, dy, 1, q) == rep).all()
+            assert (_translated_fill(img, dx, dy, 2, q) == ref).all()
+            assert (_translated_fill(img, dx, dy, 0, q) == pad).all()
+
+
+def test_translated_pattern_half_pixel_bilinear_is_the_2x2_mean():
+    # offset (0.5, 0.5): weights 64 each, i.e. floor(sum of the 2x2 block / 4)
+    rng = np.random.default_rng(4)
+    img = (rng.integers(0, 1 << 24, size=(6, 6), dtype=np.uint32) | 0xFF000000).astype(np.uint32)
+    out = _translated_fill(img, 0.5, 0.5, 1, 1)
+    ys, xs = np.mgrid[0:24, 0:40]
+    for c in range(4):
+        ch = (img >> (8 * c)) & 0xFF
+        u0, v0 = (xs - 1) % 6, (ys - 1) % 6
+        u1, v1 = xs % 6, ys % 6
+        want = (ch[v0, u0] + ch[v0, u1] + ch[v1, u0] + ch[v1, u1]) * 64 >> 8
+        assert (((out >> (8 * c)) & 0xFF) == want).all(), c

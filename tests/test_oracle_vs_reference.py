@@ -57,3 +57,11 @@ def test_frontend_calls(seed, fmt):
 def test_box_fills(seed, fmt):
     # rects, fill_all and clears only, under translate / scale transforms
     _same(S.box_scene(seed, 333, 77, 80, fmt))
+
+
+@pytest.mark.parametrize("seed", [81, 82, 83])
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_XRGB32, S.FMT_A8])
+def test_translated_patterns(seed, fmt):
+    # pure-translation pattern transforms: aligned and fractional fetchers
+    # (the reference's portable pipeline implements SRC_OVER and SRC_COPY only)
+    _same(S.translated_patterns_scene(seed, 301, 211, 120, fmt, comp_ops=(S.COMP_OP_SRC_OVER, S.COMP_OP_SRC_COPY)))
