@@ -1744,12 +1744,6 @@ BLAmdResult bl_amd_batch_replay(BLAmdBatch* b) {
 
 BLAmdResult bl_amd_set_sizing(uint32_t mode) {
   if (mode > kSizeHistory) return kErrInvalid;
-#ifndef B2AMD_ENABLE_POOL_SIZING
-  // The pool-mode frame path (learned and history sizing) is switched off
-  // in this build: a GPU run that exercised it ended in a device fault whose
-  // cause has not been identified yet.  Exact counting stays available.
-  if (mode != kSizeExact) return kErrNotImpl;
-#endif
   g_sizing = mode;
   return BL_AMD_SUCCESS;
 }

@@ -351,6 +351,13 @@ struct ChunkEmit {
     if (pos == end) {
       uint32_t base = atomicAdd(cursor, chunk);
       if (base > cap || cap - base < chunk) {
+        // Overflow.  A chunk that straddles the end of the pool owns the
+        // slots [base, cap): mark them unused, since the binning kernels
+        // read every slot below min(cursor, cap).
+        for (uint32_t k = base; k < cap; k++) {
+          segs[k] = DevSeg{0, 0, 0, 0};
+          seg_cmd[k] = 0xFFFFFFFFu;
+        }
         atomicOr(flags, 1u);
         dead = true;
         return;
@@ -426,9 +433,13 @@ __device__ __forceinline__ uint32_t seg_limit(const FrameScratch& s, uint32_t n)
   return c < n ? c : n;
 }
 
+// Pool mode: a segment-pool overflow (flag bit 0) leaves the frame to be
+// redone with exact counting, so binning is skipped.
+__device__ __forceinline__ bool pool_overflowed(const FrameScratch& s) { return s.flags && (*s.flags & 1u); }
+
 __global__ void k_bin_count(FrameDev f, FrameScratch s, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= seg_limit(s, n)) return;
+  if (i >= seg_limit(s, n) || pool_overflowed(s)) return;
   DevSeg g = s.segs[i];
   uint32_t cmd = s.seg_cmd[i];
   if (cmd == 0xFFFFFFFFu) return;
@@ -476,7 +487,7 @@ void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
 
 __global__ void k_bin_scatter(FrameDev f, FrameScratch s, uint32_t n, uint32_t* cursor) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= seg_limit(s, n)) return;
+  if (i >= seg_limit(s, n) || pool_overflowed(s)) return;
   // pool mode: the binned pool is sized ahead of time; a larger total is an
   // overflow (the frame falls back to exact counting)
   if (s.binned_cap && s.mat_off[f.n_mat] > s.binned_cap) {

@@ -29,12 +29,16 @@ def test_reference_gradient_matches_finite_differences():
         assert abs(num - g[i]) <= 1e-6 + 1e-5 * abs(num), (i, num, g[i])
 
 
-def test_pool_sizing_is_switched_off_in_this_build():
+def test_sizing_modes_are_accepted_and_others_rejected():
     import blend2d_amd
-    with pytest.raises(blend2d_amd.BLError) as e:
-        blend2d_amd.set_sizing(blend2d_amd.SIZING_LEARNED)
-    assert e.value.code == 0x10007
-    blend2d_amd.set_sizing(blend2d_amd.SIZING_EXACT)
+    try:
+        for mode in (blend2d_amd.SIZING_LEARNED, blend2d_amd.SIZING_HISTORY, blend2d_amd.SIZING_EXACT):
+            blend2d_amd.set_sizing(mode)
+        with pytest.raises(blend2d_amd.BLError) as e:
+            blend2d_amd.set_sizing(3)
+        assert e.value.code == 0x10001      # BL_ERROR_INVALID_VALUE
+    finally:
+        blend2d_amd.set_sizing(blend2d_amd.SIZING_EXACT)
 
 
 def test_reference_adam_matches_closed_form_first_step():
@@ -134,12 +138,7 @@ def test_training_on_logged_counts_predicts_held_out_counts(amd, trained):
     assert np.median(err[:, 0]) < 0.25 * np.median(err0[:, 0])
 
 
-POOL_SKIP = ("pool-mode sizing (learned / history) is switched off in the GPU build: a run that "
-             "exercised it ended in a device fault whose cause has not been identified yet")
-
-
 @pytest.mark.gpu
-@pytest.mark.skip(reason=POOL_SKIP)
 def test_learned_sizing_keeps_pixels_and_skips_the_counting_pass(amd, trained):
     from blend2d_amd import scene as S
     amd.cost_model_set(trained["params"])
@@ -176,7 +175,6 @@ def test_learned_sizing_keeps_pixels_and_skips_the_counting_pass(amd, trained):
 
 
 @pytest.mark.gpu
-@pytest.mark.skip(reason=POOL_SKIP)
 def test_untrained_model_falls_back_to_exact_counting(amd):
     from blend2d_amd import scene as S
     import cost_model_ref as Rf
