@@ -423,8 +423,10 @@ uint32_t Runner::execute(Stats& stats) {
 uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   auto t0 = std::chrono::steady_clock::now();
   const uint64_t n_mat = f.n_mat;
-  size_t scan_words = scan_tmp_words(std::max<size_t>((size_t)n_mat, f.n_bands) + 1);
-  if (!bounds.reserve((size_t)f.n_cmds * sizeof(CmdBounds)) || !mat_cnt.reserve((n_mat + 1) * 4) ||
+  size_t scan_words = scan_tmp_words(std::max<size_t>({(size_t)n_src, (size_t)n_mat, (size_t)f.n_bands}) + 1);
+  if (!item_cnt.reserve((n_src + 1) * 4) || !item_off.reserve((n_src + 1) * 4) ||
+      !items.reserve(std::max<size_t>(item_cap, 1) * sizeof(EdgeItem)) ||
+      !bounds.reserve((size_t)f.n_cmds * sizeof(CmdBounds)) || !mat_cnt.reserve((n_mat + 1) * 4) ||
       !mat_off.reserve((n_mat + 1) * 4) || !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
       !list.reserve((n_mat + 1) * 16) || !list_style.reserve((n_mat + 1) * 4) || !cursor.reserve((n_mat + 1) * 4) ||
       !scan_tmp.reserve(scan_words * 4) ||
@@ -445,6 +447,9 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   s.sgroup_off = s.tile_off + (f.n_bands + 1);
   s.ggroup_off = s.sgroup_off + (f.n_bands + 1);
   s.tile_px = tile_px.as<unsigned long long>();
+  s.item_cnt = item_cnt.as<uint32_t>();
+  s.item_off = item_off.as<uint32_t>();
+  s.items = items.as<EdgeItem>();
   s.bounds = bounds.as<CmdBounds>();
   s.mat_cnt = mat_cnt.as<uint32_t>();
   s.mat_off = mat_off.as<uint32_t>();
@@ -461,9 +466,12 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   launch_fill_u32(counters.as<uint32_t>(), 0u, 4, stream);
   launch_init_bounds(s.bounds, f.n_cmds, stream);
 
+  launch_edge_items(f, s, stream);
+
   // Pool sizes.  Chunked emission leaves at most one partly used chunk per
-  // source (4 slots for lines, 16 for curves).
-  const uint64_t waste = n_src_lines * 4 + (n_src - n_src_lines) * 16;
+  // work item (4 slots for lines, 8 for curve pieces; item_cap bounds the
+  // items).
+  const uint64_t waste = n_src_lines * 4 + ((uint64_t)item_cap - n_src_lines) * 8;
   uint64_t want_segs, want_bins;
   if (mode == kSizeLearned) {
     cm::FrameModel m;
@@ -494,7 +502,7 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   s.binned = binned.as<DevSeg>();
   s.binned_cap = cap_bins;
 
-  launch_edges_pool(f, s, cap_segs, stream);
+  launch_edges_pool(f, s, cap_segs, item_cap, stream);
 #ifdef B2AMD_EMU_TRACE
   fprintf(stderr, "pool: cap_segs %u cursor %u flags %u want %llu\n", cap_segs, counters.as<uint32_t>()[2], counters.as<uint32_t>()[3], (unsigned long long)want_segs);
 #endif

@@ -84,7 +84,7 @@ void scan_exclusive(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp, 
 void launch_edge_items(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_edges(const FrameDev& f, const FrameScratch& s, bool count, uint32_t cap, hipStream_t st);
 // Single pass into a pool of `cap` segments (pool mode).
-void launch_edges_pool(const FrameDev& f, const FrameScratch& s, uint32_t cap, hipStream_t st);
+void launch_edges_pool(const FrameDev& f, const FrameScratch& s, uint32_t cap, uint32_t item_cap, hipStream_t st);
 // Stage 2: binning of segments into (band, command) cells.
 void launch_bin_count(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, hipStream_t st);
 void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st);

@@ -12,6 +12,8 @@
 //            in submission order, and the tile is stored once.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_types.h"
 #include "edge_device.h"
 #include "kernels.h"
@@ -178,32 +180,6 @@ __device__ __forceinline__ edge::P xform_pt(const CmdXform& m, const double* vtx
   return edge::P{x * m.m00 + y * m.m10 + m.m20, x * m.m01 + y * m.m11 + m.m21};
 }
 
-template<typename Emit>
-__device__ __forceinline__ void build_one(const FrameDev& f, const SrcSeg& sg, Emit& e) {
-  const DevCmd& cmd = f.cmds[sg.cmd];
-  const DevCanvas& cv = f.canvases[cmd.canvas];
-  const CmdXform& m = f.xforms[sg.cmd];
-  edge::Clip clip;
-  clip.x0 = 0.0; clip.y0 = 0.0;
-  clip.x1 = (double)cv.w * 256.0;
-  clip.y1 = (double)cv.h * 256.0;
-  clip.ix0 = (int32_t)clip.x0;
-  clip.ix1 = (int32_t)clip.x1;
-  edge::Builder<Emit> b(e, clip, f.tol_sq);
-  edge::P stack[edge::Builder<Emit>::kLimit * 4];
-  if (sg.kind == kSrcLine) {
-    b.line(xform_pt(m, f.vtx, sg.i0), xform_pt(m, f.vtx, sg.i1));
-  }
-  else if (sg.kind == kSrcQuad) {
-    edge::P c[3] = { xform_pt(m, f.vtx, sg.i0), xform_pt(m, f.vtx, sg.i1), xform_pt(m, f.vtx, sg.i1 + 1) };
-    b.curve(c, 3, stack);
-  }
-  else {
-    edge::P c[4] = { xform_pt(m, f.vtx, sg.i0), xform_pt(m, f.vtx, sg.i1), xform_pt(m, f.vtx, sg.i1 + 1), xform_pt(m, f.vtx, sg.i1 + 2) };
-    b.curve(c, 4, stack);
-  }
-}
-
 // Edge work items: a line, one monotonic piece of a curve, or the border line
 // of a curve lying wholly left or right of the clip box.
 // Splitting curves into their pieces up front gives the flattening kernels
@@ -300,23 +276,53 @@ __device__ __forceinline__ void build_item(const FrameDev& f, const EdgeItem& it
   }
 }
 
+// Per-command bounds of a wave's work items with one set of atomics per run
+// of equal command indices: each lane folds in the lanes d = 1, 2, 4, ...,
+// 32 above it that carry its command (min / max are idempotent, so the
+// overlapping windows are harmless), and the first lane of each run issues
+// the atomics.  Every lane of the wave must call it; lanes without segments
+// pass the neutral bounds, lanes past the item total pass cmd 0xFFFFFFFF.
+// Items of one command are contiguous, which keeps the runs long.
+__device__ __forceinline__ void wave_cmd_bounds(CmdBounds* __restrict__ bounds, uint32_t cmd, int32_t mnx, int32_t mxx,
+                                                int32_t mny, int32_t mxy) {
+  const int lane = (int)(threadIdx.x & 63u);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t oc = __shfl_down(cmd, d, 64);
+    const int32_t a = __shfl_down(mnx, d, 64), b = __shfl_down(mxx, d, 64);
+    const int32_t c = __shfl_down(mny, d, 64), e = __shfl_down(mxy, d, 64);
+    if (lane + d < 64 && oc == cmd) {
+      mnx = a < mnx ? a : mnx;
+      mxx = b > mxx ? b : mxx;
+      mny = c < mny ? c : mny;
+      mxy = e > mxy ? e : mxy;
+    }
+  }
+  const uint32_t prev = __shfl_up(cmd, 1, 64);
+  if ((lane == 0 || prev != cmd) && cmd != 0xFFFFFFFFu && mnx <= mxx) {
+    CmdBounds* bd = &bounds[cmd];
+    atomicMin(&bd->min_x, mnx);
+    atomicMax(&bd->max_x, mxx);
+    atomicMin(&bd->min_y, mny);
+    atomicMax(&bd->max_y, mxy);
+  }
+}
+
 // Counting pass over the work items (slots past the item total get 0, so
 // the scan may run over the host's upper bound `cap`).
 __global__ __launch_bounds__(64) void k_edges_count(FrameDev f, FrameScratch s, uint32_t cap) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cap) return;
-  if (i >= s.item_off[f.n_src]) { s.src_count[i] = 0u; return; }
-  const EdgeItem it = s.items[i];
+  const bool slot = i < cap;
+  const bool active = slot && i < s.item_off[f.n_src];
   CountEmit e;
-  build_item(f, it, e);
-  s.src_count[i] = e.n;
-  if (e.n) {
-    CmdBounds* bd = &s.bounds[it.cmd];
-    atomicMin(&bd->min_x, e.min_x);
-    atomicMax(&bd->max_x, e.max_x);
-    atomicMin(&bd->min_y, e.min_y);
-    atomicMax(&bd->max_y, e.max_y);
+  uint32_t cmd = 0xFFFFFFFFu;
+  if (active) {
+    const EdgeItem it = s.items[i];
+    cmd = it.cmd;
+    build_item(f, it, e);
   }
+  if (slot) s.src_count[i] = e.n;
+  wave_cmd_bounds(s.bounds, cmd, e.min_x, e.max_x, e.min_y, e.max_y);
 }
 
 __global__ __launch_bounds__(64) void k_edges_emit(FrameDev f, FrameScratch s, uint32_t cap) {
@@ -349,25 +355,29 @@ struct ChunkEmit {
     max_y = y1 > max_y ? y1 : max_y;
     if (dead) return;
     if (pos == end) {
-      uint32_t base = atomicAdd(cursor, chunk);
-      if (base > cap || cap - base < chunk) {
-        // Overflow.  A chunk that straddles the end of the pool owns the
-        // slots [base, cap): mark them unused, since the binning kernels
-        // read every slot below min(cursor, cap).
-        for (uint32_t k = base; k < cap; k++) {
-          segs[k] = DevSeg{0, 0, 0, 0};
-          seg_cmd[k] = 0xFFFFFFFFu;
-        }
-        atomicOr(flags, 1u);
-        dead = true;
-        return;
-      }
-      pos = base;
-      end = base + chunk;
+      take(atomicAdd(cursor, chunk));
+      if (dead) return;
     }
     segs[pos] = DevSeg{x0, y0, (int32_t)((uint32_t)x1 | (sign << 31)), y1};
     seg_cmd[pos] = cmd;
     pos++;
+  }
+  // Makes [base, base + chunk) the current chunk.  Past the capacity this is
+  // an overflow; a chunk that straddles the end of the pool owns the slots
+  // [base, cap) and marks them unused, since the binning kernels read every
+  // slot below min(cursor, cap).
+  __device__ __forceinline__ void take(uint32_t base) {
+    if (base > cap || cap - base < chunk) {
+      for (uint32_t k = base; k < cap; k++) {
+        segs[k] = DevSeg{0, 0, 0, 0};
+        seg_cmd[k] = 0xFFFFFFFFu;
+      }
+      atomicOr(flags, 1u);
+      dead = true;
+      return;
+    }
+    pos = base;
+    end = base + chunk;
   }
   __device__ __forceinline__ void finish() {
     for (; pos < end; pos++) {
@@ -377,25 +387,44 @@ struct ChunkEmit {
   }
 };
 
-__global__ __launch_bounds__(64) void k_edges_emit_pool(FrameDev f, FrameScratch s, uint32_t cap) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.n_src) return;
-  SrcSeg sg = f.src[i];
-  ChunkEmit e{s.segs, s.seg_cmd, s.seg_cursor, s.flags, cap, sg.kind == kSrcLine ? 4u : 16u, sg.cmd};
-  build_one(f, sg, e);
-  e.finish();
-  if (e.min_x <= e.max_x) {
-    CmdBounds* bd = &s.bounds[sg.cmd];
-    atomicMin(&bd->min_x, e.min_x);
-    atomicMax(&bd->max_x, e.max_x);
-    atomicMin(&bd->min_y, e.min_y);
-    atomicMax(&bd->max_y, e.max_y);
+// One thread per work item (item_cap is the host's upper bound of the item
+// count); lines and borders take chunks of 4 slots, curve pieces of 8.  The
+// first chunk of every lane comes from one cursor atomic per wave (the
+// cursor is a single address shared by all XCDs, so per-lane atomics on it
+// serialize); later chunks, for items with more segments, are taken one at
+// a time.
+__global__ __launch_bounds__(64) void k_edges_emit_pool(FrameDev f, FrameScratch s, uint32_t cap, uint32_t item_cap) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)(threadIdx.x & 63u);
+  const bool active = i < item_cap && i < s.item_off[f.n_src];
+  EdgeItem it;
+  uint32_t need = 0;
+  if (active) {
+    it = s.items[i];
+    need = (it.kind == kItemQuad || it.kind == kItemCubic) ? 8u : 4u;
   }
+  uint32_t incl = need;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += t;
+  }
+  const uint32_t total = __shfl(incl, 63, 64);
+  uint32_t base = 0;
+  if (lane == 63 && total) base = atomicAdd(s.seg_cursor, total);
+  base = __shfl(base, 63, 64);
+  ChunkEmit e{s.segs, s.seg_cmd, s.seg_cursor, s.flags, cap, need, active ? it.cmd : 0xFFFFFFFFu};
+  if (active) {
+    e.take(base + incl - need);
+    build_item(f, it, e);
+    e.finish();
+  }
+  wave_cmd_bounds(s.bounds, active ? it.cmd : 0xFFFFFFFFu, e.min_x, e.max_x, e.min_y, e.max_y);
 }
 
-void launch_edges_pool(const FrameDev& f, const FrameScratch& s, uint32_t cap, hipStream_t st) {
-  if (!f.n_src) return;
-  hipLaunchKernelGGL(k_edges_emit_pool, dim3((f.n_src + 63) / 64), dim3(64), 0, st, f, s, cap);
+void launch_edges_pool(const FrameDev& f, const FrameScratch& s, uint32_t cap, uint32_t item_cap, hipStream_t st) {
+  if (!f.n_src || !item_cap) return;
+  hipLaunchKernelGGL(k_edges_emit_pool, dim3((item_cap + 63) / 64), dim3(64), 0, st, f, s, cap, item_cap);
 }
 
 void launch_edge_items(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
@@ -437,22 +466,30 @@ __device__ __forceinline__ uint32_t seg_limit(const FrameScratch& s, uint32_t n)
 // redone with exact counting, so binning is skipped.
 __device__ __forceinline__ bool pool_overflowed(const FrameScratch& s) { return s.flags && (*s.flags & 1u); }
 
+// Grid of the binning kernels: one thread per segment slot, at most
+// kBinBlocks workgroups striding over the slots (in pool mode `n` is the pool
+// capacity, an upper bound of the slots used).
+constexpr uint32_t kBinBlocks = 8192;
+inline dim3 bin_grid(uint32_t n) { return dim3(std::min<uint32_t>((n + 255) / 256, kBinBlocks)); }
+
 __global__ void k_bin_count(FrameDev f, FrameScratch s, uint32_t n) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= seg_limit(s, n) || pool_overflowed(s)) return;
-  DevSeg g = s.segs[i];
-  uint32_t cmd = s.seg_cmd[i];
-  if (cmd == 0xFFFFFFFFu) return;
-  const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
-  int b0, b1;
-  seg_bands(g, (int)cv.bands, b0, b1);
-  uint64_t row = cv.mat_base + (uint64_t)(cmd - cv.cmd_base);
-  for (int b = b0; b <= b1; b++) atomicAdd(&s.mat_cnt[row + (uint64_t)b * cv.n_cmds], 1u);
+  const uint32_t lim = seg_limit(s, n);
+  if (pool_overflowed(s)) return;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lim; i += gridDim.x * blockDim.x) {
+    const DevSeg g = s.segs[i];
+    const uint32_t cmd = s.seg_cmd[i];
+    if (cmd == 0xFFFFFFFFu) continue;
+    const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
+    int b0, b1;
+    seg_bands(g, (int)cv.bands, b0, b1);
+    const uint64_t row = cv.mat_base + (uint64_t)(cmd - cv.cmd_base);
+    for (int b = b0; b <= b1; b++) atomicAdd(&s.mat_cnt[row + (uint64_t)b * cv.n_cmds], 1u);
+  }
 }
 
 void launch_bin_count(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, hipStream_t st) {
   if (!n_segs) return;
-  hipLaunchKernelGGL(k_bin_count, dim3((n_segs + 255) / 256), dim3(256), 0, st, f, s, n_segs);
+  hipLaunchKernelGGL(k_bin_count, bin_grid(n_segs), dim3(256), 0, st, f, s, n_segs);
 }
 
 __global__ void k_presence(FrameScratch s, uint64_t n) {
@@ -486,30 +523,32 @@ void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
 }
 
 __global__ void k_bin_scatter(FrameDev f, FrameScratch s, uint32_t n, uint32_t* cursor) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= seg_limit(s, n) || pool_overflowed(s)) return;
+  const uint32_t lim = seg_limit(s, n);
+  if (pool_overflowed(s)) return;
   // pool mode: the binned pool is sized ahead of time; a larger total is an
   // overflow (the frame falls back to exact counting)
   if (s.binned_cap && s.mat_off[f.n_mat] > s.binned_cap) {
-    if (i == 0) atomicOr(s.flags, 2u);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(s.flags, 2u);
     return;
   }
-  DevSeg g = s.segs[i];
-  uint32_t cmd = s.seg_cmd[i];
-  if (cmd == 0xFFFFFFFFu) return;
-  const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
-  int b0, b1;
-  seg_bands(g, (int)cv.bands, b0, b1);
-  uint64_t row = cv.mat_base + (uint64_t)(cmd - cv.cmd_base);
-  for (int b = b0; b <= b1; b++) {
-    uint32_t pos = atomicAdd(&cursor[row + (uint64_t)b * cv.n_cmds], 1u);
-    s.binned[pos] = g;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lim; i += gridDim.x * blockDim.x) {
+    const DevSeg g = s.segs[i];
+    const uint32_t cmd = s.seg_cmd[i];
+    if (cmd == 0xFFFFFFFFu) continue;
+    const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
+    int b0, b1;
+    seg_bands(g, (int)cv.bands, b0, b1);
+    const uint64_t row = cv.mat_base + (uint64_t)(cmd - cv.cmd_base);
+    for (int b = b0; b <= b1; b++) {
+      const uint32_t pos = atomicAdd(&cursor[row + (uint64_t)b * cv.n_cmds], 1u);
+      s.binned[pos] = g;
+    }
   }
 }
 
 void launch_bin_scatter(const FrameDev& f, const FrameScratch& s, uint32_t n_segs, uint32_t* cursor, hipStream_t st) {
   if (!n_segs) return;
-  hipLaunchKernelGGL(k_bin_scatter, dim3((n_segs + 255) / 256), dim3(256), 0, st, f, s, n_segs, cursor);
+  hipLaunchKernelGGL(k_bin_scatter, bin_grid(n_segs), dim3(256), 0, st, f, s, n_segs, cursor);
 }
 
 // Command lists of the bands: every present (band, command) cell becomes one

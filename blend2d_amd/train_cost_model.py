@@ -4,7 +4,8 @@
 
 Samples: every fill command of seeded cost-model canvases (1 to 128 paths
 of Pareto-distributed segment counts, lines / quads / cubics, on canvases of
-256-2048 px), rendered with exact counting; per command the 16 features and
+256-2048 px) and of glyph canvases (1-200 quadratic contours of 20-80
+segments on canvases of 32-256 px), rendered with exact counting; per command the 16 features and
 the exact targets log2(1 + segments), log2(1 + (segment, band) copies) come
 from bl_amd_context_collect_costs.  Training is Adam on mini-batches with
 the library's own forward / backward kernels.  A held-out set reports the
@@ -39,22 +40,27 @@ def init_params(seed: int = 0) -> np.ndarray:
 
 def collect(seeds, count: int, side_lo: int, side_hi: int, paths: int, max_segments: int):
     """(features, targets) of every fill command of the cost canvases."""
+    return collect_scenes(sc for seed in seeds
+                          for sc in S.cost_canvases(seed, count, side_lo, side_hi, paths, max_segments))
+
+
+def collect_scenes(scenes):
+    """(features, targets) of every fill command of `scenes`, one context each."""
     xs, ts = [], []
-    for seed in seeds:
-        for sc in S.cost_canvases(seed, count, side_lo, side_hi, paths, max_segments):
-            images = []
-            for fmt, px in sc.images:
-                im = B.BLImage(px.shape[1], px.shape[0], fmt)
-                im.upload(px)
-                images.append(im)
-            img = B.BLImage(sc.width, sc.height, sc.format)
-            ctx = B.BLContext(img)
-            B.replay_into(ctx, sc, images)
-            ctx.flush()
-            X, T = ctx.collect_costs()
-            ctx.end()
-            xs.append(X)
-            ts.append(T)
+    for sc in scenes:
+        images = []
+        for fmt, px in sc.images:
+            im = B.BLImage(px.shape[1], px.shape[0], fmt)
+            im.upload(px)
+            images.append(im)
+        img = B.BLImage(sc.width, sc.height, sc.format)
+        ctx = B.BLContext(img)
+        B.replay_into(ctx, sc, images)
+        ctx.flush()
+        X, T = ctx.collect_costs()
+        ctx.end()
+        xs.append(X)
+        ts.append(T)
     return np.concatenate(xs), np.concatenate(ts)
 
 
@@ -82,8 +88,14 @@ def main():
     args = ap.parse_args()
 
     t0 = time.time()
+    # cost canvases (Pareto segment counts, every source kind) and glyph
+    # canvases (many small quadratic contours on small canvases)
     X, T = collect(range(100, 100 + args.seeds), args.canvases, 256, 2048, 128, 20000)
+    Xg, Tg = collect_scenes(sc for seed in range(200, 200 + args.seeds) for sc in S.glyph_canvases(seed, args.canvases, 200))
+    X, T = np.concatenate([X, Xg]), np.concatenate([T, Tg])
     Xv, Tv = collect([999], 32, 256, 2048, 128, 20000)
+    Xgv, Tgv = collect_scenes(S.glyph_canvases(998, 32, 200))
+    Xv, Tv = np.concatenate([Xv, Xgv]), np.concatenate([Tv, Tgv])
     print(f"samples: {len(X)} train, {len(Xv)} held out ({time.time() - t0:.1f} s)", flush=True)
     t0 = time.time()
     params, losses = train(X, T, args.epochs, args.batch, args.lr)
