@@ -470,8 +470,8 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
 
   // Pool sizes.  Chunked emission leaves at most one partly used chunk per
   // work item (4 slots for lines, 8 for curve pieces; item_cap bounds the
-  // items).
-  const uint64_t waste = n_src_lines * 4 + ((uint64_t)item_cap - n_src_lines) * 8;
+  // items) plus a spare region of 256 slots per wave of 64 items.
+  const uint64_t waste = n_src_lines * 4 + ((uint64_t)item_cap - n_src_lines) * 8 + ((uint64_t)item_cap + 63) / 64 * 256;
   uint64_t want_segs, want_bins;
   if (mode == kSizeLearned) {
     cm::FrameModel m;
@@ -492,10 +492,12 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
     want_segs = hist_segs + hist_segs / 4 + waste + 1024;
     want_bins = hist_bins + hist_bins / 4 + 1024;
   }
-  if (want_segs > 0xFFFFF000ull || want_bins > 0xFFFFF000ull) return 1;
+  // (the cursor may run past the capacity by up to the slots of the
+  // allocations that fail, so the capacity stays below 2^31)
+  if (want_segs > 0x7FFFF000ull || want_bins > 0xFFFFF000ull) return 1;
   if (!segs.reserve(want_segs * sizeof(DevSeg)) || !seg_cmd.reserve(want_segs * 4) || !binned.reserve(want_bins * sizeof(DevSeg)))
     return kErrOOM;
-  const uint32_t cap_segs = (uint32_t)std::min<size_t>(std::min(segs.cap / sizeof(DevSeg), seg_cmd.cap / 4), 0xFFFFF000ull);
+  const uint32_t cap_segs = (uint32_t)std::min<size_t>(std::min(segs.cap / sizeof(DevSeg), seg_cmd.cap / 4), 0x7FFFF000ull);
   const uint32_t cap_bins = (uint32_t)std::min<size_t>(binned.cap / sizeof(DevSeg), 0xFFFFF000ull);
   s.segs = segs.as<DevSeg>();
   s.seg_cmd = seg_cmd.as<uint32_t>();

@@ -36,55 +36,118 @@ __device__ __forceinline__ uint32_t px16(double fixed_len) {
   return v > 4.0e9 ? 4000000000u : (uint32_t)v;
 }
 
-__global__ void k_features_acc(FrameDev f, FrameModel m) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.n_src) return;
-  const SrcSeg sg = f.src[i];
-  const DevCmd& cmd = f.cmds[sg.cmd];
-  if (cmd.kind != kCmdAnalytic) return;
-  const DevCanvas& cv = f.canvases[cmd.canvas];
-  const CmdXform& x = f.xforms[sg.cmd];
-  Pt p[4];
-  int np = sg.kind == kSrcLine ? 2 : sg.kind == kSrcQuad ? 3 : 4;
-  p[0] = fx_pt(x, f.vtx, sg.i0);
-  p[1] = fx_pt(x, f.vtx, sg.i1);
-  if (np > 2) p[2] = fx_pt(x, f.vtx, sg.i1 + 1);
-  if (np > 3) p[3] = fx_pt(x, f.vtx, sg.i1 + 2);
-  double x0 = p[0].x, x1 = p[0].x, y0 = p[0].y, y1 = p[0].y, poly = 0.0;
-  bool finite = true;
-  for (int k = 0; k < np; k++) {
-    finite = finite && isfinite(p[k].x) && isfinite(p[k].y);
-    x0 = fmin(x0, p[k].x); x1 = fmax(x1, p[k].x);
-    y0 = fmin(y0, p[k].y); y1 = fmax(y1, p[k].y);
-    if (k) poly += dist(p[k - 1], p[k]);
-  }
-  if (!finite) return;
-  double dd = 0.0;
-  if (np == 3) dd = dist(Pt{p[0].x - 2.0 * p[1].x + p[2].x, p[0].y - 2.0 * p[1].y + p[2].y}, Pt{0.0, 0.0});
-  if (np == 4) {
-    double a = dist(Pt{p[0].x - 2.0 * p[1].x + p[2].x, p[0].y - 2.0 * p[1].y + p[2].y}, Pt{0.0, 0.0});
-    double b = dist(Pt{p[1].x - 2.0 * p[2].x + p[3].x, p[1].y - 2.0 * p[2].y + p[3].y}, Pt{0.0, 0.0});
-    dd = fmax(a, b);
-  }
-  const double W = (double)cv.w * 256.0, H = (double)cv.h * 256.0;
-  const bool vis = y1 > 0.0 && y0 < H && y1 > y0;
-  const uint32_t flat = px16(sqrt(dd * 256.0));   // sqrt of |2nd difference| in px, x16
+__device__ __forceinline__ unsigned long long shfl_down_u64(unsigned long long v, int d) {
+  const uint32_t lo = __shfl_down((uint32_t)v, d, 64), hi = __shfl_down((uint32_t)(v >> 32), d, 64);
+  return ((unsigned long long)hi << 32) | lo;
+}
 
-  CmdAcc* a = &m.acc[sg.cmd];
-  atomicAdd(np == 2 ? &a->n_line : np == 3 ? &a->n_quad : &a->n_cubic, 1u);
-  if (vis) {
-    atomicAdd(&a->n_vis, 1u);
-    if (x0 < 0.0) atomicAdd(&a->n_left, 1u);
-    if (x1 > W) atomicAdd(&a->n_right, 1u);
-    atomicAdd(&a->flat_vis, (unsigned long long)flat);
+// Per-source feature terms, summed per command.  The lanes of a wave first
+// reduce the runs of equal command indices among themselves (sources of a
+// command are contiguous), then the first lane of each run adds the run's
+// totals with one set of atomics: integer sums and min / max, so the result
+// does not depend on the grouping.
+__global__ void k_features_acc(FrameDev f, FrameModel m) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)(threadIdx.x & 63u);
+  uint32_t key = 0xFFFFFFFFu;
+  uint32_t cnt[6] = {0, 0, 0, 0, 0, 0};          // line, quad, cubic, vis, left, right
+  unsigned long long sum[4] = {0, 0, 0, 0};      // chord, poly, flat, flat_vis
+  int32_t bx0 = 1073741824, by0 = 1073741824, bx1 = -1073741824, by1 = -1073741824;
+  if (i < f.n_src) {
+    const SrcSeg sg = f.src[i];
+    const DevCmd& cmd = f.cmds[sg.cmd];
+    if (cmd.kind == kCmdAnalytic) {
+      const DevCanvas& cv = f.canvases[cmd.canvas];
+      const CmdXform& x = f.xforms[sg.cmd];
+      Pt p[4];
+      const int np = sg.kind == kSrcLine ? 2 : sg.kind == kSrcQuad ? 3 : 4;
+      p[0] = fx_pt(x, f.vtx, sg.i0);
+      p[1] = fx_pt(x, f.vtx, sg.i1);
+      if (np > 2) p[2] = fx_pt(x, f.vtx, sg.i1 + 1);
+      if (np > 3) p[3] = fx_pt(x, f.vtx, sg.i1 + 2);
+      double x0 = p[0].x, x1 = p[0].x, y0 = p[0].y, y1 = p[0].y, poly = 0.0;
+      bool finite = true;
+      for (int k = 0; k < np; k++) {
+        finite = finite && isfinite(p[k].x) && isfinite(p[k].y);
+        x0 = fmin(x0, p[k].x); x1 = fmax(x1, p[k].x);
+        y0 = fmin(y0, p[k].y); y1 = fmax(y1, p[k].y);
+        if (k) poly += dist(p[k - 1], p[k]);
+      }
+      if (finite) {
+        double dd = 0.0;
+        if (np == 3) dd = dist(Pt{p[0].x - 2.0 * p[1].x + p[2].x, p[0].y - 2.0 * p[1].y + p[2].y}, Pt{0.0, 0.0});
+        if (np == 4) {
+          double a = dist(Pt{p[0].x - 2.0 * p[1].x + p[2].x, p[0].y - 2.0 * p[1].y + p[2].y}, Pt{0.0, 0.0});
+          double b = dist(Pt{p[1].x - 2.0 * p[2].x + p[3].x, p[1].y - 2.0 * p[2].y + p[3].y}, Pt{0.0, 0.0});
+          dd = fmax(a, b);
+        }
+        const double W = (double)cv.w * 256.0, H = (double)cv.h * 256.0;
+        const bool vis = y1 > 0.0 && y0 < H && y1 > y0;
+        const uint32_t flat = px16(sqrt(dd * 256.0));   // sqrt of |2nd difference| in px, x16
+        key = sg.cmd;
+        cnt[np - 2] = 1u;
+        if (vis) {
+          cnt[3] = 1u;
+          cnt[4] = x0 < 0.0 ? 1u : 0u;
+          cnt[5] = x1 > W ? 1u : 0u;
+          sum[3] = flat;
+        }
+        sum[0] = px16(dist(p[0], p[np - 1]));
+        sum[1] = px16(poly);
+        sum[2] = flat;
+        bx0 = clamp_fx(floor(x0));
+        by0 = clamp_fx(floor(y0));
+        bx1 = clamp_fx(ceil(x1));
+        by1 = clamp_fx(ceil(y1));
+      }
+    }
   }
-  atomicAdd(&a->chord, (unsigned long long)px16(dist(p[0], p[np - 1])));
-  atomicAdd(&a->poly, (unsigned long long)px16(poly));
-  atomicAdd(&a->flat, (unsigned long long)flat);
-  atomicMin(&a->bx0, clamp_fx(floor(x0)));
-  atomicMin(&a->by0, clamp_fx(floor(y0)));
-  atomicMax(&a->bx1, clamp_fx(ceil(x1)));
-  atomicMax(&a->by1, clamp_fx(ceil(y1)));
+  // runs of equal keys: run index = number of run starts up to this lane
+  const uint32_t prev = __shfl_up(key, 1, 64);   // every lane takes part in the shuffle
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const uint32_t run = (uint32_t)__popcll(heads & ((2ull << lane) - 1ull));
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t orun = __shfl_down(run, d, 64);
+    const bool take = lane + d < 64 && orun == run;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const uint32_t o = __shfl_down(cnt[k], d, 64);
+      if (take) cnt[k] += o;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const unsigned long long o = shfl_down_u64(sum[k], d);
+      if (take) sum[k] += o;
+    }
+    const int32_t a = __shfl_down(bx0, d, 64), b = __shfl_down(by0, d, 64);
+    const int32_t c = __shfl_down(bx1, d, 64), e = __shfl_down(by1, d, 64);
+    if (take) {
+      bx0 = a < bx0 ? a : bx0;
+      by0 = b < by0 ? b : by0;
+      bx1 = c > bx1 ? c : bx1;
+      by1 = e > by1 ? e : by1;
+    }
+  }
+  if (!head || key == 0xFFFFFFFFu) return;
+  CmdAcc* acc = &m.acc[key];
+  if (cnt[0]) atomicAdd(&acc->n_line, cnt[0]);
+  if (cnt[1]) atomicAdd(&acc->n_quad, cnt[1]);
+  if (cnt[2]) atomicAdd(&acc->n_cubic, cnt[2]);
+  if (cnt[3]) {
+    atomicAdd(&acc->n_vis, cnt[3]);
+    if (cnt[4]) atomicAdd(&acc->n_left, cnt[4]);
+    if (cnt[5]) atomicAdd(&acc->n_right, cnt[5]);
+    atomicAdd(&acc->flat_vis, sum[3]);
+  }
+  atomicAdd(&acc->chord, sum[0]);
+  atomicAdd(&acc->poly, sum[1]);
+  atomicAdd(&acc->flat, sum[2]);
+  atomicMin(&acc->bx0, bx0);
+  atomicMin(&acc->by0, by0);
+  atomicMax(&acc->bx1, bx1);
+  atomicMax(&acc->by1, by1);
 }
 
 __global__ void k_features_init(FrameDev f, FrameModel m) {

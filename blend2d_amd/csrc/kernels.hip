@@ -344,6 +344,11 @@ struct ChunkEmit {
   uint32_t* cursor;
   uint32_t* flags;
   uint32_t cap, chunk, cmd;
+  // optional wave-local spare region [.., *spare_end) handed out by an LDS
+  // cursor; *spare_hi is the end of the slots handed out from it
+  uint32_t* spare_pos = nullptr;
+  uint32_t* spare_hi = nullptr;
+  uint32_t spare_end = 0;
   uint32_t pos = 0, end = 0;
   bool dead = false;
   int32_t min_x = INT32_MAX, max_x = INT32_MIN, min_y = INT32_MAX, max_y = INT32_MIN;
@@ -355,7 +360,13 @@ struct ChunkEmit {
     max_y = y1 > max_y ? y1 : max_y;
     if (dead) return;
     if (pos == end) {
-      take(atomicAdd(cursor, chunk));
+      uint32_t p = 0xFFFFFFFFu;
+      if (spare_pos) {
+        p = atomicAdd(spare_pos, chunk);
+        if (p <= spare_end && spare_end - p >= chunk) atomicMax(spare_hi, p + chunk);
+        else p = 0xFFFFFFFFu;
+      }
+      take(p != 0xFFFFFFFFu ? p : atomicAdd(cursor, chunk));
       if (dead) return;
     }
     segs[pos] = DevSeg{x0, y0, (int32_t)((uint32_t)x1 | (sign << 31)), y1};
@@ -387,6 +398,9 @@ struct ChunkEmit {
   }
 };
 
+// Slots of the wave-local spare region of the pool emitter.
+constexpr uint32_t kPoolSpare = 256;
+
 // One thread per work item (item_cap is the host's upper bound of the item
 // count); lines and borders take chunks of 4 slots, curve pieces of 8.  The
 // first chunk of every lane comes from one cursor atomic per wave (the
@@ -394,6 +408,7 @@ struct ChunkEmit {
 // serialize); later chunks, for items with more segments, are taken one at
 // a time.
 __global__ __launch_bounds__(64) void k_edges_emit_pool(FrameDev f, FrameScratch s, uint32_t cap, uint32_t item_cap) {
+  __shared__ uint32_t spare_pos, spare_hi;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = (int)(threadIdx.x & 63u);
   const bool active = i < item_cap && i < s.item_off[f.n_src];
@@ -410,16 +425,30 @@ __global__ __launch_bounds__(64) void k_edges_emit_pool(FrameDev f, FrameScratch
     if (lane >= d) incl += t;
   }
   const uint32_t total = __shfl(incl, 63, 64);
+  // one cursor atomic per wave: every lane's first chunk plus a spare region
+  // from which later chunks are handed out by an LDS cursor
+  const uint32_t spare = total ? kPoolSpare : 0u;
   uint32_t base = 0;
-  if (lane == 63 && total) base = atomicAdd(s.seg_cursor, total);
+  if (lane == 63 && total) base = atomicAdd(s.seg_cursor, total + spare);
   base = __shfl(base, 63, 64);
-  ChunkEmit e{s.segs, s.seg_cmd, s.seg_cursor, s.flags, cap, need, active ? it.cmd : 0xFFFFFFFFu};
+  if (lane == 0) { spare_pos = base + total; spare_hi = base + total; }
+  __syncthreads();
+  ChunkEmit e{s.segs, s.seg_cmd, s.seg_cursor, s.flags, cap, need, active ? it.cmd : 0xFFFFFFFFu,
+              &spare_pos, &spare_hi, base + total + spare};
   if (active) {
     e.take(base + incl - need);
     build_item(f, it, e);
     e.finish();
   }
   wave_cmd_bounds(s.bounds, active ? it.cmd : 0xFFFFFFFFu, e.min_x, e.max_x, e.min_y, e.max_y);
+  // the spare slots nobody took are marked unused (binning reads every slot
+  // below min(cursor, cap))
+  __syncthreads();
+  const uint32_t hi = spare_hi, lim = base + total + spare < cap ? base + total + spare : cap;
+  for (uint32_t k = hi + (uint32_t)lane; k < lim; k += 64u) {
+    s.segs[k] = DevSeg{0, 0, 0, 0};
+    s.seg_cmd[k] = 0xFFFFFFFFu;
+  }
 }
 
 void launch_edges_pool(const FrameDev& f, const FrameScratch& s, uint32_t cap, uint32_t item_cap, hipStream_t st) {
