@@ -468,10 +468,7 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
 
   launch_edge_items(f, s, stream);
 
-  // Pool sizes.  Chunked emission leaves at most one partly used chunk per
-  // work item (4 slots for lines, 8 for curve pieces; item_cap bounds the
-  // items) plus a spare region of 256 slots per wave of 64 items.
-  const uint64_t waste = n_src_lines * 4 + ((uint64_t)item_cap - n_src_lines) * 8 + ((uint64_t)item_cap + 63) / 64 * 256;
+  // Pool sizes: the emitter writes segments densely (exact slots per item).
   uint64_t want_segs, want_bins;
   if (mode == kSizeLearned) {
     cm::FrameModel m;
@@ -485,11 +482,11 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
     memcpy(tot, host_readback + 4, 16);
     stats.pred_segs = tot[0];
     stats.pred_bins = tot[1];
-    want_segs = tot[0] + tot[0] / 4 + waste + 1024;
+    want_segs = tot[0] + tot[0] / 4 + 1024;
     want_bins = tot[1] + tot[1] / 4 + 1024;
   }
   else {
-    want_segs = hist_segs + hist_segs / 4 + waste + 1024;
+    want_segs = hist_segs + hist_segs / 4 + 1024;
     want_bins = hist_bins + hist_bins / 4 + 1024;
   }
   // (the cursor may run past the capacity by up to the slots of the

@@ -333,91 +333,25 @@ __global__ __launch_bounds__(64) void k_edges_emit(FrameDev f, FrameScratch s, u
   build_item(f, it, e);
 }
 
-// Single-pass emission into a pool sized ahead of time: each thread takes
-// chunks of slots from a global cursor, so the number of atomics is about
-// segments / chunk.  Unused slots of a chunk are marked with command
-// index 0xFFFFFFFF, which binning skips.  Reaching the pool capacity raises the
-// overflow flag; the frame is then redone with exact counting.
-struct ChunkEmit {
-  DevSeg* segs;
-  uint32_t* seg_cmd;
-  uint32_t* cursor;
-  uint32_t* flags;
-  uint32_t cap, chunk, cmd;
-  // optional wave-local spare region [.., *spare_end) handed out by an LDS
-  // cursor; *spare_hi is the end of the slots handed out from it
-  uint32_t* spare_pos = nullptr;
-  uint32_t* spare_hi = nullptr;
-  uint32_t spare_end = 0;
-  uint32_t pos = 0, end = 0;
-  bool dead = false;
-  int32_t min_x = INT32_MAX, max_x = INT32_MIN, min_y = INT32_MAX, max_y = INT32_MIN;
-  __device__ __forceinline__ void push(int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint32_t sign) {
-    int32_t lo = (x0 < x1 ? x0 : x1) >> 8, hi = (x0 < x1 ? x1 : x0) >> 8;
-    min_x = lo < min_x ? lo : min_x;
-    max_x = hi > max_x ? hi : max_x;
-    min_y = y0 < min_y ? y0 : min_y;
-    max_y = y1 > max_y ? y1 : max_y;
-    if (dead) return;
-    if (pos == end) {
-      uint32_t p = 0xFFFFFFFFu;
-      if (spare_pos) {
-        p = atomicAdd(spare_pos, chunk);
-        if (p <= spare_end && spare_end - p >= chunk) atomicMax(spare_hi, p + chunk);
-        else p = 0xFFFFFFFFu;
-      }
-      take(p != 0xFFFFFFFFu ? p : atomicAdd(cursor, chunk));
-      if (dead) return;
-    }
-    segs[pos] = DevSeg{x0, y0, (int32_t)((uint32_t)x1 | (sign << 31)), y1};
-    seg_cmd[pos] = cmd;
-    pos++;
-  }
-  // Makes [base, base + chunk) the current chunk.  Past the capacity this is
-  // an overflow; a chunk that straddles the end of the pool owns the slots
-  // [base, cap) and marks them unused, since the binning kernels read every
-  // slot below min(cursor, cap).
-  __device__ __forceinline__ void take(uint32_t base) {
-    if (base > cap || cap - base < chunk) {
-      for (uint32_t k = base; k < cap; k++) {
-        segs[k] = DevSeg{0, 0, 0, 0};
-        seg_cmd[k] = 0xFFFFFFFFu;
-      }
-      atomicOr(flags, 1u);
-      dead = true;
-      return;
-    }
-    pos = base;
-    end = base + chunk;
-  }
-  __device__ __forceinline__ void finish() {
-    for (; pos < end; pos++) {
-      segs[pos] = DevSeg{0, 0, 0, 0};
-      seg_cmd[pos] = 0xFFFFFFFFu;   // unused slot
-    }
-  }
-};
-
-// Slots of the wave-local spare region of the pool emitter.
-constexpr uint32_t kPoolSpare = 256;
-
-// One thread per work item (item_cap is the host's upper bound of the item
-// count); lines and borders take chunks of 4 slots, curve pieces of 8.  The
-// first chunk of every lane comes from one cursor atomic per wave (the
-// cursor is a single address shared by all XCDs, so per-lane atomics on it
-// serialize); later chunks, for items with more segments, are taken one at
-// a time.
+// Single pass into a pool sized ahead of time: every work item is flattened
+// twice in the same thread, first counting its segments (and their bounds),
+// then writing them at a position taken from one cursor atomic per wave
+// (the cursor is one address shared by all XCDs, so per-lane atomics on it
+// would serialize).  An item whose slots do not fit below the capacity
+// raises the overflow flag and marks the slots it owns below the capacity
+// unused, since binning reads every slot below min(cursor, capacity); the
+// frame is then redone with exact counting.
 __global__ __launch_bounds__(64) void k_edges_emit_pool(FrameDev f, FrameScratch s, uint32_t cap, uint32_t item_cap) {
-  __shared__ uint32_t spare_pos, spare_hi;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = (int)(threadIdx.x & 63u);
   const bool active = i < item_cap && i < s.item_off[f.n_src];
   EdgeItem it;
-  uint32_t need = 0;
+  CountEmit c;
   if (active) {
     it = s.items[i];
-    need = (it.kind == kItemQuad || it.kind == kItemCubic) ? 8u : 4u;
+    build_item(f, it, c);
   }
+  const uint32_t need = c.n;
   uint32_t incl = need;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -425,30 +359,24 @@ __global__ __launch_bounds__(64) void k_edges_emit_pool(FrameDev f, FrameScratch
     if (lane >= d) incl += t;
   }
   const uint32_t total = __shfl(incl, 63, 64);
-  // one cursor atomic per wave: every lane's first chunk plus a spare region
-  // from which later chunks are handed out by an LDS cursor
-  const uint32_t spare = total ? kPoolSpare : 0u;
   uint32_t base = 0;
-  if (lane == 63 && total) base = atomicAdd(s.seg_cursor, total + spare);
+  if (lane == 63 && total) base = atomicAdd(s.seg_cursor, total);
   base = __shfl(base, 63, 64);
-  if (lane == 0) { spare_pos = base + total; spare_hi = base + total; }
-  __syncthreads();
-  ChunkEmit e{s.segs, s.seg_cmd, s.seg_cursor, s.flags, cap, need, active ? it.cmd : 0xFFFFFFFFu,
-              &spare_pos, &spare_hi, base + total + spare};
-  if (active) {
-    e.take(base + incl - need);
-    build_item(f, it, e);
-    e.finish();
+  if (need) {
+    const uint32_t start = base + incl - need;
+    if (start <= cap && cap - start >= need) {
+      WriteEmit e{s.segs, s.seg_cmd, start, it.cmd};
+      build_item(f, it, e);
+    }
+    else {
+      for (uint32_t k = start; k < cap; k++) {
+        s.segs[k] = DevSeg{0, 0, 0, 0};
+        s.seg_cmd[k] = 0xFFFFFFFFu;
+      }
+      atomicOr(s.flags, 1u);
+    }
   }
-  wave_cmd_bounds(s.bounds, active ? it.cmd : 0xFFFFFFFFu, e.min_x, e.max_x, e.min_y, e.max_y);
-  // the spare slots nobody took are marked unused (binning reads every slot
-  // below min(cursor, cap))
-  __syncthreads();
-  const uint32_t hi = spare_hi, lim = base + total + spare < cap ? base + total + spare : cap;
-  for (uint32_t k = hi + (uint32_t)lane; k < lim; k += 64u) {
-    s.segs[k] = DevSeg{0, 0, 0, 0};
-    s.seg_cmd[k] = 0xFFFFFFFFu;
-  }
+  wave_cmd_bounds(s.bounds, active ? it.cmd : 0xFFFFFFFFu, c.min_x, c.max_x, c.min_y, c.max_y);
 }
 
 void launch_edges_pool(const FrameDev& f, const FrameScratch& s, uint32_t cap, uint32_t item_cap, hipStream_t st) {
