@@ -429,18 +429,50 @@ __device__ __forceinline__ bool pool_overflowed(const FrameScratch& s) { return 
 constexpr uint32_t kBinBlocks = 8192;
 inline dim3 bin_grid(uint32_t n) { return dim3(std::min<uint32_t>((n + 255) / 256, kBinBlocks)); }
 
+// Runs of equal keys among the lanes of a wave (key ~0 = no key).  Each lane
+// gets the lane index of its run's first lane and the run's length there.
+struct WaveRun {
+  uint32_t head;   // lane index of the first lane of this lane's run
+  uint32_t len;    // run length (meaningful in the first lane)
+  bool first;
+};
+__device__ __forceinline__ WaveRun wave_run(uint64_t key) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t plo = __shfl_up((uint32_t)key, 1, 64), phi = __shfl_up((uint32_t)(key >> 32), 1, 64);
+  const bool first = lane == 0 || (((uint64_t)phi << 32) | plo) != key;
+  const unsigned long long heads = __ballot(first);
+  WaveRun r;
+  r.first = first;
+  r.head = 63u - (uint32_t)__builtin_clzll(heads & ((2ull << lane) - 1ull));
+  const unsigned long long above = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1ull);
+  r.len = (above ? (uint32_t)__builtin_ctzll(above) : 64u) - lane;
+  return r;
+}
+
+// Segments that stay within one band (the common case) are counted per run
+// of equal (band, command) cells among the lanes of a wave, one atomic per
+// run; segments crossing bands add to each of their cells.
 __global__ void k_bin_count(FrameDev f, FrameScratch s, uint32_t n) {
   const uint32_t lim = seg_limit(s, n);
   if (pool_overflowed(s)) return;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lim; i += gridDim.x * blockDim.x) {
-    const DevSeg g = s.segs[i];
-    const uint32_t cmd = s.seg_cmd[i];
-    if (cmd == 0xFFFFFFFFu) continue;
-    const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
-    int b0, b1;
-    seg_bands(g, (int)cv.bands, b0, b1);
-    const uint64_t row = cv.mat_base + (uint64_t)(cmd - cv.cmd_base);
-    for (int b = b0; b <= b1; b++) atomicAdd(&s.mat_cnt[row + (uint64_t)b * cv.n_cmds], 1u);
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < lim; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;
+    uint64_t key = ~0ull;
+    if (i < lim) {
+      const DevSeg g = s.segs[i];
+      const uint32_t cmd = s.seg_cmd[i];
+      if (cmd != 0xFFFFFFFFu) {
+        const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
+        int b0, b1;
+        seg_bands(g, (int)cv.bands, b0, b1);
+        const uint64_t row = cv.mat_base + (uint64_t)(cmd - cv.cmd_base);
+        if (b0 == b1) key = row + (uint64_t)b0 * cv.n_cmds;
+        else
+          for (int b = b0; b <= b1; b++) atomicAdd(&s.mat_cnt[row + (uint64_t)b * cv.n_cmds], 1u);
+      }
+    }
+    const WaveRun r = wave_run(key);
+    if (r.first && key != ~0ull) atomicAdd(&s.mat_cnt[key], r.len);
   }
 }
 
@@ -479,6 +511,8 @@ void launch_presence(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
   if (f.n_cmds) hipLaunchKernelGGL(k_box_presence, dim3((unsigned)(((uint64_t)f.n_cmds * 16 + 255) / 256)), dim3(256), 0, st, f, s);
 }
 
+// As k_bin_count: a run of lanes with the same single-band cell takes its
+// slots with one cursor atomic; lane k of the run writes slot base + k.
 __global__ void k_bin_scatter(FrameDev f, FrameScratch s, uint32_t n, uint32_t* cursor) {
   const uint32_t lim = seg_limit(s, n);
   if (pool_overflowed(s)) return;
@@ -488,18 +522,32 @@ __global__ void k_bin_scatter(FrameDev f, FrameScratch s, uint32_t n, uint32_t* 
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(s.flags, 2u);
     return;
   }
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lim; i += gridDim.x * blockDim.x) {
-    const DevSeg g = s.segs[i];
-    const uint32_t cmd = s.seg_cmd[i];
-    if (cmd == 0xFFFFFFFFu) continue;
-    const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
-    int b0, b1;
-    seg_bands(g, (int)cv.bands, b0, b1);
-    const uint64_t row = cv.mat_base + (uint64_t)(cmd - cv.cmd_base);
-    for (int b = b0; b <= b1; b++) {
-      const uint32_t pos = atomicAdd(&cursor[row + (uint64_t)b * cv.n_cmds], 1u);
-      s.binned[pos] = g;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < lim; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;
+    uint64_t key = ~0ull;
+    DevSeg g{0, 0, 0, 0};
+    if (i < lim) {
+      g = s.segs[i];
+      const uint32_t cmd = s.seg_cmd[i];
+      if (cmd != 0xFFFFFFFFu) {
+        const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
+        int b0, b1;
+        seg_bands(g, (int)cv.bands, b0, b1);
+        const uint64_t row = cv.mat_base + (uint64_t)(cmd - cv.cmd_base);
+        if (b0 == b1) key = row + (uint64_t)b0 * cv.n_cmds;
+        else
+          for (int b = b0; b <= b1; b++) {
+            const uint32_t pos = atomicAdd(&cursor[row + (uint64_t)b * cv.n_cmds], 1u);
+            s.binned[pos] = g;
+          }
+      }
     }
+    const WaveRun r = wave_run(key);
+    uint32_t base = 0;
+    if (r.first && key != ~0ull) base = atomicAdd(&cursor[key], r.len);
+    base = __shfl(base, (int)r.head, 64);
+    if (key != ~0ull) s.binned[base + (lane - r.head)] = g;
   }
 }
 

@@ -181,7 +181,10 @@ def test_untrained_model_falls_back_to_exact_counting(amd):
     sc = S.cost_canvases(31, 1, 256, 256, 128, 3000)[0]   # 8287 edge segments
     amd.set_sizing(amd.SIZING_EXACT)
     _, want, _ = _render(amd, sc)
-    # a model that predicts nothing: every pool overflows
+    # a model that predicts nothing: every pool overflows.  On a fresh
+    # context the pool buffers hold stale device memory, so this also checks
+    # that the slots an overflowing item owns below the capacity are marked
+    # unused before binning reads them.
     zero = np.zeros(Rf.N_PARAMS, dtype=np.float32)
     zero[-2:] = -10.0
     amd.cost_model_set(zero)
