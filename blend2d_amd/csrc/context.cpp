@@ -164,7 +164,7 @@ struct Runner {
   // Resident frame.
   bool resident = false;
   FrameDev f{};
-  uint64_t n_src = 0, n_src_lines = 0;
+  uint64_t n_src = 0;
   uint32_t item_cap = 0;   // upper bound of edge work items: lines + 3 per quad + 8 per cubic
   uint32_t bands_max = 0;
   uint32_t max_bands() const { return bands_max; }
@@ -268,7 +268,6 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   std::vector<PartBase> bases(parts.size());
   PartBase tot{0, 0, 0, 0, 0, 0, 0};
   uint64_t n_src_total = 0;
-  n_src_lines = 0;
   uint64_t n_items_max = 0;
   for (size_t k = 0; k < parts.size(); k++) {
     const FrameHost& fh = *parts[k];
@@ -284,10 +283,7 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
     tot.luts += fh.luts.size();
     tot.images += fh.images.size();
     n_src_total += fh.src.size();
-    for (const SrcSeg& sg : fh.src) {
-      n_src_lines += sg.kind == kSrcLine ? 1u : 0u;
-      n_items_max += sg.kind == kSrcLine ? 1u : sg.kind == kSrcQuad ? 3u : 8u;
-    }
+    for (const SrcSeg& sg : fh.src) n_items_max += sg.kind == kSrcLine ? 1u : sg.kind == kSrcQuad ? 3u : 8u;
   }
 
   uint64_t n_mat = 0;

@@ -315,8 +315,9 @@ def cubic_gradient_scene(seed: int = 1, width: int = 1920, height: int = 1080, c
     sc.set_fill_rule(1)
     for i in range(count):
         side = float(rng.uniform(16, 600))
-        bx = float(rng.uniform(-side * 0.25, width - side * 0.75))
-        by = float(rng.uniform(-side * 0.25, height - side * 0.75))
+        # boxes larger than the canvas start a quarter side above / left of it
+        bx = float(rng.uniform(-side * 0.25, max(width - side * 0.75, -side * 0.25)))
+        by = float(rng.uniform(-side * 0.25, max(height - side * 0.75, -side * 0.25)))
         k = int(rng.integers(4, 17))
         pts = rng.random((3 * k, 2)) * side + np.array([bx, by])
         cmds = [CMD_MOVE] + [CMD_CUBIC, CMD_CUBIC, CMD_ON] * k + [CMD_CLOSE]

@@ -55,3 +55,11 @@ def test_gpu_matches_golden(name):
         got = cases.digest(px)
         assert err == want["error"], (name, k)
         assert got["sha256"] == want["sha256"], f"{name}[{k}]: first differing row band at y={_first_bad_band(got, want)}"
+
+
+def test_small_canvas_scene_renders_in_the_oracle():
+    # the scene of __graft_entry__.smoke(): boxes larger than the canvas
+    from blend2d_amd import scene as S
+    sc = S.cubic_gradient_scene(seed=7, width=256, height=192, count=40)
+    px, err = checkers.render_oracle(sc)
+    assert err == 0 and px.shape == (192, 256)
