@@ -5,6 +5,7 @@ trips and no counting pass, fewer fallbacks than history sizing)."""
 import numpy as np
 import pytest
 
+import checkers
 import cost_model_ref as R
 
 
@@ -197,3 +198,24 @@ def test_untrained_model_falls_back_to_exact_counting(amd):
         amd.cost_model_set(None)
     assert st["fallback"] == 1
     assert (got == want).all()
+
+
+@pytest.mark.gpu
+def test_learned_sizing_of_a_batched_frame_matches_the_oracle(amd):
+    # the shipped model sizes one batched frame of glyph and cost canvases
+    from blend2d_amd import scene as S, train_cost_model as TC
+    params = TC.load()
+    if params is None:
+        pytest.skip("no trained model parameters in the package")
+    scenes = S.glyph_canvases(9, 24, 60) + S.cost_canvases(9, 4, 128, 512, 32, 3000)
+    amd.cost_model_set(params)
+    amd.set_sizing(amd.SIZING_LEARNED)
+    try:
+        outs, errs = amd.render_scenes_batched(scenes)
+    finally:
+        amd.set_sizing(amd.SIZING_EXACT)
+        amd.cost_model_set(None)
+    for sc, out, err in zip(scenes, outs, errs):
+        want, oerr = checkers.render_oracle(sc)
+        assert err == oerr
+        assert checkers.diff_info(out, want)[0] <= 1
