@@ -216,7 +216,9 @@ BLAmdResult bl_amd_batch_get_stats(const BLAmdBatch* batch, BLAmdStats* out);
  * fill them run.  BL_AMD_SIZING_EXACT runs a counting pass of the edge
  * builder and reads the sizes back; BL_AMD_SIZING_LEARNED predicts them with
  * a small MLP (16 per-command features -> 64 -> 64 -> 2, fp32, trained on
- * the GPU by the kernels below) and runs the edge builder once;
+ * the GPU by the kernels below) and runs the edge builder in one kernel
+ * with no host round trip (each work item counted, then written at a slot
+ * range taken per wave);
  * BL_AMD_SIZING_HISTORY reuses the totals of the last exact frame.  In both
  * predicted modes an overflow is detected on the device, nothing is
  * composited, and the frame is redone exactly: pixels never depend on the
