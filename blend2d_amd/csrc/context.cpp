@@ -320,7 +320,7 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   uint8_t* st = q.staging;
   memcpy(st + off[0], canvases.data(), sizes[0]);
   const unsigned hw = std::thread::hardware_concurrency();
-  const size_t n_threads = std::min<size_t>({parts.size(), (size_t)(hw ? hw : 1), (size_t)8,
+  const size_t n_threads = std::min<size_t>({parts.size(), (size_t)(hw ? hw : 1), (size_t)16,
                                             blob_size / ((size_t)16 << 20) + 1});
   if (n_threads <= 1) {
     for (size_t k = 0; k < parts.size(); k++) pack_part(st, *parts[k], (uint32_t)k, bases[k], off);
