@@ -616,6 +616,41 @@ def translated_patterns_scene(seed: int = 1, width: int = 301, height: int = 211
     return sc
 
 
+def cancelling_pattern_transforms_scene(seed: int = 1, width: int = 200, height: int = 150, count: int = 40,
+                                        fmt: int = FMT_PRGB32) -> Scene:
+    """Patterns whose transform cancels the user transform (scale 2 / 0.5,
+    3 / (1/3), rotation by +a / -a), so the final transform is a translation
+    exactly or to within rounding (the near-identity test of the aligned /
+    fractional fetchers), PAD / REPEAT / REFLECT, nearest and bilinear."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(width, height, fmt, 0xFF203040 if fmt != FMT_A8 else 0x40)
+    img = sc.add_image(premultiplied_noise(rng, 37, 29))
+    for i in range(count):
+        sc.set_comp_op(COMP_OP_SRC_OVER if i % 3 else COMP_OP_SRC_COPY)
+        sc.set_pattern_quality(int(rng.integers(0, 2)))
+        k = i % 3
+        cx, cy = float(rng.uniform(0, width)), float(rng.uniform(0, height))
+        if k == 0:
+            u, p = [2.0, 0.0, 0.0, 2.0, cx, cy], [0.5, 0.0, 0.0, 0.5, float(rng.uniform(-20, 20)), float(rng.uniform(-20, 20))]
+        elif k == 1:
+            u, p = [3.0, 0.0, 0.0, 3.0, cx, cy], [1.0 / 3.0, 0.0, 0.0, 1.0 / 3.0, float(rng.uniform(-9, 9)), float(rng.uniform(-9, 9))]
+        else:
+            a = float(rng.uniform(0, 2 * math.pi))
+            c_, s_ = math.cos(a), math.sin(a)
+            u = [c_, s_, -s_, c_, cx, cy]
+            p = [c_, -s_, s_, c_, float(rng.uniform(-30, 30)), float(rng.uniform(-30, 30))]
+        sc.set_transform(u)
+        sc.set_fill_style_pattern(img, int(rng.integers(0, 3)), p)
+        w, h = float(rng.uniform(5, 60)), float(rng.uniform(5, 60))
+        if i % 2:
+            sc.fill_rect_d(-w / 2, -h / 2, w, h)
+        else:
+            cm, v = ellipse_path(0.0, 0.0, w / 2, h / 2)
+            sc.fill_path(cm, v)
+    sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+    return sc
+
+
 def box_scene(seed: int = 1, width: int = 333, height: int = 77, count: int = 60, fmt: int = FMT_PRGB32,
               comp_ops=(COMP_OP_SRC_OVER, COMP_OP_SRC_COPY)) -> Scene:
     """Box fills only (integer and fractional rects, fill_all, clears) under

@@ -92,6 +92,12 @@ def test_translated_patterns(amd, seed, fmt):
     _cmp(amd, S.translated_patterns_scene(seed, 301 + 64 * seed, 211, 120, fmt, comp_ops=ops))
 
 
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_A8])
+def test_cancelling_pattern_transforms(amd, fmt):
+    # user and pattern transforms that cancel to a translation
+    _cmp(amd, S.cancelling_pattern_transforms_scene(93, 200, 150, 40, fmt))
+
+
 def test_box_and_analytic_canvases_batched(amd):
     scenes = [S.box_scene(3, 255, 33, 30), S.polygons_scene(9, 70, 40, 50), S.box_scene(4, 1, 1, 5, S.FMT_A8),
               S.box_scene(5, 64, 16, 20, S.FMT_XRGB32)]

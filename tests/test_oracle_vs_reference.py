@@ -65,3 +65,10 @@ def test_translated_patterns(seed, fmt):
     # pure-translation pattern transforms: aligned and fractional fetchers
     # (the reference's portable pipeline implements SRC_OVER and SRC_COPY only)
     _same(S.translated_patterns_scene(seed, 301, 211, 120, fmt, comp_ops=(S.COMP_OP_SRC_OVER, S.COMP_OP_SRC_COPY)))
+
+
+@pytest.mark.parametrize("seed", [91, 92])
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_A8])
+def test_cancelling_pattern_transforms(seed, fmt):
+    # user and pattern transforms that cancel: the near-identity test
+    _same(S.cancelling_pattern_transforms_scene(seed, 200, 150, 40, fmt))
