@@ -111,7 +111,7 @@ class _Frame:
             img.fill_raw(sc.clear_value)
             ctx = B.BLContext(img)
             errs = B.replay_into(ctx, sc, images)
-            self.errors.append(errs[0] if errs else 0)
+            self.errors.append(errs[0][1] if errs else 0)
             if self.batch is not None:
                 self.batch.add(ctx)
             self.keep.append((ctx, images))

@@ -39,6 +39,7 @@ SYMBOLS = [
     "bl_amd_context_set_fill_style_pattern", "bl_amd_context_fill_path_d", "bl_amd_context_fill_polygon_d",
     "bl_amd_context_fill_rect_d", "bl_amd_context_fill_rect_i", "bl_amd_context_fill_all",
     "bl_amd_context_get_stats", "bl_amd_context_replay",
+    "bl_amd_context_get_error_flags", "bl_amd_context_get_deferred_error",
     "bl_amd_context_fill_all_rgba32", "bl_amd_context_fill_rect_i_rgba32", "bl_amd_context_fill_rect_d_rgba32",
     "bl_amd_context_fill_path_d_rgba32", "bl_amd_context_fill_geometry", "bl_amd_context_fill_geometry_rgba32",
     "bl_amd_context_clear_all", "bl_amd_context_clear_rect_i", "bl_amd_context_clear_rect_d",
@@ -155,6 +156,8 @@ def lib():
         "bl_amd_context_fill_all": ([P], U32),
         "bl_amd_context_get_stats": ([P, ctypes.POINTER(BLAmdStats)], U32),
         "bl_amd_context_replay": ([P], U32),
+        "bl_amd_context_get_error_flags": ([P, ctypes.POINTER(U32)], U32),
+        "bl_amd_context_get_deferred_error": ([P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(U32)], U32),
         "bl_amd_context_fill_all_rgba32": ([P, U32], U32),
         "bl_amd_context_fill_rect_i_rgba32": ([P, I, I, I, I, U32], U32),
         "bl_amd_context_fill_rect_d_rgba32": ([P, D, D, D, D, U32], U32),
@@ -422,6 +425,19 @@ class BLContext:
             _check(lib().bl_amd_context_fill_all(self._h), "fill_all")
         else:
             _check(lib().bl_amd_context_fill_all_rgba32(self._h, rgba32 & 0xFFFFFFFF), "fill_all")
+
+    def error_flags(self) -> int:
+        """BLContextErrorFlags accumulated since begin() (accumulated_error_flags)."""
+        v = ctypes.c_uint32()
+        _check(lib().bl_amd_context_get_error_flags(self._h, ctypes.byref(v)), "get_error_flags")
+        return v.value
+
+    def deferred_error(self):
+        """(call index, BLResult) of the first rendering call of the last
+        flush that failed on the device, or None."""
+        idx, code = ctypes.c_uint64(), ctypes.c_uint32()
+        _check(lib().bl_amd_context_get_deferred_error(self._h, ctypes.byref(idx), ctypes.byref(code)), "get_deferred_error")
+        return (idx.value, code.value) if code.value else None
 
     def last_stats(self) -> dict:
         """Counters of the last flush or replay (BLAmdStats)."""

@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <memory>
 #include <mutex>
@@ -82,9 +83,12 @@ struct FrameHost {
   std::vector<DevStyle> styles;
   std::vector<uint32_t> luts;
   std::vector<DevImage> images;
+  // Host only: per command, the rendering call (counted from begin()) that
+  // recorded it, so errors found on the device name their call.
+  std::vector<uint64_t> cmd_call;
   void clear() {
     canvases.clear(); cmds.clear(); xforms.clear(); src.clear(); vtx.clear();
-    styles.clear(); luts.clear(); images.clear();
+    styles.clear(); luts.clear(); images.clear(); cmd_call.clear();
   }
 };
 
@@ -158,6 +162,10 @@ struct Runner {
   DevBuf src_count, src_off, bounds, segs, seg_cmd, mat_cnt, mat_off, mat_pres, list_off, list, binned, cursor, scan_tmp, counters, cmd_tiles;
   DevBuf tile_hdr, group_hdr, tile_px, list_style, band_buf, item_cnt, item_off, items;
   DevBuf cm_params, cm_acc, cm_feat, cm_pred, cm_act;
+  DevBuf cmd_err;
+  // Commands of the last execution that failed on the device: (command
+  // index within the frame, kFailInvalidGeometry / kFailTooManySegments bits).
+  std::vector<std::pair<uint32_t, uint32_t>> failed;
   uint64_t cm_version = 0;
   uint32_t* host_readback = nullptr;
 
@@ -189,6 +197,9 @@ struct Runner {
 
   uint32_t upload(const std::vector<FrameHost*>& parts);
   uint32_t execute(Stats& stats);
+  // After the frame's last sync: reads the per-command failure bits back
+  // when the device flagged any (`any` = the flag word read back).
+  uint32_t read_failures(const FrameScratch& s, uint32_t any);
   uint32_t execute_exact(Stats& stats);
   // 0 done, 1 pool overflow (redo exactly), otherwise an error
   uint32_t execute_pool(Stats& stats, uint32_t mode);
@@ -389,8 +400,20 @@ bool Runner::model_views(cm::FrameModel& m) {
   return true;
 }
 
+uint32_t Runner::read_failures(const FrameScratch& s, uint32_t any) {
+  failed.clear();
+  if (!any) return BL_AMD_SUCCESS;
+  std::vector<uint32_t> bits(f.n_cmds);
+  HIP_OK(hipMemcpyAsync(bits.data(), s.cmd_err, (size_t)f.n_cmds * 4, hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipStreamSynchronize(stream));
+  for (uint32_t c = 0; c < f.n_cmds; c++)
+    if (bits[c]) failed.emplace_back(c, bits[c]);
+  return BL_AMD_SUCCESS;
+}
+
 uint32_t Runner::execute(Stats& stats) {
   stats = Stats{};
+  failed.clear();
   if (!resident) return BL_AMD_SUCCESS;
   uint32_t mode = g_sizing;
   if (collect || (mode == kSizeLearned && g_model.empty())) mode = kSizeExact;
@@ -428,7 +451,8 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
       !scan_tmp.reserve(scan_words * 4) ||
       !counters.reserve(64) || !cmd_tiles.reserve((size_t)f.n_cmds * 4) ||
       !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 32) ||
-      !band_buf.reserve(((size_t)f.n_bands + 1) * 28) || !tile_px.reserve((size_t)f.n_tiles * 8))
+      !band_buf.reserve(((size_t)f.n_bands + 1) * 28) || !tile_px.reserve((size_t)f.n_tiles * 8) ||
+      !cmd_err.reserve((size_t)f.n_cmds * 4))
     return kErrOOM;
 
   FrameScratch s;
@@ -446,6 +470,8 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   s.item_cnt = item_cnt.as<uint32_t>();
   s.item_off = item_off.as<uint32_t>();
   s.items = items.as<EdgeItem>();
+  s.cmd_err = cmd_err.as<uint32_t>();
+  s.err_any = counters.as<uint32_t>() + 4;
   s.bounds = bounds.as<CmdBounds>();
   s.mat_cnt = mat_cnt.as<uint32_t>();
   s.mat_off = mat_off.as<uint32_t>();
@@ -459,7 +485,8 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   s.flags = counters.as<uint32_t>() + 3;
   s.cmd_tiles = cmd_tiles.as<uint32_t>();
 
-  launch_fill_u32(counters.as<uint32_t>(), 0u, 4, stream);
+  launch_fill_u32(counters.as<uint32_t>(), 0u, 5, stream);
+  launch_fill_u32(s.cmd_err, 0u, f.n_cmds, stream);
   launch_init_bounds(s.bounds, f.n_cmds, stream);
 
   launch_edge_items(f, s, stream);
@@ -518,7 +545,7 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   stats.render_kernels = f.n_tiles ? 7u : 0u;
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
-  HIP_OK(hipMemcpyAsync(host_readback, counters.ptr, 16, hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync(host_readback, counters.ptr, 20, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipMemcpyAsync(host_readback + 8, s.mat_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipStreamSynchronize(stream));
   stats.syncs++;
@@ -527,6 +554,7 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   stats.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   stats.sizing = mode;
   if (host_readback[3]) return 1;
+  if (uint32_t r = read_failures(s, host_readback[4])) return r;
 
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, ev0, ev1);
@@ -563,7 +591,7 @@ uint32_t Runner::execute_exact(Stats& stats) {
       !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) || !counters.reserve(64) ||
       !cmd_tiles.reserve((size_t)f.n_cmds * 4) || !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 32) ||
       !band_buf.reserve(((size_t)f.n_bands + 1) * 28) ||
-      !tile_px.reserve((size_t)f.n_tiles * 8))
+      !tile_px.reserve((size_t)f.n_tiles * 8) || !cmd_err.reserve((size_t)f.n_cmds * 4))
     return kErrOOM;
 
   FrameScratch s;
@@ -591,6 +619,8 @@ uint32_t Runner::execute_exact(Stats& stats) {
   s.scan_tmp = scan_tmp.as<uint32_t>();
   s.pixels = counters.as<unsigned long long>();
   s.cmd_tiles = cmd_tiles.as<uint32_t>();
+  s.cmd_err = cmd_err.as<uint32_t>();
+  s.err_any = counters.as<uint32_t>() + 4;
 
   cm::FrameModel model{};
   if (collect) {
@@ -599,7 +629,8 @@ uint32_t Runner::execute_exact(Stats& stats) {
   }
 
   // Stage 1: edges (work items; count, scan, emit).
-  launch_fill_u32(counters.as<uint32_t>(), 0u, 4, stream);
+  launch_fill_u32(counters.as<uint32_t>(), 0u, 5, stream);
+  launch_fill_u32(s.cmd_err, 0u, f.n_cmds, stream);
   launch_init_bounds(s.bounds, f.n_cmds, stream);
   launch_edge_items(f, s, stream);
   launch_edges(f, s, true, item_cap, stream);
@@ -609,10 +640,13 @@ uint32_t Runner::execute_exact(Stats& stats) {
   if (n_src) {
     scan_exclusive(s.src_count, s.src_off, item_cap, s.scan_tmp, stream);
     HIP_OK(hipMemcpyAsync(host_readback, s.src_off + item_cap, 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(host_readback + 5, s.err_any, 4, hipMemcpyDeviceToHost, stream));
     HIP_OK(hipStreamSynchronize(stream));
     stats.syncs++;
     n_segs = host_readback[0];
   }
+  // segments of failed commands are not binned
+  const bool some_failed = n_src && host_readback[5];
   if (n_segs >= 0x80000000u) return kErrDevice;
   if (!segs.reserve((size_t)std::max<uint32_t>(n_segs, 1) * sizeof(DevSeg)) || !seg_cmd.reserve((size_t)std::max<uint32_t>(n_segs, 1) * 4))
     return kErrOOM;
@@ -642,7 +676,8 @@ uint32_t Runner::execute_exact(Stats& stats) {
   if (n_atiles > f.n_tiles || (uint64_t)n_sgroups + n_ggroups > f.n_groups) return kErrDevice;
   // every edge segment crosses at least one band and at most all of them;
   // the command lists hold at most one entry per (band, command) cell
-  if (n_binned < n_segs || (uint64_t)n_binned > (uint64_t)n_segs * std::max<uint32_t>(max_bands(), 1) || n_list > n_mat)
+  if ((n_binned < n_segs && !some_failed) || (uint64_t)n_binned > (uint64_t)n_segs * std::max<uint32_t>(max_bands(), 1) ||
+      n_list > n_mat)
     return kErrDevice;
   if (!binned.reserve((size_t)std::max<uint32_t>(n_binned, 1) * sizeof(DevSeg)) || !list.reserve((size_t)std::max<uint32_t>(n_list, 1) * 16) ||
       !list_style.reserve((size_t)std::max<uint32_t>(n_list, 1) * 4))
@@ -662,6 +697,7 @@ uint32_t Runner::execute_exact(Stats& stats) {
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback + 2, s.pixels, 8, hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync(host_readback + 4, s.err_any, 4, hipMemcpyDeviceToHost, stream));
   if (collect) {
     cm::launch_actuals(f, s, model, n_segs, stream);
     const size_t nc = f.n_cmds;
@@ -683,6 +719,7 @@ uint32_t Runner::execute_exact(Stats& stats) {
   HIP_OK(hipStreamSynchronize(stream));
   stats.syncs++;
   if (hipError_t e_ = hipGetLastError(); e_ != hipSuccess) { report_hip("kernel launch", e_, __LINE__); return kErrDevice; }
+  if (uint32_t r = read_failures(s, host_readback[4])) return r;
   hist_segs = n_segs;
   hist_bins = n_binned;
   float ms = 0.f;
@@ -716,10 +753,44 @@ struct BLAmdImage {
   int64_t stride = 0;
   int w = 0, h = 0;
   uint32_t format = 0;
+  // The caller's reference (bl_amd_image_create / bl_amd_image_destroy)
+  // plus one per context that renders into the image, per fill style that
+  // samples it and per recorded or resident (replayable) frame that uses it:
+  // the pixels are freed when the last reference goes, as the reference
+  // defers the free of an image that still has users (core/image.cpp:171-175).
+  std::atomic<int> refs{1};
   ~BLAmdImage() { if (dptr) (void)hipFree(dptr); }
 };
 
 namespace {
+
+void image_retain(const BLAmdImage* img) {
+  if (img) const_cast<BLAmdImage*>(img)->refs.fetch_add(1, std::memory_order_relaxed);
+}
+void image_release(const BLAmdImage* img) {
+  if (img && const_cast<BLAmdImage*>(img)->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete img;
+}
+
+// A counted reference to an image.
+class ImageRef {
+ public:
+  ImageRef() = default;
+  explicit ImageRef(const BLAmdImage* p) : p_(p) { image_retain(p_); }
+  ImageRef(const ImageRef& o) : p_(o.p_) { image_retain(p_); }
+  ImageRef& operator=(const ImageRef& o) {
+    image_retain(o.p_);
+    image_release(p_);
+    p_ = o.p_;
+    return *this;
+  }
+  ~ImageRef() { image_release(p_); }
+  const BLAmdImage* get() const { return p_; }
+  const BLAmdImage* operator->() const { return p_; }
+  explicit operator bool() const { return p_ != nullptr; }
+
+ private:
+  const BLAmdImage* p_ = nullptr;
+};
 
 enum StyleType { kStyleSolid = 0, kStyleGradient = 1, kStylePattern = 2 };
 
@@ -731,7 +802,7 @@ struct StyleState {
   uint32_t status = 0;          // 0 renders, 1 NOP, otherwise an error code
   DevStyle dev{};
   std::vector<uint32_t> lut;
-  const BLAmdImage* image = nullptr;
+  ImageRef image;
   uint64_t version = 0;
 };
 
@@ -752,9 +823,18 @@ struct BLAmdContext {
   uint32_t frame_style_index = 0;
 
   FrameHost frame;          // commands since the last flush (one canvas)
-  std::vector<const BLAmdImage*> frame_images;
+  std::vector<ImageRef> frame_images;     // pattern images of `frame`
+  std::vector<ImageRef> resident_images;  // target and pattern images of the resident frame
   Runner runner;
   Stats stats;
+
+  // Rendering calls since begin(); errors found on the device while
+  // rendering (BLContextErrorFlags bits, context.h:156-189) and the first
+  // failing call of the last flush with its error.
+  uint64_t call_seq = 0;
+  uint32_t error_flags = 0;
+  uint64_t deferred_call = ~0ull;
+  uint32_t deferred_code = 0;
 
   void reset_state() {
     comp_op = kCompSrcOver;
@@ -775,6 +855,7 @@ struct BLAmdBatch {
   std::vector<BLAmdContext*> contexts;
   Runner runner;
   Stats stats;
+  std::vector<ImageRef> resident_images;  // targets and pattern images of the resident frame
 };
 
 namespace {
@@ -797,8 +878,8 @@ StyleState make_solid_style(BLAmdContext* c, uint32_t rgba32) {
 }
 
 uint32_t image_index(BLAmdContext* c, const BLAmdImage* img) {
-  for (size_t i = 0; i < c->frame_images.size(); i++) if (c->frame_images[i] == img) return (uint32_t)i;
-  c->frame_images.push_back(img);
+  for (size_t i = 0; i < c->frame_images.size(); i++) if (c->frame_images[i].get() == img) return (uint32_t)i;
+  c->frame_images.push_back(ImageRef(img));
   DevImage d{};
   d.ptr = (uint64_t)(uintptr_t)img->dptr;
   d.stride = img->stride;
@@ -867,7 +948,7 @@ Resolved resolve_with(BLAmdContext* c, StyleState& st, uint32_t alpha, bool nop_
       d.lut = (uint32_t)c->frame.luts.size();
       c->frame.luts.insert(c->frame.luts.end(), st.lut.begin(), st.lut.end());
     }
-    if (st.type == kStylePattern) d.img = image_index(c, st.image);
+    if (st.type == kStylePattern) d.img = image_index(c, st.image.get());
     c->frame_style_index = (uint32_t)c->frame.styles.size();
     c->frame.styles.push_back(d);
     c->frame_style_version = st.version;
@@ -919,6 +1000,7 @@ CmdXform fixed_xform(const BLAmdContext* c) {
 
 uint32_t push_cmd(BLAmdContext* c, const DevCmd& cmd, const CmdXform* xf) {
   c->frame.cmds.push_back(cmd);
+  c->frame.cmd_call.push_back(c->call_seq ? c->call_seq - 1 : 0);
   if (xf) c->frame.xforms.push_back(*xf);
   else c->frame.xforms.push_back(CmdXform{});
   return (uint32_t)(c->frame.cmds.size() - 1);
@@ -1157,6 +1239,42 @@ struct TmpPath {
 
 uint32_t check_target(const BLAmdContext* c) { return c && c->target ? BL_AMD_SUCCESS : kErrState; }
 
+// Entry of every rendering call (fill_*, clear_*): counts it, then checks the target.
+uint32_t render_call(BLAmdContext* c) {
+  if (c) c->call_seq++;
+  return check_target(c);
+}
+
+// BLContextErrorFlags (context.h:156-189) and result codes of a command that
+// failed on the device.
+constexpr uint32_t kFlagInvalidGeometry = 0x00000004u, kFlagOutOfMemory = 0x40000000u;
+constexpr uint32_t kErrInvalidGeometry = BL_AMD_ERROR_INVALID_GEOMETRY;
+
+// Hands the commands that failed in `runner`'s last execution to the
+// contexts that recorded them (`parts` in frame order, context k recorded
+// parts[k]): accumulated error flags, and the first failing call of each.
+void attribute_failures(const Runner& runner, const std::vector<BLAmdContext*>& ctxs) {
+  std::vector<uint32_t> base(ctxs.size() + 1, 0);
+  for (size_t k = 0; k < ctxs.size(); k++) {
+    base[k + 1] = base[k] + (uint32_t)ctxs[k]->frame.cmds.size();
+    ctxs[k]->deferred_call = ~0ull;
+    ctxs[k]->deferred_code = 0;
+  }
+  for (const auto& fc : runner.failed) {
+    const size_t k = (size_t)(std::upper_bound(base.begin(), base.end(), fc.first) - base.begin()) - 1;
+    if (k >= ctxs.size()) continue;
+    BLAmdContext* c = ctxs[k];
+    const uint32_t local = fc.first - base[k];
+    const uint64_t call = local < c->frame.cmd_call.size() ? c->frame.cmd_call[local] : 0;
+    const bool geom = (fc.second & kFailInvalidGeometry) != 0;
+    c->error_flags |= geom ? kFlagInvalidGeometry : kFlagOutOfMemory;
+    if (call < c->deferred_call) {
+      c->deferred_call = call;
+      c->deferred_code = geom ? kErrInvalidGeometry : kErrOOM;
+    }
+  }
+}
+
 // Prepares a context's frame for merging: canvas record.
 void finalize_frame(BLAmdContext* c) {
   c->frame.canvases.clear();
@@ -1177,9 +1295,21 @@ void reset_frame(BLAmdContext* c) {
   c->frame_style_version = ~0ull;
 }
 
+// The images a frame just uploaded uses stay referenced while it is resident
+// (replayable): the target and the pattern images.
+void keep_resident(std::vector<ImageRef>& keep, const BLAmdContext* c) {
+  keep.push_back(ImageRef(c->target));
+  keep.insert(keep.end(), c->frame_images.begin(), c->frame_images.end());
+}
+
 uint32_t flush_one(BLAmdContext* c) {
   finalize_frame(c);
   uint32_t r = c->runner.run(std::vector<FrameHost*>{&c->frame}, c->stats);
+  attribute_failures(c->runner, std::vector<BLAmdContext*>{c});
+  if (!c->frame.cmds.empty()) {
+    c->resident_images.clear();
+    keep_resident(c->resident_images, c);
+  }
   reset_frame(c);
   return r;
 }
@@ -1217,7 +1347,7 @@ BLAmdResult bl_amd_image_create(BLAmdImage** out, int w, int h, uint32_t format)
 }
 
 BLAmdResult bl_amd_image_destroy(BLAmdImage* img) {
-  delete img;
+  image_release(img);
   return BL_AMD_SUCCESS;
 }
 
@@ -1269,6 +1399,10 @@ BLAmdResult bl_amd_context_create(BLAmdContext** out) {
 }
 
 BLAmdResult bl_amd_context_destroy(BLAmdContext* ctx) {
+  if (ctx && ctx->target) {
+    image_release(ctx->target);
+    ctx->target = nullptr;
+  }
   delete ctx;
   return BL_AMD_SUCCESS;
 }
@@ -1279,9 +1413,14 @@ BLAmdResult bl_amd_context_begin(BLAmdContext* ctx, BLAmdImage* target) {
     uint32_t r = bl_amd_context_end(ctx);
     if (r) return r;
   }
+  image_retain(target);
   ctx->target = target;
   ctx->reset_state();
   reset_frame(ctx);
+  ctx->call_seq = 0;
+  ctx->error_flags = 0;
+  ctx->deferred_call = ~0ull;
+  ctx->deferred_code = 0;
   return BL_AMD_SUCCESS;
 }
 
@@ -1294,6 +1433,7 @@ BLAmdResult bl_amd_context_flush(BLAmdContext* ctx, uint32_t flags) {
 BLAmdResult bl_amd_context_end(BLAmdContext* ctx) {
   if (uint32_t r = check_target(ctx)) return r;
   uint32_t r = flush_one(ctx);
+  image_release(ctx->target);
   ctx->target = nullptr;
   return r;
 }
@@ -1397,7 +1537,7 @@ BLAmdResult bl_amd_context_set_fill_style_pattern(BLAmdContext* ctx, const BLAmd
   st.type = kStylePattern;
   st.version = ++ctx->style_counter;
   st.fmt = image->format;
-  st.image = image;
+  st.image = ImageRef(image);
   Mat t = ctx->user;
   if (transform) {
     Mat pm{transform[0], transform[1], transform[2], transform[3], transform[4], transform[5]};
@@ -1409,7 +1549,7 @@ BLAmdResult bl_amd_context_set_fill_style_pattern(BLAmdContext* ctx, const BLAmd
 }
 
 BLAmdResult bl_amd_context_fill_path_d(BLAmdContext* ctx, const double* origin, const uint8_t* cmds, const double* vertices, size_t n) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   if (n == 0) return BL_AMD_SUCCESS;
   if (!cmds || !vertices) return kErrInvalid;
   return fill_path_rule(ctx, resolve_fill(ctx), origin, cmds, vertices, n, ctx->fill_rule);
@@ -1417,61 +1557,61 @@ BLAmdResult bl_amd_context_fill_path_d(BLAmdContext* ctx, const double* origin, 
 
 BLAmdResult bl_amd_context_fill_path_d_rgba32(BLAmdContext* ctx, const double* origin, const uint8_t* cmds, const double* vertices,
                                               size_t n, uint32_t rgba32) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   if (n == 0) return BL_AMD_SUCCESS;
   if (!cmds || !vertices) return kErrInvalid;
   return fill_path_rule(ctx, resolve_solid(ctx, rgba32), origin, cmds, vertices, n, ctx->fill_rule);
 }
 
 BLAmdResult bl_amd_context_fill_polygon_d(BLAmdContext* ctx, const double* vertices, size_t n) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   if (n && !vertices) return kErrInvalid;
   if (n < 3) return BL_AMD_SUCCESS;
   return fill_poly_rule(ctx, resolve_fill(ctx), vertices, n, ctx->fill_rule);
 }
 
 BLAmdResult bl_amd_context_fill_rect_d(BLAmdContext* ctx, double x, double y, double w, double h) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   return fill_box_d(ctx, resolve_fill(ctx), x, y, x + w, y + h);
 }
 
 BLAmdResult bl_amd_context_fill_rect_d_rgba32(BLAmdContext* ctx, double x, double y, double w, double h, uint32_t rgba32) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   return fill_box_d(ctx, resolve_solid(ctx, rgba32), x, y, x + w, y + h);
 }
 
 BLAmdResult bl_amd_context_fill_rect_i(BLAmdContext* ctx, int x, int y, int w, int h) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   return fill_rect_i(ctx, resolve_fill(ctx), x, y, w, h);
 }
 
 BLAmdResult bl_amd_context_fill_rect_i_rgba32(BLAmdContext* ctx, int x, int y, int w, int h, uint32_t rgba32) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   return fill_rect_i(ctx, resolve_solid(ctx, rgba32), x, y, w, h);
 }
 
 BLAmdResult bl_amd_context_fill_all(BLAmdContext* ctx) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   return fill_all_box(ctx, resolve_fill(ctx, false));
 }
 
 BLAmdResult bl_amd_context_fill_all_rgba32(BLAmdContext* ctx, uint32_t rgba32) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   return fill_all_box(ctx, resolve_solid(ctx, rgba32, false));
 }
 
 BLAmdResult bl_amd_context_clear_all(BLAmdContext* ctx) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   return fill_all_box(ctx, resolve_clear(ctx, false));
 }
 
 BLAmdResult bl_amd_context_clear_rect_i(BLAmdContext* ctx, int x, int y, int w, int h) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   return fill_rect_i(ctx, resolve_clear(ctx, true), x, y, w, h);
 }
 
 BLAmdResult bl_amd_context_clear_rect_d(BLAmdContext* ctx, double x, double y, double w, double h) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   return fill_box_d(ctx, resolve_clear(ctx, true), x, y, x + w, y + h);
 }
 
@@ -1651,13 +1791,13 @@ static uint32_t fill_geometry_resolved(BLAmdContext* ctx, const Resolved& r, uin
 }
 
 BLAmdResult bl_amd_context_fill_geometry(BLAmdContext* ctx, uint32_t type, const void* data) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   if (type > BL_AMD_GEOMETRY_MAX_VALUE) return kErrInvalid;
   return fill_geometry_resolved(ctx, resolve_fill(ctx), type, data);
 }
 
 BLAmdResult bl_amd_context_fill_geometry_rgba32(BLAmdContext* ctx, uint32_t type, const void* data, uint32_t rgba32) {
-  if (uint32_t r = check_target(ctx)) return r;
+  if (uint32_t r = render_call(ctx)) return r;
   if (type > BL_AMD_GEOMETRY_MAX_VALUE) return kErrInvalid;
   return fill_geometry_resolved(ctx, resolve_solid(ctx, rgba32), type, data);
 }
@@ -1684,6 +1824,19 @@ static void copy_stats(const Stats& st, BLAmdStats* out) {
   out->total_ms = st.total_ms;
 }
 
+BLAmdResult bl_amd_context_get_error_flags(const BLAmdContext* ctx, uint32_t* flags) {
+  if (!ctx || !flags) return kErrInvalid;
+  *flags = ctx->error_flags;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_get_deferred_error(const BLAmdContext* ctx, uint64_t* call_index, uint32_t* code) {
+  if (!ctx || !call_index || !code) return kErrInvalid;
+  *call_index = ctx->deferred_code ? ctx->deferred_call : 0;
+  *code = ctx->deferred_code;
+  return BL_AMD_SUCCESS;
+}
+
 BLAmdResult bl_amd_context_get_stats(const BLAmdContext* ctx, BLAmdStats* out) {
   if (!ctx || !out) return kErrInvalid;
   copy_stats(ctx->stats, out);
@@ -1708,6 +1861,13 @@ BLAmdResult bl_amd_batch_destroy(BLAmdBatch* b) {
 
 BLAmdResult bl_amd_batch_add(BLAmdBatch* b, BLAmdContext* ctx) {
   if (!b || !ctx || !ctx->target) return kErrInvalid;
+  // Every context of a batch is one canvas of one frame, and the canvases
+  // are composited concurrently: a context added twice, or two contexts
+  // rendering into the same image, would race on its pixels (the reference
+  // renders each context's calls in order).  Such a context is rejected; it
+  // can be flushed on its own.
+  for (const BLAmdContext* c : b->contexts)
+    if (c == ctx || c->target == ctx->target) return kErrInvalid;
   b->contexts.push_back(ctx);
   return BL_AMD_SUCCESS;
 }
@@ -1732,6 +1892,13 @@ BLAmdResult bl_amd_batch_flush(BLAmdBatch* b) {
   }
   const double merge_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   uint32_t r = b->runner.run(parts, b->stats, merge_ms);
+  attribute_failures(b->runner, b->contexts);
+  size_t n_cmds = 0;
+  for (const BLAmdContext* c : b->contexts) n_cmds += c->frame.cmds.size();
+  if (n_cmds) {
+    b->resident_images.clear();
+    for (const BLAmdContext* c : b->contexts) keep_resident(b->resident_images, c);
+  }
   for (BLAmdContext* c : b->contexts) reset_frame(c);
   return r;
 }
@@ -1745,13 +1912,16 @@ BLAmdResult bl_amd_batch_replay(BLAmdBatch* b) {
 // Learned sizing: cost model
 // ---------------------------------------------------------------------------
 
+// Both settings are read by flushes under the device queue's mutex.
 BLAmdResult bl_amd_set_sizing(uint32_t mode) {
   if (mode > kSizeHistory) return kErrInvalid;
+  std::lock_guard<std::mutex> lock(device_queue().mu);
   g_sizing = mode;
   return BL_AMD_SUCCESS;
 }
 
 BLAmdResult bl_amd_cost_model_set(const float* params) {
+  std::lock_guard<std::mutex> lock(device_queue().mu);
   if (params) g_model.assign(params, params + cm::kParams);
   else g_model.clear();
   g_model_version++;

@@ -52,6 +52,22 @@ enum CmdKind : uint32_t {
   kCmdBoxU = 2,       // 24.8 box, border-coverage masks
 };
 
+// Why a fill command failed on the device (bits, per command).
+enum CmdFail : uint32_t {
+  // The line clipper met a clip-flag combination that no case handles (NaN
+  // coordinates), or a curve has a non-finite control point: the reference
+  // fails the fill with BL_ERROR_INVALID_GEOMETRY and reverts its edges
+  // (raster/edgebuilder_p.h:1494-1496, 1584-1586,
+  // raster/rastercontextops.cpp:38-45).
+  kFailInvalidGeometry = 1,
+  // One edge work item flattened to more than kMaxItemSegments segments
+  // (control points astronomically far apart): the fill is dropped and
+  // reported as BL_ERROR_OUT_OF_MEMORY instead of being flattened on to the
+  // subdivision limit.
+  kFailTooManySegments = 2,
+};
+constexpr uint32_t kMaxItemSegments = 1u << 22;
+
 // Source geometry unit processed by one edge-builder thread.
 enum SrcKind : uint32_t {
   kSrcLine = 0,

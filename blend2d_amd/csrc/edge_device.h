@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_types.h"
+
 namespace b2amd {
 namespace edge {
 
@@ -45,6 +47,7 @@ __device__ __forceinline__ uint32_t fxf(const Clip& c, P p) { return fx0(c, p) |
 __device__ __forceinline__ uint32_t fyf(const Clip& c, P p) { return fy0(c, p) | fy1(c, p); }
 __device__ __forceinline__ uint32_t fxy(const Clip& c, P p) { return fxf(c, p) | fyf(c, p); }
 
+
 // Emitter receives stored segments: y0 < y1, sign = 1 when the source went up.
 template<typename Emit>
 struct Builder {
@@ -53,15 +56,26 @@ struct Builder {
   double tol_sq;
   int32_t ax, ay;       // appender: last point
   uint32_t asign;
+  uint32_t invalid;     // CmdFail bits
+  uint32_t n_out;       // segments emitted so far
 
-  __device__ Builder(Emit& e, const Clip& clip, double t) : out(e), c(clip), tol_sq(t), ax(0), ay(0), asign(0) {}
+  __device__ Builder(Emit& e, const Clip& clip, double t)
+      : out(e), c(clip), tol_sq(t), ax(0), ay(0), asign(0), invalid(0), n_out(0) {}
 
+  // Past the segment budget nothing more is emitted and is_flat() holds for
+  // every piece, so the flattening loops only unwind their stacks.  Both the
+  // counting and the writing pass stop at the same segment.
+  __device__ __forceinline__ void emit(int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint32_t sign) {
+    if (n_out >= kMaxItemSegments) { invalid |= kFailTooManySegments; return; }
+    n_out++;
+    out.push(x0, y0, x1, y1, sign);
+  }
   __device__ __forceinline__ void stored(int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint32_t sign) {
-    if (y0 != y1) out.push(x0, y0, x1, y1, sign);
+    if (y0 != y1) emit(x0, y0, x1, y1, sign);
   }
   __device__ __forceinline__ void dir(int32_t x0, int32_t y0, int32_t x1, int32_t y1) {
-    if (y0 < y1) out.push(x0, y0, x1, y1, 0u);
-    else if (y0 > y1) out.push(x1, y1, x0, y0, 1u);
+    if (y0 < y1) emit(x0, y0, x1, y1, 0u);
+    else if (y0 > y1) emit(x1, y1, x0, y0, 1u);
   }
   __device__ __forceinline__ void line_seg(double x0, double y0, double x1, double y1) {
     dir(trunc_i(x0), trunc_i(y0), trunc_i(x1), trunc_i(y1));
@@ -105,6 +119,7 @@ struct Builder {
         q.y = a.y + (q.x - a.x) * d.y / d.x;
         break;
       default:
+        invalid |= kFailInvalidGeometry;   // edgebuilder_p.h:1584-1586
         return;
     }
     line_seg(p.x, p.y, q.x, q.y);
@@ -192,6 +207,7 @@ struct Builder {
         af = fyf(c, p);
         break;
       default:
+        invalid |= kFailInvalidGeometry;   // edgebuilder_p.h:1494-1496
         return;
     }
     if (af) {
@@ -235,6 +251,7 @@ struct Builder {
     for (int i = 1; i < m.n - 1; i++) { m.p[i].x = dclamp(m.p[i].x, l.x, f.x); m.p[i].y = dclamp(m.p[i].y, f.y, l.y); }
   }
   __device__ __forceinline__ bool is_flat(const Mono& m) const {
+    if (invalid & kFailTooManySegments) return true;
     double value, limit;
     if (m.n == 3) {
       P v1 = psub(m.p[1], m.p[0]);

@@ -68,6 +68,11 @@ struct FrameScratch {
   uint4* group_hdr;        // <= 2 n_groups: two header words per box tile group (see k_group_setup),
                            // the solid bands' groups first
   unsigned long long* tile_px; // n_tiles: composited pixels of each tile
+  // Failed fills: per command the kFailInvalidGeometry / kFailTooManySegments
+  // bits of its work items (the command's segments are not binned), and one
+  // word set when any command failed.  Both start at zero.
+  uint32_t* cmd_err;       // n_cmds
+  uint32_t* err_any;       // 1
   // Pool mode (sizes predicted ahead of time; null / 0 otherwise):
   uint32_t* seg_cursor;    // segments taken from the pool
   uint32_t* flags;         // bit 0: segment pool overflow, bit 1: bin pool overflow

@@ -197,7 +197,9 @@ __device__ __forceinline__ edge::Clip canvas_clip(const DevCanvas& cv) {
 }
 
 // Work items of source segment `sg`; written to `out` when it is not null.
-__device__ __forceinline__ uint32_t source_items(const FrameDev& f, const SrcSeg& sg, EdgeItem* out) {
+// `bad` (optional) receives kFailInvalidGeometry for a curve that cannot
+// be flattened.
+__device__ __forceinline__ uint32_t source_items(const FrameDev& f, const SrcSeg& sg, EdgeItem* out, uint32_t* bad = nullptr) {
   const CmdXform& m = f.xforms[sg.cmd];
   if (sg.kind == kSrcLine) {
     if (out) {
@@ -216,6 +218,15 @@ __device__ __forceinline__ uint32_t source_items(const FrameDev& f, const SrcSeg
   cv[1] = xform_pt(m, f.vtx, sg.i1);
   cv[2] = xform_pt(m, f.vtx, sg.i1 + 1);
   cv[3] = npts == 4 ? xform_pt(m, f.vtx, sg.i1 + 2) : cv[2];
+  // A curve with a NaN or infinite control point has no defined flattening
+  // (the reference subdivides it to its recursion limit or trips its own
+  // assertions): the fill fails with BL_ERROR_INVALID_GEOMETRY.
+  bool finite = true;
+  for (int i = 0; i < npts; i++) finite = finite && isfinite(cv[i].x) && isfinite(cv[i].y);
+  if (!finite) {
+    if (bad) *bad = kFailInvalidGeometry;
+    return 0u;
+  }
   const edge::Clip clip = canvas_clip(f.canvases[f.cmds[sg.cmd].canvas]);
   uint32_t common = 0xFu, any = 0u;
   for (int i = 0; i < npts; i++) { const uint32_t fl = edge::fxy(clip, cv[i]); common &= fl; any |= fl; }
@@ -250,10 +261,20 @@ __device__ __forceinline__ uint32_t source_items(const FrameDev& f, const SrcSeg
   return (uint32_t)pieces;
 }
 
+// A failed work item fails its whole command (rare; the plain stores race
+// benignly, every writer sets the same bits).
+__device__ __forceinline__ void fail_cmd(const FrameScratch& s, uint32_t cmd, uint32_t bits) {
+  atomicOr(&s.cmd_err[cmd], bits);
+  *s.err_any = 1u;
+}
+
 __global__ void k_items_count(FrameDev f, FrameScratch s) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= f.n_src) return;
-  s.item_cnt[i] = source_items(f, f.src[i], nullptr);
+  uint32_t bad = 0;
+  const SrcSeg sg = f.src[i];
+  s.item_cnt[i] = source_items(f, sg, nullptr, &bad);
+  if (bad) fail_cmd(s, sg.cmd, bad);
 }
 
 __global__ void k_items_write(FrameDev f, FrameScratch s) {
@@ -262,8 +283,10 @@ __global__ void k_items_write(FrameDev f, FrameScratch s) {
   (void)source_items(f, f.src[i], s.items + s.item_off[i]);
 }
 
+// Flattens one work item into `e`; returns its kFailInvalidGeometry /
+// kTooManySegments bits.
 template<typename Emit>
-__device__ __forceinline__ void build_item(const FrameDev& f, const EdgeItem& it, Emit& e) {
+__device__ __forceinline__ uint32_t build_item(const FrameDev& f, const EdgeItem& it, Emit& e) {
   edge::Builder<Emit> b(e, canvas_clip(f.canvases[f.cmds[it.cmd].canvas]), f.tol_sq);
   edge::P stack[edge::Builder<Emit>::kLimit * 4];
   const edge::P* ip = (const edge::P*)it.p;
@@ -274,6 +297,7 @@ __device__ __forceinline__ void build_item(const FrameDev& f, const EdgeItem& it
     case kItemBorderLeft: b.acc_left(ip[0].y, ip[1].y); break;
     default: b.acc_right(ip[0].y, ip[1].y); break;
   }
+  return b.invalid;
 }
 
 // Per-command bounds of a wave's work items with one set of atomics per run
@@ -319,7 +343,8 @@ __global__ __launch_bounds__(64) void k_edges_count(FrameDev f, FrameScratch s, 
   if (active) {
     const EdgeItem it = s.items[i];
     cmd = it.cmd;
-    build_item(f, it, e);
+    const uint32_t bad = build_item(f, it, e);
+    if (bad) fail_cmd(s, cmd, bad);
   }
   if (slot) s.src_count[i] = e.n;
   wave_cmd_bounds(s.bounds, cmd, e.min_x, e.max_x, e.min_y, e.max_y);
@@ -349,7 +374,8 @@ __global__ __launch_bounds__(64) void k_edges_emit_pool(FrameDev f, FrameScratch
   CountEmit c;
   if (active) {
     it = s.items[i];
-    build_item(f, it, c);
+    const uint32_t bad = build_item(f, it, c);
+    if (bad) fail_cmd(s, it.cmd, bad);
   }
   const uint32_t need = c.n;
   uint32_t incl = need;
@@ -461,7 +487,7 @@ __global__ void k_bin_count(FrameDev f, FrameScratch s, uint32_t n) {
     if (i < lim) {
       const DevSeg g = s.segs[i];
       const uint32_t cmd = s.seg_cmd[i];
-      if (cmd != 0xFFFFFFFFu) {
+      if (cmd != 0xFFFFFFFFu && !s.cmd_err[cmd]) {
         const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
         int b0, b1;
         seg_bands(g, (int)cv.bands, b0, b1);
@@ -530,7 +556,7 @@ __global__ void k_bin_scatter(FrameDev f, FrameScratch s, uint32_t n, uint32_t* 
     if (i < lim) {
       g = s.segs[i];
       const uint32_t cmd = s.seg_cmd[i];
-      if (cmd != 0xFFFFFFFFu) {
+      if (cmd != 0xFFFFFFFFu && !s.cmd_err[cmd]) {
         const DevCanvas& cv = f.canvases[f.cmds[cmd].canvas];
         int b0, b1;
         seg_bands(g, (int)cv.bands, b0, b1);

@@ -19,19 +19,31 @@ def _images(ctx_cls, sc):
     return out
 
 
+# Scene ops that are rendering calls (fill_* / clear_*): the context counts
+# them, and an error found on the device names its call by that count.
+RENDER_OPS = {S.OP_FILL_PATH, S.OP_FILL_POLYGON, S.OP_FILL_RECT_D, S.OP_FILL_RECT_I, S.OP_FILL_ALL,
+              S.OP_CLEAR_ALL, S.OP_CLEAR_RECT_I, S.OP_CLEAR_RECT_D, S.OP_FILL_GEOMETRY, S.OP_FILL_ALL_RGBA32,
+              S.OP_FILL_RECT_I_RGBA32, S.OP_FILL_RECT_D_RGBA32, S.OP_FILL_PATH_EX}
+
+
 def replay_into(ctx, sc: "S.Scene", images) -> list:
-    """Issues the scene's calls on `ctx`; returns the BLResult errors seen
-    (calls keep going after an error, as the reference driver does)."""
+    """Issues the scene's calls on `ctx`; returns the BLResult errors seen,
+    each as (rendering-call index, code) (calls keep going after an error, as
+    the reference driver does).  State calls that fail get the index of the
+    next rendering call."""
     from . import BLError, BLGradient, BLPattern, BLRgba, BLRgba64
     errors = []
+    n_render = 0
 
     def call(fn, *args):
         try:
             fn(*args)
         except BLError as e:
-            errors.append(e.code)
+            errors.append((n_render, e.code))
 
     for op, pl in sc.cmds:
+        if op in RENDER_OPS:
+            n_render += 1
         if op == S.OP_SET_COMP_OP:
             call(ctx.set_comp_op, struct.unpack_from("<I", pl)[0])
         elif op == S.OP_SET_FILL_RULE:
@@ -107,6 +119,17 @@ def replay_into(ctx, sc: "S.Scene", images) -> list:
     return errors
 
 
+def first_error(errors, ctx) -> int:
+    """The first error in call order: those the calls returned (index =
+    rendering calls issued so far, the failing call included) merged with
+    the one the device found during the flush (0-based call index)."""
+    found = list(errors)
+    d = ctx.deferred_error()
+    if d is not None:
+        found.append((d[0] + 1, d[1]))
+    return min(found)[1] if found else 0
+
+
 def render_scene(sc: "S.Scene", return_context: bool = False):
     """Renders one scene on the GPU; returns (pixels, first_error[, ctx])."""
     from . import BLContext, BLImage
@@ -117,11 +140,12 @@ def render_scene(sc: "S.Scene", return_context: bool = False):
     errs = replay_into(ctx, sc, images)
     ctx.flush()
     stats = ctx.last_stats()
+    err = first_error(errs, ctx)
     ctx.end()
     out = img.download()
     if return_context:
-        return out, (errs[0] if errs else 0), stats
-    return out, (errs[0] if errs else 0)
+        return out, err, stats
+    return out, err
 
 
 def render_scenes_batched(scenes):
@@ -136,10 +160,11 @@ def render_scenes_batched(scenes):
         img.fill_raw(sc.clear_value)
         ctx = BLContext(img)
         errs = replay_into(ctx, sc, images)
-        errors.append(errs[0] if errs else 0)
+        errors.append(errs)
         batch.add(ctx)
         keep.append((img, ctx, images))
     batch.flush()
+    errors = [first_error(errs, ctx) for errs, (_, ctx, _) in zip(errors, keep)]
     outs = [img.download() for img, _, _ in keep]
     for img, ctx, _ in keep:
         ctx.end()

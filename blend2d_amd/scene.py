@@ -758,6 +758,127 @@ def edge_case_scene(width: int = 257, height: int = 131, fmt: int = FMT_PRGB32) 
     return sc
 
 
+INT32_MIN, INT32_MAX = -(1 << 31), (1 << 31) - 1
+
+
+def clip_robustness_scenes(width: int = 97, height: int = 67, fmt: int = FMT_PRGB32, seed: int = 1) -> list:
+    """The fill-clipping matrix of the reference's robustness test
+    (core/context_test.cpp:389-586): under each of its nine user transforms,
+    fill_rect(BLRectI) for every (point, size) pair of its integer tables and
+    fill_rect(BLRect) for every pair of its double tables (NaN, +-Inf, INT32
+    extremes, 1e-100 scales, sizes just below a pixel).  The reference fills
+    opaque white; here each call gets its own seeded translucent colour so a
+    coverage difference anywhere shows in the pixels.  One canvas per
+    transform."""
+    nan, inf = float("nan"), float("inf")
+    cw, ch, sw, sh = width, height, 23, 23
+    matrices = [
+        (1.0, 0.0, 0.0, 1.0, 0.0, 0.0), (1.0, 0.0, 0.0, 1.0, 11.3, 11.9),
+        (100.0, 0.0, 0.0, 100.0, 0.0, 0.0), (-100.0, 0.0, 0.0, -100.0, 0.0, 0.0),
+        (1.0, 0.0, 0.0, 0.000001, 0.0, 0.0), (0.000001, 0.0, 0.0, 1.0, 0.0, 0.0),
+        (0.000001, 0.0, 0.0, 0.000001, 0.0, 0.0), (1e-20, 0.0, 0.0, 1e-20, 0.0, 0.0),
+        (1e-100, 0.0, 0.0, 1e-100, 0.0, 0.0)]
+    pts_i = [(0, 0), (0, -1), (-1, 0), (-1, -1), (0, ch - 1), (cw - 1, 0), (cw - 1, ch - 1), (0, -sh + 1),
+             (-sw + 1, 0), (-sw + 1, -sh + 1), (INT32_MIN, 0), (INT32_MIN, -1), (0, INT32_MIN), (-1, INT32_MIN),
+             (INT32_MIN, INT32_MIN), (INT32_MAX, 0), (INT32_MAX, -1), (0, INT32_MAX), (-1, INT32_MAX),
+             (INT32_MAX, INT32_MAX)]
+    pts_d = [(0.0, 0.0), (0.0, 0.3), (0.3, 0.0), (0.3, 0.3), (0.0, 100.0), (100.0, 0.0), (100.0, 100.0)]
+    pts_d += [(0.0, -sh + 10.0 ** -k) for k in range(1, 8)] + [(0.0, -sh + 1.0)]
+    pts_d += [(-sw + 10.0 ** -k, 0.0) for k in range(1, 8)] + [(-sw + 1.0, 0.0)]
+    pts_d += [(0.0, ch - 10.0 ** -k) for k in range(1, 6)] + [(cw - 10.0 ** -k, 0.0) for k in range(1, 6)]
+    pts_d += [(cw - 0.00001, ch - 0.00001), (-1000.0, 0.0), (-1000000.0, 0.0), (0.0, -1000.0), (0.0, -1000000.0),
+              (-1000.0, -1000.0), (-1000000.0, -1000000.0), (-1e50, -1e50), (-1e100, -1e100), (-1e200, -1e200),
+              (1e50, 1e50), (1e100, 1e100), (1e200, 1e200), (inf, 0.0), (0.0, inf), (inf, inf), (-inf, 0.0),
+              (0.0, -inf), (-inf, -inf), (nan, 0.0), (0.0, nan), (nan, nan)]
+    sizes_i = [(sw, sh), (sw // 2, sh // 2), (1, 1), (0, 0), (0, INT32_MIN), (INT32_MIN, 0), (INT32_MIN, INT32_MIN),
+               (0, INT32_MAX), (INT32_MAX, 0), (INT32_MAX, INT32_MAX)]
+    sizes_d = [(float(sw), float(sh)), (float(sw // 2), float(sh // 2)), (1.0, 1.0), (0.0, 0.0), (0.0, float(sh)),
+               (float(sw), 0.0), (0.1, 0.1), (0.00001, float(sh)), (float(sw), 0.00001), (0.00001, 0.00001),
+               (0.0000001, 0.0000001), (-0.00001, float(sh)), (float(sw), -0.00001), (-0.00001, -0.00001)]
+    sizes_d += [(float(sw), 10.0 ** e) for e in (40, 80, 120, 160, 200)]
+    sizes_d += [(10.0 ** e, float(sh)) for e in (40, 80, 120, 160, 200)]
+    sizes_d += [(inf, float(sh)), (inf, float(sh)), (inf, inf), (float(sw), -inf), (-inf, float(sh)), (-inf, -inf),
+                (float(sw), nan), (nan, float(sh)), (nan, nan)]
+    rng = np.random.default_rng(seed)
+    scenes = []
+    for m in matrices:
+        sc = Scene(width, height, fmt, 0xFF000000 if fmt != FMT_A8 else 0)
+        sc.set_transform(m)
+        for x, y in pts_i:
+            for w, h in sizes_i:
+                sc.set_fill_style_rgba32(int(rng.integers(0, 1 << 24)) | (int(rng.integers(8, 48)) << 24))
+                sc.fill_rect_i(x, y, w, h)
+        for x, y in pts_d:
+            for w, h in sizes_d:
+                sc.set_fill_style_rgba32(int(rng.integers(0, 1 << 24)) | (int(rng.integers(8, 48)) << 24))
+                sc.fill_rect_d(x, y, w, h)
+        scenes.append(sc)
+    return scenes
+
+
+def invalid_geometry_scene(seed: int = 1, width: int = 160, height: int = 120, fmt: int = FMT_PRGB32,
+                           count: int = 160) -> Scene:
+    """Polygons and line paths, part of them carrying a NaN vertex,
+    interleaved with valid polygons and cubic paths, under identity, scaling
+    and rotating transforms.  A line from a vertex inside the canvas to a NaN
+    vertex gets an invalid clip-flag combination in the reference: the whole
+    fill fails with BL_ERROR_INVALID_GEOMETRY and its edges are reverted
+    (raster/edgebuilder_p.h:1584-1586, raster/rastercontextops.cpp:38-45).
+    The NaN vertex is therefore the second one, after a first vertex inside
+    the canvas, so the failing line is the first line of the fill.  Two
+    other placements are left out on purpose: where the line before the NaN
+    vertex starts outside, the reference's clipper decides from clip flags
+    it carries from line to line, partly recomputed
+    (raster/edgebuilder_p.h:1300-1420); and where edges of the same fill were
+    already stored, the reference's revert leaves its band links behind (its
+    own assertions fire: support/zeroallocator.cpp:39,
+    raster/analyticrasterizer_p.h:468).  Infinite or astronomically large vertices are covered by the
+    rect matrix of clip_robustness_scenes only: in polygons and curves the
+    reference itself writes out of bounds, trips assertions
+    (raster/edgebuilder_p.h:2506, raster/analyticrasterizer_p.h:468) or
+    subdivides curves to its recursion limit."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(width, height, fmt, 0xFF204060 if fmt != FMT_A8 else 0x20)
+    W, H = float(width), float(height)
+    nan = float("nan")
+    transforms = [(1.0, 0.0, 0.0, 1.0, 0.0, 0.0), (1.5, 0.0, 0.0, 0.75, 3.0, -2.0), (0.8, 0.6, -0.6, 0.8, 40.0, -10.0)]
+    tr = transforms[0]
+    for i in range(count):
+        if i % 40 == 0:
+            tr = transforms[(i // 40) % len(transforms)]
+            sc.set_transform(tr)
+            sc.set_fill_rule(int(rng.integers(0, 2)))
+        sc.set_fill_style_rgba32(_rand_rgba(rng) | 0x40000000)
+        kind = int(rng.integers(0, 3))
+        n = int(rng.integers(3, 9))
+        cx, cy = rng.uniform(-0.2 * W, 1.2 * W), rng.uniform(-0.2 * H, 1.2 * H)
+        vtx = np.stack([cx + rng.uniform(-0.4, 0.4, n) * W, cy + rng.uniform(-0.4, 0.4, n) * H], axis=1)
+        if kind == 2:
+            # valid cubic path between the invalid fills
+            m = (n - 1) // 3
+            if m:
+                sc.fill_path([CMD_MOVE] + [CMD_CUBIC] * (3 * m) + [CMD_CLOSE],
+                             np.vstack([vtx[:1 + 3 * m], vtx[:1]]))
+                continue
+        if rng.random() < 0.6:
+            # vertices (after the user transform) strictly inside the canvas
+            ux = vtx[:, 0] * tr[0] + vtx[:, 1] * tr[2] + tr[4]
+            uy = vtx[:, 0] * tr[1] + vtx[:, 1] * tr[3] + tr[5]
+            inside = (ux > 0) & (ux < W) & (uy > 0) & (uy < H)
+            if inside[0]:
+                j = 1
+                which = int(rng.integers(0, 3))
+                if which != 1:
+                    vtx[j, 0] = nan
+                if which != 0:
+                    vtx[j, 1] = nan
+        if kind == 1:
+            sc.fill_path([CMD_MOVE] + [CMD_ON] * (n - 1) + [CMD_CLOSE], np.vstack([vtx, vtx[:1]]))
+        else:
+            sc.fill_polygon(vtx)
+    return sc
+
+
 def frontend_scene(seed: int = 1, width: int = 300, height: int = 220, fmt: int = FMT_PRGB32, count: int = 120) -> Scene:
     """The rest of the fill frontend in one seeded stream: fill_geometry of
     every non-arc geometry type, explicit-colour fills, clear_all /

@@ -33,7 +33,14 @@ enum {
   BL_AMD_ERROR_INVALID_VALUE = 0x10001u,
   BL_AMD_ERROR_INVALID_STATE = 0x10002u,
   BL_AMD_ERROR_NOT_IMPLEMENTED = 0x10007u,
+  BL_AMD_ERROR_INVALID_GEOMETRY = 0x10034u,
   BL_AMD_ERROR_DEVICE = 0x1F000u
+};
+
+/* BLContextErrorFlags (context.h:156-189) that rendering can raise. */
+enum {
+  BL_AMD_CONTEXT_ERROR_FLAG_INVALID_GEOMETRY = 0x00000004u,
+  BL_AMD_CONTEXT_ERROR_FLAG_OUT_OF_MEMORY = 0x40000000u
 };
 
 /* Pixel formats: BLFormat (core/format.h). */
@@ -64,7 +71,9 @@ int bl_amd_device_count(void);
 
 /* bl_image_create (core/image.h:157): device-resident image. */
 BLAmdResult bl_amd_image_create(BLAmdImage** out, int w, int h, uint32_t format);
-/* bl_image_destroy (core/image.h:152). */
+/* bl_image_destroy (core/image.h:152).  Drops the caller's reference: the
+ * pixels stay allocated while a context renders into the image, a fill style
+ * samples it or a recorded or replayable frame uses it. */
 BLAmdResult bl_amd_image_destroy(BLAmdImage* img);
 /* Copies host pixels into the image (bl_image_make_mutable, core/image.h:160, + write). */
 BLAmdResult bl_amd_image_upload(BLAmdImage* img, const void* pixels, intptr_t stride);
@@ -87,6 +96,24 @@ BLAmdResult bl_amd_context_begin(BLAmdContext* ctx, BLAmdImage* target);
 BLAmdResult bl_amd_context_end(BLAmdContext* ctx);
 /* bl_context_flush (context.h:649); flags: BL_CONTEXT_FLUSH_SYNC = 0x80000000 (context.h:107). */
 BLAmdResult bl_amd_context_flush(BLAmdContext* ctx, uint32_t flags);
+
+/* Errors found while the device rendered (edge building happens there, so a
+ * fill reports them at the flush, as the reference's asynchronous mode
+ * reports errors of deferred work, context.h:156-189): a fill whose line
+ * clipper meets NaN coordinates, or a curve with a non-finite control
+ * point, draws nothing and raises BL_AMD_CONTEXT_ERROR_FLAG_INVALID_GEOMETRY
+ * (raster/edgebuilder_p.h:1494-1496, raster/rastercontextops.cpp:38-45); a
+ * fill one of whose curve pieces flattens to more than 2^22 segments draws
+ * nothing and raises BL_AMD_CONTEXT_ERROR_FLAG_OUT_OF_MEMORY. */
+/* BLContext::accumulated_error_flags() (BLContextState, context.h:581-615):
+ * flags accumulated since begin(). */
+BLAmdResult bl_amd_context_get_error_flags(const BLAmdContext* ctx, uint32_t* flags);
+/* The first rendering call of the last flush that failed on the device:
+ * `call_index` counts the context's fill_* / clear_* calls from begin()
+ * (0-based, every such call counts), `code` the BLResult the reference's
+ * synchronous mode returns from that call (BL_AMD_ERROR_INVALID_GEOMETRY or
+ * BL_AMD_ERROR_OUT_OF_MEMORY); code 0 when none failed. */
+BLAmdResult bl_amd_context_get_deferred_error(const BLAmdContext* ctx, uint64_t* call_index, uint32_t* code);
 
 /* bl_context_set_comp_op (context.h:697): BLCompOp values; SRC_OVER, SRC_COPY,
  * PLUS, MULTIPLY, SCREEN render, others report NOT_IMPLEMENTED at fill time. */
@@ -204,6 +231,9 @@ BLAmdResult bl_amd_context_replay(BLAmdContext* ctx);
  * and one render grid covering every tile of every canvas. */
 BLAmdResult bl_amd_batch_create(BLAmdBatch** out);
 BLAmdResult bl_amd_batch_destroy(BLAmdBatch* batch);
+/* A context can be in a batch once, and no two contexts of a batch may
+ * render into the same image (their canvases are composited concurrently):
+ * such an add fails with BL_AMD_ERROR_INVALID_VALUE. */
 BLAmdResult bl_amd_batch_add(BLAmdBatch* batch, BLAmdContext* ctx);
 BLAmdResult bl_amd_batch_clear(BLAmdBatch* batch);
 BLAmdResult bl_amd_batch_flush(BLAmdBatch* batch);

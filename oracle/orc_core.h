@@ -101,6 +101,10 @@ OrcPt orc_mat_map(const OrcMat* m, double x, double y);
 void orc_edges_init(OrcEdges* e);
 void orc_edges_free(OrcEdges* e);
 void orc_edges_clear(OrcEdges* e);
+/* BL_ERROR_INVALID_GEOMETRY (core/api.h), returned by the two functions
+ * below when the line clipper meets NaN (raster/edgebuilder_p.h:1494-1496,
+ * 1584-1586); the caller then drops every edge of the call. */
+#define ORC_ERR_INVALID_GEOMETRY 0x10034
 /* Adds the edges of a path (BLPath command bytes + vertices), closed as for
  * filling (raster/rastercontextops.cpp add_filled_path_edges). */
 int orc_edges_add_path(OrcEdges* e, const uint8_t* cmds, const OrcPt* vtx, size_t n,

@@ -30,6 +30,10 @@ typedef struct {
   int32_t app_x, app_y;
   uint32_t app_sign;
   int app_open;
+  /* Set when the line clipper meets a clip-flag combination no case handles
+   * (NaN coordinates): the reference fails the path with
+   * BL_ERROR_INVALID_GEOMETRY (edgebuilder_p.h:1494-1496, 1584-1586). */
+  int invalid;
 } EB;
 
 /* ---- segment storage ---- */
@@ -128,7 +132,8 @@ static void clip_end(EB* eb, OrcPt p, OrcPt a, OrcPt b, uint32_t bf) {
       q.y = a.y + (q.x - a.x) * d.y / d.x;
       break;
     default:
-      return; /* NaN input: the reference reports BL_ERROR_INVALID_GEOMETRY */
+      eb->invalid = 1; /* edgebuilder_p.h:1584-1586 */
+      return;
   }
   add_line_segment(eb, p.x, p.y, q.x, q.y);
   double cby = orc_clamp(b.y, eb->clip.y0, eb->clip.y1);
@@ -223,6 +228,7 @@ static void eb_line(EB* eb, OrcPt a, OrcPt b) {
       af = fyf(eb, p);
       break;
     default:
+      eb->invalid = 1; /* edgebuilder_p.h:1494-1496 */
       return;
   }
 
@@ -715,6 +721,12 @@ static int split_cubic(const OrcPt* c, OrcPt* out) {
 
 /* quad_to / cubic_to for a single source curve (edgebuilder_p.h:1618-1890). */
 static void eb_curve(EB* eb, const OrcPt* c, int npts) {
+  /* A NaN or infinite control point has no defined flattening in the
+   * reference (it subdivides to its recursion limit or trips its own
+   * assertions, edgebuilder_p.h:2506); like the product, fail the fill. */
+  for (int i = 0; i < npts; i++) {
+    if (!isfinite(c[i].x) || !isfinite(c[i].y)) { eb->invalid = 1; return; }
+  }
   uint32_t common = 0xF, any = 0;
   for (int i = 0; i < npts; i++) { uint32_t f = fxy(eb, c[i]); common &= f; any |= f; }
   if (common) {
@@ -777,23 +789,27 @@ int orc_edges_add_path(OrcEdges* e, const uint8_t* cmds, const OrcPt* vtx, size_
         OrcPt b = xform(m, tt, vtx[i]);
         i++;
         eb_line(&eb, a, b);
+        if (eb.invalid) return ORC_ERR_INVALID_GEOMETRY;
         a = b;
       }
       else if (i + 2 <= n && cmds[i] == CMD_QUAD) {
         OrcPt c[3] = { a, xform(m, tt, vtx[i]), xform(m, tt, vtx[i + 1]) };
         i += 2;
         eb_curve(&eb, c, 3);
+        if (eb.invalid) return ORC_ERR_INVALID_GEOMETRY;
         a = c[2];
       }
       else if (i + 3 <= n && cmds[i] == CMD_CUBIC) {
         OrcPt c[4] = { a, xform(m, tt, vtx[i]), xform(m, tt, vtx[i + 1]), xform(m, tt, vtx[i + 2]) };
         i += 3;
         eb_curve(&eb, c, 4);
+        if (eb.invalid) return ORC_ERR_INVALID_GEOMETRY;
         a = c[3];
       }
       else {
         /* close the figure: fills are always closed */
         eb_line(&eb, a, start);
+        if (eb.invalid) return ORC_ERR_INVALID_GEOMETRY;
         break;
       }
     }
@@ -811,8 +827,9 @@ int orc_edges_add_poly(OrcEdges* e, const OrcPt* vtx, size_t n,
   for (size_t i = 1; i < n; i++) {
     OrcPt b = xform(m, tt, vtx[i]);
     eb_line(&eb, a, b);
+    if (eb.invalid) return ORC_ERR_INVALID_GEOMETRY;
     a = b;
   }
   eb_line(&eb, a, start);
-  return 0;
+  return eb.invalid ? ORC_ERR_INVALID_GEOMETRY : 0;
 }

@@ -238,6 +238,19 @@ static void fill_edges(Ctx* c, uint32_t fill_rule) {
   orc_fill_analytic(&c->canvas, &c->edges, &p, alpha_i(c), fill_rule);
 }
 
+/* A fill whose edges failed (INVALID_GEOMETRY): the reference resolves the
+ * style and comp op first and returns success for a NOP call before it
+ * builds any edge (BL_CONTEXT_RESOLVE_*, rastercontext.cpp:1358-1453); else
+ * the error is the call's result and the edges are reverted
+ * (rastercontextops.cpp:38-45). */
+static void fail_edges(Ctx* c, uint32_t err) {
+  OrcFetch f;
+  OrcPipe p;
+  static uint32_t lut[2048];
+  orc_edges_clear(&c->edges);
+  if (resolve(c, &f, lut, &p) != 1) note(c, err);
+}
+
 static void fill_path_origin(Ctx* c, const uint8_t* cmds, const OrcPt* vtx, size_t n, const double* origin) {
   OrcMat m; int tt; OrcBox clip;
   fixed_transform(c, &m, &tt);
@@ -250,7 +263,8 @@ static void fill_path_origin(Ctx* c, const uint8_t* cmds, const OrcPt* vtx, size
   }
   clip_fixed(c, &clip);
   orc_edges_clear(&c->edges);
-  orc_edges_add_path(&c->edges, cmds, vtx, n, &m, tt, &clip, kToleranceSq);
+  int r = orc_edges_add_path(&c->edges, cmds, vtx, n, &m, tt, &clip, kToleranceSq);
+  if (r) { fail_edges(c, (uint32_t)r); return; }
   fill_edges(c, c->fill_rule);
 }
 
@@ -262,7 +276,8 @@ static void fill_poly(Ctx* c, const OrcPt* vtx, size_t n, uint32_t rule) {
   fixed_transform(c, &m, &tt);
   clip_fixed(c, &clip);
   orc_edges_clear(&c->edges);
-  orc_edges_add_poly(&c->edges, vtx, n, &m, tt, &clip, kToleranceSq);
+  int r = orc_edges_add_poly(&c->edges, vtx, n, &m, tt, &clip, kToleranceSq);
+  if (r) { fail_edges(c, (uint32_t)r); return; }
   fill_edges(c, rule);
 }
 

@@ -218,4 +218,4 @@ def test_learned_sizing_of_a_batched_frame_matches_the_oracle(amd):
     for sc, out, err in zip(scenes, outs, errs):
         want, oerr = checkers.render_oracle(sc)
         assert err == oerr
-        assert checkers.diff_info(out, want)[0] <= 1
+        assert checkers.diff_info(out, want)[0] == 0
