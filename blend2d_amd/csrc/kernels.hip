@@ -640,19 +640,17 @@ void launch_cmd_tiles(const FrameDev& f, const FrameScratch& s, hipStream_t st) 
 
 // LDS cell buffer of one command in a tile: 16 rows x 64 cells, then one
 // backdrop word per row (the summed contributions of cells left of the tile).
+// Within a row the 16 groups of four cells are stored XOR-swizzled by the
+// row index, so the same cell column of different rows (the rows of one
+// steep edge, rasterized by neighbouring lanes) falls into different LDS
+// banks; a group stays four consecutive words, so the compositor still
+// reads a thread's four cells with one 128-bit load.
 constexpr int kCellWords = kTileH * kTileW + kTileH;
 constexpr int kBackdrop = kTileH * kTileW;
 
-struct TileSink {
-  uint32_t* cells;
-  int x0, y0;
-  __device__ __forceinline__ void add(int row, int x, uint32_t v) {
-    const int r = row - y0;
-    const int lx = x - x0;
-    const int idx = lx < 0 ? kBackdrop + r : r * kTileW + lx;
-    if ((unsigned)r < (unsigned)kTileH && lx < kTileW) atomicAdd(&cells[idx], v);
-  }
-};
+__device__ __forceinline__ int cell_index(int row, int x) {
+  return row * kTileW + ((((x >> 2) ^ row) & 15) << 2) + (x & 3);
+}
 
 __device__ __forceinline__ uint32_t calc_mask(uint32_t cov, uint32_t rule_mask, uint32_t alpha) {
   uint32_t m = ((uint32_t)((int32_t)cov >> 9) & rule_mask) - 256u;
@@ -747,7 +745,6 @@ constexpr int kRowsPerTask = B2AMD_ROWS_PER_TASK;
 // covered row to the backdrop; for one that reaches the tile the number of
 // raster tasks for the tile rows it covers is returned (raster_rows).
 __device__ __forceinline__ uint32_t seg_tile_rows(const DevSeg& g, uint32_t* cells, int X0, int Y0) {
-  const uint32_t sign = (uint32_t)g.x1 >> 31;
   const int32_t x1 = g.x1 & 0x7FFFFFFF;
   const int32_t cmin = (g.x0 < x1 ? g.x0 : x1) >> 8;
   const int32_t cmax = (g.x0 < x1 ? x1 : g.x0) >> 8;
@@ -756,12 +753,13 @@ __device__ __forceinline__ uint32_t seg_tile_rows(const DevSeg& g, uint32_t* cel
   if (r1 > Y0 + kTileH - 1) r1 = Y0 + kTileH - 1;
   if (cmin >= X0 + kTileW || r1 < r0) return 0u;
   if (cmax + 1 < X0) {
-    const uint32_t smask = 0u - sign;
+    // every cell of every row lies left of the tile: each row adds its y
+    // extent, signed by the edge direction (bit 31 of x1)
+    const int32_t sgn = g.x1 < 0 ? -1 : 1;
     for (int r = r0; r <= r1; r++) {
       const int lo_y = g.y0 > r * 256 ? g.y0 : r * 256;
       const int hi_y = g.y1 < (r + 1) * 256 ? g.y1 : (r + 1) * 256;
-      const uint32_t cover = ras::apply_sign((uint32_t)(hi_y - lo_y), smask);
-      atomicAdd(&cells[kBackdrop + r - Y0], cover << 9);
+      atomicAdd(&cells[kBackdrop + r - Y0], (uint32_t)(sgn * (hi_y - lo_y)) << 9);
     }
     return 0u;
   }
@@ -769,18 +767,20 @@ __device__ __forceinline__ uint32_t seg_tile_rows(const DevSeg& g, uint32_t* cel
 }
 
 // Rows `row` .. `row` + kRowsPerTask - 1 (within the tile) of a segment that
-// reaches the tile.  Advancing the rasterizer to any row and rasterizing
-// from there gives the cells the full walk gives for those rows (the
-// rasterizer's advance invariant, for any band height).
+// reaches the tile: the closed-form row emitter adds the cells inside the
+// tile and one backdrop value per row for the cells left of it.
 __device__ __forceinline__ void raster_rows(const DevSeg& g, uint32_t* cells, int X0, int Y0, int row) {
-  TileSink sink{cells, X0, Y0};
-  ras::Rasterizer rz;
-  rz.sign_mask = 0u - ((uint32_t)g.x1 >> 31);
-  if (!rz.prepare(g.x0, g.y0, g.x1 & 0x7FFFFFFF, g.y1)) return;
-  if (rz.st.ey0 < row) rz.advance_to_y(row);
-  const int last = row + kRowsPerTask - 1;
-  rz.band_end = last < Y0 + kTileH - 1 ? last : Y0 + kTileH - 1;
-  rz.rasterize(sink);
+  ras::Edge e;
+  if (!ras::edge_init(e, g.x0, g.y0, g.x1 & 0x7FFFFFFF, g.y1)) return;
+  const int32_t sgn = g.x1 < 0 ? -1 : 1;
+  const int last = row + kRowsPerTask - 1 < e.row1 ? row + kRowsPerTask - 1 : e.row1;
+  for (int r = row; r <= last && r < Y0 + kTileH; r++) {
+    const int lr = r - Y0;
+    const uint32_t bd = ras::emit_row(e, r, X0, kTileW, sgn, [&](int32_t k, uint32_t v) {
+      if (v) atomicAdd(&cells[cell_index(lr, k)], v);
+    });
+    if (bd) atomicAdd(&cells[kBackdrop + lr], bd);
+  }
 }
 
 // Destination pixels px .. px + 3 of a row (zero where outside the canvas).
@@ -936,9 +936,10 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
   const bool row_ok = y < (int)cv.h;
   const bool a8 = cv.format == kFmtA8;
 
-  // zero LDS
+  // zero LDS (each thread its own swizzled group of four cells)
+  const int grp = cell_index(row, q * 4);
 #pragma unroll
-  for (int b = 0; b < kBatch; b++) *(uint4*)&cells[b][row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
+  for (int b = 0; b < kBatch; b++) *(uint4*)&cells[b][grp] = make_uint4(0, 0, 0, 0);
   for (int i = t; i < kBatch * kTileH; i += kTileThreads) cells[i / kTileH][kBackdrop + i % kTileH] = 0;
 
   // load the destination tile
@@ -1109,9 +1110,9 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
 
         if (c.kind == kCmdAnalytic) {
           uint32_t* cb = cells[b];
-          uint4 cv4 = *(uint4*)&cb[row * kTileW + q * 4];
+          uint4 cv4 = *(uint4*)&cb[grp];
           uint32_t bdv = cb[kBackdrop + row];
-          *(uint4*)&cb[row * kTileW + q * 4] = make_uint4(0, 0, 0, 0);
+          *(uint4*)&cb[grp] = make_uint4(0, 0, 0, 0);
           uint32_t sum = cv4.x + cv4.y + cv4.z + cv4.w;
           uint32_t base = (256u << 9) + bdv + row_excl_scan16(sum);
           // every lane of the row has read the backdrop before the scan

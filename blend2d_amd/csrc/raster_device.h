@@ -1,31 +1,43 @@
-// Device analytic rasterizer: the reference's cover/area cell rasterizer
-// (raster/analyticrasterizer_p.h:289-1165) with banding, where the band is the
-// 16-row tile the calling workgroup owns.  A segment that starts above the
-// tile is first advanced with advance_to_y() (analyticrasterizer_p.h:371-459);
-// the reference's own test (raster/analyticrasterizer_test.cpp:34-158) pins
-// that advancing equals iterated banded rasterization, which is what lets
-// every tile start a segment at its own first row.  Cell writes go to a Sink
-// (tile-local LDS cells plus a per-row backdrop for cells left of the tile).
+// Coverage cells of one edge segment on one pixel row, in closed form.
+//
+// The reference rasterizes an edge by walking it scanline by scanline with
+// two incremental divisions (raster/analyticrasterizer_p.h:289-1165): one
+// steps x down the row boundaries, the other steps y across the column
+// boundaries, each rounding its quotient to the nearest value.  A tile of
+// the render kernel needs one row of one edge at a time, so this file
+// evaluates those two sequences directly at any index instead of stepping
+// to it:
+//
+//   x on the row boundary y = 256 k     X(k) = x0 +- floor(((256 k - y0) dx + ceil(dy / 2)) / dy)
+//   y on the column boundary x = 256 m  Y(m) = y0 + floor((|256 m - x0| dy + ceil(dx / 2)) / dx)
+//
+// (both are the fixed points of the reference's lift / remainder / error
+// accumulators, whose error term starts at floor(d / 2) - 1; the start and
+// end rows use the edge's own end points).  Within a row the edge enters at
+// X(r) and leaves at X(r + 1); every cell between gets the cover of the y
+// interval between the column crossings around it and the area of its x
+// extent.  How the crossing y is taken relative to the row differs with the
+// slope, as in the reference: an edge at least as tall as wide (at most one
+// crossing per row) keeps it modulo 256, a flatter one as is.
+//
+// Only prefix sums of a row's cells are ever used (the coverage of pixel x
+// is the sum of cells 0..x), so a tile keeps the cells inside it and folds
+// every cell left of it into one backdrop value, its cover << 9; the area a
+// cell left of the tile carries into the first cell of the tile cancels out
+// of every prefix sum inside the tile and is dropped on both sides.
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define B2AMD_HD __host__ __device__ __forceinline__
+#else
+#define B2AMD_HD inline
+#endif
 
 namespace b2amd {
 namespace ras {
-
-enum { A8_SHIFT = 8, A8_SCALE = 256, A8_MASK = 255 };
-enum { F_INITIAL = 1, F_VERT_OR_SINGLE = 2, F_RTL = 4 };
-
-struct RasState {
-  int ex0, ey0, ex1, ey1;
-  int fx0, fy0, fx1, fy1;
-  int xErr, yErr, xDlt, yDlt, xRem, yRem, xLift, yLift;
-  int dx, dy, savedFy1;
-  uint32_t flags;
-};
-
-__device__ __forceinline__ uint32_t apply_sign(uint32_t x, uint32_t mask) { return (x ^ mask) - mask; }
 
 // floor(n / d) for d > 0.  Below 2^53 the quotient comes from the correctly
 // rounded double division: with Q = n / d < 2^53 / d the rounding error is
@@ -33,687 +45,179 @@ __device__ __forceinline__ uint32_t apply_sign(uint32_t x, uint32_t mask) { retu
 // from every integer, so truncating the rounded quotient gives floor(Q).
 // The 24.8 coordinates the rasterizer sees keep every dividend far below
 // 2^53; the integer division stays as the exact path for anything larger.
-__device__ __forceinline__ uint64_t udiv_floor(uint64_t n, uint32_t d) {
+B2AMD_HD uint64_t udiv_floor(uint64_t n, uint64_t d) {
   if (n < (1ull << 53)) return (uint64_t)__builtin_trunc((double)n / (double)d);
-  return n / (uint64_t)d;
+  return n / d;
 }
 
-__device__ __forceinline__ void acc_err_step(int* acc, int* iter, int step, int correction) {
-  *iter -= step;
-  int mask = *iter >> 31;
-  *acc -= mask;
-  *iter += mask & correction;
-}
-
-__device__ __forceinline__ void acc_err_multi_step(int* acc, int* iter, int step, int correction, int count) {
-  int64_t i = (int64_t)(uint32_t)(*iter);
-  i -= (int64_t)((uint64_t)(uint32_t)step * (uint32_t)count);
-  if (i < 0) {
-    int n = (int)udiv_floor((uint64_t)(-i) + (uint32_t)correction - 1u, (uint32_t)correction);
-    *acc += n;
-    i += (int64_t)correction * n;
-  }
-  *iter = (int)i;
-}
-
-template<typename Sink>
-__device__ __forceinline__ void merge(Sink& sink, int row, int x, uint32_t cover, uint32_t area) {
-  sink.add(row, x, (cover << 9) - area);
-  sink.add(row, x + 1, area);
-}
-
-struct Rasterizer {
-  RasState st;
-  uint32_t sign_mask;
-  int band_end;   // last row this call may write
-
-  /* analyticrasterizer_p.h:289-360 */
-  __device__ __forceinline__ bool prepare(int p0x, int p0y, int p1x, int p1y) {
-    RasState* s = &st;
-    if (p0y == p1y) return 0;
-
-    s->dx = p1x - p0x;
-    s->dy = p1y - p0y;
-    s->flags = F_INITIAL;
-    if (s->dx < 0) { s->flags |= F_RTL; s->dx = -s->dx; }
-
-    s->ex0 = p0x >> A8_SHIFT;
-    s->ey0 = p0y >> A8_SHIFT;
-    s->ex1 = p1x >> A8_SHIFT;
-    s->ey1 = (p1y - 1) >> A8_SHIFT;
-
-    s->fx0 = p0x & A8_MASK;
-    s->fy0 = p0y & A8_MASK;
-    s->fx1 = p1x & A8_MASK;
-    s->fy1 = ((p1y - 1) & A8_MASK) + 1;
-
-    s->savedFy1 = s->fy1;
-    if (s->ey0 != s->ey1) s->fy1 = A8_SCALE;
-
-    if (s->ex0 == s->ex1 && (s->ey0 == s->ey1 || s->dx == 0)) {
-      s->flags |= F_VERT_OR_SINGLE;
-      return 1;
-    }
-
-    uint64_t x_base = (uint64_t)(uint32_t)s->dx * A8_SCALE;
-    uint64_t y_base = (uint64_t)(uint32_t)s->dy * A8_SCALE;
-
-    const uint64_t xq = udiv_floor(x_base, (uint32_t)s->dy);
-    const uint64_t yq = udiv_floor(y_base, (uint32_t)s->dx);
-    s->xLift = (int)xq;
-    s->xRem = (int)(x_base - xq * (uint32_t)s->dy);
-    s->yLift = (int)yq;
-    s->yRem = (int)(y_base - yq * (uint32_t)s->dx);
-
-    s->xDlt = s->dx;
-    s->yDlt = s->dy;
-    s->xErr = (s->dy >> 1) - 1;
-    s->yErr = (s->dx >> 1) - 1;
-
-    if (s->ey0 != s->ey1) {
-      uint64_t p = (uint64_t)(A8_SCALE - (uint32_t)s->fy0) * (uint32_t)s->dx;
-      const uint64_t q = udiv_floor(p, (uint32_t)s->dy);
-      s->xDlt = (int)q;
-      s->xErr -= (int)(p - q * (uint32_t)s->dy);
-      acc_err_step(&s->xDlt, &s->xErr, 0, s->dy);
-    }
-
-    if (s->ex0 != s->ex1) {
-      uint64_t p = (uint64_t)((s->flags & F_RTL) ? (uint32_t)s->fx0 : A8_SCALE - (uint32_t)s->fx0) * (uint32_t)s->dy;
-      const uint64_t q = udiv_floor(p, (uint32_t)s->dx);
-      s->yDlt = (int)q;
-      s->yErr -= (int)(p - q * (uint32_t)s->dx);
-      acc_err_step(&s->yDlt, &s->yErr, 0, s->dx);
-    }
-
-    s->yDlt += s->fy0;
-    return 1;
-  }
-
-  /* analyticrasterizer_p.h:371-459 */
-  __device__ __forceinline__ void advance_to_y(int y_target) {
-    RasState* s = &st;
-    if (y_target <= s->ey0) return;
-
-    if (!(s->flags & F_VERT_OR_SINGLE)) {
-      int ny = y_target - s->ey0;
-      s->xDlt += s->xLift * (ny - 1);
-      acc_err_multi_step(&s->xDlt, &s->xErr, s->xRem, s->dy, ny - 1);
-
-      if (s->flags & F_RTL) {
-        s->fx0 -= s->xDlt;
-        if (s->fx0 < 0) {
-          int nx = -(s->fx0 >> A8_SHIFT);
-          s->ex0 -= nx;
-          s->fx0 &= A8_MASK;
-          acc_err_multi_step(&s->yDlt, &s->yErr, s->yRem, s->dx, nx);
-          s->yDlt += s->yLift * nx;
-        }
-        if (!(s->dy >= s->dx)) {
-          if (!s->fx0) {
-            s->fx0 = A8_SCALE;
-            s->ex0--;
-            acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-            s->yDlt += s->yLift;
-          }
-        }
-        if (y_target == s->ey1 && s->dy >= s->dx) {
-          s->fy1 = s->savedFy1;
-          s->xDlt = ((s->ex0 - s->ex1) << A8_SHIFT) + s->fx0 - s->fx1;
-        }
-        else {
-          s->xDlt = s->xLift;
-          acc_err_step(&s->xDlt, &s->xErr, s->xRem, s->dy);
-        }
-      }
-      else {
-        s->fx0 += s->xDlt;
-        if (s->fx0 >= A8_SCALE) {
-          int nx = s->fx0 >> A8_SHIFT;
-          s->ex0 += nx;
-          s->fx0 &= A8_MASK;
-          acc_err_multi_step(&s->yDlt, &s->yErr, s->yRem, s->dx, nx);
-          s->yDlt += s->yLift * nx;
-        }
-        if (y_target == s->ey1 && s->dy >= s->dx) {
-          s->fy1 = s->savedFy1;
-          s->xDlt = ((s->ex1 - s->ex0) << A8_SHIFT) + s->fx1 - s->fx0;
-        }
-        else {
-          s->xDlt = s->xLift;
-          acc_err_step(&s->xDlt, &s->xErr, s->xRem, s->dy);
-        }
-      }
-
-      if (s->dy >= s->dx) {
-        s->yDlt &= A8_MASK;
-      }
-      else {
-        int y = ny;
-        if (s->flags & F_INITIAL) y--;
-        s->yDlt -= y * A8_SCALE;
-      }
-    }
-    else {
-      if (y_target == s->ey1) s->fy1 = s->savedFy1;
-    }
-
-    s->fy0 = 0;
-    s->ey0 = y_target;
-    s->flags &= ~(uint32_t)F_INITIAL;
-  }
-
-  /* analyticrasterizer_p.h:466-1165, with kOptionBandOffset | kOptionRecordMinXMaxX
-   * and kOptionBandingMode when `banding` is set.  Returns 1 when the edge is
-   * fully rasterized. */
-  template<typename Sink>
-  __device__ __forceinline__ bool rasterize(Sink& sink) {
-    const int banding = 1;
-    RasState* s = &st;
-    int ey1_end = s->ey1;
-    if (banding && ey1_end > band_end) ey1_end = band_end;
-    if (ey1_end < s->ey0) return true;
-
-    size_t i = (unsigned)ey1_end - (unsigned)s->ey0;
-    int cell = s->ey0;
-
-    if (banding) s->ey0 += (int)i + 1;
-
-    const uint32_t sm = sign_mask;
-    const uint32_t full_cover = apply_sign(A8_SCALE, sm);
-
-    if (s->flags & F_VERT_OR_SINGLE) {
-      uint32_t cover;
-      uint32_t area = (uint32_t)s->fx0 + (uint32_t)s->fx1;
-        
-      cover = apply_sign((uint32_t)(s->fy1 - s->fy0), sm);
-      merge(sink, cell, s->ex0, cover, cover * area);
-
-      if (banding) {
-        if (!i) {
-          if (s->ey0 > s->ey1) return 1;
-          s->fy0 = 0;
-          s->fy1 = s->ey0 == s->ey1 ? s->savedFy1 : A8_SCALE;
-          return 0;
-        }
-      }
-      else {
-        if (!i) return 1;
-      }
-
-      cell++;
-      cover = full_cover;
-      while (--i) {
-        merge(sink, cell, s->ex0, cover, cover * area);
-        cell++;
-      }
-
-      if (banding) {
-        if (s->ey0 <= s->ey1) {
-          merge(sink, cell, s->ex0, cover, cover * area);
-          s->fy0 = 0;
-          s->fy1 = s->ey0 == s->ey1 ? s->savedFy1 : A8_SCALE;
-          return 0;
-        }
-      }
-
-      cover = apply_sign((uint32_t)s->savedFy1, sm);
-      merge(sink, cell, s->ex0, cover, cover * area);
-      return 1;
-    }
-    else if (s->dy >= s->dx) {
-      if (banding) i += (s->ey0 <= s->ey1);
-
-      if (s->flags & F_RTL) {
-              for (;;) {
-          uint32_t area = (uint32_t)s->fx0;
-          uint32_t cov0, cov1;
-
-          s->fx0 -= s->xDlt;
-          if (s->fx0 < 0) {
-            s->ex0--;
-            s->fx0 += A8_SCALE;
-            s->yDlt &= A8_MASK;
-
-            if (!area) {
-              area = A8_SCALE;
-              acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-              s->yDlt += s->yLift;
-              goto VertRightToLeftSingleFirstOrLast;
-            }
-
-            cov0 = apply_sign((uint32_t)(s->yDlt - s->fy0), sm);
-            area = cov0 * area;
-            merge(sink, cell, s->ex0 + 1, cov0, area);
-
-            cov0 = apply_sign((uint32_t)(s->fy1 - s->yDlt), sm);
-            area = cov0 * ((uint32_t)s->fx0 + A8_SCALE);
-            merge(sink, cell, s->ex0, cov0, area);
-
-            acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-            s->yDlt += s->yLift;
-          }
-          else {
-  VertRightToLeftSingleFirstOrLast:
-            cov0 = apply_sign((uint32_t)(s->fy1 - s->fy0), sm);
-            area = cov0 * (area + (uint32_t)s->fx0);
-            merge(sink, cell, s->ex0, cov0, area);
-          }
-
-          s->fy0 = 0;
-          cell++;
-
-          if (!i) {
-                      if (banding) {
-              if (s->ey0 > s->ey1) return 1;
-              s->xDlt = s->xLift;
-              acc_err_step(&s->xDlt, &s->xErr, s->xRem, s->dy);
-              return 0;
-            }
-            return 1;
-          }
-
-          while (--i) {
-            s->xDlt = s->xLift;
-            acc_err_step(&s->xDlt, &s->xErr, s->xRem, s->dy);
-
-            area = (uint32_t)s->fx0;
-            s->fx0 -= s->xDlt;
-
-            if (s->fx0 < 0) {
-              s->ex0--;
-              s->fx0 += A8_SCALE;
-              s->yDlt &= A8_MASK;
-
-              if (!area) {
-                area = A8_SCALE;
-                acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-                s->yDlt += s->yLift;
-                goto VertRightToLeftSingleInLoop;
-              }
-
-              cov1 = apply_sign((uint32_t)s->yDlt, sm);
-              area = cov1 * area;
-              sink.add(cell, s->ex0 + 2, area);
-
-              cov0 = full_cover - cov1;
-              cov1 = (cov1 << 9) - area;
-              area = cov0 * ((uint32_t)s->fx0 + A8_SCALE);
-
-              cov0 = (cov0 << 9) - area;
-              cov1 = cov1 + area;
-
-              sink.add(cell, s->ex0 + 0, cov0);
-              sink.add(cell, s->ex0 + 1, cov1);
-              cell++;
-
-              acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-              s->yDlt += s->yLift;
-            }
-            else {
-  VertRightToLeftSingleInLoop:
-              area = full_cover * (area + (uint32_t)s->fx0);
-              merge(sink, cell, s->ex0, full_cover, area);
-              cell++;
-            }
-          }
-
-          if (banding) {
-            if (s->ey0 >= s->ey1) {
-              s->fy1 = s->savedFy1;
-              s->xDlt = ((s->ex0 - s->ex1) << A8_SHIFT) + s->fx0 - s->fx1;
-              if (s->ey0 == s->ey1) {
-                              return 0;
-              }
-            }
-            else {
-                          s->xDlt = s->xLift;
-              acc_err_step(&s->xDlt, &s->xErr, s->xRem, s->dy);
-              return 0;
-            }
-          }
-          else {
-            s->fy1 = s->savedFy1;
-            s->xDlt = ((s->ex0 - s->ex1) << A8_SHIFT) + s->fx0 - s->fx1;
-          }
-        }
-      }
-      else {
-              for (;;) {
-          uint32_t area = (uint32_t)s->fx0;
-          uint32_t cov0, cov1;
-
-          s->fx0 += s->xDlt;
-
-          if (s->fx0 <= A8_SCALE) {
-            cov0 = apply_sign((uint32_t)(s->fy1 - s->fy0), sm);
-            area = cov0 * (area + (uint32_t)s->fx0);
-            merge(sink, cell, s->ex0, cov0, area);
-
-            if (s->fx0 == A8_SCALE) {
-              s->ex0++;
-              s->fx0 = 0;
-              s->yDlt += s->yLift;
-              acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-            }
-          }
-          else {
-            s->ex0++;
-            s->fx0 &= A8_MASK;
-            s->yDlt &= A8_MASK;
-
-            cov0 = apply_sign((uint32_t)(s->yDlt - s->fy0), sm);
-            area = cov0 * (area + A8_SCALE);
-            merge(sink, cell, s->ex0 - 1, cov0, area);
-
-            cov0 = apply_sign((uint32_t)(s->fy1 - s->yDlt), sm);
-            area = cov0 * (uint32_t)s->fx0;
-            merge(sink, cell, s->ex0, cov0, area);
-
-            s->yDlt += s->yLift;
-            acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-          }
-
-          s->fy0 = 0;
-          cell++;
-
-          if (!i) {
-                      if (banding) {
-              if (s->ey0 > s->ey1) return 1;
-              s->yDlt += s->yLift;
-              acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-              return 0;
-            }
-            return 1;
-          }
-
-          while (--i) {
-            s->xDlt = s->xLift;
-            acc_err_step(&s->xDlt, &s->xErr, s->xRem, s->dy);
-
-            area = (uint32_t)s->fx0;
-            s->fx0 += s->xDlt;
-
-            if (s->fx0 <= A8_SCALE) {
-              area = full_cover * (area + (uint32_t)s->fx0);
-              merge(sink, cell, s->ex0, full_cover, area);
-              cell++;
-
-              if (s->fx0 < A8_SCALE) continue;
-
-              s->ex0++;
-              s->fx0 = 0;
-            }
-            else {
-              s->fx0 &= A8_MASK;
-              s->yDlt &= A8_MASK;
-
-              cov0 = apply_sign((uint32_t)s->yDlt, sm);
-              cov1 = cov0 * (area + A8_SCALE);
-
-              cov0 = (cov0 << 9) - cov1;
-              sink.add(cell, s->ex0 + 0, cov0);
-              s->ex0++;
-
-              cov0 = apply_sign(A8_SCALE - (uint32_t)s->yDlt, sm);
-              area = cov0 * (uint32_t)s->fx0;
-
-              cov0 = (cov0 << 9) - area + cov1;
-              sink.add(cell, s->ex0 + 0, cov0);
-              sink.add(cell, s->ex0 + 1, area);
-              cell++;
-            }
-
-            s->yDlt += s->yLift;
-            acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-          }
-
-          if (banding) {
-            if (s->ey0 >= s->ey1) {
-              s->fy1 = s->savedFy1;
-              s->xDlt = ((s->ex1 - s->ex0) << A8_SHIFT) + s->fx1 - s->fx0;
-              if (s->ey0 == s->ey1) {
-                              return 0;
-              }
-            }
-            else {
-                          s->xDlt = s->xLift;
-              acc_err_step(&s->xDlt, &s->xErr, s->xRem, s->dy);
-              return 0;
-            }
-          }
-          else {
-            s->fy1 = s->savedFy1;
-            s->xDlt = ((s->ex1 - s->ex0) << A8_SHIFT) + s->fx1 - s->fx0;
-          }
-        }
-      }
-    }
-    else {
-      size_t j = 1;
-      int x_local = (s->ex0 << A8_SHIFT) + s->fx0;
-      uint32_t cover, area;
-
-      if (s->flags & F_RTL) {
-      
-        if (s->flags & F_INITIAL) {
-          s->flags &= ~(uint32_t)F_INITIAL;
-          j = i;
-          i = 1;
-
-          cover = apply_sign((uint32_t)(s->yDlt - s->fy0), sm);
-          if (s->fx0 - s->xDlt < 0) goto HorzRightToLeftInside;
-
-          x_local -= s->xDlt;
-          cover = apply_sign((uint32_t)(s->fy1 - s->fy0), sm);
-          area = cover * (uint32_t)(s->fx0 * 2 - s->xDlt);
-          merge(sink, cell, s->ex0, cover, area);
-
-          if ((x_local & A8_MASK) == 0) {
-            s->yDlt += s->yLift;
-            acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-          }
-
-          s->xDlt = s->xLift;
-          acc_err_step(&s->xDlt, &s->xErr, s->xRem, s->dy);
-          cell++;
-          i--;
-        }
-
-        for (;;) {
-          while (i) {
-            s->ex0 = (x_local - 1) >> A8_SHIFT;
-            s->fx0 = ((x_local - 1) & A8_MASK) + 1;
-
-  HorzRightToLeftSkip:
-            s->yDlt -= A8_SCALE;
-            cover = apply_sign((uint32_t)s->yDlt, sm);
-
-  HorzRightToLeftInside:
-            x_local -= s->xDlt;
-            {
-              int ex_local = x_local >> A8_SHIFT;
-              int fx_local = x_local & A8_MASK;
-
-              area = cover * (uint32_t)s->fx0;
-              for (int guard = 0; s->ex0 != ex_local && guard < 65536; guard++) {
-                merge(sink, cell, s->ex0, cover, area);
-
-                cover = (uint32_t)s->yLift;
-                acc_err_step((int*)&cover, &s->yErr, s->yRem, s->dx);
-                s->yDlt += (int32_t)cover;
-
-                cover = apply_sign(cover, sm);
-                area = cover * A8_SCALE;
-                s->ex0--;
-              }
-
-              cover += apply_sign((uint32_t)(s->fy1 - s->yDlt), sm);
-              area = cover * ((uint32_t)fx_local + A8_SCALE);
-              merge(sink, cell, s->ex0, cover, area);
-
-              if (fx_local == 0) {
-                s->yDlt += s->yLift;
-                acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-              }
-            }
-
-            s->xDlt = s->xLift;
-            acc_err_step(&s->xDlt, &s->xErr, s->xRem, s->dy);
-            cell++;
-            i--;
-          }
-
-          s->fy0 = 0;
-          s->fy1 = A8_SCALE;
-
-          if (!j) {
-                      if (banding) {
-              s->ex0 = (x_local - 1) >> A8_SHIFT;
-              s->fx0 = ((x_local - 1) & A8_MASK) + 1;
-              return s->ey0 > s->ey1;
-            }
-            return 1;
-          }
-
-          i = j - 1;
-          j = 1;
-
-          if (!i) {
-            i = 1;
-            j = 0;
-
-            int is_last = banding ? s->ey0 > s->ey1 : 1;
-            if (!is_last) continue;
-
-            s->xDlt = x_local - ((s->ex1 << A8_SHIFT) + s->fx1);
-            s->fy1 = s->savedFy1;
-
-            s->ex0 = (x_local - 1) >> A8_SHIFT;
-            s->fx0 = ((x_local - 1) & A8_MASK) + 1;
-
-            if (s->fx0 - s->xDlt >= 0) {
-              cover = apply_sign((uint32_t)s->fy1, sm);
-              area = cover * (uint32_t)(s->fx0 * 2 - s->xDlt);
-              merge(sink, cell, s->ex0, cover, area);
-                          return 1;
-            }
-            goto HorzRightToLeftSkip;
-          }
-        }
-      }
-      else {
-      
-        if (s->flags & F_INITIAL) {
-          s->flags &= ~(uint32_t)F_INITIAL;
-          j = i;
-          i = 1;
-
-          cover = apply_sign((uint32_t)(s->yDlt - s->fy0), sm);
-          if (s->fx0 + s->xDlt > A8_SCALE) goto HorzLeftToRightInside;
-
-          x_local += s->xDlt;
-          cover = apply_sign((uint32_t)(s->fy1 - s->fy0), sm);
-          area = cover * ((uint32_t)s->fx0 * 2 + (uint32_t)s->xDlt);
-          merge(sink, cell, s->ex0, cover, area);
-
-          if (s->fx0 + s->xDlt == A8_SCALE) {
-            s->yDlt += s->yLift;
-            acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-          }
-
-          s->xDlt = s->xLift;
-          acc_err_step(&s->xDlt, &s->xErr, s->xRem, s->dy);
-          cell++;
-          i--;
-        }
-
-        for (;;) {
-          while (i) {
-            s->ex0 = x_local >> A8_SHIFT;
-            s->fx0 = x_local & A8_MASK;
-
-  HorzLeftToRightSkip:
-            s->yDlt -= A8_SCALE;
-            cover = apply_sign((uint32_t)s->yDlt, sm);
-
-  HorzLeftToRightInside:
-            x_local += s->xDlt;
-            {
-              int ex_local = (x_local - 1) >> A8_SHIFT;
-              int fx_local = ((x_local - 1) & A8_MASK) + 1;
-
-              area = cover * ((uint32_t)s->fx0 + A8_SCALE);
-              for (int guard = 0; s->ex0 != ex_local && guard < 65536; guard++) {
-                merge(sink, cell, s->ex0, cover, area);
-
-                cover = (uint32_t)s->yLift;
-                acc_err_step((int*)&cover, &s->yErr, s->yRem, s->dx);
-                s->yDlt += (int32_t)cover;
-
-                cover = apply_sign(cover, sm);
-                area = cover * A8_SCALE;
-                s->ex0++;
-              }
-
-              cover += apply_sign((uint32_t)(s->fy1 - s->yDlt), sm);
-              area = cover * (uint32_t)fx_local;
-              merge(sink, cell, s->ex0, cover, area);
-
-              if (fx_local == A8_SCALE) {
-                s->yDlt += s->yLift;
-                acc_err_step(&s->yDlt, &s->yErr, s->yRem, s->dx);
-              }
-            }
-
-            s->xDlt = s->xLift;
-            acc_err_step(&s->xDlt, &s->xErr, s->xRem, s->dy);
-            cell++;
-            i--;
-          }
-
-          s->fy0 = 0;
-          s->fy1 = A8_SCALE;
-
-          if (!j) {
-                      if (banding) {
-              s->ex0 = x_local >> A8_SHIFT;
-              s->fx0 = x_local & A8_MASK;
-              return s->ey0 > s->ey1;
-            }
-            return 1;
-          }
-
-          i = j - 1;
-          j = 1;
-
-          if (!i) {
-            i = 1;
-            j = 0;
-
-            int is_last = banding ? s->ey0 > s->ey1 : 1;
-            if (!is_last) continue;
-
-            s->xDlt = ((s->ex1 << A8_SHIFT) + s->fx1) - x_local;
-            s->fy1 = s->savedFy1;
-
-            s->ex0 = x_local >> A8_SHIFT;
-            s->fx0 = x_local & A8_MASK;
-
-            if (s->fx0 + s->xDlt <= A8_SCALE) {
-              cover = apply_sign((uint32_t)s->fy1, sm);
-              area = cover * ((uint32_t)s->fx0 * 2 + (uint32_t)s->xDlt);
-              merge(sink, cell, s->ex0, cover, area);
-                          return 1;
-            }
-            goto HorzLeftToRightSkip;
-          }
-        }
-      }
-    }
-  }
-
+// Coordinates beyond this magnitude never come out of the edge builder for a
+// canvas (it clips to the canvas in 24.8); an edge carrying one is ignored.
+constexpr int32_t kCoordLimit = 1 << 30;
+
+struct Edge {
+  int32_t x0, y0, x1, y1;  // 24.8, y0 < y1
+  int32_t dx, dy;          // |x1 - x0|, y1 - y0
+  int32_t dir;             // +1: x grows downwards (or constant), -1: x shrinks
+  int32_t row0, row1;      // first and last pixel row it touches
+  int32_t top0;            // start y within row0 (0..255)
+  int32_t bot1;            // end y within row1 (1..256)
+  bool one_column;         // vertical, or within one cell: one cell column, constant area factor
+  bool steep;              // dy >= dx: at most one column crossing per row
 };
+
+// Returns false for edges that add nothing (horizontal) or that no clipped
+// geometry can produce.
+B2AMD_HD bool edge_init(Edge& e, int32_t x0, int32_t y0, int32_t x1, int32_t y1) {
+  if (y0 >= y1) return false;
+  if (x0 < -kCoordLimit || x0 > kCoordLimit || x1 < -kCoordLimit || x1 > kCoordLimit ||
+      y0 < -kCoordLimit || y1 > kCoordLimit)
+    return false;
+  e.x0 = x0; e.y0 = y0; e.x1 = x1; e.y1 = y1;
+  e.dir = x1 < x0 ? -1 : 1;
+  e.dx = x1 < x0 ? x0 - x1 : x1 - x0;
+  e.dy = y1 - y0;
+  e.row0 = y0 >> 8;
+  e.row1 = (y1 - 1) >> 8;
+  e.top0 = y0 & 255;
+  e.bot1 = ((y1 - 1) & 255) + 1;
+  e.one_column = (x0 >> 8) == (x1 >> 8) && (e.row0 == e.row1 || e.dx == 0);
+  e.steep = e.dy >= e.dx;
+  return true;
+}
+
+// x where the edge crosses the row boundary y = 256 k, row0 < k <= row1.
+B2AMD_HD int32_t x_at_row_boundary(const Edge& e, int32_t k) {
+  const uint64_t t = (uint64_t)(int64_t)(256 * (int64_t)k - e.y0);
+  const uint64_t q = udiv_floor(t * (uint64_t)e.dx + (uint64_t)(e.dy - (e.dy >> 1)), (uint64_t)e.dy);
+  return e.x0 + e.dir * (int32_t)q;
+}
+
+// Crossings of consecutive column boundaries in the direction of travel:
+// the boundary at distance `dist` from x0 is crossed at y0 + q, and each
+// next boundary is 256 further.
+struct ColumnWalk {
+  uint64_t q, rem;          // quotient and remainder of dist * dy + ceil(dx / 2) by dx
+  uint64_t lift, lift_rem;  // the same of 256 * dy
+  uint64_t dx;
+  B2AMD_HD void start(const Edge& e, uint64_t dist) {
+    dx = (uint64_t)e.dx;
+    const uint64_t num = dist * (uint64_t)e.dy + (dx - (dx >> 1));
+    q = udiv_floor(num, dx);
+    rem = num - q * dx;
+    const uint64_t step = 256u * (uint64_t)e.dy;
+    lift = udiv_floor(step, dx);
+    lift_rem = step - lift * dx;
+  }
+  B2AMD_HD void next() {
+    q += lift;
+    rem += lift_rem;
+    if (rem >= dx) { q++; rem -= dx; }
+  }
+};
+
+// Row `r` (row0 <= r <= row1) of edge `e`, with cover sign `sgn` (+1 or -1),
+// against a tile whose pixel columns are [x_left, x_left + width): calls
+// sink(i, v) to add v to tile cell i (0 <= i < width) and returns the
+// backdrop, the cover << 9 of the row's cells left of the tile.
+template<typename Sink>
+B2AMD_HD uint32_t emit_row(const Edge& e, int32_t r, int32_t x_left, int32_t width, int32_t sgn, Sink&& sink) {
+  const int32_t ytop = r == e.row0 ? e.top0 : 0;
+  const int32_t ybot = r == e.row1 ? e.bot1 : 256;
+  const int32_t x_end = x_left + width;   // first column right of the tile
+
+  // One cell: cover over the row, area factor fsum.
+  auto single = [&](int32_t c, int32_t fsum) -> uint32_t {
+    const int32_t cover = sgn * (ybot - ytop);
+    if (c < x_left) return (uint32_t)cover << 9;
+    if (c < x_end) {
+      const int32_t area = cover * fsum;
+      sink(c - x_left, ((uint32_t)cover << 9) - (uint32_t)area);
+      if (c + 1 < x_end) sink(c + 1 - x_left, (uint32_t)area);
+    }
+    return 0u;
+  };
+
+  if (e.one_column) return single(e.x0 >> 8, (e.x0 & 255) + (e.x1 & 255));
+
+  const int32_t xe = r == e.row0 ? e.x0 : x_at_row_boundary(e, r);
+  const int32_t xx = r == e.row1 ? e.x1 : x_at_row_boundary(e, r + 1);
+  const int32_t run = e.dir > 0 ? xx - xe : xe - xx;   // >= 0
+
+  // Entry cell and entry offset: moving right a point on a boundary starts
+  // the cell right of it, moving left the cell left of it (offset 256).
+  int32_t ce, fe;
+  if (e.dir > 0) { ce = xe >> 8; fe = xe & 255; }
+  else { ce = (xe - 1) >> 8; fe = ((xe - 1) & 255) + 1; }
+  if (e.dir > 0 ? fe + run <= 256 : fe - run >= 0) return single(ce, e.dir > 0 ? 2 * fe + run : 2 * fe - run);
+
+  // Several cells, ce to cx in the direction of travel.
+  const int32_t cx = e.dir > 0 ? (xx - 1) >> 8 : xx >> 8;
+  const int32_t fx = e.dir > 0 ? ((xx - 1) & 255) + 1 : xx & 255;
+  const int32_t lo = e.dir > 0 ? ce : cx, hi = e.dir > 0 ? cx : ce;
+  if (hi < x_left) return (uint32_t)(sgn * (ybot - ytop)) << 9;
+  if (lo >= x_end) return 0u;
+
+  // In-tile cells in the direction of travel: first k0, last k1.
+  const int32_t k0 = e.dir > 0 ? (ce > x_left ? ce : x_left) : (ce < x_end - 1 ? ce : x_end - 1);
+  const int32_t k1 = e.dir > 0 ? (cx < x_end - 1 ? cx : x_end - 1) : (cx > x_left ? cx : x_left);
+  // Crossing y of the boundary a cell is entered through, row-relative.
+  const int32_t row_y = 256 * r;
+  auto rel = [&](uint64_t q) -> int32_t {
+    const int32_t y = e.y0 + (int32_t)q;
+    return e.steep ? (y & 255) : y - row_y;
+  };
+  ColumnWalk walk;
+  int32_t y_in;
+  if (k0 == ce) {
+    y_in = ytop;
+    // the next boundary: right of ce / left of ce
+    const int32_t m = e.dir > 0 ? ce + 1 : ce;
+    walk.start(e, (uint64_t)(int64_t)(e.dir > 0 ? 256 * (int64_t)m - e.x0 : e.x0 - 256 * (int64_t)m));
+  }
+  else {
+    // entered through the tile edge: the left edge x_left moving right, the
+    // right edge x_end moving left
+    const int32_t m = e.dir > 0 ? k0 : k0 + 1;
+    walk.start(e, (uint64_t)(int64_t)(e.dir > 0 ? 256 * (int64_t)m - e.x0 : e.x0 - 256 * (int64_t)m));
+    y_in = rel(walk.q);
+    walk.next();
+  }
+  // Backdrop: cells left of the tile telescope to the y interval they span.
+  uint32_t backdrop = 0;
+  if (lo < x_left) {
+    // moving right they are the first cells (ytop .. y_in of the first tile
+    // cell), moving left the last ones (the crossing at x_left .. ybot)
+    if (e.dir > 0) backdrop = (uint32_t)(sgn * (y_in - ytop)) << 9;
+  }
+  int32_t carry = 0;   // moving right: area the previous tile cell passes into this one
+  for (int32_t k = k0;; k += e.dir) {
+    const bool last = k == cx;
+    const int32_t y_out = last ? ybot : rel(walk.q);
+    const int32_t f_in = k == ce ? fe : (e.dir > 0 ? 0 : 256);
+    const int32_t f_out = last ? fx : (e.dir > 0 ? 256 : 0);
+    const int32_t cover = sgn * (y_out - y_in);
+    const int32_t area = cover * (f_in + f_out);
+    const uint32_t own = ((uint32_t)cover << 9) - (uint32_t)area;
+    if (e.dir > 0) {
+      sink(k - x_left, own + (uint32_t)carry);
+      carry = area;
+    }
+    else {
+      // moving left, the cell right of this one was visited just before
+      sink(k - x_left, own);
+      if (k + 1 < x_end) sink(k + 1 - x_left, (uint32_t)area);
+    }
+    if (k == k1) {
+      if (e.dir > 0 && k + 1 < x_end) sink(k + 1 - x_left, (uint32_t)carry);
+      // moving left past the tile: the rest of the row lies left of it
+      if (e.dir < 0 && !last) backdrop = (uint32_t)(sgn * (ybot - y_out)) << 9;
+      break;
+    }
+    y_in = y_out;
+    walk.next();
+  }
+  return backdrop;
+}
 
 }  // namespace ras
 }  // namespace b2amd

@@ -776,6 +776,27 @@ size_t orc_raster_advance_check(const int32_t* edges, size_t n, uint32_t band_he
   return bad;
 }
 
+/* Test hook: the cells one segment (24.8, y0 < y1, sign 1 when its source
+ * went up) leaves in a cell buffer of `rows` rows of `stride` cells starting
+ * at pixel row 0, rasterized in one pass.  Returns 0 when prepare() rejects
+ * it (horizontal). */
+int orc_raster_segment(int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint32_t sign, uint32_t* cells, size_t stride,
+                       uint32_t rows) {
+  Ras r;
+  memset(&r, 0, sizeof(r));
+  r.cells = cells;
+  r.cell_stride = stride;
+  r.sign_mask = (uint32_t)0 - sign;
+  r.band_height = rows;
+  r.band_offset = 0;
+  r.band_end = rows - 1;
+  r.cell_min_x = UINT32_MAX;
+  g_touch_min = UINT32_MAX;
+  if (!ras_prepare(&r, x0, y0, x1, y1)) return 0;
+  ras_rasterize(&r, 0);
+  return 1;
+}
+
 /* rastercontext.cpp:4154-4193 with thread_count == 0 */
 uint32_t orc_band_height(uint32_t width) {
   uint32_t bh = 64;
