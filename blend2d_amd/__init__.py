@@ -38,7 +38,7 @@ SYMBOLS = [
     "bl_amd_context_set_fill_style_rgba32", "bl_amd_context_set_fill_style_gradient",
     "bl_amd_context_set_fill_style_pattern", "bl_amd_context_fill_path_d", "bl_amd_context_fill_polygon_d",
     "bl_amd_context_fill_rect_d", "bl_amd_context_fill_rect_i", "bl_amd_context_fill_all",
-    "bl_amd_context_get_stats", "bl_amd_context_replay",
+    "bl_amd_context_get_stats", "bl_amd_context_replay", "bl_amd_context_replay_noalloc",
     "bl_amd_context_get_error_flags", "bl_amd_context_get_deferred_error",
     "bl_amd_context_fill_all_rgba32", "bl_amd_context_fill_rect_i_rgba32", "bl_amd_context_fill_rect_d_rgba32",
     "bl_amd_context_fill_path_d_rgba32", "bl_amd_context_fill_geometry", "bl_amd_context_fill_geometry_rgba32",
@@ -156,6 +156,7 @@ def lib():
         "bl_amd_context_fill_all": ([P], U32),
         "bl_amd_context_get_stats": ([P, ctypes.POINTER(BLAmdStats)], U32),
         "bl_amd_context_replay": ([P], U32),
+        "bl_amd_context_replay_noalloc": ([P], U32),
         "bl_amd_context_get_error_flags": ([P, ctypes.POINTER(U32)], U32),
         "bl_amd_context_get_deferred_error": ([P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(U32)], U32),
         "bl_amd_context_fill_all_rgba32": ([P, U32], U32),
@@ -448,6 +449,11 @@ class BLContext:
     def replay(self):
         """Executes the last flushed frame again from its device-resident copy."""
         _check(lib().bl_amd_context_replay(self._h), "replay")
+
+    def replay_noalloc(self):
+        """replay() with exact sizing and every intermediate buffer in the
+        library's fixed per-device scratch region (no device allocation)."""
+        _check(lib().bl_amd_context_replay_noalloc(self._h), "replay_noalloc")
 
     def collect_costs(self):
         """Features and exact-count targets of the resident frame's fill

@@ -31,6 +31,7 @@ constexpr uint32_t kErrInvalid = BL_AMD_ERROR_INVALID_VALUE;
 constexpr uint32_t kErrState = BL_AMD_ERROR_INVALID_STATE;
 constexpr uint32_t kErrNotImpl = BL_AMD_ERROR_NOT_IMPLEMENTED;
 constexpr uint32_t kErrDevice = BL_AMD_ERROR_DEVICE;
+constexpr uint32_t kErrTooLarge = BL_AMD_ERROR_DATA_TOO_LARGE;
 
 // A failing HIP call is reported once on stderr (the BLResult alone does not
 // say which call failed or why) and turned into BL_AMD_ERROR_DEVICE.
@@ -152,6 +153,20 @@ DeviceQueue& device_queue() {
   return *q;
 }
 
+// Slices of one caller-owned device region, 256-byte aligned, handed out in
+// order; used instead of the runner's grow-only buffers by the execution
+// that allocates nothing.
+struct ScratchArena {
+  uint8_t* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* take(size_t bytes) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if (at > cap || bytes > cap - at) return nullptr;
+    used = at + bytes;
+    return base + at;
+  }
+};
+
 // Device-side state of the pipeline: the resident copy of the last uploaded
 // frame and the scratch of its stages.  A frame stays resident until the next
 // upload, so it can be executed again (replayed) with no host-side work.
@@ -200,7 +215,10 @@ struct Runner {
   // After the frame's last sync: reads the per-command failure bits back
   // when the device flagged any (`any` = the flag word read back).
   uint32_t read_failures(const FrameScratch& s, uint32_t any);
-  uint32_t execute_exact(Stats& stats);
+  // `arena` null: the runner's own buffers (grown as needed); otherwise
+  // every intermediate buffer is a slice of the arena and nothing is
+  // allocated (BL_AMD_ERROR_DATA_TOO_LARGE when the frame does not fit).
+  uint32_t execute_exact(Stats& stats, ScratchArena* arena = nullptr);
   // 0 done, 1 pool overflow (redo exactly), otherwise an error
   uint32_t execute_pool(Stats& stats, uint32_t mode);
   bool model_views(cm::FrameModel& m);
@@ -221,6 +239,13 @@ struct Runner {
   uint32_t replay(Stats& stats) {
     std::lock_guard<std::mutex> lock(device_queue().mu);
     return execute(stats);
+  }
+  // The same with exact sizing and every intermediate buffer in `arena`.
+  uint32_t replay_in(Stats& stats, ScratchArena& arena) {
+    std::lock_guard<std::mutex> lock(device_queue().mu);
+    stats = Stats{};
+    if (!resident) return BL_AMD_SUCCESS;
+    return execute_exact(stats, &arena);
   }
 };
 
@@ -576,60 +601,61 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
 // Exact sizing: counting pass of the edge builder, scan, read-back, emit;
 // bin count, scans, read-back, scatter.  Two host round trips read back the
 // sizes the next stage allocates (segment count, binned count).
-uint32_t Runner::execute_exact(Stats& stats) {
+uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
   stats = Stats{};
   auto t0 = std::chrono::steady_clock::now();
   const uint64_t n_mat = f.n_mat;
+  const uint32_t kErrFit = arena ? kErrTooLarge : kErrOOM;
+  // A device buffer of `bytes`: the runner's own, grown as needed, or the
+  // next slice of the arena.
+  auto take = [&](DevBuf& own, size_t bytes) -> void* {
+    if (arena) return arena->take(bytes);
+    return own.reserve(bytes) ? own.ptr : nullptr;
+  };
 
   size_t scan_words = scan_tmp_words(std::max<size_t>({(size_t)n_src, (size_t)item_cap, (size_t)n_mat, (size_t)f.n_bands}) + 1);
-  if (!item_cnt.reserve((n_src + 1) * 4) || !item_off.reserve((n_src + 1) * 4) ||
-      !items.reserve(std::max<size_t>(item_cap, 1) * sizeof(EdgeItem)) ||
-      !src_count.reserve(((size_t)item_cap + 1) * 4) || !src_off.reserve(((size_t)item_cap + 1) * 4) ||
-      !bounds.reserve((size_t)f.n_cmds * sizeof(CmdBounds)) ||
-      !mat_cnt.reserve((n_mat + 1) * 4) || !mat_off.reserve((n_mat + 1) * 4) ||
-      !mat_pres.reserve((n_mat + 1) * 4) || !list_off.reserve((n_mat + 1) * 4) ||
-      !cursor.reserve((n_mat + 1) * 4) || !scan_tmp.reserve(scan_words * 4) || !counters.reserve(64) ||
-      !cmd_tiles.reserve((size_t)f.n_cmds * 4) || !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 32) ||
-      !band_buf.reserve(((size_t)f.n_bands + 1) * 28) ||
-      !tile_px.reserve((size_t)f.n_tiles * 8) || !cmd_err.reserve((size_t)f.n_cmds * 4))
-    return kErrOOM;
-
   FrameScratch s;
   memset(&s, 0, sizeof(s));
-  s.tile_hdr = tile_hdr.as<uint4>();
-  s.group_hdr = group_hdr.as<uint4>();
-  s.band_nonsolid = band_buf.as<uint32_t>();
+  s.item_cnt = (uint32_t*)take(item_cnt, (n_src + 1) * 4);
+  s.item_off = (uint32_t*)take(item_off, (n_src + 1) * 4);
+  s.items = (EdgeItem*)take(items, std::max<size_t>(item_cap, 1) * sizeof(EdgeItem));
+  s.src_count = (uint32_t*)take(src_count, ((size_t)item_cap + 1) * 4);
+  s.src_off = (uint32_t*)take(src_off, ((size_t)item_cap + 1) * 4);
+  s.bounds = (CmdBounds*)take(bounds, (size_t)f.n_cmds * sizeof(CmdBounds));
+  s.mat_cnt = (uint32_t*)take(mat_cnt, (n_mat + 1) * 4);
+  s.mat_off = (uint32_t*)take(mat_off, (n_mat + 1) * 4);
+  s.mat_pres = (uint32_t*)take(mat_pres, (n_mat + 1) * 4);
+  s.list_off = (uint32_t*)take(list_off, (n_mat + 1) * 4);
+  uint32_t* cursor_p = (uint32_t*)take(cursor, (n_mat + 1) * 4);
+  s.scan_tmp = (uint32_t*)take(scan_tmp, scan_words * 4);
+  uint32_t* counters_p = (uint32_t*)take(counters, 64);
+  s.cmd_tiles = (uint32_t*)take(cmd_tiles, (size_t)f.n_cmds * 4);
+  s.tile_hdr = (uint4*)take(tile_hdr, (size_t)f.n_tiles * 16);
+  s.group_hdr = (uint4*)take(group_hdr, (size_t)f.n_groups * 32);
+  s.band_nonsolid = (uint32_t*)take(band_buf, ((size_t)f.n_bands + 1) * 28);
+  s.tile_px = (unsigned long long*)take(tile_px, (size_t)f.n_tiles * 8);
+  s.cmd_err = (uint32_t*)take(cmd_err, (size_t)f.n_cmds * 4);
+  if (!s.item_cnt || !s.item_off || !s.items || !s.src_count || !s.src_off || !s.bounds || !s.mat_cnt || !s.mat_off ||
+      !s.mat_pres || !s.list_off || !cursor_p || !s.scan_tmp || !counters_p || !s.cmd_tiles || !s.tile_hdr ||
+      !s.group_hdr || !s.band_nonsolid || !s.tile_px || !s.cmd_err)
+    return kErrFit;
   s.band_tiles = s.band_nonsolid + (f.n_bands + 1);
   s.band_sgroups = s.band_tiles + (f.n_bands + 1);
   s.band_ggroups = s.band_sgroups + (f.n_bands + 1);
   s.tile_off = s.band_ggroups + (f.n_bands + 1);
   s.sgroup_off = s.tile_off + (f.n_bands + 1);
   s.ggroup_off = s.sgroup_off + (f.n_bands + 1);
-  s.tile_px = tile_px.as<unsigned long long>();
-  s.item_cnt = item_cnt.as<uint32_t>();
-  s.item_off = item_off.as<uint32_t>();
-  s.items = items.as<EdgeItem>();
-  s.src_count = src_count.as<uint32_t>();
-  s.src_off = src_off.as<uint32_t>();
-  s.bounds = bounds.as<CmdBounds>();
-  s.mat_cnt = mat_cnt.as<uint32_t>();
-  s.mat_off = mat_off.as<uint32_t>();
-  s.mat_pres = mat_pres.as<uint32_t>();
-  s.list_off = list_off.as<uint32_t>();
-  s.scan_tmp = scan_tmp.as<uint32_t>();
-  s.pixels = counters.as<unsigned long long>();
-  s.cmd_tiles = cmd_tiles.as<uint32_t>();
-  s.cmd_err = cmd_err.as<uint32_t>();
-  s.err_any = counters.as<uint32_t>() + 4;
+  s.pixels = (unsigned long long*)counters_p;
+  s.err_any = counters_p + 4;
 
   cm::FrameModel model{};
-  if (collect) {
+  if (collect && !arena) {
     if (!model_views(model)) return kErrOOM;
     cm::launch_features(f, model, stream);
   }
 
   // Stage 1: edges (work items; count, scan, emit).
-  launch_fill_u32(counters.as<uint32_t>(), 0u, 5, stream);
+  launch_fill_u32(counters_p, 0u, 5, stream);
   launch_fill_u32(s.cmd_err, 0u, f.n_cmds, stream);
   launch_init_bounds(s.bounds, f.n_cmds, stream);
   launch_edge_items(f, s, stream);
@@ -648,10 +674,9 @@ uint32_t Runner::execute_exact(Stats& stats) {
   // segments of failed commands are not binned
   const bool some_failed = n_src && host_readback[5];
   if (n_segs >= 0x80000000u) return kErrDevice;
-  if (!segs.reserve((size_t)std::max<uint32_t>(n_segs, 1) * sizeof(DevSeg)) || !seg_cmd.reserve((size_t)std::max<uint32_t>(n_segs, 1) * 4))
-    return kErrOOM;
-  s.segs = segs.as<DevSeg>();
-  s.seg_cmd = seg_cmd.as<uint32_t>();
+  s.segs = (DevSeg*)take(segs, (size_t)std::max<uint32_t>(n_segs, 1) * sizeof(DevSeg));
+  s.seg_cmd = (uint32_t*)take(seg_cmd, (size_t)std::max<uint32_t>(n_segs, 1) * 4);
+  if (!s.segs || !s.seg_cmd) return kErrFit;
   launch_edges(f, s, false, item_cap, stream);
 
   // Stage 2: binning.
@@ -679,14 +704,12 @@ uint32_t Runner::execute_exact(Stats& stats) {
   if ((n_binned < n_segs && !some_failed) || (uint64_t)n_binned > (uint64_t)n_segs * std::max<uint32_t>(max_bands(), 1) ||
       n_list > n_mat)
     return kErrDevice;
-  if (!binned.reserve((size_t)std::max<uint32_t>(n_binned, 1) * sizeof(DevSeg)) || !list.reserve((size_t)std::max<uint32_t>(n_list, 1) * 16) ||
-      !list_style.reserve((size_t)std::max<uint32_t>(n_list, 1) * 4))
-    return kErrOOM;
-  s.binned = binned.as<DevSeg>();
-  s.list_ent = list.as<uint4>();
-  s.list_style = list_style.as<uint32_t>();
-  HIP_OK(hipMemcpyAsync(cursor.ptr, s.mat_off, (n_mat + 1) * 4, hipMemcpyDeviceToDevice, stream));
-  launch_bin_scatter(f, s, n_segs, cursor.as<uint32_t>(), stream);
+  s.binned = (DevSeg*)take(binned, (size_t)std::max<uint32_t>(n_binned, 1) * sizeof(DevSeg));
+  s.list_ent = (uint4*)take(list, (size_t)std::max<uint32_t>(n_list, 1) * 16);
+  s.list_style = (uint32_t*)take(list_style, (size_t)std::max<uint32_t>(n_list, 1) * 4);
+  if (!s.binned || !s.list_ent || !s.list_style) return kErrFit;
+  HIP_OK(hipMemcpyAsync(cursor_p, s.mat_off, (n_mat + 1) * 4, hipMemcpyDeviceToDevice, stream));
+  launch_bin_scatter(f, s, n_segs, cursor_p, stream);
   launch_list_build(f, s, stream);
 
   // Stage 3: render.
@@ -698,7 +721,7 @@ uint32_t Runner::execute_exact(Stats& stats) {
   launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback + 2, s.pixels, 8, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipMemcpyAsync(host_readback + 4, s.err_any, 4, hipMemcpyDeviceToHost, stream));
-  if (collect) {
+  if (collect && !arena) {
     cm::launch_actuals(f, s, model, n_segs, stream);
     const size_t nc = f.n_cmds;
     std::vector<float> feat(nc * cm::kIn);
@@ -719,7 +742,10 @@ uint32_t Runner::execute_exact(Stats& stats) {
   HIP_OK(hipStreamSynchronize(stream));
   stats.syncs++;
   if (hipError_t e_ = hipGetLastError(); e_ != hipSuccess) { report_hip("kernel launch", e_, __LINE__); return kErrDevice; }
-  if (uint32_t r = read_failures(s, host_readback[4])) return r;
+  // (the arena execution is a replay: its failures were reported by the
+  // flush that uploaded the frame, and it keeps no host-side list)
+  if (!arena)
+    if (uint32_t r = read_failures(s, host_readback[4])) return r;
   hist_segs = n_segs;
   hist_bins = n_binned;
   float ms = 0.f;
@@ -1846,6 +1872,18 @@ BLAmdResult bl_amd_context_get_stats(const BLAmdContext* ctx, BLAmdStats* out) {
 BLAmdResult bl_amd_context_replay(BLAmdContext* ctx) {
   if (uint32_t r = check_target(ctx)) return r;
   return ctx->runner.replay(ctx->stats);
+}
+
+BLAmdResult bl_amd_context_replay_noalloc(BLAmdContext* ctx) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (!ctx->runner.resident) return BL_AMD_SUCCESS;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return kErrDevice;
+  ScratchArena arena;
+  size_t bytes = 0;
+  if (noalloc_scratch(dev, (void**)&arena.base, &bytes) != hipSuccess) return kErrDevice;
+  arena.cap = bytes;
+  return ctx->runner.replay_in(ctx->stats, arena);
 }
 
 BLAmdResult bl_amd_batch_create(BLAmdBatch** out) {

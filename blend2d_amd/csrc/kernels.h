@@ -107,6 +107,10 @@ void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint3
 void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+
+// The device scratch region of bl_amd_context_replay_noalloc on device
+// `dev`: its address (looked up once per device) and size in bytes.
+hipError_t noalloc_scratch(int dev, void** ptr, size_t* bytes);
 void launch_init_bounds(CmdBounds* b, uint32_t n, hipStream_t st);
 
 }  // namespace b2amd

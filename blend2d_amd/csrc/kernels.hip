@@ -20,6 +20,8 @@
 #include "pipe_device.h"
 #include "raster_device.h"
 
+#include <mutex>
+
 namespace b2amd {
 
 // ---------------------------------------------------------------------------
@@ -124,6 +126,34 @@ void scan_exclusive(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp, 
 // ---------------------------------------------------------------------------
 // Utilities
 // ---------------------------------------------------------------------------
+
+// Scratch of the execution that allocates nothing
+// (bl_amd_context_replay_noalloc): every intermediate buffer of a frame is a
+// slice of this region.  Each device holds its own copy for the life of the
+// process; every call on a device shares it, so calls that overlap on one
+// device race (the library's flushes and replays are serialized by its
+// device queue).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(int dev, void** ptr, size_t* bytes) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    // the array's host-side name is not a device address: look the address
+    // up on this device (current) and use only that
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = p;
+  }
+  *ptr = addr[dev];
+  *bytes = kNoAllocScratchBytes;
+  return hipSuccess;
+}
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

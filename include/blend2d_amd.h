@@ -33,6 +33,7 @@ enum {
   BL_AMD_ERROR_INVALID_VALUE = 0x10001u,
   BL_AMD_ERROR_INVALID_STATE = 0x10002u,
   BL_AMD_ERROR_NOT_IMPLEMENTED = 0x10007u,
+  BL_AMD_ERROR_DATA_TOO_LARGE = 0x10032u,
   BL_AMD_ERROR_INVALID_GEOMETRY = 0x10034u,
   BL_AMD_ERROR_DEVICE = 0x1F000u
 };
@@ -224,6 +225,19 @@ BLAmdResult bl_amd_context_get_stats(const BLAmdContext* ctx, BLAmdStats* out);
  * display list.  The target and every pattern image the frame used must
  * still be alive; the frame is replaced by the next flush that renders. */
 BLAmdResult bl_amd_context_replay(BLAmdContext* ctx);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): bl_amd_context_replay with exact sizing, whose
+ * intermediate buffers (edge work items and segments, binned segments,
+ * band matrices and lists, tile headers, counters) are slices of one
+ * 64 MiB device region that the library holds per device for the life of
+ * the process; it calls no hipMalloc / hipFree.  A frame whose buffers do
+ * not fit fails with BL_AMD_ERROR_DATA_TOO_LARGE before anything is
+ * composited.  Every call on a device shares the region and leaves it dirty
+ * (the call clears what must start at zero); calls that overlap on one
+ * device race. */
+BLAmdResult bl_amd_context_replay_noalloc(BLAmdContext* ctx);
 
 /* ---- batching (no counterpart in the reference) ------------------------
  * Renders the pending calls of many contexts (canvases of any size and
