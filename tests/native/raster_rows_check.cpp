@@ -50,13 +50,25 @@ int main(int argc, char** argv) {
   long bad = 0, rows_checked = 0;
   for (long i = 0; i < n; i++) {
     int x0, y0, x1, y1;
-    const int kind = g.uni(0, 5);
+    const int kind = g.uni(0, 7);
     y0 = g.coord(kH * 256);
     y1 = g.coord(kH * 256);
     x0 = g.coord(kW * 256);
     if (kind == 0) x1 = std::min(kW * 256, std::max(0, x0 + g.uni(-300, 300)));          // near vertical
     else if (kind == 1) { x1 = g.coord(kW * 256); y1 = std::min(kH * 256, y0 + g.uni(1, 300)); }  // flat
     else if (kind == 2) x1 = x0;                                                           // vertical
+    else if (kind == 7) {                                                                  // 45 degrees (dx = dy or dy - 1)
+      const int d = g.uni(2, 256 * 12);
+      y1 = std::min(kH * 256, y0 + d);
+      const int dxx = (y1 - y0) - g.uni(0, 1);
+      x1 = g.uni(0, 1) ? x0 + dxx : x0 - dxx;
+      if (x1 < 0 || x1 > kW * 256) x1 = x0;
+    }
+    else if (kind == 6) {                                                                  // very steep, tall
+      y0 = g.uni(0, 256 * 4);
+      y1 = std::min(kH * 256, y0 + g.uni(256 * 8, 256 * 30));
+      x1 = std::min(kW * 256, std::max(0, x0 + g.uni(-12, 12)));
+    }
     else x1 = g.coord(kW * 256);
     if (y0 == y1) continue;
     if (y0 > y1) { std::swap(y0, y1); std::swap(x0, x1); }
@@ -67,22 +79,29 @@ int main(int argc, char** argv) {
     if (!b2amd::ras::edge_init(e, x0, y0, x1, y1)) { bad++; continue; }
     const int32_t sgn = sign ? -1 : 1;
     int lefts[8] = {0, 64, 128, 192, 256, g.uni(0, kW - 1), g.uni(0, kW - 1), std::max(0, (x0 >> 8) - g.uni(0, 64))};
-    for (int r = e.row0; r <= e.row1; r++) {
-      const uint32_t* row = &cells[(size_t)r * kStride];
-      for (int li = 0; li < 8; li++) {
-        const int xl = lefts[li];
-        const int width = li < 6 ? 64 : g.uni(1, 64);
-        uint32_t tile[64] = {0};
-        const uint32_t bd = b2amd::ras::emit_row(e, r, xl, width, sgn, [&](int32_t k, uint32_t v) {
-          if (k < 0 || k >= width) { bad++; return; }
-          tile[k] += v;
-        });
+    for (int li = 0; li < 8; li++) {
+      const int xl = lefts[li];
+      const int width = li < 6 ? 64 : g.uni(1, 64);
+      // row by row, in chunks of 1..4 rows
+      std::vector<uint32_t> tile((size_t)(e.row1 - e.row0 + 1) * 64, 0u), bdr(e.row1 - e.row0 + 1, 0u);
+      for (int r = e.row0; r <= e.row1;) {
+        const int last = std::min(e.row1, r + g.uni(0, 3));
+        for (int row = r; row <= last; row++) {
+          bdr[row - e.row0] += b2amd::ras::emit_row(e, row, xl, width, sgn, [&](int32_t k, uint32_t v) {
+            if (k < 0 || k >= width) { bad++; return; }
+            tile[(size_t)(row - e.row0) * 64 + k] += v;
+          });
+        }
+        r = last + 1;
+      }
+      for (int r = e.row0; r <= e.row1; r++) {
+        const uint32_t* row = &cells[(size_t)r * kStride];
         uint32_t want = 0;
         for (int x = 0; x < xl && x < kStride; x++) want += row[x];
-        uint32_t got = bd;
+        uint32_t got = bdr[r - e.row0];
         for (int k = 0; k < width && xl + k < kStride; k++) {
           want += row[xl + k];
-          got += tile[k];
+          got += tile[(size_t)(r - e.row0) * 64 + k];
           if (want != got) {
             if (bad < 10)
               fprintf(stderr, "mismatch: seg (%d,%d)-(%d,%d) sign %u row %d tile %d+%d pixel %d: want %d got %d\n",

@@ -4,7 +4,8 @@ reference rasterizer (oracle/orc_raster.c, pinned by the reference-rendered
 golden canvases): tests/native/raster_rows_check.cpp compares the coverage
 prefix sum of every pixel of every row of random and boundary-hugging
 segments (vertical, near-vertical, flat, on pixel boundaries, both
-directions and signs) for tiles at many offsets.  Two planted mistakes in a
+directions and signs, 45-degree and very steep ones) for tiles at many
+offsets.  Two planted mistakes in a
 copy of the emitter (the rounding term of the row-boundary x, and the
 modulo-256 crossing of steep edges) must make it fail."""
 import os
