@@ -879,6 +879,62 @@ def invalid_geometry_scene(seed: int = 1, width: int = 160, height: int = 120, f
     return sc
 
 
+def comp_op_folds_scene(seed: int = 1, width: int = 211, height: int = 157, fmt: int = FMT_PRGB32,
+                        count: int = 160) -> Scene:
+    """PLUS / MULTIPLY / SCREEN where the comp-op simplification of the
+    reference (core/compopsimplifyimpl_p.h) folds them into operators its
+    portable pipeline renders: a fully transparent solid source (ZERO32)
+    makes PLUS, MULTIPLY and SCREEN a NOP on every target; on an A8 target
+    MULTIPLY becomes SRC_OVER (SRC_COPY for an opaque source) and SCREEN
+    SRC_OVER (SRC_COPY); on XRGB32 an opaque MULTIPLY becomes MODULATE, which
+    neither side renders (BL_ERROR_NOT_IMPLEMENTED).  Interleaved with
+    SRC_OVER / SRC_COPY fills of translucent, opaque and transparent solids and
+    linear gradients, global alpha and box / polygon / path geometry.
+    Combinations the portable pipeline cannot render (PLUS, MULTIPLY, SCREEN
+    of a non-zero source on PRGB32 / XRGB32, PLUS on A8) are left out."""
+    rng = np.random.default_rng(seed)
+    clear = 0xFF000000 | int(rng.integers(0, 1 << 24)) if fmt != FMT_A8 else int(rng.integers(0, 256))
+    sc = Scene(width, height, fmt, clear)
+    ops = (COMP_OP_SRC_OVER, COMP_OP_SRC_COPY, COMP_OP_PLUS, COMP_OP_MULTIPLY, COMP_OP_SCREEN)
+    W, H = float(width), float(height)
+    for i in range(count):
+        op = int(ops[int(rng.integers(0, len(ops)))])
+        kind = int(rng.integers(0, 4))   # 0 zero, 1 opaque solid, 2 translucent solid, 3 linear gradient
+        folds_to_renderable = (
+            op in (COMP_OP_SRC_OVER, COMP_OP_SRC_COPY) or kind == 0 or
+            (fmt == FMT_A8 and op in (COMP_OP_MULTIPLY, COMP_OP_SCREEN)) or
+            (fmt == FMT_XRGB32 and op == COMP_OP_MULTIPLY and kind == 1))
+        if not folds_to_renderable:
+            kind = 0
+        sc.set_comp_op(op)
+        sc.set_global_alpha(1.0 if rng.random() < 0.7 else float(rng.uniform(0.1, 1.0)))
+        sc.set_fill_rule(int(rng.integers(0, 2)))
+        if kind == 0:
+            sc.set_fill_style_rgba32(0x00000000)
+        elif kind == 1:
+            sc.set_fill_style_rgba32(0xFF000000 | int(rng.integers(0, 1 << 24)))
+        elif kind == 2:
+            sc.set_fill_style_rgba32(int(rng.integers(1, 255)) << 24 | int(rng.integers(0, 1 << 24)))
+        else:
+            x0, y0 = float(rng.uniform(0, W)), float(rng.uniform(0, H))
+            sc.set_fill_style_gradient(GRADIENT_LINEAR, [x0, y0, x0 + float(rng.uniform(-80, 80)), y0 + float(rng.uniform(-80, 80))],
+                                       _rand_stops(rng, 2, 4), int(rng.integers(0, 3)))
+        g = rng.random()
+        x0, y0 = float(rng.uniform(-0.1, 0.9) * W), float(rng.uniform(-0.1, 0.9) * H)
+        w, h = float(rng.uniform(4, W * 0.5)), float(rng.uniform(4, H * 0.5))
+        if g < 0.3:
+            sc.fill_rect_i(int(x0), int(y0), int(w), int(h))
+        elif g < 0.5:
+            sc.fill_rect_d(x0, y0, w, h)
+        elif g < 0.8:
+            n = int(rng.integers(3, 9))
+            sc.fill_polygon(rng.random((n, 2)) * np.array([w, h]) + np.array([x0, y0]))
+        else:
+            cm, v = ellipse_path(x0 + w / 2, y0 + h / 2, w / 2, h / 2)
+            sc.fill_path(cm, v)
+    return sc
+
+
 def frontend_scene(seed: int = 1, width: int = 300, height: int = 220, fmt: int = FMT_PRGB32, count: int = 120) -> Scene:
     """The rest of the fill frontend in one seeded stream: fill_geometry of
     every non-arc geometry type, explicit-colour fills, clear_all /

@@ -750,6 +750,12 @@ uint32_t orc_composite_pixel(uint32_t op, uint32_t dst_format, uint32_t d, uint3
   return r;
 }
 
+/* orc_composite_pixel over arrays: out[i] = op(d[i], s[i], m[i]). */
+void orc_composite_array(uint32_t op, uint32_t dst_format, const uint32_t* d, const uint32_t* s, const uint32_t* m,
+                         size_t n, uint32_t* out) {
+  for (size_t i = 0; i < n; i++) out[i] = orc_composite_pixel(op, dst_format, d[i], s[i], m[i]);
+}
+
 uint32_t orc_simplify_op(uint32_t op, uint32_t dst_fmt, uint32_t src_fmt) {
   uint32_t d, s, id;
   uint32_t r = orc_simplify(op, dst_fmt, src_fmt, &d, &s, &id);

@@ -28,6 +28,9 @@ def oracle():
         lib.orc_gradient_lut.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_composite_pixel.restype = ctypes.c_uint32
         lib.orc_composite_pixel.argtypes = [ctypes.c_uint32] * 5
+        lib.orc_composite_array.restype = None
+        lib.orc_composite_array.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
         lib.orc_simplify_op.restype = ctypes.c_uint32
         lib.orc_simplify_op.argtypes = [ctypes.c_uint32] * 3
         lib.orc_div255.restype = ctypes.c_uint32

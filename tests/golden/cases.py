@@ -21,6 +21,9 @@ def cases():
             out[f"frontend_{fname}_s{seed}"] = lambda seed=seed, fmt=fmt: [S.frontend_scene(seed, fmt=fmt)]
         out[f"translated_patterns_{fname}_s1"] = lambda fmt=fmt: [S.translated_patterns_scene(
             1, 301, 211, 120, fmt, comp_ops=(S.COMP_OP_SRC_OVER, S.COMP_OP_SRC_COPY))]
+    for fmt, fname in ((S.FMT_PRGB32, "prgb32"), (S.FMT_XRGB32, "xrgb32"), (S.FMT_A8, "a8")):
+        for seed in (1, 2):
+            out[f"comp_op_folds_{fname}_s{seed}"] = lambda seed=seed, fmt=fmt: [S.comp_op_folds_scene(seed, fmt=fmt)]
     for fmt, fname in ((S.FMT_PRGB32, "prgb32"), (S.FMT_A8, "a8")):
         out[f"clip_robustness_{fname}"] = lambda fmt=fmt: S.clip_robustness_scenes(fmt=fmt)
     for fmt, fname in ((S.FMT_PRGB32, "prgb32"), (S.FMT_XRGB32, "xrgb32"), (S.FMT_A8, "a8")):
