@@ -39,7 +39,11 @@ SYMBOLS = [
     "bl_amd_context_set_fill_style_pattern", "bl_amd_context_fill_path_d", "bl_amd_context_fill_polygon_d",
     "bl_amd_context_fill_rect_d", "bl_amd_context_fill_rect_i", "bl_amd_context_fill_all",
     "bl_amd_context_get_stats", "bl_amd_context_replay", "bl_amd_context_replay_noalloc",
-    "bl_amd_context_get_error_flags", "bl_amd_context_get_deferred_error",
+    "bl_amd_context_get_error_flags", "bl_amd_context_get_deferred_error", "bl_amd_context_reset",
+    "bl_amd_context_fill_all_rgba64", "bl_amd_context_fill_rect_i_rgba64", "bl_amd_context_fill_rect_d_rgba64",
+    "bl_amd_context_fill_path_d_rgba64", "bl_amd_context_fill_geometry_rgba64",
+    "bl_amd_context_fill_all_ext", "bl_amd_context_fill_rect_i_ext", "bl_amd_context_fill_rect_d_ext",
+    "bl_amd_context_fill_path_d_ext", "bl_amd_context_fill_geometry_ext",
     "bl_amd_context_fill_all_rgba32", "bl_amd_context_fill_rect_i_rgba32", "bl_amd_context_fill_rect_d_rgba32",
     "bl_amd_context_fill_path_d_rgba32", "bl_amd_context_fill_geometry", "bl_amd_context_fill_geometry_rgba32",
     "bl_amd_context_clear_all", "bl_amd_context_clear_rect_i", "bl_amd_context_clear_rect_d",
@@ -108,6 +112,17 @@ def _geometry_arg(gtype: int, data):
     return None, ()
 
 
+class BLAmdStyle(ctypes.Structure):
+    """Mirror of BLAmdStyle (include/blend2d_amd.h): the style of one call."""
+    _fields_ = [("kind", ctypes.c_uint32), ("rgba32", ctypes.c_uint32), ("rgba64", ctypes.c_uint64),
+                ("gradient_type", ctypes.c_uint32), ("extend", ctypes.c_uint32),
+                ("values", ctypes.c_void_p), ("stop_offsets", ctypes.c_void_p), ("stop_rgba32", ctypes.c_void_p),
+                ("n_stops", ctypes.c_size_t), ("transform", ctypes.c_void_p), ("image", ctypes.c_void_p)]
+
+
+STYLE_RGBA32, STYLE_RGBA64, STYLE_GRADIENT, STYLE_PATTERN = 0, 1, 2, 3
+
+
 class BLError(RuntimeError):
     def __init__(self, code: int, what: str):
         super().__init__(f"{what} failed with BLResult 0x{code:X}")
@@ -158,6 +173,17 @@ def lib():
         "bl_amd_context_replay": ([P], U32),
         "bl_amd_context_replay_noalloc": ([P], U32),
         "bl_amd_context_get_error_flags": ([P, ctypes.POINTER(U32)], U32),
+        "bl_amd_context_reset": ([P], U32),
+        "bl_amd_context_fill_all_rgba64": ([P, ctypes.c_uint64], U32),
+        "bl_amd_context_fill_rect_i_rgba64": ([P, I, I, I, I, ctypes.c_uint64], U32),
+        "bl_amd_context_fill_rect_d_rgba64": ([P, D, D, D, D, ctypes.c_uint64], U32),
+        "bl_amd_context_fill_path_d_rgba64": ([P, P, P, P, SZ, ctypes.c_uint64], U32),
+        "bl_amd_context_fill_geometry_rgba64": ([P, U32, P, ctypes.c_uint64], U32),
+        "bl_amd_context_fill_all_ext": ([P, ctypes.POINTER(BLAmdStyle)], U32),
+        "bl_amd_context_fill_rect_i_ext": ([P, I, I, I, I, ctypes.POINTER(BLAmdStyle)], U32),
+        "bl_amd_context_fill_rect_d_ext": ([P, D, D, D, D, ctypes.POINTER(BLAmdStyle)], U32),
+        "bl_amd_context_fill_path_d_ext": ([P, P, P, P, SZ, ctypes.POINTER(BLAmdStyle)], U32),
+        "bl_amd_context_fill_geometry_ext": ([P, U32, P, ctypes.POINTER(BLAmdStyle)], U32),
         "bl_amd_context_get_deferred_error": ([P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(U32)], U32),
         "bl_amd_context_fill_all_rgba32": ([P, U32], U32),
         "bl_amd_context_fill_rect_i_rgba32": ([P, I, I, I, I, U32], U32),
@@ -381,17 +407,59 @@ class BLContext:
         else:
             _check(lib().bl_amd_context_clear_rect_d(self._h, float(x), float(y), float(w), float(h)), "clear_rect")
 
-    def fill_geometry(self, gtype: int, data=(), rgba32: int | None = None):
+    @staticmethod
+    def _style_arg(style):
+        """("rgba64", value) for a BLRgba64, else a BLAmdStyle and the
+        buffers it points into (kept alive by the caller)."""
+        st = BLAmdStyle()
+        keep = []
+        if isinstance(style, BLRgba64):
+            return "rgba64", int(style) & 0xFFFFFFFFFFFFFFFF, keep
+        if isinstance(style, int):
+            st.kind, st.rgba32 = STYLE_RGBA32, style & 0xFFFFFFFF
+        elif isinstance(style, BLGradient):
+            vals = np.ascontiguousarray(style.values, dtype=np.float64)
+            offs = np.ascontiguousarray([x[0] for x in style.stops] or [0.0], dtype=np.float64)
+            cols = np.ascontiguousarray([x[1] & 0xFFFFFFFF for x in style.stops] or [0], dtype=np.uint32)
+            tr = _mat(style.transform) if style.transform is not None else None
+            keep += [vals, offs, cols, tr]
+            st.kind, st.gradient_type, st.extend = STYLE_GRADIENT, style.type, style.extend
+            st.values, st.stop_offsets, st.stop_rgba32 = vals.ctypes.data, offs.ctypes.data, cols.ctypes.data
+            st.n_stops = len(style.stops)
+            st.transform = tr.ctypes.data if tr is not None else None
+        elif isinstance(style, BLPattern):
+            tr = _mat(style.transform) if style.transform is not None else None
+            keep.append(tr)
+            st.kind, st.extend, st.image = STYLE_PATTERN, style.extend, style.image.handle.value
+            st.transform = tr.ctypes.data if tr is not None else None
+        else:
+            raise TypeError("unsupported style")
+        return "ext", st, keep
+
+    def fill_geometry(self, gtype: int, data=(), rgba32: int | None = None, style=None):
         arg, keep = _geometry_arg(gtype, data)
         ptr = ctypes.byref(arg) if arg is not None else None
-        if rgba32 is None:
-            r = lib().bl_amd_context_fill_geometry(self._h, gtype, ptr)
+        L = lib()
+        if style is not None:
+            kind, v, skeep = self._style_arg(style)
+            r = (L.bl_amd_context_fill_geometry_rgba64(self._h, gtype, ptr, v) if kind == "rgba64"
+                 else L.bl_amd_context_fill_geometry_ext(self._h, gtype, ptr, ctypes.byref(v)))
+            del skeep
+        elif rgba32 is None:
+            r = L.bl_amd_context_fill_geometry(self._h, gtype, ptr)
         else:
-            r = lib().bl_amd_context_fill_geometry_rgba32(self._h, gtype, ptr, rgba32 & 0xFFFFFFFF)
+            r = L.bl_amd_context_fill_geometry_rgba32(self._h, gtype, ptr, rgba32 & 0xFFFFFFFF)
         del keep
         _check(r, "fill_geometry")
 
-    def fill_path(self, path, origin=None, rgba32: int | None = None):
+    def reset(self):
+        """bl_context_reset: renders pending calls, detaches, default state."""
+        r = lib().bl_amd_context_reset(self._h)
+        self.target = None
+        self._keep.clear()
+        _check(r, "reset")
+
+    def fill_path(self, path, origin=None, rgba32: int | None = None, style=None):
         if isinstance(path, BLPath):
             cmds, vtx = path.arrays()
         else:
@@ -399,7 +467,13 @@ class BLContext:
             cmds = np.ascontiguousarray(cmds, dtype=np.uint8)
             vtx = np.ascontiguousarray(vtx, dtype=np.float64).reshape(-1, 2)
         org = np.ascontiguousarray(origin, dtype=np.float64) if origin is not None else None
-        if rgba32 is None:
+        if style is not None:
+            kind, v, skeep = self._style_arg(style)
+            a = (self._h, _dptr(org) if org is not None else None, _dptr(cmds), _dptr(vtx), len(cmds))
+            r = (lib().bl_amd_context_fill_path_d_rgba64(*a, v) if kind == "rgba64"
+                 else lib().bl_amd_context_fill_path_d_ext(*a, ctypes.byref(v)))
+            del skeep
+        elif rgba32 is None:
             r = lib().bl_amd_context_fill_path_d(self._h, _dptr(org) if org is not None else None,
                                                  _dptr(cmds), _dptr(vtx), len(cmds))
         else:
@@ -411,8 +485,18 @@ class BLContext:
         vtx = np.ascontiguousarray(vtx, dtype=np.float64).reshape(-1, 2)
         _check(lib().bl_amd_context_fill_polygon_d(self._h, _dptr(vtx), len(vtx)), "fill_polygon")
 
-    def fill_rect(self, x, y, w, h, rgba32: int | None = None):
+    def fill_rect(self, x, y, w, h, rgba32: int | None = None, style=None):
         L = lib()
+        if style is not None:
+            kind, v, skeep = self._style_arg(style)
+            if all(isinstance(q, (int, np.integer)) for q in (x, y, w, h)):
+                a = (self._h, int(x), int(y), int(w), int(h))
+                r = L.bl_amd_context_fill_rect_i_rgba64(*a, v) if kind == "rgba64" else L.bl_amd_context_fill_rect_i_ext(*a, ctypes.byref(v))
+            else:
+                a = (self._h, float(x), float(y), float(w), float(h))
+                r = L.bl_amd_context_fill_rect_d_rgba64(*a, v) if kind == "rgba64" else L.bl_amd_context_fill_rect_d_ext(*a, ctypes.byref(v))
+            del skeep
+            return _check(r, "fill_rect")
         if all(isinstance(v, (int, np.integer)) for v in (x, y, w, h)):
             a = (self._h, int(x), int(y), int(w), int(h))
             r = L.bl_amd_context_fill_rect_i(*a) if rgba32 is None else L.bl_amd_context_fill_rect_i_rgba32(*a, rgba32 & 0xFFFFFFFF)
@@ -421,7 +505,13 @@ class BLContext:
             r = L.bl_amd_context_fill_rect_d(*a) if rgba32 is None else L.bl_amd_context_fill_rect_d_rgba32(*a, rgba32 & 0xFFFFFFFF)
         _check(r, "fill_rect")
 
-    def fill_all(self, rgba32: int | None = None):
+    def fill_all(self, rgba32: int | None = None, style=None):
+        if style is not None:
+            kind, v, skeep = self._style_arg(style)
+            r = (lib().bl_amd_context_fill_all_rgba64(self._h, v) if kind == "rgba64"
+                 else lib().bl_amd_context_fill_all_ext(self._h, ctypes.byref(v)))
+            del skeep
+            return _check(r, "fill_all")
         if rgba32 is None:
             _check(lib().bl_amd_context_fill_all(self._h), "fill_all")
         else:

@@ -1513,13 +1513,16 @@ BLAmdResult bl_amd_context_set_fill_style_rgba32(BLAmdContext* ctx, uint32_t rgb
   return BL_AMD_SUCCESS;
 }
 
-// init_non_solid_fetch_data, gradient branch (rastercontext.cpp:416-481)
-BLAmdResult bl_amd_context_set_fill_style_gradient(BLAmdContext* ctx, uint32_t type, const double* values, uint32_t extend,
-                                                   const double* stop_offsets, const uint32_t* stop_rgba32, size_t n_stops,
-                                                   const double* transform) {
-  if (uint32_t r = check_target(ctx)) return r;
+}  // extern "C"
+
+namespace {
+
+// init_non_solid_fetch_data, gradient branch (rastercontext.cpp:416-481):
+// the style takes the user transform of the moment.
+uint32_t build_gradient_style(BLAmdContext* ctx, StyleState& st, uint32_t type, const double* values, uint32_t extend,
+                              const double* stop_offsets, const uint32_t* stop_rgba32, size_t n_stops,
+                              const double* transform) {
   if (type > 2 || extend > 2 || !values || (n_stops && (!stop_offsets || !stop_rgba32))) return kErrInvalid;
-  StyleState& st = ctx->style;
   st = StyleState{};
   st.type = kStyleGradient;
   st.version = ++ctx->style_counter;
@@ -1554,11 +1557,10 @@ BLAmdResult bl_amd_context_set_fill_style_gradient(BLAmdContext* ctx, uint32_t t
   return BL_AMD_SUCCESS;
 }
 
-// init_non_solid_fetch_data, pattern branch (rastercontext.cpp:377-414)
-BLAmdResult bl_amd_context_set_fill_style_pattern(BLAmdContext* ctx, const BLAmdImage* image, uint32_t extend, const double* transform) {
-  if (uint32_t r = check_target(ctx)) return r;
+// init_non_solid_fetch_data, pattern branch (rastercontext.cpp:377-414).
+uint32_t build_pattern_style(BLAmdContext* ctx, StyleState& st, const BLAmdImage* image, uint32_t extend,
+                             const double* transform) {
   if (!image || extend > 2) return kErrInvalid;
-  StyleState& st = ctx->style;
   st = StyleState{};
   st.type = kStylePattern;
   st.version = ++ctx->style_counter;
@@ -1571,6 +1573,61 @@ BLAmdResult bl_amd_context_set_fill_style_pattern(BLAmdContext* ctx, const BLAmd
   }
   int r = init_pattern(st.dev, (uint32_t)image->w, (uint32_t)image->h, extend, ctx->pattern_quality, t);
   if (r == 0) st.status = 1;
+  return BL_AMD_SUCCESS;
+}
+
+// rgba32FromRgba64 (core/rgba_p.h:67-74): the high byte of each channel.
+uint32_t rgba32_from_rgba64(uint64_t v) {
+  return (uint32_t)(((v >> 56) & 0xFFu) << 24 | ((v >> 40) & 0xFFu) << 16 | ((v >> 24) & 0xFFu) << 8 | ((v >> 8) & 0xFFu));
+}
+
+// resolve_explicit_style_op (rastercontext.cpp:1277-1328): the call's own
+// style, built with the user transform of the moment; the context's style is
+// not touched.  Sets `err` for an invalid style description.
+Resolved resolve_explicit(BLAmdContext* ctx, const BLAmdStyle* style, bool nop_transform, uint32_t& err) {
+  err = BL_AMD_SUCCESS;
+  Resolved nop;
+  memset(&nop.cmd, 0, sizeof(nop.cmd));
+  nop.nop = 1;
+  if (!style) { err = kErrInvalid; return nop; }
+  switch (style->kind) {
+    case BL_AMD_STYLE_RGBA32: return resolve_solid(ctx, style->rgba32, nop_transform);
+    case BL_AMD_STYLE_RGBA64: return resolve_solid(ctx, rgba32_from_rgba64(style->rgba64), nop_transform);
+    case BL_AMD_STYLE_GRADIENT:
+    case BL_AMD_STYLE_PATTERN: {
+      StyleState st;
+      err = style->kind == BL_AMD_STYLE_GRADIENT
+                ? build_gradient_style(ctx, st, style->gradient_type, style->values, style->extend, style->stop_offsets,
+                                       style->stop_rgba32, style->n_stops, style->transform)
+                : build_pattern_style(ctx, st, style->image, style->extend, style->transform);
+      if (err) return nop;
+      return resolve_with(ctx, st, alpha_i(ctx), nop_transform);
+    }
+    default:
+      err = kErrInvalid;
+      return nop;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+BLAmdResult bl_amd_context_set_fill_style_gradient(BLAmdContext* ctx, uint32_t type, const double* values, uint32_t extend,
+                                                   const double* stop_offsets, const uint32_t* stop_rgba32, size_t n_stops,
+                                                   const double* transform) {
+  if (uint32_t r = check_target(ctx)) return r;
+  StyleState st;
+  if (uint32_t r = build_gradient_style(ctx, st, type, values, extend, stop_offsets, stop_rgba32, n_stops, transform)) return r;
+  ctx->style = st;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_context_set_fill_style_pattern(BLAmdContext* ctx, const BLAmdImage* image, uint32_t extend, const double* transform) {
+  if (uint32_t r = check_target(ctx)) return r;
+  StyleState st;
+  if (uint32_t r = build_pattern_style(ctx, st, image, extend, transform)) return r;
+  ctx->style = st;
   return BL_AMD_SUCCESS;
 }
 
@@ -1652,9 +1709,7 @@ BLAmdResult bl_amd_context_set_fill_alpha(BLAmdContext* ctx, double alpha) {
 // set_style_rgba64_impl (rastercontext.cpp:588-614): the high byte of each
 // 16-bit channel (RgbaInternal::rgba32FromRgba64, core/rgba_p.h:67-74).
 BLAmdResult bl_amd_context_set_fill_style_rgba64(BLAmdContext* ctx, uint64_t rgba64) {
-  uint32_t rgba32 = (uint32_t)(((rgba64 >> 56) & 0xFFu) << 24 | ((rgba64 >> 40) & 0xFFu) << 16 |
-                               ((rgba64 >> 24) & 0xFFu) << 8 | ((rgba64 >> 8) & 0xFFu));
-  return bl_amd_context_set_fill_style_rgba32(ctx, rgba32);
+  return bl_amd_context_set_fill_style_rgba32(ctx, rgba32_from_rgba64(rgba64));
 }
 
 // set_style_rgba_impl (rastercontext.cpp:616-655): NaN disables the style;
@@ -1826,6 +1881,93 @@ BLAmdResult bl_amd_context_fill_geometry_rgba32(BLAmdContext* ctx, uint32_t type
   if (uint32_t r = render_call(ctx)) return r;
   if (type > BL_AMD_GEOMETRY_MAX_VALUE) return kErrInvalid;
   return fill_geometry_resolved(ctx, resolve_solid(ctx, rgba32), type, data);
+}
+
+// _rgba64 forms: an inline rgba64 style (core/context.cpp:853-859 builds it
+// and calls the _ext entry), converted to rgba32 by its high bytes.
+BLAmdResult bl_amd_context_fill_all_rgba64(BLAmdContext* ctx, uint64_t rgba64) {
+  if (uint32_t r = render_call(ctx)) return r;
+  return fill_all_box(ctx, resolve_solid(ctx, rgba32_from_rgba64(rgba64), false));
+}
+
+BLAmdResult bl_amd_context_fill_rect_i_rgba64(BLAmdContext* ctx, int x, int y, int w, int h, uint64_t rgba64) {
+  if (uint32_t r = render_call(ctx)) return r;
+  return fill_rect_i(ctx, resolve_solid(ctx, rgba32_from_rgba64(rgba64)), x, y, w, h);
+}
+
+BLAmdResult bl_amd_context_fill_rect_d_rgba64(BLAmdContext* ctx, double x, double y, double w, double h, uint64_t rgba64) {
+  if (uint32_t r = render_call(ctx)) return r;
+  return fill_box_d(ctx, resolve_solid(ctx, rgba32_from_rgba64(rgba64)), x, y, x + w, y + h);
+}
+
+BLAmdResult bl_amd_context_fill_path_d_rgba64(BLAmdContext* ctx, const double* origin, const uint8_t* cmds,
+                                              const double* vertices, size_t n, uint64_t rgba64) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (n == 0) return BL_AMD_SUCCESS;
+  if (!cmds || !vertices) return kErrInvalid;
+  return fill_path_rule(ctx, resolve_solid(ctx, rgba32_from_rgba64(rgba64)), origin, cmds, vertices, n, ctx->fill_rule);
+}
+
+BLAmdResult bl_amd_context_fill_geometry_rgba64(BLAmdContext* ctx, uint32_t type, const void* data, uint64_t rgba64) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (type > BL_AMD_GEOMETRY_MAX_VALUE) return kErrInvalid;
+  return fill_geometry_resolved(ctx, resolve_solid(ctx, rgba32_from_rgba64(rgba64)), type, data);
+}
+
+// _ext forms: the call's own style (rastercontext.cpp:1277-1328).
+BLAmdResult bl_amd_context_fill_all_ext(BLAmdContext* ctx, const BLAmdStyle* style) {
+  if (uint32_t r = render_call(ctx)) return r;
+  uint32_t err;
+  const Resolved res = resolve_explicit(ctx, style, false, err);
+  return err ? err : fill_all_box(ctx, res);
+}
+
+BLAmdResult bl_amd_context_fill_rect_i_ext(BLAmdContext* ctx, int x, int y, int w, int h, const BLAmdStyle* style) {
+  if (uint32_t r = render_call(ctx)) return r;
+  uint32_t err;
+  const Resolved res = resolve_explicit(ctx, style, true, err);
+  return err ? err : fill_rect_i(ctx, res, x, y, w, h);
+}
+
+BLAmdResult bl_amd_context_fill_rect_d_ext(BLAmdContext* ctx, double x, double y, double w, double h, const BLAmdStyle* style) {
+  if (uint32_t r = render_call(ctx)) return r;
+  uint32_t err;
+  const Resolved res = resolve_explicit(ctx, style, true, err);
+  return err ? err : fill_box_d(ctx, res, x, y, x + w, y + h);
+}
+
+BLAmdResult bl_amd_context_fill_path_d_ext(BLAmdContext* ctx, const double* origin, const uint8_t* cmds,
+                                           const double* vertices, size_t n, const BLAmdStyle* style) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (n == 0) return BL_AMD_SUCCESS;
+  if (!cmds || !vertices) return kErrInvalid;
+  uint32_t err;
+  const Resolved res = resolve_explicit(ctx, style, true, err);
+  return err ? err : fill_path_rule(ctx, res, origin, cmds, vertices, n, ctx->fill_rule);
+}
+
+BLAmdResult bl_amd_context_fill_geometry_ext(BLAmdContext* ctx, uint32_t type, const void* data, const BLAmdStyle* style) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (type > BL_AMD_GEOMETRY_MAX_VALUE) return kErrInvalid;
+  uint32_t err;
+  const Resolved res = resolve_explicit(ctx, style, true, err);
+  return err ? err : fill_geometry_resolved(ctx, res, type, data);
+}
+
+// bl_context_reset (core/context.cpp:263-267): the context becomes a default
+// one; like end() (which the reference maps to reset, :329-332) the pending
+// calls are rendered first when a target is attached.
+BLAmdResult bl_amd_context_reset(BLAmdContext* ctx) {
+  if (!ctx) return kErrInvalid;
+  uint32_t r = BL_AMD_SUCCESS;
+  if (ctx->target) r = bl_amd_context_end(ctx);
+  ctx->reset_state();
+  reset_frame(ctx);
+  ctx->call_seq = 0;
+  ctx->error_flags = 0;
+  ctx->deferred_call = ~0ull;
+  ctx->deferred_code = 0;
+  return r;
 }
 
 static void copy_stats(const Stats& st, BLAmdStats* out) {

@@ -23,7 +23,46 @@ def _images(ctx_cls, sc):
 # them, and an error found on the device names its call by that count.
 RENDER_OPS = {S.OP_FILL_PATH, S.OP_FILL_POLYGON, S.OP_FILL_RECT_D, S.OP_FILL_RECT_I, S.OP_FILL_ALL,
               S.OP_CLEAR_ALL, S.OP_CLEAR_RECT_I, S.OP_CLEAR_RECT_D, S.OP_FILL_GEOMETRY, S.OP_FILL_ALL_RGBA32,
-              S.OP_FILL_RECT_I_RGBA32, S.OP_FILL_RECT_D_RGBA32, S.OP_FILL_PATH_EX}
+              S.OP_FILL_RECT_I_RGBA32, S.OP_FILL_RECT_D_RGBA32, S.OP_FILL_PATH_EX, S.OP_FILL_EXPLICIT}
+
+
+def _explicit(pl, images):
+    """Decodes a FILL_EXPLICIT payload: (target, style object, geometry)."""
+    from . import BLGradient, BLPattern, BLRgba64
+    target, kind, gtype, count, rgba = struct.unpack_from("<IIIIQ", pl)
+    at = 24
+    if kind == S.STYLE_RGBA32:
+        style = int(rgba) & 0xFFFFFFFF
+    elif kind == S.STYLE_RGBA64:
+        style = BLRgba64(rgba)
+    elif kind == S.STYLE_GRADIENT:
+        h = struct.unpack_from("<IIII6d6d", pl, at)
+        n = h[2]
+        stops = [struct.unpack_from("<dI", pl, at + 112 + 16 * k) for k in range(n)]
+        style = BLGradient(h[0], list(h[4:10]), h[1], stops, list(h[10:16]) if h[3] else None)
+        at += 112 + 16 * n
+    else:
+        h = struct.unpack_from("<IIII6d", pl, at)
+        style = BLPattern(images[h[0]], h[1], list(h[4:10]) if h[2] else None)
+        at += 64
+    if target == S.TARGET_RECT_I:
+        geom = [int(v) for v in struct.unpack_from("<4i", pl, at)]
+    elif target == S.TARGET_RECT_D:
+        geom = [float(v) for v in struct.unpack_from("<4d", pl, at)]
+    elif target == S.TARGET_PATH:
+        cmds = np.frombuffer(pl, dtype=np.uint8, count=count, offset=at)
+        vtx = np.frombuffer(pl, dtype=np.float64, count=2 * count, offset=at + ((count + 7) & ~7)).reshape(-1, 2)
+        geom = (cmds, vtx)
+    elif target == S.TARGET_GEOMETRY:
+        if gtype in S.GEOMETRY_ARRAY:
+            fmt = S.GEOMETRY_ARRAY[gtype]
+            size = struct.calcsize(fmt)
+            geom = (gtype, [struct.unpack_from(fmt, pl, at + k * size) for k in range(count)])
+        else:
+            geom = (gtype, struct.unpack_from(S.GEOMETRY_STRUCT[gtype], pl, at))
+    else:
+        geom = None
+    return target, style, geom
 
 
 def replay_into(ctx, sc: "S.Scene", images) -> list:
@@ -111,6 +150,16 @@ def replay_into(ctx, sc: "S.Scene", images) -> list:
             else:
                 data = ()
             call(ctx.fill_geometry, gtype, data, col if has_col else None)
+        elif op == S.OP_FILL_EXPLICIT:
+            target, style, geom = _explicit(pl, images)
+            if target == S.TARGET_ALL:
+                call(lambda: ctx.fill_all(style=style))
+            elif target in (S.TARGET_RECT_I, S.TARGET_RECT_D):
+                call(lambda: ctx.fill_rect(*geom, style=style))
+            elif target == S.TARGET_PATH:
+                call(lambda: ctx.fill_path(geom, style=style))
+            else:
+                call(lambda: ctx.fill_geometry(geom[0], geom[1], style=style))
         elif op == S.OP_FILL_PATH_EX:
             n, flags, col, _, ox, oy = struct.unpack_from("<4I2d", pl)
             cmds = np.frombuffer(pl, dtype=np.uint8, count=n, offset=32)

@@ -47,6 +47,11 @@ OP_FILL_ALL_RGBA32 = 21
 OP_FILL_RECT_I_RGBA32 = 22
 OP_FILL_RECT_D_RGBA32 = 23
 OP_FILL_PATH_EX = 24
+OP_FILL_EXPLICIT = 25
+
+# FILL_EXPLICIT targets and style kinds
+TARGET_ALL, TARGET_RECT_I, TARGET_RECT_D, TARGET_PATH, TARGET_GEOMETRY = range(5)
+STYLE_RGBA32, STYLE_RGBA64, STYLE_GRADIENT, STYLE_PATTERN = range(4)
 
 # BLGeometryType values (core/geometry.h).
 (GEOMETRY_NONE, GEOMETRY_BOXI, GEOMETRY_BOXD, GEOMETRY_RECTI, GEOMETRY_RECTD, GEOMETRY_CIRCLE, GEOMETRY_ELLIPSE,
@@ -223,6 +228,48 @@ class Scene:
         ox, oy = origin if origin is not None else (0.0, 0.0)
         head = struct.pack("<4I2d", len(cmds), flags, 0 if rgba32 is None else rgba32 & 0xFFFFFFFF, 0, ox, oy)
         self._cmd(OP_FILL_PATH_EX, head + _pad8(cmds.tobytes()) + vtx.tobytes())
+        self.fill_count += 1
+
+    def fill_explicit(self, target: int, style, geometry=None) -> None:
+        """A fill with a style for this call only: fill_*_rgba64 / fill_*_ext
+        of the reference.  `style`: ("rgba32", v), ("rgba64", v),
+        ("gradient", type, values, stops, extend, transform) or ("pattern",
+        image_index, extend, transform).  `geometry`: None (fill_all),
+        (x, y, w, h) for rects, (cmds, vtx) for paths, (type, data) for
+        fill_geometry."""
+        kind = {"rgba32": STYLE_RGBA32, "rgba64": STYLE_RGBA64, "gradient": STYLE_GRADIENT, "pattern": STYLE_PATTERN}[style[0]]
+        rgba = style[1] if kind in (STYLE_RGBA32, STYLE_RGBA64) else 0
+        body, gtype, count = b"", 0, 0
+        if kind == STYLE_GRADIENT:
+            _, gt, values, stops, extend, transform = style
+            vals = list(values) + [0.0] * (6 - len(values))
+            tr = list(transform) if transform is not None else [1.0, 0.0, 0.0, 1.0, 0.0, 0.0]
+            body = struct.pack("<IIII6d6d", gt, extend, len(stops), 1 if transform is not None else 0, *vals, *tr)
+            for off, c in stops:
+                body += struct.pack("<dII", off, c & 0xFFFFFFFF, 0)
+        elif kind == STYLE_PATTERN:
+            _, idx, extend, transform = style
+            tr = list(transform) if transform is not None else [1.0, 0.0, 0.0, 1.0, 0.0, 0.0]
+            body = struct.pack("<IIII6d", idx, extend, 1 if transform is not None else 0, 0, *tr)
+        if target == TARGET_RECT_I:
+            body += struct.pack("<4i", *[int(v) for v in geometry])
+        elif target == TARGET_RECT_D:
+            body += struct.pack("<4d", *[float(v) for v in geometry])
+        elif target == TARGET_PATH:
+            cmds = np.asarray(geometry[0], dtype=np.uint8)
+            vtx = np.asarray(geometry[1], dtype=np.float64).reshape(-1, 2)
+            count = len(cmds)
+            body += _pad8(cmds.tobytes()) + vtx.tobytes()
+        elif target == TARGET_GEOMETRY:
+            gtype, data = geometry
+            if gtype in GEOMETRY_ARRAY:
+                items = list(data)
+                body += b"".join(struct.pack(GEOMETRY_ARRAY[gtype], *it) for it in items)
+                count = len(items)
+            else:
+                body += struct.pack(GEOMETRY_STRUCT[gtype], *data)
+        head = struct.pack("<IIIIQ", target, kind, gtype, count, rgba & 0xFFFFFFFFFFFFFFFF)
+        self._cmd(OP_FILL_EXPLICIT, head + body)
         self.fill_count += 1
 
     # -- serialization ---------------------------------------------------
@@ -932,6 +979,72 @@ def comp_op_folds_scene(seed: int = 1, width: int = 211, height: int = 157, fmt:
         else:
             cm, v = ellipse_path(x0 + w / 2, y0 + h / 2, w / 2, h / 2)
             sc.fill_path(cm, v)
+    return sc
+
+
+def explicit_styles_scene(seed: int = 1, width: int = 233, height: int = 171, fmt: int = FMT_PRGB32,
+                          count: int = 90) -> Scene:
+    """Fills with a style for one call only -- fill_all / fill_rect_i /
+    fill_rect_d / fill_path_d / fill_geometry in their _rgba64 and _ext forms
+    (core/context.h:735-758; rgba32, rgba64, linear / radial / conic gradient
+    and pattern styles) -- interleaved with ordinary fills of the context's
+    own style, under changing user transforms (an explicit style takes the
+    transform of its call), global and fill alpha and SRC_OVER / SRC_COPY."""
+    rng = np.random.default_rng(seed)
+    clear = 0xFF000000 | int(rng.integers(0, 1 << 24)) if fmt != FMT_A8 else int(rng.integers(0, 256))
+    sc = Scene(width, height, fmt, clear)
+    img = sc.add_image(premultiplied_noise(rng, 37, 29))
+    W, H = float(width), float(height)
+    sc.set_fill_style_rgba32(0x80FF8040)
+    for i in range(count):
+        if i % 15 == 0:
+            t = rng.random()
+            cx, cy = float(rng.uniform(0, W)), float(rng.uniform(0, H))
+            xf = [1.0, 0.0, 0.0, 1.0, 0.0, 0.0] if t < 0.4 else rotation_about(float(rng.uniform(0, 2 * math.pi)), cx, cy)
+            sc.set_transform(xf)
+            sc.set_comp_op(COMP_OP_SRC_OVER if rng.random() < 0.8 else COMP_OP_SRC_COPY)
+            sc.set_global_alpha(1.0 if rng.random() < 0.6 else float(rng.uniform(0.2, 1.0)))
+            sc.set_fill_alpha(1.0 if rng.random() < 0.7 else float(rng.uniform(0.2, 1.0)))
+        k = int(rng.integers(0, 4))
+        if k == 0:
+            style = ("rgba32", _rand_rgba(rng))
+        elif k == 1:
+            style = ("rgba64", int(rng.integers(0, 1 << 63, dtype=np.uint64)) * 2 + int(rng.integers(0, 2)))
+        elif k == 2:
+            gt = int(rng.integers(0, 3))
+            x0, y0 = float(rng.uniform(0, W)), float(rng.uniform(0, H))
+            if gt == GRADIENT_LINEAR:
+                vals = [x0, y0, x0 + float(rng.uniform(-90, 90)), y0 + float(rng.uniform(-90, 90))]
+            elif gt == GRADIENT_RADIAL:
+                r = float(rng.uniform(5, 90))
+                vals = [x0, y0, x0 + float(rng.uniform(-r, r)) * 0.5, y0, r, 0.0]
+            else:
+                vals = [x0, y0, float(rng.uniform(0, 6.28)), 1.0]
+            style = ("gradient", gt, vals, _rand_stops(rng, 2, 5), int(rng.integers(0, 3)), None)
+        else:
+            a = float(rng.uniform(0.2, 6.0))
+            style = ("pattern", img, int(rng.integers(0, 3)), [math.cos(a), math.sin(a), -math.sin(a), math.cos(a), 11.0, 7.0])
+        target = int(rng.integers(0, 5))
+        x0, y0 = float(rng.uniform(-0.1, 0.9) * W), float(rng.uniform(-0.1, 0.9) * H)
+        w, h = float(rng.uniform(4, W * 0.6)), float(rng.uniform(4, H * 0.6))
+        if target == TARGET_ALL:
+            if rng.random() < 0.7:
+                target = TARGET_RECT_D
+            geom = None if target == TARGET_ALL else (x0, y0, w, h)
+        elif target == TARGET_RECT_I:
+            geom = (int(x0), int(y0), int(w), int(h))
+        elif target == TARGET_RECT_D:
+            geom = (x0, y0, w, h)
+        elif target == TARGET_PATH:
+            geom = ellipse_path(x0 + w / 2, y0 + h / 2, w / 2, h / 2)
+        else:
+            n = int(rng.integers(3, 8))
+            geom = (GEOMETRY_POLYGOND, [tuple(v) for v in (rng.random((n, 2)) * np.array([w, h]) + np.array([x0, y0]))])
+        sc.fill_explicit(target, style, geom)
+        if rng.random() < 0.3:
+            # the context's own style is untouched by the explicit calls
+            n = int(rng.integers(3, 7))
+            sc.fill_polygon(rng.random((n, 2)) * np.array([W, H]))
     return sc
 
 

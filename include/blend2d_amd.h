@@ -95,6 +95,10 @@ BLAmdResult bl_amd_context_destroy(BLAmdContext* ctx);
 BLAmdResult bl_amd_context_begin(BLAmdContext* ctx, BLAmdImage* target);
 /* bl_context_end (context.h:647): flush synchronously and detach. */
 BLAmdResult bl_amd_context_end(BLAmdContext* ctx);
+/* bl_context_reset (context.h:637): renders the pending calls and detaches
+ * the target when one is attached (the reference maps end() to reset()),
+ * then restores the default state. */
+BLAmdResult bl_amd_context_reset(BLAmdContext* ctx);
 /* bl_context_flush (context.h:649); flags: BL_CONTEXT_FLUSH_SYNC = 0x80000000 (context.h:107). */
 BLAmdResult bl_amd_context_flush(BLAmdContext* ctx, uint32_t flags);
 
@@ -174,6 +178,45 @@ BLAmdResult bl_amd_context_fill_rect_d_rgba32(BLAmdContext* ctx, double x, doubl
 /* bl_context_fill_path_d_rgba32 (context.h:751). */
 BLAmdResult bl_amd_context_fill_path_d_rgba32(BLAmdContext* ctx, const double* origin, const uint8_t* cmds,
                                               const double* vertices, size_t n, uint32_t rgba32);
+
+/* _rgba64 forms (context.h:737, 742, 747, 752, 757): the colour (16 bits
+ * per channel, straight alpha) replaces the fill style for this call; its
+ * high bytes make the rgba32 colour, as the reference's conversion does. */
+BLAmdResult bl_amd_context_fill_all_rgba64(BLAmdContext* ctx, uint64_t rgba64);
+BLAmdResult bl_amd_context_fill_rect_i_rgba64(BLAmdContext* ctx, int x, int y, int w, int h, uint64_t rgba64);
+BLAmdResult bl_amd_context_fill_rect_d_rgba64(BLAmdContext* ctx, double x, double y, double w, double h, uint64_t rgba64);
+BLAmdResult bl_amd_context_fill_path_d_rgba64(BLAmdContext* ctx, const double* origin, const uint8_t* cmds,
+                                              const double* vertices, size_t n, uint64_t rgba64);
+BLAmdResult bl_amd_context_fill_geometry_rgba64(BLAmdContext* ctx, uint32_t type, const void* data, uint64_t rgba64);
+
+/* A style for one call (the BLVar that the _ext entry points of context.h:
+ * 738-758 take): a colour, or a gradient / pattern described as
+ * bl_amd_context_set_fill_style_gradient / _pattern receive them. */
+enum { BL_AMD_STYLE_RGBA32 = 0, BL_AMD_STYLE_RGBA64 = 1, BL_AMD_STYLE_GRADIENT = 2, BL_AMD_STYLE_PATTERN = 3 };
+typedef struct BLAmdStyle {
+  uint32_t kind;                 /* BL_AMD_STYLE_* */
+  uint32_t rgba32;               /* RGBA32 */
+  uint64_t rgba64;               /* RGBA64 */
+  uint32_t gradient_type;        /* GRADIENT: 0 linear, 1 radial, 2 conic */
+  uint32_t extend;               /* GRADIENT, PATTERN: 0 pad, 1 repeat, 2 reflect */
+  const double* values;          /* GRADIENT: gradient values */
+  const double* stop_offsets;    /* GRADIENT: n_stops offsets */
+  const uint32_t* stop_rgba32;   /* GRADIENT: n_stops straight-alpha colours */
+  size_t n_stops;
+  const double* transform;       /* GRADIENT, PATTERN: own transform [6] or NULL */
+  const BLAmdImage* image;       /* PATTERN */
+} BLAmdStyle;
+
+/* _ext forms (context.h:738, 743, 748, 753, 758): `style` replaces the fill
+ * style for this call only and is set up with the user transform of the
+ * moment (resolve_explicit_style_op, rastercontext.cpp:1277-1328); the
+ * context's own style is not changed. */
+BLAmdResult bl_amd_context_fill_all_ext(BLAmdContext* ctx, const BLAmdStyle* style);
+BLAmdResult bl_amd_context_fill_rect_i_ext(BLAmdContext* ctx, int x, int y, int w, int h, const BLAmdStyle* style);
+BLAmdResult bl_amd_context_fill_rect_d_ext(BLAmdContext* ctx, double x, double y, double w, double h, const BLAmdStyle* style);
+BLAmdResult bl_amd_context_fill_path_d_ext(BLAmdContext* ctx, const double* origin, const uint8_t* cmds,
+                                           const double* vertices, size_t n, const BLAmdStyle* style);
+BLAmdResult bl_amd_context_fill_geometry_ext(BLAmdContext* ctx, uint32_t type, const void* data, const BLAmdStyle* style);
 
 /* bl_context_fill_geometry (context.h:755) / _rgba32 (context.h:756).
  * `data` points to the reference's geometry struct for `type` (same memory

@@ -47,7 +47,8 @@ typedef struct {
   double global_alpha;
   double fill_alpha;
   Style style;
-  uint32_t mode;          /* 0 fill with the style, 1 explicit colour, 2 clear */
+  Style explicit_style;   /* mode 3: the style of an explicit-style call */
+  uint32_t mode;          /* 0 fill with the style, 1 explicit colour, 2 clear, 3 explicit style */
   uint32_t explicit_rgba32;
   uint32_t call_all;      /* fill_all / clear_all: an invalid transform is no NOP */
   SceneImage img_info[16];
@@ -94,6 +95,36 @@ static void add_stop(Style* s, double offset, uint32_t rgba32) {
   s->n_stops = n + 1;
 }
 
+/* set_style_impl for a gradient / pattern (rastercontext.cpp:653-695): the
+ * style captures the user transform (and pattern quality) of the moment. */
+static size_t parse_gradient_style(Style* st, const uint8_t* pl, const OrcMat* user) {
+  memset(st, 0, sizeof(*st));
+  st->type = 1;
+  st->xform = *user;
+  memcpy(&st->g, pl, sizeof(SceneGradient));
+  for (uint32_t k = 0; k < st->g.n_stops; k++) {
+    SceneStop s;
+    memcpy(&s, pl + sizeof(SceneGradient) + k * sizeof(SceneStop), sizeof(s));
+    if (!(s.offset >= 0.0 && s.offset <= 1.0)) continue;
+    add_stop(st, s.offset, s.rgba32);
+  }
+  return sizeof(SceneGradient) + st->g.n_stops * sizeof(SceneStop);
+}
+
+static size_t parse_pattern_style(Style* st, const uint8_t* pl, const OrcMat* user, uint32_t quality) {
+  memset(st, 0, sizeof(*st));
+  st->type = 2;
+  st->xform = *user;
+  st->quality = quality;
+  memcpy(&st->pat, pl, sizeof(ScenePattern));
+  return sizeof(ScenePattern);
+}
+
+/* rgba32FromRgba64 (core/rgba_p.h:67-74): the high byte of each channel */
+static uint32_t rgba32_from_rgba64(uint64_t v) {
+  return (uint32_t)(((v >> 56) & 0xFFu) << 24 | ((v >> 40) & 0xFFu) << 16 | ((v >> 24) & 0xFFu) << 8 | ((v >> 8) & 0xFFu));
+}
+
 /* Resolves style + comp op into a pipeline (fetch + op).  Returns 0 when the
  * call renders, 1 when it is a NOP, or an error code. */
 static uint32_t resolve(Ctx* c, OrcFetch* f, uint32_t* lut_buf, OrcPipe* pipe) {
@@ -131,6 +162,9 @@ static uint32_t resolve(Ctx* c, OrcFetch* f, uint32_t* lut_buf, OrcPipe* pipe) {
     tmp.type = 0;
     tmp.rgba32 = c->explicit_rgba32;
     st = &tmp;
+  }
+  else if (c->mode == 3) {
+    st = &c->explicit_style;
   }
   OrcMat ft = st->xform;  /* final_transform (user x identity meta) when the style was set */
   if (st->type == 3) return 1;
@@ -542,25 +576,49 @@ uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
         c.style.type = 0;
         memcpy(&c.style.rgba32, pl, 4);
         break;
-      case SCENE_OP_SET_STYLE_GRADIENT: {
-        c.style.type = 1;
-        c.style.xform = c.user;
-        memcpy(&c.style.g, pl, sizeof(SceneGradient));
-        c.style.n_stops = 0;
-        for (uint32_t k = 0; k < c.style.g.n_stops; k++) {
-          SceneStop s;
-          memcpy(&s, pl + sizeof(SceneGradient) + k * sizeof(SceneStop), sizeof(s));
-          if (!(s.offset >= 0.0 && s.offset <= 1.0)) continue;
-          add_stop(&c.style, s.offset, s.rgba32);
+      case SCENE_OP_SET_STYLE_GRADIENT:
+        parse_gradient_style(&c.style, pl, &c.user);
+        break;
+      case SCENE_OP_SET_STYLE_PATTERN:
+        parse_pattern_style(&c.style, pl, &c.user, c.pattern_quality);
+        break;
+      case SCENE_OP_FILL_EXPLICIT: {
+        /* fill_*_rgba64 / fill_*_ext: resolve_explicit_style_op
+         * (rastercontext.cpp:1277-1328) -- the style of this call only, with
+         * the user transform of the moment */
+        SceneExplicit ex;
+        memcpy(&ex, pl, sizeof(ex));
+        const uint8_t* p = pl + sizeof(ex);
+        if (ex.style_kind == 0 || ex.style_kind == 1) {
+          c.mode = 1;
+          c.explicit_rgba32 = ex.style_kind == 0 ? (uint32_t)ex.rgba : rgba32_from_rgba64(ex.rgba);
         }
+        else {
+          c.mode = 3;
+          p += ex.style_kind == 2 ? parse_gradient_style(&c.explicit_style, p, &c.user)
+                                  : parse_pattern_style(&c.explicit_style, p, &c.user, c.pattern_quality);
+        }
+        switch (ex.target) {
+          case 0:
+            c.call_all = 1;
+            fill_box_i(&c, 0, 0, (int)c.canvas.w, (int)c.canvas.h);
+            c.call_all = 0;
+            break;
+          case 1: { int32_t v[4]; memcpy(v, p, 16); fill_rect_i(&c, v[0], v[1], v[2], v[3]); break; }
+          case 2: { double d[4]; memcpy(d, p, 32); fill_box_d(&c, d[0], d[1], d[0] + d[2], d[1] + d[3]); break; }
+          case 3: {
+            OrcPt* vtx = (OrcPt*)malloc((size_t)ex.count * sizeof(OrcPt) + 1);
+            memcpy(vtx, p + pad8(ex.count), (size_t)ex.count * sizeof(OrcPt));
+            if (ex.count) fill_path(&c, p, vtx, ex.count);
+            free(vtx);
+            break;
+          }
+          case 4: fill_geometry(&c, ex.geom_type, p, ex.count); break;
+          default: break;
+        }
+        c.mode = 0;
         break;
       }
-      case SCENE_OP_SET_STYLE_PATTERN:
-        c.style.type = 2;
-        c.style.xform = c.user;
-        c.style.quality = c.pattern_quality;
-        memcpy(&c.style.pat, pl, sizeof(ScenePattern));
-        break;
       case SCENE_OP_FILL_PATH: {
         uint32_t n;
         memcpy(&n, pl, 4);
@@ -605,11 +663,10 @@ uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
         break;
       }
       case SCENE_OP_SET_STYLE_RGBA64: {
-        /* rgba32FromRgba64 (core/rgba_p.h:67-74): the high byte of each channel */
         uint64_t v;
         memcpy(&v, pl, 8);
         c.style.type = 0;
-        c.style.rgba32 = (uint32_t)(((v >> 56) & 0xFFu) << 24 | ((v >> 40) & 0xFFu) << 16 | ((v >> 24) & 0xFFu) << 8 | ((v >> 8) & 0xFFu));
+        c.style.rgba32 = rgba32_from_rgba64(v);
         break;
       }
       case SCENE_OP_SET_STYLE_RGBA: {

@@ -50,7 +50,41 @@ struct Call {
   uint32_t flags = 0;
   uint32_t gtype = 0;
   std::vector<uint8_t> blob;   // geometry data
+  // FILL_EXPLICIT
+  SceneExplicit ex{};
+  BLRectI rect_i;
+  BLRect rect_d;
 };
+
+// Builds the style object of a FILL_EXPLICIT payload (kinds 2, 3) into `call`;
+// returns the bytes it used or 0 when malformed.
+size_t decode_style(Call& call, const uint8_t* p, const std::vector<BLImage>& images) {
+  if (call.ex.style_kind == 2) {
+    SceneGradient g;
+    std::memcpy(&g, p, sizeof(g));
+    call.gradient = (
+      g.type == 0 ? BLGradient(BLLinearGradientValues(g.values[0], g.values[1], g.values[2], g.values[3])) :
+      g.type == 1 ? BLGradient(BLRadialGradientValues(g.values[0], g.values[1], g.values[2], g.values[3], g.values[4], g.values[5])) :
+                    BLGradient(BLConicGradientValues(g.values[0], g.values[1], g.values[2], g.values[3])));
+    call.gradient.set_extend_mode(BLExtendMode(g.extend));
+    for (uint32_t k = 0; k < g.n_stops; k++) {
+      SceneStop st;
+      std::memcpy(&st, p + sizeof(g) + k * sizeof(SceneStop), sizeof(st));
+      call.gradient.add_stop(st.offset, BLRgba32(st.rgba32));
+    }
+    if (g.has_transform) call.gradient.set_transform(matrix_from(g.transform));
+    return sizeof(g) + g.n_stops * sizeof(SceneStop);
+  }
+  if (call.ex.style_kind == 3) {
+    ScenePattern pt;
+    std::memcpy(&pt, p, sizeof(pt));
+    if (pt.image_index >= images.size()) return 0;
+    call.pattern = BLPattern(images[pt.image_index], BLExtendMode(pt.extend));
+    if (pt.has_transform) call.pattern.set_transform(matrix_from(pt.transform));
+    return sizeof(pt);
+  }
+  return 0;
+}
 
 }  // namespace
 
@@ -156,6 +190,31 @@ uint32_t ref_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
         call.path.modify_op(BL_MODIFY_OP_ASSIGN_FIT, h.n, &cmd_out, &vtx_out);
         std::memcpy(cmd_out, cmds, h.n);
         std::memcpy(vtx_out, vtx, size_t(h.n) * sizeof(BLPoint));
+        break;
+      }
+      case SCENE_OP_FILL_EXPLICIT: {
+        std::memcpy(&call.ex, payload, sizeof(call.ex));
+        const uint8_t* p = payload + sizeof(call.ex);
+        if (call.ex.style_kind >= 2) {
+          const size_t used = decode_style(call, p, images);
+          if (!used) return 0x7FFFFFFFu;
+          p += used;
+        }
+        switch (call.ex.target) {
+          case 1: { int32_t v[4]; std::memcpy(v, p, 16); call.rect_i = BLRectI(v[0], v[1], v[2], v[3]); break; }
+          case 2: { double v[4]; std::memcpy(v, p, 32); call.rect_d = BLRect(v[0], v[1], v[2], v[3]); break; }
+          case 3: {
+            const uint32_t n = call.ex.count;
+            uint8_t* cmd_out;
+            BLPoint* vtx_out;
+            call.path.modify_op(BL_MODIFY_OP_ASSIGN_FIT, n, &cmd_out, &vtx_out);
+            std::memcpy(cmd_out, p, n);
+            std::memcpy(vtx_out, p + pad8(n), size_t(n) * sizeof(BLPoint));
+            break;
+          }
+          case 4: call.blob.assign(p, payload + c.size); break;
+          default: break;
+        }
         break;
       }
       case SCENE_OP_FILL_RECT_D:
@@ -274,6 +333,51 @@ uint32_t ref_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
         if (!(call.flags & 1u)) origin.reset(0.0, 0.0);
         if (call.flags & 2u) note(ctx.fill_path(origin, call.path, BLRgba32(call.rgba32)));
         else note(ctx.fill_path(origin, call.path));
+        break;
+      }
+      case SCENE_OP_FILL_EXPLICIT: {
+        // the C++ API's style overloads route BLRgba32 / BLRgba64 to the
+        // _rgba32 / _rgba64 entry points and BLGradient / BLPattern to _ext
+        BLVar style;
+        if (call.ex.style_kind == 0) style = BLRgba32(uint32_t(call.ex.rgba));
+        else if (call.ex.style_kind == 1) style = BLRgba64(call.ex.rgba);
+        else if (call.ex.style_kind == 2) style = call.gradient;
+        else style = call.pattern;
+        BLContextCore* core = &ctx;
+        const BLUnknown* st = &style;
+        switch (call.ex.target) {
+          case 0:
+            if (call.ex.style_kind == 1) note(bl_context_fill_all_rgba64(core, call.ex.rgba));
+            else note(bl_context_fill_all_ext(core, st));
+            break;
+          case 1:
+            if (call.ex.style_kind == 1) note(bl_context_fill_rect_i_rgba64(core, &call.rect_i, call.ex.rgba));
+            else note(bl_context_fill_rect_i_ext(core, &call.rect_i, st));
+            break;
+          case 2:
+            if (call.ex.style_kind == 1) note(bl_context_fill_rect_d_rgba64(core, &call.rect_d, call.ex.rgba));
+            else note(bl_context_fill_rect_d_ext(core, &call.rect_d, st));
+            break;
+          case 3: {
+            BLPoint origin(0.0, 0.0);
+            if (call.ex.style_kind == 1) note(bl_context_fill_path_d_rgba64(core, &origin, &call.path, call.ex.rgba));
+            else note(bl_context_fill_path_d_ext(core, &origin, &call.path, st));
+            break;
+          }
+          case 4: {
+            const void* data = call.blob.data();
+            BLArrayView<uint8_t> view;
+            if (call.ex.geom_type >= BL_GEOMETRY_TYPE_POLYLINEI && call.ex.geom_type <= BL_GEOMETRY_TYPE_ARRAY_VIEW_RECTD) {
+              view.data = call.blob.data();
+              view.size = call.ex.count;
+              data = &view;
+            }
+            if (call.ex.style_kind == 1) note(bl_context_fill_geometry_rgba64(core, BLGeometryType(call.ex.geom_type), data, call.ex.rgba));
+            else note(bl_context_fill_geometry_ext(core, BLGeometryType(call.ex.geom_type), data, st));
+            break;
+          }
+          default: break;
+        }
         break;
       }
       case SCENE_OP_FILL_GEOMETRY: {

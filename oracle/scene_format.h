@@ -69,8 +69,26 @@ enum SceneOp {
   SCENE_OP_FILL_ALL_RGBA32    = 21, /* u32 rgba32, u32 pad */
   SCENE_OP_FILL_RECT_I_RGBA32 = 22, /* i32 x, y, w, h, u32 rgba32, u32 pad */
   SCENE_OP_FILL_RECT_D_RGBA32 = 23, /* f64 x, y, w, h, u32 rgba32, u32 pad */
-  SCENE_OP_FILL_PATH_EX       = 24  /* ScenePathEx, u8 cmds[n] padded to 8, f64 vtx[2n] */
+  SCENE_OP_FILL_PATH_EX       = 24, /* ScenePathEx, u8 cmds[n] padded to 8, f64 vtx[2n] */
+  SCENE_OP_FILL_EXPLICIT      = 25  /* SceneExplicit, style payload, target payload (see below) */
 };
+
+/* FILL_EXPLICIT: a fill with an explicit style for this call only -- the
+ * fill_*_rgba64 (style kind 1) and fill_*_ext (kinds 0-3) entry points of
+ * core/context.h:735-758.  After the header comes the style payload (kind 2:
+ * SceneGradient + n_stops x SceneStop; kind 3: ScenePattern; kinds 0 and 1:
+ * nothing, the colour is in `rgba`), then the target payload: target 0
+ * (fill_all) nothing, 1 (fill_rect_i) i32 x, y, w, h, 2 (fill_rect_d)
+ * f64 x, y, w, h, 3 (fill_path_d) u8 cmds[count] padded to 8 and
+ * f64 vtx[2 count], 4 (fill_geometry) the geometry data as FILL_GEOMETRY
+ * (struct, or `count` array elements). */
+typedef struct SceneExplicit {
+  uint32_t target;        /* 0 all, 1 rect_i, 2 rect_d, 3 path, 4 geometry */
+  uint32_t style_kind;    /* 0 rgba32, 1 rgba64, 2 gradient, 3 pattern */
+  uint32_t geom_type;     /* target 4: BLGeometryType */
+  uint32_t count;         /* target 3: vertices; target 4: array elements */
+  uint64_t rgba;          /* kind 0: rgba32, kind 1: rgba64 (straight alpha) */
+} SceneExplicit;
 
 /* FILL_GEOMETRY: BLGeometryType `type`; for array types (POLYGON*, POLYLINE*,
  * ARRAY_VIEW_*) `count` elements follow (BLPointI / BLPoint / BLBoxI / BLBox /

@@ -24,6 +24,9 @@ def cases():
     for fmt, fname in ((S.FMT_PRGB32, "prgb32"), (S.FMT_XRGB32, "xrgb32"), (S.FMT_A8, "a8")):
         for seed in (1, 2):
             out[f"comp_op_folds_{fname}_s{seed}"] = lambda seed=seed, fmt=fmt: [S.comp_op_folds_scene(seed, fmt=fmt)]
+    for fmt, fname in ((S.FMT_PRGB32, "prgb32"), (S.FMT_XRGB32, "xrgb32"), (S.FMT_A8, "a8")):
+        for seed in (1, 2):
+            out[f"explicit_styles_{fname}_s{seed}"] = lambda seed=seed, fmt=fmt: [S.explicit_styles_scene(seed, fmt=fmt)]
     for fmt, fname in ((S.FMT_PRGB32, "prgb32"), (S.FMT_A8, "a8")):
         out[f"clip_robustness_{fname}"] = lambda fmt=fmt: S.clip_robustness_scenes(fmt=fmt)
     for fmt, fname in ((S.FMT_PRGB32, "prgb32"), (S.FMT_XRGB32, "xrgb32"), (S.FMT_A8, "a8")):

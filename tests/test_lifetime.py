@@ -107,3 +107,30 @@ def test_batch_rejects_a_context_twice_and_a_shared_target(amd):
     assert (img.download() == want).all()
     for ctx in (a, b, c):
         ctx.end()
+
+
+def test_reset_renders_pending_calls_and_detaches(amd):
+    img = amd.BLImage(40, 30, S.FMT_PRGB32)
+    img.fill_raw(0xFF000000)
+    ctx = amd.BLContext(img)
+    ctx.fill_rect(5, 5, 20, 10, 0xFF00FF00)
+    ctx.reset()
+    sc = S.Scene(40, 30, S.FMT_PRGB32, 0xFF000000)
+    sc.fill_rect_i_rgba32(5, 5, 20, 10, 0xFF00FF00)
+    want, _ = checkers.render_oracle(sc)
+    assert (img.download() == want).all()
+    # detached: rendering calls fail until the next begin()
+    with pytest.raises(amd.BLError) as e:
+        ctx.fill_all()
+    assert e.value.code == 0x10002
+    ctx.begin(img)
+    ctx.end()
+
+
+def test_explicit_style_rejects_an_unknown_kind(amd):
+    img = amd.BLImage(16, 16, S.FMT_PRGB32)
+    ctx = amd.BLContext(img)
+    st = amd.BLAmdStyle()
+    st.kind = 9
+    assert amd.lib().bl_amd_context_fill_all_ext(ctx.handle, st) == E_INVALID_VALUE
+    ctx.end()
