@@ -139,7 +139,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing; build it with `python -m blend2d_amd.build`")
-    L = ctypes.CDLL(LIB_PATH)
+    _lib = bind(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def bind(L):
+    """Sets the argument and result types of every C-ABI entry point on the
+    loaded library `L` (include/blend2d_amd.h); returns `L`."""
     P, D, U32, SZ = ctypes.c_void_p, ctypes.c_double, ctypes.c_uint32, ctypes.c_size_t
     I, IP = ctypes.c_int, ctypes.c_ssize_t
     sig = {
@@ -215,7 +221,6 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = res
-    _lib = L
     return L
 
 

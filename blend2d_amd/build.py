@@ -4,7 +4,8 @@
 
 The library is the product: HIP kernels (csrc/kernels.hip) and the host
 frontend (csrc/context.cpp, csrc/style.cpp) behind the C ABI declared in
-include/blend2d_amd.h.
+include/blend2d_amd.h.  libb2amd_player.so (csrc/scene_player.cpp) is a
+native client of it that plays recorded scenes through that C ABI.
 """
 from __future__ import annotations
 
@@ -17,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libblend2d_amd.so")
+PLAYER = os.path.join(HERE, "libb2amd_player.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 
 ARCH = os.environ.get("BLEND2D_AMD_ARCH", "gfx950")
@@ -59,6 +61,12 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
+    # the scene player: a native client of the library (no device code)
+    tmp = PLAYER + ".tmp"
+    _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-Wall", "-shared",
+          os.path.join(CSRC, "scene_player.cpp"), "-o", tmp, "-L" + HERE, "-lblend2d_amd",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(tmp, PLAYER)
     return LIB
 
 

@@ -82,14 +82,17 @@ struct FrameHost {
   std::vector<SrcSeg> src;
   std::vector<double> vtx;
   std::vector<DevStyle> styles;
-  std::vector<uint32_t> luts;
+  // gradient tables, filled on the device from their stops at upload
+  std::vector<lut::Job> lut_jobs;
+  std::vector<Stop> lut_stops;
+  uint64_t lut_entries = 0;
   std::vector<DevImage> images;
   // Host only: per command, the rendering call (counted from begin()) that
   // recorded it, so errors found on the device name their call.
   std::vector<uint64_t> cmd_call;
   void clear() {
     canvases.clear(); cmds.clear(); xforms.clear(); src.clear(); vtx.clear();
-    styles.clear(); luts.clear(); images.clear(); cmd_call.clear();
+    styles.clear(); lut_jobs.clear(); lut_stops.clear(); lut_entries = 0; images.clear(); cmd_call.clear();
   }
 };
 
@@ -174,6 +177,7 @@ struct Runner {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf in;
+  DevBuf lut_buf;   // gradient tables of the resident frame
   DevBuf src_count, src_off, bounds, segs, seg_cmd, mat_cnt, mat_off, mat_pres, list_off, list, binned, cursor, scan_tmp, counters, cmd_tiles;
   DevBuf tile_hdr, group_hdr, tile_px, list_style, band_buf, item_cnt, item_off, items;
   DevBuf cm_params, cm_acc, cm_feat, cm_pred, cm_act;
@@ -251,14 +255,14 @@ struct Runner {
 
 // Section sizes of a packed frame: per part (context) and in total.
 struct PartBase {
-  size_t cmds, xforms, src, vtx, styles, luts, images;
+  size_t cmds, xforms, src, vtx, styles, luts, images, jobs, stops;   // luts: table entries
 };
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 // Copies part `k` of a frame into the packed staging layout, rebasing its
 // indices (canvas, command, vertex, style, LUT and image bases).
-static void pack_part(uint8_t* st, const FrameHost& fh, uint32_t k, const PartBase& b, const size_t off[8]) {
+static void pack_part(uint8_t* st, const FrameHost& fh, uint32_t k, const PartBase& b, const size_t off[9]) {
   DevCmd* cmds = (DevCmd*)(st + off[1]) + b.cmds;
   for (size_t i = 0; i < fh.cmds.size(); i++) {
     DevCmd c = fh.cmds[i];
@@ -284,7 +288,14 @@ static void pack_part(uint8_t* st, const FrameHost& fh, uint32_t k, const PartBa
     d.img += (uint32_t)b.images;
     styles[i] = d;
   }
-  if (!fh.luts.empty()) memcpy((uint32_t*)(st + off[6]) + b.luts, fh.luts.data(), fh.luts.size() * 4);
+  lut::Job* jobs = (lut::Job*)(st + off[6]) + b.jobs;
+  for (size_t i = 0; i < fh.lut_jobs.size(); i++) {
+    lut::Job j = fh.lut_jobs[i];
+    j.lut += (uint32_t)b.luts;
+    j.stop0 += (uint32_t)b.stops;
+    jobs[i] = j;
+  }
+  if (!fh.lut_stops.empty()) memcpy((Stop*)(st + off[8]) + b.stops, fh.lut_stops.data(), fh.lut_stops.size() * sizeof(Stop));
   if (!fh.images.empty()) memcpy((DevImage*)(st + off[7]) + b.images, fh.images.data(), fh.images.size() * sizeof(DevImage));
 }
 
@@ -302,7 +313,7 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   std::vector<DevCanvas> canvases;
   canvases.reserve(parts.size());
   std::vector<PartBase> bases(parts.size());
-  PartBase tot{0, 0, 0, 0, 0, 0, 0};
+  PartBase tot{0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint64_t n_src_total = 0;
   uint64_t n_items_max = 0;
   for (size_t k = 0; k < parts.size(); k++) {
@@ -316,7 +327,9 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
     tot.src += fh.src.size();
     tot.vtx += fh.vtx.size();
     tot.styles += fh.styles.size();
-    tot.luts += fh.luts.size();
+    tot.luts += fh.lut_entries;
+    tot.jobs += fh.lut_jobs.size();
+    tot.stops += fh.lut_stops.size();
     tot.images += fh.images.size();
     n_src_total += fh.src.size();
     for (const SrcSeg& sg : fh.src) n_items_max += sg.kind == kSrcLine ? 1u : sg.kind == kSrcQuad ? 3u : 8u;
@@ -339,16 +352,16 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
     n_mat += (uint64_t)c.bands * c.n_cmds;
   }
   if (n_mat >= 0xFFFFFFF0ull || n_src_total >= 0xFFFFFFF0ull || n_items_max >= 0xFFFFFFF0ull || tot.vtx / 2 >= 0xFFFFFFF0ull ||
-      tot.styles >= 0xFFFFFFF0ull || tot.luts >= 0xFFFFFFF0ull)
+      tot.styles >= 0xFFFFFFF0ull || tot.luts >= 0xFFFFFFF0ull || tot.stops >= 0xFFFFFFF0ull)
     return kErrOOM;
 
   // section offsets
-  size_t off[8];
+  size_t off[9];
   size_t at = 0;
-  const size_t sizes[8] = {canvases.size() * sizeof(DevCanvas), tot.cmds * sizeof(DevCmd), tot.xforms * sizeof(CmdXform),
+  const size_t sizes[9] = {canvases.size() * sizeof(DevCanvas), tot.cmds * sizeof(DevCmd), tot.xforms * sizeof(CmdXform),
                            tot.src * sizeof(SrcSeg), tot.vtx * sizeof(double), tot.styles * sizeof(DevStyle),
-                           tot.luts * 4, tot.images * sizeof(DevImage)};
-  for (int i = 0; i < 8; i++) { off[i] = at; at = align256(at + sizes[i]); }
+                           tot.jobs * sizeof(lut::Job), tot.images * sizeof(DevImage), tot.stops * sizeof(Stop)};
+  for (int i = 0; i < 9; i++) { off[i] = at; at = align256(at + sizes[i]); }
   const size_t blob_size = at;
 
   DeviceQueue& q = device_queue();
@@ -372,7 +385,12 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   }
 
   if (!in.reserve(blob_size + 256)) return kErrOOM;
+  if (!lut_buf.reserve(tot.luts * 4 + 256)) return kErrOOM;
   HIP_OK(hipMemcpyAsync(in.ptr, st, blob_size, hipMemcpyHostToDevice, stream));
+  // gradient tables from their stops, on the device
+  if (tot.jobs)
+    launch_gradient_luts((const lut::Job*)(in.as<uint8_t>() + off[6]), (const Stop*)(in.as<uint8_t>() + off[8]),
+                         (uint32_t)tot.jobs, lut_buf.as<uint32_t>(), stream);
   // the staging memory is reused by the next upload
   HIP_OK(hipStreamSynchronize(stream));
 
@@ -384,7 +402,7 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   f.src = (const SrcSeg*)(base + off[3]); f.n_src = (uint32_t)tot.src;
   f.vtx = (const double*)(base + off[4]);
   f.styles = (const DevStyle*)(base + off[5]);
-  f.luts = (const uint32_t*)(base + off[6]);
+  f.luts = lut_buf.as<uint32_t>();
   f.images = (const DevImage*)(base + off[7]);
   f.n_tiles = n_tiles;
   f.n_groups = n_groups;
@@ -395,8 +413,8 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   item_cap = (uint32_t)n_items_max;
   bands_max = 0;
   for (const DevCanvas& c : canvases) bands_max = std::max(bands_max, c.bands);
-  frame_bytes = blob_size;
   lut_bytes = tot.luts * 4;
+  frame_bytes = blob_size + lut_bytes;
   image_bytes = 0;
   for (const FrameHost* p : parts)
     for (const DevImage& im : p->images) image_bytes += (uint64_t)im.w * im.h * (im.format == kFmtA8 ? 1u : 4u);
@@ -827,7 +845,9 @@ struct StyleState {
   uint32_t fmt = kFxFRGB32;     // FormatExt seen by comp-op simplification
   uint32_t status = 0;          // 0 renders, 1 NOP, otherwise an error code
   DevStyle dev{};
-  std::vector<uint32_t> lut;
+  // gradient: its stops and table size (the table is built on the device)
+  std::vector<Stop> stops;
+  uint32_t lut_size = 0;
   ImageRef image;
   uint64_t version = 0;
 };
@@ -971,8 +991,12 @@ Resolved resolve_with(BLAmdContext* c, StyleState& st, uint32_t alpha, bool nop_
   if (c->frame_style_version != st.version) {
     DevStyle d = st.dev;
     if (st.type == kStyleGradient && d.kind != kFetchSolid) {
-      d.lut = (uint32_t)c->frame.luts.size();
-      c->frame.luts.insert(c->frame.luts.end(), st.lut.begin(), st.lut.end());
+      FrameHost& fr = c->frame;
+      d.lut = (uint32_t)fr.lut_entries;
+      fr.lut_jobs.push_back(lut::Job{(uint32_t)fr.lut_entries, st.lut_size, (uint32_t)fr.lut_stops.size(),
+                                     (uint32_t)st.stops.size()});
+      fr.lut_stops.insert(fr.lut_stops.end(), st.stops.begin(), st.stops.end());
+      fr.lut_entries += st.lut_size;
     }
     if (st.type == kStylePattern) d.img = image_index(c, st.image.get());
     c->frame_style_index = (uint32_t)c->frame.styles.size();
@@ -1399,11 +1423,10 @@ BLAmdResult bl_amd_image_fill_raw(BLAmdImage* img, uint32_t value) {
     if (hipMemset(img->dptr, (int)(value & 0xFFu), (size_t)img->stride * img->h) != hipSuccess) return kErrDevice;
     return BL_AMD_SUCCESS;
   }
-  std::vector<uint32_t> row((size_t)img->stride / 4, value);
-  for (int y = 0; y < img->h; y++)
-    if (hipMemcpy((uint8_t*)img->dptr + (size_t)y * img->stride, row.data(), (size_t)img->stride, hipMemcpyHostToDevice) != hipSuccess)
-      return kErrDevice;
-  return BL_AMD_SUCCESS;
+  // the row padding (stride is a multiple of 4) takes the value too
+  if (hipMemsetD32((hipDeviceptr_t)img->dptr, (int)value, (size_t)img->stride / 4 * img->h) != hipSuccess)
+    return kErrDevice;
+  return hipDeviceSynchronize() == hipSuccess ? BL_AMD_SUCCESS : kErrDevice;
 }
 
 BLAmdResult bl_amd_image_get_data(const BLAmdImage* img, int* w, int* h, uint32_t* format, void** device_pixels, intptr_t* stride) {
@@ -1548,8 +1571,8 @@ uint32_t build_gradient_style(BLAmdContext* ctx, StyleState& st, uint32_t type, 
     Mat gm{transform[0], transform[1], transform[2], transform[3], transform[4], transform[5]};
     if (transform_type(gm) != kTTIdentity) t = multiply(gm, ctx->user);
   }
-  st.lut.resize(lut_size);
-  interpolate_lut(st.lut.data(), lut_size, stops);
+  st.lut_size = lut_size;
+  st.stops = std::move(stops);
   bool ok = type == 0 ? init_linear(st.dev, lut_size, values, extend, t)
           : type == 1 ? init_radial(st.dev, lut_size, values, extend, t)
                       : init_conic(st.dev, lut_size, values, t);

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "device_types.h"
+#include "lut_device.h"
 
 namespace b2amd {
 
@@ -107,6 +108,8 @@ void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint3
 void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+// Fills the gradient tables of `n_jobs` jobs (one workgroup per table).
+void launch_gradient_luts(const lut::Job* jobs, const lut::Stop* stops, uint32_t n_jobs, uint32_t* luts, hipStream_t st);
 
 // The device scratch region of bl_amd_context_replay_noalloc on device
 // `dev`: its address (looked up once per device) and size in bytes.

@@ -165,6 +165,22 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
 }
 
+// One gradient table per workgroup, every entry computed on its own
+// (lut_device.h).  The stops of a job were validated and sorted on the
+// host (offsets in [0, 1], n >= 1), so every position lies in [0, size).
+__global__ void __launch_bounds__(256) k_gradient_luts(const lut::Job* __restrict__ jobs,
+                                                       const lut::Stop* __restrict__ stops,
+                                                       uint32_t* __restrict__ luts) {
+  const lut::Job j = jobs[blockIdx.x];
+  const lut::Stop* st = stops + j.stop0;
+  for (uint32_t e = threadIdx.x; e < j.size; e += 256u) luts[(size_t)j.lut + e] = lut::entry(st, j.n_stops, j.size, e);
+}
+
+void launch_gradient_luts(const lut::Job* jobs, const lut::Stop* stops, uint32_t n_jobs, uint32_t* luts, hipStream_t st) {
+  if (!n_jobs) return;
+  hipLaunchKernelGGL(k_gradient_luts, dim3(n_jobs), dim3(256), 0, st, jobs, stops, luts);
+}
+
 __global__ void k_init_bounds(CmdBounds* b, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) { b[i].min_x = INT32_MAX; b[i].max_x = INT32_MIN; b[i].min_y = INT32_MAX; b[i].max_y = INT32_MIN; }

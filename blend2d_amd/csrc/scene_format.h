@@ -1,15 +1,15 @@
-/* Binary scene format shared by the reference driver (ref_render.cpp), the CPU
- * oracle (orc_*.c) and the Python scene writer (blend2d_amd/scene.py).
- *
- * A scene is a canvas description followed by a list of BLContext calls, so
- * the same byte string can be replayed through the reference library, through
- * the oracle and through the MI355X backend.  All fields are little endian.
+/* Binary scene format: a recorded list of BLContext calls.  Written by the
+ * Python scene writer (blend2d_amd/scene.py); played through the MI355X
+ * backend by scene_player.cpp and blend2d_amd/replay.py, through the
+ * reference library by oracle/ref_render.cpp and through the CPU oracle by
+ * oracle/orc_scene.c, so the same byte string drives all three.  All fields
+ * are little endian.
  *
  *   SceneHeader
  *   n_images x { SceneImage, pixel bytes (h * stride, padded to 8) }
  *   n_cmds   x { SceneCmd, payload (size bytes, multiple of 8) }
  *
- * This header is test infrastructure; the product library never includes it.
+ * The rendering library (libblend2d_amd.so) never includes this header.
  */
 #ifndef B2AMD_SCENE_FORMAT_H
 #define B2AMD_SCENE_FORMAT_H

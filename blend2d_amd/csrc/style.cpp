@@ -140,74 +140,6 @@ uint32_t gradient_info(const std::vector<Stop>& stops, bool& solid, uint32_t& fo
   return lut;
 }
 
-// Gradient LUT as the reference builds it on AVX2 hosts
-// (pixelops/interpolation_avx2.cpp:18-176, selected by pixelops/funcs.cpp:31-35):
-// channels are the high bytes of the 16-bit stop colours, a span of i >= 2
-// cells steps a 9.23 value by trunc(delta * (2^23 / i)) (double precision),
-// every entry is premultiplied with div255.
-static inline uint32_t premul4(const uint32_t c[4]) {
-  uint32_t a = c[3];
-  return (udiv255(255u * a) << 24) | (udiv255(c[2] * a) << 16) | (udiv255(c[1] * a) << 8) | udiv255(c[0] * a);
-}
-static inline void hi_bytes(uint64_t v, uint32_t o[4]) {
-  o[0] = (uint32_t)(v >> 8) & 0xFFu;
-  o[1] = (uint32_t)(v >> 24) & 0xFFu;
-  o[2] = (uint32_t)(v >> 40) & 0xFFu;
-  o[3] = (uint32_t)(v >> 56) & 0xFFu;
-}
-
-void interpolate_lut(uint32_t* d_ptr, uint32_t d_size, const std::vector<Stop>& st) {
-  const size_t s_size = st.size();
-  uint32_t* span = d_ptr;
-  uint32_t i;
-  uint64_t c0 = st[0].rgba64, c1;
-  uint32_t u0 = 0, u1;
-  size_t s_index = (size_t)(st[0].offset == 0.0 && s_size > 1);
-  double f_width = (double)((int32_t)(--d_size) << 8);
-  do {
-    c1 = st[s_index].rgba64;
-    u1 = (uint32_t)round_to_int(st[s_index].offset * f_width);
-    span = d_ptr + (u0 >> 8);
-    i = (u1 >> 8) - (u0 >> 8);
-    u0 = u1;
-    uint32_t a[4], b[4];
-    hi_bytes(c0, a);
-    hi_bytes(c1, b);
-    if (i <= 1) {
-      c0 = c1;
-      *span++ = premul4(a);
-      if (i == 0) continue;
-      *span++ = premul4(b);
-    }
-    else {
-      uint32_t n = i + 1;
-      double scale = (double)(1 << 23) / (double)(int)i;
-      int32_t cx[4], dx[4];
-      for (int k = 0; k < 4; k++) {
-        dx[k] = (int32_t)((double)((int32_t)b[k] - (int32_t)a[k]) * scale);
-        cx[k] = (int32_t)(a[k] << 23) + (1 << 22);
-      }
-      for (uint32_t j = 0; j < n; j++) {
-        uint32_t px[4];
-        for (int k = 0; k < 4; k++) {
-          uint32_t v = ((uint32_t)cx[k] + (uint32_t)dx[k] * j) >> 23;
-          px[k] = v > 255u ? 255u : v;
-        }
-        *span++ = premul4(px);
-      }
-      c0 = c1;
-    }
-  } while (++s_index < s_size);
-
-  i = (uint32_t)((size_t)((d_ptr + d_size + 1) - span));
-  uint32_t last[4], first[4];
-  hi_bytes(c0, last);
-  hi_bytes(st[0].rgba64, first);
-  uint32_t lp = premul4(last);
-  while (i--) *span++ = lp;
-  d_ptr[0] = premul4(first);
-}
-
 // pipeline/pipedefs.cpp:386-434
 bool init_linear(DevStyle& f, uint32_t lut_size, const double v[4], uint32_t extend, const Mat& t) {
   Mat inv;

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "device_types.h"
+#include "lut_device.h"
 
 namespace b2amd {
 
@@ -41,11 +42,10 @@ int64_t floor_to_int64(double x);
 
 uint32_t premultiply(uint32_t argb32);
 
-struct Stop { double offset; uint64_t rgba64; };
+using Stop = lut::Stop;
 // add_stop semantics of BLGradient (sorted; a third stop at one offset replaces the second).
 void add_stop(std::vector<Stop>& stops, double offset, uint32_t rgba32);
 uint32_t gradient_info(const std::vector<Stop>& stops, bool& solid, uint32_t& format);
-void interpolate_lut(uint32_t* dst, uint32_t size, const std::vector<Stop>& stops);
 
 bool init_linear(DevStyle& s, uint32_t lut_size, const double v[4], uint32_t extend, const Mat& t);
 bool init_radial(DevStyle& s, uint32_t lut_size, const double v[6], uint32_t extend, const Mat& t);

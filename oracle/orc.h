@@ -14,7 +14,7 @@
 extern "C" {
 #endif
 
-/* Replays a scene (oracle/scene_format.h) and writes the canvas rows, packed,
+/* Replays a scene (blend2d_amd/csrc/scene_format.h) and writes the canvas rows, packed,
  * to `out`.  Returns 0 or the first BLResult error a call produced. */
 uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_size);
 

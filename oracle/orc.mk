@@ -8,9 +8,9 @@
 
 OUT    := oracle/_build
 CC     ?= gcc
-CFLAGS := -std=c11 -O2 -fPIC -ffp-contract=off -fno-math-errno -Wall -Wno-unused-label
+CFLAGS := -std=c11 -O2 -fPIC -ffp-contract=off -fno-math-errno -Wall -Wno-unused-label -Iblend2d_amd/csrc
 SRCS   := oracle/orc_edges.c oracle/orc_raster.c oracle/orc_pipe.c oracle/orc_style.c oracle/orc_scene.c
-HDRS   := oracle/orc.h oracle/orc_core.h oracle/scene_format.h
+HDRS   := oracle/orc.h oracle/orc_core.h blend2d_amd/csrc/scene_format.h
 
 all: $(OUT)/liborc.so
 

@@ -43,8 +43,8 @@ $(OBJ)/%.o: $(REF)/%.cpp
 $(OUT)/libblend2d_ref.so: $(OBJS)
 	$(CXX) -shared -o $@ $(OBJS) -lpthread
 
-$(OUT)/libref_render.so: oracle/ref_render.cpp oracle/scene_format.h $(OUT)/libblend2d_ref.so
-	$(CXX) -std=c++17 -O2 -fPIC -shared -I$(REF) -Ioracle -o $@ oracle/ref_render.cpp \
+$(OUT)/libref_render.so: oracle/ref_render.cpp blend2d_amd/csrc/scene_format.h $(OUT)/libblend2d_ref.so
+	$(CXX) -std=c++17 -O2 -fPIC -shared -I$(REF) -Iblend2d_amd/csrc -o $@ oracle/ref_render.cpp \
 	  -L$(OUT) -lblend2d_ref -Wl,-rpath,'$$ORIGIN'
 
 .PHONY: all

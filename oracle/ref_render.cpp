@@ -1,6 +1,6 @@
 // Scene driver for the reference Blend2D library built by oracle/ref.mk.
 //
-// Test infrastructure only: it replays a scene (oracle/scene_format.h) through
+// Test infrastructure only: it replays a scene (blend2d_amd/csrc/scene_format.h) through
 // the reference's public C++ API (BLContext, BLPath, BLGradient, BLPattern)
 // so the tests can compare the CPU oracle and the MI355X backend against the
 // real reference, and bench.py can time the reference CPU pipeline.  It is
