@@ -259,7 +259,8 @@ def _side_workload(name, seed, steps, learned, threads):
         "paths_per_s": round(config["paths"] * steps / el, 1),
         "ms_per_frame": round(el * 1e3 / steps, 3),
         "frame_breakdown_ms": {"record": round(float(np.median(issue)), 3), "merge": round(st["merge_ms"], 3),
-                               "upload": round(st["upload_ms"], 3), "device_stages": round(st["total_ms"], 3),
+                               "upload": round(st["upload_ms"], 3), "of_which_pack": round(st["pack_ms"], 3),
+                               "device_stages": round(st["total_ms"], 3),
                                "render": round(st["render_ms"], 3)},
         "device_only_ms_per_frame": round(float(np.mean(rep_ms)), 3),
         "device_only_mpix_per_s": round(st["pixels"] / (float(np.mean(rep_ms)) * 1e-3) / 1e6, 3),
@@ -282,7 +283,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the reference parity check")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain form and the exact-sizing ablation")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain form")
     ap.add_argument("--no-side", action="store_true", help="skip the srcover4k and c2 side measurements")
     ap.add_argument("--record-threads", type=int, default=0,
                     help="client threads recording the canvases' calls (default: host cores, at most 16)")
@@ -375,7 +376,8 @@ def main():
         "pixels_per_frame": int(pixels),
         "frame_ms": {"min": round(min(per), 3), "median": round(float(np.median(per)), 3)},
         "frame_breakdown_ms": {"record": round(float(np.median(issue)), 3), "merge": round(st["merge_ms"], 3),
-                               "upload": round(st["upload_ms"], 3), "device_stages": round(st["total_ms"], 3),
+                               "upload": round(st["upload_ms"], 3), "of_which_pack": round(st["pack_ms"], 3),
+                               "device_stages": round(st["total_ms"], 3),
                                "render": round(st["render_ms"], 3)},
         "fallback": int(st["fallback"]),
         "syncs": int(st["syncs"]),
@@ -389,6 +391,27 @@ def main():
     if any(gpu_errors):
         line["errors"] = sorted({int(e) for e in gpu_errors if e})
 
+    # ablation: the same batched frame with exact sizing, and with every call
+    # recorded on one client thread
+    B.set_sizing(B.SIZING_EXACT)
+    frame.render(player.BATCHED, threads)
+    xel, xper, xsts, xissue = _steps(frame, player.BATCHED, args.steps, threads)
+    B.set_sizing(B.SIZING_LEARNED if learned else B.SIZING_EXACT)
+    oel, oper, osts, oissue = _steps(frame, player.BATCHED, args.steps, 1)
+    bvp = {
+        "batched_ms_per_frame": round(el * 1e3 / args.steps, 3),
+        "batched_mpix_per_s": round(value, 3),
+        "batched_sizing": config["sizing"],
+        "batched_exact_ms_per_frame": round(xel * 1e3 / args.steps, 3),
+        "batched_exact_mpix_per_s": round(xsts[-1]["pixels"] * args.steps / xel / 1e6, 3),
+        "batched_exact_frame_breakdown_ms": {"record": round(float(np.median(xissue)), 3),
+                                             "upload": round(xsts[-1]["upload_ms"], 3),
+                                             "device_stages": round(xsts[-1]["total_ms"], 3)},
+        "learned_over_exact": round((xel / args.steps) / (el / args.steps), 4),
+        "batched_one_record_thread_ms_per_frame": round(oel * 1e3 / args.steps, 3),
+        "batched_one_record_thread_mpix_per_s": round(osts[-1]["pixels"] * args.steps / oel / 1e6, 3),
+        "batched_one_record_thread_record_ms": round(float(np.median(oissue)), 3),
+    }
     if not args.no_plain:
         # plain form: one context, one synchronous flush per canvas
         B.set_sizing(B.SIZING_EXACT)
@@ -396,33 +419,18 @@ def main():
         frame.render(player.PLAIN)
         plain_out = [frame.download(k) for k in range(len(scenes))]
         pel, pper, psts, pissue = _steps(frame, player.PLAIN, 1)
-        # ablation: batched with exact sizing
-        frame.render(player.BATCHED, threads)
-        xel, xper, xsts, _ = _steps(frame, player.BATCHED, args.steps, threads)
-        # the batched form with every call recorded on one client thread
-        B.set_sizing(B.SIZING_LEARNED if learned else B.SIZING_EXACT)
-        oel, oper, osts, oissue = _steps(frame, player.BATCHED, args.steps, 1)
-        bvp = {
+        bvp.update({
             "plain_ms_per_frame": round(pel * 1e3, 3),
             "plain_mpix_per_s": round(psts[-1]["pixels"] / pel / 1e6, 3),
             "plain_paths_per_s": round(config["paths"] / pel, 1),
             "plain_frame_breakdown_ms": {"record": round(pissue[-1], 3), "upload": round(psts[-1]["upload_ms"], 3),
                                          "device_stages": round(psts[-1]["total_ms"], 3)},
-            "batched_ms_per_frame": round(el * 1e3 / args.steps, 3),
-            "batched_mpix_per_s": round(value, 3),
-            "batched_sizing": config["sizing"],
             "batched_over_plain": round(value / (psts[-1]["pixels"] / pel / 1e6), 3),
-            "batched_exact_ms_per_frame": round(xel * 1e3 / args.steps, 3),
-            "batched_exact_mpix_per_s": round(xsts[-1]["pixels"] * args.steps / xel / 1e6, 3),
-            "learned_over_exact": round((xel / args.steps) / (el / args.steps), 4),
-            "batched_one_record_thread_ms_per_frame": round(oel * 1e3 / args.steps, 3),
-            "batched_one_record_thread_mpix_per_s": round(osts[-1]["pixels"] * args.steps / oel / 1e6, 3),
-            "batched_one_record_thread_record_ms": round(float(np.median(oissue)), 3),
             "plain_vs_batched_pixels": _compare(plain_out, first),
-        }
-        line["batched_vs_plain"] = bvp
-        B.set_sizing(B.SIZING_LEARNED if learned else B.SIZING_EXACT)
+        })
         print(f"# plain: {pel * 1e3:.1f} ms/frame", file=sys.stderr, flush=True)
+    line["batched_vs_plain"] = bvp
+    B.set_sizing(B.SIZING_LEARNED if learned else B.SIZING_EXACT)
     frame.close()
 
     if not args.no_side:

@@ -67,7 +67,7 @@ class BLAmdStats(ctypes.Structure):
                 ("sizing", ctypes.c_uint32), ("fallback", ctypes.c_uint32), ("syncs", ctypes.c_uint32),
                 ("render_kernels", ctypes.c_uint32),
                 ("render_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("merge_ms", ctypes.c_double), ("upload_ms", ctypes.c_double)]
+                ("merge_ms", ctypes.c_double), ("upload_ms", ctypes.c_double), ("pack_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -86,13 +86,15 @@ struct FrameHost {
   std::vector<lut::Job> lut_jobs;
   std::vector<Stop> lut_stops;
   uint64_t lut_entries = 0;
+  // upper bound of the edge work items of `src`: 1 per line, 3 per quad, 8 per cubic
+  uint64_t items = 0;
   std::vector<DevImage> images;
   // Host only: per command, the rendering call (counted from begin()) that
   // recorded it, so errors found on the device name their call.
   std::vector<uint64_t> cmd_call;
   void clear() {
     canvases.clear(); cmds.clear(); xforms.clear(); src.clear(); vtx.clear();
-    styles.clear(); lut_jobs.clear(); lut_stops.clear(); lut_entries = 0; images.clear(); cmd_call.clear();
+    styles.clear(); lut_jobs.clear(); lut_stops.clear(); lut_entries = 0; items = 0; images.clear(); cmd_call.clear();
   }
 };
 
@@ -103,7 +105,7 @@ struct Stats {
   uint32_t sizing = 0, fallback = 0, syncs = 0;
   uint32_t render_kernels = 0;   // bit 0 k_render, 1 k_render_box_solid, 2 k_render_box
   double render_ms = 0.0, total_ms = 0.0;
-  double merge_ms = 0.0, upload_ms = 0.0;
+  double merge_ms = 0.0, upload_ms = 0.0, pack_ms = 0.0;
 };
 
 // How a frame sizes its segment and bin buffers.
@@ -196,6 +198,7 @@ struct Runner {
   uint32_t bands_max = 0;
   uint32_t max_bands() const { return bands_max; }
   uint64_t frame_bytes = 0, lut_bytes = 0, image_bytes = 0;
+  double pack_ms = 0.0;   // host time the last upload spent packing
 
   // History sizing: exact totals of the last exactly counted frame.
   uint64_t hist_segs = 0, hist_bins = 0;
@@ -238,6 +241,7 @@ struct Runner {
     r = execute(stats);
     stats.merge_ms = merge_ms;
     stats.upload_ms = up;
+    stats.pack_ms = pack_ms;
     return r;
   }
   uint32_t replay(Stats& stats) {
@@ -260,43 +264,116 @@ struct PartBase {
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// Copies part `k` of a frame into the packed staging layout, rebasing its
-// indices (canvas, command, vertex, style, LUT and image bases).
-static void pack_part(uint8_t* st, const FrameHost& fh, uint32_t k, const PartBase& b, const size_t off[9]) {
-  DevCmd* cmds = (DevCmd*)(st + off[1]) + b.cmds;
-  for (size_t i = 0; i < fh.cmds.size(); i++) {
-    DevCmd c = fh.cmds[i];
-    c.style += (uint32_t)b.styles;
-    c.canvas = k;
-    cmds[i] = c;
+// Sections of a packed frame, in layout order.
+enum Section : uint32_t { kSecCanvases, kSecCmds, kSecXforms, kSecSrc, kSecVtx, kSecStyles, kSecJobs, kSecImages,
+                          kSecStops, kSections };
+
+// Elements [lo, hi) of section `sec` of part `k` into the packed staging
+// layout, rebasing their indices (canvas, command, vertex, style, LUT, stop
+// and image bases).
+static void pack_range(uint8_t* st, const FrameHost& fh, uint32_t k, const PartBase& b, const size_t off[kSections],
+                       uint32_t sec, size_t lo, size_t hi) {
+  switch (sec) {
+    case kSecCmds: {
+      DevCmd* out = (DevCmd*)(st + off[kSecCmds]) + b.cmds;
+      for (size_t i = lo; i < hi; i++) {
+        DevCmd c = fh.cmds[i];
+        c.style += (uint32_t)b.styles;
+        c.canvas = k;
+        out[i] = c;
+      }
+      break;
+    }
+    case kSecXforms:
+      memcpy((CmdXform*)(st + off[kSecXforms]) + b.xforms + lo, fh.xforms.data() + lo, (hi - lo) * sizeof(CmdXform));
+      break;
+    case kSecSrc: {
+      SrcSeg* out = (SrcSeg*)(st + off[kSecSrc]) + b.src;
+      const uint32_t vb = (uint32_t)(b.vtx / 2), cb = (uint32_t)b.cmds;
+      for (size_t i = lo; i < hi; i++) {
+        SrcSeg g = fh.src[i];
+        g.cmd += cb;
+        g.i0 += vb;
+        g.i1 += vb;
+        out[i] = g;
+      }
+      break;
+    }
+    case kSecVtx:
+      memcpy((double*)(st + off[kSecVtx]) + b.vtx + lo, fh.vtx.data() + lo, (hi - lo) * sizeof(double));
+      break;
+    case kSecStyles: {
+      DevStyle* out = (DevStyle*)(st + off[kSecStyles]) + b.styles;
+      for (size_t i = lo; i < hi; i++) {
+        DevStyle d = fh.styles[i];
+        if (d.kind != kFetchSolid) d.lut += (uint32_t)b.luts;
+        d.img += (uint32_t)b.images;
+        out[i] = d;
+      }
+      break;
+    }
+    case kSecJobs: {
+      lut::Job* out = (lut::Job*)(st + off[kSecJobs]) + b.jobs;
+      for (size_t i = lo; i < hi; i++) {
+        lut::Job j = fh.lut_jobs[i];
+        j.lut += (uint32_t)b.luts;
+        j.stop0 += (uint32_t)b.stops;
+        out[i] = j;
+      }
+      break;
+    }
+    case kSecImages:
+      memcpy((DevImage*)(st + off[kSecImages]) + b.images + lo, fh.images.data() + lo, (hi - lo) * sizeof(DevImage));
+      break;
+    case kSecStops:
+      memcpy((Stop*)(st + off[kSecStops]) + b.stops + lo, fh.lut_stops.data() + lo, (hi - lo) * sizeof(Stop));
+      break;
+    default:
+      break;
   }
-  if (!fh.xforms.empty()) memcpy((CmdXform*)(st + off[2]) + b.xforms, fh.xforms.data(), fh.xforms.size() * sizeof(CmdXform));
-  SrcSeg* src = (SrcSeg*)(st + off[3]) + b.src;
-  const uint32_t vb = (uint32_t)(b.vtx / 2);
-  for (size_t i = 0; i < fh.src.size(); i++) {
-    SrcSeg g = fh.src[i];
-    g.cmd += (uint32_t)b.cmds;
-    g.i0 += vb;
-    g.i1 += vb;
-    src[i] = g;
+}
+
+static size_t section_base(const PartBase& b, uint32_t sec) {
+  switch (sec) {
+    case kSecCmds: return b.cmds;
+    case kSecXforms: return b.xforms;
+    case kSecSrc: return b.src;
+    case kSecVtx: return b.vtx;
+    case kSecStyles: return b.styles;
+    case kSecJobs: return b.jobs;
+    case kSecImages: return b.images;
+    case kSecStops: return b.stops;
+    default: return 0;
   }
-  if (!fh.vtx.empty()) memcpy((double*)(st + off[4]) + b.vtx, fh.vtx.data(), fh.vtx.size() * sizeof(double));
-  DevStyle* styles = (DevStyle*)(st + off[5]) + b.styles;
-  for (size_t i = 0; i < fh.styles.size(); i++) {
-    DevStyle d = fh.styles[i];
-    if (d.kind != kFetchSolid) d.lut += (uint32_t)b.luts;
-    d.img += (uint32_t)b.images;
-    styles[i] = d;
+}
+
+static size_t section_elem(uint32_t sec) {
+  switch (sec) {
+    case kSecCanvases: return sizeof(DevCanvas);
+    case kSecCmds: return sizeof(DevCmd);
+    case kSecXforms: return sizeof(CmdXform);
+    case kSecSrc: return sizeof(SrcSeg);
+    case kSecVtx: return sizeof(double);
+    case kSecStyles: return sizeof(DevStyle);
+    case kSecJobs: return sizeof(lut::Job);
+    case kSecImages: return sizeof(DevImage);
+    case kSecStops: return sizeof(Stop);
+    default: return 0;
   }
-  lut::Job* jobs = (lut::Job*)(st + off[6]) + b.jobs;
-  for (size_t i = 0; i < fh.lut_jobs.size(); i++) {
-    lut::Job j = fh.lut_jobs[i];
-    j.lut += (uint32_t)b.luts;
-    j.stop0 += (uint32_t)b.stops;
-    jobs[i] = j;
+}
+
+static size_t section_count(const FrameHost& fh, uint32_t sec) {
+  switch (sec) {
+    case kSecCmds: return fh.cmds.size();
+    case kSecXforms: return fh.xforms.size();
+    case kSecSrc: return fh.src.size();
+    case kSecVtx: return fh.vtx.size();
+    case kSecStyles: return fh.styles.size();
+    case kSecJobs: return fh.lut_jobs.size();
+    case kSecImages: return fh.images.size();
+    case kSecStops: return fh.lut_stops.size();
+    default: return 0;
   }
-  if (!fh.lut_stops.empty()) memcpy((Stop*)(st + off[8]) + b.stops, fh.lut_stops.data(), fh.lut_stops.size() * sizeof(Stop));
-  if (!fh.images.empty()) memcpy((DevImage*)(st + off[7]) + b.images, fh.images.data(), fh.images.size() * sizeof(DevImage));
 }
 
 // Lays out tiles, bands and the (band, command) matrix of every canvas, packs
@@ -304,6 +381,8 @@ static void pack_part(uint8_t* st, const FrameHost& fh, uint32_t k, const PartBa
 // batch) and copies the frame to the device in one transfer.
 uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   resident = false;
+  pack_ms = 0.0;
+  const auto t_start = std::chrono::steady_clock::now();
   size_t n_cmds_total = 0;
   for (const FrameHost* p : parts) n_cmds_total += p->cmds.size();
   if (n_cmds_total == 0) return BL_AMD_SUCCESS;
@@ -332,7 +411,7 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
     tot.stops += fh.lut_stops.size();
     tot.images += fh.images.size();
     n_src_total += fh.src.size();
-    for (const SrcSeg& sg : fh.src) n_items_max += sg.kind == kSrcLine ? 1u : sg.kind == kSrcQuad ? 3u : 8u;
+    n_items_max += fh.items;
   }
 
   uint64_t n_mat = 0;
@@ -356,54 +435,106 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
     return kErrOOM;
 
   // section offsets
-  size_t off[9];
+  size_t off[kSections];
   size_t at = 0;
-  const size_t sizes[9] = {canvases.size() * sizeof(DevCanvas), tot.cmds * sizeof(DevCmd), tot.xforms * sizeof(CmdXform),
+  const size_t sizes[kSections] = {canvases.size() * sizeof(DevCanvas), tot.cmds * sizeof(DevCmd), tot.xforms * sizeof(CmdXform),
                            tot.src * sizeof(SrcSeg), tot.vtx * sizeof(double), tot.styles * sizeof(DevStyle),
                            tot.jobs * sizeof(lut::Job), tot.images * sizeof(DevImage), tot.stops * sizeof(Stop)};
-  for (int i = 0; i < 9; i++) { off[i] = at; at = align256(at + sizes[i]); }
+  for (uint32_t i = 0; i < kSections; i++) { off[i] = at; at = align256(at + sizes[i]); }
   const size_t blob_size = at;
 
   DeviceQueue& q = device_queue();
   if (!q.staging_reserve(blob_size)) return kErrOOM;
   uint8_t* st = q.staging;
-  memcpy(st + off[0], canvases.data(), sizes[0]);
-  const unsigned hw = std::thread::hardware_concurrency();
-  const size_t n_threads = std::min<size_t>({parts.size(), (size_t)(hw ? hw : 1), (size_t)16,
-                                            blob_size / ((size_t)16 << 20) + 1});
-  if (n_threads <= 1) {
-    for (size_t k = 0; k < parts.size(); k++) pack_part(st, *parts[k], (uint32_t)k, bases[k], off);
-  }
-  else {
-    std::vector<std::thread> pool;
-    for (size_t t = 0; t < n_threads; t++) {
-      pool.emplace_back([&, t]() {
-        for (size_t k = t; k < parts.size(); k += n_threads) pack_part(st, *parts[k], (uint32_t)k, bases[k], off);
-      });
-    }
-    for (std::thread& th : pool) th.join();
-  }
-
+  memcpy(st + off[kSecCanvases], canvases.data(), sizes[kSecCanvases]);
   if (!in.reserve(blob_size + 256)) return kErrOOM;
   if (!lut_buf.reserve(tot.luts * 4 + 256)) return kErrOOM;
-  HIP_OK(hipMemcpyAsync(in.ptr, st, blob_size, hipMemcpyHostToDevice, stream));
+
+  // Packing tasks: every section of every part, in chunks of at most kChunk
+  // elements, in the order of their place in the packed layout.  They are
+  // grouped into batches of about kBatch bytes of that layout; as soon as
+  // every task of the next batch is done its bytes are copied to the device,
+  // so the copies overlap the packing of the batches after them.
+  struct Task { uint32_t part, sec; size_t lo, hi, dst; uint32_t batch; };
+  constexpr size_t kChunk = (size_t)1 << 18;
+  constexpr size_t kBatch = (size_t)32 << 20;
+  std::vector<Task> tasks;
+  for (size_t k = 0; k < parts.size(); k++)
+    for (uint32_t sec = kSecCmds; sec < kSections; sec++) {
+      const size_t n = section_count(*parts[k], sec);
+      const size_t at0 = off[sec] + section_base(bases[k], sec) * section_elem(sec);
+      for (size_t lo = 0; lo < n; lo += kChunk)
+        tasks.push_back(Task{(uint32_t)k, sec, lo, std::min(n, lo + kChunk), at0 + lo * section_elem(sec), 0});
+    }
+  std::sort(tasks.begin(), tasks.end(), [](const Task& x, const Task& y) { return x.dst < y.dst; });
+  std::vector<size_t> batch_start{0};   // layout byte where each batch starts
+  std::vector<uint32_t> batch_tasks{0};
+  for (Task& t : tasks) {
+    if (t.dst - batch_start.back() >= kBatch) {
+      batch_start.push_back(t.dst);
+      batch_tasks.push_back(0);
+    }
+    t.batch = (uint32_t)(batch_start.size() - 1);
+    batch_tasks.back()++;
+  }
+  const size_t n_batches = batch_start.size();
+  std::unique_ptr<std::atomic<uint32_t>[]> remaining(new std::atomic<uint32_t>[n_batches]);
+  for (size_t b = 0; b < n_batches; b++) remaining[b].store(batch_tasks[b], std::memory_order_relaxed);
+  size_t issued = 0;
+  hipError_t copy_err = hipSuccess;
+  // (main thread only) copies every finished batch not yet copied, in order
+  auto issue_ready = [&](bool all) {
+    while (issued < n_batches && (all || remaining[issued].load(std::memory_order_acquire) == 0)) {
+      const size_t lo = batch_start[issued], hi = issued + 1 < n_batches ? batch_start[issued + 1] : blob_size;
+      if (copy_err == hipSuccess)
+        copy_err = hipMemcpyAsync(in.as<uint8_t>() + lo, st + lo, hi - lo, hipMemcpyHostToDevice, stream);
+      issued++;
+    }
+  };
+  auto run_task = [&](const Task& t) {
+    pack_range(st, *parts[t.part], t.part, bases[t.part], off, t.sec, t.lo, t.hi);
+    remaining[t.batch].fetch_sub(1, std::memory_order_release);
+  };
+  const unsigned hw = std::thread::hardware_concurrency();
+  const size_t n_threads = std::min<size_t>({tasks.size(), (size_t)(hw ? hw : 1), (size_t)16,
+                                            blob_size / ((size_t)8 << 20) + 1});
+  std::atomic<size_t> next{0};
+  auto worker = [&](bool main) {
+    for (size_t i = next.fetch_add(1, std::memory_order_relaxed); i < tasks.size();
+         i = next.fetch_add(1, std::memory_order_relaxed)) {
+      run_task(tasks[i]);
+      if (main) issue_ready(false);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < n_threads; t++) pool.emplace_back(worker, false);
+  worker(true);
+  for (std::thread& th : pool) th.join();
+  pack_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  issue_ready(true);
+  if (copy_err != hipSuccess) {
+    report_hip("hipMemcpyAsync", copy_err, __LINE__);
+    (void)hipStreamSynchronize(stream);   // copies already issued read the staging memory
+    return kErrDevice;
+  }
+
   // gradient tables from their stops, on the device
   if (tot.jobs)
-    launch_gradient_luts((const lut::Job*)(in.as<uint8_t>() + off[6]), (const Stop*)(in.as<uint8_t>() + off[8]),
+    launch_gradient_luts((const lut::Job*)(in.as<uint8_t>() + off[kSecJobs]), (const Stop*)(in.as<uint8_t>() + off[kSecStops]),
                          (uint32_t)tot.jobs, lut_buf.as<uint32_t>(), stream);
   // the staging memory is reused by the next upload
   HIP_OK(hipStreamSynchronize(stream));
 
   const uint8_t* base = in.as<uint8_t>();
   f = FrameDev{};
-  f.canvases = (const DevCanvas*)(base + off[0]); f.n_canvases = (uint32_t)canvases.size();
-  f.cmds = (const DevCmd*)(base + off[1]); f.n_cmds = (uint32_t)tot.cmds;
-  f.xforms = (const CmdXform*)(base + off[2]);
-  f.src = (const SrcSeg*)(base + off[3]); f.n_src = (uint32_t)tot.src;
-  f.vtx = (const double*)(base + off[4]);
-  f.styles = (const DevStyle*)(base + off[5]);
+  f.canvases = (const DevCanvas*)(base + off[kSecCanvases]); f.n_canvases = (uint32_t)canvases.size();
+  f.cmds = (const DevCmd*)(base + off[kSecCmds]); f.n_cmds = (uint32_t)tot.cmds;
+  f.xforms = (const CmdXform*)(base + off[kSecXforms]);
+  f.src = (const SrcSeg*)(base + off[kSecSrc]); f.n_src = (uint32_t)tot.src;
+  f.vtx = (const double*)(base + off[kSecVtx]);
+  f.styles = (const DevStyle*)(base + off[kSecStyles]);
   f.luts = lut_buf.as<uint32_t>();
-  f.images = (const DevImage*)(base + off[7]);
+  f.images = (const DevImage*)(base + off[kSecImages]);
   f.n_tiles = n_tiles;
   f.n_groups = n_groups;
   f.n_bands = band_base;
@@ -1073,21 +1204,25 @@ void record_path_segments(BLAmdContext* c, uint32_t cmd_i, const uint8_t* cmds, 
     for (;;) {
       if (i < n && cmds[i] == kCmdOn) {
         c->frame.src.push_back(SrcSeg{cmd_i, kSrcLine, prev, vbase + (uint32_t)i});
+        c->frame.items += 1;
         prev = vbase + (uint32_t)i;
         i++;
       }
       else if (i + 2 <= n && cmds[i] == kCmdQuad) {
         c->frame.src.push_back(SrcSeg{cmd_i, kSrcQuad, prev, vbase + (uint32_t)i});
+        c->frame.items += 3;
         prev = vbase + (uint32_t)i + 1;
         i += 2;
       }
       else if (i + 3 <= n && cmds[i] == kCmdCubic) {
         c->frame.src.push_back(SrcSeg{cmd_i, kSrcCubic, prev, vbase + (uint32_t)i});
+        c->frame.items += 8;
         prev = vbase + (uint32_t)i + 2;
         i += 3;
       }
       else {
         c->frame.src.push_back(SrcSeg{cmd_i, kSrcLine, prev, start});
+        c->frame.items += 1;
         break;
       }
     }
@@ -1099,6 +1234,7 @@ void record_poly_segments(BLAmdContext* c, uint32_t cmd_i, const double* vtx, si
   c->frame.vtx.insert(c->frame.vtx.end(), vtx, vtx + 2 * n);
   for (size_t i = 1; i < n; i++) c->frame.src.push_back(SrcSeg{cmd_i, kSrcLine, vbase + (uint32_t)i - 1, vbase + (uint32_t)i});
   c->frame.src.push_back(SrcSeg{cmd_i, kSrcLine, vbase + (uint32_t)n - 1, vbase});
+  c->frame.items += n;
 }
 
 // Whether the vertices' hull reaches the canvas rows: the reference builds
@@ -2011,6 +2147,7 @@ static void copy_stats(const Stats& st, BLAmdStats* out) {
   out->render_kernels = st.render_kernels;
   out->merge_ms = st.merge_ms;
   out->upload_ms = st.upload_ms;
+  out->pack_ms = st.pack_ms;
   out->render_ms = st.render_ms;
   out->total_ms = st.total_ms;
 }

@@ -404,56 +404,130 @@ __global__ __launch_bounds__(64) void k_edges_emit(FrameDev f, FrameScratch s, u
   build_item(f, it, e);
 }
 
-// Single pass into a pool sized ahead of time: every work item is flattened
-// twice in the same thread, first counting its segments (and their bounds),
-// then writing them at a position taken from one cursor atomic per wave
-// (the cursor is one address shared by all XCDs, so per-lane atomics on it
-// would serialize).  An item whose slots do not fit below the capacity
-// raises the overflow flag and marks the slots it owns below the capacity
-// unused, since binning reads every slot below min(cursor, capacity); the
-// frame is then redone with exact counting.
-__global__ __launch_bounds__(64) void k_edges_emit_pool(FrameDev f, FrameScratch s, uint32_t cap, uint32_t item_cap) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = (int)(threadIdx.x & 63u);
-  const bool active = i < item_cap && i < s.item_off[f.n_src];
-  EdgeItem it;
-  CountEmit c;
-  if (active) {
-    it = s.items[i];
-    const uint32_t bad = build_item(f, it, c);
-    if (bad) fail_cmd(s, it.cmd, bad);
+// Pool mode: segments staged per thread in LDS before the block's slot
+// range is known (work items average little more than one segment).
+constexpr uint32_t kPoolStage = 4;
+constexpr uint32_t kPoolThreads = 256;
+constexpr uint32_t kPoolBlocks = 4096;
+
+struct StageEmit {
+  DevSeg* st;   // this thread's kPoolStage slots
+  uint32_t n = 0;
+  int32_t min_x = INT32_MAX, max_x = INT32_MIN, min_y = INT32_MAX, max_y = INT32_MIN;
+  __device__ __forceinline__ void push(int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint32_t sign) {
+    if (n < kPoolStage) st[n] = DevSeg{x0, y0, (int32_t)((uint32_t)x1 | (sign << 31)), y1};
+    n++;
+    int32_t lo = (x0 < x1 ? x0 : x1) >> 8, hi = (x0 < x1 ? x1 : x0) >> 8;
+    min_x = lo < min_x ? lo : min_x;
+    max_x = hi > max_x ? hi : max_x;
+    min_y = y0 < min_y ? y0 : min_y;
+    max_y = y1 > max_y ? y1 : max_y;
   }
-  const uint32_t need = c.n;
-  uint32_t incl = need;
+};
+
+// Single pass into a pool sized ahead of time.  Each work item is flattened
+// once: its first kPoolStage segments go to the thread's slots in LDS while
+// all of them are counted (with their bounds); the block then takes one slot
+// range from the cursor -- one atomic per block and round, since the cursor
+// is one address shared by all XCDs -- and copies the staged segments out
+// with coalesced stores, each slot's owner found among the threads' prefix
+// sums.  An item with more segments than it staged is flattened again
+// straight into its slots.  When the block's range does not fit below the
+// capacity the overflow flag is raised, the items that still fit are
+// written and the slots of the others below the capacity are marked unused
+// (binning is skipped after an overflow and the frame is redone with exact
+// counting).  Blocks stride over the work items the device counted (at most
+// `item_cap`, the size of the item buffer).
+__global__ __launch_bounds__(kPoolThreads) void k_edges_emit_pool(FrameDev f, FrameScratch s, uint32_t cap,
+                                                                  uint32_t item_cap) {
+  __shared__ DevSeg stage[kPoolThreads * kPoolStage];
+  __shared__ uint32_t pre[kPoolThreads];
+  __shared__ uint32_t cmd_of[kPoolThreads];
+  __shared__ uint32_t wave_sum[kPoolThreads / 64];
+  __shared__ uint32_t block_base;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t n_items = min(s.item_off[f.n_src], item_cap);
+  for (uint32_t i0 = blockIdx.x * kPoolThreads; i0 < n_items; i0 += gridDim.x * kPoolThreads) {
+    const uint32_t i = i0 + tid;
+    const bool active = i < n_items;
+    EdgeItem it;
+    StageEmit e{stage + tid * kPoolStage};
+    if (active) {
+      it = s.items[i];
+      const uint32_t bad = build_item(f, it, e);
+      if (bad) fail_cmd(s, it.cmd, bad);
+    }
+    const uint32_t need = e.n;
+    uint32_t incl = need;
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t t = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += t;
-  }
-  const uint32_t total = __shfl(incl, 63, 64);
-  uint32_t base = 0;
-  if (lane == 63 && total) base = atomicAdd(s.seg_cursor, total);
-  base = __shfl(base, 63, 64);
-  if (need) {
-    const uint32_t start = base + incl - need;
-    if (start <= cap && cap - start >= need) {
-      WriteEmit e{s.segs, s.seg_cmd, start, it.cmd};
-      build_item(f, it, e);
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_up(incl, d, 64);
+      if ((int)lane >= d) incl += t;
     }
-    else {
-      for (uint32_t k = start; k < cap; k++) {
-        s.segs[k] = DevSeg{0, 0, 0, 0};
-        s.seg_cmd[k] = 0xFFFFFFFFu;
+    if (lane == 63) wave_sum[wave] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0, total = 0;
+    for (uint32_t w = 0; w < kPoolThreads / 64; w++) {
+      wave_off += w < wave ? wave_sum[w] : 0u;
+      total += wave_sum[w];
+    }
+    if (tid == 0) block_base = total ? atomicAdd(s.seg_cursor, total) : 0u;
+    const uint32_t excl = wave_off + incl - need;
+    pre[tid] = excl;
+    cmd_of[tid] = active ? it.cmd : 0u;
+    __syncthreads();
+    const uint32_t base = block_base;
+    if (base <= cap && cap - base >= total) {
+      for (uint32_t j = tid; j < total; j += kPoolThreads) {
+        // owner: the last thread whose range starts at or before j
+        uint32_t lo = 0, hi = kPoolThreads - 1u;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1u) >> 1;
+          if (pre[mid] <= j) lo = mid;
+          else hi = mid - 1u;
+        }
+        const uint32_t k = j - pre[lo];
+        if (k < kPoolStage) {
+          s.segs[base + j] = stage[lo * kPoolStage + k];
+          s.seg_cmd[base + j] = cmd_of[lo];
+        }
       }
-      atomicOr(s.flags, 1u);
+      if (need > kPoolStage) {
+        WriteEmit w{s.segs, s.seg_cmd, base + excl, it.cmd};
+        build_item(f, it, w);
+      }
     }
+    else if (need) {
+      const uint32_t start = base + excl;
+      if (start <= cap && cap - start >= need) {
+        if (need <= kPoolStage) {
+          for (uint32_t k = 0; k < need; k++) {
+            s.segs[start + k] = stage[tid * kPoolStage + k];
+            s.seg_cmd[start + k] = it.cmd;
+          }
+        }
+        else {
+          WriteEmit w{s.segs, s.seg_cmd, start, it.cmd};
+          build_item(f, it, w);
+        }
+      }
+      else {
+        for (uint32_t k = start; k < cap; k++) {
+          s.segs[k] = DevSeg{0, 0, 0, 0};
+          s.seg_cmd[k] = 0xFFFFFFFFu;
+        }
+        atomicOr(s.flags, 1u);
+      }
+    }
+    wave_cmd_bounds(s.bounds, active ? it.cmd : 0xFFFFFFFFu, e.min_x, e.max_x, e.min_y, e.max_y);
+    __syncthreads();   // the next round reuses the staging slots and block_base
   }
-  wave_cmd_bounds(s.bounds, active ? it.cmd : 0xFFFFFFFFu, c.min_x, c.max_x, c.min_y, c.max_y);
 }
 
 void launch_edges_pool(const FrameDev& f, const FrameScratch& s, uint32_t cap, uint32_t item_cap, hipStream_t st) {
   if (!f.n_src || !item_cap) return;
-  hipLaunchKernelGGL(k_edges_emit_pool, dim3((item_cap + 63) / 64), dim3(64), 0, st, f, s, cap, item_cap);
+  const uint32_t blocks = std::min<uint32_t>((item_cap + kPoolThreads - 1) / kPoolThreads, kPoolBlocks);
+  hipLaunchKernelGGL(k_edges_emit_pool, dim3(blocks), dim3(kPoolThreads), 0, st, f, s, cap, item_cap);
 }
 
 void launch_edge_items(const FrameDev& f, const FrameScratch& s, hipStream_t st) {

@@ -431,6 +431,7 @@ void accumulate(BLAmdStats& sum, const BLAmdStats& s) {
   sum.total_ms += s.total_ms;
   sum.merge_ms += s.merge_ms;
   sum.upload_ms += s.upload_ms;
+  sum.pack_ms += s.pack_ms;
 }
 
 // First error of a canvas in call order: the calls' own returns merged with

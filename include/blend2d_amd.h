@@ -259,6 +259,7 @@ typedef struct BLAmdStats {
   double total_ms;     /* host wall time of the device stages */
   double merge_ms;     /* flush: host time merging the batch's recorded frames (0 for one context) */
   double upload_ms;    /* flush: host time packing and uploading the frame (0 for a replay) */
+  double pack_ms;      /* flush: the part of upload_ms spent packing the frame into staging memory */
 } BLAmdStats;
 
 BLAmdResult bl_amd_context_get_stats(const BLAmdContext* ctx, BLAmdStats* out);
