@@ -102,7 +102,8 @@ def _render_bytes(st, scenes):
 
 
 def _render_kernels(mask: int) -> str:
-    names = [n for bit, n in ((1, "k_render"), (2, "k_render_box_solid"), (4, "k_render_box")) if mask & bit]
+    names = [n for bit, n in ((1, "k_render"), (2, "k_render_box_solid"), (4, "k_render_box"), (8, "k_render_box_fast"))
+             if mask & bit]
     return "+".join(names) if names else "none"
 
 

@@ -103,7 +103,7 @@ struct Stats {
   uint64_t frame_bytes = 0, lut_bytes = 0, image_bytes = 0;
   uint64_t pred_segs = 0, pred_bins = 0;
   uint32_t sizing = 0, fallback = 0, syncs = 0;
-  uint32_t render_kernels = 0;   // bit 0 k_render, 1 k_render_box_solid, 2 k_render_box
+  uint32_t render_kernels = 0;   // bit 0 k_render, 1 k_render_box_solid, 2 k_render_box, 3 k_render_box_fast
   double render_ms = 0.0, total_ms = 0.0;
   double merge_ms = 0.0, upload_ms = 0.0, pack_ms = 0.0;
 };
@@ -625,7 +625,7 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
       !scan_tmp.reserve(scan_words * 4) ||
       !counters.reserve(64) || !cmd_tiles.reserve((size_t)f.n_cmds * 4) ||
       !tile_hdr.reserve((size_t)f.n_tiles * 16) || !group_hdr.reserve((size_t)f.n_groups * 32) ||
-      !band_buf.reserve(((size_t)f.n_bands + 1) * 28) || !tile_px.reserve((size_t)f.n_tiles * 8) ||
+      !band_buf.reserve(((size_t)f.n_bands + 1) * 40) || !tile_px.reserve((size_t)f.n_tiles * 8) ||
       !cmd_err.reserve((size_t)f.n_cmds * 4))
     return kErrOOM;
 
@@ -640,6 +640,9 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   s.tile_off = s.band_ggroups + (f.n_bands + 1);
   s.sgroup_off = s.tile_off + (f.n_bands + 1);
   s.ggroup_off = s.sgroup_off + (f.n_bands + 1);
+  s.band_slow = s.ggroup_off + (f.n_bands + 1);
+  s.band_fgroups = s.band_slow + (f.n_bands + 1);
+  s.fgroup_off = s.band_fgroups + (f.n_bands + 1);
   s.tile_px = tile_px.as<unsigned long long>();
   s.item_cnt = item_cnt.as<uint32_t>();
   s.item_off = item_off.as<uint32_t>();
@@ -715,8 +718,8 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
 
   launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
-  launch_render(f, s, f.n_tiles, f.n_groups, f.n_groups, stream);
-  stats.render_kernels = f.n_tiles ? 7u : 0u;
+  launch_render(f, s, f.n_tiles, f.n_groups, f.n_groups, f.n_groups, stream);
+  stats.render_kernels = f.n_tiles ? 15u : 0u;
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback, counters.ptr, 20, hipMemcpyDeviceToHost, stream));
@@ -781,7 +784,7 @@ uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
   s.cmd_tiles = (uint32_t*)take(cmd_tiles, (size_t)f.n_cmds * 4);
   s.tile_hdr = (uint4*)take(tile_hdr, (size_t)f.n_tiles * 16);
   s.group_hdr = (uint4*)take(group_hdr, (size_t)f.n_groups * 32);
-  s.band_nonsolid = (uint32_t*)take(band_buf, ((size_t)f.n_bands + 1) * 28);
+  s.band_nonsolid = (uint32_t*)take(band_buf, ((size_t)f.n_bands + 1) * 40);
   s.tile_px = (unsigned long long*)take(tile_px, (size_t)f.n_tiles * 8);
   s.cmd_err = (uint32_t*)take(cmd_err, (size_t)f.n_cmds * 4);
   if (!s.item_cnt || !s.item_off || !s.items || !s.src_count || !s.src_off || !s.bounds || !s.mat_cnt || !s.mat_off ||
@@ -794,6 +797,9 @@ uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
   s.tile_off = s.band_ggroups + (f.n_bands + 1);
   s.sgroup_off = s.tile_off + (f.n_bands + 1);
   s.ggroup_off = s.sgroup_off + (f.n_bands + 1);
+  s.band_slow = s.ggroup_off + (f.n_bands + 1);
+  s.band_fgroups = s.band_slow + (f.n_bands + 1);
+  s.fgroup_off = s.band_fgroups + (f.n_bands + 1);
   s.pixels = (unsigned long long*)counters_p;
   s.err_any = counters_p + 4;
 
@@ -841,13 +847,15 @@ uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
     HIP_OK(hipMemcpyAsync(host_readback + 12, s.tile_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
     HIP_OK(hipMemcpyAsync(host_readback + 13, s.sgroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
     HIP_OK(hipMemcpyAsync(host_readback + 14, s.ggroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(host_readback + 15, s.fgroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
   }
   HIP_OK(hipStreamSynchronize(stream));
   stats.syncs++;
   const uint32_t n_binned = host_readback[0], n_list = host_readback[1];
   const uint32_t n_atiles = f.n_tiles ? host_readback[12] : 0u;
   const uint32_t n_sgroups = f.n_tiles ? host_readback[13] : 0u, n_ggroups = f.n_tiles ? host_readback[14] : 0u;
-  if (n_atiles > f.n_tiles || (uint64_t)n_sgroups + n_ggroups > f.n_groups) return kErrDevice;
+  const uint32_t n_fgroups = f.n_tiles ? host_readback[15] : 0u;
+  if (n_atiles > f.n_tiles || (uint64_t)n_fgroups + n_sgroups + n_ggroups > f.n_groups) return kErrDevice;
   // every edge segment crosses at least one band and at most all of them;
   // the command lists hold at most one entry per (band, command) cell
   if ((n_binned < n_segs && !some_failed) || (uint64_t)n_binned > (uint64_t)n_segs * std::max<uint32_t>(max_bands(), 1) ||
@@ -864,8 +872,8 @@ uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
   // Stage 3: render.
   launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
-  launch_render(f, s, n_atiles, n_sgroups, n_ggroups, stream);
-  stats.render_kernels = (n_atiles ? 1u : 0u) | (n_sgroups ? 2u : 0u) | (n_ggroups ? 4u : 0u);
+  launch_render(f, s, n_atiles, n_fgroups, n_sgroups, n_ggroups, stream);
+  stats.render_kernels = (n_atiles ? 1u : 0u) | (n_sgroups ? 2u : 0u) | (n_ggroups ? 4u : 0u) | (n_fgroups ? 8u : 0u);
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback + 2, s.pixels, 8, hipMemcpyDeviceToHost, stream));

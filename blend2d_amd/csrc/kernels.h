@@ -59,12 +59,16 @@ struct FrameScratch {
   // band order: per band the tile (group) counts and their exclusive scans
   // (totals at [n_bands]).
   uint32_t* band_nonsolid; // n_bands + 1: a box fill with a non-solid style touches the band
+  uint32_t* band_slow;     // n_bands + 1: a box fill other than a solid SRC_OVER / SRC_COPY aligned box on a
+                           // 32-bit canvas touches the band
   uint32_t* band_tiles;    // n_bands + 1
   uint32_t* band_sgroups;  // n_bands + 1
   uint32_t* band_ggroups;  // n_bands + 1
   uint32_t* tile_off;      // n_bands + 1
   uint32_t* sgroup_off;    // n_bands + 1
   uint32_t* ggroup_off;    // n_bands + 1
+  uint32_t* band_fgroups;  // n_bands + 1: tile groups of bands with fast box fills only
+  uint32_t* fgroup_off;    // n_bands + 1
   uint4* tile_hdr;         // <= n_tiles: {canvas, band list begin, band list end, tile column | band << 16}
   uint4* group_hdr;        // <= 2 n_groups: two header words per box tile group (see k_group_setup),
                            // the solid bands' groups first
@@ -104,7 +108,8 @@ void launch_cmd_tiles(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 // start at zero (stats).
 void launch_band_class(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t st);
-void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint32_t n_s, uint32_t n_g, hipStream_t st);
+void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint32_t n_f, uint32_t n_s, uint32_t n_g,
+                   hipStream_t st);
 void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st);
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);

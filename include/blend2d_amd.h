@@ -254,7 +254,8 @@ typedef struct BLAmdStats {
   uint32_t fallback;     /* 1: a pool overflowed and the frame was redone with exact counting */
   uint32_t syncs;        /* host round trips of the frame */
   uint32_t render_kernels; /* render kernels the frame launched: 1 k_render (bands with analytic
-                              fills), 2 k_render_box_solid (solid box fills only), 4 k_render_box */
+                              fills), 2 k_render_box_solid (solid box fills only), 4 k_render_box,
+                              8 k_render_box_fast (solid SRC_OVER / SRC_COPY aligned boxes only) */
   double render_ms;    /* device time of the render stage (k_render, k_render_box_solid, k_render_box) */
   double total_ms;     /* host wall time of the device stages */
   double merge_ms;     /* flush: host time merging the batch's recorded frames (0 for one context) */

@@ -186,3 +186,50 @@ def scan_boundary_scenes():
 def test_scan_block_boundaries(amd):
     for sc in scan_boundary_scenes():
         _cmp(amd, sc)
+
+
+def _box_scene(seed, w, h, fmt, count, fractional=False):
+    """Aligned (or, with `fractional`, unaligned) solid boxes with SRC_OVER /
+    SRC_COPY, random colours and global alphas, many overlapping one band."""
+    rng = np.random.default_rng(seed)
+    sc = S.Scene(w, h, fmt, 0xFF000000 | int(rng.integers(0, 1 << 24)) if fmt != S.FMT_A8 else int(rng.integers(0, 256)))
+    for _ in range(count):
+        r = rng.random()
+        if r < 0.1:
+            sc.set_comp_op(int(rng.choice([S.COMP_OP_SRC_OVER, S.COMP_OP_SRC_COPY])))
+        elif r < 0.2:
+            sc.set_global_alpha(float(rng.choice([1.0, rng.uniform(0.05, 1.0)])))
+        x, y = int(rng.integers(-20, w)), int(rng.integers(-20, h))
+        bw, bh = int(rng.integers(1, w // 2 + 2)), int(rng.integers(1, h // 2 + 2))
+        col = int(rng.integers(0, 1 << 32))
+        if fractional:
+            sc.fill_rect_d_rgba32(x + float(rng.random()), y + float(rng.random()), bw + float(rng.random()), bh, col)
+        else:
+            sc.fill_rect_i_rgba32(x, y, bw, bh, col)
+    return sc
+
+
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_XRGB32])
+@pytest.mark.parametrize("seed,w,h,count", [(1, 333, 70, 40), (2, 1000, 48, 300), (3, 64, 16, 5), (4, 257, 33, 600)])
+def test_aligned_solid_boxes_take_the_fast_kernel(amd, fmt, seed, w, h, count):
+    """Bands of aligned solid SRC_OVER / SRC_COPY boxes only are rendered by
+    k_render_box_fast (more than 256 commands in a band take several record
+    chunks; ragged widths and heights leave partial groups)."""
+    sc = _box_scene(seed, w, h, fmt, count)
+    got, err, st = amd.render_scene(sc, return_context=True)
+    assert st["render_kernels"] & 8
+    want, oerr = checkers.render_oracle(sc)
+    assert err == oerr
+    mx, mean, n = checkers.diff_info(got, want)
+    assert mx == 0, f"max diff {mx}, {n} channels differ"
+
+
+@pytest.mark.parametrize("fmt,fractional", [(S.FMT_A8, False), (S.FMT_PRGB32, True)])
+def test_boxes_outside_the_fast_kernel_still_match(amd, fmt, fractional):
+    """A8 canvases and unaligned boxes stay on the general box kernels."""
+    sc = _box_scene(5, 300, 64, fmt, 200, fractional)
+    got, err, st = amd.render_scene(sc, return_context=True)
+    want, oerr = checkers.render_oracle(sc)
+    assert err == oerr
+    mx, mean, n = checkers.diff_info(got, want)
+    assert mx == 0, f"max diff {mx}, {n} channels differ"
