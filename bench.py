@@ -248,12 +248,16 @@ def _side_workload(name, seed, steps, learned, threads):
     frame.render(player.BATCHED, threads)
     el, per, sts, issue = _steps(frame, player.BATCHED, steps, threads)
     st = sts[-1]
+    # device-only replays with exact sizing: the render stage then launches
+    # only the kernels and workgroups the frame uses (learned sizing launches
+    # every render kernel over upper-bound grids, idle workgroups included)
+    B.set_sizing(B.SIZING_EXACT)
+    frame.replay()
     rep_ms, rep_render = [], []
     for _ in range(steps):
         ms, rst = frame.replay()
         rep_ms.append(ms)
         rep_render.append(rst["render_ms"])
-    B.set_sizing(B.SIZING_EXACT)
     out = {
         "workload": config["workload"],
         "mpix_per_s": round(st["pixels"] * steps / el / 1e6, 3),
@@ -270,7 +274,7 @@ def _side_workload(name, seed, steps, learned, threads):
         "fallback": int(st["fallback"]),
     }
     roof = _roofline(scenes, rep_render, rst, name)
-    roof["measured_on"] = "device-resident replays"
+    roof["measured_on"] = "device-resident replays, exact sizing"
     frame.close()
     return out, roof
 
