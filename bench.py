@@ -101,10 +101,19 @@ def _render_bytes(st, scenes):
     return per_px * st["pixels"] + st["lut_bytes"] + st["image_bytes"] + 16 * st["binned"]
 
 
-def _render_kernels(mask: int) -> str:
-    names = [n for bit, n in ((1, "k_render"), (2, "k_render_box_solid"), (4, "k_render_box"), (8, "k_render_box_fast"))
+def _render_kernels(mask: int, schedule: int = 0) -> str:
+    # queue schedules run the analytic tiles through k_render_queue, after the
+    # tile-ordering kernels (features, cost model, counting sort), all timed
+    analytic = "k_render" if schedule == 0 else "k_render_queue"
+    names = [n for bit, n in ((1, analytic), (2, "k_render_box_solid"), (4, "k_render_box"), (8, "k_render_box_fast"))
              if mask & bit]
     return "+".join(names) if names else "none"
+
+
+def _schedule(st) -> str:
+    import blend2d_amd as B
+    name = B.SCHEDULE_NAMES.get(int(st.get("schedule", 0)), "?")
+    return name if not st.get("queue_wgs") else f"{name}/{int(st['queue_wgs'])}"
 
 
 def _traffic(workload: str):
@@ -126,7 +135,8 @@ def _roofline(scenes, render_ms, st, workload):
     achieved = nbytes / (avg * 1e-3) / 1e9
     return {
         "bound": "hbm",
-        "kernel": _render_kernels(st.get("render_kernels", 1)),
+        "kernel": _render_kernels(st.get("render_kernels", 1), st.get("schedule", 0)),
+        "schedule": _schedule(st),
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -159,65 +169,104 @@ def _compare(a_list, b_list):
     return {"max_diff": int(mx), "mean_diff": float(mean), "channels_differing": int(n)}
 
 
+def _reference_passes(scenes, threads, budget_s, max_passes=5):
+    """Best whole-frame time (ms) of the reference library over the passes
+    that fit the budget (at least one), and the number of passes."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import checkers
+    best, passes, t0 = float("inf"), 0, time.perf_counter()
+    while True:
+        total = 0.0
+        for sc in scenes:
+            total += checkers.render_reference(sc, threads=threads)[2]
+        best, passes = min(best, total), passes + 1
+        if passes >= max_passes or time.perf_counter() - t0 >= budget_s:
+            return best, passes
+
+
+def _reference_passes_child(queue, scenes, threads, budget_s):
+    queue.put(_reference_passes(scenes, threads, budget_s))
+
+
 def _cpu_baseline(scenes, first_gpu, gpu_errors, pixels_per_frame, budget_s: float):
     """Reference library on this host's cores: each canvas through ref_render
     (its own begin -> end clock), summed over the frame, best of the passes
     that fit the budget, with thread_count 0 and with worker threads; the
-    first pass also checks every canvas of the GPU's first frame."""
+    first pass also checks every canvas of the GPU's first frame.  The
+    worker-thread passes run in a forked child process (it uses no GPU and
+    leaves with os._exit): the reference library has been seen to crash with
+    worker threads on some scenes, which then costs only that figure."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import multiprocessing
     import checkers
     workers = max(1, min(HOST_CORES_MAX, os.cpu_count() or 1))
-    if checkers.have_ref():
-        kind = "reference"
-        settings = [0, workers]
-
-        def render(sc, threads):
-            out, err, ms = checkers.render_reference(sc, threads=threads)
-            return out, err, ms
-    else:
-        kind = "port"
-        settings = [0]
-
-        def render(sc, threads):
-            t0 = time.perf_counter()
+    parity = {"canvases": len(scenes), "errors_match": True}
+    if not checkers.have_ref():
+        # the scalar port, one thread
+        t0 = time.perf_counter()
+        outs = []
+        for i, sc in enumerate(scenes):
             out, err = checkers.render_oracle(sc)
-            return out, err, (time.perf_counter() - t0) * 1e3
+            outs.append(out)
+            parity["errors_match"] &= int(err) == int(gpu_errors[i])
+        ms = (time.perf_counter() - t0) * 1e3
+        parity.update(_compare(first_gpu, outs), against="port")
+        return {"value": round(pixels_per_frame / (ms * 1e-3) / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+                "sample": f"one pass of the whole frame by the scalar oracle port; {_cpu_model()}",
+                "ms_per_frame": round(ms, 3)}, parity
 
-    parity = {"against": kind, "max_diff": 0, "mean_diff": 0.0, "channels_differing": 0, "errors_match": True,
-              "canvases": len(scenes)}
-    best = {}
-    t_start = time.perf_counter()
-    checked = False
-    for threads in settings:
-        passes = 0
-        while True:
-            total = 0.0
-            outs = []
-            for i, sc in enumerate(scenes):
-                out, err, ms = render(sc, threads)
-                total += ms
-                if not checked:
-                    outs.append(out)
-                    if int(err) != int(gpu_errors[i]):
-                        parity["errors_match"] = False
-            if not checked:
-                parity.update(_compare(first_gpu, outs))
-                checked = True
-            passes += 1
-            best[threads] = min(best.get(threads, float("inf")), total)
-            spent = time.perf_counter() - t_start
-            if passes >= 5 or spent >= budget_s * (settings.index(threads) + 1) / len(settings):
-                break
-        best[threads] = (best[threads], passes)
-    threads, (ms, passes) = min(best.items(), key=lambda kv: kv[1][0])
+    # parity pass, thread_count 0 (also the first timing pass)
+    outs, first_ms = [], 0.0
+    for i, sc in enumerate(scenes):
+        out, err, ms = checkers.render_reference(sc, threads=0)
+        outs.append(out)
+        first_ms += ms
+        parity["errors_match"] &= int(err) == int(gpu_errors[i])
+    parity.update(_compare(first_gpu, outs), against="reference")
+    if not parity["errors_match"]:
+        # The reference's non-JIT pipeline reports NOT_IMPLEMENTED for
+        # MULTIPLY / SCREEN on PRGB32 and skips those fills (they are
+        # JIT-only there), so such a frame is checked against the port too.
+        port = {"errors_match": True}
+        port_outs = []
+        for i, sc in enumerate(scenes):
+            out, err = checkers.render_oracle(sc)
+            port_outs.append(out)
+            port["errors_match"] &= int(err) == int(gpu_errors[i])
+            if (i + 1) % 32 == 0:
+                print(f"port check: {i + 1}/{len(scenes)} canvases", file=sys.stderr, flush=True)
+        port.update(_compare(first_gpu, port_outs))
+        parity["port"] = port
+        parity["reference_note"] = ("the reference library returned an error the GPU did not (its non-JIT "
+                                    "pipeline lacks MULTIPLY/SCREEN) and left those fills out")
+    results = {}
+    ms0, n0 = _reference_passes(scenes, 0, budget_s / 2, max_passes=4)
+    results[0] = (min(ms0, first_ms), n0 + 1)
+    ctx = multiprocessing.get_context("fork")
+    queue = ctx.Queue()
+    child = ctx.Process(target=_reference_passes_child, args=(queue, scenes, workers, budget_s / 2))
+    child.start()
+    try:
+        results[workers] = queue.get(timeout=budget_s * 4 + 120)
+    except Exception:
+        results[workers] = None
+    child.join(timeout=30)
+    if child.is_alive():
+        child.kill()
+    crashed = results[workers] is None or child.exitcode != 0
+    if crashed:
+        results.pop(workers)
+    threads, (ms, passes) = min(results.items(), key=lambda kv: kv[1][0])
     cores = 1 if threads == 0 else threads
-    others = {str(t): round(pixels_per_frame / (v[0] * 1e-3) / 1e6, 3) for t, v in best.items()}
-    sample = (f"the whole frame ({len(scenes)} canvas(es)), best of {passes} pass(es) by the "
-              f"{'reference library (no JIT, asmjit is not available offline)' if kind == 'reference' else 'scalar oracle port'}"
-              f"'s own begin->end clock, thread_count {threads}; {_cpu_model()}; "
-              f"Mpix/s by thread_count: {others}")
+    others = {str(t): round(pixels_per_frame / (v[0] * 1e-3) / 1e6, 3) for t, v in results.items()}
+    note = f"; with {workers} worker threads the reference library crashed (exit {child.exitcode})" if crashed else ""
+    if not parity["errors_match"]:
+        note += "; it skipped the fills it reports NOT_IMPLEMENTED for (MULTIPLY/SCREEN), so it did less work"
+    sample = (f"the whole frame ({len(scenes)} canvas(es)), best of {passes} pass(es) by the reference library "
+              f"(no JIT, asmjit is not available offline)'s own begin->end clock, thread_count {threads}; "
+              f"{_cpu_model()}; Mpix/s by thread_count: {others}{note}")
     return {"value": round(pixels_per_frame / (ms * 1e-3) / 1e6, 3), "unit": "Mpix/s", "cores": cores,
-            "kind": kind, "sample": sample, "ms_per_frame": round(ms, 3)}, parity
+            "kind": "reference", "sample": sample, "ms_per_frame": round(ms, 3)}, parity
 
 
 def _steps(frame, mode, steps, threads=1):
@@ -246,6 +295,10 @@ def _side_workload(name, seed, steps, learned, threads):
     frame.clear()
     B.set_sizing(B.SIZING_LEARNED if learned else B.SIZING_EXACT)
     frame.render(player.BATCHED, threads)
+    tune = frame.autotune(3)
+    print(f"# {name} autotune: {tune}", file=sys.stderr, flush=True)
+    frame.clear()
+    frame.render(player.BATCHED, threads)
     el, per, sts, issue = _steps(frame, player.BATCHED, steps, threads)
     st = sts[-1]
     # device-only replays with exact sizing: the render stage then launches
@@ -272,6 +325,9 @@ def _side_workload(name, seed, steps, learned, threads):
         "pixels_per_frame": int(st["pixels"]),
         "sizing": "learned" if learned else "exact",
         "fallback": int(st["fallback"]),
+        "schedule": _schedule(st),
+        "autotune": {"picked": B.SCHEDULE_NAMES[tune["schedule"]], "wgs_per_cu": tune["wgs_per_cu"],
+                     "best_render_ms": round(tune["best_ms"], 4), "grid_render_ms": round(tune["grid_ms"], 4)},
     }
     roof = _roofline(scenes, rep_render, rst, name)
     roof["measured_on"] = "device-resident replays, exact sizing"
@@ -328,6 +384,10 @@ def main():
     if params is not None:
         B.cost_model_set(params)
     learned = params is not None
+    from blend2d_amd import train_tile_model
+    tile_params = train_tile_model.load()
+    if tile_params is not None:
+        B.tile_model_set(tile_params)
 
     threads = args.record_threads or max(1, min(HOST_CORES_MAX, os.cpu_count() or 1))
     print(f"# workload {args.workload}: generating", file=sys.stderr, flush=True)
@@ -343,6 +403,11 @@ def main():
         raise RuntimeError(f"first frame failed with BLResult 0x{r:X}")
     first = [frame.download(k) for k in range(len(scenes))]
     print(f"# first frame: {st0}", file=sys.stderr, flush=True)
+    # the autotuner times the render stage of this resident frame under every
+    # tile schedule and records the fastest for frames of its shape, which
+    # every later frame of the run then uses (AUTO)
+    tune = frame.autotune(3)
+    print(f"# autotune: {tune}\n# {B.autotune_log().strip()}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         frame.render(player.BATCHED, threads)
@@ -386,6 +451,11 @@ def main():
                                "render": round(st["render_ms"], 3)},
         "fallback": int(st["fallback"]),
         "syncs": int(st["syncs"]),
+        "schedule": _schedule(st),
+        "autotune": {"picked": B.SCHEDULE_NAMES[tune["schedule"]], "wgs_per_cu": tune["wgs_per_cu"],
+                     "candidates": tune["candidates"], "best_render_ms": round(tune["best_ms"], 4),
+                     "grid_render_ms": round(tune["grid_ms"], 4),
+                     "tile_model": tile_params is not None},
         "device_only": {"ms_per_frame": round(float(np.mean(rep_ms)), 3),
                         "mpix_per_s": round(pixels / (float(np.mean(rep_ms)) * 1e-3) / 1e6, 3)},
         "roofline": None,
@@ -418,12 +488,15 @@ def main():
         "batched_one_record_thread_record_ms": round(float(np.median(oissue)), 3),
     }
     if not args.no_plain:
-        # plain form: one context, one synchronous flush per canvas
+        # plain form: one context, one synchronous flush per canvas, one
+        # workgroup per tile (no work queue, no autotuning)
         B.set_sizing(B.SIZING_EXACT)
+        B.set_render_schedule(B.SCHEDULE_GRID)
         frame.clear()
         frame.render(player.PLAIN)
         plain_out = [frame.download(k) for k in range(len(scenes))]
         pel, pper, psts, pissue = _steps(frame, player.PLAIN, 1)
+        B.set_render_schedule(B.SCHEDULE_AUTO)
         bvp.update({
             "plain_ms_per_frame": round(pel * 1e3, 3),
             "plain_mpix_per_s": round(psts[-1]["pixels"] / pel / 1e6, 3),

@@ -52,9 +52,15 @@ SYMBOLS = [
     "bl_amd_batch_replay", "bl_amd_batch_get_stats",
     "bl_amd_set_sizing", "bl_amd_cost_model_set", "bl_amd_context_collect_costs",
     "bl_amd_cost_model_forward", "bl_amd_cost_model_grad", "bl_amd_cost_model_train",
+    "bl_amd_set_render_schedule", "bl_amd_tile_model_set", "bl_amd_context_collect_tile_costs",
+    "bl_amd_batch_collect_tile_costs", "bl_amd_context_autotune", "bl_amd_batch_autotune", "bl_amd_autotune_log",
+    "bl_amd_autotune_reset",
 ]
 
 SIZING_EXACT, SIZING_LEARNED, SIZING_HISTORY = 0, 1, 2
+SCHEDULE_GRID, SCHEDULE_QUEUE, SCHEDULE_QUEUE_HEURISTIC, SCHEDULE_QUEUE_LEARNED, SCHEDULE_AUTO = 0, 1, 2, 3, 4
+SCHEDULE_NAMES = {0: "grid", 1: "queue", 2: "queue_heuristic", 3: "queue_learned", 4: "auto"}
+TILE_FEATURES, TILE_OUTPUTS = 16, 2
 COST_FEATURES, COST_OUTPUTS, COST_PARAMS = 16, 2, 5378
 
 
@@ -67,10 +73,20 @@ class BLAmdStats(ctypes.Structure):
                 ("sizing", ctypes.c_uint32), ("fallback", ctypes.c_uint32), ("syncs", ctypes.c_uint32),
                 ("render_kernels", ctypes.c_uint32),
                 ("render_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("merge_ms", ctypes.c_double), ("upload_ms", ctypes.c_double), ("pack_ms", ctypes.c_double)]
+                ("merge_ms", ctypes.c_double), ("upload_ms", ctypes.c_double), ("pack_ms", ctypes.c_double),
+                ("schedule", ctypes.c_uint32), ("queue_wgs", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BLAmdTune(ctypes.Structure):
+    """Mirror of BLAmdTune (include/blend2d_amd.h)."""
+    _fields_ = [("schedule", ctypes.c_uint32), ("wgs_per_cu", ctypes.c_uint32), ("candidates", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("best_ms", ctypes.c_double), ("grid_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class BLRgba64(int):
@@ -216,6 +232,14 @@ def bind(L):
         "bl_amd_cost_model_forward": ([P, SZ, P, P], U32),
         "bl_amd_cost_model_grad": ([P, P, SZ, P, P, ctypes.POINTER(ctypes.c_float)], U32),
         "bl_amd_cost_model_train": ([P, P, SZ, P, U32, U32, ctypes.c_float, P], U32),
+        "bl_amd_set_render_schedule": ([U32, U32], U32),
+        "bl_amd_tile_model_set": ([P], U32),
+        "bl_amd_context_collect_tile_costs": ([P, P, P, SZ, ctypes.POINTER(SZ)], U32),
+        "bl_amd_batch_collect_tile_costs": ([P, P, P, SZ, ctypes.POINTER(SZ)], U32),
+        "bl_amd_context_autotune": ([P, U32, ctypes.POINTER(BLAmdTune)], U32),
+        "bl_amd_batch_autotune": ([P, U32, ctypes.POINTER(BLAmdTune)], U32),
+        "bl_amd_autotune_log": ([ctypes.c_char_p, SZ], SZ),
+        "bl_amd_autotune_reset": ([], U32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -658,3 +682,54 @@ def cost_model_train(X, T, params, epochs: int = 10, batch: int = 4096, lr: floa
     _check(lib().bl_amd_cost_model_train(_dptr(X), _dptr(T), len(X), _dptr(p), epochs, batch, lr, _dptr(losses)),
            "cost_model_train")
     return p, losses
+
+
+# ---------------------------------------------------------------------------
+# Tile scheduling, tile cost model and autotuner
+# ---------------------------------------------------------------------------
+
+def set_render_schedule(schedule: int, wgs_per_cu: int = 0) -> None:
+    """Process-wide schedule of the analytic tiles (SCHEDULE_*)."""
+    _check(lib().bl_amd_set_render_schedule(schedule, wgs_per_cu), "set_render_schedule")
+
+
+def tile_model_set(params) -> None:
+    """Installs the tile cost model (COST_PARAMS float32) or removes it (None)."""
+    if params is None:
+        _check(lib().bl_amd_tile_model_set(None), "tile_model_set")
+        return
+    p = np.ascontiguousarray(params, dtype=np.float32).reshape(COST_PARAMS)
+    _check(lib().bl_amd_tile_model_set(_dptr(p)), "tile_model_set")
+
+
+def collect_tile_costs(target, cap: int = 1 << 22):
+    """(features, targets) of every analytic tile of the resident frame of a
+    BLContext or BLBatch (re-executed with tile timing)."""
+    X = np.zeros((cap, TILE_FEATURES), dtype=np.float32)
+    T = np.zeros((cap, TILE_OUTPUTS), dtype=np.float32)
+    n = ctypes.c_size_t()
+    fn = lib().bl_amd_batch_collect_tile_costs if isinstance(target, BLBatch) else lib().bl_amd_context_collect_tile_costs
+    h = target._h if isinstance(target, BLBatch) else target.handle
+    _check(fn(h, _dptr(X), _dptr(T), cap, ctypes.byref(n)), "collect_tile_costs")
+    k = min(n.value, cap)
+    return X[:k].copy(), T[:k].copy()
+
+
+def autotune(target, reps: int = 3) -> dict:
+    """Runs the autotuner on the resident frame of a BLContext or BLBatch."""
+    out = BLAmdTune()
+    fn = lib().bl_amd_batch_autotune if isinstance(target, BLBatch) else lib().bl_amd_context_autotune
+    h = target._h if isinstance(target, BLBatch) else target.handle
+    _check(fn(h, reps, ctypes.byref(out)), "autotune")
+    return out.as_dict()
+
+
+def autotune_log() -> str:
+    n = lib().bl_amd_autotune_log(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib().bl_amd_autotune_log(buf, n + 1)
+    return buf.value.decode()
+
+
+def autotune_reset() -> None:
+    _check(lib().bl_amd_autotune_reset(), "autotune_reset")

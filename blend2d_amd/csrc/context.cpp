@@ -10,10 +10,12 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/blend2d_amd.h"
@@ -106,6 +108,7 @@ struct Stats {
   uint32_t render_kernels = 0;   // bit 0 k_render, 1 k_render_box_solid, 2 k_render_box, 3 k_render_box_fast
   double render_ms = 0.0, total_ms = 0.0;
   double merge_ms = 0.0, upload_ms = 0.0, pack_ms = 0.0;
+  uint32_t schedule = BL_AMD_SCHEDULE_GRID, queue_wgs = 0;
 };
 
 // How a frame sizes its segment and bin buffers.
@@ -120,6 +123,17 @@ uint32_t g_sizing = kSizeExact;
 std::vector<float> g_model;
 uint64_t g_model_version = 0;
 
+// Process-wide tile schedule, tile cost model and the autotuner's table
+// (frame shape -> schedule) with its log.
+uint32_t g_schedule = BL_AMD_SCHEDULE_AUTO, g_queue_wgs = 4;
+std::vector<float> g_tile_model;
+uint64_t g_tile_model_version = 0;
+struct TunedSchedule { uint32_t schedule, wgs; };
+std::map<uint64_t, TunedSchedule> g_tuned;
+std::string g_tune_log;
+
+uint32_t ilog2_u64(uint64_t v) { uint32_t k = 0; while (v > 1) { v >>= 1; k++; } return k; }
+
 // The device's work queue, shared by every context of the process: one HIP
 // stream, the timing events and a pinned read-back area.  Frames of
 // different contexts are executed one after another (the mutex also makes
@@ -130,6 +144,7 @@ struct DeviceQueue {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint32_t* host_readback = nullptr;
+  uint32_t cus = 0;   // compute units of the device (persistent grids)
   // Pinned staging memory of frame uploads (grown on demand, kept).
   uint8_t* staging = nullptr;
   size_t staging_cap = 0;
@@ -148,6 +163,11 @@ struct DeviceQueue {
     if (!ev0 && hipEventCreate(&ev0) != hipSuccess) { ev0 = nullptr; return false; }
     if (!ev1 && hipEventCreate(&ev1) != hipSuccess) { ev1 = nullptr; return false; }
     if (!host_readback && hipHostMalloc((void**)&host_readback, 256, hipHostMallocDefault) != hipSuccess) { host_readback = nullptr; return false; }
+    int dev = 0, n_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu <= 0)
+      return false;
+    cus = (uint32_t)n_cu;
     ready = true;
     return true;
   }
@@ -184,6 +204,16 @@ struct Runner {
   DevBuf tile_hdr, group_hdr, tile_px, list_style, band_buf, item_cnt, item_off, items;
   DevBuf cm_params, cm_acc, cm_feat, cm_pred, cm_act;
   DevBuf cmd_err;
+  // Tile work queue: queue, bucket counters + head, keys, features,
+  // predictions, logged ticks; tile cost-model parameters.
+  DevBuf q_queue, q_misc, q_key, q_feat, q_pred, q_ticks, tm_params;
+  uint64_t tm_version = 0;
+  uint32_t cus = 0;
+  // Schedule forced by the autotuner (BL_AMD_SCHEDULE_AUTO: none).
+  uint32_t force_schedule = BL_AMD_SCHEDULE_AUTO, force_wgs = 0;
+  // Tile cost logging: per analytic tile, features and targets.
+  bool collect_tiles = false;
+  std::vector<float> tile_x, tile_t;
   // Commands of the last execution that failed on the device: (command
   // index within the frame, kFailInvalidGeometry / kFailTooManySegments bits).
   std::vector<std::pair<uint32_t, uint32_t>> failed;
@@ -214,8 +244,20 @@ struct Runner {
     ev0 = q.ev0;
     ev1 = q.ev1;
     host_readback = q.host_readback;
+    cus = q.cus;
     return true;
   }
+
+  // Frame shape the autotuner keys its choices by: tile, command and canvas
+  // counts, each by its power of two.
+  uint64_t shape_key() const {
+    return ((uint64_t)ilog2_u64((uint64_t)f.n_tiles + 1) << 40) | ((uint64_t)ilog2_u64((uint64_t)f.n_cmds + 1) << 20) |
+           (uint64_t)ilog2_u64((uint64_t)f.n_canvases + 1);
+  }
+  // Launches the render stage: the analytic tiles in the schedule in force
+  // (`allow_queue` false: GRID), the box tile groups one workgroup each.
+  uint32_t render_stage(FrameScratch& s, uint32_t n_a, uint32_t n_f, uint32_t n_s, uint32_t n_g, bool allow_queue,
+                        Stats& stats);
 
   uint32_t upload(const std::vector<FrameHost*>& parts);
   uint32_t execute(Stats& stats);
@@ -590,7 +632,7 @@ uint32_t Runner::execute(Stats& stats) {
   failed.clear();
   if (!resident) return BL_AMD_SUCCESS;
   uint32_t mode = g_sizing;
-  if (collect || (mode == kSizeLearned && g_model.empty())) mode = kSizeExact;
+  if (collect || collect_tiles || (mode == kSizeLearned && g_model.empty())) mode = kSizeExact;
   if (mode != kSizeExact) {
     uint32_t r = execute_pool(stats, mode);
     if (r != 1) return r;
@@ -718,7 +760,7 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
 
   launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
-  launch_render(f, s, f.n_tiles, f.n_groups, f.n_groups, f.n_groups, stream);
+  if (uint32_t r = render_stage(s, f.n_tiles, f.n_groups, f.n_groups, f.n_groups, true, stats)) return r;
   stats.render_kernels = f.n_tiles ? 15u : 0u;
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
@@ -747,6 +789,55 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   stats.lut_bytes = lut_bytes;
   stats.image_bytes = image_bytes;
   stats.render_ms = ms;
+  return BL_AMD_SUCCESS;
+}
+
+uint32_t Runner::render_stage(FrameScratch& s, uint32_t n_a, uint32_t n_f, uint32_t n_s, uint32_t n_g, bool allow_queue,
+                              Stats& stats) {
+  uint32_t sched = BL_AMD_SCHEDULE_GRID, wgs = 0;
+  if (allow_queue && n_a) {
+    if (collect_tiles) { sched = BL_AMD_SCHEDULE_QUEUE; wgs = 4; }
+    else if (force_schedule != BL_AMD_SCHEDULE_AUTO) { sched = force_schedule; wgs = force_wgs; }
+    else if (g_schedule != BL_AMD_SCHEDULE_AUTO) { sched = g_schedule; wgs = g_queue_wgs; }
+    else {
+      auto it = g_tuned.find(shape_key());
+      if (it != g_tuned.end()) { sched = it->second.schedule; wgs = it->second.wgs; }
+    }
+  }
+  if (sched == BL_AMD_SCHEDULE_QUEUE_LEARNED && g_tile_model.empty()) sched = BL_AMD_SCHEDULE_QUEUE_HEURISTIC;
+  if (wgs < 1 || wgs > 4) wgs = 4;
+  stats.schedule = sched;
+  stats.queue_wgs = sched == BL_AMD_SCHEDULE_GRID ? 0u : wgs;
+  if (sched == BL_AMD_SCHEDULE_GRID) {
+    launch_render(f, s, n_a, n_f, n_s, n_g, stream);
+    return BL_AMD_SUCCESS;
+  }
+  const bool ordered = sched != BL_AMD_SCHEDULE_QUEUE;
+  const bool learned = sched == BL_AMD_SCHEDULE_QUEUE_LEARNED;
+  const bool feats = ordered || collect_tiles;
+  if (!q_misc.reserve(1024 + 64) || (ordered && (!q_queue.reserve((size_t)n_a * 4) || !q_key.reserve((size_t)n_a * 4))) ||
+      (feats && !q_feat.reserve((size_t)n_a * 64)) || (learned && (!q_pred.reserve((size_t)n_a * 8) || !tm_params.reserve(cm::kParams * 4))) ||
+      (collect_tiles && !q_ticks.reserve((size_t)n_a * 8)))
+    return kErrOOM;
+  s.queue_hist = q_misc.as<uint32_t>();
+  s.queue_head = q_misc.as<uint32_t>() + 256;
+  s.queue = ordered ? q_queue.as<uint32_t>() : nullptr;
+  s.tile_key = ordered ? q_key.as<uint32_t>() : nullptr;
+  s.tile_feat = feats ? q_feat.as<float>() : nullptr;
+  s.tile_cycles = collect_tiles ? q_ticks.as<uint32_t>() : nullptr;
+  if (feats) launch_tile_features(f, s, n_a, stream);
+  if (learned) {
+    if (tm_version != g_tile_model_version) {
+      HIP_OK(hipMemcpyAsync(tm_params.ptr, g_tile_model.data(), cm::kParams * 4, hipMemcpyHostToDevice, stream));
+      HIP_OK(hipStreamSynchronize(stream));
+      tm_version = g_tile_model_version;
+    }
+    cm::launch_mlp_forward(s.tile_feat, n_a, tm_params.as<float>(), q_pred.as<float>(), stream);
+  }
+  if (ordered) launch_tile_queue(f, s, n_a, learned ? q_pred.as<float>() : nullptr, stream);
+  const uint32_t grid = std::min<uint32_t>(n_a, std::max<uint32_t>(cus, 1) * wgs);
+  launch_render_queue(f, s, grid, wgs, stream);
+  launch_render(f, s, 0, n_f, n_s, n_g, stream);
   return BL_AMD_SUCCESS;
 }
 
@@ -872,7 +963,7 @@ uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
   // Stage 3: render.
   launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
-  launch_render(f, s, n_atiles, n_fgroups, n_sgroups, n_ggroups, stream);
+  if (uint32_t r = render_stage(s, n_atiles, n_fgroups, n_sgroups, n_ggroups, !arena, stats)) return r;
   stats.render_kernels = (n_atiles ? 1u : 0u) | (n_sgroups ? 2u : 0u) | (n_ggroups ? 4u : 0u) | (n_fgroups ? 8u : 0u);
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
@@ -903,6 +994,20 @@ uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
   // flush that uploaded the frame, and it keeps no host-side list)
   if (!arena)
     if (uint32_t r = read_failures(s, host_readback[4])) return r;
+  if (collect_tiles && !arena) {
+    tile_x.assign((size_t)n_atiles * 16, 0.0f);
+    std::vector<uint32_t> ticks((size_t)n_atiles * 2);
+    if (n_atiles) {
+      HIP_OK(hipMemcpyAsync(tile_x.data(), q_feat.ptr, tile_x.size() * 4, hipMemcpyDeviceToHost, stream));
+      HIP_OK(hipMemcpyAsync(ticks.data(), q_ticks.ptr, ticks.size() * 4, hipMemcpyDeviceToHost, stream));
+      HIP_OK(hipStreamSynchronize(stream));
+    }
+    tile_t.resize((size_t)n_atiles * 2);
+    for (size_t i = 0; i < n_atiles; i++) {
+      tile_t[2 * i] = (float)log2(1.0 + ticks[i]);
+      tile_t[2 * i + 1] = (float)log2(1.0 + ticks[n_atiles + i]);
+    }
+  }
   hist_segs = n_segs;
   hist_bins = n_binned;
   float ms = 0.f;
@@ -2158,6 +2263,8 @@ static void copy_stats(const Stats& st, BLAmdStats* out) {
   out->pack_ms = st.pack_ms;
   out->render_ms = st.render_ms;
   out->total_ms = st.total_ms;
+  out->schedule = st.schedule;
+  out->queue_wgs = st.queue_wgs;
 }
 
 BLAmdResult bl_amd_context_get_error_flags(const BLAmdContext* ctx, uint32_t* flags) {
@@ -2273,6 +2380,158 @@ BLAmdResult bl_amd_cost_model_set(const float* params) {
   if (params) g_model.assign(params, params + cm::kParams);
   else g_model.clear();
   g_model_version++;
+  return BL_AMD_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Tile scheduling, tile cost model, autotuner
+// ---------------------------------------------------------------------------
+
+BLAmdResult bl_amd_set_render_schedule(uint32_t schedule, uint32_t wgs_per_cu) {
+  if (schedule > BL_AMD_SCHEDULE_AUTO || wgs_per_cu > 4) return kErrInvalid;
+  std::lock_guard<std::mutex> lock(device_queue().mu);
+  g_schedule = schedule;
+  g_queue_wgs = wgs_per_cu ? wgs_per_cu : 4;
+  return BL_AMD_SUCCESS;
+}
+
+BLAmdResult bl_amd_tile_model_set(const float* params) {
+  std::lock_guard<std::mutex> lock(device_queue().mu);
+  if (params) g_tile_model.assign(params, params + cm::kParams);
+  else g_tile_model.clear();
+  g_tile_model_version++;
+  return BL_AMD_SUCCESS;
+}
+
+}  // extern "C"
+
+namespace {
+
+uint32_t collect_tiles_of(Runner& rn, Stats& stats, float* features, float* targets, size_t cap, size_t* n_out) {
+  if (!n_out) return kErrInvalid;
+  rn.collect_tiles = true;
+  uint32_t r = rn.replay(stats);
+  rn.collect_tiles = false;
+  if (r) return r;
+  const size_t rows = rn.tile_t.size() / 2;
+  const size_t k = std::min(rows, cap);
+  if (k && (!features || !targets)) return kErrInvalid;
+  if (k) {
+    memcpy(features, rn.tile_x.data(), k * 16 * sizeof(float));
+    memcpy(targets, rn.tile_t.data(), k * 2 * sizeof(float));
+  }
+  *n_out = rows;
+  return BL_AMD_SUCCESS;
+}
+
+const char* schedule_name(uint32_t s) {
+  switch (s) {
+    case BL_AMD_SCHEDULE_GRID: return "grid";
+    case BL_AMD_SCHEDULE_QUEUE: return "queue";
+    case BL_AMD_SCHEDULE_QUEUE_HEURISTIC: return "queue_heuristic";
+    case BL_AMD_SCHEDULE_QUEUE_LEARNED: return "queue_learned";
+    default: return "auto";
+  }
+}
+
+// Times the render stage of the runner's resident frame under every
+// candidate schedule (best of `reps` replays), records the fastest for the
+// frame's shape and logs every candidate's time.
+uint32_t autotune_runner(Runner& rn, Stats& last, uint32_t reps, BLAmdTune* out) {
+  if (!out) return kErrInvalid;
+  if (!rn.resident) return kErrInvalid;
+  reps = std::max<uint32_t>(reps, 1);
+  bool have_model;
+  {
+    std::lock_guard<std::mutex> lock(device_queue().mu);
+    have_model = !g_tile_model.empty();
+  }
+  std::vector<TunedSchedule> cands = {{BL_AMD_SCHEDULE_GRID, 0}, {BL_AMD_SCHEDULE_QUEUE, 4}};
+  for (uint32_t w = 2; w <= 4; w++) cands.push_back({BL_AMD_SCHEDULE_QUEUE_HEURISTIC, w});
+  if (have_model)
+    for (uint32_t w = 2; w <= 4; w++) cands.push_back({BL_AMD_SCHEDULE_QUEUE_LEARNED, w});
+  std::vector<double> times(cands.size(), 1e30);
+  uint32_t r = BL_AMD_SUCCESS;
+  for (size_t c = 0; c < cands.size() && !r; c++) {
+    rn.force_schedule = cands[c].schedule;
+    rn.force_wgs = cands[c].wgs;
+    for (uint32_t k = 0; k < reps && !r; k++) {
+      Stats st;
+      r = rn.replay(st);
+      if (!r) times[c] = std::min(times[c], st.render_ms);
+      last = st;
+    }
+  }
+  rn.force_schedule = BL_AMD_SCHEDULE_AUTO;
+  rn.force_wgs = 0;
+  if (r) return r;
+  size_t best = 0;
+  for (size_t c = 1; c < cands.size(); c++)
+    if (times[c] < times[best]) best = c;
+  {
+    std::lock_guard<std::mutex> lock(device_queue().mu);
+    const uint64_t key = rn.shape_key();
+    g_tuned[key] = cands[best];
+    char line[160];
+    snprintf(line, sizeof(line), "shape tiles~2^%u cmds~2^%u canvases~2^%u:", (unsigned)(key >> 40),
+             (unsigned)((key >> 20) & 0xFFFFF), (unsigned)(key & 0xFFFFF));
+    g_tune_log += line;
+    for (size_t c = 0; c < cands.size(); c++) {
+      snprintf(line, sizeof(line), " %s/%u %.3f", schedule_name(cands[c].schedule), cands[c].wgs, times[c]);
+      g_tune_log += line;
+    }
+    snprintf(line, sizeof(line), " ms -> %s/%u\n", schedule_name(cands[best].schedule), cands[best].wgs);
+    g_tune_log += line;
+  }
+  out->schedule = cands[best].schedule;
+  out->wgs_per_cu = cands[best].wgs;
+  out->candidates = (uint32_t)cands.size();
+  out->reserved = 0;
+  out->best_ms = times[best];
+  out->grid_ms = times[0];
+  return BL_AMD_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+BLAmdResult bl_amd_context_collect_tile_costs(BLAmdContext* ctx, float* features, float* targets, size_t cap, size_t* n_out) {
+  if (!ctx) return kErrInvalid;
+  if (uint32_t r = check_target(ctx)) return r;
+  return collect_tiles_of(ctx->runner, ctx->stats, features, targets, cap, n_out);
+}
+
+BLAmdResult bl_amd_batch_collect_tile_costs(BLAmdBatch* b, float* features, float* targets, size_t cap, size_t* n_out) {
+  if (!b) return kErrInvalid;
+  return collect_tiles_of(b->runner, b->stats, features, targets, cap, n_out);
+}
+
+BLAmdResult bl_amd_context_autotune(BLAmdContext* ctx, uint32_t reps, BLAmdTune* out) {
+  if (!ctx) return kErrInvalid;
+  if (uint32_t r = check_target(ctx)) return r;
+  return autotune_runner(ctx->runner, ctx->stats, reps, out);
+}
+
+BLAmdResult bl_amd_batch_autotune(BLAmdBatch* b, uint32_t reps, BLAmdTune* out) {
+  if (!b) return kErrInvalid;
+  return autotune_runner(b->runner, b->stats, reps, out);
+}
+
+size_t bl_amd_autotune_log(char* buf, size_t cap) {
+  std::lock_guard<std::mutex> lock(device_queue().mu);
+  if (buf && cap) {
+    const size_t k = std::min(cap - 1, g_tune_log.size());
+    memcpy(buf, g_tune_log.data(), k);
+    buf[k] = 0;
+  }
+  return g_tune_log.size();
+}
+
+BLAmdResult bl_amd_autotune_reset(void) {
+  std::lock_guard<std::mutex> lock(device_queue().mu);
+  g_tuned.clear();
+  g_tune_log.clear();
   return BL_AMD_SUCCESS;
 }
 

@@ -82,6 +82,13 @@ struct FrameScratch {
   uint32_t* seg_cursor;    // segments taken from the pool
   uint32_t* flags;         // bit 0: segment pool overflow, bit 1: bin pool overflow
   uint32_t binned_cap;     // capacity of `binned`
+  // Work queue of the analytic tiles (k_render_queue; null otherwise):
+  uint32_t* queue;         // <= n_tiles: tile slots in queue order (null: band order)
+  uint32_t* queue_head;    // 1: next queue position
+  uint32_t* queue_hist;    // kQueueBuckets: bucket sizes, then bucket cursors
+  uint32_t* tile_key;      // <= n_tiles: bucket of each slot
+  float* tile_feat;        // <= n_tiles x 16: tile cost-model features
+  uint32_t* tile_cycles;   // <= n_tiles: logged wall-clock ticks per slot (null: not logged)
 };
 
 size_t scan_tmp_words(size_t n);
@@ -111,6 +118,14 @@ void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t s
 void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint32_t n_f, uint32_t n_s, uint32_t n_g,
                    hipStream_t st);
 void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st);
+// Work-queue form of the analytic-tile render: features of the (at most n_a)
+// analytic tiles, the queue sorted by predicted cost (`pred`: the tile cost
+// model's outputs, n_a x 2, or null for the segment-count heuristic), and
+// the persistent render over the queue with `grid` workgroups, `wgs_per_cu`
+// of them per CU (s.queue null: band order).  Box tile groups still go through launch_render.
+void launch_tile_features(const FrameDev& f, const FrameScratch& s, uint32_t n_a, hipStream_t st);
+void launch_tile_queue(const FrameDev& f, const FrameScratch& s, uint32_t n_a, const float* pred, hipStream_t st);
+void launch_render_queue(const FrameDev& f, const FrameScratch& s, uint32_t grid, uint32_t wgs_per_cu, hipStream_t st);
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 // Fills the gradient tables of `n_jobs` jobs (one workgroup per table).

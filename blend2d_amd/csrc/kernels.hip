@@ -1031,14 +1031,10 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t wg, uint32_t n_tiles) {
   return (wg % kXcds) * per + wg / kXcds;
 }
 
-// Four workgroups per CU: LDS (7 cell buffers) and the register budget are
-// sized for it.
-__global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScratch s, uint32_t n) {
+// One analytic tile (header `slot`): the whole workgroup rasterizes the
+// band's commands that reach the tile and composites them in order.
+__device__ __forceinline__ void render_tile(const FrameDev& f, const FrameScratch& s, uint32_t slot) {
   __shared__ uint32_t cells[kBatch][kCellWords];
-  const uint32_t slot = xcd_tile(blockIdx.x, n);
-  // uniform: padding workgroups of the XCD mapping, and (when n is an upper
-  // bound) slots past the analytic tiles
-  if (slot >= n || slot >= s.tile_off[f.n_bands]) return;
   const int t = threadIdx.x;
   const uint4 th = s.tile_hdr[slot];
   const DevCanvas cv = f.canvases[th.x];
@@ -1287,6 +1283,169 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
 
   if (!touched || !row_ok) return;
   store4(rowp, px, (int)cv.w, a8, d);
+}
+
+// Four workgroups per CU: LDS (7 cell buffers) and the register budget are
+// sized for it.  One workgroup per analytic tile, XCD-contiguous.
+__global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScratch s, uint32_t n) {
+  const uint32_t slot = xcd_tile(blockIdx.x, n);
+  // uniform: padding workgroups of the XCD mapping, and (when n is an upper
+  // bound) slots past the analytic tiles
+  if (slot >= n || slot >= s.tile_off[f.n_bands]) return;
+  render_tile(f, s, slot);
+}
+
+// The same tiles drained from a work queue by a persistent grid: every
+// workgroup takes the next queue position with one atomic until the queue is
+// exhausted (every workgroup reaches that exit).  The queue holds the
+// analytic tile slots in the order k_tile_scatter wrote (longest predicted
+// first), or is the identity (null: band order).  With `s.tile_cycles` set
+// (2 x the tile count), the wall-clock ticks (100 MHz) of every tile and
+// then its composited pixels are logged (cost-model samples).
+template<bool kLog, int kOcc>
+__global__ __launch_bounds__(kTileThreads, kOcc) void k_render_queue(FrameDev f, FrameScratch s) {
+  __shared__ uint32_t next;
+  const uint32_t total = s.tile_off[f.n_bands];
+  for (;;) {
+    __syncthreads();   // the previous tile's LDS reads are done and `next` is free
+    if (threadIdx.x == 0) next = atomicAdd(s.queue_head, 1u);
+    __syncthreads();
+    const uint32_t i = next;
+    if (i >= total) break;   // uniform
+    const uint32_t slot = s.queue ? s.queue[i] : i;
+    if constexpr (!kLog) {
+      render_tile(f, s, slot);
+    }
+    else {
+      const long long c0 = wall_clock64();
+      render_tile(f, s, slot);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        s.tile_cycles[slot] = (uint32_t)min((long long)0xFFFFFFFF, wall_clock64() - c0);
+        // and the tile's composited pixels (written by this thread)
+        const uint4 th = s.tile_hdr[slot];
+        const DevCanvas& cv = f.canvases[th.x];
+        const unsigned long long px = s.tile_px[cv.tile_base + (th.w >> 16) * cv.tiles_x + (th.w & 0xFFFFu)];
+        s.tile_cycles[total + slot] = (uint32_t)(px < 0xFFFFFFFFull ? px : 0xFFFFFFFFull);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Work queue of the analytic tiles
+// ---------------------------------------------------------------------------
+
+// Per analytic tile the 16 features of the tile cost model, from one walk of
+// the band's command list by one wave (the list entries whose tile-column
+// range holds the tile are the commands the tile composites):
+//   0 log2(1 + commands)          1 log2(1 + binned segments of them)
+//   2 log2(1 + segments of commands starting left of the tile)
+//   3 log2(1 + list entries of the band)
+//   4..8 log2(1 + commands by style: linear, radial, conic, pattern, solid)
+//   9 log2(1 + box commands)      10 log2(1 + commands not SRC_OVER)
+//   11 A8 canvas                  12 valid rows / 16   13 valid columns / 64
+//   14 log2(1 + command batches)  15 log2(1 + largest segment count of one command)
+__global__ __launch_bounds__(256) void k_tile_features(FrameDev f, FrameScratch s, uint32_t n) {
+  const uint32_t slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (slot >= n || slot >= s.tile_off[f.n_bands]) return;   // uniform per wave
+  const uint4 th = s.tile_hdr[slot];
+  const DevCanvas& cv = f.canvases[th.x];
+  const uint32_t tcol = th.w & 0xFFFFu, band = th.w >> 16;
+  uint32_t c[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // n, segs, segs_left, lin, rad, con, pat, solid, box, op, -, max
+  for (uint32_t li = th.y + lane; li < th.z; li += 64) {
+    const uint4 ent = s.list_ent[li];
+    const uint32_t lo = ent.y & 0xFFFFu, hi = ent.y >> 16;
+    if (tcol < lo || tcol > hi) continue;
+    const DevCmd& cmd = f.cmds[ent.x];
+    const uint32_t kind = f.styles[s.list_style[li]].kind;
+    c[0] += 1u;
+    c[1] += ent.w;
+    c[2] += lo < tcol ? ent.w : 0u;
+    c[3] += kind == kFetchLinear;
+    c[4] += kind == kFetchRadial;
+    c[5] += kind == kFetchConic;
+    c[6] += kind >= kFetchPatternNN;
+    c[7] += kind == kFetchSolid;
+    c[8] += cmd.kind != kCmdAnalytic;
+    c[9] += cmd.op != kOpSrcOver;
+    c[11] = ent.w > c[11] ? ent.w : c[11];
+  }
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint32_t v = __shfl_xor(c[k], o, 64);
+      c[k] = k == 11 ? (v > c[k] ? v : c[k]) : c[k] + v;
+    }
+  }
+  if (lane != 0) return;
+  const uint32_t x0 = tcol * kTileW, y0 = band * kTileH;
+  const uint32_t rows = cv.h - y0 < (uint32_t)kTileH ? cv.h - y0 : (uint32_t)kTileH;
+  const uint32_t cols = cv.w - x0 < (uint32_t)kTileW ? cv.w - x0 : (uint32_t)kTileW;
+  float* o = s.tile_feat + (size_t)slot * 16;
+  o[0] = log2f(1.0f + (float)c[0]);
+  o[1] = log2f(1.0f + (float)c[1]);
+  o[2] = log2f(1.0f + (float)c[2]);
+  o[3] = log2f(1.0f + (float)(th.z - th.y));
+#pragma unroll
+  for (int k = 0; k < 5; k++) o[4 + k] = log2f(1.0f + (float)c[3 + k]);
+  o[9] = log2f(1.0f + (float)c[8]);
+  o[10] = log2f(1.0f + (float)c[9]);
+  o[11] = cv.format == kFmtA8 ? 1.0f : 0.0f;
+  o[12] = (float)rows / (float)kTileH;
+  o[13] = (float)cols / (float)kTileW;
+  o[14] = log2f(1.0f + (float)((c[0] + kBatch - 1) / kBatch));
+  o[15] = log2f(1.0f + (float)c[11]);
+}
+
+// Queue key of each analytic tile: its predicted cost (log2 ticks from the
+// model's first output when `pred` is set, else log2 of binned segments +
+// 16 x commands from the features) in 1/8 steps, as a bucket of a
+// descending counting sort; bucket sizes into `hist` (256 entries, zeroed).
+constexpr uint32_t kQueueBuckets = 256;
+__global__ void k_tile_keys(FrameDev f, FrameScratch s, uint32_t n, const float* __restrict__ pred) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= n || slot >= s.tile_off[f.n_bands]) return;
+  float cost;
+  if (pred) {
+    cost = pred[(size_t)slot * 2];
+  }
+  else {
+    const float* x = s.tile_feat + (size_t)slot * 16;
+    cost = log2f(1.0f + (exp2f(x[1]) - 1.0f) + 16.0f * (exp2f(x[0]) - 1.0f));
+  }
+  float q = cost * 8.0f;
+  q = q != q ? 0.0f : q;   // a NaN prediction sorts last
+  const uint32_t key = q <= 0.0f ? 0u : q >= (float)(kQueueBuckets - 1) ? kQueueBuckets - 1 : (uint32_t)q;
+  const uint32_t bucket = kQueueBuckets - 1 - key;
+  s.tile_key[slot] = bucket;
+  atomicAdd(&s.queue_hist[bucket], 1u);
+}
+
+// Exclusive scan of the bucket sizes in place (one workgroup), and the queue
+// head reset.
+__global__ __launch_bounds__(kQueueBuckets) void k_queue_scan(FrameScratch s) {
+  __shared__ uint32_t v[kQueueBuckets];
+  const uint32_t t = threadIdx.x;
+  v[t] = s.queue_hist[t];
+  __syncthreads();
+  for (uint32_t d = 1; d < kQueueBuckets; d <<= 1) {
+    const uint32_t a = t >= d ? v[t - d] : 0u;
+    __syncthreads();
+    v[t] += a;
+    __syncthreads();
+  }
+  s.queue_hist[t] = v[t] - s.queue_hist[t];
+  if (t == 0) *s.queue_head = 0u;
+}
+
+// Queue positions: within a bucket in any order (tiles are independent).
+__global__ void k_tile_scatter(FrameDev f, FrameScratch s, uint32_t n) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= n || slot >= s.tile_off[f.n_bands]) return;
+  s.queue[atomicAdd(&s.queue_hist[s.tile_key[slot]], 1u)] = slot;
 }
 
 // Canvas of a global band: the last one whose first band is <= b.
@@ -1732,6 +1891,29 @@ void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint3
   if (n_f) hipLaunchKernelGGL(k_render_box_fast, dim3((n_f + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_f);
   if (n_s) hipLaunchKernelGGL(k_render_box_solid, dim3((n_s + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_s);
   if (n_g) hipLaunchKernelGGL(k_render_box, dim3((n_g + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_g);
+}
+
+void launch_tile_features(const FrameDev& f, const FrameScratch& s, uint32_t n_a, hipStream_t st) {
+  if (!f.n_tiles || !n_a) return;
+  hipLaunchKernelGGL(k_tile_features, dim3((n_a + 3) / 4), dim3(256), 0, st, f, s, n_a);
+}
+
+void launch_tile_queue(const FrameDev& f, const FrameScratch& s, uint32_t n_a, const float* pred, hipStream_t st) {
+  if (!f.n_tiles || !n_a) return;
+  launch_fill_u32(s.queue_hist, 0u, kQueueBuckets, st);
+  hipLaunchKernelGGL(k_tile_keys, dim3((n_a + 255) / 256), dim3(256), 0, st, f, s, n_a, pred);
+  hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(kQueueBuckets), 0, st, s);
+  hipLaunchKernelGGL(k_tile_scatter, dim3((n_a + 255) / 256), dim3(256), 0, st, f, s, n_a);
+}
+
+void launch_render_queue(const FrameDev& f, const FrameScratch& s, uint32_t grid, uint32_t wgs_per_cu, hipStream_t st) {
+  if (!f.n_tiles || !grid) return;
+  if (!s.queue) launch_fill_u32(s.queue_head, 0u, 1, st);
+  // Four workgroups per CU need the four-occupancy instance (128 VGPRs,
+  // some spilled); fewer take the three-occupancy one (170 VGPRs).
+  if (s.tile_cycles) hipLaunchKernelGGL((k_render_queue<true, 4>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
+  else if (wgs_per_cu >= 4) hipLaunchKernelGGL((k_render_queue<false, 4>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
+  else hipLaunchKernelGGL((k_render_queue<false, 3>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
 }
 
 void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st) {

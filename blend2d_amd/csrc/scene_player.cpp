@@ -597,6 +597,13 @@ uint32_t b2p_frame_replay(B2PFrame* f, double* elapsed_ms, BLAmdStats* stats) {
   return r;
 }
 
+// Runs the autotuner on the batched frame resident in the device
+// (bl_amd_batch_autotune): its replays render onto the canvases again.
+uint32_t b2p_frame_autotune(B2PFrame* f, uint32_t reps, BLAmdTune* out) {
+  if (!f) return BL_AMD_ERROR_INVALID_VALUE;
+  return bl_amd_batch_autotune(f->batch, reps, out);
+}
+
 size_t b2p_frame_count(const B2PFrame* f) { return f ? f->scenes.size() : 0; }
 
 // Copies canvas `k` to `out` (rows of width * bytes-per-pixel, no padding).

@@ -14,7 +14,7 @@ import os
 
 import numpy as np
 
-from . import BLAmdStats as BLStats, BLError, lib as _lib
+from . import BLAmdStats as BLStats, BLAmdTune, BLError, lib as _lib
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PLAYER_SO = os.path.join(HERE, "libb2amd_player.so")
@@ -51,6 +51,8 @@ def bind(p):
     p.b2p_frame_replay.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(BLStats)]
     p.b2p_frame_download.restype = U32
     p.b2p_frame_download.argtypes = [P, SZ, P, SZ]
+    p.b2p_frame_autotune.restype = U32
+    p.b2p_frame_autotune.argtypes = [P, U32, ctypes.POINTER(BLAmdTune)]
     return p
 
 
@@ -79,6 +81,15 @@ class Frame:
             self.close()
         except Exception:
             pass
+
+    def autotune(self, reps: int = 3) -> dict:
+        """The autotuner on the batched frame resident on the device (after a
+        BATCHED render); its replays render onto the canvases again."""
+        out = BLAmdTune()
+        r = player().b2p_frame_autotune(self._h, reps, ctypes.byref(out))
+        if r:
+            raise BLError(r, "b2p_frame_autotune")
+        return out.as_dict()
 
     def clear(self):
         r = player().b2p_frame_clear(self._h)

@@ -261,6 +261,8 @@ typedef struct BLAmdStats {
   double merge_ms;     /* flush: host time merging the batch's recorded frames (0 for one context) */
   double upload_ms;    /* flush: host time packing and uploading the frame (0 for a replay) */
   double pack_ms;      /* flush: the part of upload_ms spent packing the frame into staging memory */
+  uint32_t schedule;   /* BL_AMD_SCHEDULE_* the analytic tiles ran with (AUTO resolved) */
+  uint32_t queue_wgs;  /* queue schedules: workgroups per CU of the persistent grid (0 for GRID) */
 } BLAmdStats;
 
 BLAmdResult bl_amd_context_get_stats(const BLAmdContext* ctx, BLAmdStats* out);
@@ -336,6 +338,69 @@ BLAmdResult bl_amd_cost_model_grad(const float* features, const float* targets, 
  * ones; `epoch_loss` (optional, `epochs` floats) the mean loss per epoch. */
 BLAmdResult bl_amd_cost_model_train(const float* features, const float* targets, size_t n, float* params,
                                     uint32_t epochs, uint32_t batch, float lr, float* epoch_loss);
+
+/* ---- tile scheduling and autotuning (no counterpart in the reference) ----
+ * The tiles of bands with path fills (the heavy part of the render stage)
+ * run in one of these schedules; none changes pixels (tiles own disjoint
+ * pixels):
+ *   GRID             one workgroup per tile, tiles dealt to the XCDs in
+ *                    contiguous ranges;
+ *   QUEUE            a persistent grid (wgs_per_cu workgroups per CU)
+ *                    draining a device work queue of the tiles in band order;
+ *   QUEUE_HEURISTIC  the queue sorted longest first by each tile's binned
+ *                    segments + 16 x commands;
+ *   QUEUE_LEARNED    the queue sorted longest first by the tile cost model:
+ *                    an MLP of the sizing model's shape (16 tile features ->
+ *                    64 -> 64 -> 2; outputs log2(1 + wall-clock ticks of the
+ *                    tile), log2(1 + composited pixels)) trained on tile
+ *                    timings logged by the render kernel (without a model it
+ *                    runs as QUEUE_HEURISTIC);
+ *   AUTO (default)   the schedule the autotuner picked for frames of this
+ *                    shape (tile, command and canvas counts, each by its
+ *                    power of two), else GRID.
+ * bl_amd_context_replay_noalloc always runs GRID. */
+enum { BL_AMD_SCHEDULE_GRID = 0, BL_AMD_SCHEDULE_QUEUE = 1, BL_AMD_SCHEDULE_QUEUE_HEURISTIC = 2,
+       BL_AMD_SCHEDULE_QUEUE_LEARNED = 3, BL_AMD_SCHEDULE_AUTO = 4 };
+enum { BL_AMD_TILE_FEATURES = 16, BL_AMD_TILE_OUTPUTS = 2 };
+
+/* Process-wide schedule (default AUTO); wgs_per_cu 1..4 for the queue
+ * schedules (0: 4). */
+BLAmdResult bl_amd_set_render_schedule(uint32_t schedule, uint32_t wgs_per_cu);
+/* Installs the tile cost model (BL_AMD_COST_PARAMS floats, the layout of
+ * bl_amd_cost_model_set); NULL removes it. */
+BLAmdResult bl_amd_tile_model_set(const float* params);
+/* Re-executes the resident frame with exact sizing on the QUEUE schedule
+ * with tile timing and returns, per analytic tile, its 16 features and the
+ * targets log2(1 + wall-clock ticks at 100 MHz), log2(1 + composited
+ * pixels).  `n_out` receives the number of rows (rows beyond `cap` are not
+ * written). */
+BLAmdResult bl_amd_context_collect_tile_costs(BLAmdContext* ctx, float* features, float* targets, size_t cap,
+                                              size_t* n_out);
+BLAmdResult bl_amd_batch_collect_tile_costs(BLAmdBatch* batch, float* features, float* targets, size_t cap,
+                                            size_t* n_out);
+
+typedef struct BLAmdTune {
+  uint32_t schedule;    /* the fastest schedule (BL_AMD_SCHEDULE_*) */
+  uint32_t wgs_per_cu;  /* its workgroups per CU (queue schedules) */
+  uint32_t candidates;  /* schedules timed */
+  uint32_t reserved;
+  double best_ms;       /* its render-stage time, best of the replays */
+  double grid_ms;       /* the GRID schedule's, for comparison */
+} BLAmdTune;
+
+/* Autotuner: times the render stage of the resident frame under every
+ * schedule (GRID; QUEUE, QUEUE_HEURISTIC and, with a tile model,
+ * QUEUE_LEARNED at 2, 3 and 4 workgroups per CU), best of `reps` replays
+ * each, records the fastest for frames of this shape (what AUTO then runs)
+ * and returns it. */
+BLAmdResult bl_amd_context_autotune(BLAmdContext* ctx, uint32_t reps, BLAmdTune* out);
+BLAmdResult bl_amd_batch_autotune(BLAmdBatch* batch, uint32_t reps, BLAmdTune* out);
+/* The autotuner's record, one line per tuned frame shape with every
+ * candidate's time; copies up to cap - 1 bytes and a terminating zero,
+ * returns the full length. */
+size_t bl_amd_autotune_log(char* buf, size_t cap);
+/* Forgets every tuned shape. */
+BLAmdResult bl_amd_autotune_reset(void);
 
 #ifdef __cplusplus
 }

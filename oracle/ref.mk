@@ -34,7 +34,7 @@ flags_for = $(strip \
   $(if $(findstring _avx2.cpp,$1),-mpopcnt -mpclmul -mbmi -mbmi2 -mavx2 -DBL_TARGET_OPT_POPCNT -DBL_TARGET_OPT_BMI2) \
   $(if $(findstring _avx512.cpp,$1),-mpopcnt -mpclmul -mbmi -mbmi2 -mavx512f -mavx512bw -mavx512dq -mavx512cd -mavx512vl -DBL_TARGET_OPT_POPCNT -DBL_TARGET_OPT_BMI2))
 
-all: $(OUT)/libblend2d_ref.so $(OUT)/libref_render.so
+all: $(OUT)/libblend2d_ref.so $(OUT)/libref_render.so $(OUT)/libamd_ref_render.so
 
 $(OBJ)/%.o: $(REF)/%.cpp
 	@mkdir -p $(dir $@)
@@ -48,3 +48,16 @@ $(OUT)/libref_render.so: oracle/ref_render.cpp blend2d_amd/csrc/scene_format.h $
 	  -L$(OUT) -lblend2d_ref -Wl,-rpath,'$$ORIGIN'
 
 .PHONY: all
+
+# The same scene driver bound to the MI355X backend through its BLContext
+# implementation (blend2d_amd/binding/amdcontext.cpp, compiled against the
+# reference headers): the integration test of that binding.
+amd: $(OUT)/libamd_ref_render.so
+
+$(OUT)/libamd_ref_render.so: oracle/ref_render.cpp blend2d_amd/binding/amdcontext.cpp blend2d_amd/csrc/scene_format.h \
+                             include/blend2d_amd.h $(OUT)/libblend2d_ref.so blend2d_amd/libblend2d_amd.so
+	$(CXX) -std=c++17 -O2 -fPIC -shared -DREF_RENDER_AMD -DBL_BUILD_NO_JIT -I$(REF) -Iblend2d_amd/csrc -Iinclude -o $@ \
+	  oracle/ref_render.cpp blend2d_amd/binding/amdcontext.cpp \
+	  -L$(OUT) -lblend2d_ref -Lblend2d_amd -lblend2d_amd -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../../blend2d_amd'
+
+.PHONY: amd

@@ -88,6 +88,17 @@ size_t decode_style(Call& call, const uint8_t* p, const std::vector<BLImage>& im
 
 }  // namespace
 
+// With REF_RENDER_AMD the same driver renders through the MI355X backend's
+// BLContext implementation (blend2d_amd/binding/amdcontext.cpp): the calls
+// go through the reference's own C++ API and its BLContextVirt table to the
+// GPU, as they would for a user of the reference with the backend bound in.
+#if defined(REF_RENDER_AMD)
+BLResult bl_amd_context_init_impl(BLContextCore* self, BLImageCore* image, const BLContextCreateInfo* cci) noexcept;
+#define REF_RENDER_ENTRY ref_render_amd
+#else
+#define REF_RENDER_ENTRY ref_render
+#endif
+
 extern "C" {
 
 // Renders `scene` and writes the canvas rows (width * bytes-per-pixel each,
@@ -95,7 +106,7 @@ extern "C" {
 // call (rendering continues after an error, as a caller ignoring errors
 // would), or 0x7FFFFFFF when the scene cannot be parsed.  `elapsed_ms`
 // receives the wall time from BLContext::begin() to BLContext::end().
-uint32_t ref_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_size,
+uint32_t REF_RENDER_ENTRY(const uint8_t* scene, size_t size, uint8_t* out, size_t out_size,
                     uint32_t thread_count, double* elapsed_ms) {
   Reader r{scene, scene + size};
   SceneHeader hdr;
@@ -302,7 +313,11 @@ uint32_t ref_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
 
   auto t0 = std::chrono::steady_clock::now();
   BLContext ctx;
+#if defined(REF_RENDER_AMD)
+  note(bl_amd_context_init_impl(&ctx, &canvas, &ci));
+#else
   note(ctx.begin(canvas, ci));
+#endif
 
   for (Call& call : calls) {
     switch (call.op) {
@@ -399,6 +414,17 @@ uint32_t ref_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
       default: break;
     }
   }
+#if defined(REF_RENDER_AMD)
+  // The bound backend checks geometry while the device renders: a fill that
+  // fails there is reported by the flush through the accumulated error
+  // flags (the reference's own asynchronous mode does the same for deferred
+  // work), so the driver flushes and turns the flags into the result the
+  // synchronous mode returns from the failing call.
+  note(ctx.flush(BL_CONTEXT_FLUSH_SYNC));
+  const BLContextErrorFlags ef = ctx.accumulated_error_flags();
+  if (ef & BL_CONTEXT_ERROR_FLAG_INVALID_GEOMETRY) note(BL_ERROR_INVALID_GEOMETRY);
+  if (ef & BL_CONTEXT_ERROR_FLAG_OUT_OF_MEMORY) note(BL_ERROR_OUT_OF_MEMORY);
+#endif
   note(ctx.end());
   auto t1 = std::chrono::steady_clock::now();
   if (elapsed_ms)

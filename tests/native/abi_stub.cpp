@@ -151,7 +151,18 @@ BLAmdResult bl_amd_image_upload(BLAmdImage* img, const void* pixels, intptr_t st
   logf("image_upload %u %016llx", s->serial, (unsigned long long)h);
   return 0;
 }
-BLAmdResult bl_amd_image_download(const BLAmdImage*, void*, intptr_t) { return 0; }
+// Writes a marker instead of pixels: byte x of row y is (3 * y + x) & 0xFF,
+// so a caller's test can see that the write-back reached its own buffer.
+BLAmdResult bl_amd_image_download(const BLAmdImage* img, void* pixels, intptr_t stride) {
+  const StubImage* s = reinterpret_cast<const StubImage*>(img);
+  const size_t row = size_t(s->w) * (s->format == BL_AMD_FORMAT_A8 ? 1 : 4);
+  for (int y = 0; y < s->h; y++) {
+    uint8_t* p = static_cast<uint8_t*>(pixels) + y * stride;
+    for (size_t x = 0; x < row; x++) p[x] = uint8_t(3 * y + x);
+  }
+  logf("image_download %u", s->serial);
+  return 0;
+}
 BLAmdResult bl_amd_image_fill_raw(BLAmdImage* img, uint32_t v) { logf("fill_raw %u %08x", serial_of(img), v); return 0; }
 BLAmdResult bl_amd_image_get_data(const BLAmdImage*, int*, int*, uint32_t*, void**, intptr_t*) { return 0; }
 
@@ -283,6 +294,14 @@ BLAmdResult bl_amd_context_collect_costs(BLAmdContext*, float*, float*, size_t, 
 BLAmdResult bl_amd_cost_model_forward(const float*, size_t, const float*, float*) { return 0; }
 BLAmdResult bl_amd_cost_model_grad(const float*, const float*, size_t, const float*, float*, float*) { return 0; }
 BLAmdResult bl_amd_cost_model_train(const float*, const float*, size_t, float*, uint32_t, uint32_t, float, float*) { return 0; }
+BLAmdResult bl_amd_set_render_schedule(uint32_t, uint32_t) { return 0; }
+BLAmdResult bl_amd_tile_model_set(const float*) { return 0; }
+BLAmdResult bl_amd_context_collect_tile_costs(BLAmdContext*, float*, float*, size_t, size_t* n) { if (n) *n = 0; return 0; }
+BLAmdResult bl_amd_batch_collect_tile_costs(BLAmdBatch*, float*, float*, size_t, size_t* n) { if (n) *n = 0; return 0; }
+BLAmdResult bl_amd_context_autotune(BLAmdContext*, uint32_t, BLAmdTune* out) { if (out) std::memset(out, 0, sizeof(*out)); return 0; }
+BLAmdResult bl_amd_batch_autotune(BLAmdBatch*, uint32_t, BLAmdTune* out) { if (out) std::memset(out, 0, sizeof(*out)); return 0; }
+size_t bl_amd_autotune_log(char* buf, size_t cap) { if (buf && cap) buf[0] = 0; return 0; }
+BLAmdResult bl_amd_autotune_reset(void) { return 0; }
 
 }  // extern "C"
 

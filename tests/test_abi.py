@@ -19,7 +19,7 @@ E_INVALID_STATE = 0x10002
 
 def _declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:BLAmdResult|uint32_t|int)\s+(bl_amd_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:BLAmdResult|uint32_t|int|size_t)\s+(bl_amd_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_what_the_mirror_binds():
@@ -94,3 +94,29 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(B, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(RuntimeError, match="missing"):
         B.lib()
+
+
+def test_python_struct_layouts_match_the_header(tmp_path):
+    """ctypes mirrors of the header's structs have the C sizes and field
+    offsets (a mirror shorter than the struct would let the library write
+    past the Python object)."""
+    import shutil
+    import subprocess
+    if not shutil.which("g++"):
+        pytest.skip("g++ is missing")
+    structs = {"BLAmdStats": B.BLAmdStats, "BLAmdTune": B.BLAmdTune}
+    lines = ["#include <cstdio>", "#include <cstddef>", '#include "blend2d_amd.h"', "int main() {"]
+    for name, cls in structs.items():
+        lines.append(f'  printf("{name} %zu\\n", sizeof({name}));')
+        for field, _ in cls._fields_:
+            lines.append(f'  printf("{name}.{field} %zu\\n", offsetof({name}, {field}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.cpp"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["g++", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
+    for name, cls in structs.items():
+        assert int(got[name]) == ctypes.sizeof(cls), name
+        for field, _ in cls._fields_:
+            assert int(got[f"{name}.{field}"]) == getattr(cls, field).offset, (name, field)

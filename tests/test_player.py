@@ -17,7 +17,7 @@ GOLDEN = json.load(open(os.path.join(HERE, "golden", "golden.json")))
 CASES = cases.cases()
 
 PLAYER_SYMBOLS = ["b2p_frame_create", "b2p_frame_destroy", "b2p_frame_clear", "b2p_frame_render",
-                  "b2p_frame_replay", "b2p_frame_download"]
+                  "b2p_frame_replay", "b2p_frame_download", "b2p_frame_autotune"]
 
 
 def test_player_library_exports_its_entry_points():
