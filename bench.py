@@ -110,6 +110,17 @@ def _render_kernels(mask: int, schedule: int = 0) -> str:
     return "+".join(names) if names else "none"
 
 
+def _tune_times() -> dict:
+    """Render-stage ms of every candidate of the autotuner's latest run (the
+    last line of its log)."""
+    import blend2d_amd as B
+    lines = [l for l in B.autotune_log().splitlines() if l.strip()]
+    if not lines:
+        return {}
+    body = lines[-1].split(":", 1)[1].split(" ms ->")[0].split()
+    return {body[i]: float(body[i + 1]) for i in range(0, len(body) - 1, 2)}
+
+
 def _schedule(st) -> str:
     import blend2d_amd as B
     name = B.SCHEDULE_NAMES.get(int(st.get("schedule", 0)), "?")
@@ -246,13 +257,25 @@ def _cpu_baseline(scenes, first_gpu, gpu_errors, pixels_per_frame, budget_s: flo
     queue = ctx.Queue()
     child = ctx.Process(target=_reference_passes_child, args=(queue, scenes, workers, budget_s / 2))
     child.start()
-    try:
-        results[workers] = queue.get(timeout=budget_s * 4 + 120)
-    except Exception:
-        results[workers] = None
-    child.join(timeout=30)
+    results[workers] = None
+    t_start = time.perf_counter()
+    deadline = t_start + budget_s * 4 + 120
+    next_note = t_start + 30
+    while time.perf_counter() < deadline:
+        try:
+            results[workers] = queue.get(timeout=1.0)
+            break
+        except Exception:
+            if not child.is_alive():   # ended without a result: it crashed
+                break
+        if time.perf_counter() >= next_note:
+            print(f"# cpu baseline: {workers}-thread passes running "
+                  f"({time.perf_counter() - t_start:.0f} s)", file=sys.stderr, flush=True)
+            next_note += 30
+    child.join(timeout=10)
     if child.is_alive():
         child.kill()
+        child.join(timeout=10)
     crashed = results[workers] is None or child.exitcode != 0
     if crashed:
         results.pop(workers)
@@ -296,7 +319,8 @@ def _side_workload(name, seed, steps, learned, threads):
     B.set_sizing(B.SIZING_LEARNED if learned else B.SIZING_EXACT)
     frame.render(player.BATCHED, threads)
     tune = frame.autotune(3)
-    print(f"# {name} autotune: {tune}", file=sys.stderr, flush=True)
+    tune_times = _tune_times()
+    print(f"# {name} autotune: {tune} {tune_times}", file=sys.stderr, flush=True)
     frame.clear()
     frame.render(player.BATCHED, threads)
     el, per, sts, issue = _steps(frame, player.BATCHED, steps, threads)
@@ -327,7 +351,8 @@ def _side_workload(name, seed, steps, learned, threads):
         "fallback": int(st["fallback"]),
         "schedule": _schedule(st),
         "autotune": {"picked": B.SCHEDULE_NAMES[tune["schedule"]], "wgs_per_cu": tune["wgs_per_cu"],
-                     "best_render_ms": round(tune["best_ms"], 4), "grid_render_ms": round(tune["grid_ms"], 4)},
+                     "best_render_ms": round(tune["best_ms"], 4), "grid_render_ms": round(tune["grid_ms"], 4),
+                     "render_ms_by_schedule": tune_times},
     }
     roof = _roofline(scenes, rep_render, rst, name)
     roof["measured_on"] = "device-resident replays, exact sizing"
@@ -407,6 +432,7 @@ def main():
     # tile schedule and records the fastest for frames of its shape, which
     # every later frame of the run then uses (AUTO)
     tune = frame.autotune(3)
+    tune_times = _tune_times()
     print(f"# autotune: {tune}\n# {B.autotune_log().strip()}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
@@ -455,6 +481,7 @@ def main():
         "autotune": {"picked": B.SCHEDULE_NAMES[tune["schedule"]], "wgs_per_cu": tune["wgs_per_cu"],
                      "candidates": tune["candidates"], "best_render_ms": round(tune["best_ms"], 4),
                      "grid_render_ms": round(tune["grid_ms"], 4),
+                     "render_ms_by_schedule": tune_times,
                      "tile_model": tile_params is not None},
         "device_only": {"ms_per_frame": round(float(np.mean(rep_ms)), 3),
                         "mpix_per_s": round(pixels / (float(np.mean(rep_ms)) * 1e-3) / 1e6, 3)},

@@ -758,10 +758,38 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   launch_bin_scatter(f, s, cap_segs, cursor.as<uint32_t>(), stream);
   launch_list_build(f, s, stream);
 
+  // One read-back before the render stage: a pool overflow ends the frame
+  // here (nothing composited; the caller redoes it exactly), otherwise the
+  // tile and tile-group counts size the render grids exactly (upper-bound
+  // grids of a frame with a million tiles spend milliseconds launching
+  // workgroups that exit at once).
+  uint32_t n_atiles = 0, n_fgroups = 0, n_sgroups = 0, n_ggroups = 0;
+  HIP_OK(hipMemcpyAsync(host_readback + 3, s.flags, 4, hipMemcpyDeviceToHost, stream));
+  if (f.n_tiles) {
+    HIP_OK(hipMemcpyAsync(host_readback + 12, s.tile_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(host_readback + 13, s.sgroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(host_readback + 14, s.ggroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(host_readback + 15, s.fgroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+  }
+  HIP_OK(hipStreamSynchronize(stream));
+  stats.syncs++;
+  if (host_readback[3]) {
+    stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats.sizing = mode;
+    return 1;
+  }
+  if (f.n_tiles) {
+    n_atiles = host_readback[12];
+    n_sgroups = host_readback[13];
+    n_ggroups = host_readback[14];
+    n_fgroups = host_readback[15];
+    if (n_atiles > f.n_tiles || (uint64_t)n_fgroups + n_sgroups + n_ggroups > f.n_groups) return kErrDevice;
+  }
+
   launch_render_setup(f, s, stream);
   HIP_OK(hipEventRecord(ev0, stream));
-  if (uint32_t r = render_stage(s, f.n_tiles, f.n_groups, f.n_groups, f.n_groups, true, stats)) return r;
-  stats.render_kernels = f.n_tiles ? 15u : 0u;
+  if (uint32_t r = render_stage(s, n_atiles, n_fgroups, n_sgroups, n_ggroups, true, stats)) return r;
+  stats.render_kernels = (n_atiles ? 1u : 0u) | (n_sgroups ? 2u : 0u) | (n_ggroups ? 4u : 0u) | (n_fgroups ? 8u : 0u);
   HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
   HIP_OK(hipMemcpyAsync(host_readback, counters.ptr, 20, hipMemcpyDeviceToHost, stream));
