@@ -446,12 +446,22 @@ def main():
     config["record_threads"] = threads
     print(f"# batched: {el * 1e3 / args.steps:.3f} ms/frame", file=sys.stderr, flush=True)
 
-    # device-only: the same frame replayed from its resident copy
+    # device-only: the same frame replayed from its resident copy, eagerly
+    # and as a captured HIP graph (the first graph replay sizes exactly and
+    # records the counts, the second captures; both untimed)
     rep_ms, rep_render = [], []
     for _ in range(args.steps):
         ms, rst = frame.replay()
         rep_ms.append(ms)
         rep_render.append(rst["render_ms"])
+    B.set_graph_replay(True)
+    frame.replay()
+    frame.replay()
+    graph_ms = []
+    for _ in range(args.steps):
+        ms, gst = frame.replay()
+        graph_ms.append(ms)
+    B.set_graph_replay(False)
 
     line = {
         "metric": METRIC,
@@ -484,7 +494,10 @@ def main():
                      "render_ms_by_schedule": tune_times,
                      "tile_model": tile_params is not None},
         "device_only": {"ms_per_frame": round(float(np.mean(rep_ms)), 3),
-                        "mpix_per_s": round(pixels / (float(np.mean(rep_ms)) * 1e-3) / 1e6, 3)},
+                        "mpix_per_s": round(pixels / (float(np.mean(rep_ms)) * 1e-3) / 1e6, 3),
+                        "graph_ms_per_frame": round(float(np.mean(graph_ms)), 3),
+                        "graph_launched": bool(gst["graph"]),
+                        "graph_mpix_per_s": round(gst["pixels"] / (float(np.mean(graph_ms)) * 1e-3) / 1e6, 3)},
         "roofline": None,
         "cpu_baseline": None,
     }

@@ -54,7 +54,7 @@ SYMBOLS = [
     "bl_amd_cost_model_forward", "bl_amd_cost_model_grad", "bl_amd_cost_model_train",
     "bl_amd_set_render_schedule", "bl_amd_tile_model_set", "bl_amd_context_collect_tile_costs",
     "bl_amd_batch_collect_tile_costs", "bl_amd_context_autotune", "bl_amd_batch_autotune", "bl_amd_autotune_log",
-    "bl_amd_autotune_reset",
+    "bl_amd_autotune_reset", "bl_amd_set_graph_replay",
 ]
 
 SIZING_EXACT, SIZING_LEARNED, SIZING_HISTORY = 0, 1, 2
@@ -74,10 +74,11 @@ class BLAmdStats(ctypes.Structure):
                 ("render_kernels", ctypes.c_uint32),
                 ("render_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("merge_ms", ctypes.c_double), ("upload_ms", ctypes.c_double), ("pack_ms", ctypes.c_double),
-                ("schedule", ctypes.c_uint32), ("queue_wgs", ctypes.c_uint32)]
+                ("schedule", ctypes.c_uint32), ("queue_wgs", ctypes.c_uint32), ("graph", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class BLAmdTune(ctypes.Structure):
@@ -240,6 +241,7 @@ def bind(L):
         "bl_amd_batch_autotune": ([P, U32, ctypes.POINTER(BLAmdTune)], U32),
         "bl_amd_autotune_log": ([ctypes.c_char_p, SZ], SZ),
         "bl_amd_autotune_reset": ([], U32),
+        "bl_amd_set_graph_replay": ([U32], U32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -687,6 +689,11 @@ def cost_model_train(X, T, params, epochs: int = 10, batch: int = 4096, lr: floa
 # ---------------------------------------------------------------------------
 # Tile scheduling, tile cost model and autotuner
 # ---------------------------------------------------------------------------
+
+def set_graph_replay(on: bool) -> None:
+    """Process-wide: replays run as captured HIP graphs."""
+    _check(lib().bl_amd_set_graph_replay(1 if on else 0), "set_graph_replay")
+
 
 def set_render_schedule(schedule: int, wgs_per_cu: int = 0) -> None:
     """Process-wide schedule of the analytic tiles (SCHEDULE_*)."""

@@ -109,6 +109,7 @@ struct Stats {
   double render_ms = 0.0, total_ms = 0.0;
   double merge_ms = 0.0, upload_ms = 0.0, pack_ms = 0.0;
   uint32_t schedule = BL_AMD_SCHEDULE_GRID, queue_wgs = 0;
+  uint32_t graph = 0;   // 1: a replay launched as a captured HIP graph
 };
 
 // How a frame sizes its segment and bin buffers.
@@ -129,6 +130,8 @@ uint32_t g_schedule = BL_AMD_SCHEDULE_AUTO, g_queue_wgs = 4;
 std::vector<float> g_tile_model;
 uint64_t g_tile_model_version = 0;
 struct TunedSchedule { uint32_t schedule, wgs; };
+// Replays run as captured HIP graphs (bl_amd_set_graph_replay).
+uint32_t g_graph_replay = 0;
 std::map<uint64_t, TunedSchedule> g_tuned;
 std::string g_tune_log;
 
@@ -214,6 +217,28 @@ struct Runner {
   // Tile cost logging: per analytic tile, features and targets.
   bool collect_tiles = false;
   std::vector<float> tile_x, tile_t;
+
+  // Upload generation: bumped by every upload (a new resident frame).
+  uint64_t gen = 0;
+  // Counts of the last exact execution of the resident frame: what a graph
+  // replay bakes in instead of reading them back.
+  struct ExactPlan {
+    bool valid = false;
+    uint64_t gen = 0;
+    uint32_t n_segs = 0, some_failed = 0, n_binned = 0, n_list = 0;
+    uint32_t n_atiles = 0, n_fgroups = 0, n_sgroups = 0, n_ggroups = 0;
+  } plan;
+  // The captured replay and what it was captured with (buffers, frame,
+  // schedule, tile model): any difference captures it again.
+  hipGraphExec_t graph_exec = nullptr;
+  FrameScratch graph_s{};
+  void* graph_q[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint32_t* graph_cursor = nullptr;
+  uint64_t graph_gen = ~0ull, graph_model = ~0ull;
+  uint32_t graph_sched = ~0u, graph_wgs = ~0u;
+  ~Runner() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  }
   // Commands of the last execution that failed on the device: (command
   // index within the frame, kFailInvalidGeometry / kFailTooManySegments bits).
   std::vector<std::pair<uint32_t, uint32_t>> failed;
@@ -267,7 +292,12 @@ struct Runner {
   // `arena` null: the runner's own buffers (grown as needed); otherwise
   // every intermediate buffer is a slice of the arena and nothing is
   // allocated (BL_AMD_ERROR_DATA_TOO_LARGE when the frame does not fit).
-  uint32_t execute_exact(Stats& stats, ScratchArena* arena = nullptr);
+  // `known`: a graph replay -- the launches are captured into a HIP graph
+  // with the counts of `known` in place of the read-backs (or the graph
+  // captured before is launched when nothing changed).
+  uint32_t execute_exact(Stats& stats, ScratchArena* arena = nullptr, const ExactPlan* known = nullptr);
+  // The schedule the analytic tiles of this frame run with.
+  void resolve_schedule(uint32_t n_a, bool allow_queue, uint32_t& sched, uint32_t& wgs) const;
   // 0 done, 1 pool overflow (redo exactly), otherwise an error
   uint32_t execute_pool(Stats& stats, uint32_t mode);
   bool model_views(cm::FrameModel& m);
@@ -592,6 +622,8 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   for (const FrameHost* p : parts)
     for (const DevImage& im : p->images) image_bytes += (uint64_t)im.w * im.h * (im.format == kFmtA8 ? 1u : 4u);
   resident = true;
+  gen++;
+  plan.valid = false;
   return BL_AMD_SUCCESS;
 }
 
@@ -633,6 +665,12 @@ uint32_t Runner::execute(Stats& stats) {
   if (!resident) return BL_AMD_SUCCESS;
   uint32_t mode = g_sizing;
   if (collect || collect_tiles || (mode == kSizeLearned && g_model.empty())) mode = kSizeExact;
+  // Graph replays: the first execution of a resident frame runs exactly and
+  // records its counts; later ones launch the captured graph.
+  if (g_graph_replay && !collect && !collect_tiles) {
+    if (plan.valid && plan.gen == gen) return execute_exact(stats, nullptr, &plan);
+    return execute_exact(stats);
+  }
   if (mode != kSizeExact) {
     uint32_t r = execute_pool(stats, mode);
     if (r != 1) return r;
@@ -820,9 +858,9 @@ uint32_t Runner::execute_pool(Stats& stats, uint32_t mode) {
   return BL_AMD_SUCCESS;
 }
 
-uint32_t Runner::render_stage(FrameScratch& s, uint32_t n_a, uint32_t n_f, uint32_t n_s, uint32_t n_g, bool allow_queue,
-                              Stats& stats) {
-  uint32_t sched = BL_AMD_SCHEDULE_GRID, wgs = 0;
+void Runner::resolve_schedule(uint32_t n_a, bool allow_queue, uint32_t& sched, uint32_t& wgs) const {
+  sched = BL_AMD_SCHEDULE_GRID;
+  wgs = 0;
   if (allow_queue && n_a) {
     if (collect_tiles) { sched = BL_AMD_SCHEDULE_QUEUE; wgs = 4; }
     else if (force_schedule != BL_AMD_SCHEDULE_AUTO) { sched = force_schedule; wgs = force_wgs; }
@@ -834,6 +872,12 @@ uint32_t Runner::render_stage(FrameScratch& s, uint32_t n_a, uint32_t n_f, uint3
   }
   if (sched == BL_AMD_SCHEDULE_QUEUE_LEARNED && g_tile_model.empty()) sched = BL_AMD_SCHEDULE_QUEUE_HEURISTIC;
   if (wgs < 1 || wgs > 4) wgs = 4;
+}
+
+uint32_t Runner::render_stage(FrameScratch& s, uint32_t n_a, uint32_t n_f, uint32_t n_s, uint32_t n_g, bool allow_queue,
+                              Stats& stats) {
+  uint32_t sched, wgs;
+  resolve_schedule(n_a, allow_queue, sched, wgs);
   stats.schedule = sched;
   stats.queue_wgs = sched == BL_AMD_SCHEDULE_GRID ? 0u : wgs;
   if (sched == BL_AMD_SCHEDULE_GRID) {
@@ -872,9 +916,10 @@ uint32_t Runner::render_stage(FrameScratch& s, uint32_t n_a, uint32_t n_f, uint3
 // Exact sizing: counting pass of the edge builder, scan, read-back, emit;
 // bin count, scans, read-back, scatter.  Two host round trips read back the
 // sizes the next stage allocates (segment count, binned count).
-uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
+uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena, const ExactPlan* known) {
   stats = Stats{};
   auto t0 = std::chrono::steady_clock::now();
+  const bool graph = known != nullptr && !arena;
   const uint64_t n_mat = f.n_mat;
   const uint32_t kErrFit = arena ? kErrTooLarge : kErrOOM;
   // A device buffer of `bytes`: the runner's own, grown as needed, or the
@@ -922,6 +967,87 @@ uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
   s.pixels = (unsigned long long*)counters_p;
   s.err_any = counters_p + 4;
 
+  // Graph replay: every buffer is taken now (the counts are known), the
+  // render stage's queue buffers and the tile model are made ready, and the
+  // graph captured before is launched if it was captured with the same
+  // buffers, frame and schedule; otherwise the launches below are captured.
+  uint32_t g_sched = 0, g_wgs = 0;
+  void* q_ptrs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  struct CaptureGuard {
+    hipStream_t st;
+    bool on = false;
+    ~CaptureGuard() {
+      if (!on) return;
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } capture{stream};
+  auto run_graph = [&]() -> uint32_t {
+    HIP_OK(hipEventRecord(ev0, stream));
+    HIP_OK(hipGraphLaunch(graph_exec, stream));
+    HIP_OK(hipEventRecord(ev1, stream));
+    HIP_OK(hipMemcpyAsync(host_readback + 2, s.pixels, 8, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(host_readback + 4, s.err_any, 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    stats.syncs++;
+    if (hipError_t e_ = hipGetLastError(); e_ != hipSuccess) { report_hip("graph launch", e_, __LINE__); return kErrDevice; }
+    if (uint32_t r = read_failures(s, host_readback[4])) return r;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    uint64_t pixels;
+    memcpy(&pixels, host_readback + 2, 8);
+    stats.cmds = f.n_cmds;
+    stats.src = n_src;
+    stats.segs = known->n_segs;
+    stats.binned = known->n_binned;
+    stats.tiles = f.n_tiles;
+    stats.pixels = pixels;
+    stats.frame_bytes = frame_bytes;
+    stats.lut_bytes = lut_bytes;
+    stats.image_bytes = image_bytes;
+    stats.render_ms = ms;   // the whole graph: every device stage
+    stats.render_kernels = (known->n_atiles ? 1u : 0u) | (known->n_sgroups ? 2u : 0u) | (known->n_ggroups ? 4u : 0u) |
+                           (known->n_fgroups ? 8u : 0u);
+    stats.schedule = graph_sched;
+    stats.queue_wgs = graph_sched == BL_AMD_SCHEDULE_GRID ? 0u : graph_wgs;
+    stats.graph = 1;
+    stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return BL_AMD_SUCCESS;
+  };
+  if (graph) {
+    s.segs = (DevSeg*)take(segs, (size_t)std::max<uint32_t>(known->n_segs, 1) * sizeof(DevSeg));
+    s.seg_cmd = (uint32_t*)take(seg_cmd, (size_t)std::max<uint32_t>(known->n_segs, 1) * 4);
+    s.binned = (DevSeg*)take(binned, (size_t)std::max<uint32_t>(known->n_binned, 1) * sizeof(DevSeg));
+    s.list_ent = (uint4*)take(list, (size_t)std::max<uint32_t>(known->n_list, 1) * 16);
+    s.list_style = (uint32_t*)take(list_style, (size_t)std::max<uint32_t>(known->n_list, 1) * 4);
+    if (!s.segs || !s.seg_cmd || !s.binned || !s.list_ent || !s.list_style) return kErrOOM;
+    resolve_schedule(known->n_atiles, true, g_sched, g_wgs);
+    if (g_sched != BL_AMD_SCHEDULE_GRID) {
+      const size_t n_a = known->n_atiles;
+      if (!q_misc.reserve(1024 + 64) || !q_queue.reserve(n_a * 4) || !q_key.reserve(n_a * 4) || !q_feat.reserve(n_a * 64) ||
+          !q_pred.reserve(n_a * 8) || !tm_params.reserve(cm::kParams * 4))
+        return kErrOOM;
+      if (g_sched == BL_AMD_SCHEDULE_QUEUE_LEARNED && tm_version != g_tile_model_version) {
+        HIP_OK(hipMemcpyAsync(tm_params.ptr, g_tile_model.data(), cm::kParams * 4, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        tm_version = g_tile_model_version;
+      }
+      q_ptrs[0] = q_misc.ptr; q_ptrs[1] = q_queue.ptr; q_ptrs[2] = q_key.ptr;
+      q_ptrs[3] = q_feat.ptr; q_ptrs[4] = q_pred.ptr; q_ptrs[5] = tm_params.ptr;
+    }
+    const bool same = graph_exec && graph_gen == gen && graph_sched == g_sched && graph_wgs == g_wgs &&
+                      graph_model == g_tile_model_version && graph_cursor == cursor_p &&
+                      memcmp(&graph_s, &s, sizeof(s)) == 0 && memcmp(graph_q, q_ptrs, sizeof(q_ptrs)) == 0;
+    if (same) return run_graph();
+    if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+    graph_s = s;
+    graph_cursor = cursor_p;
+    memcpy(graph_q, q_ptrs, sizeof(q_ptrs));
+    HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+    capture.on = true;
+  }
+
   cm::FrameModel model{};
   if (collect && !arena) {
     if (!model_views(model)) return kErrOOM;
@@ -937,20 +1063,28 @@ uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
   launch_cmd_tiles(f, s, stream);
   // A frame of box fills only has no edges: no counting round trip.
   uint32_t n_segs = 0;
-  if (n_src) {
-    scan_exclusive(s.src_count, s.src_off, item_cap, s.scan_tmp, stream);
-    HIP_OK(hipMemcpyAsync(host_readback, s.src_off + item_cap, 4, hipMemcpyDeviceToHost, stream));
-    HIP_OK(hipMemcpyAsync(host_readback + 5, s.err_any, 4, hipMemcpyDeviceToHost, stream));
-    HIP_OK(hipStreamSynchronize(stream));
-    stats.syncs++;
-    n_segs = host_readback[0];
+  bool some_failed = false;
+  if (graph) {
+    if (n_src) scan_exclusive(s.src_count, s.src_off, item_cap, s.scan_tmp, stream);
+    n_segs = known->n_segs;
+    some_failed = known->some_failed != 0;
   }
-  // segments of failed commands are not binned
-  const bool some_failed = n_src && host_readback[5];
-  if (n_segs >= 0x80000000u) return kErrDevice;
-  s.segs = (DevSeg*)take(segs, (size_t)std::max<uint32_t>(n_segs, 1) * sizeof(DevSeg));
-  s.seg_cmd = (uint32_t*)take(seg_cmd, (size_t)std::max<uint32_t>(n_segs, 1) * 4);
-  if (!s.segs || !s.seg_cmd) return kErrFit;
+  else {
+    if (n_src) {
+      scan_exclusive(s.src_count, s.src_off, item_cap, s.scan_tmp, stream);
+      HIP_OK(hipMemcpyAsync(host_readback, s.src_off + item_cap, 4, hipMemcpyDeviceToHost, stream));
+      HIP_OK(hipMemcpyAsync(host_readback + 5, s.err_any, 4, hipMemcpyDeviceToHost, stream));
+      HIP_OK(hipStreamSynchronize(stream));
+      stats.syncs++;
+      n_segs = host_readback[0];
+    }
+    // segments of failed commands are not binned
+    some_failed = n_src && host_readback[5];
+    if (n_segs >= 0x80000000u) return kErrDevice;
+    s.segs = (DevSeg*)take(segs, (size_t)std::max<uint32_t>(n_segs, 1) * sizeof(DevSeg));
+    s.seg_cmd = (uint32_t*)take(seg_cmd, (size_t)std::max<uint32_t>(n_segs, 1) * 4);
+    if (!s.segs || !s.seg_cmd) return kErrFit;
+  }
   launch_edges(f, s, false, item_cap, stream);
 
   // Stage 2: binning.
@@ -960,41 +1094,67 @@ uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
   scan_exclusive(s.mat_cnt, s.mat_off, n_mat, s.scan_tmp, stream);
   scan_exclusive(s.mat_pres, s.list_off, n_mat, s.scan_tmp, stream);
   launch_band_class(f, s, stream);
-  HIP_OK(hipMemcpyAsync(host_readback, s.mat_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
-  HIP_OK(hipMemcpyAsync(host_readback + 1, s.list_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
-  if (f.n_tiles) {
-    HIP_OK(hipMemcpyAsync(host_readback + 12, s.tile_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
-    HIP_OK(hipMemcpyAsync(host_readback + 13, s.sgroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
-    HIP_OK(hipMemcpyAsync(host_readback + 14, s.ggroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
-    HIP_OK(hipMemcpyAsync(host_readback + 15, s.fgroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+  uint32_t n_binned, n_list, n_atiles, n_sgroups, n_ggroups, n_fgroups;
+  if (graph) {
+    n_binned = known->n_binned;
+    n_list = known->n_list;
+    n_atiles = known->n_atiles;
+    n_sgroups = known->n_sgroups;
+    n_ggroups = known->n_ggroups;
+    n_fgroups = known->n_fgroups;
   }
-  HIP_OK(hipStreamSynchronize(stream));
-  stats.syncs++;
-  const uint32_t n_binned = host_readback[0], n_list = host_readback[1];
-  const uint32_t n_atiles = f.n_tiles ? host_readback[12] : 0u;
-  const uint32_t n_sgroups = f.n_tiles ? host_readback[13] : 0u, n_ggroups = f.n_tiles ? host_readback[14] : 0u;
-  const uint32_t n_fgroups = f.n_tiles ? host_readback[15] : 0u;
-  if (n_atiles > f.n_tiles || (uint64_t)n_fgroups + n_sgroups + n_ggroups > f.n_groups) return kErrDevice;
-  // every edge segment crosses at least one band and at most all of them;
-  // the command lists hold at most one entry per (band, command) cell
-  if ((n_binned < n_segs && !some_failed) || (uint64_t)n_binned > (uint64_t)n_segs * std::max<uint32_t>(max_bands(), 1) ||
-      n_list > n_mat)
-    return kErrDevice;
-  s.binned = (DevSeg*)take(binned, (size_t)std::max<uint32_t>(n_binned, 1) * sizeof(DevSeg));
-  s.list_ent = (uint4*)take(list, (size_t)std::max<uint32_t>(n_list, 1) * 16);
-  s.list_style = (uint32_t*)take(list_style, (size_t)std::max<uint32_t>(n_list, 1) * 4);
-  if (!s.binned || !s.list_ent || !s.list_style) return kErrFit;
-  HIP_OK(hipMemcpyAsync(cursor_p, s.mat_off, (n_mat + 1) * 4, hipMemcpyDeviceToDevice, stream));
+  else {
+    HIP_OK(hipMemcpyAsync(host_readback, s.mat_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(host_readback + 1, s.list_off + n_mat, 4, hipMemcpyDeviceToHost, stream));
+    if (f.n_tiles) {
+      HIP_OK(hipMemcpyAsync(host_readback + 12, s.tile_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+      HIP_OK(hipMemcpyAsync(host_readback + 13, s.sgroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+      HIP_OK(hipMemcpyAsync(host_readback + 14, s.ggroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+      HIP_OK(hipMemcpyAsync(host_readback + 15, s.fgroup_off + f.n_bands, 4, hipMemcpyDeviceToHost, stream));
+    }
+    HIP_OK(hipStreamSynchronize(stream));
+    stats.syncs++;
+    n_binned = host_readback[0];
+    n_list = host_readback[1];
+    n_atiles = f.n_tiles ? host_readback[12] : 0u;
+    n_sgroups = f.n_tiles ? host_readback[13] : 0u;
+    n_ggroups = f.n_tiles ? host_readback[14] : 0u;
+    n_fgroups = f.n_tiles ? host_readback[15] : 0u;
+    if (n_atiles > f.n_tiles || (uint64_t)n_fgroups + n_sgroups + n_ggroups > f.n_groups) return kErrDevice;
+    // every edge segment crosses at least one band and at most all of them;
+    // the command lists hold at most one entry per (band, command) cell
+    if ((n_binned < n_segs && !some_failed) || (uint64_t)n_binned > (uint64_t)n_segs * std::max<uint32_t>(max_bands(), 1) ||
+        n_list > n_mat)
+      return kErrDevice;
+    s.binned = (DevSeg*)take(binned, (size_t)std::max<uint32_t>(n_binned, 1) * sizeof(DevSeg));
+    s.list_ent = (uint4*)take(list, (size_t)std::max<uint32_t>(n_list, 1) * 16);
+    s.list_style = (uint32_t*)take(list_style, (size_t)std::max<uint32_t>(n_list, 1) * 4);
+    if (!s.binned || !s.list_ent || !s.list_style) return kErrFit;
+  }
+  launch_copy_u32(cursor_p, s.mat_off, n_mat + 1, stream);
   launch_bin_scatter(f, s, n_segs, cursor_p, stream);
   launch_list_build(f, s, stream);
 
   // Stage 3: render.
   launch_render_setup(f, s, stream);
-  HIP_OK(hipEventRecord(ev0, stream));
+  if (!graph) HIP_OK(hipEventRecord(ev0, stream));
   if (uint32_t r = render_stage(s, n_atiles, n_fgroups, n_sgroups, n_ggroups, !arena, stats)) return r;
   stats.render_kernels = (n_atiles ? 1u : 0u) | (n_sgroups ? 2u : 0u) | (n_ggroups ? 4u : 0u) | (n_fgroups ? 8u : 0u);
-  HIP_OK(hipEventRecord(ev1, stream));
+  if (!graph) HIP_OK(hipEventRecord(ev1, stream));
   launch_render_stats(f, s, stream);
+  if (graph) {
+    hipGraph_t g = nullptr;
+    capture.on = false;
+    HIP_OK(hipStreamEndCapture(stream, &g));
+    const hipError_t e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { graph_exec = nullptr; report_hip("hipGraphInstantiate", e, __LINE__); return kErrDevice; }
+    graph_gen = gen;
+    graph_sched = g_sched;
+    graph_wgs = g_wgs;
+    graph_model = g_tile_model_version;
+    return run_graph();
+  }
   HIP_OK(hipMemcpyAsync(host_readback + 2, s.pixels, 8, hipMemcpyDeviceToHost, stream));
   HIP_OK(hipMemcpyAsync(host_readback + 4, s.err_any, 4, hipMemcpyDeviceToHost, stream));
   if (collect && !arena) {
@@ -1038,6 +1198,18 @@ uint32_t Runner::execute_exact(Stats& stats, ScratchArena* arena) {
   }
   hist_segs = n_segs;
   hist_bins = n_binned;
+  if (!arena) {
+    plan.valid = true;
+    plan.gen = gen;
+    plan.n_segs = n_segs;
+    plan.some_failed = some_failed ? 1u : 0u;
+    plan.n_binned = n_binned;
+    plan.n_list = n_list;
+    plan.n_atiles = n_atiles;
+    plan.n_fgroups = n_fgroups;
+    plan.n_sgroups = n_sgroups;
+    plan.n_ggroups = n_ggroups;
+  }
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, ev0, ev1);
   auto t1 = std::chrono::steady_clock::now();
@@ -2293,6 +2465,7 @@ static void copy_stats(const Stats& st, BLAmdStats* out) {
   out->total_ms = st.total_ms;
   out->schedule = st.schedule;
   out->queue_wgs = st.queue_wgs;
+  out->graph = st.graph;
 }
 
 BLAmdResult bl_amd_context_get_error_flags(const BLAmdContext* ctx, uint32_t* flags) {
@@ -2414,6 +2587,12 @@ BLAmdResult bl_amd_cost_model_set(const float* params) {
 // ---------------------------------------------------------------------------
 // Tile scheduling, tile cost model, autotuner
 // ---------------------------------------------------------------------------
+
+BLAmdResult bl_amd_set_graph_replay(uint32_t on) {
+  std::lock_guard<std::mutex> lock(device_queue().mu);
+  g_graph_replay = on ? 1u : 0u;
+  return BL_AMD_SUCCESS;
+}
 
 BLAmdResult bl_amd_set_render_schedule(uint32_t schedule, uint32_t wgs_per_cu) {
   if (schedule > BL_AMD_SCHEDULE_AUTO || wgs_per_cu > 4) return kErrInvalid;

@@ -128,6 +128,8 @@ void launch_tile_queue(const FrameDev& f, const FrameScratch& s, uint32_t n_a, c
 void launch_render_queue(const FrameDev& f, const FrameScratch& s, uint32_t grid, uint32_t wgs_per_cu, hipStream_t st);
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+// Device-to-device copy of n words by a kernel (a graph node like any other).
+void launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t st);
 // Fills the gradient tables of `n_jobs` jobs (one workgroup per table).
 void launch_gradient_luts(const lut::Job* jobs, const lut::Stop* stops, uint32_t n_jobs, uint32_t* luts, hipStream_t st);
 

@@ -165,6 +165,16 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
 }
 
+__global__ void k_copy_u32(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+void launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_copy_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n);
+}
+
 // One gradient table per workgroup, every entry computed on its own
 // (lut_device.h).  The stops of a job were validated and sorted on the
 // host (offsets in [0, 1], n >= 1), so every position lies in [0, size).

@@ -263,6 +263,8 @@ typedef struct BLAmdStats {
   double pack_ms;      /* flush: the part of upload_ms spent packing the frame into staging memory */
   uint32_t schedule;   /* BL_AMD_SCHEDULE_* the analytic tiles ran with (AUTO resolved) */
   uint32_t queue_wgs;  /* queue schedules: workgroups per CU of the persistent grid (0 for GRID) */
+  uint32_t graph;      /* 1: a replay launched as a captured HIP graph (render_ms then covers every device stage) */
+  uint32_t reserved;
 } BLAmdStats;
 
 BLAmdResult bl_amd_context_get_stats(const BLAmdContext* ctx, BLAmdStats* out);
@@ -272,6 +274,17 @@ BLAmdResult bl_amd_context_get_stats(const BLAmdContext* ctx, BLAmdStats* out);
  * display list.  The target and every pattern image the frame used must
  * still be alive; the frame is replaced by the next flush that renders. */
 BLAmdResult bl_amd_context_replay(BLAmdContext* ctx);
+
+/* No counterpart in the reference.  With graph replay on (process-wide,
+ * default off), bl_amd_context_replay and bl_amd_batch_replay run the
+ * resident frame's device stages as one captured HIP graph: flushes then
+ * size exactly and record the frame's counts (segments, binned segments,
+ * list entries, tiles and tile groups), the first replay captures every
+ * launch with those counts in place of the read-backs, and it and later
+ * replays launch the captured graph (one launch, one synchronization).  A new
+ * upload, a different schedule or tile model, or a grown buffer captures it
+ * again.  Pixels are those of the eager replay. */
+BLAmdResult bl_amd_set_graph_replay(uint32_t on);
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -302,6 +302,7 @@ BLAmdResult bl_amd_context_autotune(BLAmdContext*, uint32_t, BLAmdTune* out) { i
 BLAmdResult bl_amd_batch_autotune(BLAmdBatch*, uint32_t, BLAmdTune* out) { if (out) std::memset(out, 0, sizeof(*out)); return 0; }
 size_t bl_amd_autotune_log(char* buf, size_t cap) { if (buf && cap) buf[0] = 0; return 0; }
 BLAmdResult bl_amd_autotune_reset(void) { return 0; }
+BLAmdResult bl_amd_set_graph_replay(uint32_t) { return 0; }
 
 }  // extern "C"
 
