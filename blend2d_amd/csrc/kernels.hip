@@ -1413,25 +1413,33 @@ __global__ __launch_bounds__(256) void k_tile_features(FrameDev f, FrameScratch 
 // Queue key of each analytic tile: its predicted cost (log2 ticks from the
 // model's first output when `pred` is set, else log2 of binned segments +
 // 16 x commands from the features) in 1/8 steps, as a bucket of a
-// descending counting sort; bucket sizes into `hist` (256 entries, zeroed).
+// descending counting sort; bucket sizes into `hist` (256 entries, zeroed),
+// counted per workgroup in LDS first (most tiles of a frame fall into a few
+// buckets, so per-tile global atomics would queue on them).
 constexpr uint32_t kQueueBuckets = 256;
-__global__ void k_tile_keys(FrameDev f, FrameScratch s, uint32_t n, const float* __restrict__ pred) {
+__global__ __launch_bounds__(kQueueBuckets) void k_tile_keys(FrameDev f, FrameScratch s, uint32_t n, const float* __restrict__ pred) {
+  __shared__ uint32_t lhist[kQueueBuckets];
+  lhist[threadIdx.x] = 0u;
+  __syncthreads();
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= n || slot >= s.tile_off[f.n_bands]) return;
-  float cost;
-  if (pred) {
-    cost = pred[(size_t)slot * 2];
+  if (slot < n && slot < s.tile_off[f.n_bands]) {
+    float cost;
+    if (pred) {
+      cost = pred[(size_t)slot * 2];
+    }
+    else {
+      const float* x = s.tile_feat + (size_t)slot * 16;
+      cost = log2f(1.0f + (exp2f(x[1]) - 1.0f) + 16.0f * (exp2f(x[0]) - 1.0f));
+    }
+    float q = cost * 8.0f;
+    q = q != q ? 0.0f : q;   // a NaN prediction sorts last
+    const uint32_t key = q <= 0.0f ? 0u : q >= (float)(kQueueBuckets - 1) ? kQueueBuckets - 1 : (uint32_t)q;
+    const uint32_t bucket = kQueueBuckets - 1 - key;
+    s.tile_key[slot] = bucket;
+    atomicAdd(&lhist[bucket], 1u);
   }
-  else {
-    const float* x = s.tile_feat + (size_t)slot * 16;
-    cost = log2f(1.0f + (exp2f(x[1]) - 1.0f) + 16.0f * (exp2f(x[0]) - 1.0f));
-  }
-  float q = cost * 8.0f;
-  q = q != q ? 0.0f : q;   // a NaN prediction sorts last
-  const uint32_t key = q <= 0.0f ? 0u : q >= (float)(kQueueBuckets - 1) ? kQueueBuckets - 1 : (uint32_t)q;
-  const uint32_t bucket = kQueueBuckets - 1 - key;
-  s.tile_key[slot] = bucket;
-  atomicAdd(&s.queue_hist[bucket], 1u);
+  __syncthreads();
+  if (lhist[threadIdx.x]) atomicAdd(&s.queue_hist[threadIdx.x], lhist[threadIdx.x]);
 }
 
 // Exclusive scan of the bucket sizes in place (one workgroup), and the queue
@@ -1451,11 +1459,24 @@ __global__ __launch_bounds__(kQueueBuckets) void k_queue_scan(FrameScratch s) {
   if (t == 0) *s.queue_head = 0u;
 }
 
-// Queue positions: within a bucket in any order (tiles are independent).
-__global__ void k_tile_scatter(FrameDev f, FrameScratch s, uint32_t n) {
+// Queue positions: each workgroup takes one range per bucket from the
+// bucket cursors and numbers its tiles within it in LDS (within a bucket
+// the order is any: tiles are independent).
+__global__ __launch_bounds__(kQueueBuckets) void k_tile_scatter(FrameDev f, FrameScratch s, uint32_t n) {
+  __shared__ uint32_t lcount[kQueueBuckets], lbase[kQueueBuckets];
+  lcount[threadIdx.x] = 0u;
+  __syncthreads();
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= n || slot >= s.tile_off[f.n_bands]) return;
-  s.queue[atomicAdd(&s.queue_hist[s.tile_key[slot]], 1u)] = slot;
+  const bool on = slot < n && slot < s.tile_off[f.n_bands];
+  uint32_t bucket = 0, rank = 0;
+  if (on) {
+    bucket = s.tile_key[slot];
+    rank = atomicAdd(&lcount[bucket], 1u);
+  }
+  __syncthreads();
+  if (lcount[threadIdx.x]) lbase[threadIdx.x] = atomicAdd(&s.queue_hist[threadIdx.x], lcount[threadIdx.x]);
+  __syncthreads();
+  if (on) s.queue[lbase[bucket] + rank] = slot;
 }
 
 // Canvas of a global band: the last one whose first band is <= b.
@@ -1911,9 +1932,9 @@ void launch_tile_features(const FrameDev& f, const FrameScratch& s, uint32_t n_a
 void launch_tile_queue(const FrameDev& f, const FrameScratch& s, uint32_t n_a, const float* pred, hipStream_t st) {
   if (!f.n_tiles || !n_a) return;
   launch_fill_u32(s.queue_hist, 0u, kQueueBuckets, st);
-  hipLaunchKernelGGL(k_tile_keys, dim3((n_a + 255) / 256), dim3(256), 0, st, f, s, n_a, pred);
+  hipLaunchKernelGGL(k_tile_keys, dim3((n_a + kQueueBuckets - 1) / kQueueBuckets), dim3(kQueueBuckets), 0, st, f, s, n_a, pred);
   hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(kQueueBuckets), 0, st, s);
-  hipLaunchKernelGGL(k_tile_scatter, dim3((n_a + 255) / 256), dim3(256), 0, st, f, s, n_a);
+  hipLaunchKernelGGL(k_tile_scatter, dim3((n_a + kQueueBuckets - 1) / kQueueBuckets), dim3(kQueueBuckets), 0, st, f, s, n_a);
 }
 
 void launch_render_queue(const FrameDev& f, const FrameScratch& s, uint32_t grid, uint32_t wgs_per_cu, hipStream_t st) {

@@ -70,6 +70,9 @@ int main(int argc, char** argv) {
       x1 = std::min(kW * 256, std::max(0, x0 + g.uni(-12, 12)));
     }
     else x1 = g.coord(kW * 256);
+    // some end points exactly on cell boundaries
+    if (g.uni(0, 3) == 0) x0 &= ~255;
+    if (g.uni(0, 3) == 0) x1 &= ~255;
     if (y0 == y1) continue;
     if (y0 > y1) { std::swap(y0, y1); std::swap(x0, x1); }
     const uint32_t sign = (uint32_t)g.uni(0, 1);
