@@ -55,6 +55,8 @@ SYMBOLS = [
     "bl_amd_set_render_schedule", "bl_amd_tile_model_set", "bl_amd_context_collect_tile_costs",
     "bl_amd_batch_collect_tile_costs", "bl_amd_context_autotune", "bl_amd_batch_autotune", "bl_amd_autotune_log",
     "bl_amd_autotune_reset", "bl_amd_set_graph_replay",
+    "bl_amd_context_blit_image_i", "bl_amd_context_blit_image_d", "bl_amd_context_blit_scaled_image_i",
+    "bl_amd_context_blit_scaled_image_d",
 ]
 
 SIZING_EXACT, SIZING_LEARNED, SIZING_HISTORY = 0, 1, 2
@@ -242,6 +244,10 @@ def bind(L):
         "bl_amd_autotune_log": ([ctypes.c_char_p, SZ], SZ),
         "bl_amd_autotune_reset": ([], U32),
         "bl_amd_set_graph_replay": ([U32], U32),
+        "bl_amd_context_blit_image_i": ([P, I, I, P, P], U32),
+        "bl_amd_context_blit_image_d": ([P, D, D, P, P], U32),
+        "bl_amd_context_blit_scaled_image_i": ([P, I, I, I, I, P, P], U32),
+        "bl_amd_context_blit_scaled_image_d": ([P, D, D, D, D, P, P], U32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -431,6 +437,26 @@ class BLContext:
 
     def clear_all(self):
         _check(lib().bl_amd_context_clear_all(self._h), "clear_all")
+
+    def blit_image(self, dst, image: "BLImage", area=None, integer: bool = False):
+        """BLContext::blit_image (core/context.h:3860-3900): `dst` is an origin
+        (x, y) or a rectangle (x, y, w, h) the area is scaled onto; `area`
+        (x, y, w, h) selects part of the image; `integer` takes the _i form."""
+        self._keep.append(image)
+        a = None
+        if area is not None:
+            a = (ctypes.c_int32 * 4)(*[int(v) for v in area])
+        L = lib()
+        if len(dst) == 2:
+            if integer:
+                r = L.bl_amd_context_blit_image_i(self._h, int(dst[0]), int(dst[1]), image.handle, a)
+            else:
+                r = L.bl_amd_context_blit_image_d(self._h, float(dst[0]), float(dst[1]), image.handle, a)
+        elif integer:
+            r = L.bl_amd_context_blit_scaled_image_i(self._h, *[int(v) for v in dst], image.handle, a)
+        else:
+            r = L.bl_amd_context_blit_scaled_image_d(self._h, *[float(v) for v in dst], image.handle, a)
+        _check(r, "blit_image")
 
     def clear_rect(self, x, y, w, h):
         if all(isinstance(v, (int, np.integer)) for v in (x, y, w, h)):

@@ -16,7 +16,8 @@
 // the context style, an explicit rgba32 / rgba64 colour or an explicit style
 // object), clear_*, the fill style slot (colours, BLGradient, BLPattern),
 // fill alpha, global alpha, comp op, fill rule, transforms and the pattern
-// quality hint.  Every other entry (stroking, text, masks, blits, clipping,
+// quality hint, and image blits (blit_image / blit_scaled_image).  Every
+// other entry (stroking, text, masks, clipping,
 // save / restore) reports BL_ERROR_NOT_IMPLEMENTED.
 
 #include <blend2d/blend2d.h>
@@ -278,6 +279,38 @@ BLResult BL_CDECL set_fill_rule_impl(BLContextImpl* impl, BLFillRule rule) noexc
   return BL_SUCCESS;
 }
 
+// Blits: the image (uploaded once per context) and its area as x, y, w, h.
+template<typename Fn>
+BLResult with_image(BLContextImpl* impl, const BLImageCore* img, const BLRectI* area, Fn&& fn) {
+  AmdContextImpl* self = self_of(impl);
+  BLAmdImage* dev = device_image(self, *static_cast<const BLImage*>(img));
+  if (!dev) return BL_ERROR_INVALID_VALUE;
+  int32_t a[4];
+  if (area) { a[0] = area->x; a[1] = area->y; a[2] = area->w; a[3] = area->h; }
+  return result_of(fn(self->gpu, dev, area ? a : nullptr));
+}
+
+BLResult BL_CDECL blit_image_i_impl(BLContextImpl* impl, const BLPointI* o, const BLImageCore* img, const BLRectI* area) noexcept {
+  return with_image(impl, img, area, [o](BLAmdContext* g, BLAmdImage* d, const int32_t* a) {
+    return bl_amd_context_blit_image_i(g, o->x, o->y, d, a);
+  });
+}
+BLResult BL_CDECL blit_image_d_impl(BLContextImpl* impl, const BLPoint* o, const BLImageCore* img, const BLRectI* area) noexcept {
+  return with_image(impl, img, area, [o](BLAmdContext* g, BLAmdImage* d, const int32_t* a) {
+    return bl_amd_context_blit_image_d(g, o->x, o->y, d, a);
+  });
+}
+BLResult BL_CDECL blit_scaled_image_i_impl(BLContextImpl* impl, const BLRectI* r, const BLImageCore* img, const BLRectI* area) noexcept {
+  return with_image(impl, img, area, [r](BLAmdContext* g, BLAmdImage* d, const int32_t* a) {
+    return bl_amd_context_blit_scaled_image_i(g, r->x, r->y, r->w, r->h, d, a);
+  });
+}
+BLResult BL_CDECL blit_scaled_image_d_impl(BLContextImpl* impl, const BLRect* r, const BLImageCore* img, const BLRectI* area) noexcept {
+  return with_image(impl, img, area, [r](BLAmdContext* g, BLAmdImage* d, const int32_t* a) {
+    return bl_amd_context_blit_scaled_image_d(g, r->x, r->y, r->w, r->h, d, a);
+  });
+}
+
 BLResult BL_CDECL clear_all_impl(BLContextImpl* impl) noexcept { return result_of(bl_amd_context_clear_all(self_of(impl)->gpu)); }
 BLResult BL_CDECL clear_recti_impl(BLContextImpl* impl, const BLRectI* r) noexcept {
   return result_of(bl_amd_context_clear_rect_i(self_of(impl)->gpu, r->x, r->y, r->w, r->h));
@@ -390,8 +423,8 @@ BLContextVirt make_virt() {
   v.fill_path_d = fill_path_d_impl;
   v.fill_path_d_rgba32 = fill_path_d_rgba32_impl;
   v.fill_path_d_ext = fill_path_d_ext_impl;
-  not_implemented(v.blit_image_i);
-  not_implemented(v.blit_scaled_image_i);
+  v.blit_image_i = blit_image_i_impl;
+  v.blit_scaled_image_i = blit_scaled_image_i_impl;
 
   v.flush = flush_impl;
   not_implemented(v.save);
@@ -458,8 +491,8 @@ BLContextVirt make_virt() {
   not_implemented(v.stroke_text_op_d);
   not_implemented(v.stroke_text_op_d_rgba32);
   not_implemented(v.stroke_text_op_d_ext);
-  not_implemented(v.blit_image_d);
-  not_implemented(v.blit_scaled_image_d);
+  v.blit_image_d = blit_image_d_impl;
+  v.blit_scaled_image_d = blit_scaled_image_d_impl;
   return v;
 }
 

@@ -1293,6 +1293,9 @@ struct StyleState {
   std::vector<Stop> stops;
   uint32_t lut_size = 0;
   ImageRef image;
+  // blits: the pattern samples the rectangle view[0..3] = x, y, w, h of the image
+  bool has_view = false;
+  int32_t view[4] = {0, 0, 0, 0};
   uint64_t version = 0;
 };
 
@@ -1380,6 +1383,25 @@ uint32_t image_index(BLAmdContext* c, const BLAmdImage* img) {
   return (uint32_t)(c->frame_images.size() - 1);
 }
 
+// The rectangle x, y, w, h of `img` as an image of its own (a blit's source
+// area, rastercontext.cpp:3944-4017: init_image_source).
+uint32_t image_view_index(BLAmdContext* c, const BLAmdImage* img, const int32_t v[4]) {
+  const int64_t bpp = img->format == kFmtA8 ? 1 : 4;
+  DevImage d{};
+  d.ptr = (uint64_t)(uintptr_t)img->dptr + (uint64_t)((int64_t)v[1] * img->stride + (int64_t)v[0] * bpp);
+  d.stride = img->stride;
+  d.w = (uint32_t)v[2];
+  d.h = (uint32_t)v[3];
+  d.format = img->format;
+  for (size_t i = 0; i < c->frame.images.size(); i++) {
+    const DevImage& e = c->frame.images[i];
+    if (c->frame_images[i].get() == img && e.ptr == d.ptr && e.w == d.w && e.h == d.h) return (uint32_t)i;
+  }
+  c->frame_images.push_back(ImageRef(img));
+  c->frame.images.push_back(d);
+  return (uint32_t)(c->frame_images.size() - 1);
+}
+
 // A resolved render call: the command template and the error the call
 // reports if its geometry turns out to be non-empty (the reference looks the
 // pipeline up only then, rastercontext.cpp:2734-2749).
@@ -1387,6 +1409,7 @@ struct Resolved {
   DevCmd cmd;
   uint32_t nop = 0;       // 1: the call renders nothing
   uint32_t status = 0;    // deferred error
+  uint32_t overridden = 0;   // the comp op replaced the style by a solid colour
 };
 
 uint32_t exec_op(uint32_t comp) {
@@ -1430,6 +1453,7 @@ Resolved resolve_with(BLAmdContext* c, StyleState& st, uint32_t alpha, bool nop_
     d.solid = s.solid == kSolidTransparent ? 0u : s.solid == kSolidOpaqueBlack ? 0xFF000000u : 0xFFFFFFFFu;
     cmd.style = (uint32_t)c->frame.styles.size();
     c->frame.styles.push_back(d);
+    r.overridden = 1;
     return r;
   }
   if (c->frame_style_version != st.version) {
@@ -1442,7 +1466,7 @@ Resolved resolve_with(BLAmdContext* c, StyleState& st, uint32_t alpha, bool nop_
       fr.lut_stops.insert(fr.lut_stops.end(), st.stops.begin(), st.stops.end());
       fr.lut_entries += st.lut_size;
     }
-    if (st.type == kStylePattern) d.img = image_index(c, st.image.get());
+    if (st.type == kStylePattern) d.img = st.has_view ? image_view_index(c, st.image.get(), st.view) : image_index(c, st.image.get());
     c->frame_style_index = (uint32_t)c->frame.styles.size();
     c->frame.styles.push_back(d);
     c->frame_style_version = st.version;
@@ -1682,23 +1706,27 @@ uint32_t fill_box_d(BLAmdContext* c, const Resolved& r, double bx0, double by0, 
   return fill_poly_rule(c, r, poly, 4, 1);
 }
 
+// ContextFlags::kInfoIntegralTranslation (rastercontext.cpp:773-789): the user
+// transform is a translation by whole pixels within the safe fixed-point range
+// (fp_min/max_safe_coord_d, rastercontext.cpp:4357-4358).
+bool integral_translation(const BLAmdContext* c, int& tx, int& ty) {
+  if (transform_type(c->user) > kTTTranslate) return false;
+  double fx = c->user.m20 * 256.0, fy = c->user.m21 * 256.0;
+  const double lo = floor((double)(INT32_MIN + 1) * 256.0);
+  const double hi = floor((double)(INT32_MAX - 1 - std::max(c->target->w, c->target->h)) * 256.0);
+  if (!(fx >= lo && fx <= hi && fy >= lo && fy <= hi)) return false;
+  int64_t tx64 = floor_to_int64(fx), ty64 = floor_to_int64(fy);
+  if (((tx64 | ty64) & 0xFF) != 0) return false;
+  tx = (int)(tx64 >> 8);
+  ty = (int)(ty64 >> 8);
+  return true;
+}
+
 // fillUnclippedRectI (rastercontext.cpp:2898-2917)
 uint32_t fill_rect_i(BLAmdContext* c, const Resolved& r, int x, int y, int w, int h) {
   if (r.nop) return BL_AMD_SUCCESS;
-  int tt = transform_type(c->user);
-  bool integral = false;
   int tx = 0, ty = 0;
-  if (tt <= kTTTranslate) {
-    double fx = c->user.m20 * 256.0, fy = c->user.m21 * 256.0;
-    // rastercontext.cpp:773-789: only translations within the safe fixed-point
-    // range (fp_min/max_safe_coord_d, rastercontext.cpp:4357-4358)
-    const double lo = floor((double)(INT32_MIN + 1) * 256.0);
-    const double hi = floor((double)(INT32_MAX - 1 - std::max(c->target->w, c->target->h)) * 256.0);
-    if (fx >= lo && fx <= hi && fy >= lo && fy <= hi) {
-      int64_t tx64 = floor_to_int64(fx), ty64 = floor_to_int64(fy);
-      if (((tx64 | ty64) & 0xFF) == 0) { integral = true; tx = (int)(tx64 >> 8); ty = (int)(ty64 >> 8); }
-    }
-  }
+  const bool integral = integral_translation(c, tx, ty);
   if (!integral) {
     if (w <= 0 || h <= 0) return BL_AMD_SUCCESS;
     return fill_box_d(c, r, (double)x, (double)y, (double)x + (double)w, (double)y + (double)h);
@@ -1709,6 +1737,137 @@ uint32_t fill_rect_i(BLAmdContext* c, const Resolved& r, int x, int y, int w, in
   x1 = std::min<int64_t>(x1, c->target->w); y1 = std::min<int64_t>(y1, c->target->h);
   if (x0 >= x1 || y0 >= y1) return BL_AMD_SUCCESS;
   return fill_box_a(c, r, (int)x0, (int)y0, (int)x1, (int)y1);
+}
+
+// ---- blits (rastercontext.cpp:3938-4150) ----------------------------------
+//
+// A blit is a box fill whose style is the image's source area as a pattern
+// (extend REFLECT, BL_RASTER_CONTEXT_PREFERRED_BLIT_EXTEND, rastercontext_p.h:51)
+// under global alpha alone (BL_CONTEXT_RESOLVE_BLIT_OP, rastercontext.cpp:
+// 1332-1342, 1445-1453); comp-op simplification sees the image's format.
+
+constexpr uint32_t kBlitExtend = 2;   // REFLECT
+
+// check_image_area (rastercontext.cpp:2305-2320): the whole image, or `area`
+// when it lies inside the image.
+bool blit_area(const BLAmdImage* img, const int32_t* area, int32_t out[4]) {
+  out[0] = 0; out[1] = 0; out[2] = img->w; out[3] = img->h;
+  if (area) {
+    const unsigned max_w = (unsigned)img->w - (unsigned)area[0];
+    const unsigned max_h = (unsigned)img->h - (unsigned)area[1];
+    if ((max_w > (unsigned)img->w) | ((unsigned)area[2] > max_w) | (max_h > (unsigned)img->h) | ((unsigned)area[3] > max_h))
+      return false;
+    for (int k = 0; k < 4; k++) out[k] = area[k];
+  }
+  return true;
+}
+
+// The comp op replaces the image by a solid colour (CompOpSolidId of the
+// simplification with the image's format): the blit renders its destination
+// box without a pattern (prepare_overridden_fetch).
+bool blit_overridden(const BLAmdContext* c, const BLAmdImage* img) {
+  const Simplified s = simplify(c->comp_op, c->target->format, img->format);
+  return s.op != kCompNop && s.solid != kSolidNone;
+}
+
+Resolved resolve_blit(BLAmdContext* c, const BLAmdImage* img, const int32_t view[4], const Mat& t) {
+  StyleState st;
+  st.type = kStylePattern;
+  st.version = ++c->style_counter;
+  st.fmt = img->format;
+  st.image = ImageRef(img);
+  st.has_view = true;
+  for (int k = 0; k < 4; k++) st.view[k] = view[k];
+  // The pattern is set up only when the comp op keeps the image; a pattern
+  // that cannot be set up makes the blit a NOP (setup_pattern_affine
+  // failing returns BL_SUCCESS, rastercontext.cpp:4019-4020).
+  if (!blit_overridden(c, img) &&
+      init_pattern(st.dev, (uint32_t)view[2], (uint32_t)view[3], kBlitExtend, c->pattern_quality, t) == 0)
+    st.status = 1;
+  return resolve_with(c, st, (uint32_t)round_to_int(c->global_alpha * 255.0), true);
+}
+
+double bl_max_d(double a, double b) { return a < b ? b : a; }
+double bl_min_d(double a, double b) { return b < a ? b : a; }
+
+// blit_image_d_impl (rastercontext.cpp:3940-4031).
+uint32_t blit_d(BLAmdContext* c, double x, double y, const BLAmdImage* img, const int32_t* area) {
+  int32_t src[4];
+  if (!blit_area(img, area, src)) return kErrInvalid;
+  const Mat& u = c->user;
+  const int tt = transform_type(u);
+  if (tt == kTTInvalid) return BL_AMD_SUCCESS;
+  if (tt <= kTTTranslate) {
+    // the pixel box of the blit, from the 24.8 start and the source size
+    const CmdXform m = fixed_xform(c);
+    const double sx = x * m.m00 + m.m20, sy = y * m.m11 + m.m21;
+    const double dx0 = bl_max_d(sx, 0.0), dy0 = bl_max_d(sy, 0.0);
+    const double dx1 = bl_min_d(sx + (double)src[2] * m.m00, (double)c->target->w * 256.0);
+    const double dy1 = bl_min_d(sy + (double)src[3] * m.m11, (double)c->target->h * 256.0);
+    // clipped out, invalid coordinates or an empty area: only a solid
+    // override (which fills the unclipped box) can still render
+    int ix0 = 0, iy0 = 0, ix1 = 0, iy1 = 0;
+    bool empty = !((dx0 < dx1) & (dy0 < dy1));
+    if (!empty) {
+      ix0 = (int)dx0; iy0 = (int)dy0; ix1 = (int)dx1; iy1 = (int)dy1;
+      empty = !((ix0 < ix1) & (iy0 < iy1));
+    }
+    if (empty && !blit_overridden(c, img)) return BL_AMD_SUCCESS;
+    Mat t{1.0, 0.0, 0.0, 1.0, sx / 256.0, sy / 256.0};
+    Resolved r = resolve_blit(c, img, src, t);
+    if (r.nop) return BL_AMD_SUCCESS;
+    if (r.overridden) return fill_box_d(c, r, x, y, x + (double)src[2], y + (double)src[3]);
+    // fill_clipped_box_f: a pixel-aligned box composites with a constant mask
+    if (((ix0 | iy0 | ix1 | iy1) & 0xFF) == 0) return fill_box_a(c, r, ix0 >> 8, iy0 >> 8, ix1 >> 8, iy1 >> 8);
+    return fill_box_u(c, r, ix0, iy0, ix1, iy1);
+  }
+  // any other transform: the source translated to the origin, through it
+  const Mat tr{1.0, 0.0, 0.0, 1.0, x, y};
+  Resolved r = resolve_blit(c, img, src, multiply(tr, u));
+  if (r.nop) return BL_AMD_SUCCESS;
+  return fill_box_d(c, r, x, y, x + (double)src[2], y + (double)src[3]);
+}
+
+// blit_image_i_impl (rastercontext.cpp:4033-4066): with an integral
+// translation the destination box is clipped in integers; otherwise as
+// blit_image_d.
+uint32_t blit_i(BLAmdContext* c, int x, int y, const BLAmdImage* img, const int32_t* area) {
+  int tx = 0, ty = 0;
+  if (!integral_translation(c, tx, ty)) return blit_d(c, (double)x, (double)y, img, area);
+  int32_t src[4];
+  if (!blit_area(img, area, src)) return kErrInvalid;
+  const int64_t dx = (int64_t)x + tx, dy = (int64_t)y + ty;
+  const int64_t x0 = std::max<int64_t>(dx, 0), y0 = std::max<int64_t>(dy, 0);
+  const int64_t x1 = std::min<int64_t>(dx + src[2], c->target->w), y1 = std::min<int64_t>(dy + src[3], c->target->h);
+  // clipped out: a NOP whatever the comp op (translate_and_clip_rect_to_blit_i)
+  if (x0 >= x1 || y0 >= y1) return BL_AMD_SUCCESS;
+  // the source pixel of destination pixel X is X - dx (+ the area's origin)
+  Mat t{1.0, 0.0, 0.0, 1.0, (double)dx, (double)dy};
+  Resolved r = resolve_blit(c, img, src, t);
+  if (r.nop) return BL_AMD_SUCCESS;
+  return fill_box_a(c, r, (int)x0, (int)y0, (int)x1, (int)y1);
+}
+
+// blit_scaled_image_d_impl (rastercontext.cpp:4072-4110).
+uint32_t blit_scaled_d(BLAmdContext* c, double x, double y, double w, double h, const BLAmdImage* img, const int32_t* area) {
+  int32_t src[4];
+  if (!blit_area(img, area, src)) return kErrInvalid;
+  if (w == (double)src[2] && h == (double)src[3]) return blit_d(c, x, y, img, area);
+  const Mat sm{w / (double)src[2], 0.0, 0.0, h / (double)src[3], x, y};
+  Resolved r = resolve_blit(c, img, src, multiply(sm, c->user));
+  if (r.nop) return BL_AMD_SUCCESS;
+  return fill_box_d(c, r, x, y, x + w, y + h);
+}
+
+// blit_scaled_image_i_impl (rastercontext.cpp:4112-4150).
+uint32_t blit_scaled_i(BLAmdContext* c, int x, int y, int w, int h, const BLAmdImage* img, const int32_t* area) {
+  int32_t src[4];
+  if (!blit_area(img, area, src)) return kErrInvalid;
+  if (w == src[2] && h == src[3]) return blit_i(c, x, y, img, area);
+  const Mat sm{(double)w / (double)src[2], 0.0, 0.0, (double)h / (double)src[3], (double)x, (double)y};
+  Resolved r = resolve_blit(c, img, src, multiply(sm, c->user));
+  if (r.nop) return BL_AMD_SUCCESS;
+  return fill_box_d(c, r, (double)x, (double)y, (double)x + (double)w, (double)y + (double)h);
 }
 
 // fill_all (rastercontext.cpp:2720-2725) with the canvas as the clip box.
@@ -2133,6 +2292,32 @@ BLAmdResult bl_amd_context_fill_rect_d(BLAmdContext* ctx, double x, double y, do
 BLAmdResult bl_amd_context_fill_rect_d_rgba32(BLAmdContext* ctx, double x, double y, double w, double h, uint32_t rgba32) {
   if (uint32_t r = render_call(ctx)) return r;
   return fill_box_d(ctx, resolve_solid(ctx, rgba32), x, y, x + w, y + h);
+}
+
+BLAmdResult bl_amd_context_blit_image_i(BLAmdContext* ctx, int x, int y, const BLAmdImage* image, const int32_t* area) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!image) return kErrInvalid;
+  return blit_i(ctx, x, y, image, area);
+}
+
+BLAmdResult bl_amd_context_blit_image_d(BLAmdContext* ctx, double x, double y, const BLAmdImage* image, const int32_t* area) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!image) return kErrInvalid;
+  return blit_d(ctx, x, y, image, area);
+}
+
+BLAmdResult bl_amd_context_blit_scaled_image_i(BLAmdContext* ctx, int x, int y, int w, int h, const BLAmdImage* image,
+                                               const int32_t* area) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!image) return kErrInvalid;
+  return blit_scaled_i(ctx, x, y, w, h, image, area);
+}
+
+BLAmdResult bl_amd_context_blit_scaled_image_d(BLAmdContext* ctx, double x, double y, double w, double h,
+                                               const BLAmdImage* image, const int32_t* area) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!image) return kErrInvalid;
+  return blit_scaled_d(ctx, x, y, w, h, image, area);
 }
 
 BLAmdResult bl_amd_context_fill_rect_i(BLAmdContext* ctx, int x, int y, int w, int h) {

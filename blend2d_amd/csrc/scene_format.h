@@ -70,8 +70,22 @@ enum SceneOp {
   SCENE_OP_FILL_RECT_I_RGBA32 = 22, /* i32 x, y, w, h, u32 rgba32, u32 pad */
   SCENE_OP_FILL_RECT_D_RGBA32 = 23, /* f64 x, y, w, h, u32 rgba32, u32 pad */
   SCENE_OP_FILL_PATH_EX       = 24, /* ScenePathEx, u8 cmds[n] padded to 8, f64 vtx[2n] */
-  SCENE_OP_FILL_EXPLICIT      = 25  /* SceneExplicit, style payload, target payload (see below) */
+  SCENE_OP_FILL_EXPLICIT      = 25, /* SceneExplicit, style payload, target payload (see below) */
+  SCENE_OP_BLIT_IMAGE         = 26  /* SceneBlit */
 };
+
+/* BLIT_IMAGE: blit_image_i / _d (kinds 0, 1: dst = x, y) and
+ * blit_scaled_image_i / _d (kinds 2, 3: dst = x, y, w, h) of image
+ * `image_index`; `area` (x, y, w, h in the image) when has_area, else the
+ * whole image.  The _i kinds carry integers in `dst`. */
+typedef struct SceneBlit {
+  uint32_t kind;
+  uint32_t image_index;
+  uint32_t has_area;
+  uint32_t pad;
+  int32_t area[4];
+  double dst[4];
+} SceneBlit;
 
 /* FILL_EXPLICIT: a fill with an explicit style for this call only -- the
  * fill_*_rgba64 (style kind 1) and fill_*_ext (kinds 0-3) entry points of

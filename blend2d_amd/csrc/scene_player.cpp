@@ -57,6 +57,7 @@ struct Call {
   std::vector<uint32_t> stop_rgba;
   size_t image = 0;
   BLAmdStyle style{};
+  bool has_area = false;          // BLIT_IMAGE: area in i[], kind in u0, destination in d[0..3]
 };
 
 struct Scene {
@@ -79,6 +80,7 @@ bool is_render_op(uint32_t op) {
     case SCENE_OP_FILL_ALL: case SCENE_OP_CLEAR_ALL: case SCENE_OP_CLEAR_RECT_I: case SCENE_OP_CLEAR_RECT_D:
     case SCENE_OP_FILL_GEOMETRY: case SCENE_OP_FILL_ALL_RGBA32: case SCENE_OP_FILL_RECT_I_RGBA32:
     case SCENE_OP_FILL_RECT_D_RGBA32: case SCENE_OP_FILL_PATH_EX: case SCENE_OP_FILL_EXPLICIT:
+    case SCENE_OP_BLIT_IMAGE:
       return true;
     default:
       return false;
@@ -236,6 +238,18 @@ uint32_t decode(Scene& sc) {
       case SCENE_OP_SET_STYLE_PATTERN:
         if (!decode_pattern(c, pl, pe, sc.images.size())) return kBadScene;
         break;
+      case SCENE_OP_BLIT_IMAGE: {
+        SceneBlit b;
+        if (size_t(pe - pl) < sizeof(b)) return kBadScene;
+        std::memcpy(&b, pl, sizeof(b));
+        if (b.image_index >= sc.images.size()) return kBadScene;
+        c.u0 = b.kind;
+        c.image = b.image_index;
+        c.has_area = b.has_area != 0;
+        std::memcpy(c.i, b.area, 16);
+        std::memcpy(c.d, b.dst, 32);
+        break;
+      }
       case SCENE_OP_FILL_EXPLICIT: {
         SceneExplicit ex;
         std::memcpy(&ex, pl, sizeof(ex));
@@ -347,6 +361,19 @@ void issue(BLAmdContext* ctx, const Scene& sc, Issued& out) {
       case SCENE_OP_CLEAR_RECT_I: r = bl_amd_context_clear_rect_i(ctx, c.i[0], c.i[1], c.i[2], c.i[3]); break;
       case SCENE_OP_CLEAR_RECT_D: r = bl_amd_context_clear_rect_d(ctx, c.d[0], c.d[1], c.d[2], c.d[3]); break;
       case SCENE_OP_FILL_ALL_RGBA32: r = bl_amd_context_fill_all_rgba32(ctx, c.u0); break;
+      case SCENE_OP_BLIT_IMAGE: {
+        const int32_t* area = c.has_area ? c.i : nullptr;
+        BLAmdImage* img = sc.images[c.image];
+        switch (c.u0) {
+          case 0: r = bl_amd_context_blit_image_i(ctx, int(c.d[0]), int(c.d[1]), img, area); break;
+          case 1: r = bl_amd_context_blit_image_d(ctx, c.d[0], c.d[1], img, area); break;
+          case 2:
+            r = bl_amd_context_blit_scaled_image_i(ctx, int(c.d[0]), int(c.d[1]), int(c.d[2]), int(c.d[3]), img, area);
+            break;
+          default: r = bl_amd_context_blit_scaled_image_d(ctx, c.d[0], c.d[1], c.d[2], c.d[3], img, area); break;
+        }
+        break;
+      }
       case SCENE_OP_FILL_RECT_I_RGBA32:
         r = bl_amd_context_fill_rect_i_rgba32(ctx, c.i[0], c.i[1], c.i[2], c.i[3], c.u0);
         break;

@@ -310,7 +310,10 @@ int init_pattern(DevStyle& f, uint32_t img_w, uint32_t img_h, uint32_t extend, u
   Mat inv;
   if (!invert(inv, t)) return 0;
   int tw = (int)img_w, th = (int)img_h;
-  if (tw == 0) return 0;
+  // An empty source renders nothing.  The reference tests only the width;
+  // a zero-height blit area reaches its pattern fetch, which divides by the
+  // height (SIGFPE in the portable pipeline).
+  if (tw == 0 || th == 0) return 0;
   double xx = inv.m00, xy = inv.m01, yx = inv.m10, yy = inv.m11;
   const double eps = 1e-14;   // Math::epsilon<double>, support/math_p.h:42
   if (fabs(xx - 1.0) <= eps && fabs(xy) <= eps && fabs(yx) <= eps && fabs(yy - 1.0) <= eps) {

@@ -23,7 +23,8 @@ def _images(ctx_cls, sc):
 # them, and an error found on the device names its call by that count.
 RENDER_OPS = {S.OP_FILL_PATH, S.OP_FILL_POLYGON, S.OP_FILL_RECT_D, S.OP_FILL_RECT_I, S.OP_FILL_ALL,
               S.OP_CLEAR_ALL, S.OP_CLEAR_RECT_I, S.OP_CLEAR_RECT_D, S.OP_FILL_GEOMETRY, S.OP_FILL_ALL_RGBA32,
-              S.OP_FILL_RECT_I_RGBA32, S.OP_FILL_RECT_D_RGBA32, S.OP_FILL_PATH_EX, S.OP_FILL_EXPLICIT}
+              S.OP_FILL_RECT_I_RGBA32, S.OP_FILL_RECT_D_RGBA32, S.OP_FILL_PATH_EX, S.OP_FILL_EXPLICIT,
+              S.OP_BLIT_IMAGE}
 
 
 def _explicit(pl, images):
@@ -160,6 +161,12 @@ def replay_into(ctx, sc: "S.Scene", images) -> list:
                 call(lambda: ctx.fill_path(geom, style=style))
             else:
                 call(lambda: ctx.fill_geometry(geom[0], geom[1], style=style))
+        elif op == S.OP_BLIT_IMAGE:
+            kind, index, has_area, _, ax, ay, aw, ah, dx, dy, dw, dh = struct.unpack_from("<4I4i4d", pl)
+            area = (ax, ay, aw, ah) if has_area else None
+            dst = (dx, dy) if kind in (S.BLIT_I, S.BLIT_D) else (dx, dy, dw, dh)
+            integer = kind in (S.BLIT_I, S.BLIT_SCALED_I)
+            call(ctx.blit_image, dst, images[index], area, integer)
         elif op == S.OP_FILL_PATH_EX:
             n, flags, col, _, ox, oy = struct.unpack_from("<4I2d", pl)
             cmds = np.frombuffer(pl, dtype=np.uint8, count=n, offset=32)

@@ -48,6 +48,8 @@ OP_FILL_RECT_I_RGBA32 = 22
 OP_FILL_RECT_D_RGBA32 = 23
 OP_FILL_PATH_EX = 24
 OP_FILL_EXPLICIT = 25
+OP_BLIT_IMAGE = 26
+BLIT_I, BLIT_D, BLIT_SCALED_I, BLIT_SCALED_D = 0, 1, 2, 3
 
 # FILL_EXPLICIT targets and style kinds
 TARGET_ALL, TARGET_RECT_I, TARGET_RECT_D, TARGET_PATH, TARGET_GEOMETRY = range(5)
@@ -168,6 +170,16 @@ class Scene:
 
     def fill_all(self) -> None:
         self._cmd(OP_FILL_ALL)
+        self.fill_count += 1
+
+    def blit_image(self, kind: int, image_index: int, dst, area=None) -> None:
+        """blit_image_i / _d (dst: x, y) and blit_scaled_image_i / _d (dst: x, y,
+        w, h) of image `image_index`, `area` (x, y, w, h) or the whole image
+        (scene_format.h: SceneBlit)."""
+        integer = kind in (BLIT_I, BLIT_SCALED_I)
+        d = [float(int(v)) if integer else float(v) for v in dst] + [0.0] * (4 - len(dst))
+        a = list(area) if area is not None else [0, 0, 0, 0]
+        self._cmd(OP_BLIT_IMAGE, struct.pack("<4I4i4d", kind, image_index, 0 if area is None else 1, 0, *a, *d))
         self.fill_count += 1
 
     def set_fill_alpha(self, a: float) -> None:
@@ -808,17 +820,11 @@ def edge_case_scene(width: int = 257, height: int = 131, fmt: int = FMT_PRGB32) 
 INT32_MIN, INT32_MAX = -(1 << 31), (1 << 31) - 1
 
 
-def clip_robustness_scenes(width: int = 97, height: int = 67, fmt: int = FMT_PRGB32, seed: int = 1) -> list:
-    """The fill-clipping matrix of the reference's robustness test
-    (core/context_test.cpp:389-586): under each of its nine user transforms,
-    fill_rect(BLRectI) for every (point, size) pair of its integer tables and
-    fill_rect(BLRect) for every pair of its double tables (NaN, +-Inf, INT32
-    extremes, 1e-100 scales, sizes just below a pixel).  The reference fills
-    opaque white; here each call gets its own seeded translucent colour so a
-    coverage difference anywhere shows in the pixels.  One canvas per
-    transform."""
+def _clip_tables(cw: int, ch: int, sw: int, sh: int):
+    """The user transforms, points and sizes of the reference's clipping
+    robustness test (core/context_test.cpp:389-545) for a cw x ch canvas and
+    an sw x sh sprite."""
     nan, inf = float("nan"), float("inf")
-    cw, ch, sw, sh = width, height, 23, 23
     matrices = [
         (1.0, 0.0, 0.0, 1.0, 0.0, 0.0), (1.0, 0.0, 0.0, 1.0, 11.3, 11.9),
         (100.0, 0.0, 0.0, 100.0, 0.0, 0.0), (-100.0, 0.0, 0.0, -100.0, 0.0, 0.0),
@@ -846,6 +852,19 @@ def clip_robustness_scenes(width: int = 97, height: int = 67, fmt: int = FMT_PRG
     sizes_d += [(10.0 ** e, float(sh)) for e in (40, 80, 120, 160, 200)]
     sizes_d += [(inf, float(sh)), (inf, float(sh)), (inf, inf), (float(sw), -inf), (-inf, float(sh)), (-inf, -inf),
                 (float(sw), nan), (nan, float(sh)), (nan, nan)]
+    return matrices, pts_i, pts_d, sizes_i, sizes_d
+
+
+def clip_robustness_scenes(width: int = 97, height: int = 67, fmt: int = FMT_PRGB32, seed: int = 1) -> list:
+    """The fill-clipping matrix of the reference's robustness test
+    (core/context_test.cpp:389-586): under each of its nine user transforms,
+    fill_rect(BLRectI) for every (point, size) pair of its integer tables and
+    fill_rect(BLRect) for every pair of its double tables (NaN, +-Inf, INT32
+    extremes, 1e-100 scales, sizes just below a pixel).  The reference fills
+    opaque white; here each call gets its own seeded translucent colour so a
+    coverage difference anywhere shows in the pixels.  One canvas per
+    transform."""
+    matrices, pts_i, pts_d, sizes_i, sizes_d = _clip_tables(width, height, 23, 23)
     rng = np.random.default_rng(seed)
     scenes = []
     for m in matrices:
@@ -859,6 +878,38 @@ def clip_robustness_scenes(width: int = 97, height: int = 67, fmt: int = FMT_PRG
             for w, h in sizes_d:
                 sc.set_fill_style_rgba32(int(rng.integers(0, 1 << 24)) | (int(rng.integers(8, 48)) << 24))
                 sc.fill_rect_d(x, y, w, h)
+        scenes.append(sc)
+    return scenes
+
+
+def blit_clip_robustness_scenes(width: int = 97, height: int = 67, fmt: int = FMT_PRGB32, seed: int = 1) -> list:
+    """The blit-clipping matrix of the reference's robustness test
+    (core/context_test.cpp:567-586): under each of its nine user transforms,
+    blit_image(BLPointI) and blit_image(BLRectI) for every point and (point,
+    size) pair of its integer tables, blit_image(BLPoint) and
+    blit_image(BLRect) for its double tables, of a 23 x 23 sprite.  The
+    reference blits an opaque white sprite; here the sprite is seeded noise
+    and each call gets its own global alpha, so a difference in coverage or
+    sampling anywhere shows in the pixels.  One canvas per transform."""
+    matrices, pts_i, pts_d, sizes_i, sizes_d = _clip_tables(width, height, 23, 23)
+    rng = np.random.default_rng(seed)
+    scenes = []
+    for m in matrices:
+        sc = Scene(width, height, fmt, 0xFF000000 if fmt != FMT_A8 else 0)
+        sprite = sc.add_image(premultiplied_noise(rng, 23, 23))
+        sc.set_transform(m)
+        for x, y in pts_i:
+            sc.set_global_alpha(float(rng.uniform(0.05, 0.3)))
+            sc.blit_image(BLIT_I, sprite, (x, y))
+            for w, h in sizes_i:
+                sc.set_global_alpha(float(rng.uniform(0.05, 0.3)))
+                sc.blit_image(BLIT_SCALED_I, sprite, (x, y, w, h))
+        for x, y in pts_d:
+            sc.set_global_alpha(float(rng.uniform(0.05, 0.3)))
+            sc.blit_image(BLIT_D, sprite, (x, y))
+            for w, h in sizes_d:
+                sc.set_global_alpha(float(rng.uniform(0.05, 0.3)))
+                sc.blit_image(BLIT_SCALED_D, sprite, (x, y, w, h))
         scenes.append(sc)
     return scenes
 
@@ -1045,6 +1096,110 @@ def explicit_styles_scene(seed: int = 1, width: int = 233, height: int = 171, fm
             # the context's own style is untouched by the explicit calls
             n = int(rng.integers(3, 7))
             sc.fill_polygon(rng.random((n, 2)) * np.array([W, H]))
+    return sc
+
+
+def blit_scene(seed: int = 1, width: int = 257, height: int = 181, fmt: int = FMT_PRGB32, count: int = 90,
+               bad_areas: bool = False) -> Scene:
+    """blit_image_i / _d and blit_scaled_image_i / _d (core/context.h:870-873)
+    of PRGB32, XRGB32 and A8 images, whole or through an area (with
+    `bad_areas`, now and then an area outside the image, which the reference
+    rejects with BL_ERROR_INVALID_VALUE and renders nothing): integer, 1/256 and
+    arbitrary fractional origins, partly or wholly off the canvas; scaled
+    rectangles larger, smaller, mirrored, empty or of the area's own size;
+    under identity, integral and fractional translations, scales, rotations
+    and a singular user transform; SRC_OVER / SRC_COPY, global alpha (0 makes
+    a blit a NOP; fill alpha and the fill style are ignored) and both
+    pattern qualities, interleaved with ordinary fills."""
+    rng = np.random.default_rng(seed)
+    clear = 0xFF000000 | int(rng.integers(0, 1 << 24)) if fmt != FMT_A8 else int(rng.integers(0, 256))
+    sc = Scene(width, height, fmt, clear)
+    imgs = [sc.add_image(premultiplied_noise(rng, 61, 47)),
+            sc.add_image(premultiplied_noise(rng, 1, 37)),
+            sc.add_image(premultiplied_noise(rng, 29, 1)),
+            sc.add_image(premultiplied_noise(rng, 3, 2)),
+            sc.add_image(premultiplied_noise(rng, 130, 90)),
+            sc.add_image(rng.integers(0, 1 << 32, size=(23, 19), dtype=np.uint32), FMT_XRGB32),
+            sc.add_image(rng.integers(0, 256, size=(17, 33), dtype=np.uint8), FMT_A8)]
+    sizes = [(61, 47), (1, 37), (29, 1), (3, 2), (130, 90), (19, 23), (33, 17)]
+    W, H = float(width), float(height)
+
+    def coord(extent):
+        t = rng.random()
+        if t < 0.35:
+            return float(rng.integers(int(-0.3 * extent), int(extent)))
+        if t < 0.6:
+            return float(rng.integers(int(-0.3 * extent) * 256, int(extent) * 256)) / 256.0
+        return float(rng.uniform(-0.3 * extent, extent))
+
+    for i in range(count):
+        r = rng.random()
+        if r < 0.06:
+            t = rng.random()
+            if t < 0.35:
+                sc.set_transform([1.0, 0.0, 0.0, 1.0, float(rng.integers(-20, 21)), float(rng.integers(-20, 21))])
+            elif t < 0.55:
+                sc.set_transform([1.0, 0.0, 0.0, 1.0, float(rng.uniform(-20, 20)), float(rng.uniform(-20, 20))])
+            elif t < 0.7:
+                sc.set_transform([float(rng.uniform(0.5, 2)), 0.0, 0.0, float(rng.uniform(0.5, 2)),
+                                  float(rng.uniform(-20, 20)), float(rng.uniform(-20, 20))])
+            elif t < 0.82:
+                sc.set_transform(rotation_about(float(rng.uniform(0, 6.3)), W / 2, H / 2))
+            elif t < 0.87:
+                sc.set_transform([0.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+            else:
+                sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+            continue
+        if r < 0.1:
+            sc.set_comp_op(int(rng.choice([COMP_OP_SRC_OVER, COMP_OP_SRC_COPY])))
+            continue
+        if r < 0.14:
+            a = rng.random()
+            sc.set_global_alpha(0.0 if a < 0.15 else 1.0 if a < 0.6 else float(rng.uniform(0.0, 1.0)))
+            continue
+        if r < 0.17:
+            sc.set_fill_alpha(float(rng.uniform(0.0, 1.0)))
+            sc.set_pattern_quality(int(rng.integers(0, 2)))
+            continue
+        if r < 0.22:
+            sc.set_fill_style_rgba32(_rand_rgba(rng))
+            sc.fill_rect_d(coord(W), coord(H), float(rng.uniform(1, W / 2)), float(rng.uniform(1, H / 2)))
+            continue
+        k = int(rng.integers(0, len(imgs)))
+        iw, ih = sizes[k]
+        area = None
+        a = rng.random()
+        if a < 0.4:
+            ax, ay = int(rng.integers(0, iw)), int(rng.integers(0, ih))
+            # zero widths are NOPs; zero heights are left out (the reference's
+            # pattern fetch divides by the area height)
+            area = (ax, ay, int(rng.integers(0, iw - ax + 1)), int(rng.integers(1, ih - ay + 1)))
+        elif a < 0.45 and bad_areas:
+            area = (int(rng.integers(-3, iw + 3)), int(rng.integers(-3, ih + 3)), iw, ih)
+            if area[0] == 0 and area[1] == 0:
+                area = (1, 0, iw, ih)
+        aw, ah = (area[2], area[3]) if area is not None else (iw, ih)
+        kind = int(rng.integers(0, 4))
+        x, y = coord(W), coord(H)
+        if kind == BLIT_I:
+            sc.blit_image(kind, imgs[k], (int(x), int(y)), area)
+        elif kind == BLIT_D:
+            sc.blit_image(kind, imgs[k], (x, y), area)
+        else:
+            s = rng.random()
+            if s < 0.15:
+                w, h = float(aw), float(ah)
+            elif s < 0.25:
+                w, h = float(rng.uniform(-80, -1)), float(rng.uniform(1, 80))
+            elif s < 0.3:
+                w, h = 0.0, float(rng.uniform(1, 40))
+            else:
+                w, h = float(rng.uniform(1, W * 0.6)), float(rng.uniform(1, H * 0.6))
+            if kind == BLIT_SCALED_I:
+                sc.blit_image(kind, imgs[k], (int(x), int(y), int(w), int(h)), area)
+            else:
+                sc.blit_image(kind, imgs[k], (x, y, w, h), area)
+    sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
     return sc
 
 

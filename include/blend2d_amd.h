@@ -235,6 +235,23 @@ BLAmdResult bl_amd_context_clear_all(BLAmdContext* ctx);
 BLAmdResult bl_amd_context_clear_rect_i(BLAmdContext* ctx, int x, int y, int w, int h);
 BLAmdResult bl_amd_context_clear_rect_d(BLAmdContext* ctx, double x, double y, double w, double h);
 
+/* bl_context_blit_image_i / _d (context.h:870-871, raster/rastercontext.cpp:
+ * 3940-4066): composites `area` (x, y, w, h of the image; NULL: all of it)
+ * of `image` with its top-left corner at (x, y), through the user
+ * transform; the comp op and global alpha apply, the fill style and fill
+ * alpha do not.  Fractional and transformed positions sample the area with
+ * the pattern quality hint, extend REFLECT.  An area outside the image is
+ * BL_AMD_ERROR_INVALID_VALUE. */
+BLAmdResult bl_amd_context_blit_image_i(BLAmdContext* ctx, int x, int y, const BLAmdImage* image, const int32_t* area);
+BLAmdResult bl_amd_context_blit_image_d(BLAmdContext* ctx, double x, double y, const BLAmdImage* image,
+                                        const int32_t* area);
+/* bl_context_blit_scaled_image_i / _d (context.h:872-873, rastercontext.cpp:
+ * 4072-4150): the area scaled onto the rectangle (x, y, w, h). */
+BLAmdResult bl_amd_context_blit_scaled_image_i(BLAmdContext* ctx, int x, int y, int w, int h, const BLAmdImage* image,
+                                               const int32_t* area);
+BLAmdResult bl_amd_context_blit_scaled_image_d(BLAmdContext* ctx, double x, double y, double w, double h,
+                                               const BLAmdImage* image, const int32_t* area);
+
 /* ---- statistics and replay (no counterpart in the reference) ----------- */
 
 /* Counters of the last flush or replay. */

@@ -38,6 +38,8 @@ typedef struct {
   ScenePattern pat;
   OrcMat xform;       /* user transform captured by set_fill_style (rastercontext.cpp:653-695) */
   uint32_t quality;   /* pattern quality hint captured by set_fill_style */
+  uint32_t has_view;  /* blits: the pattern samples view = x, y, w, h of the image */
+  int32_t view[4];
 } Style;
 
 typedef struct {
@@ -48,7 +50,7 @@ typedef struct {
   double fill_alpha;
   Style style;
   Style explicit_style;   /* mode 3: the style of an explicit-style call */
-  uint32_t mode;          /* 0 fill with the style, 1 explicit colour, 2 clear, 3 explicit style */
+  uint32_t mode;          /* 0 fill with the style, 1 explicit colour, 2 clear, 3 explicit style, 4 blit */
   uint32_t explicit_rgba32;
   uint32_t call_all;      /* fill_all / clear_all: an invalid transform is no NOP */
   SceneImage img_info[16];
@@ -64,6 +66,8 @@ static void note(Ctx* c, uint32_t err) { if (err && !c->first_error) c->first_er
  * clear calls use the full alpha (rastercontext.cpp:1385-1391). */
 static uint32_t alpha_i(const Ctx* c) {
   if (c->mode == 2) return 255u;
+  /* blits: global alpha alone (BL_CONTEXT_RESOLVE_BLIT_OP, rastercontext.cpp:1445-1453) */
+  if (c->mode == 4) return (uint32_t)orc_round_to_int(c->global_alpha * 255.0);
   return (uint32_t)orc_round_to_int(c->global_alpha * 255.0 * c->fill_alpha);
 }
 
@@ -163,7 +167,7 @@ static uint32_t resolve(Ctx* c, OrcFetch* f, uint32_t* lut_buf, OrcPipe* pipe) {
     tmp.rgba32 = c->explicit_rgba32;
     st = &tmp;
   }
-  else if (c->mode == 3) {
+  else if (c->mode == 3 || c->mode == 4) {
     st = &c->explicit_style;
   }
   OrcMat ft = st->xform;  /* final_transform (user x identity meta) when the style was set */
@@ -237,6 +241,12 @@ static uint32_t resolve(Ctx* c, OrcFetch* f, uint32_t* lut_buf, OrcPipe* pipe) {
     f->img_w = ii->width;
     f->img_h = ii->height;
     f->img_format = ii->format;
+    if (st->has_view) {
+      /* init_image_source (raster/renderfetchdata_p.h:146-157) */
+      f->img += (size_t)st->view[1] * ii->stride + (size_t)st->view[0] * (ii->format == ORC_FMT_A8 ? 1u : 4u);
+      f->img_w = (uint32_t)st->view[2];
+      f->img_h = (uint32_t)st->view[3];
+    }
     int r = orc_init_pattern(f, st->pat.extend, st->quality, &t);
     if (r < 0) return 1;
     if (r > 0) return ERR_NOT_IMPLEMENTED;
@@ -380,6 +390,132 @@ static void fill_rect_i(Ctx* c, int x, int y, int w, int h) {
   if (y1 > (int64_t)c->canvas.h) y1 = c->canvas.h;
   if (x0 >= x1 || y0 >= y1) return;
   fill_box_i(c, (int)x0, (int)y0, (int)x1, (int)y1);
+}
+
+/* ---- blits (rastercontext.cpp:3938-4150) ----
+ * A blit fills a box with the image area as a REFLECT pattern
+ * (BL_RASTER_CONTEXT_PREFERRED_BLIT_EXTEND, raster/rastercontext_p.h:51)
+ * under global alpha alone; comp-op simplification sees the image format. */
+
+/* check_image_area (rastercontext.cpp:2305-2320) */
+static int blit_area(const Ctx* c, uint32_t img, const int32_t* area, int32_t out[4]) {
+  const SceneImage* ii = &c->img_info[img];
+  out[0] = 0; out[1] = 0; out[2] = (int32_t)ii->width; out[3] = (int32_t)ii->height;
+  if (!area) return 1;
+  unsigned max_w = (unsigned)ii->width - (unsigned)area[0];
+  unsigned max_h = (unsigned)ii->height - (unsigned)area[1];
+  if ((max_w > ii->width) | ((unsigned)area[2] > max_w) | (max_h > ii->height) | ((unsigned)area[3] > max_h)) return 0;
+  memcpy(out, area, 16);
+  return 1;
+}
+
+/* The comp op replaces the image by a solid colour (solid_id of the
+ * simplification with the image's format, prepare_overridden_fetch). */
+static int blit_overridden(const Ctx* c, uint32_t img) {
+  uint32_t pd, ps, sid;
+  uint32_t op = orc_simplify(c->comp_op, c->canvas.format, c->img_info[img].format, &pd, &ps, &sid);
+  return op != ORC_OP_NOP && sid != ORC_SOLID_NONE;
+}
+
+static void set_blit_style(Ctx* c, uint32_t img, const int32_t src[4], const OrcMat* t) {
+  Style* st = &c->explicit_style;
+  memset(st, 0, sizeof(*st));
+  st->type = 2;
+  st->xform = *t;
+  st->quality = c->pattern_quality;
+  st->pat.image_index = img;
+  st->pat.extend = 2;   /* BL_EXTEND_MODE_REFLECT */
+  st->has_view = 1;
+  memcpy(st->view, src, 16);
+  c->mode = 4;
+}
+
+/* blit_image_d_impl (rastercontext.cpp:3940-4031) */
+static void blit_d(Ctx* c, double x, double y, uint32_t img, const int32_t* area) {
+  int32_t src[4];
+  if (!blit_area(c, img, area, src)) { note(c, ERR_INVALID_VALUE); return; }
+  int tt = orc_mat_type(&c->user);
+  if (tt == ORC_TT_INVALID) return;
+  if (tt <= ORC_TT_TRANSLATE) {
+    OrcMat m; int ftt;
+    fixed_transform(c, &m, &ftt);
+    double sx = x * m.m00 + m.m20, sy = y * m.m11 + m.m21;
+    double dx0 = orc_max(sx, 0.0), dy0 = orc_max(sy, 0.0);
+    double dx1 = orc_min(sx + (double)src[2] * m.m00, (double)c->canvas.w * 256.0);
+    double dy1 = orc_min(sy + (double)src[3] * m.m11, (double)c->canvas.h * 256.0);
+    int ix0 = 0, iy0 = 0, ix1 = 0, iy1 = 0;
+    int empty = !((dx0 < dx1) & (dy0 < dy1));
+    if (!empty) {
+      ix0 = orc_trunc(dx0); iy0 = orc_trunc(dy0); ix1 = orc_trunc(dx1); iy1 = orc_trunc(dy1);
+      empty = !((ix0 < ix1) & (iy0 < iy1));
+    }
+    if (empty && !blit_overridden(c, img)) return;
+    OrcMat t = { 1.0, 0.0, 0.0, 1.0, sx / 256.0, sy / 256.0 };
+    set_blit_style(c, img, src, &t);
+    OrcFetch f;
+    OrcPipe p;
+    static uint32_t lut[2048];
+    uint32_t r = resolve(c, &f, lut, &p);
+    if (r == 1) return;
+    if (r == 0 && f.type == ORC_FETCH_SOLID) {
+      /* the comp op replaced the image by a solid colour: the unclipped box */
+      fill_box_d(c, x, y, x + (double)src[2], y + (double)src[3]);
+      return;
+    }
+    if (r > 1) { note(c, r); return; }
+    /* fill_clipped_box_f (rastercontext.cpp:2687-2696) */
+    if (((ix0 | iy0 | ix1 | iy1) & 0xFF) == 0)
+      orc_fill_box_a(&c->canvas, &p, alpha_i(c), ix0 >> 8, iy0 >> 8, ix1 >> 8, iy1 >> 8);
+    else
+      orc_fill_box_u(&c->canvas, &p, alpha_i(c), ix0, iy0, ix1, iy1);
+    return;
+  }
+  /* the area at the origin, translated to (x, y), through the user transform */
+  OrcMat tr = { 1.0, 0.0, 0.0, 1.0, x, y }, t;
+  orc_mat_multiply(&t, &tr, &c->user);
+  set_blit_style(c, img, src, &t);
+  fill_box_d(c, x, y, x + (double)src[2], y + (double)src[3]);
+}
+
+/* blit_image_i_impl (rastercontext.cpp:4033-4066) with
+ * translate_and_clip_rect_to_blit_i (rastercontext.cpp:922-993) */
+static void blit_i(Ctx* c, int x, int y, uint32_t img, const int32_t* area) {
+  int tt = orc_mat_type(&c->user);
+  int integral = 0, tx = 0, ty = 0;
+  if (tt <= ORC_TT_TRANSLATE) {
+    int64_t tx64 = orc_floor_to_int64(c->user.m20 * 256.0), ty64 = orc_floor_to_int64(c->user.m21 * 256.0);
+    if (((tx64 | ty64) & 0xFF) == 0) { integral = 1; tx = (int)(tx64 >> 8); ty = (int)(ty64 >> 8); }
+  }
+  if (!integral) { blit_d(c, (double)x, (double)y, img, area); return; }
+  int32_t src[4];
+  if (!blit_area(c, img, area, src)) { note(c, ERR_INVALID_VALUE); return; }
+  int64_t dx = (int64_t)x + tx, dy = (int64_t)y + ty;
+  int64_t x0 = dx < 0 ? 0 : dx, y0 = dy < 0 ? 0 : dy;
+  int64_t x1 = dx + src[2], y1 = dy + src[3];
+  if (x1 > (int64_t)c->canvas.w) x1 = c->canvas.w;
+  if (y1 > (int64_t)c->canvas.h) y1 = c->canvas.h;
+  if (x0 >= x1 || y0 >= y1) return;
+  OrcMat t = { 1.0, 0.0, 0.0, 1.0, (double)dx, (double)dy };
+  set_blit_style(c, img, src, &t);
+  fill_box_i(c, (int)x0, (int)y0, (int)x1, (int)y1);
+}
+
+/* blit_scaled_image_d_impl / _i_impl (rastercontext.cpp:4072-4150): the
+ * area scaled onto the rectangle, through the user transform; a rectangle
+ * of the area's size is a plain blit. */
+static void blit_scaled(Ctx* c, int integer, double x, double y, double w, double h, uint32_t img, const int32_t* area) {
+  int32_t src[4];
+  if (!blit_area(c, img, area, src)) { note(c, ERR_INVALID_VALUE); return; }
+  if (w == (double)src[2] && h == (double)src[3]) {
+    if (integer) blit_i(c, (int)x, (int)y, img, area);
+    else blit_d(c, x, y, img, area);
+    return;
+  }
+  if (orc_mat_type(&c->user) == ORC_TT_INVALID) return;
+  OrcMat sm = { w / (double)src[2], 0.0, 0.0, h / (double)src[3], x, y }, t;
+  orc_mat_multiply(&t, &sm, &c->user);
+  set_blit_style(c, img, src, &t);
+  fill_box_d(c, x, y, x + w, y + h);
 }
 
 /* Path builder for BLPath::add_geometry (core/path.cpp:1396-1640),
@@ -732,6 +868,22 @@ uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
         memcpy(&g, pl, sizeof(g));
         if (g.has_rgba32) { c.mode = 1; c.explicit_rgba32 = g.rgba32; }
         fill_geometry(&c, g.type, pl + sizeof(g), g.count);
+        c.mode = 0;
+        break;
+      }
+      case SCENE_OP_BLIT_IMAGE: {
+        SceneBlit b;
+        memcpy(&b, pl, sizeof(b));
+        if (b.image_index >= c.n_images) { note(&c, ERR_INVALID_VALUE); break; }
+        const int32_t* area = b.has_area ? b.area : NULL;
+        switch (b.kind) {
+          case 0: blit_i(&c, (int)b.dst[0], (int)b.dst[1], b.image_index, area); break;
+          case 1: blit_d(&c, b.dst[0], b.dst[1], b.image_index, area); break;
+          case 2: blit_scaled(&c, 1, (double)(int)b.dst[0], (double)(int)b.dst[1], (double)(int)b.dst[2],
+                              (double)(int)b.dst[3], b.image_index, area); break;
+          case 3: blit_scaled(&c, 0, b.dst[0], b.dst[1], b.dst[2], b.dst[3], b.image_index, area); break;
+          default: break;
+        }
         c.mode = 0;
         break;
       }

@@ -400,7 +400,10 @@ int orc_init_pattern(OrcFetch* f, uint32_t extend, uint32_t quality, const OrcMa
   OrcMat inv;
   if (orc_mat_invert(&inv, t) != 0) return -1;
   int tw = (int)f->img_w, th = (int)f->img_h;
-  if (tw == 0) return -1;
+  /* pipedefs.cpp:170-175 tests the width only; a zero-height blit area makes
+   * the reference's pattern fetch divide by zero, so an empty height is a
+   * NOP here (no reference output to follow). */
+  if (tw == 0 || th == 0) return -1;
 
   double xx = inv.m00, xy = inv.m01, yx = inv.m10, yy = inv.m11;
   const double eps = 1e-14;   /* Math::epsilon<double>, support/math_p.h:42 */

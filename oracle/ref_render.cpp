@@ -54,6 +54,8 @@ struct Call {
   SceneExplicit ex{};
   BLRectI rect_i;
   BLRect rect_d;
+  // BLIT_IMAGE
+  SceneBlit blit{};
 };
 
 // Builds the style object of a FILL_EXPLICIT payload (kinds 2, 3) into `call`;
@@ -274,6 +276,10 @@ uint32_t REF_RENDER_ENTRY(const uint8_t* scene, size_t size, uint8_t* out, size_
           call.gradient.set_transform(matrix_from(g.transform));
         break;
       }
+      case SCENE_OP_BLIT_IMAGE:
+        std::memcpy(&call.blit, payload, sizeof(SceneBlit));
+        if (call.blit.image_index >= images.size()) return 0x7FFFFFFFu;
+        break;
       case SCENE_OP_SET_STYLE_PATTERN: {
         ScenePattern p;
         std::memcpy(&p, payload, sizeof(p));
@@ -334,6 +340,34 @@ uint32_t REF_RENDER_ENTRY(const uint8_t* scene, size_t size, uint8_t* out, size_
       case SCENE_OP_FILL_RECT_D: note(ctx.fill_rect(BLRect(call.d[0], call.d[1], call.d[2], call.d[3]))); break;
       case SCENE_OP_FILL_RECT_I: note(ctx.fill_rect(BLRectI(int(call.d[0]), int(call.d[1]), int(call.d[2]), int(call.d[3])))); break;
       case SCENE_OP_FILL_ALL: note(ctx.fill_all()); break;
+      case SCENE_OP_BLIT_IMAGE: {
+        const SceneBlit& b = call.blit;
+        const BLImage& img = images[b.image_index];
+        const BLRectI area(b.area[0], b.area[1], b.area[2], b.area[3]);
+        switch (b.kind) {
+          case 0: {
+            const BLPointI o(int(b.dst[0]), int(b.dst[1]));
+            note(b.has_area ? ctx.blit_image(o, img, area) : ctx.blit_image(o, img));
+            break;
+          }
+          case 1: {
+            const BLPoint o(b.dst[0], b.dst[1]);
+            note(b.has_area ? ctx.blit_image(o, img, area) : ctx.blit_image(o, img));
+            break;
+          }
+          case 2: {
+            const BLRectI r(int(b.dst[0]), int(b.dst[1]), int(b.dst[2]), int(b.dst[3]));
+            note(b.has_area ? ctx.blit_image(r, img, area) : ctx.blit_image(r, img));
+            break;
+          }
+          default: {
+            const BLRect r(b.dst[0], b.dst[1], b.dst[2], b.dst[3]);
+            note(b.has_area ? ctx.blit_image(r, img, area) : ctx.blit_image(r, img));
+            break;
+          }
+        }
+        break;
+      }
       case SCENE_OP_SET_FILL_ALPHA: note(ctx.set_fill_alpha(call.d[0])); break;
       case SCENE_OP_SET_STYLE_RGBA64: note(ctx.set_fill_style(BLRgba64(call.u64))); break;
       case SCENE_OP_SET_STYLE_RGBA: note(ctx.set_fill_style(BLRgba(call.f[0], call.f[1], call.f[2], call.f[3]))); break;

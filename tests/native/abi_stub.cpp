@@ -125,6 +125,11 @@ std::string path(const double* origin, const uint8_t* cmds, const double* vtx, s
   return buf;
 }
 
+std::string area_of(const int32_t* a) {
+  return a ? std::to_string(a[0]) + " " + std::to_string(a[1]) + " " + std::to_string(a[2]) + " " + std::to_string(a[3])
+           : std::string("none");
+}
+
 }  // namespace
 
 extern "C" {
@@ -251,6 +256,26 @@ BLAmdResult bl_amd_context_fill_rect_i_ext(BLAmdContext*, int x, int y, int w, i
 }
 BLAmdResult bl_amd_context_fill_rect_d_ext(BLAmdContext*, double x, double y, double w, double h, const BLAmdStyle* st) {
   logf("fill_rect_d %.17g %.17g %.17g %.17g %s", x, y, w, h, style(st).c_str());
+  return 0;
+}
+
+BLAmdResult bl_amd_context_blit_image_i(BLAmdContext*, int x, int y, const BLAmdImage* image, const int32_t* area) {
+  logf("blit_image_i %d %d image=%u area=%s", x, y, serial_of(image), area_of(area).c_str());
+  return 0;
+}
+BLAmdResult bl_amd_context_blit_image_d(BLAmdContext*, double x, double y, const BLAmdImage* image, const int32_t* area) {
+  logf("blit_image_d %.17g %.17g image=%u area=%s", x, y, serial_of(image), area_of(area).c_str());
+  return 0;
+}
+BLAmdResult bl_amd_context_blit_scaled_image_i(BLAmdContext*, int x, int y, int w, int h, const BLAmdImage* image,
+                                               const int32_t* area) {
+  logf("blit_scaled_image_i %d %d %d %d image=%u area=%s", x, y, w, h, serial_of(image), area_of(area).c_str());
+  return 0;
+}
+BLAmdResult bl_amd_context_blit_scaled_image_d(BLAmdContext*, double x, double y, double w, double h,
+                                               const BLAmdImage* image, const int32_t* area) {
+  logf("blit_scaled_image_d %.17g %.17g %.17g %.17g image=%u area=%s", x, y, w, h, serial_of(image),
+       area_of(area).c_str());
   return 0;
 }
 BLAmdResult bl_amd_context_fill_path_d_ext(BLAmdContext*, const double* o, const uint8_t* c, const double* v, size_t n, const BLAmdStyle* st) {

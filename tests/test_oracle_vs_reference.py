@@ -72,3 +72,10 @@ def test_translated_patterns(seed, fmt):
 def test_cancelling_pattern_transforms(seed, fmt):
     # user and pattern transforms that cancel: the near-identity test
     _same(S.cancelling_pattern_transforms_scene(seed, 200, 150, 40, fmt))
+
+
+@pytest.mark.parametrize("seed", [81, 82, 83, 84])
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_XRGB32, S.FMT_A8])
+def test_blits(seed, fmt):
+    # blit_image / blit_scaled_image in all four forms, areas, transforms
+    _same(S.blit_scene(seed, 257, 181, fmt, 140, bad_areas=seed == 84))
