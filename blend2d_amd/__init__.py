@@ -57,6 +57,9 @@ SYMBOLS = [
     "bl_amd_autotune_reset", "bl_amd_set_graph_replay",
     "bl_amd_context_blit_image_i", "bl_amd_context_blit_image_d", "bl_amd_context_blit_scaled_image_i",
     "bl_amd_context_blit_scaled_image_d",
+    "bl_amd_context_fill_mask_i", "bl_amd_context_fill_mask_d", "bl_amd_context_fill_mask_i_rgba32",
+    "bl_amd_context_fill_mask_d_rgba32", "bl_amd_context_fill_mask_i_rgba64", "bl_amd_context_fill_mask_d_rgba64",
+    "bl_amd_context_fill_mask_i_ext", "bl_amd_context_fill_mask_d_ext",
 ]
 
 SIZING_EXACT, SIZING_LEARNED, SIZING_HISTORY = 0, 1, 2
@@ -248,6 +251,14 @@ def bind(L):
         "bl_amd_context_blit_image_d": ([P, D, D, P, P], U32),
         "bl_amd_context_blit_scaled_image_i": ([P, I, I, I, I, P, P], U32),
         "bl_amd_context_blit_scaled_image_d": ([P, D, D, D, D, P, P], U32),
+        "bl_amd_context_fill_mask_i": ([P, I, I, P, P], U32),
+        "bl_amd_context_fill_mask_d": ([P, D, D, P, P], U32),
+        "bl_amd_context_fill_mask_i_rgba32": ([P, I, I, P, P, U32], U32),
+        "bl_amd_context_fill_mask_d_rgba32": ([P, D, D, P, P, U32], U32),
+        "bl_amd_context_fill_mask_i_rgba64": ([P, I, I, P, P, ctypes.c_uint64], U32),
+        "bl_amd_context_fill_mask_d_rgba64": ([P, D, D, P, P, ctypes.c_uint64], U32),
+        "bl_amd_context_fill_mask_i_ext": ([P, I, I, P, P, ctypes.POINTER(BLAmdStyle)], U32),
+        "bl_amd_context_fill_mask_d_ext": ([P, D, D, P, P, ctypes.POINTER(BLAmdStyle)], U32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -457,6 +468,31 @@ class BLContext:
         else:
             r = L.bl_amd_context_blit_scaled_image_d(self._h, *[float(v) for v in dst], image.handle, a)
         _check(r, "blit_image")
+
+    def fill_mask(self, dst, mask: "BLImage", area=None, integer: bool = False, rgba32: int | None = None, style=None):
+        """BLContext::fill_mask (core/context.h:800-808): the fill style (or
+        `rgba32` / `style` for this call) through the bytes of `mask`'s
+        `area` (x, y, w, h; None: all of it) placed at dst = (x, y);
+        `integer` takes the _i form."""
+        self._keep.append(mask)
+        a = None
+        if area is not None:
+            a = (ctypes.c_int32 * 4)(*[int(v) for v in area])
+        L = lib()
+        head = ((self._h, int(dst[0]), int(dst[1])) if integer else (self._h, float(dst[0]), float(dst[1])))
+        sfx = "i" if integer else "d"
+        if style is not None:
+            kind, v, skeep = self._style_arg(style)
+            if kind == "rgba64":
+                r = getattr(L, f"bl_amd_context_fill_mask_{sfx}_rgba64")(*head, mask.handle, a, v)
+            else:
+                r = getattr(L, f"bl_amd_context_fill_mask_{sfx}_ext")(*head, mask.handle, a, ctypes.byref(v))
+            del skeep
+        elif rgba32 is not None:
+            r = getattr(L, f"bl_amd_context_fill_mask_{sfx}_rgba32")(*head, mask.handle, a, rgba32 & 0xFFFFFFFF)
+        else:
+            r = getattr(L, f"bl_amd_context_fill_mask_{sfx}")(*head, mask.handle, a)
+        _check(r, "fill_mask")
 
     def clear_rect(self, x, y, w, h):
         if all(isinstance(v, (int, np.integer)) for v in (x, y, w, h)):

@@ -16,9 +16,9 @@
 // the context style, an explicit rgba32 / rgba64 colour or an explicit style
 // object), clear_*, the fill style slot (colours, BLGradient, BLPattern),
 // fill alpha, global alpha, comp op, fill rule, transforms and the pattern
-// quality hint, and image blits (blit_image / blit_scaled_image).  Every
-// other entry (stroking, text, masks, clipping,
-// save / restore) reports BL_ERROR_NOT_IMPLEMENTED.
+// quality hint, image blits (blit_image / blit_scaled_image) and mask
+// fills (fill_mask in all its forms).  Every other entry (stroking, text,
+// clipping, save / restore) reports BL_ERROR_NOT_IMPLEMENTED.
 
 #include <blend2d/blend2d.h>
 #include <blend2d/core/object_p.h>
@@ -311,6 +311,48 @@ BLResult BL_CDECL blit_scaled_image_d_impl(BLContextImpl* impl, const BLRect* r,
   });
 }
 
+// Mask fills: the mask image (uploaded once per context) and its area; an
+// explicit style is converted as for the other fills.
+
+BLResult BL_CDECL fill_mask_i_impl(BLContextImpl* impl, const BLPointI* o, const BLImageCore* mask, const BLRectI* area) noexcept {
+  return with_image(impl, mask, area, [o](BLAmdContext* g, BLAmdImage* m, const int32_t* a) {
+    return bl_amd_context_fill_mask_i(g, o->x, o->y, m, a);
+  });
+}
+BLResult BL_CDECL fill_mask_i_rgba32_impl(BLContextImpl* impl, const BLPointI* o, const BLImageCore* mask, const BLRectI* area,
+                                          uint32_t c) noexcept {
+  return with_image(impl, mask, area, [o, c](BLAmdContext* g, BLAmdImage* m, const int32_t* a) {
+    return bl_amd_context_fill_mask_i_rgba32(g, o->x, o->y, m, a, c);
+  });
+}
+BLResult BL_CDECL fill_mask_i_ext_impl(BLContextImpl* impl, const BLPointI* o, const BLImageCore* mask, const BLRectI* area,
+                                       const BLObjectCore* style) noexcept {
+  return with_image(impl, mask, area, [impl, o, style](BLAmdContext*, BLAmdImage* m, const int32_t* a) {
+    return with_style(impl, style,
+                      [o, m, a](BLAmdContext* g, const BLAmdStyle* s) { return bl_amd_context_fill_mask_i_ext(g, o->x, o->y, m, a, s); },
+                      [o, m, a](BLAmdContext* g, uint64_t c) { return bl_amd_context_fill_mask_i_rgba64(g, o->x, o->y, m, a, c); });
+  });
+}
+BLResult BL_CDECL fill_mask_d_impl(BLContextImpl* impl, const BLPoint* o, const BLImageCore* mask, const BLRectI* area) noexcept {
+  return with_image(impl, mask, area, [o](BLAmdContext* g, BLAmdImage* m, const int32_t* a) {
+    return bl_amd_context_fill_mask_d(g, o->x, o->y, m, a);
+  });
+}
+BLResult BL_CDECL fill_mask_d_rgba32_impl(BLContextImpl* impl, const BLPoint* o, const BLImageCore* mask, const BLRectI* area,
+                                          uint32_t c) noexcept {
+  return with_image(impl, mask, area, [o, c](BLAmdContext* g, BLAmdImage* m, const int32_t* a) {
+    return bl_amd_context_fill_mask_d_rgba32(g, o->x, o->y, m, a, c);
+  });
+}
+BLResult BL_CDECL fill_mask_d_ext_impl(BLContextImpl* impl, const BLPoint* o, const BLImageCore* mask, const BLRectI* area,
+                                       const BLObjectCore* style) noexcept {
+  return with_image(impl, mask, area, [impl, o, style](BLAmdContext*, BLAmdImage* m, const int32_t* a) {
+    return with_style(impl, style,
+                      [o, m, a](BLAmdContext* g, const BLAmdStyle* s) { return bl_amd_context_fill_mask_d_ext(g, o->x, o->y, m, a, s); },
+                      [o, m, a](BLAmdContext* g, uint64_t c) { return bl_amd_context_fill_mask_d_rgba64(g, o->x, o->y, m, a, c); });
+  });
+}
+
 BLResult BL_CDECL clear_all_impl(BLContextImpl* impl) noexcept { return result_of(bl_amd_context_clear_all(self_of(impl)->gpu)); }
 BLResult BL_CDECL clear_recti_impl(BLContextImpl* impl, const BLRectI* r) noexcept {
   return result_of(bl_amd_context_clear_rect_i(self_of(impl)->gpu, r->x, r->y, r->w, r->h));
@@ -473,12 +515,12 @@ BLContextVirt make_virt() {
   not_implemented(v.fill_text_op_d);
   not_implemented(v.fill_text_op_d_rgba32);
   not_implemented(v.fill_text_op_d_ext);
-  not_implemented(v.fill_mask_i);
-  not_implemented(v.fill_mask_i_rgba32);
-  not_implemented(v.fill_mask_i_ext);
-  not_implemented(v.fill_mask_d);
-  not_implemented(v.fill_mask_d_Rgba32);
-  not_implemented(v.fill_mask_d_ext);
+  v.fill_mask_i = fill_mask_i_impl;
+  v.fill_mask_i_rgba32 = fill_mask_i_rgba32_impl;
+  v.fill_mask_i_ext = fill_mask_i_ext_impl;
+  v.fill_mask_d = fill_mask_d_impl;
+  v.fill_mask_d_Rgba32 = fill_mask_d_rgba32_impl;
+  v.fill_mask_d_ext = fill_mask_d_ext_impl;
   not_implemented(v.stroke_path_d);
   not_implemented(v.stroke_path_d_rgba32);
   not_implemented(v.stroke_path_d_ext);

@@ -351,6 +351,7 @@ static void pack_range(uint8_t* st, const FrameHost& fh, uint32_t k, const PartB
       for (size_t i = lo; i < hi; i++) {
         DevCmd c = fh.cmds[i];
         c.style += (uint32_t)b.styles;
+        if (c.kind == kCmdBoxM) c.box_mask[0] += (uint32_t)b.images;
         c.canvas = k;
         out[i] = c;
       }
@@ -1870,6 +1871,65 @@ uint32_t blit_scaled_i(BLAmdContext* c, int x, int y, int w, int h, const BLAmdI
   return fill_box_d(c, r, (double)x, (double)y, (double)x + (double)w, (double)y + (double)h);
 }
 
+// ---- mask fills (rastercontext.cpp:3590-3780) ----------------------------
+//
+// The reference renders a mask only pixel-aligned under a translation: a box
+// whose pixels take their coverage from bytes of the mask area's rows
+// (fill_clipped_box_masked_a); any other placement is NOT_IMPLEMENTED there
+// (fill_unclipped_mask_d), and so here.
+
+// The box x0..x1 x y0..y1 with the mask area at (mx, my) on its top-left
+// pixel: the area's view is exactly the box.
+uint32_t fill_box_masked(BLAmdContext* c, const Resolved& r, int x0, int y0, int x1, int y1, const BLAmdImage* mask,
+                         int mx, int my) {
+  if (r.nop) return BL_AMD_SUCCESS;
+  if (r.status) return r.status;
+  DevCmd cmd = r.cmd;
+  cmd.kind = kCmdBoxM;
+  cmd.box[0] = x0; cmd.box[1] = y0; cmd.box[2] = x1; cmd.box[3] = y1;
+  cmd.fetch_row0 = (uint32_t)y0;
+  const int32_t view[4] = { mx, my, x1 - x0, y1 - y0 };
+  cmd.box_mask[0] = image_view_index(c, mask, view);
+  push_cmd(c, cmd, nullptr);
+  return BL_AMD_SUCCESS;
+}
+
+// fill_mask_d_impl + fill_unclipped_mask_d (rastercontext.cpp:3594-3690).
+uint32_t fill_mask_d(BLAmdContext* c, const Resolved& r, double x, double y, const BLAmdImage* mask, const int32_t* area) {
+  int32_t src[4];
+  if (!blit_area(mask, area, src)) return kErrInvalid;
+  if (r.nop) return BL_AMD_SUCCESS;
+  if (transform_type(c->user) > kTTTranslate) return kErrNotImpl;
+  const CmdXform m = fixed_xform(c);
+  const double sx = x * m.m00 + m.m20, sy = y * m.m11 + m.m21;
+  const double dx0 = bl_max_d(sx, 0.0), dy0 = bl_max_d(sy, 0.0);
+  const double dx1 = bl_min_d(sx + (double)src[2] * m.m00, (double)c->target->w * 256.0);
+  const double dy1 = bl_min_d(sy + (double)src[3] * m.m11, (double)c->target->h * 256.0);
+  // clipped out, invalid coordinates or an empty area
+  if (!((dx0 < dx1) & (dy0 < dy1))) return BL_AMD_SUCCESS;
+  const int64_t fx = floor_to_int64(sx), fy = floor_to_int64(sy);
+  if ((fx | fy) & 0xFF) return kErrNotImpl;
+  const int ix0 = (int)dx0, iy0 = (int)dy0, ix1 = (int)dx1, iy1 = (int)dy1;
+  const int x0 = ix0 >> 8, y0 = iy0 >> 8, x1 = (ix1 + 0xFF) >> 8, y1 = (iy1 + 0xFF) >> 8;
+  if (((ix0 | iy0 | ix1 | iy1) & 0xFF) != 0) return kErrNotImpl;
+  const int tx = (int)(fx >> 8), ty = (int)(fy >> 8);
+  return fill_box_masked(c, r, x0, y0, x1, y1, mask, src[0] + x0 - tx, src[1] + y0 - ty);
+}
+
+// fill_mask_i_impl (rastercontext.cpp:3723-3743): an integral translation
+// clips in integers (translate_and_clip_rect_to_blit_i), else fill_mask_d.
+uint32_t fill_mask_i(BLAmdContext* c, const Resolved& r, int x, int y, const BLAmdImage* mask, const int32_t* area) {
+  int tx = 0, ty = 0;
+  if (!integral_translation(c, tx, ty)) return fill_mask_d(c, r, (double)x, (double)y, mask, area);
+  int32_t src[4];
+  if (!blit_area(mask, area, src)) return kErrInvalid;
+  const int64_t dx = (int64_t)x + tx, dy = (int64_t)y + ty;
+  const int64_t x0 = std::max<int64_t>(dx, 0), y0 = std::max<int64_t>(dy, 0);
+  const int64_t x1 = std::min<int64_t>(dx + src[2], c->target->w), y1 = std::min<int64_t>(dy + src[3], c->target->h);
+  if (x0 >= x1 || y0 >= y1) return BL_AMD_SUCCESS;
+  return fill_box_masked(c, r, (int)x0, (int)y0, (int)x1, (int)y1, mask, src[0] + (int)(x0 - dx), src[1] + (int)(y0 - dy));
+}
+
 // fill_all (rastercontext.cpp:2720-2725) with the canvas as the clip box.
 uint32_t fill_all_box(BLAmdContext* c, const Resolved& r) {
   return fill_box_a(c, r, 0, 0, c->target->w, c->target->h);
@@ -2292,6 +2352,64 @@ BLAmdResult bl_amd_context_fill_rect_d(BLAmdContext* ctx, double x, double y, do
 BLAmdResult bl_amd_context_fill_rect_d_rgba32(BLAmdContext* ctx, double x, double y, double w, double h, uint32_t rgba32) {
   if (uint32_t r = render_call(ctx)) return r;
   return fill_box_d(ctx, resolve_solid(ctx, rgba32), x, y, x + w, y + h);
+}
+
+BLAmdResult bl_amd_context_fill_mask_i(BLAmdContext* ctx, int x, int y, const BLAmdImage* mask, const int32_t* area) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!mask) return kErrInvalid;
+  return fill_mask_i(ctx, resolve_fill(ctx), x, y, mask, area);
+}
+
+BLAmdResult bl_amd_context_fill_mask_d(BLAmdContext* ctx, double x, double y, const BLAmdImage* mask, const int32_t* area) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!mask) return kErrInvalid;
+  return fill_mask_d(ctx, resolve_fill(ctx), x, y, mask, area);
+}
+
+BLAmdResult bl_amd_context_fill_mask_i_rgba32(BLAmdContext* ctx, int x, int y, const BLAmdImage* mask,
+                                              const int32_t* area, uint32_t rgba32) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!mask) return kErrInvalid;
+  return fill_mask_i(ctx, resolve_solid(ctx, rgba32), x, y, mask, area);
+}
+
+BLAmdResult bl_amd_context_fill_mask_d_rgba32(BLAmdContext* ctx, double x, double y, const BLAmdImage* mask,
+                                              const int32_t* area, uint32_t rgba32) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!mask) return kErrInvalid;
+  return fill_mask_d(ctx, resolve_solid(ctx, rgba32), x, y, mask, area);
+}
+
+BLAmdResult bl_amd_context_fill_mask_i_rgba64(BLAmdContext* ctx, int x, int y, const BLAmdImage* mask,
+                                              const int32_t* area, uint64_t rgba64) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!mask) return kErrInvalid;
+  return fill_mask_i(ctx, resolve_solid(ctx, rgba32_from_rgba64(rgba64)), x, y, mask, area);
+}
+
+BLAmdResult bl_amd_context_fill_mask_d_rgba64(BLAmdContext* ctx, double x, double y, const BLAmdImage* mask,
+                                              const int32_t* area, uint64_t rgba64) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!mask) return kErrInvalid;
+  return fill_mask_d(ctx, resolve_solid(ctx, rgba32_from_rgba64(rgba64)), x, y, mask, area);
+}
+
+BLAmdResult bl_amd_context_fill_mask_i_ext(BLAmdContext* ctx, int x, int y, const BLAmdImage* mask,
+                                           const int32_t* area, const BLAmdStyle* style) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!mask) return kErrInvalid;
+  uint32_t err;
+  const Resolved res = resolve_explicit(ctx, style, true, err);
+  return err ? err : fill_mask_i(ctx, res, x, y, mask, area);
+}
+
+BLAmdResult bl_amd_context_fill_mask_d_ext(BLAmdContext* ctx, double x, double y, const BLAmdImage* mask,
+                                           const int32_t* area, const BLAmdStyle* style) {
+  if (uint32_t r = render_call(ctx)) return r;
+  if (!mask) return kErrInvalid;
+  uint32_t err;
+  const Resolved res = resolve_explicit(ctx, style, true, err);
+  return err ? err : fill_mask_d(ctx, res, x, y, mask, area);
 }
 
 BLAmdResult bl_amd_context_blit_image_i(BLAmdContext* ctx, int x, int y, const BLAmdImage* image, const int32_t* area) {

@@ -50,6 +50,9 @@ enum CmdKind : uint32_t {
   kCmdAnalytic = 0,   // edges -> cells -> coverage
   kCmdBoxA = 1,       // pixel-aligned box, constant mask
   kCmdBoxU = 2,       // 24.8 box, border-coverage masks
+  kCmdBoxM = 3,       // pixel-aligned box, per-pixel masks read from a mask
+                      // image (box_mask[0]: its image index, the view's row 0
+                      // and byte 0 at the box's top-left pixel)
 };
 
 // Why a fill command failed on the device (bits, per command).

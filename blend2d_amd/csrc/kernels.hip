@@ -970,13 +970,29 @@ __device__ __forceinline__ void store4(uint8_t* rowp, int px, int w, bool a8, co
 }
 
 // Masks of pixels px .. px + 3 of row y for a box fill.
-__device__ __forceinline__ void box_masks4(const DevCmd& c, int y, int px, uint32_t m[4]) {
+__device__ __forceinline__ void box_masks4(const FrameDev& f, const DevCmd& c, int y, int px, uint32_t m[4]) {
 #pragma unroll
   for (int k = 0; k < 4; k++) m[k] = 0u;
   if (y < c.box[1] || y >= c.box[3]) return;
   if (c.kind == kCmdBoxA) {
 #pragma unroll
     for (int k = 0; k < 4; k++) m[k] = (px + k >= c.box[0] && px + k < c.box[2]) ? c.alpha : 0u;
+  }
+  else if (c.kind == kCmdBoxM) {
+    // FillMask with a VMaskA8 command (rendercommandprocsync_p.h:65-84,
+    // compopgeneric_p.h:165-183): byte x - x0 of the mask row, times the
+    // alpha below full alpha.  The view is exactly the box (host-checked), so
+    // every byte read lies in row y - y0 of the view.
+    const DevImage& mi = f.images[c.box_mask[0]];
+    const uint8_t* row = (const uint8_t*)mi.ptr + (int64_t)(y - c.box[1]) * mi.stride;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int x = px + k;
+      if (x >= c.box[0] && x < c.box[2]) {
+        const uint32_t v = row[x - c.box[0]];
+        m[k] = c.alpha >= 255u ? v : pipe::div255(v * c.alpha);
+      }
+    }
   }
   else {
     const int kind = (y == c.box_row_first) ? 0 : (y == c.box_row_last) ? 2 : 1;
@@ -1256,7 +1272,7 @@ __device__ __forceinline__ void render_tile(const FrameDev& f, const FrameScratc
           m[3] = calc_mask(c3, c.rule_mask, c.alpha);
         }
         else {
-          box_masks4(c, y, px, m);
+          box_masks4(f, c, y, px, m);
         }
 
         if (row_ok && (m[0] | m[1] | m[2] | m[3]) != 0) {
@@ -1726,7 +1742,7 @@ __device__ __forceinline__ void render_box(const FrameDev& f, const FrameScratch
         for (int g = 0; g < 4; g++) {
           uint32_t m[4];
           const int px = qx(g);
-          box_masks4(c, (int)band * kTileH + qrow(g), px, m);
+          box_masks4(f, c, (int)band * kTileH + qrow(g), px, m);
           if (qrow(g) < rows) npix += composite4<kSolid>(f, c, st, rs, px, w, a8, m, d[0]);
 #pragma unroll
           for (int k = 0; k < 4; k++) {

@@ -71,8 +71,22 @@ enum SceneOp {
   SCENE_OP_FILL_RECT_D_RGBA32 = 23, /* f64 x, y, w, h, u32 rgba32, u32 pad */
   SCENE_OP_FILL_PATH_EX       = 24, /* ScenePathEx, u8 cmds[n] padded to 8, f64 vtx[2n] */
   SCENE_OP_FILL_EXPLICIT      = 25, /* SceneExplicit, style payload, target payload (see below) */
-  SCENE_OP_BLIT_IMAGE         = 26  /* SceneBlit */
+  SCENE_OP_BLIT_IMAGE         = 26, /* SceneBlit */
+  SCENE_OP_FILL_MASK          = 27  /* SceneMask */
 };
+
+/* FILL_MASK: fill_mask_i / _d (kinds 0, 1) with the context's fill style of
+ * image `image_index` as the mask, its `area` (x, y, w, h) when has_area,
+ * else the whole image, placed at dst = x, y (integers for kind 0).  The
+ * same struct is the target payload of FILL_EXPLICIT target 5. */
+typedef struct SceneMask {
+  uint32_t kind;
+  uint32_t image_index;
+  uint32_t has_area;
+  uint32_t pad;
+  int32_t area[4];
+  double dst[2];
+} SceneMask;
 
 /* BLIT_IMAGE: blit_image_i / _d (kinds 0, 1: dst = x, y) and
  * blit_scaled_image_i / _d (kinds 2, 3: dst = x, y, w, h) of image
@@ -95,9 +109,9 @@ typedef struct SceneBlit {
  * (fill_all) nothing, 1 (fill_rect_i) i32 x, y, w, h, 2 (fill_rect_d)
  * f64 x, y, w, h, 3 (fill_path_d) u8 cmds[count] padded to 8 and
  * f64 vtx[2 count], 4 (fill_geometry) the geometry data as FILL_GEOMETRY
- * (struct, or `count` array elements). */
+ * (struct, or `count` array elements), 5 (fill_mask) a SceneMask. */
 typedef struct SceneExplicit {
-  uint32_t target;        /* 0 all, 1 rect_i, 2 rect_d, 3 path, 4 geometry */
+  uint32_t target;        /* 0 all, 1 rect_i, 2 rect_d, 3 path, 4 geometry, 5 mask */
   uint32_t style_kind;    /* 0 rgba32, 1 rgba64, 2 gradient, 3 pattern */
   uint32_t geom_type;     /* target 4: BLGeometryType */
   uint32_t count;         /* target 3: vertices; target 4: array elements */

@@ -57,7 +57,9 @@ struct Call {
   std::vector<uint32_t> stop_rgba;
   size_t image = 0;
   BLAmdStyle style{};
-  bool has_area = false;          // BLIT_IMAGE: area in i[], kind in u0, destination in d[0..3]
+  bool has_area = false;          // BLIT_IMAGE / FILL_MASK: area in i[], kind in u0, destination in d[]
+  size_t mask = 0;                // FILL_MASK / FILL_EXPLICIT target 5: the mask image (`image` may be
+                                  // the explicit style's pattern)
 };
 
 struct Scene {
@@ -80,11 +82,25 @@ bool is_render_op(uint32_t op) {
     case SCENE_OP_FILL_ALL: case SCENE_OP_CLEAR_ALL: case SCENE_OP_CLEAR_RECT_I: case SCENE_OP_CLEAR_RECT_D:
     case SCENE_OP_FILL_GEOMETRY: case SCENE_OP_FILL_ALL_RGBA32: case SCENE_OP_FILL_RECT_I_RGBA32:
     case SCENE_OP_FILL_RECT_D_RGBA32: case SCENE_OP_FILL_PATH_EX: case SCENE_OP_FILL_EXPLICIT:
-    case SCENE_OP_BLIT_IMAGE:
+    case SCENE_OP_BLIT_IMAGE: case SCENE_OP_FILL_MASK:
       return true;
     default:
       return false;
   }
+}
+
+// SceneMask at p into `c` (kind in u0, area in i[], x, y in d[0..1]).
+bool decode_mask(Call& c, const uint8_t* p, const uint8_t* end, size_t n_images) {
+  SceneMask m;
+  if (size_t(end - p) < sizeof(m)) return false;
+  std::memcpy(&m, p, sizeof(m));
+  if (m.image_index >= n_images) return false;
+  c.u0 = m.kind;
+  c.mask = m.image_index;
+  c.has_area = m.has_area != 0;
+  std::memcpy(c.i, m.area, 16);
+  std::memcpy(c.d, m.dst, 16);
+  return true;
 }
 
 // SceneGradient + stops at p into `c`; returns the bytes used (0: malformed).
@@ -250,6 +266,9 @@ uint32_t decode(Scene& sc) {
         std::memcpy(c.d, b.dst, 32);
         break;
       }
+      case SCENE_OP_FILL_MASK:
+        if (!decode_mask(c, pl, pe, sc.images.size())) return kBadScene;
+        break;
       case SCENE_OP_FILL_EXPLICIT: {
         SceneExplicit ex;
         std::memcpy(&ex, pl, sizeof(ex));
@@ -287,6 +306,9 @@ uint32_t decode(Scene& sc) {
             else {
               c.geom = q < pe ? q : nullptr;
             }
+            break;
+          case 5:
+            if (!decode_mask(c, q, pe, sc.images.size())) return kBadScene;
             break;
           default: break;
         }
@@ -374,6 +396,13 @@ void issue(BLAmdContext* ctx, const Scene& sc, Issued& out) {
         }
         break;
       }
+      case SCENE_OP_FILL_MASK: {
+        const int32_t* area = c.has_area ? c.i : nullptr;
+        BLAmdImage* mask = sc.images[c.mask];
+        r = c.u0 == 0 ? bl_amd_context_fill_mask_i(ctx, int(c.d[0]), int(c.d[1]), mask, area)
+                      : bl_amd_context_fill_mask_d(ctx, c.d[0], c.d[1], mask, area);
+        break;
+      }
       case SCENE_OP_FILL_RECT_I_RGBA32:
         r = bl_amd_context_fill_rect_i_rgba32(ctx, c.i[0], c.i[1], c.i[2], c.i[3], c.u0);
         break;
@@ -411,6 +440,19 @@ void issue(BLAmdContext* ctx, const Scene& sc, Issued& out) {
             r = r64 ? bl_amd_context_fill_geometry_rgba64(ctx, c.geom_type, c.geom, c.u64)
                     : bl_amd_context_fill_geometry_ext(ctx, c.geom_type, c.geom, st);
             break;
+          case 5: {
+            const int32_t* area = c.has_area ? c.i : nullptr;
+            BLAmdImage* mask = sc.images[c.mask];
+            if (c.u0 == 0) {
+              r = r64 ? bl_amd_context_fill_mask_i_rgba64(ctx, int(c.d[0]), int(c.d[1]), mask, area, c.u64)
+                      : bl_amd_context_fill_mask_i_ext(ctx, int(c.d[0]), int(c.d[1]), mask, area, st);
+            }
+            else {
+              r = r64 ? bl_amd_context_fill_mask_d_rgba64(ctx, c.d[0], c.d[1], mask, area, c.u64)
+                      : bl_amd_context_fill_mask_d_ext(ctx, c.d[0], c.d[1], mask, area, st);
+            }
+            break;
+          }
           default: break;
         }
         break;

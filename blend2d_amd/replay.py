@@ -24,7 +24,13 @@ def _images(ctx_cls, sc):
 RENDER_OPS = {S.OP_FILL_PATH, S.OP_FILL_POLYGON, S.OP_FILL_RECT_D, S.OP_FILL_RECT_I, S.OP_FILL_ALL,
               S.OP_CLEAR_ALL, S.OP_CLEAR_RECT_I, S.OP_CLEAR_RECT_D, S.OP_FILL_GEOMETRY, S.OP_FILL_ALL_RGBA32,
               S.OP_FILL_RECT_I_RGBA32, S.OP_FILL_RECT_D_RGBA32, S.OP_FILL_PATH_EX, S.OP_FILL_EXPLICIT,
-              S.OP_BLIT_IMAGE}
+              S.OP_BLIT_IMAGE, S.OP_FILL_MASK}
+
+
+def _mask(pl, at=0):
+    """Decodes a SceneMask: (integer, image index, dst, area)."""
+    kind, index, has_area, _, ax, ay, aw, ah, dx, dy = struct.unpack_from("<4I4i2d", pl, at)
+    return kind == S.MASK_I, index, (dx, dy), ((ax, ay, aw, ah) if has_area else None)
 
 
 def _explicit(pl, images):
@@ -54,6 +60,8 @@ def _explicit(pl, images):
         cmds = np.frombuffer(pl, dtype=np.uint8, count=count, offset=at)
         vtx = np.frombuffer(pl, dtype=np.float64, count=2 * count, offset=at + ((count + 7) & ~7)).reshape(-1, 2)
         geom = (cmds, vtx)
+    elif target == S.TARGET_MASK:
+        geom = _mask(pl, at)
     elif target == S.TARGET_GEOMETRY:
         if gtype in S.GEOMETRY_ARRAY:
             fmt = S.GEOMETRY_ARRAY[gtype]
@@ -159,6 +167,9 @@ def replay_into(ctx, sc: "S.Scene", images) -> list:
                 call(lambda: ctx.fill_rect(*geom, style=style))
             elif target == S.TARGET_PATH:
                 call(lambda: ctx.fill_path(geom, style=style))
+            elif target == S.TARGET_MASK:
+                integer, index, dst, area = geom
+                call(lambda: ctx.fill_mask(dst, images[index], area, integer, style=style))
             else:
                 call(lambda: ctx.fill_geometry(geom[0], geom[1], style=style))
         elif op == S.OP_BLIT_IMAGE:
@@ -167,6 +178,9 @@ def replay_into(ctx, sc: "S.Scene", images) -> list:
             dst = (dx, dy) if kind in (S.BLIT_I, S.BLIT_D) else (dx, dy, dw, dh)
             integer = kind in (S.BLIT_I, S.BLIT_SCALED_I)
             call(ctx.blit_image, dst, images[index], area, integer)
+        elif op == S.OP_FILL_MASK:
+            integer, index, dst, area = _mask(pl)
+            call(ctx.fill_mask, dst, images[index], area, integer)
         elif op == S.OP_FILL_PATH_EX:
             n, flags, col, _, ox, oy = struct.unpack_from("<4I2d", pl)
             cmds = np.frombuffer(pl, dtype=np.uint8, count=n, offset=32)

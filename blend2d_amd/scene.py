@@ -50,9 +50,11 @@ OP_FILL_PATH_EX = 24
 OP_FILL_EXPLICIT = 25
 OP_BLIT_IMAGE = 26
 BLIT_I, BLIT_D, BLIT_SCALED_I, BLIT_SCALED_D = 0, 1, 2, 3
+OP_FILL_MASK = 27
+MASK_I, MASK_D = 0, 1
 
 # FILL_EXPLICIT targets and style kinds
-TARGET_ALL, TARGET_RECT_I, TARGET_RECT_D, TARGET_PATH, TARGET_GEOMETRY = range(5)
+TARGET_ALL, TARGET_RECT_I, TARGET_RECT_D, TARGET_PATH, TARGET_GEOMETRY, TARGET_MASK = range(6)
 STYLE_RGBA32, STYLE_RGBA64, STYLE_GRADIENT, STYLE_PATTERN = range(4)
 
 # BLGeometryType values (core/geometry.h).
@@ -182,6 +184,19 @@ class Scene:
         self._cmd(OP_BLIT_IMAGE, struct.pack("<4I4i4d", kind, image_index, 0 if area is None else 1, 0, *a, *d))
         self.fill_count += 1
 
+    @staticmethod
+    def _mask_payload(kind: int, image_index: int, dst, area) -> bytes:
+        d = [float(int(v)) if kind == MASK_I else float(v) for v in dst]
+        a = list(area) if area is not None else [0, 0, 0, 0]
+        return struct.pack("<4I4i2d", kind, image_index, 0 if area is None else 1, 0, *a, *d)
+
+    def fill_mask(self, kind: int, image_index: int, dst, area=None) -> None:
+        """fill_mask_i / _d (kind MASK_I / MASK_D) with the fill style: image
+        `image_index` (its `area` x, y, w, h, or all of it) as the coverage,
+        placed at dst = (x, y) (scene_format.h: SceneMask)."""
+        self._cmd(OP_FILL_MASK, self._mask_payload(kind, image_index, dst, area))
+        self.fill_count += 1
+
     def set_fill_alpha(self, a: float) -> None:
         self._cmd(OP_SET_FILL_ALPHA, struct.pack("<d", a))
 
@@ -272,6 +287,9 @@ class Scene:
             vtx = np.asarray(geometry[1], dtype=np.float64).reshape(-1, 2)
             count = len(cmds)
             body += _pad8(cmds.tobytes()) + vtx.tobytes()
+        elif target == TARGET_MASK:
+            kind_m, index, dst, area = geometry
+            body += self._mask_payload(kind_m, index, dst, area)
         elif target == TARGET_GEOMETRY:
             gtype, data = geometry
             if gtype in GEOMETRY_ARRAY:
@@ -1199,6 +1217,121 @@ def blit_scene(seed: int = 1, width: int = 257, height: int = 181, fmt: int = FM
                 sc.blit_image(kind, imgs[k], (int(x), int(y), int(w), int(h)), area)
             else:
                 sc.blit_image(kind, imgs[k], (x, y, w, h), area)
+    sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+    return sc
+
+
+def mask_scene(seed: int = 1, width: int = 241, height: int = 173, fmt: int = FMT_PRGB32, count: int = 90,
+               unaligned: bool = False) -> Scene:
+    """fill_mask_i / _d (core/context.h:800-808) with the fill style (solid,
+    linear / radial gradient, pattern) and with a style for the call alone
+    (rgba32, rgba64, gradient, pattern): A8 masks (noise, smooth ramps, one
+    pixel wide / high), and a PRGB32 and an XRGB32 image used as masks (the
+    reference reads their bytes as an A8 row); whole masks and areas,
+    partly or wholly off the canvas; identity and integral translations, and
+    fractional origins whose translation makes them pixel-aligned; SRC_OVER
+    / SRC_COPY, global and fill alpha (full alpha and below).  With
+    `unaligned`, also the placements the reference reports as
+    BL_ERROR_NOT_IMPLEMENTED (a fractional start, a rotation or scale), global
+    alpha 0 over such placements (a NOP, no error), and areas outside the mask
+    (BL_ERROR_INVALID_VALUE)."""
+    rng = np.random.default_rng(seed)
+    clear = 0xFF000000 | int(rng.integers(0, 1 << 24)) if fmt != FMT_A8 else int(rng.integers(0, 256))
+    sc = Scene(width, height, fmt, clear)
+    yy, xx = np.mgrid[0:40, 0:57]
+    ramp = ((xx * 4 + yy * 3) % 256).astype(np.uint8)
+    masks = [sc.add_image(rng.integers(0, 256, size=(45, 61), dtype=np.uint8), FMT_A8),
+             sc.add_image(ramp, FMT_A8),
+             sc.add_image(rng.integers(0, 256, size=(1, 30), dtype=np.uint8), FMT_A8),
+             sc.add_image(rng.integers(0, 256, size=(25, 1), dtype=np.uint8), FMT_A8),
+             sc.add_image(premultiplied_noise(rng, 33, 21)),
+             sc.add_image(rng.integers(0, 1 << 32, size=(17, 29), dtype=np.uint32), FMT_XRGB32)]
+    sizes = [(61, 45), (57, 40), (30, 1), (1, 25), (33, 21), (29, 17)]
+    pattern = sc.add_image(premultiplied_noise(rng, 23, 19))
+    W, H = float(width), float(height)
+
+    def style():
+        k = rng.random()
+        if k < 0.4:
+            return ("rgba32", _rand_rgba(rng))
+        if k < 0.55:
+            return ("rgba64", int(rng.integers(0, 1 << 63)) * 2 + int(rng.integers(0, 2)))
+        if k < 0.8:
+            x0, y0 = float(rng.uniform(0, W)), float(rng.uniform(0, H))
+            if rng.random() < 0.5:
+                return ("gradient", GRADIENT_LINEAR, [x0, y0, x0 + float(rng.uniform(-90, 90)), y0 + float(rng.uniform(-90, 90))],
+                        _rand_stops(rng, 2, 4), int(rng.integers(0, 3)), None)
+            return ("gradient", GRADIENT_RADIAL, [x0, y0, x0 + 5.0, y0 - 3.0, float(rng.uniform(20, 90)), 0.0],
+                    _rand_stops(rng, 2, 4), int(rng.integers(0, 3)), None)
+        return ("pattern", pattern, int(rng.integers(0, 3)),
+                [1.0, 0.0, 0.0, 1.0, float(rng.integers(-30, 30)), float(rng.integers(-30, 30))] if rng.random() < 0.5 else None)
+
+    translation = (0.0, 0.0)
+    for i in range(count):
+        r = rng.random()
+        if r < 0.07:
+            t = rng.random()
+            if t < 0.4:
+                translation = (float(rng.integers(-15, 16)), float(rng.integers(-15, 16)))
+            elif t < 0.6:
+                translation = (0.5, -0.25)   # fractional: aligned again by origins x.5 / y.25
+            elif t < 0.8 or not unaligned:
+                translation = (0.0, 0.0)
+            if unaligned and t >= 0.8:
+                sc.set_transform(rotation_about(float(rng.uniform(0, 6.3)), W / 2, H / 2) if t < 0.9 else
+                                 [1.5, 0.0, 0.0, 1.5, 0.0, 0.0])
+                translation = None
+            else:
+                sc.set_transform([1.0, 0.0, 0.0, 1.0, translation[0], translation[1]])
+            continue
+        if r < 0.1:
+            sc.set_comp_op(int(rng.choice([COMP_OP_SRC_OVER, COMP_OP_SRC_COPY])))
+            continue
+        if r < 0.14:
+            a = rng.random()
+            sc.set_global_alpha(0.0 if (a < 0.1 and unaligned) else 1.0 if a < 0.6 else float(rng.uniform(0.05, 1.0)))
+            continue
+        if r < 0.17:
+            sc.set_fill_alpha(1.0 if rng.random() < 0.5 else float(rng.uniform(0.05, 1.0)))
+            continue
+        if r < 0.25:
+            s_ = style()
+            if s_[0] == "rgba32":
+                sc.set_fill_style_rgba32(s_[1])
+            elif s_[0] == "rgba64":
+                sc.set_fill_style_rgba64(s_[1])
+            elif s_[0] == "gradient":
+                sc.set_fill_style_gradient(s_[1], s_[2], s_[3], s_[4], s_[5])
+            else:
+                sc.set_fill_style_pattern(s_[1], s_[2], s_[3])
+            continue
+        k = int(rng.integers(0, len(masks)))
+        mw, mh = sizes[k]
+        area = None
+        a = rng.random()
+        if a < 0.35:
+            ax, ay = int(rng.integers(0, mw)), int(rng.integers(0, mh))
+            area = (ax, ay, int(rng.integers(0, mw - ax + 1)), int(rng.integers(0, mh - ay + 1)))
+        elif a < 0.4 and unaligned:
+            area = (int(rng.integers(-2, mw + 2)), int(rng.integers(-2, mh + 2)), mw, mh)
+            if area[0] == 0 and area[1] == 0:
+                area = (0, 1, mw, mh)
+        x, y = int(rng.integers(int(-0.2 * W), int(W))), int(rng.integers(int(-0.2 * H), int(H)))
+        kind = MASK_I if rng.random() < 0.5 else MASK_D
+        dst = (x, y)
+        if kind == MASK_D:
+            if translation == (0.5, -0.25):
+                dst = (x + 0.5, y + 0.25)
+            elif unaligned and rng.random() < 0.3:
+                dst = (x + float(rng.uniform(0.01, 0.99)), float(y))
+            else:
+                dst = (float(x), float(y))
+        elif translation == (0.5, -0.25) and not unaligned:
+            kind, dst = MASK_D, (x + 0.5, y + 0.25)
+        if rng.random() < 0.4:
+            sc.fill_explicit(TARGET_MASK, style(), (kind, masks[k], dst, area))
+        else:
+            sc.fill_mask(kind, masks[k], dst, area)
     sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
     return sc
 

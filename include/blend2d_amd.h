@@ -252,6 +252,33 @@ BLAmdResult bl_amd_context_blit_scaled_image_i(BLAmdContext* ctx, int x, int y, 
 BLAmdResult bl_amd_context_blit_scaled_image_d(BLAmdContext* ctx, double x, double y, double w, double h,
                                                const BLAmdImage* image, const int32_t* area);
 
+/* bl_context_fill_mask_i / _d and their _rgba32 / _rgba64 / _ext forms
+ * (context.h:800-808, raster/rastercontext.cpp:3590-3780): fills the
+ * pixels of `area` (x, y, w, h of `mask`; NULL: all of it) placed with its
+ * top-left corner at (x, y) with the fill style (or the call's own colour /
+ * style), each pixel's coverage being byte x of the mask area's row (an A8
+ * mask's value; for a 32-bit mask the reference reads its bytes as an A8
+ * row too), times the fill alpha.  As in the reference, only pixel-aligned
+ * placements under a translation render: a fractional start or any other
+ * transform is BL_AMD_ERROR_NOT_IMPLEMENTED (a fill that renders nothing
+ * anyway returns success), an area outside the mask
+ * BL_AMD_ERROR_INVALID_VALUE. */
+BLAmdResult bl_amd_context_fill_mask_i(BLAmdContext* ctx, int x, int y, const BLAmdImage* mask, const int32_t* area);
+BLAmdResult bl_amd_context_fill_mask_d(BLAmdContext* ctx, double x, double y, const BLAmdImage* mask,
+                                       const int32_t* area);
+BLAmdResult bl_amd_context_fill_mask_i_rgba32(BLAmdContext* ctx, int x, int y, const BLAmdImage* mask,
+                                              const int32_t* area, uint32_t rgba32);
+BLAmdResult bl_amd_context_fill_mask_d_rgba32(BLAmdContext* ctx, double x, double y, const BLAmdImage* mask,
+                                              const int32_t* area, uint32_t rgba32);
+BLAmdResult bl_amd_context_fill_mask_i_rgba64(BLAmdContext* ctx, int x, int y, const BLAmdImage* mask,
+                                              const int32_t* area, uint64_t rgba64);
+BLAmdResult bl_amd_context_fill_mask_d_rgba64(BLAmdContext* ctx, double x, double y, const BLAmdImage* mask,
+                                              const int32_t* area, uint64_t rgba64);
+BLAmdResult bl_amd_context_fill_mask_i_ext(BLAmdContext* ctx, int x, int y, const BLAmdImage* mask,
+                                           const int32_t* area, const BLAmdStyle* style);
+BLAmdResult bl_amd_context_fill_mask_d_ext(BLAmdContext* ctx, double x, double y, const BLAmdImage* mask,
+                                           const int32_t* area, const BLAmdStyle* style);
+
 /* ---- statistics and replay (no counterpart in the reference) ----------- */
 
 /* Counters of the last flush or replay. */

@@ -153,5 +153,7 @@ uint32_t orc_band_height(uint32_t width);
 void orc_fill_analytic(OrcCanvas* c, OrcEdges* e, OrcPipe* p, uint32_t alpha, uint32_t fill_rule);
 void orc_fill_box_a(OrcCanvas* c, OrcPipe* p, uint32_t alpha, int x0, int y0, int x1, int y1);
 void orc_fill_box_u(OrcCanvas* c, OrcPipe* p, uint32_t alpha, int x0, int y0, int x1, int y1);
+void orc_fill_box_masked(OrcCanvas* c, OrcPipe* p, uint32_t alpha, int x0, int y0, int x1, int y1,
+                         const uint8_t* mask, intptr_t mask_stride);
 
 #endif /* ORC_CORE_H */

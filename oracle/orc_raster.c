@@ -957,6 +957,23 @@ void orc_fill_box_a(OrcCanvas* c, OrcPipe* p, uint32_t alpha, int x0, int y0, in
   }
 }
 
+/* FillMask_Base with one VMaskA8 command repeated over the rows
+ * (rendercommandprocsync_p.h:65-84, fillgeneric_p.h:67-158,
+ * compopgeneric_p.h:165-183): the mask of pixel x of row y is byte x - x0 of
+ * mask row y - y0, multiplied by `alpha` (udiv255) below full alpha. */
+void orc_fill_box_masked(OrcCanvas* c, OrcPipe* p, uint32_t alpha, int x0, int y0, int x1, int y1,
+                         const uint8_t* mask, intptr_t mask_stride) {
+  orc_pipe_begin(p, (uint32_t)y0);
+  for (int y = y0; y < y1; y++) {
+    orc_pipe_row(p, (uint32_t)y);
+    const uint8_t* row = mask + (intptr_t)(y - y0) * mask_stride;
+    for (int x = x0; x < x1; x++) {
+      uint32_t v = row[x - x0];
+      orc_pipe_composite(p, c, (uint32_t)x, (uint32_t)y, alpha >= 255u ? v : orc_div255(v * alpha));
+    }
+  }
+}
+
 /* FillData::init_box_u_8bpc_24x8 (pipedefs_p.h:644-815) followed by
  * FillMask_Base: the per-pixel mask is (fx(x) * fy(y) * alpha) >> 16 with
  * border coverages fx/fy; rows whose interior mask is zero are skipped, which

@@ -518,6 +518,67 @@ static void blit_scaled(Ctx* c, int integer, double x, double y, double w, doubl
   fill_box_d(c, x, y, x + w, y + h);
 }
 
+/* ---- mask fills (rastercontext.cpp:3590-3780) ----
+ * Rendered only pixel-aligned under a translation
+ * (fill_clipped_box_masked_a); any other placement is NOT_IMPLEMENTED in
+ * the reference (fill_unclipped_mask_d). */
+
+static void fill_masked_box(Ctx* c, int x0, int y0, int x1, int y1, uint32_t img, int mx, int my) {
+  OrcFetch f;
+  OrcPipe p;
+  static uint32_t lut[2048];
+  uint32_t r = resolve(c, &f, lut, &p);
+  if (r > 1) { note(c, r); return; }
+  if (r == 1) return;
+  const SceneImage* ii = &c->img_info[img];
+  const uint8_t* base = c->img_data[img] + (size_t)my * ii->stride + (size_t)mx * (ii->format == ORC_FMT_A8 ? 1u : 4u);
+  orc_fill_box_masked(&c->canvas, &p, alpha_i(c), x0, y0, x1, y1, base, (intptr_t)ii->stride);
+}
+
+/* fill_mask_d_impl + fill_unclipped_mask_d (rastercontext.cpp:3594-3690) */
+static void fill_mask_d(Ctx* c, double x, double y, uint32_t img, const int32_t* area) {
+  int32_t src[4];
+  if (!blit_area(c, img, area, src)) { note(c, ERR_INVALID_VALUE); return; }
+  OrcFetch f;
+  OrcPipe p;
+  static uint32_t lut[2048];
+  if (resolve(c, &f, lut, &p) == 1) return;   /* a NOP renders nothing and reports nothing */
+  if (orc_mat_type(&c->user) > ORC_TT_TRANSLATE) { note(c, ERR_NOT_IMPLEMENTED); return; }
+  OrcMat m; int ftt;
+  fixed_transform(c, &m, &ftt);
+  double sx = x * m.m00 + m.m20, sy = y * m.m11 + m.m21;
+  double dx0 = orc_max(sx, 0.0), dy0 = orc_max(sy, 0.0);
+  double dx1 = orc_min(sx + (double)src[2] * m.m00, (double)c->canvas.w * 256.0);
+  double dy1 = orc_min(sy + (double)src[3] * m.m11, (double)c->canvas.h * 256.0);
+  if (!((dx0 < dx1) & (dy0 < dy1))) return;
+  int64_t fx = orc_floor_to_int64(sx), fy = orc_floor_to_int64(sy);
+  int ix0 = orc_trunc(dx0), iy0 = orc_trunc(dy0), ix1 = orc_trunc(dx1), iy1 = orc_trunc(dy1);
+  if (((fx | fy) & 0xFF) || ((ix0 | iy0 | ix1 | iy1) & 0xFF)) { note(c, ERR_NOT_IMPLEMENTED); return; }
+  int x0 = ix0 >> 8, y0 = iy0 >> 8, x1 = (ix1 + 0xFF) >> 8, y1 = (iy1 + 0xFF) >> 8;
+  int tx = (int)(fx >> 8), ty = (int)(fy >> 8);
+  fill_masked_box(c, x0, y0, x1, y1, img, src[0] + x0 - tx, src[1] + y0 - ty);
+}
+
+/* fill_mask_i_impl (rastercontext.cpp:3723-3743) */
+static void fill_mask_i(Ctx* c, int x, int y, uint32_t img, const int32_t* area) {
+  int tt = orc_mat_type(&c->user);
+  int integral = 0, tx = 0, ty = 0;
+  if (tt <= ORC_TT_TRANSLATE) {
+    int64_t tx64 = orc_floor_to_int64(c->user.m20 * 256.0), ty64 = orc_floor_to_int64(c->user.m21 * 256.0);
+    if (((tx64 | ty64) & 0xFF) == 0) { integral = 1; tx = (int)(tx64 >> 8); ty = (int)(ty64 >> 8); }
+  }
+  if (!integral) { fill_mask_d(c, (double)x, (double)y, img, area); return; }
+  int32_t src[4];
+  if (!blit_area(c, img, area, src)) { note(c, ERR_INVALID_VALUE); return; }
+  int64_t dx = (int64_t)x + tx, dy = (int64_t)y + ty;
+  int64_t x0 = dx < 0 ? 0 : dx, y0 = dy < 0 ? 0 : dy;
+  int64_t x1 = dx + src[2], y1 = dy + src[3];
+  if (x1 > (int64_t)c->canvas.w) x1 = c->canvas.w;
+  if (y1 > (int64_t)c->canvas.h) y1 = c->canvas.h;
+  if (x0 >= x1 || y0 >= y1) return;
+  fill_masked_box(c, (int)x0, (int)y0, (int)x1, (int)y1, img, src[0] + (int)(x0 - dx), src[1] + (int)(y0 - dy));
+}
+
 /* Path builder for BLPath::add_geometry (core/path.cpp:1396-1640),
  * clockwise direction. */
 typedef struct { uint8_t* cmd; OrcPt* vtx; size_t n, cap; } TmpPath;
@@ -750,9 +811,25 @@ uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
             break;
           }
           case 4: fill_geometry(&c, ex.geom_type, p, ex.count); break;
+          case 5: {
+            SceneMask mk;
+            memcpy(&mk, p, sizeof(mk));
+            if (mk.image_index >= c.n_images) { note(&c, ERR_INVALID_VALUE); break; }
+            if (mk.kind == 0) fill_mask_i(&c, (int)mk.dst[0], (int)mk.dst[1], mk.image_index, mk.has_area ? mk.area : NULL);
+            else fill_mask_d(&c, mk.dst[0], mk.dst[1], mk.image_index, mk.has_area ? mk.area : NULL);
+            break;
+          }
           default: break;
         }
         c.mode = 0;
+        break;
+      }
+      case SCENE_OP_FILL_MASK: {
+        SceneMask mk;
+        memcpy(&mk, pl, sizeof(mk));
+        if (mk.image_index >= c.n_images) { note(&c, ERR_INVALID_VALUE); break; }
+        if (mk.kind == 0) fill_mask_i(&c, (int)mk.dst[0], (int)mk.dst[1], mk.image_index, mk.has_area ? mk.area : NULL);
+        else fill_mask_d(&c, mk.dst[0], mk.dst[1], mk.image_index, mk.has_area ? mk.area : NULL);
         break;
       }
       case SCENE_OP_FILL_PATH: {

@@ -56,6 +56,8 @@ struct Call {
   BLRect rect_d;
   // BLIT_IMAGE
   SceneBlit blit{};
+  // FILL_MASK, FILL_EXPLICIT target 5
+  SceneMask mask{};
 };
 
 // Builds the style object of a FILL_EXPLICIT payload (kinds 2, 3) into `call`;
@@ -226,10 +228,18 @@ uint32_t REF_RENDER_ENTRY(const uint8_t* scene, size_t size, uint8_t* out, size_
             break;
           }
           case 4: call.blob.assign(p, payload + c.size); break;
+          case 5:
+            std::memcpy(&call.mask, p, sizeof(SceneMask));
+            if (call.mask.image_index >= images.size()) return 0x7FFFFFFFu;
+            break;
           default: break;
         }
         break;
       }
+      case SCENE_OP_FILL_MASK:
+        std::memcpy(&call.mask, payload, sizeof(SceneMask));
+        if (call.mask.image_index >= images.size()) return 0x7FFFFFFFu;
+        break;
       case SCENE_OP_FILL_RECT_D:
         std::memcpy(call.d, payload, 32);
         break;
@@ -368,6 +378,20 @@ uint32_t REF_RENDER_ENTRY(const uint8_t* scene, size_t size, uint8_t* out, size_
         }
         break;
       }
+      case SCENE_OP_FILL_MASK: {
+        const SceneMask& m = call.mask;
+        const BLImage& img = images[m.image_index];
+        const BLRectI area(m.area[0], m.area[1], m.area[2], m.area[3]);
+        if (m.kind == 0) {
+          const BLPointI o(int(m.dst[0]), int(m.dst[1]));
+          note(m.has_area ? ctx.fill_mask(o, img, area) : ctx.fill_mask(o, img));
+        }
+        else {
+          const BLPoint o(m.dst[0], m.dst[1]);
+          note(m.has_area ? ctx.fill_mask(o, img, area) : ctx.fill_mask(o, img));
+        }
+        break;
+      }
       case SCENE_OP_SET_FILL_ALPHA: note(ctx.set_fill_alpha(call.d[0])); break;
       case SCENE_OP_SET_STYLE_RGBA64: note(ctx.set_fill_style(BLRgba64(call.u64))); break;
       case SCENE_OP_SET_STYLE_RGBA: note(ctx.set_fill_style(BLRgba(call.f[0], call.f[1], call.f[2], call.f[3]))); break;
@@ -423,6 +447,23 @@ uint32_t REF_RENDER_ENTRY(const uint8_t* scene, size_t size, uint8_t* out, size_
             }
             if (call.ex.style_kind == 1) note(bl_context_fill_geometry_rgba64(core, BLGeometryType(call.ex.geom_type), data, call.ex.rgba));
             else note(bl_context_fill_geometry_ext(core, BLGeometryType(call.ex.geom_type), data, st));
+            break;
+          }
+          case 5: {
+            const SceneMask& m = call.mask;
+            const BLImageCore* img = &images[m.image_index];
+            const BLRectI area(m.area[0], m.area[1], m.area[2], m.area[3]);
+            const BLRectI* a = m.has_area ? &area : nullptr;
+            if (m.kind == 0) {
+              const BLPointI o(int(m.dst[0]), int(m.dst[1]));
+              if (call.ex.style_kind == 1) note(bl_context_fill_mask_i_rgba64(core, &o, img, a, call.ex.rgba));
+              else note(bl_context_fill_mask_i_ext(core, &o, img, a, st));
+            }
+            else {
+              const BLPoint o(m.dst[0], m.dst[1]);
+              if (call.ex.style_kind == 1) note(bl_context_fill_mask_d_rgba64(core, &o, img, a, call.ex.rgba));
+              else note(bl_context_fill_mask_d_ext(core, &o, img, a, st));
+            }
             break;
           }
           default: break;

@@ -259,6 +259,47 @@ BLAmdResult bl_amd_context_fill_rect_d_ext(BLAmdContext*, double x, double y, do
   return 0;
 }
 
+BLAmdResult bl_amd_context_fill_mask_i(BLAmdContext*, int x, int y, const BLAmdImage* mask, const int32_t* area) {
+  logf("fill_mask_i %d %d mask image=%u area=%s", x, y, serial_of(mask), area_of(area).c_str());
+  return 0;
+}
+BLAmdResult bl_amd_context_fill_mask_d(BLAmdContext*, double x, double y, const BLAmdImage* mask, const int32_t* area) {
+  logf("fill_mask_d %.17g %.17g mask image=%u area=%s", x, y, serial_of(mask), area_of(area).c_str());
+  return 0;
+}
+BLAmdResult bl_amd_context_fill_mask_i_rgba32(BLAmdContext*, int x, int y, const BLAmdImage* mask, const int32_t* area,
+                                              uint32_t c) {
+  logf("fill_mask_i %d %d mask image=%u area=%s rgba32=%08x", x, y, serial_of(mask), area_of(area).c_str(), c);
+  return 0;
+}
+BLAmdResult bl_amd_context_fill_mask_d_rgba32(BLAmdContext*, double x, double y, const BLAmdImage* mask,
+                                              const int32_t* area, uint32_t c) {
+  logf("fill_mask_d %.17g %.17g mask image=%u area=%s rgba32=%08x", x, y, serial_of(mask), area_of(area).c_str(), c);
+  return 0;
+}
+BLAmdResult bl_amd_context_fill_mask_i_rgba64(BLAmdContext*, int x, int y, const BLAmdImage* mask, const int32_t* area,
+                                              uint64_t c) {
+  logf("fill_mask_i %d %d mask image=%u area=%s rgba64=%016llx", x, y, serial_of(mask), area_of(area).c_str(),
+       (unsigned long long)c);
+  return 0;
+}
+BLAmdResult bl_amd_context_fill_mask_d_rgba64(BLAmdContext*, double x, double y, const BLAmdImage* mask,
+                                              const int32_t* area, uint64_t c) {
+  logf("fill_mask_d %.17g %.17g mask image=%u area=%s rgba64=%016llx", x, y, serial_of(mask), area_of(area).c_str(),
+       (unsigned long long)c);
+  return 0;
+}
+BLAmdResult bl_amd_context_fill_mask_i_ext(BLAmdContext*, int x, int y, const BLAmdImage* mask, const int32_t* area,
+                                           const BLAmdStyle* st) {
+  logf("fill_mask_i %d %d mask image=%u area=%s %s", x, y, serial_of(mask), area_of(area).c_str(), style(st).c_str());
+  return 0;
+}
+BLAmdResult bl_amd_context_fill_mask_d_ext(BLAmdContext*, double x, double y, const BLAmdImage* mask,
+                                           const int32_t* area, const BLAmdStyle* st) {
+  logf("fill_mask_d %.17g %.17g mask image=%u area=%s %s", x, y, serial_of(mask), area_of(area).c_str(),
+       style(st).c_str());
+  return 0;
+}
 BLAmdResult bl_amd_context_blit_image_i(BLAmdContext*, int x, int y, const BLAmdImage* image, const int32_t* area) {
   logf("blit_image_i %d %d image=%u area=%s", x, y, serial_of(image), area_of(area).c_str());
   return 0;

@@ -79,3 +79,11 @@ def test_cancelling_pattern_transforms(seed, fmt):
 def test_blits(seed, fmt):
     # blit_image / blit_scaled_image in all four forms, areas, transforms
     _same(S.blit_scene(seed, 257, 181, fmt, 140, bad_areas=seed == 84))
+
+
+@pytest.mark.parametrize("seed", [91, 92, 93])
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_XRGB32, S.FMT_A8])
+def test_mask_fills(seed, fmt):
+    # fill_mask in all forms; seed 93 adds the placements the reference
+    # reports as NOT_IMPLEMENTED and invalid areas
+    _same(S.mask_scene(seed, 241, 173, fmt, 120, unaligned=seed == 93))
