@@ -59,7 +59,7 @@ SYMBOLS = [
     "bl_amd_context_blit_scaled_image_d",
     "bl_amd_context_fill_mask_i", "bl_amd_context_fill_mask_d", "bl_amd_context_fill_mask_i_rgba32",
     "bl_amd_context_fill_mask_d_rgba32", "bl_amd_context_fill_mask_i_rgba64", "bl_amd_context_fill_mask_d_rgba64",
-    "bl_amd_context_fill_mask_i_ext", "bl_amd_context_fill_mask_d_ext",
+    "bl_amd_context_fill_mask_i_ext", "bl_amd_context_fill_mask_d_ext", "bl_amd_context_set_gradient_quality",
 ]
 
 SIZING_EXACT, SIZING_LEARNED, SIZING_HISTORY = 0, 1, 2
@@ -251,6 +251,7 @@ def bind(L):
         "bl_amd_context_blit_image_d": ([P, D, D, P, P], U32),
         "bl_amd_context_blit_scaled_image_i": ([P, I, I, I, I, P, P], U32),
         "bl_amd_context_blit_scaled_image_d": ([P, D, D, D, D, P, P], U32),
+        "bl_amd_context_set_gradient_quality": ([P, U32], U32),
         "bl_amd_context_fill_mask_i": ([P, I, I, P, P], U32),
         "bl_amd_context_fill_mask_d": ([P, D, D, P, P], U32),
         "bl_amd_context_fill_mask_i_rgba32": ([P, I, I, P, P, U32], U32),
@@ -418,6 +419,9 @@ class BLContext:
 
     def set_pattern_quality(self, q: int):
         _check(lib().bl_amd_context_set_pattern_quality(self._h, q), "set_pattern_quality")
+
+    def set_gradient_quality(self, q: int):
+        _check(lib().bl_amd_context_set_gradient_quality(self._h, q), "set_gradient_quality")
 
     def set_fill_style(self, style):
         L = lib()

@@ -15,8 +15,8 @@
 // Supported here: every fill entry (fill_all / rect / path / geometry with
 // the context style, an explicit rgba32 / rgba64 colour or an explicit style
 // object), clear_*, the fill style slot (colours, BLGradient, BLPattern),
-// fill alpha, global alpha, comp op, fill rule, transforms and the pattern
-// quality hint, image blits (blit_image / blit_scaled_image) and mask
+// fill alpha, global alpha, comp op, fill rule, transforms, the pattern and
+// gradient quality hints, image blits (blit_image / blit_scaled_image) and mask
 // fills (fill_mask in all its forms).  Every other entry (stroking, text,
 // clipping, save / restore) reports BL_ERROR_NOT_IMPLEMENTED.
 
@@ -211,6 +211,7 @@ BLResult BL_CDECL apply_transform_op_impl(BLContextImpl* impl, BLTransformOp op,
 BLResult BL_CDECL set_hint_impl(BLContextImpl* impl, BLContextHint hint, uint32_t value) noexcept {
   AmdContextImpl* self = self_of(impl);
   if (hint == BL_CONTEXT_HINT_PATTERN_QUALITY) BL_PROPAGATE(result_of(bl_amd_context_set_pattern_quality(self->gpu, value)));
+  else if (hint == BL_CONTEXT_HINT_GRADIENT_QUALITY) BL_PROPAGATE(result_of(bl_amd_context_set_gradient_quality(self->gpu, value)));
   else if (uint32_t(hint) > BL_CONTEXT_HINT_MAX_VALUE) return BL_ERROR_INVALID_VALUE;
   self->state_data.hints.hints[hint] = uint8_t(value);
   return BL_SUCCESS;

@@ -88,6 +88,7 @@ struct FrameHost {
   std::vector<lut::Job> lut_jobs;
   std::vector<Stop> lut_stops;
   uint64_t lut_entries = 0;
+  bool dither = false;   // some style samples a dithered gradient
   // upper bound of the edge work items of `src`: 1 per line, 3 per quad, 8 per cubic
   uint64_t items = 0;
   std::vector<DevImage> images;
@@ -96,7 +97,8 @@ struct FrameHost {
   std::vector<uint64_t> cmd_call;
   void clear() {
     canvases.clear(); cmds.clear(); xforms.clear(); src.clear(); vtx.clear();
-    styles.clear(); lut_jobs.clear(); lut_stops.clear(); lut_entries = 0; items = 0; images.clear(); cmd_call.clear();
+    styles.clear(); lut_jobs.clear(); lut_stops.clear(); lut_entries = 0; dither = false; items = 0; images.clear();
+    cmd_call.clear();
   }
 };
 
@@ -468,8 +470,10 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   PartBase tot{0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint64_t n_src_total = 0;
   uint64_t n_items_max = 0;
+  bool dither = false;
   for (size_t k = 0; k < parts.size(); k++) {
     const FrameHost& fh = *parts[k];
+    dither |= fh.dither;
     bases[k] = tot;
     DevCanvas cv = fh.canvases[0];
     cv.cmd_base = (uint32_t)tot.cmds;
@@ -607,6 +611,7 @@ uint32_t Runner::upload(const std::vector<FrameHost*>& parts) {
   f.vtx = (const double*)(base + off[kSecVtx]);
   f.styles = (const DevStyle*)(base + off[kSecStyles]);
   f.luts = lut_buf.as<uint32_t>();
+  f.dither = dither ? 1u : 0u;
   f.images = (const DevImage*)(base + off[kSecImages]);
   f.n_tiles = n_tiles;
   f.n_groups = n_groups;
@@ -1307,6 +1312,7 @@ struct BLAmdContext {
   uint32_t comp_op = kCompSrcOver;
   uint32_t fill_rule = 0;
   uint32_t pattern_quality = 1;
+  uint32_t gradient_quality = 0;
   Mat user{1, 0, 0, 1, 0, 0};
   double global_alpha = 1.0;
   double fill_alpha = 1.0;
@@ -1334,6 +1340,7 @@ struct BLAmdContext {
     comp_op = kCompSrcOver;
     fill_rule = 0;
     pattern_quality = 1;
+    gradient_quality = 0;
     user = Mat{1, 0, 0, 1, 0, 0};
     global_alpha = 1.0;
     fill_alpha = 1.0;
@@ -1463,9 +1470,10 @@ Resolved resolve_with(BLAmdContext* c, StyleState& st, uint32_t alpha, bool nop_
       FrameHost& fr = c->frame;
       d.lut = (uint32_t)fr.lut_entries;
       fr.lut_jobs.push_back(lut::Job{(uint32_t)fr.lut_entries, st.lut_size, (uint32_t)fr.lut_stops.size(),
-                                     (uint32_t)st.stops.size()});
+                                     (uint32_t)st.stops.size(), d.dither});
       fr.lut_stops.insert(fr.lut_stops.end(), st.stops.begin(), st.stops.end());
-      fr.lut_entries += st.lut_size;
+      fr.lut_entries += d.dither ? 2u * st.lut_size : st.lut_size;
+      fr.dither |= d.dither != 0;
     }
     if (st.type == kStylePattern) d.img = st.has_view ? image_view_index(c, st.image.get(), st.view) : image_index(c, st.image.get());
     c->frame_style_index = (uint32_t)c->frame.styles.size();
@@ -2191,6 +2199,13 @@ BLAmdResult bl_amd_context_set_pattern_quality(BLAmdContext* ctx, uint32_t quali
   return BL_AMD_SUCCESS;
 }
 
+BLAmdResult bl_amd_context_set_gradient_quality(BLAmdContext* ctx, uint32_t quality) {
+  if (uint32_t r = check_target(ctx)) return r;
+  if (quality > 2) return kErrInvalid;
+  ctx->gradient_quality = quality;
+  return BL_AMD_SUCCESS;
+}
+
 // resolve_explicit_solid_op (rastercontext.cpp:1259-1275, :90-93)
 BLAmdResult bl_amd_context_set_fill_style_rgba32(BLAmdContext* ctx, uint32_t rgba32) {
   if (uint32_t r = check_target(ctx)) return r;
@@ -2238,6 +2253,12 @@ uint32_t build_gradient_style(BLAmdContext* ctx, StyleState& st, uint32_t type, 
   if (transform) {
     Mat gm{transform[0], transform[1], transform[2], transform[3], transform[4], transform[5]};
     if (transform_type(gm) != kTTIdentity) t = multiply(gm, ctx->user);
+  }
+  // DITHER samples a 64-bit table of twice the size; A8 targets never dither
+  // (rastercontext.cpp:456-467, GradientInfo::lut_size, core/gradient_p.h:113-120)
+  if (ctx->gradient_quality >= 2 && ctx->target && ctx->target->format != kFmtA8) {
+    st.dev.dither = 1;
+    lut_size = std::min<uint32_t>(lut_size * 2u, 1024u);
   }
   st.lut_size = lut_size;
   st.stops = std::move(stops);

@@ -123,7 +123,8 @@ struct DevStyle {
   uint32_t extend;       // 0 pad, 1 repeat, 2 reflect
   uint32_t maxi, rori;
   uint32_t img;          // image index
-  uint32_t pad0;
+  uint32_t dither;       // gradients: 1 samples a 64-bit table (two words an
+                         // entry at `lut`) with the pixel's Bayer threshold
   // Fetcher parameters; only those of `kind` are meaningful.
   union {
     struct {             // linear

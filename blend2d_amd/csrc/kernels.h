@@ -20,6 +20,7 @@ struct FrameDev {
   const double* vtx;
   const DevStyle* styles;
   const uint32_t* luts;
+  uint32_t dither;         // some style samples a dithered gradient (64-bit table)
   const DevImage* images;
   uint32_t n_tiles;
   uint32_t n_groups;       // tile groups: up to 4 consecutive tiles of one band

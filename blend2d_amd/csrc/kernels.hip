@@ -183,6 +183,14 @@ __global__ void __launch_bounds__(256) k_gradient_luts(const lut::Job* __restric
                                                        uint32_t* __restrict__ luts) {
   const lut::Job j = jobs[blockIdx.x];
   const lut::Stop* st = stops + j.stop0;
+  if (j.wide) {
+    for (uint32_t e = threadIdx.x; e < j.size; e += 256u) {
+      const uint64_t v = lut::entry64(st, j.n_stops, j.size, e);
+      luts[(size_t)j.lut + 2u * e] = (uint32_t)v;
+      luts[(size_t)j.lut + 2u * e + 1u] = (uint32_t)(v >> 32);
+    }
+    return;
+  }
   for (uint32_t e = threadIdx.x; e < j.size; e += 256u) luts[(size_t)j.lut + e] = lut::entry(st, j.n_stops, j.size, e);
 }
 
@@ -1010,14 +1018,14 @@ __device__ __forceinline__ void box_masks4(const FrameDev& f, const DevCmd& c, i
 
 // Fetch and composite of pixels px .. px + 3 under masks m; returns how many
 // were composited.
-template<bool kSolid = false>
+template<bool kSolid = false, bool kDither = false>
 __device__ __forceinline__ uint32_t composite4(const FrameDev& f, const DevCmd& c, const DevStyle& st, const pipe::Row& rs,
                                                int px, int w, bool a8, const uint32_t m[4], uint32_t d[4]) {
   uint32_t n = 0;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     if (!m[k] || px + k >= w) continue;
-    const uint32_t src = kSolid ? st.solid : pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
+    const uint32_t src = kSolid ? st.solid : pipe::fetch<kDither>(st, rs, px + k, f.luts, f.images, c.src_fmt);
     d[k] = a8 ? pipe::composite8(c.op, d[k], src >> 24, m[k]) : pipe::composite32(c.op, d[k], src, m[k]);
     n++;
   }
@@ -1026,6 +1034,7 @@ __device__ __forceinline__ uint32_t composite4(const FrameDev& f, const DevCmd& 
 
 // The same with all four fetches issued before the first is used, so their
 // gradient-LUT / texel loads are in flight together (more registers).
+template<bool kDither>
 __device__ __forceinline__ uint32_t composite4_overlapped(const FrameDev& f, const DevCmd& c, const DevStyle& st,
                                                           const pipe::Row& rs, int px, int w, bool a8,
                                                           const uint32_t m[4], uint32_t d[4]) {
@@ -1036,7 +1045,7 @@ __device__ __forceinline__ uint32_t composite4_overlapped(const FrameDev& f, con
   for (int k = 0; k < 4; k++) {
     on[k] = m[k] && px + k < w;
     src[k] = 0u;
-    if (on[k]) src[k] = kSolid ? st.solid : pipe::fetch(st, rs, px + k, f.luts, f.images, c.src_fmt);
+    if (on[k]) src[k] = kSolid ? st.solid : pipe::fetch<kDither>(st, rs, px + k, f.luts, f.images, c.src_fmt);
   }
   uint32_t n = 0;
 #pragma unroll
@@ -1059,6 +1068,7 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t wg, uint32_t n_tiles) {
 
 // One analytic tile (header `slot`): the whole workgroup rasterizes the
 // band's commands that reach the tile and composites them in order.
+template<bool kDither>
 __device__ __forceinline__ void render_tile(const FrameDev& f, const FrameScratch& s, uint32_t slot) {
   __shared__ uint32_t cells[kBatch][kCellWords];
   const int t = threadIdx.x;
@@ -1280,7 +1290,7 @@ __device__ __forceinline__ void render_tile(const FrameDev& f, const FrameScratc
           const DevStyle& st = rec[cur][b].r.style;
           const int fill_row0 = (c.kind == kCmdAnalytic) ? (rec[cur][b].r.min_y >> 8) : (int)c.fetch_row0;
           const pipe::Row rs = pipe::row_setup(st, y, fill_row0, c.kind == kCmdAnalytic ? cv.ref_band_h : 0u);
-          npix += composite4_overlapped(f, c, st, rs, px, (int)cv.w, a8, m, d);
+          npix += composite4_overlapped<kDither>(f, c, st, rs, px, (int)cv.w, a8, m, d);
         }
       }
       if (has_next) {
@@ -1313,12 +1323,13 @@ __device__ __forceinline__ void render_tile(const FrameDev& f, const FrameScratc
 
 // Four workgroups per CU: LDS (7 cell buffers) and the register budget are
 // sized for it.  One workgroup per analytic tile, XCD-contiguous.
+template<bool kDither>
 __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScratch s, uint32_t n) {
   const uint32_t slot = xcd_tile(blockIdx.x, n);
   // uniform: padding workgroups of the XCD mapping, and (when n is an upper
   // bound) slots past the analytic tiles
   if (slot >= n || slot >= s.tile_off[f.n_bands]) return;
-  render_tile(f, s, slot);
+  render_tile<kDither>(f, s, slot);
 }
 
 // The same tiles drained from a work queue by a persistent grid: every
@@ -1328,7 +1339,7 @@ __global__ __launch_bounds__(kTileThreads, 4) void k_render(FrameDev f, FrameScr
 // first), or is the identity (null: band order).  With `s.tile_cycles` set
 // (2 x the tile count), the wall-clock ticks (100 MHz) of every tile and
 // then its composited pixels are logged (cost-model samples).
-template<bool kLog, int kOcc>
+template<bool kLog, int kOcc, bool kDither>
 __global__ __launch_bounds__(kTileThreads, kOcc) void k_render_queue(FrameDev f, FrameScratch s) {
   __shared__ uint32_t next;
   const uint32_t total = s.tile_off[f.n_bands];
@@ -1340,11 +1351,11 @@ __global__ __launch_bounds__(kTileThreads, kOcc) void k_render_queue(FrameDev f,
     if (i >= total) break;   // uniform
     const uint32_t slot = s.queue ? s.queue[i] : i;
     if constexpr (!kLog) {
-      render_tile(f, s, slot);
+      render_tile<kDither>(f, s, slot);
     }
     else {
       const long long c0 = wall_clock64();
-      render_tile(f, s, slot);
+      render_tile<kDither>(f, s, slot);
       __syncthreads();
       if (threadIdx.x == 0) {
         s.tile_cycles[slot] = (uint32_t)min((long long)0xFFFFFFFF, wall_clock64() - c0);
@@ -1590,7 +1601,7 @@ __global__ void k_group_setup(FrameDev f, FrameScratch s) {
 // faster than 64 contiguous bytes per thread).  Other styles keep one row
 // per thread, four consecutive quads along it, so their per-row fetch setup
 // is done once per command.
-template<bool kSolid>
+template<bool kSolid, bool kDither = false>
 __device__ __forceinline__ void render_box(const FrameDev& f, const FrameScratch& s, uint32_t n) {
   const uint32_t slot = xcd_tile(blockIdx.x, n);
   if (slot >= n) return;
@@ -1743,7 +1754,7 @@ __device__ __forceinline__ void render_box(const FrameDev& f, const FrameScratch
           uint32_t m[4];
           const int px = qx(g);
           box_masks4(f, c, (int)band * kTileH + qrow(g), px, m);
-          if (qrow(g) < rows) npix += composite4<kSolid>(f, c, st, rs, px, w, a8, m, d[0]);
+          if (qrow(g) < rows) npix += composite4<kSolid, kDither>(f, c, st, rs, px, w, a8, m, d[0]);
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             const uint32_t r0 = d[0][k];
@@ -1912,8 +1923,9 @@ __global__ __launch_bounds__(kTileThreads, 6) void k_render_box_solid(FrameDev f
   render_box<true>(f, s, n);
 }
 
+template<bool kDither>
 __global__ __launch_bounds__(kTileThreads) void k_render_box(FrameDev f, FrameScratch s, uint32_t n) {
-  render_box<false>(f, s, n);
+  render_box<false, kDither>(f, s, n);
 }
 
 void launch_band_class(const FrameDev& f, const FrameScratch& s, hipStream_t st) {
@@ -1934,10 +1946,14 @@ void launch_render_setup(const FrameDev& f, const FrameScratch& s, hipStream_t s
 void launch_render(const FrameDev& f, const FrameScratch& s, uint32_t n_a, uint32_t n_f, uint32_t n_s, uint32_t n_g,
                    hipStream_t st) {
   if (!f.n_tiles) return;
-  if (n_a) hipLaunchKernelGGL(k_render, dim3((n_a + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_a);
+  // frames with dithered gradients take the instances that carry the dither
+  // path; the others leave it out
+  if (n_a && f.dither) hipLaunchKernelGGL(k_render<true>, dim3((n_a + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_a);
+  else if (n_a) hipLaunchKernelGGL(k_render<false>, dim3((n_a + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_a);
   if (n_f) hipLaunchKernelGGL(k_render_box_fast, dim3((n_f + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_f);
   if (n_s) hipLaunchKernelGGL(k_render_box_solid, dim3((n_s + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_s);
-  if (n_g) hipLaunchKernelGGL(k_render_box, dim3((n_g + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_g);
+  if (n_g && f.dither) hipLaunchKernelGGL(k_render_box<true>, dim3((n_g + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_g);
+  else if (n_g) hipLaunchKernelGGL(k_render_box<false>, dim3((n_g + kXcds - 1) / kXcds * kXcds), dim3(kTileThreads), 0, st, f, s, n_g);
 }
 
 void launch_tile_features(const FrameDev& f, const FrameScratch& s, uint32_t n_a, hipStream_t st) {
@@ -1958,9 +1974,14 @@ void launch_render_queue(const FrameDev& f, const FrameScratch& s, uint32_t grid
   if (!s.queue) launch_fill_u32(s.queue_head, 0u, 1, st);
   // Four workgroups per CU need the four-occupancy instance (128 VGPRs,
   // some spilled); fewer take the three-occupancy one (170 VGPRs).
-  if (s.tile_cycles) hipLaunchKernelGGL((k_render_queue<true, 4>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
-  else if (wgs_per_cu >= 4) hipLaunchKernelGGL((k_render_queue<false, 4>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
-  else hipLaunchKernelGGL((k_render_queue<false, 3>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
+  if (f.dither) {
+    if (s.tile_cycles) hipLaunchKernelGGL((k_render_queue<true, 4, true>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
+    else if (wgs_per_cu >= 4) hipLaunchKernelGGL((k_render_queue<false, 4, true>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
+    else hipLaunchKernelGGL((k_render_queue<false, 3, true>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
+  }
+  else if (s.tile_cycles) hipLaunchKernelGGL((k_render_queue<true, 4, false>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
+  else if (wgs_per_cu >= 4) hipLaunchKernelGGL((k_render_queue<false, 4, false>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
+  else hipLaunchKernelGGL((k_render_queue<false, 3, false>), dim3(grid), dim3(kTileThreads), 0, st, f, s);
 }
 
 void launch_render_stats(const FrameDev& f, const FrameScratch& s, hipStream_t st) {

@@ -36,13 +36,15 @@ struct Stop {
   uint64_t rgba64;
 };
 
-// One table to fill: `size` entries at `lut` (in entries) from `n_stops`
-// stops at `stop0`, sorted by offset.
+// One table to fill: `size` entries at `lut` (in 32-bit words) from
+// `n_stops` stops at `stop0`, sorted by offset; `wide` tables hold 64-bit
+// entries (two words each, low word first).
 struct Job {
   uint32_t lut;
   uint32_t size;
   uint32_t stop0;
   uint32_t n_stops;
+  uint32_t wide;
 };
 
 B2AMD_LUT_HD uint32_t div255(uint32_t x) { return ((x + 128u) * 257u) >> 16; }
@@ -103,6 +105,70 @@ B2AMD_LUT_HD uint32_t entry(const Stop* st, uint32_t n, uint32_t size, uint32_t 
     out |= (uint64_t)(v << 8) << (16 * k);
   }
   return premultiplied(out);
+}
+
+// ---- 64-bit tables (dithered gradients) ----------------------------------
+//
+// The reference fills a dithered gradient's table with the generic
+// interpolate_prgb64 (pixelops/interpolation.cpp:161-275, the only one
+// pixelops/funcs.cpp:23 installs): span t (t = 0 .. n - 1) runs from entry
+// start(t) (0 for the first, else the end of span t - 1) to its end
+// pos(t) = round(offset_t * f) >> 8, f = (size - 1) << 8, inclusive.  It
+// writes the previous stop's colour (stop 0's for span 0; the span's own for
+// an empty span) at start(t) and, when the two colours differ, steps each
+// 16-bit channel in 16.15 fixed point by (c1 - c0) / (pos - start)
+// (truncated) from c0 + 1/2; later spans overwrite the entry they share.
+// Entries past the last span take the last colour, entry 0 the first stop's.
+// Interpolated entries of two opaque colours keep their 16-bit channels;
+// others are premultiplied channel by channel with udiv65535, as every
+// stop colour is (cvt_prgb64_8888_from_argb64_8888, pixelops/scalar_p.h:
+// 179-222).  Layout: a in bits 48-63, r, g, b below (BLRgba64).
+
+B2AMD_LUT_HD uint32_t div65535(uint32_t x) { return ((x + 0x8000u) + ((x + 0x8000u) >> 16)) >> 16; }
+
+B2AMD_LUT_HD uint64_t premultiplied64(uint64_t c) {
+  const uint32_t a = (uint32_t)(c >> 48);
+  const uint64_t r = div65535((uint32_t)((c >> 32) & 0xFFFFu) * a);
+  const uint64_t g = div65535((uint32_t)((c >> 16) & 0xFFFFu) * a);
+  const uint64_t b = div65535((uint32_t)(c & 0xFFFFu) * a);
+  return ((uint64_t)a << 48) | (r << 32) | (g << 16) | b;
+}
+
+// Entry `e` (0 <= e < size) of the 64-bit table of `n` >= 1 stops.
+B2AMD_LUT_HD uint64_t entry64(const Stop* st, uint32_t n, uint32_t size, uint32_t e) {
+  if (e == 0 || n == 1u) return premultiplied64(st[0].rgba64);
+  const double f_width = (double)((int32_t)(size - 1u) << 8);
+  auto pos = [&](uint32_t k) -> uint32_t { return (uint32_t)round_to_int(st[k].offset * f_width) >> 8; };
+  auto start = [&](uint32_t t) -> uint32_t { return t == 0 ? 0u : pos(t - 1u); };
+  if (e > pos(n - 1u)) return premultiplied64(st[n - 1u].rgba64);
+  // the last span starting at or before e (starts never decrease)
+  uint32_t lo = 0, hi = n - 1u;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1u) >> 1;
+    if (start(mid) <= e) lo = mid;
+    else hi = mid - 1u;
+  }
+  const uint32_t p = start(lo), q = pos(lo);
+  const uint64_t c1 = st[lo].rgba64;
+  const uint64_t c0 = (q == p) ? c1 : st[lo == 0 ? 0u : lo - 1u].rgba64;
+  const uint32_t j = e - p;
+  if (c0 == c1 || j == 0) return premultiplied64(c0);
+  const int32_t i = (int32_t)(q - p);
+  const uint32_t kMask = 0xFFFFu << 15;
+  auto step = [&](uint32_t a0, uint32_t a1) -> uint32_t {
+    const uint32_t inc = (uint32_t)((int32_t)(a1 - a0) / i);
+    return a0 + (1u << 14) + inc * j;
+  };
+  const uint32_t rp = step((uint32_t)(c0 >> 17) & kMask, (uint32_t)(c1 >> 17) & kMask);
+  const uint32_t gp = step((uint32_t)(c0 >> 1) & kMask, (uint32_t)(c1 >> 1) & kMask);
+  const uint32_t bp = step((uint32_t)(c0 << 15) & kMask, (uint32_t)(c1 << 15) & kMask);
+  if (((c0 & c1) >> 48) == 0xFFFFu)
+    return ((uint64_t)(rp & kMask) << 17) | ((uint64_t)(gp & kMask) << 1) | ((uint64_t)(bp & kMask) >> 15) |
+           0xFFFF000000000000ull;
+  const uint32_t ap = step((uint32_t)(c0 >> 33) & kMask, (uint32_t)(c1 >> 33) & kMask);
+  const uint32_t ca = ap >> 15;
+  const uint64_t cr = div65535((rp >> 15) * ca), cg = div65535((gp >> 15) * ca), cb = div65535((bp >> 15) * ca);
+  return ((uint64_t)ca << 48) | (cr << 32) | (cg << 16) | cb;
 }
 
 }  // namespace lut

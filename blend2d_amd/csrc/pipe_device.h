@@ -77,13 +77,14 @@ __device__ __forceinline__ uint32_t sp_col(const DevStyle& s, int64_t u) {
 
 // Per (command, row) fetch state.
 struct Row {
-  uint64_t a, b;       // linear: pt at x = 0; pattern: tx, ty of the row
+  uint64_t a, b;       // linear: pt at x = 0; pattern: tx, ty of the row;
+                       // gradients: the row in b (dither thresholds)
   float f0, f1, f2;    // radial: b, d, dd;  conic: tx, ay, by
 };
 
 __device__ __forceinline__ Row row_setup(const DevStyle& s, int y, int fill_row0, uint32_t ref_band_h) {
   Row r;
-  r.a = 0; r.b = 0; r.f0 = 0.f; r.f1 = 0.f; r.f2 = 0.f;
+  r.a = 0; r.b = (uint32_t)y; r.f0 = 0.f; r.f1 = 0.f; r.f2 = 0.f;
   switch (s.kind) {
     case kFetchLinear:
       r.a = s.lin_pt0 + (uint64_t)(uint32_t)y * s.lin_dy;
@@ -134,6 +135,48 @@ __device__ __forceinline__ Row row_setup(const DevStyle& s, int y, int fill_row0
   return r;
 }
 
+// bayer_matrix_16x16 (tables/tables_p.h:448-469): dither thresholds of
+// pixel (x & 15, y & 15), row by row.
+__constant__ uint8_t kBayer16[256] = {
+    0, 191,  48, 239,  12, 203,  60, 251,   3, 194,  51, 242,  15, 206,  63, 254,
+  127,  64, 175, 112, 139,  76, 187, 124, 130,  67, 178, 115, 142,  79, 190, 127,
+   32, 223,  16, 207,  44, 235,  28, 219,  35, 226,  19, 210,  47, 238,  31, 222,
+  159,  96, 143,  80, 171, 108, 155,  92, 162,  99, 146,  83, 174, 111, 158,  95,
+    8, 199,  56, 247,   4, 195,  52, 243,  11, 202,  59, 250,   7, 198,  55, 246,
+  135,  72, 183, 120, 131,  68, 179, 116, 138,  75, 186, 123, 134,  71, 182, 119,
+   40, 231,  24, 215,  36, 227,  20, 211,  43, 234,  27, 218,  39, 230,  23, 214,
+  167, 104, 151,  88, 163, 100, 147,  84, 170, 107, 154,  91, 166, 103, 150,  87,
+    2, 193,  50, 241,  14, 205,  62, 253,   1, 192,  49, 240,  13, 204,  61, 252,
+  129,  66, 177, 114, 141,  78, 189, 126, 128,  65, 176, 113, 140,  77, 188, 125,
+   34, 225,  18, 209,  46, 237,  30, 221,  33, 224,  17, 208,  45, 236,  29, 220,
+  161,  98, 145,  82, 173, 110, 157,  94, 160,  97, 144,  81, 172, 109, 156,  93,
+   10, 201,  58, 249,   6, 197,  54, 245,   9, 200,  57, 248,   5, 196,  53, 244,
+  137,  74, 185, 122, 133,  70, 181, 118, 136,  73, 184, 121, 132,  69, 180, 117,
+   42, 233,  26, 217,  38, 229,  22, 213,  41, 232,  25, 216,  37, 228,  21, 212,
+  169, 106, 153,  90, 165, 102, 149,  86, 168, 105, 152,  89, 164, 101, 148,  85};
+
+// Gradient table entry `i` at pixel (x, row y): the 32-bit entry, or for a
+// dithered gradient the 64-bit one with the pixel's Bayer threshold added to
+// each colour channel before it is cut to 8 bits, limited by alpha
+// (FetchGradientBase<DITHER>::fetch_pixel, fetchgeneric_p.h:899-934).
+// kDither = false (frames without dithered gradients) leaves the dither path
+// out of the code, and its registers out of the render kernels.
+template<bool kDither>
+__device__ __forceinline__ uint32_t lut_px(const DevStyle& s, const uint32_t* __restrict__ luts, uint32_t i, int x,
+                                           uint32_t y) {
+  if (!kDither || !s.dither) return luts[s.lut + i];
+  const uint32_t lo = luts[s.lut + 2u * i], hi = luts[s.lut + 2u * i + 1u];
+  const uint32_t dd = kBayer16[((y & 15u) << 4) | ((uint32_t)x & 15u)];
+  const uint32_t a = hi >> 24;
+  uint32_t r = ((hi & 0xFFFFu) + dd) >> 8;
+  uint32_t g = ((lo >> 16) + dd) >> 8;
+  uint32_t b = ((lo & 0xFFFFu) + dd) >> 8;
+  r = r < a ? r : a;
+  g = g < a ? g : a;
+  b = b < a ? b : a;
+  return (a << 24) | (r << 16) | (g << 8) | b;
+}
+
 __device__ __forceinline__ uint32_t grad_index(uint32_t idx, const DevStyle& s) {
   if (s.extend == 0) {
     int32_t i = (int32_t)idx;
@@ -166,6 +209,7 @@ __device__ __forceinline__ uint32_t texel(const DevImage& im, uint32_t fmt, int3
   return v;
 }
 
+template<bool kDither = false>
 __device__ __forceinline__ uint32_t fetch(const DevStyle& s, const Row& r, int x, const uint32_t* __restrict__ luts,
                                           const DevImage* __restrict__ images, uint32_t src_fmt) {
   switch (s.kind) {
@@ -173,14 +217,14 @@ __device__ __forceinline__ uint32_t fetch(const DevStyle& s, const Row& r, int x
       return s.solid;
     case kFetchLinear: {
       uint64_t pt = r.a + (uint64_t)(uint32_t)x * s.lin_dt;
-      return luts[s.lut + grad_index((uint32_t)(pt >> 32), s)];
+      return lut_px<kDither>(s, luts, grad_index((uint32_t)(pt >> 32), s), x, (uint32_t)r.b);
     }
     case kFetchRadial: {
       float fx = (float)x;
       float sq_x = fx * fx;
       float a = __builtin_sqrtf(__builtin_fabsf(sq_x * s.rad_ddd + (fx * r.f2 + r.f1)));
       float v = (fx * s.rad_bd + r.f0) + a;
-      return luts[s.lut + grad_index((uint32_t)trunc_f32_to_i32(v), s)];
+      return lut_px<kDither>(s, luts, grad_index((uint32_t)trunc_f32_to_i32(v), s), x, (uint32_t)r.b);
     }
     case kFetchConic: {
       float fx = (float)x;
@@ -200,7 +244,7 @@ __device__ __forceinline__ uint32_t fetch(const DevStyle& s, const Row& r, int x
       v = __builtin_fabsf(v - r.f2) + s.con_offset;
       int32_t i = rint_f32_to_i32(v);
       if (i > s.con_maxi) i = s.con_maxi;
-      return luts[s.lut + ((uint32_t)i & (uint32_t)s.con_rori)];
+      return lut_px<kDither>(s, luts, (uint32_t)i & (uint32_t)s.con_rori, x, (uint32_t)r.b);
     }
     case kFetchPatternNN:
     case kFetchPatternBI: {

@@ -72,7 +72,8 @@ enum SceneOp {
   SCENE_OP_FILL_PATH_EX       = 24, /* ScenePathEx, u8 cmds[n] padded to 8, f64 vtx[2n] */
   SCENE_OP_FILL_EXPLICIT      = 25, /* SceneExplicit, style payload, target payload (see below) */
   SCENE_OP_BLIT_IMAGE         = 26, /* SceneBlit */
-  SCENE_OP_FILL_MASK          = 27  /* SceneMask */
+  SCENE_OP_FILL_MASK          = 27, /* SceneMask */
+  SCENE_OP_SET_GRADIENT_QUALITY = 28 /* u32 quality (0 nearest, 1 smooth, 2 dither), u32 pad */
 };
 
 /* FILL_MASK: fill_mask_i / _d (kinds 0, 1) with the context's fill style of

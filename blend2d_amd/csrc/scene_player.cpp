@@ -172,7 +172,7 @@ uint32_t decode(Scene& sc) {
     c.op = cmd.op;
     switch (cmd.op) {
       case SCENE_OP_SET_COMP_OP: case SCENE_OP_SET_FILL_RULE: case SCENE_OP_SET_STYLE_SOLID:
-      case SCENE_OP_SET_PATTERN_QUALITY: case SCENE_OP_FILL_ALL_RGBA32:
+      case SCENE_OP_SET_PATTERN_QUALITY: case SCENE_OP_SET_GRADIENT_QUALITY: case SCENE_OP_FILL_ALL_RGBA32:
         std::memcpy(&c.u0, pl, 4);
         break;
       case SCENE_OP_SET_TRANSFORM:
@@ -361,6 +361,7 @@ void issue(BLAmdContext* ctx, const Scene& sc, Issued& out) {
       case SCENE_OP_SET_TRANSFORM: r = bl_amd_context_set_transform(ctx, c.d); break;
       case SCENE_OP_SET_GLOBAL_ALPHA: r = bl_amd_context_set_global_alpha(ctx, c.d[0]); break;
       case SCENE_OP_SET_PATTERN_QUALITY: r = bl_amd_context_set_pattern_quality(ctx, c.u0); break;
+      case SCENE_OP_SET_GRADIENT_QUALITY: r = bl_amd_context_set_gradient_quality(ctx, c.u0); break;
       case SCENE_OP_SET_STYLE_SOLID: r = bl_amd_context_set_fill_style_rgba32(ctx, c.u0); break;
       case SCENE_OP_SET_STYLE_RGBA64: r = bl_amd_context_set_fill_style_rgba64(ctx, c.u64); break;
       case SCENE_OP_SET_STYLE_RGBA: r = bl_amd_context_set_fill_style_rgba(ctx, c.f); break;

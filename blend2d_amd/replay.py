@@ -102,6 +102,8 @@ def replay_into(ctx, sc: "S.Scene", images) -> list:
             call(ctx.set_global_alpha, struct.unpack_from("<d", pl)[0])
         elif op == S.OP_SET_PATTERN_QUALITY:
             call(ctx.set_pattern_quality, struct.unpack_from("<I", pl)[0])
+        elif op == S.OP_SET_GRADIENT_QUALITY:
+            call(ctx.set_gradient_quality, struct.unpack_from("<I", pl)[0])
         elif op == S.OP_SET_STYLE_SOLID:
             call(ctx.set_fill_style, struct.unpack_from("<I", pl)[0])
         elif op == S.OP_SET_STYLE_GRADIENT:

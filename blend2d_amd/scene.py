@@ -52,6 +52,8 @@ OP_BLIT_IMAGE = 26
 BLIT_I, BLIT_D, BLIT_SCALED_I, BLIT_SCALED_D = 0, 1, 2, 3
 OP_FILL_MASK = 27
 MASK_I, MASK_D = 0, 1
+OP_SET_GRADIENT_QUALITY = 28
+GRADIENT_QUALITY_NEAREST, GRADIENT_QUALITY_SMOOTH, GRADIENT_QUALITY_DITHER = 0, 1, 2
 
 # FILL_EXPLICIT targets and style kinds
 TARGET_ALL, TARGET_RECT_I, TARGET_RECT_D, TARGET_PATH, TARGET_GEOMETRY, TARGET_MASK = range(6)
@@ -128,6 +130,9 @@ class Scene:
 
     def set_global_alpha(self, a: float) -> None:
         self._cmd(OP_SET_GLOBAL_ALPHA, struct.pack("<d", a))
+
+    def set_gradient_quality(self, q: int) -> None:
+        self._cmd(OP_SET_GRADIENT_QUALITY, struct.pack("<II", q, 0))
 
     def set_pattern_quality(self, q: int) -> None:
         self._cmd(OP_SET_PATTERN_QUALITY, struct.pack("<II", q, 0))
@@ -1332,6 +1337,90 @@ def mask_scene(seed: int = 1, width: int = 241, height: int = 173, fmt: int = FM
             sc.fill_explicit(TARGET_MASK, style(), (kind, masks[k], dst, area))
         else:
             sc.fill_mask(kind, masks[k], dst, area)
+    sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+    return sc
+
+
+def dither_scene(seed: int = 1, width: int = 223, height: int = 167, fmt: int = FMT_PRGB32, count: int = 80,
+                 bad_hint: bool = False) -> Scene:
+    """Gradients under the gradient quality hint (BL_CONTEXT_HINT_GRADIENT_
+    QUALITY): NEAREST, SMOOTH (rendered as nearest) and DITHER (a 64-bit table
+    of twice the size plus the pixel's Bayer threshold); linear, radial and
+    conic, PAD / REPEAT / REFLECT, 1-12 stops opaque and translucent, with
+    gradient and user transforms, as the fill style (which keeps the hint of
+    the moment it was set) and as a style for one call; rects, ellipses,
+    glyph contours and fill_all; SRC_OVER / SRC_COPY and global alpha.  A8
+    targets never dither.  With `bad_hint`, also a hint value out of range
+    (BL_ERROR_INVALID_VALUE, the hint unchanged)."""
+    rng = np.random.default_rng(seed)
+    clear = 0xFF000000 | int(rng.integers(0, 1 << 24)) if fmt != FMT_A8 else int(rng.integers(0, 256))
+    sc = Scene(width, height, fmt, clear)
+    W, H = float(width), float(height)
+
+    def gradient():
+        gtype = int(rng.integers(0, 3))
+        cx, cy = float(rng.uniform(0, W)), float(rng.uniform(0, H))
+        size = float(rng.uniform(10, max(W, H)))
+        if gtype == GRADIENT_LINEAR:
+            p = rng.random(4) * size + np.array([cx, cy, cx, cy]) - size / 2
+            vals = [float(v) for v in p]
+        elif gtype == GRADIENT_RADIAL:
+            r = float(rng.uniform(4, size))
+            f = (rng.random(2) - 0.5) * r
+            vals = [cx, cy, cx + float(f[0]), cy + float(f[1]), r, 0.0]
+        else:
+            vals = [cx, cy, float(rng.uniform(0, 2 * math.pi)), float(rng.choice([1.0, 2.0, 0.5]))]
+        stops = _rand_stops(rng, 1, 12)
+        if rng.random() < 0.4:   # opaque stops: the table keeps 16-bit channels between them
+            stops = [(o, c | 0xFF000000) for o, c in stops]
+        gtr = rotation_about(float(rng.uniform(0, 2 * math.pi)), cx, cy) if rng.random() < 0.25 else None
+        return gtype, vals, stops, int(rng.integers(0, 3)), gtr
+
+    for i in range(count):
+        r = rng.random()
+        if r < 0.15:
+            q = int(rng.choice([0, 1, 2, 2, 2]))
+            if bad_hint and rng.random() < 0.2:
+                q = 3
+            sc.set_gradient_quality(q)
+            continue
+        if r < 0.2:
+            t = rng.random()
+            cx, cy = W / 2, H / 2
+            if t < 0.4:
+                sc.set_transform([1.0, 0.0, 0.0, 1.0, float(rng.integers(-9, 10)), float(rng.integers(-9, 10))])
+            elif t < 0.7:
+                sc.set_transform(rotation_about(float(rng.uniform(0, 2 * math.pi)), cx, cy))
+            else:
+                sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+            continue
+        if r < 0.25:
+            sc.set_comp_op(int(rng.choice([COMP_OP_SRC_OVER, COMP_OP_SRC_COPY])))
+            sc.set_global_alpha(1.0 if rng.random() < 0.6 else float(rng.uniform(0.1, 1.0)))
+            continue
+        explicit = rng.random() < 0.3
+        if not explicit:
+            gtype, vals, stops, extend, gtr = gradient()
+            sc.set_fill_style_gradient(gtype, vals, stops, extend, gtr)
+        g = rng.random()
+        x0, y0 = float(rng.uniform(-0.2, 0.9) * W), float(rng.uniform(-0.2, 0.9) * H)
+        w, h = float(rng.uniform(4, W * 0.7)), float(rng.uniform(4, H * 0.7))
+        if explicit:
+            gtype, vals, stops, extend, gtr = gradient()
+            style = ("gradient", gtype, vals, stops, extend, gtr)
+            sc.fill_explicit(TARGET_RECT_D if g < 0.6 else TARGET_ALL, style, (x0, y0, w, h) if g < 0.6 else None)
+        elif g < 0.25:
+            sc.fill_rect_i(int(x0), int(y0), int(w), int(h))
+        elif g < 0.5:
+            sc.fill_rect_d(x0, y0, w, h)
+        elif g < 0.75:
+            cm, v = ellipse_path(x0 + w / 2, y0 + h / 2, w / 2, h / 2)
+            sc.fill_path(cm, v)
+        elif g < 0.95:
+            cm, v = glyph_path(rng, w, h)
+            sc.fill_path(cm, np.asarray(v) + np.array([x0, y0]))
+        else:
+            sc.fill_all()
     sc.set_transform([1.0, 0.0, 0.0, 1.0, 0.0, 0.0])
     return sc
 

@@ -132,6 +132,14 @@ BLAmdResult bl_amd_context_set_transform(BLAmdContext* ctx, const double m[6]);
 /* bl_context_set_hint(BL_CONTEXT_HINT_PATTERN_QUALITY) (context.h:408-422):
  * 0 nearest, 1 bilinear (default). */
 BLAmdResult bl_amd_context_set_pattern_quality(BLAmdContext* ctx, uint32_t quality);
+/* BLContext::set_hint(BL_CONTEXT_HINT_GRADIENT_QUALITY) (context.h,
+ * rastercontext.cpp:1816-1821): 0 nearest, 1 smooth (rendered as nearest),
+ * 2 dither; other values are BL_AMD_ERROR_INVALID_VALUE.  A gradient style
+ * takes the hint when it is set (explicit styles: when the call is made); a
+ * dithered gradient samples a 64-bit table of twice the size (at most 1024
+ * entries) and adds the 16 x 16 Bayer threshold of the pixel before cutting
+ * each channel to 8 bits.  A8 targets never dither. */
+BLAmdResult bl_amd_context_set_gradient_quality(BLAmdContext* ctx, uint32_t quality);
 
 /* bl_context_set_fill_style_rgba32 (context.h:674). */
 BLAmdResult bl_amd_context_set_fill_style_rgba32(BLAmdContext* ctx, uint32_t rgba32);

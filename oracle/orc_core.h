@@ -59,6 +59,7 @@ typedef struct {
   uint32_t solid_prgb32;
   /* gradient */
   const uint32_t* lut;
+  const uint64_t* lut64;      /* dithered gradients: the 64-bit table (else NULL) */
   uint32_t lut_size;
   uint32_t extend;            /* 0 pad, 1 repeat, 2 reflect */
   uint32_t maxi, rori;
@@ -116,6 +117,7 @@ int orc_edges_add_poly(OrcEdges* e, const OrcPt* vtx, size_t n,
 typedef struct { double offset; uint64_t rgba64; } OrcStop;
 uint32_t orc_premultiply(uint32_t argb32);
 void orc_interpolate_lut(uint32_t* dst, uint32_t dst_size, const OrcStop* stops, size_t n);
+void orc_interpolate_lut64(uint64_t* dst, uint32_t dst_size, const OrcStop* stops, size_t n);
 /* Returns lut size and sets solid/format as ensure_info (core/gradient.cpp:202-287). */
 uint32_t orc_gradient_info(const OrcStop* stops, size_t n, int* solid, uint32_t* format);
 int orc_init_linear(OrcFetch* f, const double v[4], uint32_t extend, const OrcMat* t);

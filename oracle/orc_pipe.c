@@ -84,6 +84,43 @@ static inline int32_t pa_index(int32_t v, int32_t mn, int32_t mx, int32_t cor) {
   return v ^ (v >> 31);
 }
 
+/* bayer_matrix_16x16 (tables/tables_p.h:448-469): the dither thresholds of
+ * pixel (x & 15, y & 15). */
+static const uint8_t kBayer16[16][16] = {
+  {  0, 191,  48, 239,  12, 203,  60, 251,   3, 194,  51, 242,  15, 206,  63, 254},
+  {127,  64, 175, 112, 139,  76, 187, 124, 130,  67, 178, 115, 142,  79, 190, 127},
+  { 32, 223,  16, 207,  44, 235,  28, 219,  35, 226,  19, 210,  47, 238,  31, 222},
+  {159,  96, 143,  80, 171, 108, 155,  92, 162,  99, 146,  83, 174, 111, 158,  95},
+  {  8, 199,  56, 247,   4, 195,  52, 243,  11, 202,  59, 250,   7, 198,  55, 246},
+  {135,  72, 183, 120, 131,  68, 179, 116, 138,  75, 186, 123, 134,  71, 182, 119},
+  { 40, 231,  24, 215,  36, 227,  20, 211,  43, 234,  27, 218,  39, 230,  23, 214},
+  {167, 104, 151,  88, 163, 100, 147,  84, 170, 107, 154,  91, 166, 103, 150,  87},
+  {  2, 193,  50, 241,  14, 205,  62, 253,   1, 192,  49, 240,  13, 204,  61, 252},
+  {129,  66, 177, 114, 141,  78, 189, 126, 128,  65, 176, 113, 140,  77, 188, 125},
+  { 34, 225,  18, 209,  46, 237,  30, 221,  33, 224,  17, 208,  45, 236,  29, 220},
+  {161,  98, 145,  82, 173, 110, 157,  94, 160,  97, 144,  81, 172, 109, 156,  93},
+  { 10, 201,  58, 249,   6, 197,  54, 245,   9, 200,  57, 248,   5, 196,  53, 244},
+  {137,  74, 185, 122, 133,  70, 181, 118, 136,  73, 184, 121, 132,  69, 180, 117},
+  { 42, 233,  26, 217,  38, 229,  22, 213,  41, 232,  25, 216,  37, 228,  21, 212},
+  {169, 106, 153,  90, 165, 102, 149,  86, 168, 105, 152,  89, 164, 101, 148,  85}};
+
+/* Entry `idx` of a gradient table at pixel (x, y): the 32-bit entry, or for a
+ * dithered gradient the 64-bit one with the pixel's Bayer threshold added to
+ * each colour channel, cut to 8 bits and limited by alpha
+ * (FetchGradientBase<DITHER>::fetch_pixel, fetchgeneric_p.h:899-934; pixel
+ * origin 0, 0). */
+static uint32_t lut_px(const OrcFetch* f, uint32_t idx, uint32_t x, uint32_t y) {
+  if (!f->lut64) return f->lut[idx];
+  uint64_t v = f->lut64[idx];
+  uint32_t dd = kBayer16[y & 15u][x & 15u];
+  uint32_t a = (uint32_t)(v >> 56);
+  uint32_t r = (uint32_t)(((v >> 32) & 0xFFFFu) + dd) >> 8;
+  uint32_t g = (uint32_t)(((v >> 16) & 0xFFFFu) + dd) >> 8;
+  uint32_t b = (uint32_t)((v & 0xFFFFu) + dd) >> 8;
+  r = r < a ? r : a; g = g < a ? g : a; b = b < a ? b : a;
+  return (a << 24) | (r << 16) | (g << 8) | b;
+}
+
 static uint32_t fetch_px(const OrcPipe* p, uint32_t x, uint32_t y) {
   const OrcFetch* f = p->fetch;
   switch (f->type) {
@@ -103,7 +140,7 @@ static uint32_t fetch_px(const OrcPipe* p, uint32_t x, uint32_t y) {
         uint32_t a = idx & f->maxi, b = a ^ f->rori;
         idx = b < a ? b : a;
       }
-      return f->lut[idx];
+      return lut_px(f, idx, x, y);
     }
 
     case ORC_FETCH_RADIAL: {
@@ -130,7 +167,7 @@ static uint32_t fetch_px(const OrcPipe* p, uint32_t x, uint32_t y) {
         uint32_t aa = idx & f->maxi, bb = aa ^ f->rori;
         idx = bb < aa ? bb : aa;
       }
-      return f->lut[idx];
+      return lut_px(f, idx, x, y);
     }
 
     case ORC_FETCH_CONIC: {
@@ -154,7 +191,7 @@ static uint32_t fetch_px(const OrcPipe* p, uint32_t x, uint32_t y) {
       v = fabsf(v - by) + f->con_offset;
       int32_t i = rint_f32_to_i32(v);
       if (i > f->con_maxi) i = f->con_maxi;
-      return f->lut[(uint32_t)i & (uint32_t)f->con_rori];
+      return lut_px(f, (uint32_t)i & (uint32_t)f->con_rori, x, y);
     }
 
     case ORC_FETCH_PATTERN_AFFINE_NN:

@@ -38,13 +38,14 @@ typedef struct {
   ScenePattern pat;
   OrcMat xform;       /* user transform captured by set_fill_style (rastercontext.cpp:653-695) */
   uint32_t quality;   /* pattern quality hint captured by set_fill_style */
+  uint32_t dither;    /* gradients: the gradient quality hint was DITHER at set_fill_style */
   uint32_t has_view;  /* blits: the pattern samples view = x, y, w, h of the image */
   int32_t view[4];
 } Style;
 
 typedef struct {
   OrcCanvas canvas;
-  uint32_t comp_op, fill_rule, pattern_quality;
+  uint32_t comp_op, fill_rule, pattern_quality, gradient_quality;
   OrcMat user;
   double global_alpha;
   double fill_alpha;
@@ -101,10 +102,11 @@ static void add_stop(Style* s, double offset, uint32_t rgba32) {
 
 /* set_style_impl for a gradient / pattern (rastercontext.cpp:653-695): the
  * style captures the user transform (and pattern quality) of the moment. */
-static size_t parse_gradient_style(Style* st, const uint8_t* pl, const OrcMat* user) {
+static size_t parse_gradient_style(Style* st, const uint8_t* pl, const OrcMat* user, uint32_t quality) {
   memset(st, 0, sizeof(*st));
   st->type = 1;
   st->xform = *user;
+  st->dither = quality >= 2u;   /* BL_GRADIENT_QUALITY_DITHER */
   memcpy(&st->g, pl, sizeof(SceneGradient));
   for (uint32_t k = 0; k < st->g.n_stops; k++) {
     SceneStop s;
@@ -220,7 +222,17 @@ static uint32_t resolve(Ctx* c, OrcFetch* f, uint32_t* lut_buf, OrcPipe* pipe) {
       OrcMat gm = { st->g.transform[0], st->g.transform[1], st->g.transform[2], st->g.transform[3], st->g.transform[4], st->g.transform[5] };
       if (orc_mat_type(&gm) != ORC_TT_IDENTITY) orc_mat_multiply(&t, &gm, &ft);
     }
-    orc_interpolate_lut(lut_buf, lut_size, st->stops, st->n_stops);
+    if (st->dither && c->canvas.format != ORC_FMT_A8) {
+      /* a dithered gradient (not on A8 targets, rastercontext.cpp:456-467)
+       * samples a 64-bit table of twice the size, at most 1024 entries */
+      static uint64_t lut64[1024];
+      lut_size = lut_size * 2u < 1024u ? lut_size * 2u : 1024u;
+      orc_interpolate_lut64(lut64, lut_size, st->stops, st->n_stops);
+      f->lut64 = lut64;
+    }
+    else {
+      orc_interpolate_lut(lut_buf, lut_size, st->stops, st->n_stops);
+    }
     f->lut = lut_buf;
     f->lut_size = lut_size;
     int r;
@@ -761,6 +773,14 @@ uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
       case SCENE_OP_SET_COMP_OP: memcpy(&c.comp_op, pl, 4); break;
       case SCENE_OP_SET_FILL_RULE: memcpy(&c.fill_rule, pl, 4); break;
       case SCENE_OP_SET_PATTERN_QUALITY: memcpy(&c.pattern_quality, pl, 4); break;
+      case SCENE_OP_SET_GRADIENT_QUALITY: {
+        /* set_hint(BL_CONTEXT_HINT_GRADIENT_QUALITY) (rastercontext.cpp:1816-1821) */
+        uint32_t q;
+        memcpy(&q, pl, 4);
+        if (q > 2u) { note(&c, ERR_INVALID_VALUE); break; }
+        c.gradient_quality = q;
+        break;
+      }
       case SCENE_OP_SET_TRANSFORM: memcpy(&c.user, pl, 48); break;
       case SCENE_OP_SET_GLOBAL_ALPHA: {
         double a;
@@ -774,7 +794,7 @@ uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
         memcpy(&c.style.rgba32, pl, 4);
         break;
       case SCENE_OP_SET_STYLE_GRADIENT:
-        parse_gradient_style(&c.style, pl, &c.user);
+        parse_gradient_style(&c.style, pl, &c.user, c.gradient_quality);
         break;
       case SCENE_OP_SET_STYLE_PATTERN:
         parse_pattern_style(&c.style, pl, &c.user, c.pattern_quality);
@@ -792,7 +812,7 @@ uint32_t orc_render(const uint8_t* scene, size_t size, uint8_t* out, size_t out_
         }
         else {
           c.mode = 3;
-          p += ex.style_kind == 2 ? parse_gradient_style(&c.explicit_style, p, &c.user)
+          p += ex.style_kind == 2 ? parse_gradient_style(&c.explicit_style, p, &c.user, c.gradient_quality)
                                   : parse_pattern_style(&c.explicit_style, p, &c.user, c.pattern_quality);
         }
         switch (ex.target) {
@@ -1016,6 +1036,20 @@ void orc_gradient_lut(const double* offsets, const uint32_t* rgba32, size_t n, u
   uint32_t lut_size = orc_gradient_info(s.stops, s.n_stops, &solid, &fmt);
   *lut_size_out = lut_size;
   if (lut_size) orc_interpolate_lut(out, lut_size, s.stops, s.n_stops);
+}
+
+/* The 64-bit table of a dithered gradient: twice the 32-bit size, at most
+ * 1024 entries (GradientInfo::lut_size, core/gradient_p.h:113-120). */
+void orc_gradient_lut64(const double* offsets, const uint32_t* rgba32, size_t n, uint64_t* out, uint32_t* lut_size_out) {
+  Style s;
+  memset(&s, 0, sizeof(s));
+  for (size_t i = 0; i < n; i++) add_stop(&s, offsets[i], rgba32[i]);
+  int solid;
+  uint32_t fmt;
+  uint32_t lut_size = orc_gradient_info(s.stops, s.n_stops, &solid, &fmt);
+  if (lut_size) lut_size = lut_size * 2u < 1024u ? lut_size * 2u : 1024u;
+  *lut_size_out = lut_size;
+  if (lut_size) orc_interpolate_lut64(out, lut_size, s.stops, s.n_stops);
 }
 
 uint32_t orc_composite_pixel(uint32_t op, uint32_t dst_format, uint32_t d, uint32_t s, uint32_t m) {

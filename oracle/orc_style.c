@@ -126,6 +126,72 @@ static inline void hi_bytes(uint64_t rgba64, uint32_t out[4]) {
   out[3] = (uint32_t)(rgba64 >> 56) & 0xFFu;  /* a */
 }
 
+/* udiv65535 (pixelops/scalar_p.h:37) */
+static uint32_t div65535(uint32_t x) { return ((x + 0x8000u) + ((x + 0x8000u) >> 16)) >> 16; }
+
+/* cvt_prgb64_8888_from_argb64_8888 (pixelops/scalar_p.h:179-222) */
+static uint64_t premul64(uint64_t c) {
+  uint32_t a = (uint32_t)(c >> 48);
+  uint64_t r = div65535((uint32_t)((c >> 32) & 0xFFFFu) * a);
+  uint64_t g = div65535((uint32_t)((c >> 16) & 0xFFFFu) * a);
+  uint64_t b = div65535((uint32_t)(c & 0xFFFFu) * a);
+  return ((uint64_t)a << 48) | (r << 32) | (g << 16) | b;
+}
+
+/* interpolate_prgb64 (pixelops/interpolation.cpp:161-275): the 64-bit table
+ * of dithered gradients, span by span. */
+void orc_interpolate_lut64(uint64_t* d_ptr, uint32_t d_size, const OrcStop* s_ptr, size_t s_size) {
+  const uint32_t shift = 15, mask = 0xFFFFu << 15;
+  uint64_t first = premul64(s_ptr[0].rgba64);
+  uint64_t c0 = s_ptr[0].rgba64;
+  double f_width = (double)((int32_t)(d_size - 1u) << 8);
+  uint32_t p0 = 0;
+  uint32_t written = 0;   /* entries 0 .. written - 1 hold their span's values */
+  if (s_size == 1) {
+    for (uint32_t e = 0; e < d_size; e++) d_ptr[e] = first;
+    return;
+  }
+  for (size_t k = 0; k < s_size; k++) {
+    uint64_t c1 = s_ptr[k].rgba64;
+    uint32_t p1 = (uint32_t)orc_round_to_int(s_ptr[k].offset * f_width);
+    uint32_t at = p0 >> 8, count = (p1 >> 8) - (p0 >> 8);
+    if (count == 0) c0 = c1;
+    p0 = p1;
+    d_ptr[at] = premul64(c0);
+    if (c0 == c1) {
+      for (uint32_t j = 1; j <= count; j++) d_ptr[at + j] = d_ptr[at];
+    }
+    else if (count) {
+      uint32_t pos[4], inc[4];
+      for (int ch = 0; ch < 4; ch++) {          /* b, g, r, a */
+        uint32_t a0 = (uint32_t)(((c0 >> (16 * ch)) & 0xFFFFu) << shift) & mask;
+        uint32_t a1 = (uint32_t)(((c1 >> (16 * ch)) & 0xFFFFu) << shift) & mask;
+        inc[ch] = (uint32_t)((int32_t)(a1 - a0) / (int32_t)count);
+        pos[ch] = a0 + (1u << (shift - 1));
+      }
+      int opaque = ((c0 & c1) >> 48) == 0xFFFFu;
+      for (uint32_t j = 1; j <= count; j++) {
+        for (int ch = 0; ch < 4; ch++) pos[ch] += inc[ch];
+        uint64_t v;
+        if (opaque) {
+          v = 0xFFFF000000000000ull;
+          for (int ch = 0; ch < 3; ch++) v |= (uint64_t)((pos[ch] & mask) >> shift) << (16 * ch);
+        }
+        else {
+          uint32_t ca = pos[3] >> shift;
+          v = (uint64_t)ca << 48;
+          for (int ch = 0; ch < 3; ch++) v |= (uint64_t)div65535((pos[ch] >> shift) * ca) << (16 * ch);
+        }
+        d_ptr[at + j] = v;
+      }
+    }
+    written = at + count + 1u;
+    c0 = c1;
+  }
+  for (uint32_t e = written; e < d_size; e++) d_ptr[e] = premul64(c0);
+  d_ptr[0] = first;
+}
+
 void orc_interpolate_lut(uint32_t* d_ptr, uint32_t d_size, const OrcStop* s_ptr, size_t s_size) {
   uint32_t* span = d_ptr;
   uint32_t i;

@@ -149,6 +149,7 @@ uint32_t REF_RENDER_ENTRY(const uint8_t* scene, size_t size, uint8_t* out, size_
       case SCENE_OP_SET_FILL_RULE:
       case SCENE_OP_SET_STYLE_SOLID:
       case SCENE_OP_SET_PATTERN_QUALITY:
+      case SCENE_OP_SET_GRADIENT_QUALITY:
         std::memcpy(&call.u0, payload, 4);
         break;
       case SCENE_OP_SET_TRANSFORM:
@@ -345,6 +346,7 @@ uint32_t REF_RENDER_ENTRY(const uint8_t* scene, size_t size, uint8_t* out, size_
       case SCENE_OP_SET_STYLE_GRADIENT: note(ctx.set_fill_style(call.gradient)); break;
       case SCENE_OP_SET_STYLE_PATTERN: note(ctx.set_fill_style(call.pattern)); break;
       case SCENE_OP_SET_PATTERN_QUALITY: note(ctx.set_hint(BL_CONTEXT_HINT_PATTERN_QUALITY, call.u0)); break;
+      case SCENE_OP_SET_GRADIENT_QUALITY: note(ctx.set_hint(BL_CONTEXT_HINT_GRADIENT_QUALITY, call.u0)); break;
       case SCENE_OP_FILL_PATH: note(ctx.fill_path(call.path)); break;
       case SCENE_OP_FILL_POLYGON: note(ctx.fill_polygon(call.poly.data(), call.poly.size())); break;
       case SCENE_OP_FILL_RECT_D: note(ctx.fill_rect(BLRect(call.d[0], call.d[1], call.d[2], call.d[3]))); break;

@@ -38,6 +38,9 @@ def cases():
     for fmt, fname in ((S.FMT_PRGB32, "prgb32"), (S.FMT_XRGB32, "xrgb32"), (S.FMT_A8, "a8")):
         for seed in (1, 2, 3):
             out[f"masks_{fname}_s{seed}"] = lambda seed=seed, fmt=fmt: [S.mask_scene(seed, fmt=fmt, unaligned=seed == 3)]
+    for fmt, fname in ((S.FMT_PRGB32, "prgb32"), (S.FMT_XRGB32, "xrgb32"), (S.FMT_A8, "a8")):
+        for seed in (1, 2, 3):
+            out[f"dither_{fname}_s{seed}"] = lambda seed=seed, fmt=fmt: [S.dither_scene(seed, fmt=fmt, bad_hint=seed == 3)]
     for fmt, fname in ((S.FMT_PRGB32, "prgb32"), (S.FMT_A8, "a8")):
         out[f"blit_clip_robustness_{fname}"] = lambda fmt=fmt: S.blit_clip_robustness_scenes(fmt=fmt)
     return out

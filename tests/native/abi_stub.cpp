@@ -259,6 +259,10 @@ BLAmdResult bl_amd_context_fill_rect_d_ext(BLAmdContext*, double x, double y, do
   return 0;
 }
 
+BLAmdResult bl_amd_context_set_gradient_quality(BLAmdContext*, uint32_t q) {
+  logf("set_gradient_quality %u", q);
+  return 0;
+}
 BLAmdResult bl_amd_context_fill_mask_i(BLAmdContext*, int x, int y, const BLAmdImage* mask, const int32_t* area) {
   logf("fill_mask_i %d %d mask image=%u area=%s", x, y, serial_of(mask), area_of(area).c_str());
   return 0;

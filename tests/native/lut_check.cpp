@@ -1,6 +1,7 @@
-// Checks the per-entry gradient table of blend2d_amd/csrc/lut_device.h
-// (compiled here for the host) against the oracle's span-by-span table
-// (orc_gradient_lut, oracle/orc_scene.c, pinned by the reference-rendered
+// Checks the per-entry gradient tables of blend2d_amd/csrc/lut_device.h
+// (compiled here for the host), 32-bit and 64-bit (dithered gradients),
+// against the oracle's span-by-span tables (orc_gradient_lut /
+// orc_gradient_lut64, oracle/orc_scene.c, pinned by the reference-rendered
 // gradient goldens) for random stop lists: sorted offsets drawn from a mix of
 // exact positions (0, 0.002, 0.5, 0.998, 1), clusters and uniform values,
 // pairs of equal offsets, 1-12 stops and up to 64 (the oracle's capacity).
@@ -19,6 +20,8 @@
 
 extern "C" void orc_gradient_lut(const double* offsets, const uint32_t* rgba32, size_t n, uint32_t* out,
                                  uint32_t* lut_size_out);
+extern "C" void orc_gradient_lut64(const double* offsets, const uint32_t* rgba32, size_t n, uint64_t* out,
+                                   uint32_t* lut_size_out);
 
 int main(int argc, char** argv) {
   const int cases = argc > 1 ? atoi(argv[1]) : 20000;
@@ -27,6 +30,7 @@ int main(int argc, char** argv) {
   const double exact[] = {0.0, 0.002, 0.5, 0.998, 1.0, 0.25, 0.75};
   long entries = 0, bad = 0;
   std::vector<uint32_t> out(1024);
+  std::vector<uint64_t> out64(1024);
   for (int c = 0; c < cases; c++) {
     size_t n = 1 + rng() % 12;
     if (c % 97 == 0) n = 40 + rng() % 25;   // the oracle keeps up to 64 stops
@@ -67,6 +71,19 @@ int main(int argc, char** argv) {
       const uint32_t got = b2amd::lut::entry(st.data(), (uint32_t)n, size, e);
       if (got != out[e]) {
         if (bad < 5) printf("case %d n %zu size %u entry %u: got %08x want %08x\n", c, n, size, e, got, out[e]);
+        bad++;
+      }
+    }
+    // the 64-bit table of the same stops (dithered gradients)
+    uint32_t size64 = 0;
+    orc_gradient_lut64(off.data(), col.data(), n, out64.data(), &size64);
+    for (uint32_t e = 0; e < size64; e++) {
+      entries++;
+      const uint64_t got = b2amd::lut::entry64(st.data(), (uint32_t)n, size64, e);
+      if (got != out64[e]) {
+        if (bad < 5)
+          printf("case %d n %zu size64 %u entry %u: got %016llx want %016llx\n", c, n, size64, e,
+                 (unsigned long long)got, (unsigned long long)out64[e]);
         bad++;
       }
     }

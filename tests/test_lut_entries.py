@@ -1,9 +1,10 @@
 """Gradient tables computed entry by entry on the device
 (blend2d_amd/csrc/lut_device.h, compiled here for the host) against the
 oracle's span-by-span restatement of the reference's table builder
-(oracle/orc_scene.c orc_gradient_lut, pinned by the reference-rendered
-gradient goldens): tests/native/lut_check.cpp compares every entry of random
-stop lists.  Planted mistakes in a copy of the header -- the earlier span
+(oracle/orc_scene.c orc_gradient_lut and orc_gradient_lut64 -- the 64-bit
+tables of dithered gradients -- pinned by the reference-rendered gradient
+goldens): tests/native/lut_check.cpp compares every entry of random stop
+lists.  Planted mistakes in a copy of the header -- the earlier span
 winning a shared entry, the tail taking the first colour, the half-step
 rounding bias of the interpolation dropped -- must make it fail."""
 import os
@@ -51,6 +52,12 @@ def test_entries_match_the_span_builder(tmp_path, seed):
     ("if (start(mid) <= e) lo = mid;", "if (start(mid) < e) lo = mid;"),
     ("if (e > last_end) return premultiplied(st[n - 1u].rgba64);", "if (e > last_end) return premultiplied(st[0].rgba64);"),
     ("const uint32_t cx = (uint32_t)(ca << 23) + (1u << 22);", "const uint32_t cx = (uint32_t)(ca << 23);"),
+    # 64-bit tables: an empty span interpolating from the stop before, the
+    # half-step bias dropped, both colours taken as opaque when either is
+    ("const uint64_t c0 = (q == p) ? c1 : st[lo == 0 ? 0u : lo - 1u].rgba64;",
+     "const uint64_t c0 = st[lo == 0 ? 0u : lo - 1u].rgba64;"),
+    ("return a0 + (1u << 14) + inc * j;", "return a0 + inc * j;"),
+    ("if (((c0 & c1) >> 48) == 0xFFFFu)", "if (((c0 | c1) >> 48) == 0xFFFFu)"),
 ])
 def test_planted_mistakes_are_caught(tmp_path, old, new):
     text = open(HEADER).read()

@@ -87,3 +87,11 @@ def test_mask_fills(seed, fmt):
     # fill_mask in all forms; seed 93 adds the placements the reference
     # reports as NOT_IMPLEMENTED and invalid areas
     _same(S.mask_scene(seed, 241, 173, fmt, 120, unaligned=seed == 93))
+
+
+@pytest.mark.parametrize("seed", [101, 102, 103])
+@pytest.mark.parametrize("fmt", [S.FMT_PRGB32, S.FMT_XRGB32, S.FMT_A8])
+def test_dithered_gradients(seed, fmt):
+    # the gradient quality hint: nearest, smooth and dither, styles and
+    # explicit styles, every gradient type and extend mode
+    _same(S.dither_scene(seed, 223, 167, fmt, 120, bad_hint=seed == 103))
