@@ -249,7 +249,9 @@ BLAmdResult bl_amd_context_clear_rect_d(BLAmdContext* ctx, double x, double y, d
  * transform; the comp op and global alpha apply, the fill style and fill
  * alpha do not.  Fractional and transformed positions sample the area with
  * the pattern quality hint, extend REFLECT.  An area outside the image is
- * BL_AMD_ERROR_INVALID_VALUE. */
+ * BL_AMD_ERROR_INVALID_VALUE.  As for pattern styles and masks, `image` must
+ * not be the context's target: tiles are composited in parallel, so reading
+ * pixels the same frame writes gives an undefined result. */
 BLAmdResult bl_amd_context_blit_image_i(BLAmdContext* ctx, int x, int y, const BLAmdImage* image, const int32_t* area);
 BLAmdResult bl_amd_context_blit_image_d(BLAmdContext* ctx, double x, double y, const BLAmdImage* image,
                                         const int32_t* area);
