@@ -28,6 +28,11 @@ roofline   the render stage of the headline frame (k_render, plus the box
            edge segment it reads.  roofline_srcover4k: the same for ten
            distinct 3840x2160 canvases with one full-canvas SRC_OVER layer
            each (device-resident, larger than the last-level cache).
+           roofline_blit4k / roofline_mask4k: ten 3840x2160 canvases with one
+           full-canvas blit_image (12 B per pixel: destination read + write,
+           source read) / fill_mask of an A8 mask (9 B per pixel) each, with
+           the reference's single-core time for the same frame beside them
+           (blit4k.cpu_reference, mask4k.cpu_reference).
 cpu_baseline  the reference library (oracle/_ref, compiled from the reference
            sources without its JIT) rendering the same frame on this host's
            cores, timed by its own begin -> end clock, best of R, with
@@ -86,6 +91,20 @@ def _workload(name: str, seed: int):
         return scenes, {
             "workload": "10 distinct 3840x2160 PRGB32 canvases, one full-canvas semi-transparent SRC_OVER "
                         "layer each, one batched frame",
+            "canvases": 10, "paths": 10, "seed": seed}
+    if name == "blit4k":
+        img = S.premultiplied_noise(np.random.default_rng(seed), 3840, 2160)
+        scenes = [S.blit_layer_scene(img, seed + i) for i in range(10)]
+        return scenes, {
+            "workload": "10 distinct 3840x2160 PRGB32 canvases, one full-canvas blit_image (integer origin, "
+                        "SRC_OVER) of a 3840x2160 translucent PRGB32 image each, one batched frame",
+            "canvases": 10, "paths": 10, "seed": seed}
+    if name == "mask4k":
+        mask = np.random.default_rng(seed).integers(0, 256, size=(2160, 3840), dtype=np.uint8)
+        scenes = [S.mask_layer_scene(mask, seed + i) for i in range(10)]
+        return scenes, {
+            "workload": "10 distinct 3840x2160 PRGB32 canvases, one full-canvas fill_mask (A8 mask, integer "
+                        "origin, translucent solid, SRC_OVER) each, one batched frame",
             "canvases": 10, "paths": 10, "seed": seed}
     raise SystemExit(f"unknown workload {name}")
 
@@ -357,6 +376,13 @@ def _side_workload(name, seed, steps, learned, threads):
     roof = _roofline(scenes, rep_render, rst, name)
     roof["measured_on"] = "device-resident replays, exact sizing"
     frame.close()
+    if name in ("blit4k", "mask4k"):
+        # the reference library on one host core, the same frame (its own
+        # begin -> end clock, best of the passes that fit 10 s)
+        ref_ms, passes = _reference_passes(scenes, 0, 10.0, 3)
+        out["cpu_reference"] = {"ms_per_frame": round(ref_ms, 3),
+                                "mpix_per_s": round(st["pixels"] / (ref_ms * 1e-3) / 1e6, 3),
+                                "threads": 0, "passes": passes}
     return out, roof
 
 
@@ -365,12 +391,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4", "c5", "srcover4k"])
+    ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4", "c5", "srcover4k", "blit4k", "mask4k"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the reference parity check")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain form")
-    ap.add_argument("--no-side", action="store_true", help="skip the srcover4k and c2 side measurements")
+    ap.add_argument("--no-side", action="store_true", help="skip the srcover4k, c2, blit4k and mask4k side measurements")
     ap.add_argument("--record-threads", type=int, default=0,
                     help="client threads recording the canvases' calls (default: host cores, at most 16)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -552,7 +578,7 @@ def main():
     frame.close()
 
     if not args.no_side:
-        for name in ("srcover4k", "c2"):
+        for name in ("srcover4k", "c2", "blit4k", "mask4k"):
             if name == args.workload:
                 continue
             print(f"# side workload {name}", file=sys.stderr, flush=True)

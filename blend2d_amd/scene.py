@@ -552,6 +552,32 @@ def srcover_layers_scene(seed: int = 1, width: int = 3840, height: int = 2160, l
     return sc
 
 
+def blit_layer_scene(image: np.ndarray, seed: int = 1) -> Scene:
+    """A canvas the size of `image` (premultiplied PRGB32) cleared to seeded
+    opaque colour, with the image blitted over it once at the origin
+    (blit_image_i, SRC_OVER): 12 bytes of traffic per pixel (destination
+    read and write, source read)."""
+    rng = np.random.default_rng(seed)
+    h, w = image.shape
+    sc = Scene(w, h, FMT_PRGB32, 0xFF000000 | int(rng.integers(0, 1 << 24)))
+    k = sc.add_image(image)
+    sc.blit_image(BLIT_I, k, (0, 0))
+    return sc
+
+
+def mask_layer_scene(mask: np.ndarray, seed: int = 1) -> Scene:
+    """A PRGB32 canvas the size of the A8 `mask` cleared to seeded opaque
+    colour, with one fill_mask of a semi-transparent solid colour through the
+    mask at the origin (SRC_OVER): 9 bytes of traffic per pixel."""
+    rng = np.random.default_rng(seed)
+    h, w = mask.shape
+    sc = Scene(w, h, FMT_PRGB32, 0xFF000000 | int(rng.integers(0, 1 << 24)))
+    k = sc.add_image(mask, FMT_A8)
+    sc.set_fill_style_rgba32((int(rng.integers(64, 240)) << 24) | int(rng.integers(0, 1 << 24)))
+    sc.fill_mask(MASK_I, k, (0, 0))
+    return sc
+
+
 def mixed_scene(seed: int = 1, width: int = 513, height: int = 513, count: int = 120,
                 fmt: int = FMT_PRGB32, comp_ops=(COMP_OP_SRC_OVER, COMP_OP_SRC_COPY),
                 pattern_size: int = 64) -> Scene:
